@@ -1,0 +1,365 @@
+/*
+ * ref_harness.c -- drives the reference library (oracle/_ref/libgviz_ref.so,
+ * compiled from the reference's own sources) on inputs written by
+ * tests/golden/make_golden.py and writes its outputs as raw little-endian
+ * arrays. Used only in a container that has the reference tree: to write the
+ * committed golden fixtures and to cross-check the FP64 restatement.
+ *
+ * Usage: ref_harness <mode> <in.bin> <out.bin>
+ * Input files are a sequence of int64 / double fields in the order each mode
+ * reads them; outputs likewise (see tests/golden/refio.py).
+ */
+#include "algorithms/search/gvizKNearest.h"
+#include "ds/gvizGraph.h"
+#include "ds/gvizQuadtree.h"
+#include "ds/gvizSubgraph.h"
+#include "embedders/gvizEmbeddedGraph.h"
+#include "embedders/gvizForceEmbedder.h"
+#include "embedders/gvizGRIPEmbedder.h"
+#include "embedders/gvizGRIPInternal.h"
+#include "embedders/gvizTutteEmbedder.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static FILE *fin, *fout;
+
+static int64_t rd_i(void) {
+  int64_t v;
+  if (fread(&v, 8, 1, fin) != 1) {
+    fprintf(stderr, "short read\n");
+    exit(2);
+  }
+  return v;
+}
+static double rd_d(void) {
+  double v;
+  if (fread(&v, 8, 1, fin) != 1) {
+    fprintf(stderr, "short read\n");
+    exit(2);
+  }
+  return v;
+}
+static void wr_i(int64_t v) { fwrite(&v, 8, 1, fout); }
+static void wr_d(double v) { fwrite(&v, 8, 1, fout); }
+
+/* graph: n, m, then m (u, v) pairs in insertion order */
+static void read_graph(gvizGraph *g, int64_t *nOut) {
+  int64_t n = rd_i(), m = rd_i();
+  gvizGraphInit(g, 0);
+  for (int64_t i = 0; i < n; i++)
+    gvizGraphAddVertex(g, NULL, NULL, NULL);
+  for (int64_t e = 0; e < m; e++) {
+    int64_t u = rd_i(), v = rd_i();
+    gvizGraphAddEdge(g, (size_t)u, (size_t)v);
+  }
+  gvizGraphBuildLayout(g);
+  *nOut = n;
+}
+
+/* ---------------------------------------------------------------- quadtree */
+static void dump_node(const gvizQuadtreeNode *nd, int depth, uint64_t kh,
+                      uint64_t kl, int64_t *leafDepth, uint64_t *leafHi,
+                      uint64_t *leafLo, int64_t *nodeCount) {
+  if (!nd)
+    return;
+  (*nodeCount)++;
+  /* node record: depth, keyHi, keyLo, isLeaf, count, mass, comX, comY, h */
+  int leaf = gvizQuadtreeNodeIsLeaf(nd);
+  wr_i(depth);
+  wr_i((int64_t)kh);
+  wr_i((int64_t)kl);
+  wr_i(leaf);
+  wr_i(leaf ? (int64_t)gvizQuadtreeNodePointCount(nd) : 0);
+  wr_d(nd->mass);
+  wr_d(nd->comX);
+  wr_d(nd->comY);
+  wr_d(nd->halfSize);
+  if (leaf) {
+    for (size_t i = 0; i < gvizQuadtreeNodePointCount(nd); i++) {
+      size_t p = gvizQuadtreeNodePointAt(nd, i);
+      leafDepth[p] = depth;
+      leafHi[p] = kh;
+      leafLo[p] = kl;
+    }
+    return;
+  }
+  for (int q = 0; q < 4; q++) {
+    uint64_t h = kh, l = kl;
+    int level = depth + 1;
+    if (level <= 32)
+      h |= (uint64_t)q << (2 * (32 - level));
+    else if (level <= 64)
+      l |= (uint64_t)q << (2 * (64 - level));
+    dump_node(gvizQuadtreeNodeChild(nd, (gvizQuadrant)q), depth + 1, h, l,
+              leafDepth, leafHi, leafLo, nodeCount);
+  }
+}
+
+static int mode_quadtree(void) {
+  int64_t n = rd_i(), npc = rd_i();
+  double *pts = malloc(sizeof(double) * 2 * n);
+  double *ms = malloc(sizeof(double) * n);
+  for (int64_t i = 0; i < 2 * n; i++)
+    pts[i] = rd_d();
+  for (int64_t i = 0; i < n; i++)
+    ms[i] = rd_d();
+  gvizQuadtree t;
+  if (gvizQuadtreeInit(&t, pts, ms, (size_t)n, (size_t)npc) != 0)
+    return 1;
+  int64_t *ld = calloc(n, 8);
+  uint64_t *lh = calloc(n, 8), *ll = calloc(n, 8);
+  int64_t nodeCount = 0;
+  dump_node(gvizQuadtreeRoot(&t), 0, 0, 0, ld, lh, ll, &nodeCount);
+  for (int64_t i = 0; i < n; i++) {
+    wr_i(ld[i]);
+    wr_i((int64_t)lh[i]);
+    wr_i((int64_t)ll[i]);
+  }
+  wr_i(nodeCount);
+  gvizQuadtreeRelease(&t);
+  return 0;
+}
+
+/* ------------------------------------------------------------------- force */
+static int mode_force(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t model = rd_i(), bh = rd_i(), overlap = rd_i(), npc = rd_i();
+  int64_t iters = rd_i(), dumpEvery = rd_i();
+  double edgeLength = rd_d(), theta = rd_d(), gravityK = rd_d();
+  double overlapC = rd_d(), rBase = rd_d(), rPerDeg = rd_d(), jitter = rd_d();
+  double *pos = malloc(sizeof(double) * 2 * n);
+  for (int64_t i = 0; i < 2 * n; i++)
+    pos[i] = rd_d();
+
+  gvizForceEmbedderState s;
+  int rc = gvizForceEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2,
+                                 (gvizForceModelKind)model);
+  if (rc != 0)
+    return 1;
+  gvizForceEmbedderConfigure(&s, edgeLength, 0.0);
+  gvizForceEmbedderConfigureSpeed(&s, jitter);
+  gvizForceEmbedderConfigureBarnesHut(&s, theta, (size_t)npc);
+  gvizForceEmbedderSetBarnesHutEnabled(&s, (int)bh);
+  gvizForceEmbedderConfigureGravity(&s, gravityK);
+  gvizForceEmbedderConfigureRadius(&s, rBase, rPerDeg);
+  gvizForceEmbedderConfigureOverlapPrevention(&s, overlapC);
+  gvizForceEmbedderSetPreventOverlapEnabled(&s, (int)overlap);
+  gvizForceEmbedderBegin(&s, 12345u);
+  for (int64_t i = 0; i < n; i++)
+    gvizEmbeddedGraphSetVPosition((gvizEmbeddedGraph *)&s, (size_t)i,
+                                  pos + 2 * i);
+  for (int64_t it = 0; it < iters; it++) {
+    double maxDisp = gvizForceEmbedderStep(&s);
+    const gvizStatSeries *sp =
+        gvizEmbeddedGraphFindStatSeries((gvizEmbeddedGraph *)&s,
+                                        "forceEmbedder.speed");
+    const gvizStatSeries *sa =
+        gvizEmbeddedGraphFindStatSeries((gvizEmbeddedGraph *)&s,
+                                        "forceEmbedder.attractiveForce");
+    const gvizStatSeries *sr =
+        gvizEmbeddedGraphFindStatSeries((gvizEmbeddedGraph *)&s,
+                                        "forceEmbedder.repulsiveForce");
+    const gvizStatSeries *sgv =
+        gvizEmbeddedGraphFindStatSeries((gvizEmbeddedGraph *)&s,
+                                        "forceEmbedder.gravityForce");
+    wr_d(maxDisp);
+    wr_d(sp->samples[sp->count - 1]);
+    wr_d(sa->samples[sa->count - 1]);
+    wr_d(sr->samples[sr->count - 1]);
+    wr_d(sgv->samples[sgv->count - 1]);
+    wr_d(s.speedEfficiency);
+    int dump = dumpEvery > 0 && (it % dumpEvery == 0 || it == iters - 1);
+    if (dump) {
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(s.disp[i]);
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(s.structForce[i]);
+      for (int64_t i = 0; i < n; i++)
+        wr_d(s.attForceMag[i]);
+      for (int64_t i = 0; i < n; i++)
+        wr_d(s.repForceMag[i]);
+      const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(p[i]);
+    }
+  }
+  gvizForceEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
+/* ------------------------------------------------------------------- tutte */
+static int mode_tutte(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t nb = rd_i(), iters = rd_i(), gs = rd_i();
+  size_t *boundary = malloc(sizeof(size_t) * nb);
+  for (int64_t i = 0; i < nb; i++)
+    boundary[i] = (size_t)rd_i();
+  double radius = rd_d(), dt = rd_d();
+  gvizTutteState s;
+  if (gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, 1e-300) != 0)
+    return 1;
+  s.useGaussSeidel = (int)gs;
+  gvizTutteFixConvexPolygon(&s, boundary, (size_t)nb, radius);
+  gvizTutteEmbedderSeedInterior(&s);
+  const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  for (int64_t it = 0; it < iters; it++)
+    wr_d(gvizTutteEmbedderStep(&s, dt));
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  gvizTutteEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
+/* --------------------------------------------------------------------- kNN */
+static int mode_knn(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t k = rd_i(), nsrc = rd_i();
+  gvizSubgraph sg = gvizSubgraphCreateFull(&g);
+  gvizVertexSubset vis = gvizVertexSubsetCreateEmpty(&g);
+  for (int64_t i = 0; i < n; i++)
+    if (rd_i())
+      gvizVertexSubsetShowVertex(vis, (size_t)i);
+  gvizFoundVertex *out = malloc(sizeof(gvizFoundVertex) * (k + 1));
+  for (int64_t s = 0; s < nsrc; s++) {
+    int64_t src = rd_i();
+    int c = gvizSearchKNearest(&sg, out, (size_t)k, (size_t)src, vis);
+    wr_i(c);
+    for (int64_t j = 0; j < k; j++) {
+      wr_i(j < c ? (int64_t)out[j].v : -1);
+      wr_i(j < c ? (int64_t)out[j].dist : -1);
+    }
+  }
+  /* batch path over all non-visible vertices as targets */
+  int64_t nt = 0;
+  gvizKNNBatchTarget *targets = malloc(sizeof(*targets) * n);
+  gvizFoundVertex *found = malloc(sizeof(gvizFoundVertex) * n * k);
+  for (int64_t i = 0; i < n; i++)
+    if (!gvizVertexSubsetTest(vis, (size_t)i)) {
+      targets[nt].vertex = (size_t)i;
+      targets[nt].out = found + nt * k;
+      targets[nt].count = 0;
+      nt++;
+    }
+  gvizKNearestScratch scratch;
+  gvizKNearestScratchInit(&scratch, (size_t)n);
+  int rc = gvizSearchKNearestFromVisibleBatch(&sg, vis, (size_t)k, targets,
+                                              (size_t)nt, &scratch);
+  wr_i(rc);
+  wr_i(nt);
+  for (int64_t t = 0; t < nt; t++) {
+    wr_i((int64_t)targets[t].vertex);
+    wr_i((int64_t)targets[t].count);
+    for (int64_t j = 0; j < k; j++) {
+      wr_i(j < (int64_t)targets[t].count ? (int64_t)targets[t].out[j].v : -1);
+      wr_i(j < (int64_t)targets[t].count ? (int64_t)targets[t].out[j].dist
+                                          : -1);
+    }
+  }
+  gvizKNearestScratchRelease(&scratch);
+  return 0;
+}
+
+/* -------------------------------------------------------------------- GRIP */
+static void dump_grip_state(gvizGRIPState *st, int64_t n) {
+  const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)st);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  for (int64_t v = 0; v < n; v++) {
+    gvizGRIPDecorators *d = &st->dec[v];
+    wr_d(d->disp[0]);
+    wr_d(d->disp[1]);
+    wr_d(d->oldDisp[0]);
+    wr_d(d->oldDisp[1]);
+    wr_d(d->heat);
+    wr_d(d->oldCos);
+    wr_d(d->lastForceMag);
+    wr_i((int64_t)d->knn.count);
+    wr_i(gvizTestBit(st->dispCalculated, (size_t)v) ? 1 : 0);
+  }
+}
+
+static int mode_grip(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t kcap = rd_i(), kmax = rd_i(), roundsPerLayer = rd_i();
+  gvizGRIPState st;
+  memset(&st, 0, sizeof(st));
+  gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+    return 1;
+  gvizGRIPEmbedderConfigureK(&st, (size_t)kmax, (size_t)kmax,
+                             GVIZ_GRIP_K_CONSTANT);
+  gvizGRIPEmbedderBegin(&st);
+  wr_i((int64_t)st.layerCount);
+  for (size_t i = 0; i < st.layerCount; i++)
+    wr_i((int64_t)*(size_t *)gvizArrayAtIndex(&st.misBorder, i));
+  size_t active0 = *(size_t *)gvizArrayAtIndex(&st.misBorder, 0);
+  for (size_t i = 0; i < active0; i++)
+    wr_i((int64_t)st.misFiltration[i]);
+  for (;;) {
+    for (int64_t r = 0; r < roundsPerLayer; r++) {
+      wr_i((int64_t)st.currLayer);
+      dump_grip_state(&st, n);
+      /* kNN lists of every vertex, kcap slots */
+      for (int64_t v = 0; v < n; v++) {
+        gvizGRIPDecorators *d = &st.dec[v];
+        for (int64_t j = 0; j < kcap; j++) {
+          gvizFoundVertex *f = (gvizFoundVertex *)d->knn.arr + j;
+          int ok = j < (int64_t)d->knn.count;
+          wr_i(ok ? (int64_t)f->v : -1);
+          wr_i(ok ? (int64_t)f->dist : -1);
+        }
+      }
+      gvizGRIPEmbedderRefineRound(&st);
+      dump_grip_state(&st, n);
+      gvizGRIPRoundStats rs = gvizGRIPEmbedderLastRoundStats(&st);
+      wr_d(rs.maxDisplacement);
+      wr_d(rs.meanDisplacement);
+      wr_d(rs.meanForce);
+    }
+    if (st.currLayer == 0)
+      break;
+    gvizGRIPEmbedderNextStage(&st);
+  }
+  wr_i(-1);
+  gvizGRIPEmbedderRelease(&st);
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc != 4) {
+    fprintf(stderr, "usage: %s mode in out\n", argv[0]);
+    return 2;
+  }
+  fin = fopen(argv[2], "rb");
+  fout = fopen(argv[3], "wb");
+  if (!fin || !fout)
+    return 2;
+  int rc = 2;
+  if (!strcmp(argv[1], "quadtree"))
+    rc = mode_quadtree();
+  else if (!strcmp(argv[1], "force"))
+    rc = mode_force();
+  else if (!strcmp(argv[1], "tutte"))
+    rc = mode_tutte();
+  else if (!strcmp(argv[1], "knn"))
+    rc = mode_knn();
+  else if (!strcmp(argv[1], "grip"))
+    rc = mode_grip();
+  fclose(fin);
+  fclose(fout);
+  return rc;
+}

@@ -1,0 +1,210 @@
+"""Input/output framing for oracle/_ref/ref_harness (the reference library
+driven on fixed inputs). Used by make_golden.py and, in a container that has
+the reference tree, by tests that cross-check the oracle against it live.
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARNESS = os.path.join(HERE, "..", "..", "oracle", "_ref", "ref_harness")
+
+
+def harness_available() -> bool:
+    return os.path.exists(HARNESS)
+
+
+class _W:
+    def __init__(self):
+        self.parts = []
+
+    def i(self, *vals):
+        for v in vals:
+            self.parts.append(struct.pack("<q", int(v)))
+
+    def d(self, *vals):
+        for v in vals:
+            self.parts.append(struct.pack("<d", float(v)))
+
+    def ia(self, arr):
+        self.parts.append(np.ascontiguousarray(arr, dtype="<i8").tobytes())
+
+    def da(self, arr):
+        self.parts.append(np.ascontiguousarray(arr, dtype="<f8").tobytes())
+
+    def graph(self, n, edges):
+        edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+        self.i(n, edges.shape[0])
+        self.ia(edges.reshape(-1))
+
+    def bytes(self):
+        return b"".join(self.parts)
+
+
+class _R:
+    def __init__(self, buf):
+        self.buf = buf
+        self.off = 0
+
+    def i(self):
+        v = struct.unpack_from("<q", self.buf, self.off)[0]
+        self.off += 8
+        return v
+
+    def d(self):
+        v = struct.unpack_from("<d", self.buf, self.off)[0]
+        self.off += 8
+        return v
+
+    def ia(self, n):
+        a = np.frombuffer(self.buf, dtype="<i8", count=n, offset=self.off).copy()
+        self.off += 8 * n
+        return a
+
+    def da(self, n):
+        a = np.frombuffer(self.buf, dtype="<f8", count=n, offset=self.off).copy()
+        self.off += 8 * n
+        return a
+
+
+def _run(mode, payload: bytes) -> bytes:
+    with tempfile.TemporaryDirectory() as td:
+        fi = os.path.join(td, "in.bin")
+        fo = os.path.join(td, "out.bin")
+        with open(fi, "wb") as f:
+            f.write(payload)
+        subprocess.run([HARNESS, mode, fi, fo], check=True)
+        with open(fo, "rb") as f:
+            return f.read()
+
+
+def quadtree(points, masses, npc):
+    points = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+    n = points.shape[0]
+    w = _W()
+    w.i(n, npc)
+    w.da(points.reshape(-1))
+    w.da(masses)
+    r = _R(_run("quadtree", w.bytes()))
+    nodes = []
+    # node records are written in DFS order before the per-point table; read
+    # them until the table fits exactly at the end of the buffer.
+    total = len(r.buf) // 8
+    node_words = total - 3 * n - 1
+    count = node_words // 9
+    for _ in range(count):
+        depth, kh, kl, leaf, cnt = r.i(), r.i(), r.i(), r.i(), r.i()
+        mass, cx, cy, h = r.d(), r.d(), r.d(), r.d()
+        nodes.append((depth, kh & 0xFFFFFFFFFFFFFFFF, kl & 0xFFFFFFFFFFFFFFFF,
+                      leaf, cnt, mass, cx, cy, h))
+    leaf = r.ia(3 * n).reshape(n, 3)
+    node_count = r.i()
+    assert node_count == count
+    dt = np.dtype([("depth", "i8"), ("keyHi", "u8"), ("keyLo", "u8"),
+                   ("leaf", "i8"), ("count", "i8"), ("mass", "f8"),
+                   ("comX", "f8"), ("comY", "f8"), ("halfSize", "f8")])
+    arr = np.array(nodes, dtype=dt)
+    return {
+        "nodes": arr,
+        "leaf_depth": leaf[:, 0].copy(),
+        "leaf_keyHi": leaf[:, 1].astype(np.uint64),
+        "leaf_keyLo": leaf[:, 2].astype(np.uint64),
+    }
+
+
+def force(n, edges, pos, *, model, barnes_hut, prevent_overlap, npc, iters,
+          dump_every, edge_length, theta, gravity, overlap_constant,
+          radius_base, radius_per_degree, jitter):
+    w = _W()
+    w.graph(n, edges)
+    w.i(model, barnes_hut, prevent_overlap, npc, iters, dump_every)
+    w.d(edge_length, theta, gravity, overlap_constant, radius_base,
+        radius_per_degree, jitter)
+    w.da(np.asarray(pos, dtype=np.float64).reshape(-1))
+    r = _R(_run("force", w.bytes()))
+    stats = []
+    dumps = {}
+    for it in range(iters):
+        stats.append([r.d() for _ in range(6)])
+        if dump_every > 0 and (it % dump_every == 0 or it == iters - 1):
+            dumps[it] = {
+                "disp": r.da(2 * n).reshape(n, 2),
+                "structForce": r.da(2 * n).reshape(n, 2),
+                "attMag": r.da(n),
+                "repMag": r.da(n),
+                "pos_after": r.da(2 * n).reshape(n, 2),
+            }
+    return np.asarray(stats), dumps
+
+
+def tutte(n, edges, boundary, iters, radius, dt, gauss_seidel=False):
+    w = _W()
+    w.graph(n, edges)
+    w.i(len(boundary), iters, 1 if gauss_seidel else 0)
+    w.ia(np.asarray(boundary, dtype=np.int64))
+    w.d(radius, dt)
+    r = _R(_run("tutte", w.bytes()))
+    p0 = r.da(2 * n).reshape(n, 2)
+    deltas = r.da(iters)
+    p1 = r.da(2 * n).reshape(n, 2)
+    return p0, deltas, p1
+
+
+def knn(n, edges, visible, k, sources):
+    w = _W()
+    w.graph(n, edges)
+    w.i(k, len(sources))
+    w.ia(np.asarray(visible, dtype=np.int64))
+    w.ia(np.asarray(sources, dtype=np.int64))
+    r = _R(_run("knn", w.bytes()))
+    per = []
+    for _ in sources:
+        c = r.i()
+        lst = r.ia(2 * k).reshape(k, 2)
+        per.append((c, lst))
+    rc = r.i()
+    nt = r.i()
+    batch = []
+    for _ in range(nt):
+        v = r.i()
+        c = r.i()
+        lst = r.ia(2 * k).reshape(k, 2)
+        batch.append((v, c, lst))
+    return per, rc, batch
+
+
+def grip(n, edges, kcap, kmax, rounds_per_layer):
+    w = _W()
+    w.graph(n, edges)
+    w.i(kcap, kmax, rounds_per_layer)
+    r = _R(_run("grip", w.bytes()))
+    layers = r.i()
+    borders = r.ia(layers)
+    filtration = r.ia(int(borders[0]))
+
+    def state():
+        pos = r.da(2 * n).reshape(n, 2)
+        dec = np.frombuffer(r.buf, dtype=np.dtype([
+            ("disp", "<f8", 2), ("oldDisp", "<f8", 2), ("heat", "<f8"),
+            ("oldCos", "<f8"), ("lastForceMag", "<f8"), ("knnCount", "<i8"),
+            ("dispCalculated", "<i8")]), count=n, offset=r.off).copy()
+        r.off += n * 72
+        return pos, dec
+
+    rounds = []
+    while True:
+        layer = r.i()
+        if layer == -1:
+            break
+        pos0, dec0 = state()
+        knnl = r.ia(n * kcap * 2).reshape(n, kcap, 2)
+        pos1, dec1 = state()
+        st = (r.d(), r.d(), r.d())
+        rounds.append(dict(layer=layer, pos0=pos0, dec0=dec0, knn=knnl,
+                           pos1=pos1, dec1=dec1, stats=st))
+    return borders, filtration, rounds
