@@ -1,0 +1,159 @@
+"""Writes the golden fixtures in tests/golden/*.npz by running the reference
+library (oracle/_ref, compiled from the reference's own sources by
+`make -C oracle ref`) on small seeded inputs.
+
+Each fixture holds the inputs and the reference's outputs; nothing of the
+reference's code. Run from the repository root:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(HERE, "..", ".."))
+
+import refio  # noqa: E402
+from grapher_amd import graphs as G  # noqa: E402
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print(f"{name}: {os.path.getsize(path) / 1024:.1f} KiB")
+
+
+def quadtree_cases():
+    pts = G.uniform_positions(300, 3)
+    r = refio.quadtree(pts, np.ones(300), 1)
+    save("quadtree_uniform300", points=pts, masses=np.ones(300), npc=1, **r)
+
+    rng = np.random.default_rng(7)
+    centers = rng.uniform(-5, 5, size=(4, 2))
+    pts = np.concatenate([c + rng.normal(scale=0.05, size=(100, 2)) for c in centers])
+    pts[10] = pts[11]          # exact duplicates force overflow leaves
+    pts[200] = pts[201] = pts[202]
+    ms = 1.0 + rng.integers(0, 6, size=400).astype(np.float64)
+    r = refio.quadtree(pts, ms, 3)
+    save("quadtree_clustered_npc3", points=pts, masses=ms, npc=3, **r)
+
+    pts = G.uniform_positions(200, 11) * 1000.0
+    pts[5] = pts[6]
+    pts[50] = pts[51]
+    r = refio.quadtree(pts, np.ones(200), 1)
+    save("quadtree_dup_npc1", points=pts, masses=np.ones(200), npc=1, **r)
+
+
+FORCE_DEFAULTS = dict(npc=1, edge_length=10.0, theta=1.0, overlap_constant=100.0,
+                      radius_base=0.0, radius_per_degree=0.0, jitter=1.0,
+                      gravity=0.0, prevent_overlap=0)
+
+
+def force_case(name, n, edges, pos, iters, dump_every, **kw):
+    args = dict(FORCE_DEFAULTS)
+    args.update(kw)
+    stats, dumps = refio.force(n, edges, pos, iters=iters, dump_every=dump_every,
+                               **args)
+    out = dict(n=n, edges=edges, pos0=pos, iters=iters, stats=stats,
+               dump_iters=np.array(sorted(dumps), dtype=np.int64))
+    for k, v in args.items():
+        out["param_" + k] = v
+    for it, d in dumps.items():
+        for k, v in d.items():
+            out[f"it{it}_{k}"] = v
+    save(name, **out)
+
+
+def force_cases():
+    n, e = G.grid(12, 12)
+    force_case("force_fr_bh_grid12", n, e, G.uniform_positions(n, 0), 8, 1,
+               model=0, barnes_hut=1, theta=0.5)
+    n, e = G.barabasi_albert(300, 3, 5)
+    force_case("force_linlog_bh_ba300", n, e, G.uniform_positions(n, 1), 8, 1,
+               model=1, barnes_hut=1, theta=0.7, gravity=0.05,
+               prevent_overlap=1, radius_base=0.1, radius_per_degree=1.0)
+    n, e = G.erdos_renyi(200, 8, 9)
+    force_case("force_fr_exact_er200", n, e, G.uniform_positions(n, 2), 5, 1,
+               model=0, barnes_hut=0)
+    n, e = G.grid(15, 15)
+    force_case("force_fr_bh_npc2_grid15", n, e, G.uniform_positions(n, 4), 5, 1,
+               model=0, barnes_hut=1, theta=0.8, npc=2)
+    # The CPU-runnable headline case: 100x100 grid, FR, BH theta 0.5, 500 its.
+    n, e = G.grid(100, 100)
+    force_case("force_fr_bh_grid100", n, e, G.uniform_positions(n, 0), 500, 499,
+               model=0, barnes_hut=1, theta=0.5)
+
+
+def tutte_cases():
+    n, e = G.grid(6, 6)
+    ring = [0, 1, 2, 3, 4, 5, 11, 17, 23, 29, 35, 34, 33, 32, 31, 30, 24, 18, 12, 6]
+    for gs in (0, 1):
+        p0, deltas, p1 = refio.tutte(n, e, ring, 40, 200.0, 1.0, bool(gs))
+        save(f"tutte_grid6_{'gs' if gs else 'jacobi'}", n=n, edges=e,
+             boundary=np.array(ring), p0=p0, deltas=deltas, p1=p1, iters=40,
+             radius=200.0, dt=1.0)
+    n, e, pts, hull = G.delaunay(300, 3)
+    p0, deltas, p1 = refio.tutte(n, e, hull, 200, 200.0, 1.0, False)
+    save("tutte_delaunay300_jacobi", n=n, edges=e, boundary=hull, p0=p0,
+         deltas=deltas, p1=p1, iters=200, radius=200.0, dt=1.0)
+    # Relaxation below 1 (dt = 1/60: alpha = 5/60) exercises the blend.
+    p0, deltas, p1 = refio.tutte(n, e, hull, 30, 200.0, 1.0 / 60.0, False)
+    save("tutte_delaunay300_dt60", n=n, edges=e, boundary=hull, p0=p0,
+         deltas=deltas, p1=p1, iters=30, radius=200.0, dt=1.0 / 60.0)
+    k4 = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]])
+    p0, deltas, p1 = refio.tutte(4, k4, [0, 1, 2], 5, 200.0, 1.0, False)
+    save("tutte_k4", n=4, edges=k4, boundary=np.array([0, 1, 2]), p0=p0,
+         deltas=deltas, p1=p1, iters=5, radius=200.0, dt=1.0)
+
+
+def knn_cases():
+    n, e, _ = G.random_geometric(300, 8.0, 1)
+    rng = np.random.default_rng(0)
+    vis = (rng.random(n) < 0.1).astype(np.int64)
+    srcs = np.arange(n)
+    per, rc, batch = refio.knn(n, e, vis, 6, srcs)
+    save("knn_rgg300", n=n, edges=e, visible=vis, k=6, sources=srcs,
+         per_count=np.array([c for c, _ in per]),
+         per_list=np.stack([lst for _, lst in per]), batch_rc=rc,
+         batch_targets=np.array([b[0] for b in batch]),
+         batch_count=np.array([b[1] for b in batch]),
+         batch_list=np.stack([b[2] for b in batch]))
+
+
+def grip_case(name, n, e, kcap, rounds):
+    borders, filt, rr = refio.grip(n, e, kcap, kcap, rounds)
+    out = dict(n=n, edges=e, kcap=kcap, borders=borders, filtration=filt,
+               rounds=len(rr))
+    for i, r in enumerate(rr):
+        out[f"r{i}_layer"] = r["layer"]
+        out[f"r{i}_pos0"] = r["pos0"]
+        out[f"r{i}_dec0"] = r["dec0"]
+        out[f"r{i}_knn"] = r["knn"]
+        out[f"r{i}_pos1"] = r["pos1"]
+        out[f"r{i}_dec1"] = r["dec1"]
+        out[f"r{i}_stats"] = np.array(r["stats"])
+    save(name, **out)
+
+
+def grip_cases():
+    n, e, _ = G.random_geometric(300, 8.0, 2)
+    grip_case("grip_rgg300", n, e, 10, 2)
+    n, e = G.grid(10, 10)
+    grip_case("grip_grid10", n, e, 8, 2)
+    n, e = G.barabasi_albert(200, 2, 1)
+    grip_case("grip_ba200", n, e, 10, 1)
+
+
+if __name__ == "__main__":
+    if not refio.harness_available():
+        sys.exit("oracle/_ref/ref_harness is not built: make -C oracle ref")
+    quadtree_cases()
+    force_cases()
+    tutte_cases()
+    knn_cases()
+    grip_cases()
