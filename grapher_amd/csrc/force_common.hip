@@ -1,0 +1,543 @@
+// force_common.hip -- precision-independent stages of the force step:
+// bounding box and root cell, quadrant-path keys, key sort, binary radix
+// tree, double-double mass prefix sums, the FA2 speed update and the step
+// statistics. Compiled with -ffp-contract=off so that the FP64 key
+// recurrence and the FA2 scalar update round exactly like the reference.
+#include "dd.hpp"
+#include "force_engine.hpp"
+
+#include <cstdarg>
+
+namespace gvh {
+
+static thread_local char g_err[512];
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char *last_error() { return g_err; }
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------- root cell
+// Min/max over all positions: exact in any order, so a tree reduction gives
+// the reference's serial loop result (src/ds/gvizQuadtree.c:165-173).
+__global__ __launch_bounds__(kBlock) void k_bbox(const double2 *__restrict__ pos,
+                                                 int64_t n,
+                                                 double *__restrict__ partial) {
+  __shared__ double sm[4 * 16];
+  double v[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double2 p = pos[i];
+    v[0] = fmin(v[0], p.x);
+    v[1] = fmax(v[1], p.x);
+    v[2] = fmin(v[2], p.y);
+    v[3] = fmax(v[3], p.y);
+  }
+  // max over (-min) keeps one reduction kind
+  v[0] = -v[0];
+  v[2] = -v[2];
+  const bool isMax[4] = {true, true, true, true};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x * 4 + 0] = -v[0];
+    partial[blockIdx.x * 4 + 1] = v[1];
+    partial[blockIdx.x * 4 + 2] = -v[2];
+    partial[blockIdx.x * 4 + 3] = v[3];
+  }
+}
+
+// Root centre, half size and depth limit: src/ds/gvizQuadtree.c:175-178 and
+// the split guard halfSize <= 1e-9 (:6, :134).
+__global__ void k_root(const double *__restrict__ partial, int blocks,
+                       int64_t n, Scalars *__restrict__ sc) {
+  __shared__ double sm[4 * 16];
+  double v[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] = fmax(v[0], -partial[b * 4 + 0]);
+    v[1] = fmax(v[1], partial[b * 4 + 1]);
+    v[2] = fmax(v[2], -partial[b * 4 + 2]);
+    v[3] = fmax(v[3], partial[b * 4 + 3]);
+  }
+  const bool isMax[4] = {true, true, true, true};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    double minX = -v[0], maxX = v[1], minY = -v[2], maxY = v[3];
+    double sx = maxX - minX, sy = maxY - minY;
+    double span = sx > sy ? sx : sy;
+    double half = span > 0.0 ? span / 2.0 : 1.0;
+    sc->rootCx = (minX + maxX) / 2.0;
+    sc->rootCy = (minY + maxY) / 2.0;
+    sc->rootHalf = half;
+    int d = 0;
+    double h = half;
+    while (!(h <= 1e-9) && d < 64) {
+      h = h / 2.0;
+      d++;
+    }
+    sc->maxDepth = d;
+    sc->rootRef = n >= 2 ? 0 : -1;
+    sc->traversalOverflow = 0;
+  }
+}
+
+// Quadrant path of every point by replaying the reference's FP64 centre
+// recurrence (src/ds/gvizQuadtree.c:65-92): digit = 2*(y < cy) + (x >= cx),
+// child centre = centre +/- halfSize/2. Level 1 sits in bits 63..62 of keyHi,
+// level 33 in bits 63..62 of keyLo; levels past the depth limit stay 0.
+__global__ __launch_bounds__(kBlock) void k_keys(const double2 *__restrict__ pos,
+                                                 int64_t n,
+                                                 const Scalars *__restrict__ sc,
+                                                 uint64_t *__restrict__ keyHi,
+                                                 uint64_t *__restrict__ keyLo,
+                                                 uint32_t *__restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || sc->stopped)
+    return;
+  double2 p = pos[i];
+  double cx = sc->rootCx, cy = sc->rootCy, h = sc->rootHalf;
+  const int levels = sc->maxDepth;
+  uint64_t hi = 0, lo = 0;
+  for (int level = 1; level <= levels; level++) {
+    bool east = p.x >= cx;
+    bool north = p.y >= cy;
+    uint64_t digit = (north ? 0u : 2u) | (east ? 1u : 0u);
+    if (level <= 32)
+      hi |= digit << (2 * (32 - level));
+    else
+      lo |= digit << (2 * (64 - level));
+    double quarter = h / 2.0;
+    cx = cx + (east ? quarter : -quarter);
+    cy = cy + (north ? quarter : -quarter);
+    h = quarter;
+  }
+  keyHi[i] = hi;
+  keyLo[i] = lo;
+  vals[i] = (uint32_t)i;
+}
+
+// The sort orders by keyHi only; a run of equal keyHi (two points sharing
+// their first 32 levels) is put in (keyLo, id) order here. Runs start in id
+// order (the sort is stable), so this is an insertion sort on keyLo.
+__global__ __launch_bounds__(kBlock) void k_fix_runs(
+    const uint64_t *__restrict__ keyHi, const uint64_t *__restrict__ keyLo,
+    uint32_t *__restrict__ vals, int64_t n, const Scalars *__restrict__ sc) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || sc->stopped || sc->maxDepth <= 32)
+    return;
+  uint64_t k = keyHi[s];
+  if (s > 0 && keyHi[s - 1] == k)
+    return;
+  int64_t e = s + 1;
+  while (e < n && keyHi[e] == k)
+    e++;
+  for (int64_t a = s + 1; a < e; a++) {
+    uint32_t v = vals[a];
+    uint64_t kl = keyLo[v];
+    int64_t b = a - 1;
+    while (b >= s && keyLo[vals[b]] > kl) {
+      vals[b + 1] = vals[b];
+      b--;
+    }
+    vals[b + 1] = v;
+  }
+}
+
+// Longest common prefix in bits of the 128-bit keys at sorted positions i, j;
+// identical keys compare their positions (Karras 2012), so delta is unique.
+__device__ __forceinline__ int lcp(const uint64_t *__restrict__ hi,
+                                   const uint64_t *__restrict__ lo, int64_t n,
+                                   int64_t i, int64_t j) {
+  if (j < 0 || j >= n)
+    return -1;
+  uint64_t a = hi[i] ^ hi[j];
+  if (a)
+    return __clzll(a);
+  uint64_t b = lo[i] ^ lo[j];
+  if (b)
+    return 64 + __clzll(b);
+  return 128 + __clz((uint32_t)(i ^ j));
+}
+
+// Binary radix tree over sorted keys (Karras, "Maximizing parallelism in the
+// construction of BVHs, octrees and k-d trees", HPG 2012). Internal node i
+// covers [first, last]; children < 0 encode leaves: -(s + 1).
+__global__ __launch_bounds__(kBlock) void k_karras(
+    const uint64_t *__restrict__ hi, const uint64_t *__restrict__ lo, int64_t n,
+    int32_t *__restrict__ left, int32_t *__restrict__ right,
+    int32_t *__restrict__ parent, int32_t *__restrict__ first,
+    int32_t *__restrict__ last, int32_t *__restrict__ delta,
+    const Scalars *__restrict__ sc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1 || sc->stopped)
+    return;
+  int d = lcp(hi, lo, n, i, i + 1) - lcp(hi, lo, n, i, i - 1) > 0 ? 1 : -1;
+  int dmin = lcp(hi, lo, n, i, i - d);
+  int64_t lmax = 2;
+  while (lcp(hi, lo, n, i, i + lmax * d) > dmin)
+    lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (lcp(hi, lo, n, i, i + (l + t) * d) > dmin)
+      l += t;
+  int64_t j = i + l * d;
+  int dnode = lcp(hi, lo, n, i, j);
+  int64_t s = 0;
+  int64_t t = l;
+  do {
+    t = (t + 1) / 2;
+    if (lcp(hi, lo, n, i, i + (s + t) * d) > dnode)
+      s += t;
+  } while (t > 1);
+  int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+  int64_t lo_ = i < j ? i : j;
+  int64_t hi_ = i < j ? j : i;
+  int32_t lc = (lo_ == gamma) ? -(int32_t)(gamma + 1) : (int32_t)gamma;
+  int32_t rc = (hi_ == gamma + 1) ? -(int32_t)(gamma + 2) : (int32_t)(gamma + 1);
+  left[i] = lc;
+  right[i] = rc;
+  first[i] = (int32_t)lo_;
+  last[i] = (int32_t)hi_;
+  delta[i] = dnode;
+  if (lc >= 0)
+    parent[lc] = (int32_t)i;
+  if (rc >= 0)
+    parent[rc] = (int32_t)i;
+  if (i == 0)
+    parent[0] = -1;
+}
+
+// ----------------------------------------------- mass prefix sums (dd)
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+struct Acc {
+  double m;
+  dd x, y;
+};
+
+__device__ __forceinline__ Acc acc_add(const Acc &a, const Acc &b) {
+  return {a.m + b.m, dd_add(a.x, b.x), dd_add(a.y, b.y)};
+}
+
+__device__ __forceinline__ Acc load_elem(const double *__restrict__ spos64,
+                                         const double *__restrict__ mass64,
+                                         const uint32_t *__restrict__ vals,
+                                         int64_t s) {
+  double m = mass64[vals[s]];
+  return {m, two_prod(m, spos64[2 * s]), two_prod(m, spos64[2 * s + 1])};
+}
+
+__device__ Acc block_inclusive_scan(Acc v, Acc *sm) {
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    Acc o = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+    if (threadIdx.x >= (unsigned)off)
+      o = sm[threadIdx.x - off];
+    __syncthreads();
+    if (threadIdx.x >= (unsigned)off) {
+      v = acc_add(o, v);
+      sm[threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(
+    const double *__restrict__ spos64, const double *__restrict__ mass64,
+    const uint32_t *__restrict__ vals, int64_t n, double *__restrict__ part,
+    const Scalars *__restrict__ sc) {
+  __shared__ Acc sm[kBlock];
+  if (sc->stopped)
+    return;
+  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+  for (int k = 0; k < kScanItems; k++)
+    if (base + k < n)
+      a = acc_add(a, load_elem(spos64, mass64, vals, base + k));
+  a = block_inclusive_scan(a, sm);
+  if (threadIdx.x == kBlock - 1) {
+    double *p = part + 5 * blockIdx.x;
+    p[0] = a.m;
+    p[1] = a.x.hi;
+    p[2] = a.x.lo;
+    p[3] = a.y.hi;
+    p[4] = a.y.lo;
+  }
+}
+
+// Exclusive scan of the per-tile totals, in place (one block of 1024).
+__global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ part,
+                                                        int tiles,
+                                                        const Scalars *__restrict__ sc) {
+  __shared__ Acc sm[1024];
+  if (sc->stopped)
+    return;
+  const int chunk = (tiles + 1023) / 1024;
+  const int b = threadIdx.x * chunk;
+  const int e = b + chunk < tiles ? b + chunk : tiles;
+  Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+  for (int t = b; t < e; t++) {
+    const double *p = part + 5 * t;
+    a = acc_add(a, Acc{p[0], {p[1], p[2]}, {p[3], p[4]}});
+  }
+  sm[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    Acc o = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+    if (threadIdx.x >= (unsigned)off)
+      o = sm[threadIdx.x - off];
+    __syncthreads();
+    if (threadIdx.x >= (unsigned)off)
+      sm[threadIdx.x] = acc_add(o, sm[threadIdx.x]);
+    __syncthreads();
+  }
+  Acc run = threadIdx.x ? sm[threadIdx.x - 1] : Acc{0.0, {0.0, 0.0}, {0.0, 0.0}};
+  for (int t = b; t < e; t++) {
+    double *p = part + 5 * t;
+    Acc v = {p[0], {p[1], p[2]}, {p[3], p[4]}};
+    p[0] = run.m;
+    p[1] = run.x.hi;
+    p[2] = run.x.lo;
+    p[3] = run.y.hi;
+    p[4] = run.y.lo;
+    run = acc_add(run, v);
+  }
+}
+
+// prefix[s + 1] = sum over sorted points 0..s; prefix[0] = 0.
+__global__ __launch_bounds__(kBlock) void k_scan_final(
+    const double *__restrict__ spos64, const double *__restrict__ mass64,
+    const uint32_t *__restrict__ vals, int64_t n, const double *__restrict__ part,
+    double *__restrict__ pm, double *__restrict__ pxh, double *__restrict__ pxl,
+    double *__restrict__ pyh, double *__restrict__ pyl,
+    const Scalars *__restrict__ sc) {
+  __shared__ Acc sm[kBlock];
+  if (sc->stopped)
+    return;
+  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  Acc loc[kScanItems];
+  Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < n)
+      a = acc_add(a, load_elem(spos64, mass64, vals, base + k));
+    loc[k] = a;
+  }
+  Acc incl = block_inclusive_scan(a, sm);
+  const double *p = part + 5 * blockIdx.x;
+  Acc off = {p[0], {p[1], p[2]}, {p[3], p[4]}};
+  Acc ex = threadIdx.x ? sm[threadIdx.x - 1] : Acc{0.0, {0.0, 0.0}, {0.0, 0.0}};
+  (void)incl;
+  off = acc_add(off, ex);
+  for (int k = 0; k < kScanItems; k++) {
+    int64_t s = base + k;
+    if (s < n) {
+      Acc v = acc_add(off, loc[k]);
+      pm[s + 1] = v.m;
+      pxh[s + 1] = v.x.hi;
+      pxl[s + 1] = v.x.lo;
+      pyh[s + 1] = v.y.hi;
+      pyl[s + 1] = v.y.lo;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    pm[0] = 0.0;
+    pxh[0] = pxl[0] = pyh[0] = pyl[0] = 0.0;
+  }
+}
+
+// -------------------------------------------------------- FA2 speed update
+// src/embedders/gvizForceEmbedder.c:140-185, with S and T reduced from the
+// per-block FP64 partials of the force kernel.
+__global__ void k_fa2(const double *__restrict__ partial, int blocks, int64_t n,
+                      double jitterTolerance, Scalars *__restrict__ sc,
+                      StepRecord *__restrict__ rec, int64_t recCap) {
+  __shared__ double smem[2 * 16];
+  if (sc->stopped)
+    return;
+  double v[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] += partial[2 * b];
+    v[1] += partial[2 * b + 1];
+  }
+  const bool isMax[2] = {false, false};
+  block_reduce<2>(v, isMax, smem);
+  if (threadIdx.x != 0)
+    return;
+  double totalSwinging = v[0], totalEffectiveTraction = v[1];
+  int64_t slot = sc->iteration - sc->recordBase;
+  if (slot >= 0 && slot < recCap) {
+    rec[slot].totalSwinging = totalSwinging;
+    rec[slot].totalTraction = totalEffectiveTraction;
+  }
+  if (totalEffectiveTraction < 1e-7)
+    return;
+  double gs = sc->globalSpeed, eff = sc->speedEfficiency;
+  double nn = (double)n;
+  double estimatedOptimalJT = 0.05 * sqrt(nn);
+  double minJT = sqrt(estimatedOptimalJT);
+  double maxJT = 10.0;
+  double jt = jitterTolerance *
+              fmax(minJT, fmin(maxJT, estimatedOptimalJT *
+                                          totalEffectiveTraction / (nn * nn)));
+  if (totalSwinging / totalEffectiveTraction > 2.0) {
+    if (eff > 0.05)
+      eff *= 0.5;
+    jt = fmax(jt, jitterTolerance);
+  }
+  double targetSpeed = totalSwinging > 1e-7
+                           ? jt * eff * totalEffectiveTraction / totalSwinging
+                           : gs;
+  if (totalSwinging > jt * totalEffectiveTraction) {
+    if (eff > 0.05)
+      eff *= 0.7;
+  } else if (gs < 1000.0) {
+    eff *= 1.3;
+  }
+  if (eff < 0.05)
+    eff = 0.05;
+  else if (eff > 1.0)
+    eff = 1.0;
+  gs += fmin(targetSpeed - gs, 0.5 * gs);
+  sc->globalSpeed = gs;
+  sc->speedEfficiency = eff;
+}
+
+// Step tail (src/embedders/gvizForceEmbedder.c:467-494): max |applied|, mean
+// |att|, mean |rep|, mean gravity; advances the iteration and, for Run,
+// raises the stop flag once maxDisp < epsilon.
+__global__ void k_stats(const double *__restrict__ partial, int blocks,
+                        int64_t n, double gravityK, Scalars *__restrict__ sc,
+                        StepRecord *__restrict__ rec, int64_t recCap) {
+  __shared__ double smem[3 * 16];
+  if (sc->stopped)
+    return;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] = fmax(v[0], partial[3 * b]);
+    v[1] += partial[3 * b + 1];
+    v[2] += partial[3 * b + 2];
+  }
+  const bool isMax[3] = {true, false, false};
+  block_reduce<3>(v, isMax, smem);
+  if (threadIdx.x != 0)
+    return;
+  double maxDisp = v[0];
+  int64_t slot = sc->iteration - sc->recordBase;
+  if (slot >= 0 && slot < recCap) {
+    StepRecord &r = rec[slot];
+    r.maxDisp = maxDisp;
+    r.speed = sc->globalSpeed;
+    r.meanAtt = n ? v[1] / (double)n : 0.0;
+    r.meanRep = n ? v[2] / (double)n : 0.0;
+    r.meanGrav = gravityK * sc->meanDegreePlusOne;
+    r.speedEfficiency = sc->speedEfficiency;
+  }
+  sc->iteration++;
+  if (sc->epsilon >= 0.0 && maxDisp < sc->epsilon)
+    sc->stopped = 1;
+}
+
+} // namespace
+
+int ForceCommon::alloc(int64_t nv, int64_t ne) {
+  n = nv;
+  m2 = ne;
+  const int64_t n1 = nv > 0 ? nv : 1;
+  if (pos64.alloc(2 * n1) || csrOff.alloc(n1 + 1) || csrAdj.alloc(ne > 0 ? ne : 1) ||
+      degree.alloc(n1) || mass64.alloc(n1) || radius64.alloc(n1))
+    return -1;
+  if (keyHi.alloc(n1) || keyLo.alloc(n1) || keyLoS.alloc(n1) || vals.alloc(n1) ||
+      sorter.reserve(n1))
+    return -1;
+  if (left.alloc(n1) || right.alloc(n1) || parent.alloc(n1) || first.alloc(n1) ||
+      last.alloc(n1) || delta.alloc(n1))
+    return -1;
+  if (pm.alloc(n1 + 1) || pmxHi.alloc(n1 + 1) || pmxLo.alloc(n1 + 1) ||
+      pmyHi.alloc(n1 + 1) || pmyLo.alloc(n1 + 1))
+    return -1;
+  const int64_t tiles = (n1 + kScanTile - 1) / kScanTile;
+  if (scanPartial.alloc(5 * tiles))
+    return -1;
+  if (nodes.alloc((size_t)n1 * sizeof(NodeRec<double, double>)))
+    return -1;
+  const size_t w = sizeof(double);
+  if (posT.alloc(2 * n1 * w) || spos.alloc(2 * n1 * w) || smr.alloc(2 * n1 * w) ||
+      rep.alloc(2 * n1 * w) || disp.alloc(2 * n1 * w) || sf.alloc(2 * n1 * w) ||
+      sw.alloc(n1 * w) || attMag.alloc(n1 * w) || repMag.alloc(n1 * w) ||
+      massT.alloc(n1 * w) || radT.alloc(n1 * w) || spos64.alloc(2 * n1))
+    return -1;
+  reductionBlocks = 1024;
+  if (partials.alloc(4 * (size_t)reductionBlocks) || sc.alloc(1) ||
+      interactions.alloc(n1))
+    return -1;
+  return 0;
+}
+
+// Bounding box, keys, sort and radix tree (everything before the typed
+// gather). Typed gather + prefix sums + quad records follow in the typed TU.
+int enqueue_keys_and_sort(ForceCommon &c) {
+  const int64_t n = c.n;
+  hipStream_t st = c.stream;
+  const int bb = (int)(grid_for(n, kBlock) < 1024 ? grid_for(n, kBlock) : 1024);
+  hipLaunchKernelGGL(k_bbox, dim3(bb), dim3(kBlock), 0, st,
+                     (const double2 *)c.pos64.p, n, c.partials.p);
+  hipLaunchKernelGGL(k_root, dim3(1), dim3(256), 0, st, c.partials.p, bb, n,
+                     c.sc.p);
+  hipLaunchKernelGGL(k_keys, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     (const double2 *)c.pos64.p, n, c.sc.p, c.keyHi.p, c.keyLo.p,
+                     c.vals.p);
+  GVH_CHECK_LAUNCH();
+  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 0, 64, st))
+    return -1;
+  hipLaunchKernelGGL(k_fix_runs, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int enqueue_radix_tree(ForceCommon &c) {
+  const int64_t n = c.n;
+  hipStream_t st = c.stream;
+  if (n >= 2)
+    hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, kBlock)), dim3(kBlock), 0,
+                       st, c.keyHi.p, c.keyLoS.p, n, c.left.p, c.right.p,
+                       c.parent.p, c.first.p, c.last.p, c.delta.p, c.sc.p);
+  const int tiles = (int)((n + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(tiles > 0 ? tiles : 1), dim3(kBlock), 0,
+                     st, c.spos64.p, c.mass64.p, c.vals.p, n, c.scanPartial.p,
+                     c.sc.p);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, c.scanPartial.p,
+                     tiles, c.sc.p);
+  hipLaunchKernelGGL(k_scan_final, dim3(tiles > 0 ? tiles : 1), dim3(kBlock), 0,
+                     st, c.spos64.p, c.mass64.p, c.vals.p, n, c.scanPartial.p,
+                     c.pm.p, c.pmxHi.p, c.pmxLo.p, c.pmyHi.p, c.pmyLo.p, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int enqueue_fa2(ForceCommon &c, int blocks) {
+  hipLaunchKernelGGL(k_fa2, dim3(1), dim3(256), 0, c.stream, c.partials.p, blocks,
+                     c.n, c.prm.jitterTolerance, c.sc.p, c.records.p, c.recordCap);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int enqueue_stats(ForceCommon &c, int blocks) {
+  hipLaunchKernelGGL(k_stats, dim3(1), dim3(256), 0, c.stream, c.partials.p,
+                     blocks, c.n, c.prm.gravityK, c.sc.p, c.records.p,
+                     c.recordCap);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+} // namespace gvh

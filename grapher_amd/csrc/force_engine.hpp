@@ -1,0 +1,125 @@
+// force_engine.hpp -- device state and pipeline of the force-directed step
+// (the reference's gvizForceEmbedderStep, src/embedders/gvizForceEmbedder.c:
+// 445-495) on one gfx950 GPU.
+//
+// Data layout in HBM (n = active vertices, compact ids in ascending parent id
+// order, m2 = directed adjacency entries):
+//   pos64      double2[n]   master positions (FP64, compact order)
+//   csrOff/Adj int32        adjacency in insertion order (attraction order)
+//   keyHi/Lo   uint64[n]    quadrant-path keys, 2 bits per level
+//   vals       uint32[n]    original ids, permuted by the key sort
+//   sorted SoA  spos/smr    positions and (mass, radius) in key order
+//   tree       int32        binary radix tree over the sorted keys (Karras);
+//                           the quadtree is its subset of "quad heads"
+//   prefix     double-double prefix sums of m, m*x, m*y in key order: the
+//                           mass and centre of mass of any key range in O(1)
+//   nodes      NodeRec[n]   traversal records of the quad heads
+//   forces     typed        rep, disp, structForce, swinging, |att|, |rep|
+#pragma once
+
+#include "common.hpp"
+#include "radix_sort.hpp"
+
+namespace gvh {
+
+enum Precision : int32_t { kFP32 = 0, kFP64 = 1, kMixed = 2 };
+
+struct ForceParams {
+  int32_t model;          // 0 FR, 1 LinLog
+  int32_t barnesHut;
+  int32_t preventOverlap;
+  int32_t precision;
+  int64_t nodesPerCell;
+  double edgeLength;
+  double theta;
+  double gravityK;
+  double overlapConstant;
+  double radiusBase;
+  double radiusPerDegree;
+  double jitterTolerance;
+};
+
+// One record per step, written on the device by the stats kernel.
+struct StepRecord {
+  double maxDisp, speed, meanAtt, meanRep, meanGrav;
+  double speedEfficiency, totalSwinging, totalTraction;
+};
+
+// Device-resident scalars of one embedder.
+struct Scalars {
+  double rootCx, rootCy, rootHalf;
+  int32_t maxDepth;  // levels that carry digits (capped at 64)
+  int32_t rootRef;
+  double globalSpeed, speedEfficiency;
+  double epsilon;    // Run's threshold; < 0 disables the stop flag
+  int64_t iteration; // steps taken since Begin
+  int64_t recordBase;// iteration whose record lands in records[0]
+  int32_t stopped;
+  int32_t traversalOverflow;
+  double meanDegreePlusOne;
+  double pad[4];
+};
+
+constexpr int32_t kNone = INT32_MIN;
+
+template <class Real, class PosT> struct NodeRec {
+  PosT comx, comy;
+  Real mass, twoh;
+  int32_t child[4];
+  int32_t first, count; // count > 0: leaf holding sorted points [first, first+count)
+};
+
+struct ForceCommon {
+  int64_t n = 0;
+  int64_t m2 = 0;
+  ForceParams prm{};
+  hipStream_t stream = nullptr;
+
+  DevBuf<double> pos64;       // 2n
+  DevBuf<int32_t> csrOff;     // n+1
+  DevBuf<int32_t> csrAdj;     // m2
+  DevBuf<int32_t> degree;     // n
+  DevBuf<double> mass64;      // n  (exact: 1 or 1+deg)
+  DevBuf<double> radius64;    // n  (0 when overlap prevention is off)
+
+  DevBuf<uint64_t> keyHi, keyLo, keyLoS;
+  DevBuf<uint32_t> vals;
+  RadixSorter sorter;
+
+  DevBuf<int32_t> left, right, parent, first, last, delta;
+  DevBuf<double> pm, pmxHi, pmxLo, pmyHi, pmyLo; // n+1 prefix sums
+  DevBuf<double> scanPartial;                    // 5 per scan block
+  DevBuf<unsigned char> nodes;                   // NodeRec[n] (typed)
+
+  // typed (allocated for the widest type)
+  DevBuf<unsigned char> posT, spos, smr, rep, disp, sf, sw, attMag, repMag,
+      massT, radT;
+  DevBuf<double> spos64;   // 2n, FP64 sorted positions (COM prefix sums)
+  DevBuf<double> partials; // reduction partials
+  DevBuf<Scalars> sc;
+  DevBuf<StepRecord> records;
+  int64_t recordCap = 0;
+  DevBuf<int32_t> interactions; // optional per-vertex repulsion count
+  int wantInteractions = 0;
+  // measurement hook: events around the traversal kernel (never captured)
+  hipEvent_t travBegin = nullptr, travEnd = nullptr;
+  int timeTraversal = 0;
+  int64_t stepsSinceRecordBase = 0;
+  int reductionBlocks = 0;
+
+  int alloc(int64_t nVertices, int64_t nEntries);
+};
+
+// Step modes: the whole step, the force evaluation without the speed update
+// and position update, or only the quadtree build.
+enum StepMode : int { kStepFull = 0, kStepForcesOnly = 1, kStepTreeOnly = 2 };
+
+// Enqueues one step on c.stream. Instantiated in force_f64.hip
+// (<double,double>) and force_f32.hip (<float,float>, <float,double>).
+template <class Real, class PosT> int typed_step(ForceCommon &c, int mode);
+// Refreshes the typed compact-order mass and radius arrays.
+template <class Real> int typed_upload(ForceCommon &c);
+
+const char *last_error();
+
+} // namespace gvh

@@ -1,0 +1,647 @@
+// force_typed.hpp -- precision-typed stages of the force step.
+//
+// Real = arithmetic type of the force math and its accumulation;
+// PosT = type positions are differenced in. Instantiated as
+//   <double, double>  FP64: the reference's formulas in the reference's
+//                     evaluation order (force_f64.hip, no FMA contraction);
+//   <float,  double>  mixed: FP64 position differences, FP32 force math;
+//   <float,  float>   FP32 throughout (force_f32.hip).
+#pragma once
+
+#include "dd.hpp"
+#include "force_engine.hpp"
+
+namespace gvh {
+
+template <class T> struct V2;
+template <> struct V2<float> { using type = float2; };
+template <> struct V2<double> { using type = double2; };
+
+constexpr int kTB = 256;            // threads per block
+constexpr int kTW = kTB / kWave;    // waves per block
+constexpr int kStack = 256;         // traversal stack entries per wave
+
+// Direction for exactly coincident points: splitmix64 finaliser over the two
+// vertex ids mapped to an angle. The reference hashes buffer addresses here
+// (include/core/gvizVec.h:385-394), which nothing else can reproduce.
+static __device__ __noinline__ void coincident_dir(int64_t selfId, int64_t otherId,
+                                            double &ux, double &uy) {
+  uint64_t bits = (uint64_t)selfId ^ ((uint64_t)otherId * 0x9E3779B97F4A7C15ull);
+  bits ^= bits >> 33;
+  bits *= 0xff51afd7ed558ccdull;
+  bits ^= bits >> 33;
+  double angle = (double)(bits % 1000000u) / 1000000.0 * 2.0 * M_PI;
+  ux = cos(angle);
+  uy = sin(angle);
+}
+
+// ------------------------------------------------------- pairwise physics
+// MODEL 0 = Fruchterman-Reingold, 1 = LinLog. `mOther` is the other body's
+// mass (a pseudo-body's aggregated mass); FR scales by its integer part
+// (src/embedders/gvizForceModel.c:15-23), LinLog multiplies the masses
+// (include/core/gvizVec.h:552-575).
+template <class Real> struct Phys;
+
+template <> struct Phys<double> {
+  // FP64: the reference's operation order; sqrt and division are correctly
+  // rounded, and this TU is compiled without FMA contraction.
+  template <int MODEL>
+  __device__ __forceinline__ static void rep(double dx, double dy, double dist,
+                                             double radiusSum, double mSelf,
+                                             double mOther, double kSq,
+                                             double oc, int64_t selfId,
+                                             int64_t otherId, double &ax,
+                                             double &ay) {
+    double gap = dist - radiusSum;
+    double ux, uy;
+    if (dist > 0.0) {
+      ux = dx / dist;
+      uy = dy / dist;
+    } else {
+      coincident_dir(selfId, otherId, ux, uy);
+    }
+    if (MODEL == 1) {
+      double mp = mSelf * mOther;
+      double mag = gap > 0.0 ? mp / gap : mp * oc;
+      ax -= mag * ux;
+      ay -= mag * uy;
+    } else {
+      double mag = gap > 0.0 ? kSq / gap : kSq * oc;
+      double t0 = 0.0 - mag * ux;
+      double t1 = 0.0 - mag * uy;
+      double m = trunc(mOther);
+      ax += m * t0;
+      ay += m * t1;
+    }
+  }
+  __device__ __forceinline__ static double dist(double dx, double dy) {
+    return sqrt(dx * dx + dy * dy);
+  }
+  // opening test of src/embedders/gvizForceEmbedder.c:58-63
+  __device__ __forceinline__ static bool accept(double twoh, double dist,
+                                                double theta) {
+    return (twoh / dist) < theta;
+  }
+  template <int MODEL>
+  __device__ __forceinline__ static void att(double dx, double dy, double k,
+                                             double &ax, double &ay) {
+    double dist = sqrt(dx * dx + dy * dy);
+    if (dist < 1e-9)
+      return;
+    double s = MODEL == 1 ? log(1.0 + dist) / dist : dist / k;
+    ax += s * dx;
+    ay += s * dy;
+  }
+  __device__ __forceinline__ static void gravity(double x, double y, double mag,
+                                                 double &fx, double &fy) {
+    double dist = sqrt(x * x + y * y);
+    if (dist < 1e-9)
+      return;
+    double s = mag / dist;
+    fx -= s * x;
+    fy -= s * y;
+  }
+  __device__ __forceinline__ static double norm(double x, double y) {
+    return sqrt(x * x + y * y);
+  }
+};
+
+template <> struct Phys<float> {
+  // FP32: hardware reciprocal and reciprocal square root (1 ulp each).
+  template <int MODEL>
+  __device__ __forceinline__ static void rep(float dx, float dy, float dist,
+                                             float radiusSum, float mSelf,
+                                             float mOther, float kSq, float oc,
+                                             int64_t selfId, int64_t otherId,
+                                             float &ax, float &ay) {
+    float gap = dist - radiusSum;
+    float ux, uy;
+    if (dist > 0.0f) {
+      float inv = __builtin_amdgcn_rcpf(dist);
+      ux = dx * inv;
+      uy = dy * inv;
+    } else {
+      double dux, duy;
+      coincident_dir(selfId, otherId, dux, duy);
+      ux = (float)dux;
+      uy = (float)duy;
+    }
+    float num = MODEL == 1 ? mSelf * mOther : kSq * truncf(mOther);
+    float mag = gap > 0.0f ? num * __builtin_amdgcn_rcpf(gap) : num * oc;
+    ax -= mag * ux;
+    ay -= mag * uy;
+  }
+  __device__ __forceinline__ static float dist(float dx, float dy) {
+    float d2 = dx * dx + dy * dy;
+    return d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+  }
+  __device__ __forceinline__ static bool accept(float twoh, float dist,
+                                                float theta) {
+    return twoh < theta * dist;
+  }
+  template <int MODEL>
+  __device__ __forceinline__ static void att(float dx, float dy, float k,
+                                             float &ax, float &ay) {
+    float d2 = dx * dx + dy * dy;
+    float dist = d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+    if (dist < 1e-9f)
+      return;
+    float s = MODEL == 1 ? log1pf(dist) * __builtin_amdgcn_rcpf(dist)
+                         : dist * __builtin_amdgcn_rcpf(k);
+    ax += s * dx;
+    ay += s * dy;
+  }
+  __device__ __forceinline__ static void gravity(float x, float y, float mag,
+                                                 float &fx, float &fy) {
+    float d2 = x * x + y * y;
+    float dist = d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+    if (dist < 1e-9f)
+      return;
+    float s = mag * __builtin_amdgcn_rcpf(dist);
+    fx -= s * x;
+    fy -= s * y;
+  }
+  __device__ __forceinline__ static float norm(float x, float y) {
+    float d2 = x * x + y * y;
+    return d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+  }
+};
+
+// ------------------------------------------------------------ kernels
+
+// Sorted copies: FP64 positions (prefix sums), typed positions and
+// (mass, radius) for the traversal, and the low key word for the tree.
+template <class Real, class PosT>
+__global__ __launch_bounds__(kTB) void k_gather(
+    const double2 *__restrict__ pos64, const double *__restrict__ mass64,
+    const double *__restrict__ radius64, const uint32_t *__restrict__ vals,
+    const uint64_t *__restrict__ keyLo, int64_t n, double2 *__restrict__ spos64,
+    typename V2<PosT>::type *__restrict__ spos,
+    typename V2<Real>::type *__restrict__ smr, uint64_t *__restrict__ keyLoS,
+    const Scalars *__restrict__ sc) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || sc->stopped)
+    return;
+  uint32_t i = vals[s];
+  double2 p = pos64[i];
+  spos64[s] = p;
+  spos[s] = {(PosT)p.x, (PosT)p.y};
+  smr[s] = {(Real)mass64[i], (Real)radius64[i]};
+  keyLoS[s] = keyLo[i];
+}
+
+// Positions in compact order for attraction and gravity (FP32 mode only).
+template <class PosT>
+__global__ __launch_bounds__(kTB) void k_post(const double2 *__restrict__ pos64,
+                                              int64_t n,
+                                              typename V2<PosT>::type *__restrict__ out,
+                                              const Scalars *__restrict__ sc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || sc->stopped)
+    return;
+  double2 p = pos64[i];
+  out[i] = {(PosT)p.x, (PosT)p.y};
+}
+
+__device__ __forceinline__ int quad_depth(int delta, int maxDepth) {
+  int q = delta >> 1;
+  return q < maxDepth ? q : maxDepth;
+}
+
+// One traversal record per quad head: a binary radix node whose quad depth
+// floor(delta/2) exceeds its parent's. Heads holding <= nodesPerCell points,
+// or sitting at the depth limit, are leaves (the reference's leaf cells and
+// overflow leaves, src/ds/gvizQuadtree.c:129-139); the others are the
+// reference's branching cells (chains of single-child cells collapse onto
+// the deepest one, which carries the same mass and centre of mass and a
+// smaller size, so it takes the same opening decisions).
+template <class Real, class PosT>
+__global__ __launch_bounds__(kTB) void k_quad(
+    const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+    const int32_t *__restrict__ parent, const int32_t *__restrict__ first,
+    const int32_t *__restrict__ last, const int32_t *__restrict__ delta,
+    const double *__restrict__ pm, const double *__restrict__ pxh,
+    const double *__restrict__ pxl, const double *__restrict__ pyh,
+    const double *__restrict__ pyl, int64_t n, int64_t npc,
+    NodeRec<Real, PosT> *__restrict__ nodes, const Scalars *__restrict__ sc) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n - 1 || sc->stopped)
+    return;
+  const int maxDepth = sc->maxDepth;
+  const int q = quad_depth(delta[b], maxDepth);
+  if (b != 0 && quad_depth(delta[parent[b]], maxDepth) >= q)
+    return;
+  NodeRec<Real, PosT> r;
+  const int32_t f = first[b], l = last[b];
+  const int32_t count = l - f + 1;
+  r.first = f;
+  r.child[0] = r.child[1] = r.child[2] = r.child[3] = kNone;
+  if (count <= npc || q >= maxDepth) {
+    r.count = count;
+    r.comx = r.comy = (PosT)0;
+    r.mass = r.twoh = (Real)0;
+    nodes[b] = r;
+    return;
+  }
+  r.count = 0;
+  double M = pm[l + 1] - pm[f];
+  dd sx = dd_add({pxh[l + 1], pxl[l + 1]}, {-pxh[f], -pxl[f]});
+  dd sy = dd_add({pyh[l + 1], pyl[l + 1]}, {-pyh[f], -pyl[f]});
+  r.comx = (PosT)((sx.hi + sx.lo) / M);
+  r.comy = (PosT)((sy.hi + sy.lo) / M);
+  r.mass = (Real)M;
+  r.twoh = (Real)(2.0 * ldexp(sc->rootHalf, -q));
+  int k = 0;
+  const int32_t kids[2] = {left[b], right[b]};
+  for (int c = 0; c < 2; c++) {
+    int32_t ch = kids[c];
+    if (ch < 0 || quad_depth(delta[ch], maxDepth) > q) {
+      r.child[k++] = ch;
+    } else {
+      r.child[k++] = left[ch];
+      r.child[k++] = right[ch];
+    }
+  }
+  nodes[b] = r;
+}
+
+// Barnes-Hut repulsion, one wavefront per 64 consecutive targets in key
+// order. The wavefront walks the union of its lanes' trees with one shared
+// stack of (node, lane mask) entries: every lane takes its own opening
+// decision (the reference's per-vertex recursion, src/embedders/
+// gvizForceEmbedder.c:35-74), node records are fetched once per wavefront,
+// and children are popped in NW, NE, SW, SE order so each lane accumulates
+// in the reference's depth-first order.
+template <class Real, class PosT, int MODEL>
+__global__ __launch_bounds__(kTB) void k_traverse(
+    const NodeRec<Real, PosT> *__restrict__ nodes,
+    const typename V2<PosT>::type *__restrict__ spos,
+    const typename V2<Real>::type *__restrict__ smr,
+    const uint32_t *__restrict__ sidx, int64_t n, Real theta, Real kSq, Real oc,
+    typename V2<Real>::type *__restrict__ rep, int32_t *__restrict__ inter,
+    Scalars *__restrict__ sc) {
+  __shared__ int32_t sNode[kTW][kStack];
+  __shared__ uint64_t sMask[kTW][kStack];
+  if (sc->stopped)
+    return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int64_t s = ((int64_t)blockIdx.x * kTW + wave) * kWave + lane;
+  const bool valid = s < n;
+  const uint64_t live = __ballot(valid);
+  if (live == 0)
+    return;
+  typename V2<PosT>::type me = valid ? spos[s] : typename V2<PosT>::type{0, 0};
+  typename V2<Real>::type mr = valid ? smr[s] : typename V2<Real>::type{0, 0};
+  const int64_t selfId = valid ? (int64_t)sidx[s] : 0;
+  Real ax = 0, ay = 0;
+  int32_t cnt = 0;
+  int sp = 0;
+  sNode[wave][0] = n >= 2 ? 0 : -1;
+  sMask[wave][0] = live;
+  sp = 1;
+  while (sp > 0) {
+    sp--;
+    const int32_t node = __builtin_amdgcn_readfirstlane(sNode[wave][sp]);
+    const uint64_t mask = sMask[wave][sp];
+    const bool in = (mask >> lane) & 1ull;
+    if (node < 0) {
+      const int32_t p = -node - 1;
+      if (in && p != s) {
+        typename V2<PosT>::type o = spos[p];
+        typename V2<Real>::type om = smr[p];
+        Real dx = (Real)(o.x - me.x), dy = (Real)(o.y - me.y);
+        Real dist = Phys<Real>::dist(dx, dy);
+        Phys<Real>::template rep<MODEL>(dx, dy, dist, mr.y + om.y, mr.x, om.x,
+                                        kSq, oc, selfId, (int64_t)sidx[p], ax, ay);
+        cnt++;
+      }
+      continue;
+    }
+    const NodeRec<Real, PosT> r = nodes[node];
+    if (r.count > 0) {
+      for (int32_t k = 0; k < r.count; k++) {
+        const int32_t p = r.first + k;
+        if (in && p != s) {
+          typename V2<PosT>::type o = spos[p];
+          typename V2<Real>::type om = smr[p];
+          Real dx = (Real)(o.x - me.x), dy = (Real)(o.y - me.y);
+          Real dist = Phys<Real>::dist(dx, dy);
+          Phys<Real>::template rep<MODEL>(dx, dy, dist, mr.y + om.y, mr.x, om.x,
+                                          kSq, oc, selfId, (int64_t)sidx[p], ax,
+                                          ay);
+          cnt++;
+        }
+      }
+      continue;
+    }
+    bool open = false;
+    if (in) {
+      Real dx = (Real)(r.comx - me.x), dy = (Real)(r.comy - me.y);
+      Real dist = Phys<Real>::dist(dx, dy);
+      if (Phys<Real>::accept(r.twoh, dist, theta)) {
+        Phys<Real>::template rep<MODEL>(dx, dy, dist, mr.y + (Real)0, mr.x,
+                                        r.mass, kSq, oc, selfId, n, ax, ay);
+        cnt++;
+      } else {
+        open = true;
+      }
+    }
+    const uint64_t om = __ballot(open);
+    if (om) {
+      if (sp + 4 > kStack) {
+        sc->traversalOverflow = 1;
+        break;
+      }
+#pragma unroll
+      for (int c = 3; c >= 0; c--) {
+        if (r.child[c] != kNone) {
+          sNode[wave][sp] = r.child[c];
+          sMask[wave][sp] = om;
+          sp++;
+        }
+      }
+    }
+  }
+  if (valid) {
+    rep[selfId] = {ax, ay};
+    if (inter)
+      inter[selfId] = cnt;
+  }
+}
+
+// Exact all-pairs repulsion (src/embedders/gvizForceEmbedder.c:104-115):
+// sources stream through LDS in tiles of 256, in ascending id order.
+template <class Real, class PosT, int MODEL>
+__global__ __launch_bounds__(kTB) void k_allpairs(
+    const typename V2<PosT>::type *__restrict__ pos,
+    const Real *__restrict__ massT, const Real *__restrict__ radT, int64_t n,
+    Real kSq, Real oc, typename V2<Real>::type *__restrict__ rep,
+    int32_t *__restrict__ inter, const Scalars *__restrict__ sc) {
+  __shared__ typename V2<PosT>::type tPos[kTB];
+  __shared__ Real tMass[kTB];
+  __shared__ Real tRad[kTB];
+  if (sc->stopped)
+    return;
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const bool valid = i < n;
+  typename V2<PosT>::type me = valid ? pos[i] : typename V2<PosT>::type{0, 0};
+  const Real mv = valid ? massT[i] : (Real)0;
+  const Real rv = valid ? radT[i] : (Real)0;
+  Real ax = 0, ay = 0;
+  for (int64_t jb = 0; jb < n; jb += kTB) {
+    const int64_t j = jb + threadIdx.x;
+    if (j < n) {
+      tPos[threadIdx.x] = pos[j];
+      tMass[threadIdx.x] = massT[j];
+      tRad[threadIdx.x] = radT[j];
+    }
+    __syncthreads();
+    const int len = (int)((n - jb) < kTB ? (n - jb) : kTB);
+    if (valid) {
+      for (int t = 0; t < len; t++) {
+        const int64_t jj = jb + t;
+        if (jj == i)
+          continue;
+        Real dx = (Real)(tPos[t].x - me.x), dy = (Real)(tPos[t].y - me.y);
+        Real dist = Phys<Real>::dist(dx, dy);
+        Phys<Real>::template rep<MODEL>(dx, dy, dist, rv + tRad[t], mv, tMass[t],
+                                        kSq, oc, i, jj, ax, ay);
+      }
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    rep[i] = {ax, ay};
+    if (inter)
+      inter[i] = (int32_t)(n - 1);
+  }
+}
+
+// Attraction along the adjacency in insertion order, then the per-vertex
+// epilogue of src/embedders/gvizForceEmbedder.c:117-124 and the swinging /
+// traction of :128-138; per-block FP64 partial sums of both go to partial.
+template <class Real, class PosT, int MODEL>
+__global__ __launch_bounds__(kTB) void k_forces(
+    const typename V2<PosT>::type *__restrict__ pos,
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const int32_t *__restrict__ degree, const Real *__restrict__ massT,
+    const typename V2<Real>::type *__restrict__ rep, int64_t n, Real edgeLength,
+    double gravityK, typename V2<Real>::type *__restrict__ sf,
+    typename V2<Real>::type *__restrict__ disp, Real *__restrict__ swOut,
+    Real *__restrict__ attMagOut, Real *__restrict__ repMagOut,
+    double *__restrict__ partial, const Scalars *__restrict__ sc) {
+  __shared__ double smem[2 * 16];
+  if (sc->stopped)
+    return;
+  double accS = 0.0, accT = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kTB) {
+    typename V2<PosT>::type v = pos[i];
+    Real ax = 0, ay = 0;
+    const int32_t e1 = off[i + 1];
+    for (int32_t e = off[i]; e < e1; e++) {
+      typename V2<PosT>::type u = pos[adj[e]];
+      Phys<Real>::template att<MODEL>((Real)(u.x - v.x), (Real)(u.y - v.y),
+                                      edgeLength, ax, ay);
+    }
+    typename V2<Real>::type rp = rep[i];
+    Real fx = (Real)0, fy = (Real)0;
+    fx += (Real)1 * ax;
+    fy += (Real)1 * ay;
+    fx += (Real)1 * rp.x;
+    fy += (Real)1 * rp.y;
+    attMagOut[i] = Phys<Real>::norm(ax, ay);
+    repMagOut[i] = Phys<Real>::norm(rp.x, rp.y);
+    typename V2<Real>::type old = sf[i];
+    sf[i] = {fx, fy};
+    const Real m = massT[i];
+    Real sw = m * Phys<Real>::norm(fx - old.x, fy - old.y);
+    Real tr = m * (Real)0.5 * Phys<Real>::norm(fx + old.x, fy + old.y);
+    Real gmag = (Real)(gravityK * (double)(degree[i] + 1));
+    Phys<Real>::gravity((Real)v.x, (Real)v.y, gmag, fx, fy);
+    disp[i] = {fx, fy};
+    swOut[i] = sw;
+    accS += (double)sw;
+    accT += (double)tr;
+  }
+  double vals[2] = {accS, accT};
+  const bool isMax[2] = {false, false};
+  block_reduce<2>(vals, isMax, smem);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = vals[0];
+    partial[2 * blockIdx.x + 1] = vals[1];
+  }
+}
+
+// Speed-scaled displacement (src/embedders/gvizForceEmbedder.c:187-197) and
+// the position update of :467-477, in FP64 on the master positions.
+template <class Real>
+__global__ __launch_bounds__(kTB) void k_apply(
+    const typename V2<Real>::type *__restrict__ disp,
+    const Real *__restrict__ sw, const Real *__restrict__ attMag,
+    const Real *__restrict__ repMag, int64_t n, double2 *__restrict__ pos64,
+    double *__restrict__ partial, const Scalars *__restrict__ sc) {
+  __shared__ double smem[3 * 16];
+  if (sc->stopped)
+    return;
+  const double gs = sc->globalSpeed;
+  double vmax = 0.0, sa = 0.0, sr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kTB) {
+    double factor = gs / (1.0 + sqrt(gs * (double)sw[i]));
+    typename V2<Real>::type d = disp[i];
+    double o0 = (double)d.x * factor;
+    double o1 = (double)d.y * factor;
+    double applied = sqrt(o0 * o0 + o1 * o1);
+    vmax = applied > vmax ? applied : vmax;
+    sa += (double)attMag[i];
+    sr += (double)repMag[i];
+    double2 p = pos64[i];
+    p.x += o0;
+    p.y += o1;
+    pos64[i] = p;
+  }
+  double vals[3] = {vmax, sa, sr};
+  const bool isMax[3] = {true, false, false};
+  block_reduce<3>(vals, isMax, smem);
+  if (threadIdx.x == 0) {
+    partial[3 * blockIdx.x] = vals[0];
+    partial[3 * blockIdx.x + 1] = vals[1];
+    partial[3 * blockIdx.x + 2] = vals[2];
+  }
+}
+
+// Typed copies of mass and radius in compact order (all-pairs and forces).
+template <class Real>
+__global__ __launch_bounds__(kTB) void k_typed_mr(const double *__restrict__ m64,
+                                                  const double *__restrict__ r64,
+                                                  int64_t n, Real *__restrict__ m,
+                                                  Real *__restrict__ r) {
+  int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i >= n)
+    return;
+  m[i] = (Real)m64[i];
+  r[i] = (Real)r64[i];
+}
+
+// ------------------------------------------------------------ host glue
+int enqueue_keys_and_sort(ForceCommon &c);
+int enqueue_radix_tree(ForceCommon &c);
+int enqueue_fa2(ForceCommon &c, int blocks);
+int enqueue_stats(ForceCommon &c, int blocks);
+
+template <class Real, class PosT>
+int typed_step(ForceCommon &c, int mode) {
+  using P2 = typename V2<PosT>::type;
+  using R2 = typename V2<Real>::type;
+  const int64_t n = c.n;
+  hipStream_t st = c.stream;
+  if (n == 0)
+    return 0;
+  const Real kSq = (Real)(c.prm.edgeLength * c.prm.edgeLength);
+  const Real oc = (Real)c.prm.overlapConstant;
+  const Real theta = (Real)c.prm.theta;
+  int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
+  const P2 *posC;
+  if (sizeof(PosT) == sizeof(double)) {
+    posC = (const P2 *)c.pos64.p;
+  } else {
+    hipLaunchKernelGGL(k_post<PosT>, dim3(grid_for(n, kTB)), dim3(kTB), 0, st,
+                       (const double2 *)c.pos64.p, n, (P2 *)c.posT.p, c.sc.p);
+    posC = (const P2 *)c.posT.p;
+  }
+  const int model = c.prm.model == 1 ? 1 : 0;
+  if (c.prm.barnesHut || mode == kStepTreeOnly) {
+    if (enqueue_keys_and_sort(c))
+      return -1;
+    hipLaunchKernelGGL((k_gather<Real, PosT>), dim3(grid_for(n, kTB)), dim3(kTB),
+                       0, st, (const double2 *)c.pos64.p, c.mass64.p,
+                       c.radius64.p, c.vals.p, c.keyLo.p, n,
+                       (double2 *)c.spos64.p, (P2 *)c.spos.p, (R2 *)c.smr.p,
+                       c.keyLoS.p, c.sc.p);
+    GVH_CHECK_LAUNCH();
+    if (enqueue_radix_tree(c))
+      return -1;
+    if (n >= 2)
+      hipLaunchKernelGGL((k_quad<Real, PosT>), dim3(grid_for(n - 1, kTB)),
+                         dim3(kTB), 0, st, c.left.p, c.right.p, c.parent.p,
+                         c.first.p, c.last.p, c.delta.p, c.pm.p, c.pmxHi.p,
+                         c.pmxLo.p, c.pmyHi.p, c.pmyLo.p, n,
+                         c.prm.nodesPerCell > 0 ? c.prm.nodesPerCell : 1,
+                         (NodeRec<Real, PosT> *)c.nodes.p, c.sc.p);
+    if (mode == kStepTreeOnly) {
+      GVH_CHECK_LAUNCH();
+      return 0;
+    }
+    const unsigned tb = (unsigned)((n + kTB - 1) / kTB);
+    if (c.timeTraversal)
+      GVH_CHECK(hipEventRecord(c.travBegin, st));
+    if (model == 1)
+      hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
+                         (const NodeRec<Real, PosT> *)c.nodes.p,
+                         (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
+                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+    else
+      hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTB), 0, st,
+                         (const NodeRec<Real, PosT> *)c.nodes.p,
+                         (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
+                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+    GVH_CHECK_LAUNCH();
+    if (c.timeTraversal)
+      GVH_CHECK(hipEventRecord(c.travEnd, st));
+  } else {
+    const unsigned tb = (unsigned)((n + kTB - 1) / kTB);
+    if (model == 1)
+      hipLaunchKernelGGL((k_allpairs<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
+                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
+                         kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+    else
+      hipLaunchKernelGGL((k_allpairs<Real, PosT, 0>), dim3(tb), dim3(kTB), 0, st,
+                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
+                         kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+    GVH_CHECK_LAUNCH();
+  }
+
+  const unsigned rb = grid_for(n, kTB) < (unsigned)c.reductionBlocks
+                          ? grid_for(n, kTB)
+                          : (unsigned)c.reductionBlocks;
+  if (model == 1)
+    hipLaunchKernelGGL((k_forces<Real, PosT, 1>), dim3(rb), dim3(kTB), 0, st,
+                       posC, c.csrOff.p, c.csrAdj.p, c.degree.p,
+                       (const Real *)c.massT.p, (const R2 *)c.rep.p, n,
+                       (Real)c.prm.edgeLength, c.prm.gravityK, (R2 *)c.sf.p,
+                       (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
+                       (Real *)c.repMag.p, c.partials.p, c.sc.p);
+  else
+    hipLaunchKernelGGL((k_forces<Real, PosT, 0>), dim3(rb), dim3(kTB), 0, st,
+                       posC, c.csrOff.p, c.csrAdj.p, c.degree.p,
+                       (const Real *)c.massT.p, (const R2 *)c.rep.p, n,
+                       (Real)c.prm.edgeLength, c.prm.gravityK, (R2 *)c.sf.p,
+                       (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
+                       (Real *)c.repMag.p, c.partials.p, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  if (mode == kStepForcesOnly)
+    return 0;
+  if (enqueue_fa2(c, (int)rb))
+    return -1;
+  hipLaunchKernelGGL(k_apply<Real>, dim3(rb), dim3(kTB), 0, st,
+                     (const R2 *)c.disp.p, (const Real *)c.sw.p,
+                     (const Real *)c.attMag.p, (const Real *)c.repMag.p, n,
+                     (double2 *)c.pos64.p, c.partials.p, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  return enqueue_stats(c, (int)rb);
+}
+
+template <class Real>
+int typed_upload(ForceCommon &c) {
+  if (c.n == 0)
+    return 0;
+  hipLaunchKernelGGL(k_typed_mr<Real>, dim3(grid_for(c.n, kTB)), dim3(kTB), 0,
+                     c.stream, c.mass64.p, c.radius64.p, c.n, (Real *)c.massT.p,
+                     (Real *)c.radT.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+} // namespace gvh

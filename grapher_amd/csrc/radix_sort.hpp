@@ -1,0 +1,30 @@
+// radix_sort.hpp -- stable LSD radix sort of (uint64 key, uint32 value)
+// pairs for gfx950, 8-bit digits.
+//
+// Per pass: (1) per-tile 256-bin digit histograms, (2) one-block exclusive
+// scan over the digit-major [256 x tiles] table, (3) a stable scatter: each
+// tile ranks its keys with wave ballots (stable: tile order = key order),
+// stages them digit-sorted in LDS and writes each digit's run contiguously.
+#pragma once
+
+#include "common.hpp"
+
+namespace gvh {
+
+struct RadixSorter {
+  static constexpr int kThreads = 256;
+  static constexpr int kItems = 16;                  // keys per thread
+  static constexpr int kTile = kThreads * kItems;    // 4096 keys per tile
+  DevBuf<uint64_t> keysAlt;
+  DevBuf<uint32_t> valsAlt;
+  DevBuf<uint32_t> table; // 256 * tiles
+  int64_t capacity = 0;
+
+  int reserve(int64_t n);
+  // Sorts keys[0..n) / vals[0..n) by key bits [beginBit, endBit). On return the
+  // sorted data is in keys/vals (the alternate buffers are scratch).
+  int sort(uint64_t *keys, uint32_t *vals, int64_t n, int beginBit, int endBit,
+           hipStream_t stream);
+};
+
+} // namespace gvh

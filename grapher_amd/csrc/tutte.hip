@@ -1,0 +1,403 @@
+// tutte.hip -- Jacobi sweep of the Tutte barycentric embedder
+// (src/embedders/gvizTutteEmbedder.c:249-335) on gfx950.
+//
+// One thread per vertex reads its CSR row (insertion order), sums the
+// neighbours' previous positions, blends towards the barycentre with
+// relaxation alpha and writes the next positions into the other half of a
+// ping-pong pair; pinned boundary vertices and isolated vertices are copied.
+// Each sweep's max displacement is reduced per block and finished by a
+// one-block kernel that records it and raises the stop flag. Compiled with
+// -ffp-contract=off: in FP64 every sum and product rounds as the reference's.
+#include "../../include/gviz_hip.h"
+#include "common.hpp"
+
+#include <algorithm>
+#include <vector>
+
+using namespace gvh;
+
+namespace {
+
+constexpr int kB = 256;
+
+struct TutteScalars {
+  int64_t sweeps;    // sweeps applied since upload
+  int64_t recordBase;
+  double epsilon;
+  int32_t stopped;
+  int32_t pad;
+};
+
+template <class T> struct V2;
+template <> struct V2<float> { using type = float2; };
+template <> struct V2<double> { using type = double2; };
+
+template <class Real>
+__global__ __launch_bounds__(kB) void k_sweep(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const uint8_t *__restrict__ isBoundary,
+    const typename V2<Real>::type *__restrict__ in,
+    typename V2<Real>::type *__restrict__ out, int64_t n, Real alpha,
+    double *__restrict__ partial, const TutteScalars *__restrict__ sc) {
+  __shared__ double sm[16];
+  if (sc->stopped)
+    return;
+  double vmax = 0.0;
+  for (int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * kB) {
+    typename V2<Real>::type old = in[u];
+    const int32_t b = off[u], e = off[u + 1];
+    if (isBoundary[u] || e == b) {
+      out[u] = old;
+      continue;
+    }
+    Real bx = 0, by = 0;
+    for (int32_t k = b; k < e; k++) {
+      typename V2<Real>::type p = in[adj[k]];
+      bx += p.x;
+      by += p.y;
+    }
+    const Real cnt = (Real)(e - b);
+    bx /= cnt;
+    by /= cnt;
+    Real nx = old.x + alpha * (bx - old.x);
+    Real ny = old.y + alpha * (by - old.y);
+    Real d0 = nx - old.x, d1 = ny - old.y;
+    Real delta = (Real)0;
+    delta += d0 * d0;
+    delta += d1 * d1;
+    double d = (double)sqrt(delta);
+    vmax = d > vmax ? d : vmax;
+    out[u] = {nx, ny};
+  }
+  vmax = wave_max(vmax);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0)
+    sm[wave] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kB / kWave; w++)
+      m = sm[w] > m ? sm[w] : m;
+    partial[blockIdx.x] = m;
+  }
+}
+
+// Finishes a sweep: max over blocks, the "tutte.maxDelta" record, the stop
+// flag of gvizTutteEmbedderRun (converged once maxDelta < epsilon).
+__global__ void k_finish(const double *__restrict__ partial, int blocks,
+                         TutteScalars *__restrict__ sc, double *__restrict__ rec,
+                         int64_t recCap) {
+  __shared__ double sm[16];
+  if (sc->stopped)
+    return;
+  double m = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+    m = partial[b] > m ? partial[b] : m;
+  m = wave_max(m);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0)
+    sm[wave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / kWave); w++)
+      r = sm[w] > r ? sm[w] : r;
+    int64_t slot = sc->sweeps - sc->recordBase;
+    if (slot >= 0 && slot < recCap)
+      rec[slot] = r;
+    sc->sweeps++;
+    if (sc->epsilon >= 0.0 && r < sc->epsilon)
+      sc->stopped = 1;
+  }
+}
+
+__global__ void k_begin(TutteScalars *sc, double epsilon) {
+  sc->recordBase = sc->sweeps;
+  sc->epsilon = epsilon;
+  sc->stopped = 0;
+}
+
+} // namespace
+
+struct gvizhip_tutte {
+  int64_t n = 0, m2 = 0;
+  int32_t precision = GVIZHIP_PRECISION_FP64;
+  hipStream_t stream = nullptr;
+  DevBuf<int32_t> off, adj;
+  DevBuf<uint8_t> boundary;
+  DevBuf<unsigned char> pos[2];
+  DevBuf<double> partial;
+  DevBuf<double> records;
+  int64_t recordCap = 0;
+  DevBuf<TutteScalars> sc;
+  int blocks = 1;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  double graphAlpha = -1.0;
+  TutteScalars *pinned = nullptr;
+};
+
+namespace {
+
+size_t elem(const gvizhip_tutte *t) {
+  return t->precision == GVIZHIP_PRECISION_FP64 ? sizeof(double2) : sizeof(float2);
+}
+
+int enqueue_sweep(gvizhip_tutte *t, int from, double alpha) {
+  const unsigned g = (unsigned)t->blocks;
+  if (t->precision == GVIZHIP_PRECISION_FP64)
+    hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+                       t->adj.p, t->boundary.p, (const double2 *)t->pos[from].p,
+                       (double2 *)t->pos[1 - from].p, t->n, alpha, t->partial.p,
+                       t->sc.p);
+  else
+    hipLaunchKernelGGL(k_sweep<float>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+                       t->adj.p, t->boundary.p, (const float2 *)t->pos[from].p,
+                       (float2 *)t->pos[1 - from].p, t->n, (float)alpha,
+                       t->partial.p, t->sc.p);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, t->stream, t->partial.p,
+                     (int)g, t->sc.p, t->records.p, t->recordCap);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+void drop_graph(gvizhip_tutte *t) {
+  if (t->exec)
+    (void)hipGraphExecDestroy(t->exec);
+  if (t->graph)
+    (void)hipGraphDestroy(t->graph);
+  t->exec = nullptr;
+  t->graph = nullptr;
+  t->graphAlpha = -1.0;
+}
+
+// Two sweeps (buffer 0 -> 1 -> 0) captured once per alpha.
+int build_graph(gvizhip_tutte *t, double alpha) {
+  if (t->exec && t->graphAlpha == alpha)
+    return 0;
+  drop_graph(t);
+  GVH_CHECK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_sweep(t, 0, alpha);
+  if (rc == 0)
+    rc = enqueue_sweep(t, 1, alpha);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(t->stream, &g);
+  if (rc || e != hipSuccess) {
+    if (g)
+      (void)hipGraphDestroy(g);
+    if (!rc)
+      set_error("stream capture of the Tutte sweep failed: %s", hipGetErrorString(e));
+    return -1;
+  }
+  t->graph = g;
+  GVH_CHECK(hipGraphInstantiate(&t->exec, g, nullptr, nullptr, 0));
+  t->graphAlpha = alpha;
+  return 0;
+}
+
+int read_sc(gvizhip_tutte *t, TutteScalars *out) {
+  GVH_CHECK(hipMemcpyAsync(t->pinned, t->sc.p, sizeof(TutteScalars),
+                           hipMemcpyDeviceToHost, t->stream));
+  GVH_CHECK(hipStreamSynchronize(t->stream));
+  *out = *t->pinned;
+  return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int gvizhip_tutte_create(gvizhip_tutte **out, int64_t n, const int64_t *offsets,
+                         const int64_t *adj, int32_t precision) {
+  if (!out || n < 0 || (n > 0 && !offsets)) {
+    set_error("gvizhip_tutte_create: invalid arguments");
+    return -1;
+  }
+  *out = nullptr;
+  int dc = 0;
+  if (hipGetDeviceCount(&dc) != hipSuccess || dc <= 0) {
+    set_error("gvizhip_tutte_create: no GPU is visible to HIP");
+    return -1;
+  }
+  const int64_t m2 = n > 0 ? offsets[n] : 0;
+  if (n >= INT32_MAX || m2 >= INT32_MAX) {
+    set_error("gvizhip_tutte_create: graph exceeds 2^31 vertices or entries");
+    return -1;
+  }
+  std::vector<int32_t> o32(n + 1), a32(m2 > 0 ? m2 : 1);
+  for (int64_t i = 0; i <= n; i++)
+    o32[i] = n > 0 ? (int32_t)offsets[i] : 0;
+  for (int64_t e = 0; e < m2; e++) {
+    if (adj[e] < 0 || adj[e] >= n) {
+      set_error("gvizhip_tutte_create: adjacency entry %lld out of range",
+                (long long)e);
+      return -1;
+    }
+    a32[e] = (int32_t)adj[e];
+  }
+  gvizhip_tutte *t = new gvizhip_tutte();
+  t->n = n;
+  t->m2 = m2;
+  t->precision = precision == GVIZHIP_PRECISION_FP64 ? GVIZHIP_PRECISION_FP64
+                                                     : GVIZHIP_PRECISION_FP32;
+  const int64_t n1 = n > 0 ? n : 1;
+  t->blocks = (int)std::min<int64_t>(grid_for(n1, kB), 2048);
+  if (hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&t->pinned, sizeof(TutteScalars), hipHostMallocDefault) !=
+          hipSuccess ||
+      t->off.alloc(n1 + 1) || t->adj.alloc(m2 > 0 ? m2 : 1) ||
+      t->boundary.alloc(n1) || t->pos[0].alloc(n1 * sizeof(double2)) ||
+      t->pos[1].alloc(n1 * sizeof(double2)) || t->partial.alloc(t->blocks) ||
+      t->records.alloc(4096) || t->sc.alloc(1)) {
+    if (!gvizhip_last_error()[0])
+      set_error("gvizhip_tutte_create: allocation failed");
+    gvizhip_tutte_destroy(t);
+    return -1;
+  }
+  t->recordCap = 4096;
+  TutteScalars s{};
+  s.epsilon = -1.0;
+  if (hipMemcpy(t->off.p, o32.data(), sizeof(int32_t) * (n + 1),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      (m2 > 0 && hipMemcpy(t->adj.p, a32.data(), sizeof(int32_t) * m2,
+                           hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(t->sc.p, &s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(t->boundary.p, 0, n1) != hipSuccess) {
+    set_error("gvizhip_tutte_create: upload failed");
+    gvizhip_tutte_destroy(t);
+    return -1;
+  }
+  *out = t;
+  return 0;
+}
+
+int gvizhip_tutte_upload(gvizhip_tutte *t, const double *positions,
+                         const uint8_t *isBoundary) {
+  if (!t) {
+    set_error("gvizhip_tutte_upload: null context");
+    return -1;
+  }
+  const int64_t n = t->n;
+  if (n == 0)
+    return 0;
+  if (positions) {
+    if (t->precision == GVIZHIP_PRECISION_FP64) {
+      GVH_CHECK(hipMemcpy(t->pos[0].p, positions, sizeof(double) * 2 * n,
+                          hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> f(2 * n);
+      for (int64_t i = 0; i < 2 * n; i++)
+        f[i] = (float)positions[i];
+      GVH_CHECK(hipMemcpy(t->pos[0].p, f.data(), sizeof(float) * 2 * n,
+                          hipMemcpyHostToDevice));
+    }
+  }
+  if (isBoundary)
+    GVH_CHECK(hipMemcpy(t->boundary.p, isBoundary, n, hipMemcpyHostToDevice));
+  TutteScalars s{};
+  s.epsilon = -1.0;
+  GVH_CHECK(hipMemcpy(t->sc.p, &s, sizeof(s), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int gvizhip_tutte_download(gvizhip_tutte *t, double *positions) {
+  if (!t || (!positions && t->n > 0)) {
+    set_error("gvizhip_tutte_download: invalid arguments");
+    return -1;
+  }
+  const int64_t n = t->n;
+  if (n == 0)
+    return 0;
+  TutteScalars s;
+  if (read_sc(t, &s))
+    return -1;
+  const int cur = (int)(s.sweeps & 1);
+  if (t->precision == GVIZHIP_PRECISION_FP64) {
+    GVH_CHECK(hipMemcpy(positions, t->pos[cur].p, sizeof(double) * 2 * n,
+                        hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(2 * n);
+    GVH_CHECK(hipMemcpy(f.data(), t->pos[cur].p, sizeof(float) * 2 * n,
+                        hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < 2 * n; i++)
+      positions[i] = (double)f[i];
+  }
+  return 0;
+}
+
+int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
+                             double epsilon, double *maxDeltas) {
+  if (!t || iters < 0 || !(alpha > 0.0) || alpha > 1.0) {
+    set_error("gvizhip_tutte_sweeps: invalid arguments");
+    return -1;
+  }
+  if (iters == 0 || t->n == 0)
+    return 0;
+  if (maxDeltas && iters > t->recordCap) {
+    if (t->records.alloc(iters))
+      return -1;
+    t->recordCap = iters;
+    drop_graph(t);
+  }
+  TutteScalars before;
+  if (read_sc(t, &before))
+    return -1;
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, t->stream, t->sc.p, epsilon);
+  GVH_CHECK_LAUNCH();
+  int64_t done = 0;
+  int cur = (int)(before.sweeps & 1);
+  // An odd starting parity takes one plain sweep so the two-sweep graph
+  // always starts from buffer 0.
+  if (cur == 1) {
+    if (enqueue_sweep(t, 1, alpha))
+      return -1;
+    done++;
+  }
+  if (iters - done >= 2 && build_graph(t, alpha))
+    return -1;
+  int64_t sinceCheck = 0;
+  while (iters - done >= 2) {
+    GVH_CHECK(hipGraphLaunch(t->exec, t->stream));
+    done += 2;
+    sinceCheck += 2;
+    if (epsilon >= 0.0 && sinceCheck >= 32 && done < iters) {
+      TutteScalars s;
+      if (read_sc(t, &s))
+        return -1;
+      sinceCheck = 0;
+      if (s.stopped)
+        break;
+    }
+  }
+  if (done < iters) {
+    TutteScalars s;
+    if (read_sc(t, &s))
+      return -1;
+    if (!s.stopped && enqueue_sweep(t, (int)(s.sweeps & 1), alpha))
+      return -1;
+  }
+  TutteScalars after;
+  if (read_sc(t, &after))
+    return -1;
+  const int64_t ran = after.sweeps - before.sweeps;
+  if (maxDeltas && ran > 0)
+    GVH_CHECK(hipMemcpy(maxDeltas, t->records.p, sizeof(double) * ran,
+                        hipMemcpyDeviceToHost));
+  return ran;
+}
+
+void gvizhip_tutte_destroy(gvizhip_tutte *t) {
+  if (!t)
+    return;
+  drop_graph(t);
+  if (t->stream)
+    (void)hipStreamSynchronize(t->stream);
+  if (t->pinned)
+    (void)hipHostFree(t->pinned);
+  if (t->stream)
+    (void)hipStreamDestroy(t->stream);
+  delete t;
+}
+
+} // extern "C"

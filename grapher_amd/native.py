@@ -1,0 +1,336 @@
+"""ctypes binding of libgvizhip.so (include/gviz_hip.h).
+
+This is the Python side of the drop-in boundary: tests and bench.py drive the
+HIP engine through exactly the C entry points a C caller would use. Nothing
+here computes anything itself, and there is no fallback: if the library is
+not built or no GPU is visible, the constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(HERE, "libgvizhip.so")
+
+PRECISION_FP32 = 0
+PRECISION_FP64 = 1
+PRECISION_MIXED = 2
+PRECISIONS = {"fp32": PRECISION_FP32, "fp64": PRECISION_FP64, "mixed": PRECISION_MIXED}
+
+MODEL_FR = 0
+MODEL_LINLOG = 1
+
+
+class GvizHipError(RuntimeError):
+    pass
+
+
+class ForceConfig(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("barnesHut", C.c_int32),
+        ("preventOverlap", C.c_int32), ("precision", C.c_int32),
+        ("nodesPerCell", C.c_int64), ("edgeLength", C.c_double),
+        ("theta", C.c_double), ("gravityK", C.c_double),
+        ("overlapConstant", C.c_double), ("radiusBase", C.c_double),
+        ("radiusPerDegree", C.c_double), ("jitterTolerance", C.c_double),
+    ]
+
+
+class ForceStats(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "maxDisp", "speed", "meanAtt", "meanRep", "meanGrav",
+        "speedEfficiency", "totalSwinging", "totalTraction")]
+
+
+STATS_DTYPE = np.dtype([(k, "f8") for k, _ in ForceStats._fields_])
+
+
+class GripVertex(C.Structure):
+    _fields_ = [
+        ("dispX", C.c_double), ("dispY", C.c_double),
+        ("oldDispX", C.c_double), ("oldDispY", C.c_double),
+        ("heat", C.c_double), ("oldCos", C.c_double),
+        ("lastForceMag", C.c_double), ("knnCount", C.c_int64),
+        ("dispCalculated", C.c_int64),
+    ]
+
+
+GRIP_VERTEX_DTYPE = np.dtype([
+    ("dispX", "f8"), ("dispY", "f8"), ("oldDispX", "f8"), ("oldDispY", "f8"),
+    ("heat", "f8"), ("oldCos", "f8"), ("lastForceMag", "f8"),
+    ("knnCount", "i8"), ("dispCalculated", "i8")])
+
+
+class GripRoundStats(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("maxDisp", "meanDisp", "meanForce",
+                                          "meanHeat")]
+
+
+_lib = None
+_vp = C.c_void_p
+_i64 = C.c_int64
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def lib():
+    """Loads libgvizhip.so, raising if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(HIP_LIB):
+        raise GvizHipError(f"{HIP_LIB} is not built (python -m grapher_amd.build)")
+    L = C.CDLL(HIP_LIB)
+    L.gvizhip_last_error.restype = C.c_char_p
+    L.gvizhip_device_count.restype = C.c_int
+    sig = {
+        "gvizhip_force_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, C.POINTER(ForceConfig)]),
+        "gvizhip_force_configure": (C.c_int, [_vp, C.POINTER(ForceConfig)]),
+        "gvizhip_force_begin": (C.c_int, [_vp, _vp]),
+        "gvizhip_force_upload_positions": (C.c_int, [_vp, _vp]),
+        "gvizhip_force_download_positions": (C.c_int, [_vp, _vp]),
+        "gvizhip_force_step": (C.c_int, [_vp, C.POINTER(ForceStats)]),
+        "gvizhip_force_run": (_i64, [_vp, _i64, C.c_double, _vp]),
+        "gvizhip_force_evaluate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_force_get_speed": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_i64)]),
+        "gvizhip_force_set_speed": (C.c_int, [_vp, C.c_double, C.c_double]),
+        "gvizhip_force_quadtree_leaves": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_force_quadtree_cells": (_i64, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_force_profile": (C.c_int, [_vp, _i64, C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(C.c_double)]),
+        "gvizhip_force_destroy": (None, [_vp]),
+        "gvizhip_tutte_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, C.c_int32]),
+        "gvizhip_tutte_upload": (C.c_int, [_vp, _vp, _vp]),
+        "gvizhip_tutte_download": (C.c_int, [_vp, _vp]),
+        "gvizhip_tutte_sweeps": (_i64, [_vp, _i64, C.c_double, C.c_double, _vp]),
+        "gvizhip_tutte_destroy": (None, [_vp]),
+        "gvizhip_grip_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
+        "gvizhip_grip_mis_filtration": (_i64, [_vp, _i64, _vp, _vp]),
+        "gvizhip_grip_knn": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
+        "gvizhip_grip_knn_batch": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
+        "gvizhip_grip_upload": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_grip_download": (C.c_int, [_vp, _vp, _vp]),
+        "gvizhip_grip_update_knn": (C.c_int, [_vp, _vp, _i64, _vp, _i64]),
+        "gvizhip_grip_refine_round": (C.c_int, [_vp, _vp, _i64, _i64, C.POINTER(GripRoundStats)]),
+        "gvizhip_grip_destroy": (None, [_vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+# The entry points include/gviz_hip.h declares; tests check every one is
+# exported by the built library.
+DECLARED = [
+    "gvizhip_last_error", "gvizhip_device_count",
+    "gvizhip_force_create", "gvizhip_force_configure", "gvizhip_force_begin",
+    "gvizhip_force_upload_positions", "gvizhip_force_download_positions",
+    "gvizhip_force_step", "gvizhip_force_run", "gvizhip_force_evaluate",
+    "gvizhip_force_get_speed", "gvizhip_force_set_speed",
+    "gvizhip_force_quadtree_leaves", "gvizhip_force_quadtree_cells",
+    "gvizhip_force_profile", "gvizhip_force_destroy",
+    "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
+    "gvizhip_tutte_sweeps", "gvizhip_tutte_destroy",
+    "gvizhip_grip_create", "gvizhip_grip_mis_filtration", "gvizhip_grip_knn",
+    "gvizhip_grip_knn_batch", "gvizhip_grip_upload", "gvizhip_grip_download",
+    "gvizhip_grip_update_knn", "gvizhip_grip_refine_round",
+    "gvizhip_grip_destroy",
+]
+
+
+def _err(what):
+    msg = lib().gvizhip_last_error()
+    raise GvizHipError(f"{what}: {msg.decode() if msg else 'unknown error'}")
+
+
+def device_count() -> int:
+    return int(lib().gvizhip_device_count())
+
+
+def _csr(offsets, adj):
+    o = np.ascontiguousarray(offsets, dtype=np.int64)
+    a = np.ascontiguousarray(adj, dtype=np.int64)
+    if a.size == 0:
+        a = np.zeros(1, dtype=np.int64)
+    return o, a
+
+
+class ForceEngine:
+    """One device embedder: gvizhip_force_* on a CSR graph."""
+
+    def __init__(self, offsets, adj, *, model=MODEL_FR, barnes_hut=True,
+                 prevent_overlap=False, precision="fp32", nodes_per_cell=1,
+                 edge_length=10.0, theta=1.0, gravity=0.0,
+                 overlap_constant=100.0, radius_base=0.0,
+                 radius_per_degree=0.0, jitter=1.0):
+        self._o, self._a = _csr(offsets, adj)
+        self.n = self._o.shape[0] - 1
+        self.cfg = ForceConfig(model, int(barnes_hut), int(prevent_overlap),
+                               PRECISIONS.get(precision, precision), nodes_per_cell,
+                               edge_length, theta, gravity, overlap_constant,
+                               radius_base, radius_per_degree, jitter)
+        h = _vp()
+        if lib().gvizhip_force_create(C.byref(h), self.n, _p(self._o), _p(self._a),
+                                      C.byref(self.cfg)) != 0:
+            _err("gvizhip_force_create")
+        self.h = h
+
+    def configure(self, **kw):
+        names = {"model": "model", "barnes_hut": "barnesHut",
+                 "prevent_overlap": "preventOverlap", "precision": "precision",
+                 "nodes_per_cell": "nodesPerCell", "edge_length": "edgeLength",
+                 "theta": "theta", "gravity": "gravityK",
+                 "overlap_constant": "overlapConstant", "radius_base": "radiusBase",
+                 "radius_per_degree": "radiusPerDegree", "jitter": "jitterTolerance"}
+        for k, v in kw.items():
+            if k == "precision":
+                v = PRECISIONS.get(v, v)
+            setattr(self.cfg, names[k], int(v) if isinstance(v, bool) else v)
+        if lib().gvizhip_force_configure(self.h, C.byref(self.cfg)) != 0:
+            _err("gvizhip_force_configure")
+
+    def begin(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 2)
+        if lib().gvizhip_force_begin(self.h, _p(pos)) != 0:
+            _err("gvizhip_force_begin")
+
+    def set_positions(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 2)
+        if lib().gvizhip_force_upload_positions(self.h, _p(pos)) != 0:
+            _err("gvizhip_force_upload_positions")
+
+    def positions(self):
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_force_download_positions(self.h, _p(out)) != 0:
+            _err("gvizhip_force_download_positions")
+        return out
+
+    def step(self):
+        st = ForceStats()
+        if lib().gvizhip_force_step(self.h, C.byref(st)) != 0:
+            _err("gvizhip_force_step")
+        return st
+
+    def run(self, iters, epsilon=-1.0, stats=True):
+        out = np.zeros(max(iters, 1), dtype=STATS_DTYPE) if stats else None
+        r = lib().gvizhip_force_run(self.h, iters, epsilon, _p(out))
+        if r < 0:
+            _err("gvizhip_force_run")
+        return (out[:r] if stats else None), int(r)
+
+    def evaluate(self, interactions=False):
+        n = self.n
+        disp = np.zeros((n, 2))
+        sf = np.zeros((n, 2))
+        am = np.zeros(n)
+        rm = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32) if interactions else None
+        if lib().gvizhip_force_evaluate(self.h, _p(disp), _p(sf), _p(am), _p(rm),
+                                        _p(it)) != 0:
+            _err("gvizhip_force_evaluate")
+        out = {"disp": disp, "structForce": sf, "attMag": am, "repMag": rm}
+        if interactions:
+            out["interactions"] = it
+        return out
+
+    def speed(self):
+        g, e, it = C.c_double(), C.c_double(), _i64()
+        if lib().gvizhip_force_get_speed(self.h, C.byref(g), C.byref(e), C.byref(it)) != 0:
+            _err("gvizhip_force_get_speed")
+        return g.value, e.value, it.value
+
+    def set_speed(self, gs, eff):
+        if lib().gvizhip_force_set_speed(self.h, gs, eff) != 0:
+            _err("gvizhip_force_set_speed")
+
+    def quadtree_leaves(self):
+        n = self.n
+        d = np.zeros(n, dtype=np.int32)
+        kh = np.zeros(n, dtype=np.uint64)
+        kl = np.zeros(n, dtype=np.uint64)
+        root = np.zeros(4)
+        if lib().gvizhip_force_quadtree_leaves(self.h, _p(d), _p(kh), _p(kl), _p(root)) != 0:
+            _err("gvizhip_force_quadtree_leaves")
+        return d, kh, kl, root
+
+    def quadtree_cells(self):
+        cap = max(self.n, 1)
+        d = np.zeros(cap, dtype=np.int32)
+        kh = np.zeros(cap, dtype=np.uint64)
+        kl = np.zeros(cap, dtype=np.uint64)
+        m = np.zeros(cap)
+        cx = np.zeros(cap)
+        cy = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int64)
+        k = lib().gvizhip_force_quadtree_cells(self.h, cap, _p(d), _p(kh), _p(kl),
+                                               _p(m), _p(cx), _p(cy), _p(cnt))
+        if k < 0:
+            _err("gvizhip_force_quadtree_cells")
+        k = min(k, cap)
+        return {"depth": d[:k], "keyHi": kh[:k], "keyLo": kl[:k], "mass": m[:k],
+                "comX": cx[:k], "comY": cy[:k], "count": cnt[:k]}
+
+    def profile(self, steps):
+        """(traversal ms, repulsion evaluations, step ms) summed over steps."""
+        t, i, s = C.c_double(), _i64(), C.c_double()
+        if lib().gvizhip_force_profile(self.h, steps, C.byref(t), C.byref(i),
+                                       C.byref(s)) != 0:
+            _err("gvizhip_force_profile")
+        return t.value, i.value, s.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_force_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class TutteEngine:
+    """Device Jacobi sweeps: gvizhip_tutte_*."""
+
+    def __init__(self, offsets, adj, precision="fp64"):
+        self._o, self._a = _csr(offsets, adj)
+        self.n = self._o.shape[0] - 1
+        h = _vp()
+        if lib().gvizhip_tutte_create(C.byref(h), self.n, _p(self._o), _p(self._a),
+                                      PRECISIONS.get(precision, precision)) != 0:
+            _err("gvizhip_tutte_create")
+        self.h = h
+
+    def upload(self, positions, is_boundary):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 2)
+        b = np.ascontiguousarray(is_boundary, dtype=np.uint8)
+        if lib().gvizhip_tutte_upload(self.h, _p(pos), _p(b)) != 0:
+            _err("gvizhip_tutte_upload")
+
+    def positions(self):
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_tutte_download(self.h, _p(out)) != 0:
+            _err("gvizhip_tutte_download")
+        return out
+
+    def sweeps(self, iters, alpha=1.0, epsilon=-1.0, record=True):
+        out = np.zeros(max(iters, 1)) if record else None
+        r = lib().gvizhip_tutte_sweeps(self.h, iters, alpha, epsilon, _p(out))
+        if r < 0:
+            _err("gvizhip_tutte_sweeps")
+        return (out[:r] if record else None), int(r)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_tutte_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

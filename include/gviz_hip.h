@@ -1,0 +1,202 @@
+/*
+ * gviz_hip.h -- C ABI of the gfx950 layout engine (libgvizhip.so).
+ *
+ * Plain pointers and sizes only. A caller (the C11 host layer in
+ * grapher_amd/host, or the reference's own embedders patched as described in
+ * INTEGRATION.md) flattens its graph once into CSR over compact vertex ids,
+ * keeps the adjacency in insertion order, and drives the device through these
+ * entry points. Every function returns 0 (or a count) on success and a
+ * negative value on failure; gvizhip_last_error() describes the failure.
+ * Nothing here falls back to the CPU: without a usable GPU the create calls
+ * fail.
+ */
+#ifndef GVIZ_HIP_H
+#define GVIZ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Human-readable reason for the last failure on this thread. */
+const char *gvizhip_last_error(void);
+/* Number of visible GPUs (0 when HIP finds none). */
+int gvizhip_device_count(void);
+
+/* Arithmetic of the force math. FP64 reproduces the reference's FP64 formulas
+ * in its evaluation order; MIXED differences FP64 positions and does the
+ * force math and accumulation in FP32; FP32 is FP32 throughout. The quadtree
+ * keys, mass sums, centre-of-mass sums, FA2 speed update and position update
+ * are FP64 in every mode. */
+enum {
+  GVIZHIP_PRECISION_FP32 = 0,
+  GVIZHIP_PRECISION_FP64 = 1,
+  GVIZHIP_PRECISION_MIXED = 2
+};
+
+/* ------------------------------------------------------------------ force */
+/* Mirrors the configuration fields of gvizForceEmbedderState
+ * (include/embedders/gvizForceEmbedder.h:42-138) that the step reads. */
+typedef struct gvizhip_force_config {
+  int32_t model;          /* gvizForceModelKind: 0 FR, 1 LinLog */
+  int32_t barnesHut;      /* barnesHutEnabled */
+  int32_t preventOverlap; /* preventOverlap */
+  int32_t precision;      /* GVIZHIP_PRECISION_* */
+  int64_t nodesPerCell;
+  double edgeLength;
+  double theta;
+  double gravityK;
+  double overlapConstant;
+  double radiusBase;
+  double radiusPerDegree;
+  double jitterTolerance;
+} gvizhip_force_config;
+
+/* What gvizForceEmbedderStep reports: its return value and the five stat
+ * series it appends (src/embedders/gvizForceEmbedder.c:467-494), plus the
+ * FA2 state after the step. */
+typedef struct gvizhip_force_stats {
+  double maxDisp;
+  double speed;
+  double meanAtt;
+  double meanRep;
+  double meanGrav;
+  double speedEfficiency;
+  double totalSwinging;
+  double totalTraction;
+} gvizhip_force_stats;
+
+typedef struct gvizhip_force gvizhip_force;
+
+/* Replaces the allocation and setup half of gvizForceEmbedderInit
+ * (src/embedders/gvizForceEmbedder.c:219-325): n compact vertices, CSR
+ * offsets[n+1] / adj[offsets[n]] over compact ids in insertion order. Masses
+ * and degrees follow from the adjacency and cfg->model. */
+int gvizhip_force_create(gvizhip_force **out, int64_t n, const int64_t *offsets,
+                         const int64_t *adj, const gvizhip_force_config *cfg);
+/* Replaces the Configure* setters (:357-410): takes effect at the next step. */
+int gvizhip_force_configure(gvizhip_force *f, const gvizhip_force_config *cfg);
+/* Replaces gvizForceEmbedderBegin (:412-443) after positions are chosen:
+ * uploads positions[2n] (compact order, interleaved x,y), speed 1, efficiency
+ * 1, forces zero, iteration 0. */
+int gvizhip_force_begin(gvizhip_force *f, const double *positions);
+/* Positions in and out, compact order (the reference's gatherPositions and
+ * AddVPosition ends of a step, :19-26 and :476). */
+int gvizhip_force_upload_positions(gvizhip_force *f, const double *positions);
+int gvizhip_force_download_positions(gvizhip_force *f, double *positions);
+/* Replaces gvizForceEmbedderStep (:445-495) on device-resident positions. */
+int gvizhip_force_step(gvizhip_force *f, gvizhip_force_stats *stats);
+/* Replaces gvizForceEmbedderRun (:497-510): steps until maxDisp < epsilon or
+ * maxIters steps; returns the number of steps. stats (maxIters entries, may be
+ * NULL) receives every step's record. */
+int64_t gvizhip_force_run(gvizhip_force *f, int64_t maxIters, double epsilon,
+                          gvizhip_force_stats *stats);
+/* One force evaluation at the current positions without applying it (the
+ * outputs of computeForceRange, :76-126); the FA2 state is left untouched.
+ * Any output may be NULL. interactions: repulsion evaluations per vertex. */
+int gvizhip_force_evaluate(gvizhip_force *f, double *disp, double *structForce,
+                           double *attMag, double *repMag,
+                           int32_t *interactions);
+/* FA2 state: global speed, speed efficiency, iteration. */
+int gvizhip_force_get_speed(gvizhip_force *f, double *globalSpeed,
+                            double *speedEfficiency, int64_t *iteration);
+int gvizhip_force_set_speed(gvizhip_force *f, double globalSpeed,
+                            double speedEfficiency);
+/* The quadtree of the current positions (gvizQuadtreeBuild,
+ * src/ds/gvizQuadtree.c:159-192): per vertex the depth and quadrant-path key
+ * of its leaf; root[4] = centre x, centre y, half size, depth limit. */
+int gvizhip_force_quadtree_leaves(gvizhip_force *f, int32_t *leafDepth,
+                                  uint64_t *leafKeyHi, uint64_t *leafKeyLo,
+                                  double *root);
+/* Branching cells of that quadtree (cells with two or more non-empty
+ * children): depth, path key, mass, centre of mass, point count. Returns the
+ * number of cells (entries beyond cap are not written). */
+int64_t gvizhip_force_quadtree_cells(gvizhip_force *f, int64_t cap,
+                                     int32_t *depth, uint64_t *keyHi,
+                                     uint64_t *keyLo, double *mass,
+                                     double *comX, double *comY,
+                                     int64_t *count);
+/* Measurement hook (no reference counterpart): runs `steps` steps outside
+ * the captured graph with HIP events on the engine's stream around the
+ * Barnes-Hut traversal kernel and around the whole step. Outputs (may be
+ * NULL): summed traversal kernel time, summed repulsion evaluations over all
+ * vertices and steps, summed step time. */
+int gvizhip_force_profile(gvizhip_force *f, int64_t steps, double *traversalMs,
+                          int64_t *interactions, double *stepMs);
+void gvizhip_force_destroy(gvizhip_force *f);
+
+/* ------------------------------------------------------------------ Tutte */
+typedef struct gvizhip_tutte gvizhip_tutte;
+
+/* Graph over parent ids 0..n-1 (CSR in insertion order, subgraph edges only)
+ * for the Jacobi sweep of gvizTutteEmbedderStep
+ * (src/embedders/gvizTutteEmbedder.c:305-335). */
+int gvizhip_tutte_create(gvizhip_tutte **out, int64_t n, const int64_t *offsets,
+                         const int64_t *adj, int32_t precision);
+/* Live positions (2n) and the pinned-boundary flags (n bytes). */
+int gvizhip_tutte_upload(gvizhip_tutte *t, const double *positions,
+                         const uint8_t *isBoundary);
+int gvizhip_tutte_download(gvizhip_tutte *t, double *positions);
+/* `iters` Jacobi sweeps with relaxation alpha (already clamped to (0, 1]).
+ * Stops after the first sweep whose max displacement is < epsilon when
+ * epsilon >= 0. maxDeltas (iters entries, may be NULL) receives each sweep's
+ * max displacement. Returns the number of sweeps run. */
+int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
+                             double epsilon, double *maxDeltas);
+void gvizhip_tutte_destroy(gvizhip_tutte *t);
+
+/* ------------------------------------------------------------------- GRIP */
+typedef struct gvizhip_grip gvizhip_grip;
+
+/* Graph over parent ids 0..n-1 restricted to the subgraph; active[n] marks
+ * subgraph vertices (NULL: all). kcap = kNN capacity per vertex. */
+int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
+                        const int64_t *adj, const uint8_t *active,
+                        int64_t kcap);
+/* createMISFiltration (src/embedders/gvizGRIPEmbedder.c:741-777):
+ * filtration[activeCount], borders[<=128]; returns the layer count. */
+int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
+                                    int64_t *filtration, int64_t *borders);
+/* gvizSearchKNearestScratch (src/algorithms/search/gvizKNearest.c:244-306)
+ * from each of sources[count], over the `visible` filter (n bytes):
+ * out[count*k] of (vertex, dist) pairs, counts[count]. */
+int gvizhip_grip_knn(gvizhip_grip *g, const uint8_t *visible,
+                     const int64_t *sources, int64_t count, int64_t k,
+                     int64_t *outVertex, int64_t *outDist, int64_t *counts);
+/* gvizSearchKNearestFromVisibleBatch (:168-210). */
+int gvizhip_grip_knn_batch(gvizhip_grip *g, const uint8_t *visible,
+                           const int64_t *targets, int64_t count, int64_t k,
+                           int64_t *outVertex, int64_t *outDist,
+                           int64_t *counts);
+/* Per-vertex GRIP state (gvizGRIPDecorators without the kNN array). */
+typedef struct gvizhip_grip_vertex {
+  double dispX, dispY, oldDispX, oldDispY;
+  double heat, oldCos, lastForceMag;
+  int64_t knnCount;
+  int64_t dispCalculated;
+} gvizhip_grip_vertex;
+typedef struct gvizhip_grip_round_stats {
+  double maxDisp, meanDisp, meanForce, meanHeat;
+} gvizhip_grip_round_stats;
+/* Uploads positions (2n), vertex state (n) and kNN lists (n*kcap pairs). */
+int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
+                        const gvizhip_grip_vertex *dec,
+                        const int64_t *knnVertex, const int64_t *knnDist);
+int gvizhip_grip_download(gvizhip_grip *g, double *positions,
+                          gvizhip_grip_vertex *dec);
+/* Recomputes the device kNN lists of filtration[0..active) against the
+ * visible set (the refinement kNN of updateKNNs, :954-1006). */
+int gvizhip_grip_update_knn(gvizhip_grip *g, const int64_t *filtration,
+                            int64_t active, const uint8_t *visible, int64_t k);
+/* gvizGRIPEmbedderRefineRound (:1289-1333) over filtration[0..active). */
+int gvizhip_grip_refine_round(gvizhip_grip *g, const int64_t *filtration,
+                              int64_t active, int64_t layer,
+                              gvizhip_grip_round_stats *stats);
+void gvizhip_grip_destroy(gvizhip_grip *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

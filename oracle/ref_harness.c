@@ -18,10 +18,12 @@
 #include "embedders/gvizGRIPEmbedder.h"
 #include "embedders/gvizGRIPInternal.h"
 #include "embedders/gvizTutteEmbedder.h"
+#include "core/gvizThreadPool.h"
 
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 static FILE *fin, *fout;
 
@@ -192,6 +194,50 @@ static int mode_force(void) {
   return 0;
 }
 
+/* CPU baseline: wall time of each gvizForceEmbedderStep with the reference's
+ * own thread pool of `workers` threads (plus the calling thread). */
+static int mode_time_force(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t model = rd_i(), bh = rd_i(), overlap = rd_i(), npc = rd_i();
+  int64_t iters = rd_i(), workers = rd_i();
+  double edgeLength = rd_d(), theta = rd_d(), gravityK = rd_d();
+  double overlapC = rd_d(), rBase = rd_d(), rPerDeg = rd_d(), jitter = rd_d();
+  double *pos = malloc(sizeof(double) * 2 * n);
+  for (int64_t i = 0; i < 2 * n; i++)
+    pos[i] = rd_d();
+  gvizForceEmbedderState s;
+  if (gvizForceEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2,
+                            (gvizForceModelKind)model) != 0)
+    return 1;
+  gvizThreadPoolDestroy(s.pool);
+  s.pool = workers > 0 ? gvizThreadPoolCreate((size_t)workers) : NULL;
+  gvizForceEmbedderConfigure(&s, edgeLength, 0.0);
+  gvizForceEmbedderConfigureSpeed(&s, jitter);
+  gvizForceEmbedderConfigureBarnesHut(&s, theta, (size_t)npc);
+  gvizForceEmbedderSetBarnesHutEnabled(&s, (int)bh);
+  gvizForceEmbedderConfigureGravity(&s, gravityK);
+  gvizForceEmbedderConfigureRadius(&s, rBase, rPerDeg);
+  gvizForceEmbedderConfigureOverlapPrevention(&s, overlapC);
+  gvizForceEmbedderSetPreventOverlapEnabled(&s, (int)overlap);
+  gvizForceEmbedderBegin(&s, 12345u);
+  for (int64_t i = 0; i < n; i++)
+    gvizEmbeddedGraphSetVPosition((gvizEmbeddedGraph *)&s, (size_t)i,
+                                  pos + 2 * i);
+  for (int64_t it = 0; it < iters; it++) {
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    gvizForceEmbedderStep(&s);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    wr_d((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
+    fflush(fout);
+  }
+  gvizForceEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
 /* ------------------------------------------------------------------- tutte */
 static int mode_tutte(void) {
   gvizGraph g;
@@ -353,6 +399,8 @@ int main(int argc, char **argv) {
     rc = mode_quadtree();
   else if (!strcmp(argv[1], "force"))
     rc = mode_force();
+  else if (!strcmp(argv[1], "time_force"))
+    rc = mode_time_force();
   else if (!strcmp(argv[1], "tutte"))
     rc = mode_tutte();
   else if (!strcmp(argv[1], "knn"))

@@ -1,0 +1,194 @@
+"""HIP force engine against the reference's fixtures and the FP64 oracle.
+
+Tolerances, per precision:
+  fp64  forces normwise <= 1e-11 relative, per vertex <= 1e-9 of |att|+|rep|;
+        repulsion interaction sets equal (same per-vertex counts)
+  mixed/fp32 forces normwise <= 1e-5 relative (the 1e-5 FP32 bar)
+Quadtree membership (leaf depth and quadrant path) is bit-exact everywhere.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import golden
+from grapher_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+FORCE_FIXTURES = ["force_fr_bh_grid12", "force_linlog_bh_ba300",
+                  "force_fr_exact_er200", "force_fr_bh_npc2_grid15"]
+
+
+def _engine(native, f, precision):
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    return native.ForceEngine(
+        off, adj, model=int(f["param_model"]),
+        barnes_hut=bool(f["param_barnes_hut"]),
+        prevent_overlap=bool(f["param_prevent_overlap"]), precision=precision,
+        nodes_per_cell=int(f["param_npc"]), edge_length=float(f["param_edge_length"]),
+        theta=float(f["param_theta"]), gravity=float(f["param_gravity"]),
+        overlap_constant=float(f["param_overlap_constant"]),
+        radius_base=float(f["param_radius_base"]),
+        radius_per_degree=float(f["param_radius_per_degree"]),
+        jitter=float(f["param_jitter"]))
+
+
+def _check_forces(got, ref_disp, ref_att, ref_rep, tol_norm, tol_vertex):
+    d = got["disp"] - ref_disp
+    assert np.linalg.norm(d) <= tol_norm * np.linalg.norm(ref_disp)
+    scale = ref_att + ref_rep + 1e-300
+    per = np.linalg.norm(d, axis=1) / scale
+    assert per.max() <= tol_vertex, (per.max(), int(per.argmax()))
+
+
+@pytest.mark.parametrize("name", FORCE_FIXTURES)
+def test_fp64_forces_match_reference_each_iteration(gpu, name):
+    f = golden(name)
+    eng = _engine(gpu, f, "fp64")
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    g = O.Graph(off, adj)
+    p = O.params(model=int(f["param_model"]), barnes_hut=bool(f["param_barnes_hut"]),
+                 prevent_overlap=bool(f["param_prevent_overlap"]),
+                 npc=int(f["param_npc"]), theta=float(f["param_theta"]),
+                 gravity=float(f["param_gravity"]),
+                 radius_base=float(f["param_radius_base"]),
+                 radius_per_degree=float(f["param_radius_per_degree"]))
+    pos = f["pos0"]
+    for it in f["dump_iters"]:
+        eng.set_positions(pos)
+        got = eng.evaluate(interactions=True)
+        _check_forces(got, f[f"it{it}_disp"], f[f"it{it}_attMag"],
+                      f[f"it{it}_repMag"], 1e-11, 1e-9)
+        np.testing.assert_allclose(got["attMag"], f[f"it{it}_attMag"], rtol=1e-12,
+                                   atol=1e-12)
+        # identical Barnes-Hut interaction sets
+        orc = O.force_compute(g, pos, p)
+        assert np.array_equal(got["interactions"], orc["interactions"])
+        pos = f[f"it{it}_pos_after"]
+
+
+@pytest.mark.parametrize("name", FORCE_FIXTURES)
+def test_fp64_trajectory_matches_reference(gpu, name):
+    f = golden(name)
+    eng = _engine(gpu, f, "fp64")
+    eng.begin(f["pos0"])
+    stats, r = eng.run(int(f["iters"]))
+    assert r == int(f["iters"])
+    ref = f["stats"]
+    for k, col in (("maxDisp", 0), ("speed", 1), ("meanAtt", 2), ("meanRep", 3),
+                   ("meanGrav", 4), ("speedEfficiency", 5)):
+        np.testing.assert_allclose(stats[k], ref[:, col], rtol=1e-9, atol=1e-12,
+                                   err_msg=k)
+    last = int(f["dump_iters"][-1])
+    np.testing.assert_allclose(eng.positions(), f[f"it{last}_pos_after"],
+                               rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+@pytest.mark.parametrize("name", FORCE_FIXTURES)
+def test_fp32_forces_within_1e5(gpu, name, precision):
+    f = golden(name)
+    eng = _engine(gpu, f, precision)
+    pos = f["pos0"]
+    for it in f["dump_iters"][:3]:
+        eng.set_positions(pos)
+        got = eng.evaluate()
+        d = got["disp"] - f[f"it{it}_disp"]
+        assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(f[f"it{it}_disp"])
+        pos = f[f"it{it}_pos_after"]
+
+
+@pytest.mark.parametrize("name", ["quadtree_uniform300", "quadtree_dup_npc1",
+                                  "quadtree_clustered_npc3"])
+def test_quadtree_membership_bit_exact(gpu, name):
+    f = golden(name)
+    pts = f["points"]
+    n = pts.shape[0]
+    eng = gpu.ForceEngine(np.zeros(n + 1, np.int64), np.zeros(0, np.int64),
+                          precision="fp64", nodes_per_cell=int(f["npc"]))
+    eng.set_positions(pts)
+    depth, kh, kl, root = eng.quadtree_leaves()
+    assert np.array_equal(depth, f["leaf_depth"])
+    assert np.array_equal(kh, f["leaf_keyHi"])
+    assert np.array_equal(kl, f["leaf_keyLo"])
+    if np.all(f["masses"] == 1.0):
+        # branching cells: mass exact, centre of mass to FP64 round-off
+        cells = eng.quadtree_cells()
+        ref = f["nodes"]
+        branching = {}
+        for r in ref:
+            if r["leaf"]:
+                continue
+            branching[(int(r["depth"]), int(r["keyHi"]), int(r["keyLo"]))] = r
+        got = {(int(d), int(h), int(l)): (m, x, y) for d, h, l, m, x, y in zip(
+            cells["depth"], cells["keyHi"], cells["keyLo"], cells["mass"],
+            cells["comX"], cells["comY"])}
+        for key, (m, x, y) in got.items():
+            r = branching[key]
+            assert m == r["mass"]
+            assert abs(x - r["comX"]) <= 1e-12 * (1 + abs(r["comX"]))
+            assert abs(y - r["comY"]) <= 1e-12 * (1 + abs(r["comY"]))
+
+
+def test_grid100_500_iterations_stress_within_1pct(gpu):
+    """The CPU-runnable headline case: 100x100 grid, FR, BH theta 0.5, 500
+    iterations from the same seeded positions; final stress and edge energy
+    within 1% of the reference's final layout, in every precision."""
+    f = golden("force_fr_bh_grid100")
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    g = O.Graph(off, adj)
+    pool = O.Pool(0)
+    ref_pos = f["it499_pos_after"]
+    rng = np.random.default_rng(0)
+    src = rng.choice(int(f["n"]), 256, replace=False)
+    s_ref = O.stress(g, ref_pos, 10.0, src, pool)
+    e_ref = O.edge_energy(g, ref_pos)
+    for prec in ("fp64", "mixed", "fp32"):
+        eng = _engine(gpu, f, prec)
+        eng.begin(f["pos0"])
+        _, r = eng.run(500)
+        assert r == 500
+        pos = eng.positions()
+        assert np.all(np.isfinite(pos))
+        s = O.stress(g, pos, 10.0, src, pool)
+        e = O.edge_energy(g, pos)
+        assert abs(s - s_ref) <= 0.01 * s_ref, (prec, s, s_ref)
+        assert abs(e - e_ref) <= 0.01 * e_ref + 1e-3, (prec, e, e_ref)
+
+
+def test_run_stops_at_epsilon_and_step_matches_run(gpu):
+    f = golden("force_fr_bh_grid12")
+    a = _engine(gpu, f, "fp64")
+    b = _engine(gpu, f, "fp64")
+    a.begin(f["pos0"])
+    b.begin(f["pos0"])
+    stats, r = a.run(8)
+    steps = [b.step().maxDisp for _ in range(8)]
+    np.testing.assert_array_equal(stats["maxDisp"], np.array(steps))
+    np.testing.assert_array_equal(a.positions(), b.positions())
+    # epsilon above the third step's maxDisp stops right there
+    c = _engine(gpu, f, "fp64")
+    c.begin(f["pos0"])
+    eps = stats["maxDisp"][2] * 1.0000001
+    if stats["maxDisp"][0] >= eps and stats["maxDisp"][1] >= eps:
+        _, r2 = c.run(8, eps)
+        assert r2 == 3
+
+
+def test_large_ba_linlog_mixed_matches_fp64_engine(gpu):
+    """1M-vertex Barabasi-Albert LinLog + gravity + overlap, theta 0.7: the
+    mixed-precision forces against the FP64 engine on the same positions
+    (normwise 1e-5), and every quadtree leaf holds its own vertex."""
+    n, e = G.barabasi_albert(1_000_000, 3, 0)
+    off, adj = G.csr_from_edges(n, e)
+    pos = G.uniform_positions(n, 0)
+    kw = dict(model=1, theta=0.7, gravity=0.05, prevent_overlap=True,
+              radius_base=0.1, radius_per_degree=1.0)
+    hi = gpu.ForceEngine(off, adj, precision="fp64", **kw)
+    lo = gpu.ForceEngine(off, adj, precision="mixed", **kw)
+    hi.set_positions(pos)
+    lo.set_positions(pos)
+    a = hi.evaluate()["disp"]
+    b = lo.evaluate()["disp"]
+    assert np.all(np.isfinite(b))
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a)

@@ -1,0 +1,150 @@
+"""The FP64 oracle against the fixtures the reference library wrote
+(tests/golden/make_golden.py). Every comparison is bit-exact: the oracle
+restates the reference's arithmetic in the reference's order."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import golden
+from grapher_amd import graphs as G
+
+
+def _graph(f):
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    return O.Graph(off, adj)
+
+
+@pytest.mark.parametrize("name", ["quadtree_uniform300", "quadtree_clustered_npc3",
+                                  "quadtree_dup_npc1"])
+def test_quadtree_membership_and_cells(name):
+    f = golden(name)
+    o = O.quadtree(f["points"], f["masses"], int(f["npc"]))
+    assert np.array_equal(o["leaf_depth"], f["leaf_depth"])
+    assert np.array_equal(o["leaf_keyHi"], f["leaf_keyHi"])
+    assert np.array_equal(o["leaf_keyLo"], f["leaf_keyLo"])
+    # node set in DFS order: depth, key, mass, COM and half size all equal
+    nodes = o["nodes"]
+    ref = f["nodes"]
+    assert len(nodes) == len(ref)
+    # the oracle stores nodes in allocation order; index them by (depth, key)
+    mine = {(int(d), int(h), int(l)): (m, cx, cy, hs) for d, h, l, m, cx, cy, hs in
+            zip(nodes["depth"], nodes["keyHi"], nodes["keyLo"], nodes["mass"],
+                nodes["comX"], nodes["comY"], nodes["halfSize"])}
+    for r in ref:
+        key = (int(r["depth"]), int(r["keyHi"]), int(r["keyLo"]))
+        m, cx, cy, hs = mine[key]
+        assert m == r["mass"] and cx == r["comX"] and cy == r["comY"]
+        assert hs == r["halfSize"]
+
+
+def _params_from(f):
+    return O.params(model=int(f["param_model"]), barnes_hut=bool(f["param_barnes_hut"]),
+                    prevent_overlap=bool(f["param_prevent_overlap"]),
+                    npc=int(f["param_npc"]), edge_length=float(f["param_edge_length"]),
+                    theta=float(f["param_theta"]), gravity=float(f["param_gravity"]),
+                    overlap_constant=float(f["param_overlap_constant"]),
+                    radius_base=float(f["param_radius_base"]),
+                    radius_per_degree=float(f["param_radius_per_degree"]),
+                    jitter=float(f["param_jitter"]))
+
+
+@pytest.mark.parametrize("name", ["force_fr_bh_grid12", "force_linlog_bh_ba300",
+                                  "force_fr_exact_er200", "force_fr_bh_npc2_grid15"])
+def test_force_step_bit_exact(name):
+    f = golden(name)
+    g = _graph(f)
+    p = _params_from(f)
+    run = O.ForceRun(g, p, f["pos0"])
+    pool = O.Pool(4)
+    for it in range(int(f["iters"])):
+        before = run.pos.copy()
+        forces = O.force_compute(g, before, p, pool)
+        st = run.step()
+        ref = f["stats"][it]
+        assert (st.maxDisp, st.speed, st.meanAtt, st.meanRep, st.meanGrav,
+                st.speedEfficiency) == tuple(ref)
+        if it in f["dump_iters"]:
+            for k in ("disp", "structForce", "attMag", "repMag"):
+                assert np.array_equal(forces[k], f[f"it{it}_{k}"]), k
+            assert np.array_equal(run.pos, f[f"it{it}_pos_after"])
+
+
+def test_force_grid100_first_iteration_and_final_positions():
+    f = golden("force_fr_bh_grid100")
+    g = _graph(f)
+    p = _params_from(f)
+    forces = O.force_compute(g, f["pos0"], p, O.Pool(0))
+    for k in ("disp", "structForce", "attMag", "repMag"):
+        assert np.array_equal(forces[k], f[f"it0_{k}"]), k
+
+
+@pytest.mark.parametrize("name,gs", [("tutte_grid6_jacobi", 0), ("tutte_grid6_gs", 1),
+                                     ("tutte_delaunay300_jacobi", 0),
+                                     ("tutte_delaunay300_dt60", 0), ("tutte_k4", 0)])
+def test_tutte_bit_exact(name, gs):
+    f = golden(name)
+    g = _graph(f)
+    n = int(f["n"])
+    isb = np.zeros(n, np.uint8)
+    isb[f["boundary"]] = 1
+    # FixConvexPolygon positions and the interior seed are what p0 holds
+    poly = O.tutte_polygon(len(f["boundary"]), float(f["radius"]))
+    assert np.array_equal(poly, f["p0"][f["boundary"]])
+    pos = f["p0"].copy()
+    scratch = np.zeros_like(pos)
+    deltas = [O.tutte_step(g, isb, pos, scratch, 5.0, float(f["dt"]), bool(gs))
+              for _ in range(int(f["iters"]))]
+    assert np.array_equal(np.array(deltas), f["deltas"])
+    assert np.array_equal(pos, f["p1"])
+
+
+def test_knn_both_paths_exact():
+    f = golden("knn_rgg300")
+    g = _graph(f)
+    vis = f["visible"].astype(np.uint8)
+    k = int(f["k"])
+    for s in f["sources"]:
+        o = O.knn_bfs(g, int(s), k, vis)
+        c = f["per_count"][s]
+        assert len(o) == c
+        assert np.array_equal(o, f["per_list"][s][:c])
+    out, counts = O.knn_batch(g, vis, k, f["batch_targets"])
+    assert np.array_equal(counts, f["batch_count"])
+    for i, c in enumerate(counts):
+        assert np.array_equal(out[i][:c], f["batch_list"][i][:c])
+
+
+@pytest.mark.parametrize("name", ["grip_rgg300", "grip_grid10", "grip_ba200"])
+def test_grip_filtration_and_rounds_exact(name):
+    f = golden(name)
+    g = _graph(f)
+    filt, borders = O.mis_filtration(g)
+    assert np.array_equal(borders, f["borders"])
+    assert np.array_equal(filt, f["filtration"])
+    kcap = int(f["kcap"])
+    n = int(f["n"])
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        d0 = f[f"r{i}_dec0"]
+        dec = np.zeros(n, dtype=O.GRIP_VERTEX_DTYPE)
+        dec["dispX"], dec["dispY"] = d0["disp"][:, 0], d0["disp"][:, 1]
+        dec["oldDispX"], dec["oldDispY"] = d0["oldDisp"][:, 0], d0["oldDisp"][:, 1]
+        for k in ("heat", "oldCos", "lastForceMag", "knnCount"):
+            dec[k] = d0[k]
+        dec["dispCalculated"] = d0["dispCalculated"]
+        pos = f[f"r{i}_pos0"].copy()
+        active = int(f["borders"][layer])
+        # the kNN lists the reference used are the oracle's BFS over the
+        # visible set (vertices of layers >= current)
+        vis = np.zeros(n, np.uint8)
+        vis[filt[:active]] = 1
+        knn = f[f"r{i}_knn"]
+        for v in filt[:active]:
+            c = int(d0["knnCount"][v])
+            o = O.knn_bfs(g, int(v), kcap, vis)
+            assert np.array_equal(o[:c], knn[v][:c])
+        st = O.grip_refine_round(g, filt, active, layer, knn, kcap, dec, pos)
+        assert np.array_equal(pos, f[f"r{i}_pos1"])
+        d1 = f[f"r{i}_dec1"]
+        assert np.array_equal(dec["heat"], d1["heat"])
+        assert (st.maxDisp, st.meanDisp, st.meanForce) == tuple(f[f"r{i}_stats"])
