@@ -1,0 +1,1408 @@
+// grip.hip -- GRIP multilevel layout on gfx950 (the reference's
+// src/embedders/gvizGRIPEmbedder.c and src/algorithms/search/gvizKNearest.c):
+//
+//   * wavefront BFS: one wavefront per query walks its FIFO queue, expanding
+//     a vertex's adjacency 64 entries at a time; new vertices are claimed in a
+//     per-wavefront visited bitmap and appended in adjacency (discovery)
+//     order with ballots, so kNN lists come out in the reference's order;
+//   * MIS filtration as the lexicographically-first independent set of each
+//     layer's conflict graph (two candidates conflict when their graph
+//     distance is <= 2^(i-1)), which is exactly what the reference's
+//     ascending greedy pass produces: conflicts come from bounded BFS balls,
+//     the independent set from parallel decision rounds;
+//   * the refinement round as a map over the active vertices followed by
+//     FP64 block reductions for translation / rotation removal and the stats.
+// Compiled with -ffp-contract=off: the FP64 arithmetic rounds like the
+// reference's.
+#include "../../include/gviz_hip.h"
+#include "common.hpp"
+#include "radix_sort.hpp"
+
+#include <algorithm>
+#include <deque>
+#include <vector>
+
+using namespace gvh;
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kWpb = kB / kWave;
+constexpr double kEdge = 10.0;      // GVIZ_EDGE_LENGTH (gvizGRIPEmbedder.c:31)
+constexpr double kGripR = 0.15;     // gvizGRIPr (:45)
+constexpr double kGripS = 3.0;      // gvizGRIPs (:46)
+constexpr double kFrScale = 0.05;   // FR_SCALE_FACTOR (gvizForceDirected.c:4)
+constexpr double kEps = 1e-7;       // gvizNumericEpsilon
+
+// The BFS queue is written and read back by the same wavefront; both go
+// through device-scope atomics so the read-after-write order holds at L2.
+__device__ __forceinline__ int32_t ld_l2(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_l2(int32_t *p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct BfsSlots {
+  uint32_t *bits;  // S * words
+  int32_t *queue;  // S * cap
+  int32_t *qdist;  // S * cap
+  int64_t words, cap;
+};
+
+// One wavefront's BFS from `source` (gvizKNearest.c:244-306 and, with
+// maxDepth > 0, verticesWithinRadius of gvizGRIPEmbedder.c:599-670).
+// Discovered vertices passing `filter` (and, when lowerOnly, with an id below
+// the source) are written to out[0..k) in discovery order with their depth.
+// kNN mode (stopAtK) ends the search at the k-th hit; ball mode explores the
+// whole radius and reports the total hit count (may exceed k).
+__device__ int wave_bfs(const int32_t *__restrict__ off,
+                        const int32_t *__restrict__ adj,
+                        const uint8_t *__restrict__ filter, int32_t source, int k,
+                        int maxDepth, bool lowerOnly, bool stopAtK,
+                        uint32_t *bits, int32_t *queue, int32_t *qdist,
+                        int32_t *outV, int32_t *outD) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t lt = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  if (lane == 0) {
+    atomicOr(&bits[source >> 5], 1u << (source & 31));
+    st_l2(queue, source);
+    st_l2(qdist, 0);
+  }
+  int head = 0, tail = 1, found = 0;
+  bool done = false;
+  while (head < tail && !done) {
+    const int32_t u = __builtin_amdgcn_readfirstlane(ld_l2(queue + head));
+    const int32_t du = __builtin_amdgcn_readfirstlane(ld_l2(qdist + head));
+    head++;
+    if (maxDepth > 0 && du >= maxDepth)
+      continue;
+    const int32_t e0 = off[u], e1 = off[u + 1];
+    for (int32_t base = e0; base < e1 && !done; base += kWave) {
+      const int32_t e = base + lane;
+      const bool valid = e < e1;
+      const int32_t w = valid ? adj[e] : -1;
+      // a vertex listed twice in one adjacency: only its first entry counts
+      bool dup = false;
+      for (int o = 1; o < kWave; o++) {
+        int32_t other = __shfl_up(w, o, kWave);
+        if (lane >= o && other == w)
+          dup = true;
+      }
+      bool fresh = false;
+      if (valid && !dup) {
+        const uint32_t bit = 1u << (w & 31);
+        fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
+      }
+      const uint64_t fm = __ballot(fresh);
+      const bool pass = fresh && (!filter || filter[w]) && (!lowerOnly || w < source);
+      const uint64_t pm = __ballot(pass);
+      if (pass) {
+        const int slot = found + (int)__popcll(pm & lt);
+        if (slot < k) {
+          outV[slot] = w;
+          outD[slot] = du + 1;
+        }
+      }
+      found += (int)__popcll(pm);
+      if (fresh) {
+        const int at = tail + (int)__popcll(fm & lt);
+        st_l2(queue + at, w);
+        st_l2(qdist + at, du + 1);
+      }
+      tail += (int)__popcll(fm);
+      if (stopAtK && found >= k)
+        done = true;
+    }
+  }
+  // clear this query's bits: every visited vertex sits in the queue
+  for (int i = lane; i < tail; i += kWave) {
+    const int32_t w = ld_l2(queue + i);
+    atomicAnd(&bits[w >> 5], 0u);
+  }
+  return found;
+}
+
+// Persistent kernel: wavefront `slot` serves queries slot, slot + S, ...
+__global__ __launch_bounds__(kB) void k_bfs_queries(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const uint8_t *__restrict__ filter, const int32_t *__restrict__ sources,
+    const int32_t *__restrict__ outIndex, int64_t count, int k, int64_t stride,
+    int maxDepth, int lowerOnly, int stopAtK, BfsSlots sl, int64_t slots,
+    int32_t *__restrict__ outV, int32_t *__restrict__ outD,
+    int32_t *__restrict__ outCount, int32_t *__restrict__ overflow) {
+  const int64_t slot = (int64_t)blockIdx.x * kWpb + threadIdx.x / kWave;
+  if (slot >= slots)
+    return;
+  uint32_t *bits = sl.bits + slot * sl.words;
+  int32_t *q = sl.queue + slot * sl.cap;
+  int32_t *qd = sl.qdist + slot * sl.cap;
+  for (int64_t qi = slot; qi < count; qi += slots) {
+    const int32_t src = sources[qi];
+    const int64_t o = (outIndex ? (int64_t)outIndex[qi] : qi) * stride;
+    int found = wave_bfs(off, adj, filter, src, k, maxDepth, lowerOnly != 0,
+                         stopAtK != 0, bits, q, qd, outV + o, outD + o);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+      outCount[outIndex ? outIndex[qi] : qi] = found < k ? found : k;
+      if (!stopAtK && found > k)
+        atomicOr(overflow, 1);
+    }
+  }
+}
+
+// ------------------------------------------------------- MIS decisions
+// Lexicographically-first MIS over conflict lists of lower-id neighbours:
+// IN when every lower conflict is OUT, OUT as soon as one is IN.
+constexpr uint8_t kUndecided = 0, kIn = 1, kOut = 2;
+
+__global__ __launch_bounds__(kB) void k_lfmis_lists(
+    const int32_t *__restrict__ lists, const int32_t *__restrict__ counts,
+    int64_t m, int64_t stride, uint8_t *__restrict__ state,
+    int32_t *__restrict__ undecided) {
+  int64_t c = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (c >= m || state[c] != kUndecided)
+    return;
+  bool allOut = true;
+  const int32_t cnt = counts[c];
+  for (int32_t j = 0; j < cnt; j++) {
+    uint8_t s = ((volatile uint8_t *)state)[lists[c * stride + j]];
+    if (s == kIn) {
+      state[c] = kOut;
+      return;
+    }
+    if (s != kOut)
+      allOut = false;
+  }
+  if (allOut)
+    state[c] = kIn;
+  else
+    atomicAdd(undecided, 1);
+}
+
+// Layer 1: the conflict graph is the subgraph itself.
+__global__ __launch_bounds__(kB) void k_lfmis_graph(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const uint8_t *__restrict__ active, int64_t n, uint8_t *__restrict__ state,
+    int32_t *__restrict__ undecided) {
+  int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (v >= n || !active[v] || state[v] != kUndecided)
+    return;
+  bool allOut = true;
+  for (int32_t e = off[v]; e < off[v + 1]; e++) {
+    const int32_t u = adj[e];
+    if (u >= v)
+      continue;
+    uint8_t s = ((volatile uint8_t *)state)[u];
+    if (s == kIn) {
+      state[v] = kOut;
+      return;
+    }
+    if (s != kOut)
+      allOut = false;
+  }
+  if (allOut)
+    state[v] = kIn;
+  else
+    atomicAdd(undecided, 1);
+}
+
+// Layer-1 tail order (gvizGRIPEmbedder.c:462-478): a vertex left out of the
+// MIS is written when its smallest-id MIS neighbour p scans its adjacency, at
+// p's adjacency position: sort key (p, first index of v in adj(p)).
+__global__ __launch_bounds__(kB) void k_tail_keys(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const uint8_t *__restrict__ active, const uint8_t *__restrict__ state,
+    int64_t n, uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+    int32_t *__restrict__ count) {
+  int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (v >= n || !active[v] || state[v] != kOut)
+    return;
+  int32_t p = INT32_MAX;
+  for (int32_t e = off[v]; e < off[v + 1]; e++) {
+    const int32_t u = adj[e];
+    if (state[u] == kIn && u < p)
+      p = u;
+  }
+  int32_t idx = 0;
+  for (int32_t e = off[p]; e < off[p + 1]; e++)
+    if (adj[e] == (int32_t)v) {
+      idx = e - off[p];
+      break;
+    }
+  const int32_t at = atomicAdd(count, 1);
+  keys[at] = ((uint64_t)(uint32_t)p << 32) | (uint32_t)idx;
+  vals[at] = (uint32_t)v;
+}
+
+// ------------------------------------------------------ refinement round
+struct GripRound {
+  double sdx, sdy, spx, spy; // sums of normalised disp and positions
+  double meanX, meanY, comX, comY;
+  double L, I, omega;
+  int32_t rotate;
+  int32_t pad;
+  double maxDisp, sumDisp, sumHeat, sumForce;
+};
+
+// refinementPass1Range + calculateSpringForces + updateLocalTemp
+// (gvizGRIPEmbedder.c:1071-1099, 892-934, 870-889).
+__global__ __launch_bounds__(kB) void k_refine(
+    const int32_t *__restrict__ filt, int64_t active, int64_t layer,
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const double2 *__restrict__ pos, const int32_t *__restrict__ knnV,
+    const int32_t *__restrict__ knnD, const int32_t *__restrict__ knnCount,
+    int64_t kcap, double2 *__restrict__ disp, double2 *__restrict__ oldDisp,
+    double *__restrict__ heat, double *__restrict__ oldCos,
+    double *__restrict__ lastF, uint8_t *__restrict__ dispCalc,
+    double *__restrict__ partial) {
+  __shared__ double sm[4 * 16];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = filt[i];
+    const double2 d0 = disp[v];
+    oldDisp[v] = d0;
+    const double2 pv = pos[v];
+    double f0 = 0.0, f1 = 0.0;
+    const int32_t kc = knnCount[v];
+    const int64_t kb = (int64_t)v * kcap;
+    if (layer > 0) {
+      for (int32_t j = 0; j < kc; j++) {
+        const double2 pu = pos[knnV[kb + j]];
+        const double L = (double)knnD[kb + j] * kEdge;
+        if (L < 1e-9)
+          continue;
+        double dx = pu.x - pv.x, dy = pu.y - pv.y;
+        double s = sqrt(dx * dx + dy * dy) / L - 1.0;
+        f0 += s * dx;
+        f1 += s * dy;
+      }
+    } else {
+      const double eSq = kEdge * kEdge;
+      for (int32_t e = off[v]; e < off[v + 1]; e++) {
+        const double2 pu = pos[adj[e]];
+        double dx = pu.x - pv.x, dy = pu.y - pv.y;
+        double dsq = dx * dx + dy * dy;
+        if (dsq < 1e-18)
+          continue;
+        double s = dsq / eSq;
+        f0 += s * dx;
+        f1 += s * dy;
+      }
+      for (int32_t j = 0; j < kc; j++) {
+        const double2 pu = pos[knnV[kb + j]];
+        double dx = pv.x - pu.x, dy = pv.y - pu.y;
+        double dsq = dx * dx + dy * dy;
+        if (dsq < 1e-18)
+          continue;
+        double s = kFrScale * eSq / dsq;
+        f0 += s * dx;
+        f1 += s * dy;
+      }
+    }
+    double h = heat[v];
+    if (!dispCalc[v]) {
+      dispCalc[v] = 1;
+      h = kEdge / 6.0;
+    } else {
+      double nrm = sqrt(f0 * f0 + f1 * f1);
+      double oldNrm = sqrt(d0.x * d0.x + d0.y * d0.y);
+      if (!(fabs(nrm) < kEps || fabs(oldNrm) < kEps)) {
+        double cs = (f0 * d0.x + f1 * d0.y) / (nrm * oldNrm);
+        if (cs > 0 && oldCos[v] > 0)
+          h *= (1 + cs * kGripR * kGripS);
+        else
+          h *= (1 + cs * kGripR);
+        oldCos[v] = cs;
+      }
+    }
+    h = fmin(h, kEdge * 10);
+    h = fmax(h, kEdge * 1e-4);
+    heat[v] = h;
+    double nrm = sqrt(f0 * f0 + f1 * f1);
+    lastF[v] = nrm;
+    if (nrm > kEps) {
+      double s = h / nrm;
+      f0 *= s;
+      f1 *= s;
+    }
+    disp[v] = {f0, f1};
+    acc[0] += f0;
+    acc[1] += f1;
+    acc[2] += pv.x;
+    acc[3] += pv.y;
+  }
+  const bool isMax[4] = {false, false, false, false};
+  block_reduce<4>(acc, isMax, sm);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++)
+      partial[4 * blockIdx.x + k] = acc[k];
+}
+
+__global__ void k_round_means(const double *__restrict__ partial, int blocks,
+                              int64_t active, GripRound *__restrict__ r) {
+  __shared__ double sm[4 * 16];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+    for (int k = 0; k < 4; k++)
+      v[k] += partial[4 * b + k];
+  const bool isMax[4] = {false, false, false, false};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    // removeNetTranslation (gvizGRIPEmbedder.c:1106-1121): scale by 1/count
+    double inv = 1.0 / (double)active;
+    r->meanX = v[0] * inv;
+    r->meanY = v[1] * inv;
+    // removeNetRotation's centre (:1205-1214): sum / count
+    r->comX = v[2] / (double)active;
+    r->comY = v[3] / (double)active;
+  }
+}
+
+// Translation removal and the rotation sums of the 2-D branch of
+// removeNetRotation (gvizGRIPEmbedder.c:1216-1245).
+__global__ __launch_bounds__(kB) void k_rotation_sums(
+    const int32_t *__restrict__ filt, int64_t active,
+    const double2 *__restrict__ pos, double2 *__restrict__ disp,
+    const GripRound *__restrict__ r, double *__restrict__ partial) {
+  __shared__ double sm[2 * 16];
+  double acc[2] = {0.0, 0.0};
+  const double mx = r->meanX, my = r->meanY, cx = r->comX, cy = r->comY;
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = filt[i];
+    double2 d = disp[v];
+    d.x += -1.0 * mx;
+    d.y += -1.0 * my;
+    disp[v] = d;
+    const double2 p = pos[v];
+    double r0 = p.x - cx, r1 = p.y - cy, r2v = 0.0;
+    acc[0] += r0 * d.y - r1 * d.x;
+    double r2 = r0 * r0 + r1 * r1 + r2v * r2v;
+    acc[1] += r2 - r2v * r2v;
+  }
+  const bool isMax[2] = {false, false};
+  block_reduce<2>(acc, isMax, sm);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = acc[0];
+    partial[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+__global__ void k_round_omega(const double *__restrict__ partial, int blocks,
+                              int64_t active, GripRound *__restrict__ r) {
+  __shared__ double sm[2 * 16];
+  double v[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] += partial[2 * b];
+    v[1] += partial[2 * b + 1];
+  }
+  const bool isMax[2] = {false, false};
+  block_reduce<2>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    r->L = v[0];
+    r->I = v[1];
+    r->rotate = active >= 3 && !(fabs(v[1]) < kEps);
+    r->omega = r->rotate ? v[0] / v[1] : 0.0;
+  }
+}
+
+// Rotation removal, position update and round stats
+// (gvizGRIPEmbedder.c:1246-1260, 1299-1318).
+__global__ __launch_bounds__(kB) void k_round_apply(
+    const int32_t *__restrict__ filt, int64_t active, double2 *__restrict__ pos,
+    double2 *__restrict__ disp, const double *__restrict__ heat,
+    const double *__restrict__ lastF, const GripRound *__restrict__ r,
+    double *__restrict__ partial) {
+  __shared__ double sm[4 * 16];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool rot = r->rotate != 0;
+  const double w = r->omega, cx = r->comX, cy = r->comY;
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = filt[i];
+    double2 d = disp[v];
+    double2 p = pos[v];
+    if (rot) {
+      double r0 = p.x - cx, r1 = p.y - cy, r2v = 0.0;
+      d.x -= 0.0 * r2v - w * r1;
+      d.y -= w * r0 - 0.0 * r2v;
+      disp[v] = d;
+    }
+    double nrm = sqrt(d.x * d.x + d.y * d.y);
+    acc[0] = nrm > acc[0] ? nrm : acc[0];
+    acc[1] += nrm;
+    acc[2] += heat[v];
+    acc[3] += lastF[v];
+    p.x += d.x;
+    p.y += d.y;
+    pos[v] = p;
+  }
+  const bool isMax[4] = {true, false, false, false};
+  block_reduce<4>(acc, isMax, sm);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++)
+      partial[4 * blockIdx.x + k] = acc[k];
+}
+
+__global__ void k_round_stats(const double *__restrict__ partial, int blocks,
+                              int64_t active, GripRound *__restrict__ r) {
+  __shared__ double sm[4 * 16];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] = fmax(v[0], partial[4 * b]);
+    for (int k = 1; k < 4; k++)
+      v[k] += partial[4 * b + k];
+  }
+  const bool isMax[4] = {true, false, false, false};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    r->maxDisp = v[0];
+    r->sumDisp = v[1];
+    r->sumHeat = v[2];
+    r->sumForce = v[3];
+  }
+}
+
+// clearDecorators (gvizGRIPEmbedder.c:936-952)
+__global__ __launch_bounds__(kB) void k_clear(const int32_t *__restrict__ filt,
+                                              int64_t active, int64_t n,
+                                              double2 *disp, double2 *oldDisp,
+                                              double *heat, double *oldCos,
+                                              int32_t *knnCount, uint8_t *dispCalc) {
+  int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n)
+    dispCalc[i] = 0;
+  if (i < active) {
+    const int32_t v = filt[i];
+    heat[v] = 0.0;
+    oldCos[v] = 0.0;
+    knnCount[v] = 0;
+    disp[v] = {0.0, 0.0};
+    oldDisp[v] = {0.0, 0.0};
+  }
+}
+
+// gripBarycenter (gvizGRIPEmbedder.c:21-27) of each placed vertex's hits.
+__global__ __launch_bounds__(kB) void k_place(
+    const int32_t *__restrict__ filt, int64_t begin, int64_t end,
+    const int32_t *__restrict__ hitV, const int32_t *__restrict__ hitCount,
+    int64_t stride, double2 *__restrict__ pos, int32_t *__restrict__ empty) {
+  int64_t i = begin + (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= end)
+    return;
+  const int64_t q = i - begin;
+  const int32_t c = hitCount[q];
+  if (c <= 0) {
+    atomicOr(empty, 1);
+    return;
+  }
+  double o0 = 0.0, o1 = 0.0;
+  for (int32_t j = 0; j < c; j++) {
+    const double2 p = pos[hitV[q * stride + j]];
+    o0 += 1.0 * p.x;
+    o1 += 1.0 * p.y;
+  }
+  const double s = 1.0 / (double)c;
+  o0 *= s;
+  o1 *= s;
+  pos[filt[i]] = {o0, o1};
+}
+
+// Batched search, per target: the k best (depth, seed order) among the seeds
+// processed before every target was full (gvizKNearest.c:62-89, 168-210).
+__global__ __launch_bounds__(kB) void k_batch_kth(const int32_t *__restrict__ table,
+                                                  int64_t nseeds, int64_t ntargets,
+                                                  int k, int32_t *__restrict__ stopSeed,
+                                                  int32_t *__restrict__ notFull) {
+  int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntargets)
+    return;
+  int reached = 0;
+  for (int64_t s = 0; s < nseeds; s++) {
+    if (table[s * ntargets + t] >= 0 && ++reached == k) {
+      atomicMax(stopSeed, (int32_t)s);
+      return;
+    }
+  }
+  atomicOr(notFull, 1);
+}
+
+__global__ __launch_bounds__(kB) void k_batch_select(
+    const int32_t *__restrict__ table, const int32_t *__restrict__ seeds,
+    int64_t nseeds, int64_t ntargets, int k, const int32_t *__restrict__ stopSeed,
+    const int32_t *__restrict__ notFull, const int32_t *__restrict__ outIndex,
+    int64_t stride, int32_t *__restrict__ outV, int32_t *__restrict__ outD,
+    int32_t *__restrict__ outCount) {
+  int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (t >= ntargets)
+    return;
+  const int64_t last = *notFull ? nseeds - 1 : (int64_t)*stopSeed;
+  const int64_t o = (outIndex ? (int64_t)outIndex[t] : t) * stride;
+  int cnt = 0;
+  for (int64_t s = 0; s <= last; s++) {
+    const int32_t d = table[s * ntargets + t];
+    if (d < 0)
+      continue;
+    // knn_insert_sorted: append or replace the last, then bubble up while
+    // strictly smaller; equal depths keep the earlier seed first
+    int i;
+    if (cnt < k) {
+      i = cnt++;
+    } else {
+      if (d >= outD[o + k - 1])
+        continue;
+      i = k - 1;
+    }
+    outV[o + i] = seeds[s];
+    outD[o + i] = d;
+    while (i > 0 && outD[o + i] < outD[o + i - 1]) {
+      int32_t tv = outV[o + i], td = outD[o + i];
+      outV[o + i] = outV[o + i - 1];
+      outD[o + i] = outD[o + i - 1];
+      outV[o + i - 1] = tv;
+      outD[o + i - 1] = td;
+      i--;
+    }
+  }
+  outCount[outIndex ? outIndex[t] : t] = cnt;
+}
+
+// Full BFS per seed writing target depths (the batch path's seed sweep).
+__global__ __launch_bounds__(kB) void k_seed_depths(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const int32_t *__restrict__ seeds, int64_t nseeds,
+    const int32_t *__restrict__ targetMap, int64_t ntargets, BfsSlots sl,
+    int64_t slots, int32_t *__restrict__ table) {
+  const int64_t slot = (int64_t)blockIdx.x * kWpb + threadIdx.x / kWave;
+  if (slot >= slots)
+    return;
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t *bits = sl.bits + slot * sl.words;
+  int32_t *q = sl.queue + slot * sl.cap;
+  int32_t *qd = sl.qdist + slot * sl.cap;
+  for (int64_t si = slot; si < nseeds; si += slots) {
+    int32_t *row = table + si * ntargets;
+    for (int64_t t = lane; t < ntargets; t += kWave)
+      row[t] = -1;
+    int32_t ov = 0, od = 0;
+    const int32_t seed = seeds[si];
+    // a full BFS; hits are not collected (k = 0), the queue keeps every
+    // visited vertex with its depth
+    int head = 0, tail = 1;
+    if (lane == 0) {
+      atomicOr(&bits[seed >> 5], 1u << (seed & 31));
+      st_l2(q, seed);
+      st_l2(qd, 0);
+    }
+    const uint64_t lt = lane ? (~0ull >> (kWave - lane)) : 0ull;
+    while (head < tail) {
+      const int32_t u = __builtin_amdgcn_readfirstlane(ld_l2(q + head));
+      const int32_t du = __builtin_amdgcn_readfirstlane(ld_l2(qd + head));
+      head++;
+      const int32_t e0 = off[u], e1 = off[u + 1];
+      for (int32_t base = e0; base < e1; base += kWave) {
+        const int32_t e = base + lane;
+        const bool valid = e < e1;
+        const int32_t w = valid ? adj[e] : -1;
+        bool dup = false;
+        for (int o2 = 1; o2 < kWave; o2++) {
+          int32_t other = __shfl_up(w, o2, kWave);
+          if (lane >= o2 && other == w)
+            dup = true;
+        }
+        bool fresh = false;
+        if (valid && !dup) {
+          const uint32_t bit = 1u << (w & 31);
+          fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
+        }
+        const uint64_t fm = __ballot(fresh);
+        if (fresh) {
+          const int at = tail + (int)__popcll(fm & lt);
+          st_l2(q + at, w);
+          st_l2(qd + at, du + 1);
+          const int32_t ti = targetMap[w];
+          if (ti >= 0)
+            row[ti] = du + 1;
+        }
+        tail += (int)__popcll(fm);
+      }
+    }
+    for (int i = lane; i < tail; i += kWave)
+      atomicAnd(&bits[ld_l2(q + i) >> 5], 0u);
+    (void)ov;
+    (void)od;
+  }
+}
+
+} // namespace
+
+struct gvizhip_grip {
+  int64_t n = 0, m2 = 0, kcap = 0, activeCount = 0, edgeCount = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int64_t> hoff, hadj;
+  std::vector<uint8_t> hactive;
+  DevBuf<int32_t> off, adj, filt;
+  DevBuf<uint8_t> active, visible, state;
+  DevBuf<double2> pos, disp, oldDisp;
+  DevBuf<double> heat, oldCos, lastF, partial;
+  DevBuf<uint8_t> dispCalc;
+  DevBuf<int32_t> knnV, knnD, knnCount;
+  DevBuf<int32_t> qSrc, qOut, hitV, hitD, hitCount, flags;
+  DevBuf<uint32_t> bits;
+  DevBuf<int32_t> queue, qdist;
+  DevBuf<int32_t> table, targetMap;
+  DevBuf<GripRound> round;
+  DevBuf<uint64_t> keys;
+  DevBuf<uint32_t> vals;
+  RadixSorter sorter;
+  int64_t slots = 0;
+  int64_t filtCount = 0;
+};
+
+namespace {
+
+int ensure_slots(gvizhip_grip *g) {
+  if (g->slots)
+    return 0;
+  const int64_t words = (g->n + 31) / 32 + 1;
+  const int64_t cap = g->n + 1;
+  const double perSlot = (double)words * 4 + (double)cap * 8;
+  const double budget = 12.0 * (1ull << 30);
+  int64_t s = (int64_t)std::min(1024.0, std::max(4.0, budget / perSlot));
+  s = (s / kWpb) * kWpb;
+  if (s < kWpb)
+    s = kWpb;
+  if (g->bits.alloc(s * words) || g->queue.alloc(s * cap) || g->qdist.alloc(s * cap))
+    return -1;
+  GVH_CHECK(hipMemsetAsync(g->bits.p, 0, sizeof(uint32_t) * s * words, g->stream));
+  g->slots = s;
+  return 0;
+}
+
+BfsSlots slot_view(gvizhip_grip *g) {
+  BfsSlots v;
+  v.bits = g->bits.p;
+  v.queue = g->queue.p;
+  v.qdist = g->qdist.p;
+  v.words = (g->n + 31) / 32 + 1;
+  v.cap = g->n + 1;
+  return v;
+}
+
+int upload_visible(gvizhip_grip *g, const uint8_t *visible) {
+  if (!visible)
+    return 0;
+  GVH_CHECK(hipMemcpyAsync(g->visible.p, visible, g->n, hipMemcpyHostToDevice,
+                           g->stream));
+  return 0;
+}
+
+// BFS queries on the device: sources (device int32), results into outV/outD
+// rows of `stride` selected by outIndex (device, may be null).
+int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
+                int64_t count, int k, int64_t stride, const uint8_t *filterDev,
+                int maxDepth, bool lowerOnly, bool stopAtK, int32_t *outV,
+                int32_t *outD, int32_t *outCount) {
+  if (count == 0)
+    return 0;
+  if (ensure_slots(g))
+    return -1;
+  const int64_t waves = std::min<int64_t>(g->slots, count);
+  const unsigned blocks = (unsigned)((waves + kWpb - 1) / kWpb);
+  hipLaunchKernelGGL(k_bfs_queries, dim3(blocks), dim3(kB), 0, g->stream,
+                     g->off.p, g->adj.p, filterDev, srcDev, outIndex, count, k,
+                     stride, maxDepth, lowerOnly ? 1 : 0, stopAtK ? 1 : 0,
+                     slot_view(g), (int64_t)blocks * kWpb, outV, outD, outCount,
+                     g->flags.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+// The batched search of gvizSearchKNearestFromVisibleBatch for targets
+// (device ids, count), writing rows selected by outIndex.
+int run_batch(gvizhip_grip *g, const uint8_t *visibleHost, const int32_t *targetsDev,
+              const std::vector<int32_t> &targetsHost, const int32_t *outIndex,
+              int64_t count, int k, int64_t stride, int32_t *outV, int32_t *outD,
+              int32_t *outCount) {
+  std::vector<int32_t> seeds;
+  for (int64_t v = 0; v < g->n; v++)
+    if (visibleHost[v])
+      seeds.push_back((int32_t)v);
+  const int64_t ns = (int64_t)seeds.size();
+  if (ns == 0 || count == 0)
+    return 0;
+  if (ensure_slots(g))
+    return -1;
+  std::vector<int32_t> tmap(g->n, -1);
+  for (int64_t t = 0; t < count; t++)
+    tmap[targetsHost[t]] = (int32_t)t;
+  if (g->table.alloc(ns * count) || g->targetMap.alloc(g->n) ||
+      g->qSrc.alloc(std::max<int64_t>(ns, count)))
+    return -1;
+  GVH_CHECK(hipMemcpyAsync(g->targetMap.p, tmap.data(), sizeof(int32_t) * g->n,
+                           hipMemcpyHostToDevice, g->stream));
+  DevBuf<int32_t> seedDev;
+  if (seedDev.alloc(ns))
+    return -1;
+  GVH_CHECK(hipMemcpyAsync(seedDev.p, seeds.data(), sizeof(int32_t) * ns,
+                           hipMemcpyHostToDevice, g->stream));
+  const int64_t waves = std::min<int64_t>(g->slots, ns);
+  const unsigned blocks = (unsigned)((waves + kWpb - 1) / kWpb);
+  hipLaunchKernelGGL(k_seed_depths, dim3(blocks), dim3(kB), 0, g->stream, g->off.p,
+                     g->adj.p, seedDev.p, ns, g->targetMap.p, count, slot_view(g),
+                     (int64_t)blocks * kWpb, g->table.p);
+  int32_t init[4] = {0, -1, 0, 0}; // overflow, stopSeed, notFull
+  GVH_CHECK(hipMemcpyAsync(g->flags.p, init, sizeof(init), hipMemcpyHostToDevice,
+                           g->stream));
+  hipLaunchKernelGGL(k_batch_kth, dim3(grid_for(count, kB)), dim3(kB), 0, g->stream,
+                     g->table.p, ns, count, k, g->flags.p + 1, g->flags.p + 2);
+  hipLaunchKernelGGL(k_batch_select, dim3(grid_for(count, kB)), dim3(kB), 0,
+                     g->stream, g->table.p, seedDev.p, ns, count, k, g->flags.p + 1,
+                     g->flags.p + 2, outIndex, stride, outV, outD, outCount);
+  GVH_CHECK_LAUNCH();
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  (void)targetsDev;
+  return 0;
+}
+
+int read_flag(gvizhip_grip *g, int idx, int32_t *out) {
+  GVH_CHECK(hipMemcpyAsync(out, g->flags.p + idx, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int set_flag(gvizhip_grip *g, int idx, int32_t v) {
+  GVH_CHECK(hipMemcpyAsync(g->flags.p + idx, &v, sizeof(int32_t),
+                           hipMemcpyHostToDevice, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+// gripPickFarCandidate / migrateOneToFinalLayer (gvizGRIPEmbedder.c:507-597):
+// a multi-source BFS of depth <= 64 from the final set, run on the host (the
+// final layer holds at most dim + 1 vertices).
+void migrate_one(gvizhip_grip *g, std::vector<int64_t> &filt,
+                 std::vector<int64_t> &borders, int64_t count) {
+  const int64_t finalEnd = borders[count - 1];
+  int64_t srcLayer = count - 1, candEnd;
+  do {
+    if (srcLayer == 0)
+      return;
+    srcLayer--;
+    candEnd = borders[srcLayer];
+  } while (candEnd <= finalEnd);
+  std::vector<int64_t> depth(g->n, -1);
+  std::deque<int64_t> q;
+  for (int64_t i = 0; i < finalEnd; i++) {
+    depth[filt[i]] = 0;
+    q.push_back(filt[i]);
+  }
+  while (!q.empty()) {
+    int64_t v = q.front();
+    q.pop_front();
+    if (depth[v] >= 64)
+      continue;
+    for (int64_t e = g->hoff[v]; e < g->hoff[v + 1]; e++) {
+      int64_t u = g->hadj[e];
+      if (depth[u] >= 0)
+        continue;
+      depth[u] = depth[v] + 1;
+      q.push_back(u);
+    }
+  }
+  int64_t bestIdx = finalEnd, bestScore = 0;
+  for (int64_t j = finalEnd; j < candEnd; j++) {
+    int64_t v = filt[j];
+    int64_t score = depth[v] >= 0 ? depth[v] : 65;
+    if (score > bestScore || (score == bestScore && j > bestIdx)) {
+      bestScore = score;
+      bestIdx = j;
+    }
+  }
+  std::swap(filt[bestIdx], filt[finalEnd]);
+  borders[count - 1]++;
+}
+
+} // namespace
+
+extern "C" {
+
+int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
+                        const int64_t *adj, const uint8_t *active, int64_t kcap) {
+  if (!out || n <= 0 || !offsets || kcap <= 0) {
+    set_error("gvizhip_grip_create: invalid arguments");
+    return -1;
+  }
+  *out = nullptr;
+  int dc = 0;
+  if (hipGetDeviceCount(&dc) != hipSuccess || dc <= 0) {
+    set_error("gvizhip_grip_create: no GPU is visible to HIP");
+    return -1;
+  }
+  const int64_t m2 = offsets[n];
+  if (n >= INT32_MAX || m2 >= INT32_MAX || n * kcap >= ((int64_t)1 << 40)) {
+    set_error("gvizhip_grip_create: graph too large");
+    return -1;
+  }
+  gvizhip_grip *g = new gvizhip_grip();
+  g->n = n;
+  g->m2 = m2;
+  g->kcap = kcap;
+  g->hoff.assign(offsets, offsets + n + 1);
+  g->hadj.assign(adj, adj + m2);
+  g->hactive.assign(n, 1);
+  if (active)
+    g->hactive.assign(active, active + n);
+  std::vector<int32_t> o32(n + 1), a32(m2 > 0 ? m2 : 1);
+  for (int64_t i = 0; i <= n; i++)
+    o32[i] = (int32_t)offsets[i];
+  for (int64_t e = 0; e < m2; e++) {
+    if (adj[e] < 0 || adj[e] >= n) {
+      set_error("gvizhip_grip_create: adjacency entry out of range");
+      delete g;
+      return -1;
+    }
+    a32[e] = (int32_t)adj[e];
+  }
+  for (int64_t v = 0; v < n; v++)
+    g->activeCount += g->hactive[v] != 0;
+  g->edgeCount = m2 / 2;
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+      g->off.alloc(n + 1) || g->adj.alloc(m2 > 0 ? m2 : 1) || g->filt.alloc(n) ||
+      g->active.alloc(n) || g->visible.alloc(n) || g->state.alloc(n) ||
+      g->pos.alloc(n) || g->disp.alloc(n) || g->oldDisp.alloc(n) ||
+      g->heat.alloc(n) || g->oldCos.alloc(n) || g->lastF.alloc(n) ||
+      g->dispCalc.alloc(n) || g->knnV.alloc(n * kcap) || g->knnD.alloc(n * kcap) ||
+      g->knnCount.alloc(n) || g->partial.alloc(4 * 1024) || g->flags.alloc(8) ||
+      g->round.alloc(1) || g->qSrc.alloc(n) || g->qOut.alloc(n) ||
+      g->hitCount.alloc(n)) {
+    gvizhip_grip_destroy(g);
+    return -1;
+  }
+  hipStream_t st = g->stream;
+  if (hipMemcpyAsync(g->off.p, o32.data(), sizeof(int32_t) * (n + 1),
+                     hipMemcpyHostToDevice, st) != hipSuccess ||
+      (m2 > 0 && hipMemcpyAsync(g->adj.p, a32.data(), sizeof(int32_t) * m2,
+                                hipMemcpyHostToDevice, st) != hipSuccess) ||
+      hipMemcpyAsync(g->active.p, g->hactive.data(), n, hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemsetAsync(g->pos.p, 0, sizeof(double2) * n, st) != hipSuccess ||
+      hipMemsetAsync(g->disp.p, 0, sizeof(double2) * n, st) != hipSuccess ||
+      hipMemsetAsync(g->oldDisp.p, 0, sizeof(double2) * n, st) != hipSuccess ||
+      hipMemsetAsync(g->heat.p, 0, sizeof(double) * n, st) != hipSuccess ||
+      hipMemsetAsync(g->oldCos.p, 0, sizeof(double) * n, st) != hipSuccess ||
+      hipMemsetAsync(g->lastF.p, 0, sizeof(double) * n, st) != hipSuccess ||
+      hipMemsetAsync(g->dispCalc.p, 0, n, st) != hipSuccess ||
+      hipMemsetAsync(g->knnCount.p, 0, sizeof(int32_t) * n, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    set_error("gvizhip_grip_create: upload failed");
+    gvizhip_grip_destroy(g);
+    return -1;
+  }
+  *out = g;
+  return 0;
+}
+
+int gvizhip_grip_set_filtration(gvizhip_grip *g, const int64_t *filtration,
+                                int64_t count) {
+  if (!g || !filtration || count < 0 || count > g->n) {
+    set_error("gvizhip_grip_set_filtration: invalid arguments");
+    return -1;
+  }
+  std::vector<int32_t> f32(count > 0 ? count : 1);
+  for (int64_t i = 0; i < count; i++)
+    f32[i] = (int32_t)filtration[i];
+  if (count > 0)
+    GVH_CHECK(hipMemcpy(g->filt.p, f32.data(), sizeof(int32_t) * count,
+                        hipMemcpyHostToDevice));
+  g->filtCount = count;
+  return 0;
+}
+
+int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
+                                    int64_t *filtration, int64_t *borders) {
+  if (!g || !filtration || !borders) {
+    set_error("gvizhip_grip_mis_filtration: invalid arguments");
+    return -1;
+  }
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  std::vector<int64_t> bord;
+  std::vector<int64_t> filt(g->activeCount);
+  bord.push_back(g->activeCount);
+  // ---- layer 1: lexicographically-first MIS of the subgraph
+  GVH_CHECK(hipMemsetAsync(g->state.p, 0, n, st));
+  int32_t undecided = 1;
+  int rounds = 0;
+  while (undecided) {
+    if (set_flag(g, 0, 0))
+      return -1;
+    for (int r = 0; r < 4; r++)
+      hipLaunchKernelGGL(k_lfmis_graph, dim3(grid_for(n, kB)), dim3(kB), 0, st,
+                         g->off.p, g->adj.p, g->active.p, n, g->state.p, g->flags.p);
+    GVH_CHECK_LAUNCH();
+    if (read_flag(g, 0, &undecided))
+      return -1;
+    if (++rounds > 4 * (n + 4)) {
+      set_error("gvizhip_grip_mis_filtration: layer-1 rounds did not settle");
+      return -1;
+    }
+  }
+  std::vector<uint8_t> state(n);
+  GVH_CHECK(hipMemcpy(state.data(), g->state.p, n, hipMemcpyDeviceToHost));
+  std::vector<int64_t> cur;
+  for (int64_t v = 0; v < n; v++)
+    if (g->hactive[v] && state[v] == kIn)
+      cur.push_back(v);
+  const int64_t tailCount = g->activeCount - (int64_t)cur.size();
+  if (tailCount > 0) {
+    if (g->keys.alloc(tailCount) || g->vals.alloc(tailCount))
+      return -1;
+    if (set_flag(g, 0, 0))
+      return -1;
+    hipLaunchKernelGGL(k_tail_keys, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->off.p,
+                       g->adj.p, g->active.p, g->state.p, n, g->keys.p, g->vals.p,
+                       g->flags.p);
+    GVH_CHECK_LAUNCH();
+    if (g->sorter.sort(g->keys.p, g->vals.p, tailCount, 0, 64, st))
+      return -1;
+    std::vector<uint32_t> order(tailCount);
+    GVH_CHECK(hipMemcpyAsync(order.data(), g->vals.p, sizeof(uint32_t) * tailCount,
+                             hipMemcpyDeviceToHost, st));
+    GVH_CHECK(hipStreamSynchronize(st));
+    for (int64_t r = 0; r < tailCount; r++)
+      filt[g->activeCount - 1 - r] = order[r];
+  }
+  bord.push_back((int64_t)cur.size());
+  // ---- layers i >= 2: LF-MIS of the distance-2^(i-1) conflict graph
+  DevBuf<int32_t> candDev, lists, listsD, listCount;
+  DevBuf<uint8_t> candFlag, cstate;
+  if (candFlag.alloc(n))
+    return -1;
+  int64_t i = 2;
+  int64_t stride = 32;
+  for (;;) {
+    const int64_t m = (int64_t)cur.size();
+    const int64_t radius = (int64_t)1 << (i - 1 < 62 ? i - 1 : 62);
+    std::vector<int32_t> c32(m > 0 ? m : 1);
+    std::vector<uint8_t> flag(n, 0);
+    for (int64_t j = 0; j < m; j++) {
+      c32[j] = (int32_t)cur[j];
+      flag[cur[j]] = 1;
+    }
+    if (candDev.alloc(m > 0 ? m : 1) || cstate.alloc(m > 0 ? m : 1) ||
+        listCount.alloc(m > 0 ? m : 1))
+      return -1;
+    GVH_CHECK(hipMemcpyAsync(candDev.p, c32.data(), sizeof(int32_t) * (m > 0 ? m : 1),
+                             hipMemcpyHostToDevice, st));
+    GVH_CHECK(hipMemcpyAsync(candFlag.p, flag.data(), n, hipMemcpyHostToDevice, st));
+    // conflict lists: lower-id candidates within the radius (candidate index
+    // space: the candidates are ascending, so a lower id is a lower index)
+    std::vector<int32_t> idxOf;
+    for (;;) {
+      if (lists.alloc(m * stride) || listsD.alloc(m * stride) || set_flag(g, 0, 0))
+        return -1;
+      const int maxDepth = radius > INT32_MAX ? INT32_MAX : (int)radius;
+      if (run_queries(g, candDev.p, nullptr, m, (int)stride, stride, candFlag.p,
+                      maxDepth, true, false, lists.p, listsD.p,
+                      listCount.p))
+        return -1;
+      int32_t ov = 0;
+      if (read_flag(g, 0, &ov))
+        return -1;
+      if (!ov)
+        break;
+      stride *= 2;
+    }
+    // map conflict vertex ids to candidate indices on the host
+    std::vector<int32_t> lst(m * stride), cnt(m > 0 ? m : 1);
+    GVH_CHECK(hipMemcpy(lst.data(), lists.p, sizeof(int32_t) * m * stride,
+                        hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(cnt.data(), listCount.p, sizeof(int32_t) * (m > 0 ? m : 1),
+                        hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < m; j++)
+      for (int32_t t = 0; t < cnt[j]; t++) {
+        int32_t v = lst[j * stride + t];
+        lst[j * stride + t] = (int32_t)(std::lower_bound(cur.begin(), cur.end(),
+                                                         (int64_t)v) -
+                                        cur.begin());
+      }
+    GVH_CHECK(hipMemcpy(lists.p, lst.data(), sizeof(int32_t) * m * stride,
+                        hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemsetAsync(cstate.p, 0, m > 0 ? m : 1, st));
+    undecided = m > 0;
+    int64_t guard = 0;
+    while (undecided) {
+      if (set_flag(g, 0, 0))
+        return -1;
+      for (int r = 0; r < 4; r++)
+        hipLaunchKernelGGL(k_lfmis_lists, dim3(grid_for(m, kB)), dim3(kB), 0, st,
+                           lists.p, listCount.p, m, stride, cstate.p, g->flags.p);
+      GVH_CHECK_LAUNCH();
+      if (read_flag(g, 0, &undecided))
+        return -1;
+      if (++guard > 4 * (m + 4)) {
+        set_error("gvizhip_grip_mis_filtration: layer rounds did not settle");
+        return -1;
+      }
+    }
+    std::vector<uint8_t> cs(m > 0 ? m : 1);
+    GVH_CHECK(hipMemcpy(cs.data(), cstate.p, m > 0 ? m : 1, hipMemcpyDeviceToHost));
+    std::vector<int64_t> next;
+    int64_t ptr = bord[i - 1] - 1;
+    for (int64_t j = 0; j < m; j++) {
+      if (cs[j] == kIn)
+        next.push_back(cur[j]);
+      else
+        filt[ptr--] = cur[j];
+    }
+    bord.push_back((int64_t)next.size());
+    cur.swap(next);
+    if ((int64_t)cur.size() > dim + 1) {
+      i++;
+      if (i > 126) {
+        set_error("gvizhip_grip_mis_filtration: too many layers");
+        return -1;
+      }
+      continue;
+    }
+    break;
+  }
+  for (int64_t k = 0; k < (int64_t)cur.size(); k++)
+    filt[k] = cur[k];
+  while (bord[i] < dim + 1) {
+    int64_t before = bord[i];
+    migrate_one(g, filt, bord, i + 1);
+    if (bord[i] == before)
+      break;
+  }
+  const int64_t layers = i + 1;
+  for (int64_t k = 0; k < g->activeCount; k++)
+    filtration[k] = filt[k];
+  for (int64_t k = 0; k < layers; k++)
+    borders[k] = bord[k];
+  if (gvizhip_grip_set_filtration(g, filtration, g->activeCount))
+    return -1;
+  return layers;
+}
+
+int gvizhip_grip_knn(gvizhip_grip *g, const uint8_t *visible, const int64_t *sources,
+                     int64_t count, int64_t k, int64_t *outVertex, int64_t *outDist,
+                     int64_t *counts) {
+  if (!g || count < 0 || k < 0 || (count > 0 && (!sources || !counts))) {
+    set_error("gvizhip_grip_knn: invalid arguments");
+    return -1;
+  }
+  if (count == 0)
+    return 0;
+  for (int64_t q = 0; q < count; q++)
+    if (sources[q] < 0 || sources[q] >= g->n || !g->hactive[sources[q]]) {
+      set_error("gvizhip_grip_knn: source %lld is not a subgraph vertex",
+                (long long)sources[q]);
+      return -1;
+    }
+  std::vector<int32_t> s32(count);
+  for (int64_t q = 0; q < count; q++)
+    s32[q] = (int32_t)sources[q];
+  const int64_t kk = k > 0 ? k : 1;
+  if (g->qSrc.alloc(count) || g->hitV.alloc(count * kk) || g->hitD.alloc(count * kk) ||
+      g->hitCount.alloc(count))
+    return -1;
+  GVH_CHECK(hipMemcpyAsync(g->qSrc.p, s32.data(), sizeof(int32_t) * count,
+                           hipMemcpyHostToDevice, g->stream));
+  if (upload_visible(g, visible))
+    return -1;
+  if (k == 0) {
+    for (int64_t q = 0; q < count; q++)
+      counts[q] = 0;
+    return 0;
+  }
+  if (run_queries(g, g->qSrc.p, nullptr, count, (int)k, k,
+                  visible ? g->visible.p : nullptr, 0, false, true, g->hitV.p,
+                  g->hitD.p, g->hitCount.p))
+    return -1;
+  std::vector<int32_t> hv(count * k), hd(count * k), hc(count);
+  GVH_CHECK(hipMemcpyAsync(hv.data(), g->hitV.p, sizeof(int32_t) * count * k,
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipMemcpyAsync(hd.data(), g->hitD.p, sizeof(int32_t) * count * k,
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipMemcpyAsync(hc.data(), g->hitCount.p, sizeof(int32_t) * count,
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  for (int64_t q = 0; q < count; q++) {
+    counts[q] = hc[q];
+    for (int64_t j = 0; j < k; j++) {
+      bool ok = j < hc[q];
+      if (outVertex)
+        outVertex[q * k + j] = ok ? hv[q * k + j] : -1;
+      if (outDist)
+        outDist[q * k + j] = ok ? hd[q * k + j] : -1;
+    }
+  }
+  return 0;
+}
+
+int gvizhip_grip_knn_batch(gvizhip_grip *g, const uint8_t *visible,
+                           const int64_t *targets, int64_t count, int64_t k,
+                           int64_t *outVertex, int64_t *outDist, int64_t *counts) {
+  if (!g || !visible || count < 0 || k <= 0 || (count > 0 && (!targets || !counts))) {
+    set_error("gvizhip_grip_knn_batch: invalid arguments");
+    return -1;
+  }
+  if (count == 0)
+    return 0;
+  std::vector<int32_t> t32(count);
+  for (int64_t t = 0; t < count; t++)
+    t32[t] = (int32_t)targets[t];
+  if (g->hitV.alloc(count * k) || g->hitD.alloc(count * k) ||
+      g->hitCount.alloc(count))
+    return -1;
+  if (run_batch(g, visible, nullptr, t32, nullptr, count, (int)k, k, g->hitV.p,
+                g->hitD.p, g->hitCount.p))
+    return -1;
+  std::vector<int32_t> hv(count * k), hd(count * k), hc(count);
+  GVH_CHECK(hipMemcpy(hv.data(), g->hitV.p, sizeof(int32_t) * count * k,
+                      hipMemcpyDeviceToHost));
+  GVH_CHECK(hipMemcpy(hd.data(), g->hitD.p, sizeof(int32_t) * count * k,
+                      hipMemcpyDeviceToHost));
+  GVH_CHECK(hipMemcpy(hc.data(), g->hitCount.p, sizeof(int32_t) * count,
+                      hipMemcpyDeviceToHost));
+  for (int64_t q = 0; q < count; q++) {
+    counts[q] = hc[q];
+    for (int64_t j = 0; j < k; j++) {
+      bool ok = j < hc[q];
+      if (outVertex)
+        outVertex[q * k + j] = ok ? hv[q * k + j] : -1;
+      if (outDist)
+        outDist[q * k + j] = ok ? hd[q * k + j] : -1;
+    }
+  }
+  return 0;
+}
+
+int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
+                        const gvizhip_grip_vertex *dec, const int64_t *knnVertex,
+                        const int64_t *knnDist) {
+  if (!g) {
+    set_error("gvizhip_grip_upload: null context");
+    return -1;
+  }
+  const int64_t n = g->n;
+  if (positions)
+    GVH_CHECK(hipMemcpy(g->pos.p, positions, sizeof(double2) * n,
+                        hipMemcpyHostToDevice));
+  if (dec) {
+    std::vector<double2> d(n), od(n);
+    std::vector<double> h(n), oc(n), lf(n);
+    std::vector<uint8_t> dc(n);
+    std::vector<int32_t> kc(n);
+    for (int64_t v = 0; v < n; v++) {
+      d[v] = {dec[v].dispX, dec[v].dispY};
+      od[v] = {dec[v].oldDispX, dec[v].oldDispY};
+      h[v] = dec[v].heat;
+      oc[v] = dec[v].oldCos;
+      lf[v] = dec[v].lastForceMag;
+      dc[v] = dec[v].dispCalculated ? 1 : 0;
+      kc[v] = (int32_t)dec[v].knnCount;
+    }
+    GVH_CHECK(hipMemcpy(g->disp.p, d.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->oldDisp.p, od.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->heat.p, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->oldCos.p, oc.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->lastF.p, lf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->dispCalc.p, dc.data(), n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->knnCount.p, kc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  }
+  const int64_t tot = n * g->kcap;
+  if (knnVertex) {
+    std::vector<int32_t> v(tot);
+    for (int64_t i = 0; i < tot; i++)
+      v[i] = (int32_t)knnVertex[i];
+    GVH_CHECK(hipMemcpy(g->knnV.p, v.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice));
+  }
+  if (knnDist) {
+    std::vector<int32_t> v(tot);
+    for (int64_t i = 0; i < tot; i++)
+      v[i] = (int32_t)knnDist[i];
+    GVH_CHECK(hipMemcpy(g->knnD.p, v.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int gvizhip_grip_download(gvizhip_grip *g, double *positions, gvizhip_grip_vertex *dec,
+                          int64_t *knnVertex, int64_t *knnDist) {
+  if (!g) {
+    set_error("gvizhip_grip_download: null context");
+    return -1;
+  }
+  const int64_t n = g->n;
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  if (positions)
+    GVH_CHECK(hipMemcpy(positions, g->pos.p, sizeof(double2) * n,
+                        hipMemcpyDeviceToHost));
+  if (dec) {
+    std::vector<double2> d(n), od(n);
+    std::vector<double> h(n), oc(n), lf(n);
+    std::vector<uint8_t> dc(n);
+    std::vector<int32_t> kc(n);
+    GVH_CHECK(hipMemcpy(d.data(), g->disp.p, sizeof(double2) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(od.data(), g->oldDisp.p, sizeof(double2) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(h.data(), g->heat.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(oc.data(), g->oldCos.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(lf.data(), g->lastF.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(dc.data(), g->dispCalc.p, n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(kc.data(), g->knnCount.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    for (int64_t v = 0; v < n; v++) {
+      dec[v].dispX = d[v].x;
+      dec[v].dispY = d[v].y;
+      dec[v].oldDispX = od[v].x;
+      dec[v].oldDispY = od[v].y;
+      dec[v].heat = h[v];
+      dec[v].oldCos = oc[v];
+      dec[v].lastForceMag = lf[v];
+      dec[v].dispCalculated = dc[v];
+      dec[v].knnCount = kc[v];
+    }
+  }
+  const int64_t tot = n * g->kcap;
+  if (knnVertex || knnDist) {
+    std::vector<int32_t> v(tot), d(tot);
+    GVH_CHECK(hipMemcpy(v.data(), g->knnV.p, sizeof(int32_t) * tot, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(d.data(), g->knnD.p, sizeof(int32_t) * tot, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < tot; i++) {
+      if (knnVertex)
+        knnVertex[i] = v[i];
+      if (knnDist)
+        knnDist[i] = d[i];
+    }
+  }
+  return 0;
+}
+
+int gvizhip_grip_clear_decorators(gvizhip_grip *g, int64_t active) {
+  if (!g || active < 0 || active > g->filtCount) {
+    set_error("gvizhip_grip_clear_decorators: invalid arguments");
+    return -1;
+  }
+  hipLaunchKernelGGL(k_clear, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
+                     g->filt.p, active, g->n, g->disp.p, g->oldDisp.p, g->heat.p,
+                     g->oldCos.p, g->knnCount.p, g->dispCalc.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int gvizhip_grip_update_knn(gvizhip_grip *g, int64_t active, const uint8_t *visible,
+                            int64_t k, int32_t batch) {
+  if (!g || !visible || active < 0 || active > g->filtCount || k < 0 ||
+      k > g->kcap) {
+    set_error("gvizhip_grip_update_knn: invalid arguments");
+    return -1;
+  }
+  if (active == 0 || k == 0)
+    return 0;
+  if (upload_visible(g, visible))
+    return -1;
+  if (batch) {
+    std::vector<int32_t> f(active);
+    GVH_CHECK(hipMemcpy(f.data(), g->filt.p, sizeof(int32_t) * active,
+                        hipMemcpyDeviceToHost));
+    return run_batch(g, visible, g->filt.p, f, g->filt.p, active, (int)k, g->kcap,
+                     g->knnV.p, g->knnD.p, g->knnCount.p);
+  }
+  return run_queries(g, g->filt.p, g->filt.p, active, (int)k, g->kcap, g->visible.p,
+                     0, false, true, g->knnV.p, g->knnD.p, g->knnCount.p);
+}
+
+int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
+                       const uint8_t *visible, int64_t k, int32_t batch) {
+  if (!g || !visible || begin < 0 || end < begin || end > g->filtCount || k <= 0) {
+    set_error("gvizhip_grip_place: invalid arguments");
+    return -1;
+  }
+  const int64_t count = end - begin;
+  if (count == 0)
+    return 0;
+  if (g->hitV.alloc(count * k) || g->hitD.alloc(count * k) || g->hitCount.alloc(count))
+    return -1;
+  if (upload_visible(g, visible))
+    return -1;
+  if (batch) {
+    std::vector<int32_t> f(count);
+    GVH_CHECK(hipMemcpy(f.data(), g->filt.p + begin, sizeof(int32_t) * count,
+                        hipMemcpyDeviceToHost));
+    if (run_batch(g, visible, g->filt.p + begin, f, nullptr, count, (int)k, k,
+                  g->hitV.p, g->hitD.p, g->hitCount.p))
+      return -1;
+  } else if (run_queries(g, g->filt.p + begin, nullptr, count, (int)k, k,
+                         g->visible.p, 0, false, true, g->hitV.p, g->hitD.p,
+                         g->hitCount.p)) {
+    return -1;
+  }
+  if (set_flag(g, 3, 0))
+    return -1;
+  hipLaunchKernelGGL(k_place, dim3(grid_for(count, kB)), dim3(kB), 0, g->stream,
+                     g->filt.p, begin, end, g->hitV.p, g->hitCount.p, k, g->pos.p,
+                     g->flags.p + 3);
+  GVH_CHECK_LAUNCH();
+  int32_t empty = 0;
+  if (read_flag(g, 3, &empty))
+    return -1;
+  if (empty) {
+    set_error("gvizhip_grip_place: a vertex has no visible vertex in reach");
+    return -1;
+  }
+  return 0;
+}
+
+int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
+                              gvizhip_grip_round_stats *stats) {
+  if (!g || active < 0 || active > g->filtCount) {
+    set_error("gvizhip_grip_refine_round: invalid arguments");
+    return -1;
+  }
+  hipStream_t st = g->stream;
+  GripRound r{};
+  if (active > 0) {
+    const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
+    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kB), 0, st, g->filt.p, active,
+                       layer, g->off.p, g->adj.p, g->pos.p, g->knnV.p, g->knnD.p,
+                       g->knnCount.p, g->kcap, g->disp.p, g->oldDisp.p, g->heat.p,
+                       g->oldCos.p, g->lastF.p, g->dispCalc.p, g->partial.p);
+    hipLaunchKernelGGL(k_round_means, dim3(1), dim3(256), 0, st, g->partial.p,
+                       (int)blocks, active, g->round.p);
+    hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, g->filt.p,
+                       active, g->pos.p, g->disp.p, g->round.p, g->partial.p);
+    hipLaunchKernelGGL(k_round_omega, dim3(1), dim3(256), 0, st, g->partial.p,
+                       (int)blocks, active, g->round.p);
+    hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, g->filt.p,
+                       active, g->pos.p, g->disp.p, g->heat.p, g->lastF.p,
+                       g->round.p, g->partial.p);
+    hipLaunchKernelGGL(k_round_stats, dim3(1), dim3(256), 0, st, g->partial.p,
+                       (int)blocks, active, g->round.p);
+    GVH_CHECK_LAUNCH();
+    GVH_CHECK(hipMemcpyAsync(&r, g->round.p, sizeof(GripRound), hipMemcpyDeviceToHost,
+                             st));
+  }
+  GVH_CHECK(hipStreamSynchronize(st));
+  if (stats) {
+    stats->maxDisp = r.maxDisp;
+    stats->meanDisp = active ? r.sumDisp / (double)active : 0.0;
+    stats->meanForce = active ? r.sumForce / (double)active : 0.0;
+    stats->meanHeat = active ? r.sumHeat / (double)active : 0.0;
+  }
+  return 0;
+}
+
+void gvizhip_grip_destroy(gvizhip_grip *g) {
+  if (!g)
+    return;
+  if (g->stream) {
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipStreamDestroy(g->stream);
+  }
+  delete g;
+}
+
+} // extern "C"

@@ -110,12 +110,15 @@ def lib():
         "gvizhip_tutte_destroy": (None, [_vp]),
         "gvizhip_grip_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
         "gvizhip_grip_mis_filtration": (_i64, [_vp, _i64, _vp, _vp]),
+        "gvizhip_grip_set_filtration": (C.c_int, [_vp, _vp, _i64]),
         "gvizhip_grip_knn": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
         "gvizhip_grip_knn_batch": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
         "gvizhip_grip_upload": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
-        "gvizhip_grip_download": (C.c_int, [_vp, _vp, _vp]),
-        "gvizhip_grip_update_knn": (C.c_int, [_vp, _vp, _i64, _vp, _i64]),
-        "gvizhip_grip_refine_round": (C.c_int, [_vp, _vp, _i64, _i64, C.POINTER(GripRoundStats)]),
+        "gvizhip_grip_download": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_grip_clear_decorators": (C.c_int, [_vp, _i64]),
+        "gvizhip_grip_update_knn": (C.c_int, [_vp, _i64, _vp, _i64, C.c_int32]),
+        "gvizhip_grip_place": (C.c_int, [_vp, _i64, _i64, _vp, _i64, C.c_int32]),
+        "gvizhip_grip_refine_round": (C.c_int, [_vp, _i64, _i64, C.POINTER(GripRoundStats)]),
         "gvizhip_grip_destroy": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
@@ -140,10 +143,11 @@ DECLARED = [
     "gvizhip_force_profile", "gvizhip_force_destroy",
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
     "gvizhip_tutte_sweeps", "gvizhip_tutte_destroy",
-    "gvizhip_grip_create", "gvizhip_grip_mis_filtration", "gvizhip_grip_knn",
-    "gvizhip_grip_knn_batch", "gvizhip_grip_upload", "gvizhip_grip_download",
-    "gvizhip_grip_update_knn", "gvizhip_grip_refine_round",
-    "gvizhip_grip_destroy",
+    "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
+    "gvizhip_grip_set_filtration", "gvizhip_grip_knn", "gvizhip_grip_knn_batch",
+    "gvizhip_grip_upload", "gvizhip_grip_download",
+    "gvizhip_grip_clear_decorators", "gvizhip_grip_update_knn",
+    "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_destroy",
 ]
 
 
@@ -330,6 +334,101 @@ class TutteEngine:
     def close(self):
         if getattr(self, "h", None):
             lib().gvizhip_tutte_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class GripEngine:
+    """Device GRIP state: gvizhip_grip_*."""
+
+    def __init__(self, offsets, adj, active=None, kcap=10):
+        self._o, self._a = _csr(offsets, adj)
+        self.n = self._o.shape[0] - 1
+        self.kcap = kcap
+        act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        h = _vp()
+        if lib().gvizhip_grip_create(C.byref(h), self.n, _p(self._o), _p(self._a),
+                                     _p(act), kcap) != 0:
+            _err("gvizhip_grip_create")
+        self.h = h
+
+    def mis_filtration(self, dim=2):
+        filt = np.zeros(self.n, dtype=np.int64)
+        borders = np.zeros(128, dtype=np.int64)
+        layers = lib().gvizhip_grip_mis_filtration(self.h, dim, _p(filt), _p(borders))
+        if layers < 0:
+            _err("gvizhip_grip_mis_filtration")
+        return filt[: borders[0]].copy(), borders[:layers].copy()
+
+    def set_filtration(self, filt):
+        f = np.ascontiguousarray(filt, dtype=np.int64)
+        if lib().gvizhip_grip_set_filtration(self.h, _p(f), f.shape[0]) != 0:
+            _err("gvizhip_grip_set_filtration")
+
+    def knn(self, sources, k, visible=None):
+        s = np.ascontiguousarray(sources, dtype=np.int64)
+        vis = None if visible is None else np.ascontiguousarray(visible, dtype=np.uint8)
+        ov = np.zeros(max(len(s) * k, 1), dtype=np.int64)
+        od = np.zeros(max(len(s) * k, 1), dtype=np.int64)
+        cnt = np.zeros(max(len(s), 1), dtype=np.int64)
+        if lib().gvizhip_grip_knn(self.h, _p(vis), _p(s), len(s), k, _p(ov), _p(od),
+                                  _p(cnt)) != 0:
+            _err("gvizhip_grip_knn")
+        return ov[: len(s) * k].reshape(len(s), k), od[: len(s) * k].reshape(len(s), k), cnt[: len(s)]
+
+    def knn_batch(self, targets, k, visible):
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        vis = np.ascontiguousarray(visible, dtype=np.uint8)
+        ov = np.zeros(max(len(t) * k, 1), dtype=np.int64)
+        od = np.zeros(max(len(t) * k, 1), dtype=np.int64)
+        cnt = np.zeros(max(len(t), 1), dtype=np.int64)
+        if lib().gvizhip_grip_knn_batch(self.h, _p(vis), _p(t), len(t), k, _p(ov),
+                                        _p(od), _p(cnt)) != 0:
+            _err("gvizhip_grip_knn_batch")
+        return ov[: len(t) * k].reshape(len(t), k), od[: len(t) * k].reshape(len(t), k), cnt[: len(t)]
+
+    def upload(self, positions=None, dec=None, knn_vertex=None, knn_dist=None):
+        pos = None if positions is None else np.ascontiguousarray(positions, dtype=np.float64)
+        d = None if dec is None else np.ascontiguousarray(dec, dtype=GRIP_VERTEX_DTYPE)
+        kv = None if knn_vertex is None else np.ascontiguousarray(knn_vertex, dtype=np.int64)
+        kd = None if knn_dist is None else np.ascontiguousarray(knn_dist, dtype=np.int64)
+        if lib().gvizhip_grip_upload(self.h, _p(pos), _p(d), _p(kv), _p(kd)) != 0:
+            _err("gvizhip_grip_upload")
+
+    def download(self):
+        pos = np.zeros((self.n, 2))
+        dec = np.zeros(self.n, dtype=GRIP_VERTEX_DTYPE)
+        kv = np.zeros((self.n, self.kcap), dtype=np.int64)
+        kd = np.zeros((self.n, self.kcap), dtype=np.int64)
+        if lib().gvizhip_grip_download(self.h, _p(pos), _p(dec), _p(kv), _p(kd)) != 0:
+            _err("gvizhip_grip_download")
+        return pos, dec, kv, kd
+
+    def clear_decorators(self, active):
+        if lib().gvizhip_grip_clear_decorators(self.h, active) != 0:
+            _err("gvizhip_grip_clear_decorators")
+
+    def update_knn(self, active, visible, k, batch=False):
+        vis = np.ascontiguousarray(visible, dtype=np.uint8)
+        if lib().gvizhip_grip_update_knn(self.h, active, _p(vis), k, int(batch)) != 0:
+            _err("gvizhip_grip_update_knn")
+
+    def place(self, begin, end, visible, k, batch=False):
+        vis = np.ascontiguousarray(visible, dtype=np.uint8)
+        if lib().gvizhip_grip_place(self.h, begin, end, _p(vis), k, int(batch)) != 0:
+            _err("gvizhip_grip_place")
+
+    def refine_round(self, active, layer):
+        st = GripRoundStats()
+        if lib().gvizhip_grip_refine_round(self.h, active, layer, C.byref(st)) != 0:
+            _err("gvizhip_grip_refine_round")
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_grip_destroy(self.h)
             self.h = None
 
     def __del__(self):

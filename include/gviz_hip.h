@@ -150,27 +150,37 @@ void gvizhip_tutte_destroy(gvizhip_tutte *t);
 /* ------------------------------------------------------------------- GRIP */
 typedef struct gvizhip_grip gvizhip_grip;
 
-/* Graph over parent ids 0..n-1 restricted to the subgraph; active[n] marks
- * subgraph vertices (NULL: all). kcap = kNN capacity per vertex. */
+/* Graph over parent ids 0..n-1 restricted to the subgraph (CSR in insertion
+ * order); active[n] marks subgraph vertices (NULL: all); kcap = kNN capacity
+ * per vertex (gvizGRIPEmbedderConfigureKnnCapacity). Replaces the
+ * allocations of gvizGRIPEmbedderInit (src/embedders/gvizGRIPEmbedder.c:
+ * 194-320). */
 int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
                         const int64_t *adj, const uint8_t *active,
                         int64_t kcap);
-/* createMISFiltration (src/embedders/gvizGRIPEmbedder.c:741-777):
- * filtration[activeCount], borders[<=128]; returns the layer count. */
+/* createMISFiltration (:741-777) for `dim` dimensions: writes
+ * filtration[activeCount] and borders[layers] (room for 128) and keeps the
+ * filtration on the device; returns the layer count. */
 int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
                                     int64_t *filtration, int64_t *borders);
+/* Installs a filtration computed elsewhere (count entries). */
+int gvizhip_grip_set_filtration(gvizhip_grip *g, const int64_t *filtration,
+                                int64_t count);
 /* gvizSearchKNearestScratch (src/algorithms/search/gvizKNearest.c:244-306)
- * from each of sources[count], over the `visible` filter (n bytes):
- * out[count*k] of (vertex, dist) pairs, counts[count]. */
+ * from each of sources[count] over the `visible` filter (n bytes, NULL: all):
+ * out*[count*k], counts[count]. */
 int gvizhip_grip_knn(gvizhip_grip *g, const uint8_t *visible,
                      const int64_t *sources, int64_t count, int64_t k,
                      int64_t *outVertex, int64_t *outDist, int64_t *counts);
-/* gvizSearchKNearestFromVisibleBatch (:168-210). */
+/* gvizSearchKNearestFromVisibleBatch (:168-210): one BFS per visible seed in
+ * ascending id, ties kept in seed order, early stop once every target holds
+ * k entries. */
 int gvizhip_grip_knn_batch(gvizhip_grip *g, const uint8_t *visible,
                            const int64_t *targets, int64_t count, int64_t k,
                            int64_t *outVertex, int64_t *outDist,
                            int64_t *counts);
-/* Per-vertex GRIP state (gvizGRIPDecorators without the kNN array). */
+/* Per-vertex GRIP state (gvizGRIPDecorators without the kNN array, plus the
+ * dispCalculated bit). */
 typedef struct gvizhip_grip_vertex {
   double dispX, dispY, oldDispX, oldDispY;
   double heat, oldCos, lastForceMag;
@@ -180,19 +190,26 @@ typedef struct gvizhip_grip_vertex {
 typedef struct gvizhip_grip_round_stats {
   double maxDisp, meanDisp, meanForce, meanHeat;
 } gvizhip_grip_round_stats;
-/* Uploads positions (2n), vertex state (n) and kNN lists (n*kcap pairs). */
+/* Host <-> device copies of positions (2n), vertex state (n) and kNN lists
+ * (n*kcap vertex and dist entries); any pointer may be NULL. */
 int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
                         const gvizhip_grip_vertex *dec,
                         const int64_t *knnVertex, const int64_t *knnDist);
 int gvizhip_grip_download(gvizhip_grip *g, double *positions,
-                          gvizhip_grip_vertex *dec);
-/* Recomputes the device kNN lists of filtration[0..active) against the
- * visible set (the refinement kNN of updateKNNs, :954-1006). */
-int gvizhip_grip_update_knn(gvizhip_grip *g, const int64_t *filtration,
-                            int64_t active, const uint8_t *visible, int64_t k);
+                          gvizhip_grip_vertex *dec, int64_t *knnVertex,
+                          int64_t *knnDist);
+/* clearDecorators (:936-952) for filtration[0..active). */
+int gvizhip_grip_clear_decorators(gvizhip_grip *g, int64_t active);
+/* updateKNNs (:954-1006): kNN lists of filtration[0..active) against the
+ * visible set, per-vertex BFS or (batch != 0) the batched search. */
+int gvizhip_grip_update_knn(gvizhip_grip *g, int64_t active,
+                            const uint8_t *visible, int64_t k, int32_t batch);
+/* placeLayerVertices (:816-868): filtration[begin..end) placed at the
+ * barycentre of their k visible BFS-nearest vertices. */
+int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
+                       const uint8_t *visible, int64_t k, int32_t batch);
 /* gvizGRIPEmbedderRefineRound (:1289-1333) over filtration[0..active). */
-int gvizhip_grip_refine_round(gvizhip_grip *g, const int64_t *filtration,
-                              int64_t active, int64_t layer,
+int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
                               gvizhip_grip_round_stats *stats);
 void gvizhip_grip_destroy(gvizhip_grip *g);
 

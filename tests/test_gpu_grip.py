@@ -1,0 +1,139 @@
+"""HIP GRIP pieces against the reference's fixtures and the oracle: MIS
+filtration and kNN lists bit-exact (integer results), refinement rounds to
+FP64 round-off (1e-12 relative; only the reduction order differs)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import golden
+from grapher_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _dec_from(d0, n):
+    from grapher_amd import native
+    dec = np.zeros(n, dtype=native.GRIP_VERTEX_DTYPE)
+    dec["dispX"], dec["dispY"] = d0["disp"][:, 0], d0["disp"][:, 1]
+    dec["oldDispX"], dec["oldDispY"] = d0["oldDisp"][:, 0], d0["oldDisp"][:, 1]
+    for k in ("heat", "oldCos", "lastForceMag", "knnCount", "dispCalculated"):
+        dec[k] = d0[k]
+    return dec
+
+
+@pytest.mark.parametrize("name", ["grip_rgg300", "grip_grid10", "grip_ba200"])
+def test_mis_filtration_exact(gpu, name):
+    f = golden(name)
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    eng = gpu.GripEngine(off, adj, kcap=int(f["kcap"]))
+    filt, borders = eng.mis_filtration(2)
+    assert np.array_equal(borders, f["borders"])
+    assert np.array_equal(filt, f["filtration"])
+
+
+@pytest.mark.parametrize("make", [
+    lambda: G.random_geometric(20000, 8.0, 3)[:2],
+    lambda: G.grid(60, 70),
+    lambda: G.barabasi_albert(5000, 3, 2),
+    lambda: G.random_tree(3000, 4),
+])
+def test_mis_filtration_matches_oracle_larger(gpu, make):
+    n, e = make()
+    off, adj = G.csr_from_edges(n, e)
+    of, ob = O.mis_filtration(O.Graph(off, adj))
+    eng = gpu.GripEngine(off, adj, kcap=10)
+    filt, borders = eng.mis_filtration(2)
+    assert np.array_equal(borders, ob)
+    assert np.array_equal(filt, of)
+
+
+def test_knn_per_vertex_and_batch_exact(gpu):
+    f = golden("knn_rgg300")
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    eng = gpu.GripEngine(off, adj, kcap=16)
+    k = int(f["k"])
+    vis = f["visible"].astype(np.uint8)
+    v, d, c = eng.knn(f["sources"], k, vis)
+    assert np.array_equal(c, f["per_count"])
+    for i in range(len(c)):
+        assert np.array_equal(v[i, :c[i]], f["per_list"][i, :c[i], 0])
+        assert np.array_equal(d[i, :c[i]], f["per_list"][i, :c[i], 1])
+    v, d, c = eng.knn_batch(f["batch_targets"], k, vis)
+    assert np.array_equal(c, f["batch_count"])
+    for i in range(len(c)):
+        assert np.array_equal(v[i, :c[i]], f["batch_list"][i, :c[i], 0])
+        assert np.array_equal(d[i, :c[i]], f["batch_list"][i, :c[i], 1])
+
+
+def test_knn_matches_oracle_on_larger_graph(gpu):
+    n, e, _ = G.random_geometric(50000, 8.0, 7)
+    off, adj = G.csr_from_edges(n, e)
+    g = O.Graph(off, adj)
+    rng = np.random.default_rng(1)
+    vis = (rng.random(n) < 0.02).astype(np.uint8)
+    src = rng.choice(n, 500, replace=False)
+    eng = gpu.GripEngine(off, adj, kcap=10)
+    v, d, c = eng.knn(src, 10, vis)
+    for i, s in enumerate(src):
+        o = O.knn_bfs(g, int(s), 10, vis)
+        assert c[i] == len(o)
+        assert np.array_equal(v[i, :c[i]], o[:, 0])
+        assert np.array_equal(d[i, :c[i]], o[:, 1])
+    targets = np.where(vis == 0)[0][:2000]
+    sv = (rng.random(n) < 0.002).astype(np.uint8)
+    sv[targets] = 0
+    ov, cnt = O.knn_batch(g, sv, 6, targets)
+    v, d, c = eng.knn_batch(targets, 6, sv)
+    assert np.array_equal(c, cnt)
+    for i in range(len(targets)):
+        assert np.array_equal(v[i, :c[i]], ov[i, :c[i], 0])
+        assert np.array_equal(d[i, :c[i]], ov[i, :c[i], 1])
+
+
+@pytest.mark.parametrize("name", ["grip_rgg300", "grip_grid10", "grip_ba200"])
+def test_refine_rounds_match_reference(gpu, name):
+    f = golden(name)
+    n = int(f["n"])
+    kcap = int(f["kcap"])
+    off, adj = G.csr_from_edges(n, f["edges"])
+    eng = gpu.GripEngine(off, adj, kcap=kcap)
+    eng.set_filtration(f["filtration"])
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(f["borders"][layer])
+        knn = f[f"r{i}_knn"]
+        eng.upload(f[f"r{i}_pos0"], _dec_from(f[f"r{i}_dec0"], n), knn[:, :, 0],
+                   knn[:, :, 1])
+        st = eng.refine_round(active, layer)
+        pos, dec, _, _ = eng.download()
+        ref = f[f"r{i}_pos1"]
+        scale = np.abs(ref).max()
+        assert np.abs(pos - ref).max() <= 1e-12 * scale, (i, layer)
+        d1 = f[f"r{i}_dec1"]
+        np.testing.assert_allclose(dec["heat"], d1["heat"], rtol=1e-12)
+        np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
+                                   f[f"r{i}_stats"], rtol=1e-11)
+
+
+def test_update_knn_matches_reference_lists(gpu):
+    f = golden("grip_rgg300")
+    n = int(f["n"])
+    kcap = int(f["kcap"])
+    off, adj = G.csr_from_edges(n, f["edges"])
+    eng = gpu.GripEngine(off, adj, kcap=kcap)
+    filt = f["filtration"]
+    eng.set_filtration(filt)
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(f["borders"][layer])
+        vis = np.zeros(n, np.uint8)
+        vis[filt[:active]] = 1
+        eng.update_knn(active, vis, kcap)
+        _, dec, kv, kd = eng.download()
+        d0 = f[f"r{i}_dec0"]
+        ref = f[f"r{i}_knn"]
+        for v in filt[:active]:
+            c = int(d0["knnCount"][v])
+            assert dec["knnCount"][v] == c
+            assert np.array_equal(kv[v, :c], ref[v, :c, 0])
+            assert np.array_equal(kd[v, :c], ref[v, :c, 1])
