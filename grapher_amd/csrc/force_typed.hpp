@@ -45,13 +45,14 @@ template <class Real> struct Phys;
 template <> struct Phys<double> {
   // FP64: the reference's operation order; sqrt and division are correctly
   // rounded, and this TU is compiled without FMA contraction.
-  template <int MODEL>
+  template <int MODEL, bool NEAR>
   __device__ __forceinline__ static void rep(double dx, double dy, double dist,
                                              double radiusSum, double mSelf,
                                              double mOther, double kSq,
                                              double oc, int64_t selfId,
                                              int64_t otherId, double &ax,
-                                             double &ay) {
+                                             double &ay, double, double,
+                                             const double *, int64_t) {
     double gap = dist - radiusSum;
     double ux, uy;
     if (dist > 0.0) {
@@ -108,13 +109,23 @@ template <> struct Phys<double> {
 
 template <> struct Phys<float> {
   // FP32: hardware reciprocal and reciprocal square root (1 ulp each).
-  template <int MODEL>
+  // NEAR (FP64 positions): when two circles nearly touch, the 1/gap
+  // magnitude is ill-conditioned, so the gap is recomputed from the FP64
+  // position difference and FP64 radii.
+  template <int MODEL, bool NEAR>
   __device__ __forceinline__ static void rep(float dx, float dy, float dist,
                                              float radiusSum, float mSelf,
                                              float mOther, float kSq, float oc,
                                              int64_t selfId, int64_t otherId,
-                                             float &ax, float &ay) {
+                                             float &ax, float &ay, double dxd,
+                                             double dyd,
+                                             const double *__restrict__ r64,
+                                             int64_t n) {
     float gap = dist - radiusSum;
+    if (NEAR && radiusSum > 0.0f && fabsf(gap) < 1e-3f * dist) {
+      double rs = r64[selfId] + (otherId < n ? r64[otherId] : 0.0);
+      gap = (float)(sqrt(dxd * dxd + dyd * dyd) - rs);
+    }
     float ux, uy;
     if (dist > 0.0f) {
       float inv = __builtin_amdgcn_rcpf(dist);
@@ -279,7 +290,8 @@ __global__ __launch_bounds__(kTB) void k_traverse(
     const typename V2<Real>::type *__restrict__ smr,
     const uint32_t *__restrict__ sidx, int64_t n, Real theta, Real kSq, Real oc,
     typename V2<Real>::type *__restrict__ rep, int32_t *__restrict__ inter,
-    Scalars *__restrict__ sc) {
+    const double *__restrict__ r64, Scalars *__restrict__ sc) {
+  constexpr bool NEAR = sizeof(PosT) == sizeof(double);
   __shared__ int32_t sNode[kTW][kStack];
   __shared__ uint64_t sMask[kTW][kStack];
   if (sc->stopped)
@@ -310,10 +322,13 @@ __global__ __launch_bounds__(kTB) void k_traverse(
       if (in && p != s) {
         typename V2<PosT>::type o = spos[p];
         typename V2<Real>::type om = smr[p];
-        Real dx = (Real)(o.x - me.x), dy = (Real)(o.y - me.y);
+        const PosT ex = o.x - me.x, ey = o.y - me.y;
+        Real dx = (Real)ex, dy = (Real)ey;
         Real dist = Phys<Real>::dist(dx, dy);
-        Phys<Real>::template rep<MODEL>(dx, dy, dist, mr.y + om.y, mr.x, om.x,
-                                        kSq, oc, selfId, (int64_t)sidx[p], ax, ay);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + om.y, mr.x,
+                                              om.x, kSq, oc, selfId,
+                                              (int64_t)sidx[p], ax, ay, ex, ey,
+                                              r64, n);
         cnt++;
       }
       continue;
@@ -325,11 +340,13 @@ __global__ __launch_bounds__(kTB) void k_traverse(
         if (in && p != s) {
           typename V2<PosT>::type o = spos[p];
           typename V2<Real>::type om = smr[p];
-          Real dx = (Real)(o.x - me.x), dy = (Real)(o.y - me.y);
+          const PosT ex = o.x - me.x, ey = o.y - me.y;
+          Real dx = (Real)ex, dy = (Real)ey;
           Real dist = Phys<Real>::dist(dx, dy);
-          Phys<Real>::template rep<MODEL>(dx, dy, dist, mr.y + om.y, mr.x, om.x,
-                                          kSq, oc, selfId, (int64_t)sidx[p], ax,
-                                          ay);
+          Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + om.y, mr.x,
+                                                om.x, kSq, oc, selfId,
+                                                (int64_t)sidx[p], ax, ay, ex, ey,
+                                                r64, n);
           cnt++;
         }
       }
@@ -337,11 +354,13 @@ __global__ __launch_bounds__(kTB) void k_traverse(
     }
     bool open = false;
     if (in) {
-      Real dx = (Real)(r.comx - me.x), dy = (Real)(r.comy - me.y);
+      const PosT ex = r.comx - me.x, ey = r.comy - me.y;
+      Real dx = (Real)ex, dy = (Real)ey;
       Real dist = Phys<Real>::dist(dx, dy);
       if (Phys<Real>::accept(r.twoh, dist, theta)) {
-        Phys<Real>::template rep<MODEL>(dx, dy, dist, mr.y + (Real)0, mr.x,
-                                        r.mass, kSq, oc, selfId, n, ax, ay);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + (Real)0, mr.x,
+                                              r.mass, kSq, oc, selfId, n, ax, ay,
+                                              ex, ey, r64, n);
         cnt++;
       } else {
         open = true;
@@ -377,7 +396,9 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
     const typename V2<PosT>::type *__restrict__ pos,
     const Real *__restrict__ massT, const Real *__restrict__ radT, int64_t n,
     Real kSq, Real oc, typename V2<Real>::type *__restrict__ rep,
-    int32_t *__restrict__ inter, const Scalars *__restrict__ sc) {
+    int32_t *__restrict__ inter, const double *__restrict__ r64,
+    const Scalars *__restrict__ sc) {
+  constexpr bool NEAR = sizeof(PosT) == sizeof(double);
   __shared__ typename V2<PosT>::type tPos[kTB];
   __shared__ Real tMass[kTB];
   __shared__ Real tRad[kTB];
@@ -403,10 +424,12 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
         const int64_t jj = jb + t;
         if (jj == i)
           continue;
-        Real dx = (Real)(tPos[t].x - me.x), dy = (Real)(tPos[t].y - me.y);
+        const PosT ex = tPos[t].x - me.x, ey = tPos[t].y - me.y;
+        Real dx = (Real)ex, dy = (Real)ey;
         Real dist = Phys<Real>::dist(dx, dy);
-        Phys<Real>::template rep<MODEL>(dx, dy, dist, rv + tRad[t], mv, tMass[t],
-                                        kSq, oc, i, jj, ax, ay);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, rv + tRad[t], mv,
+                                              tMass[t], kSq, oc, i, jj, ax, ay, ex,
+                                              ey, r64, n);
       }
     }
     __syncthreads();
@@ -581,12 +604,14 @@ int typed_step(ForceCommon &c, int mode) {
       hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
                          (const NodeRec<Real, PosT> *)c.nodes.p,
                          (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
-                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
+                         c.sc.p);
     else
       hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTB), 0, st,
                          (const NodeRec<Real, PosT> *)c.nodes.p,
                          (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
-                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
+                         c.sc.p);
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
@@ -595,11 +620,11 @@ int typed_step(ForceCommon &c, int mode) {
     if (model == 1)
       hipLaunchKernelGGL((k_allpairs<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
                          posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
-                         kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+                         kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p, c.sc.p);
     else
       hipLaunchKernelGGL((k_allpairs<Real, PosT, 0>), dim3(tb), dim3(kTB), 0, st,
                          posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
-                         kSq, oc, (R2 *)c.rep.p, inter, c.sc.p);
+                         kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p, c.sc.p);
     GVH_CHECK_LAUNCH();
   }
 

@@ -199,3 +199,20 @@ def csr_from_edges(n: int, edges: np.ndarray):
     offsets = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(counts, out=offsets[1:])
     return offsets, np.ascontiguousarray(adj)
+
+
+def largest_component(n: int, edges: np.ndarray):
+    """The vertex-induced subgraph on the largest connected component (what
+    the reference's GRIP benches lay out, tests/embedders/gvizGRIPPlaceBench.c:
+    41-95): returns (active mask uint8[n], edges inside the component, in the
+    original insertion order)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    A = sp.coo_matrix((np.ones(len(edges)), (edges[:, 0], edges[:, 1])), shape=(n, n))
+    ncomp, labels = connected_components(A, directed=False)
+    big = np.bincount(labels).argmax()
+    active = (labels == big).astype(np.uint8)
+    keep = active[edges[:, 0]].astype(bool) & active[edges[:, 1]].astype(bool)
+    return active, edges[keep]

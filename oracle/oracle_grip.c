@@ -337,8 +337,15 @@ int64_t oc_mis_filtration(const oc_csr *g, const uint8_t *active,
   uint8_t *cur = calloc((size_t)g->n, 1);
   mis_first(&c, cur);
   int64_t i = 2;
-  while (mis_iter(&c, i, cur, dim))
-    i++;
+  /* A subgraph with more than dim + 1 components never coarsens to dim + 1
+   * vertices (the reference keeps adding layers forever); stop at 126. */
+  while (mis_iter(&c, i, cur, dim)) {
+    if (++i > 126) {
+      free(cur);
+      mis_free(&c);
+      return -1;
+    }
+  }
   int64_t k = 0;
   for (int64_t v = 0; v < g->n; v++)
     if (cur[v])

@@ -345,6 +345,8 @@ def mis_filtration(g: Graph, active=None, dim=2):
     borders = np.zeros(256, dtype=np.int64)
     a = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
     layers = lib().oc_mis_filtration(C.byref(g.c), _ptr(a), dim, filt, borders)
+    if layers < 0:
+        raise ValueError("MIS filtration does not terminate: more than dim+1 components")
     return filt[: borders[0]].copy(), borders[:layers].copy()
 
 

@@ -38,10 +38,13 @@ def test_mis_filtration_exact(gpu, name):
     lambda: G.random_tree(3000, 4),
 ])
 def test_mis_filtration_matches_oracle_larger(gpu, make):
+    """On the largest connected component (a subgraph with more than dim+1
+    components never coarsens, in the reference as here)."""
     n, e = make()
+    active, e = G.largest_component(n, e)
     off, adj = G.csr_from_edges(n, e)
-    of, ob = O.mis_filtration(O.Graph(off, adj))
-    eng = gpu.GripEngine(off, adj, kcap=10)
+    of, ob = O.mis_filtration(O.Graph(off, adj), active)
+    eng = gpu.GripEngine(off, adj, active=active, kcap=10)
     filt, borders = eng.mis_filtration(2)
     assert np.array_equal(borders, ob)
     assert np.array_equal(filt, of)
