@@ -39,6 +39,23 @@ FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_INTERACTION = 15  # FR / LinLog repulsion as written in the reference
 
 
+PROFILE = os.path.join(ROOT, "profiles", "force_step_ba1m_linlog_mixed.md")
+
+
+def traversal_traffic_bytes():
+    """HBM bytes per traversal launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
+    import re
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"Median per launch: ([0-9.]+) MB read \+ ([0-9.]+) MB written", text)
+    if not m:
+        return None
+    return int((float(m.group(1)) + float(m.group(2))) * 1e6)
+
+
 def workload(n):
     g_t0 = time.time()
     n, edges = G.barabasi_albert(n, 3, 0)
@@ -109,7 +126,19 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"])
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copy calibration: 3 copies of 268435456 bytes")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
               "supported (the multi-GPU design is described in DESIGN.md)",
@@ -143,7 +172,9 @@ def main():
     achieved = flops / trav_avg_s / 1e12 if trav_avg_s > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                "traffic": traversal_traffic_bytes() if args.n == 1_000_000 else None,
+                "traffic_source": "profiles/force_step_ba1m_linlog_mixed.md (pmc)",
                 "kernel": "k_traverse (Barnes-Hut repulsion)",
                 "note": "compute-bound on the FP32 vector ALU (no matrix op on this "
                         "path); peak = MI355X FP32 vector; achieved = 15 FLOP x "

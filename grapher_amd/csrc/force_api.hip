@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 using namespace gvh;
@@ -233,9 +234,22 @@ int gvizhip_force_create(gvizhip_force **out, int64_t n, const int64_t *offsets,
   }
   f->degreeHost.resize(n);
   double sumDeg1 = 0.0;
+  std::vector<int32_t> heavy;
   for (int64_t i = 0; i < n; i++) {
     f->degreeHost[i] = deg32[i];
     sumDeg1 += (double)(deg32[i] + 1);
+    if (deg32[i] > 32)
+      heavy.push_back((int32_t)i);
+  }
+  c.nHeavy = (int64_t)heavy.size();
+  if (c.heavyRows.alloc(heavy.empty() ? 1 : heavy.size()) ||
+      c.attHeavy.alloc(2 * (n > 0 ? n : 1) * sizeof(double)) ||
+      (!heavy.empty() &&
+       hipMemcpy(c.heavyRows.p, heavy.data(), sizeof(int32_t) * heavy.size(),
+                 hipMemcpyHostToDevice) != hipSuccess)) {
+    set_error("gvizhip_force_create: heavy-row setup failed");
+    gvizhip_force_destroy(f);
+    return -1;
   }
   if (hipMemcpy(c.csrOff.p, off32.data(), sizeof(int32_t) * (n + 1),
                 hipMemcpyHostToDevice) != hipSuccess ||
@@ -720,6 +734,13 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
     GVH_CHECK(hipEventCreate(&c.travBegin));
     GVH_CHECK(hipEventCreate(&c.travEnd));
   }
+  if (getenv("GVIZHIP_TRAVERSAL_STATS") && !c.travStats.p) {
+    if (c.travStats.alloc(6))
+      return -1;
+    f->graphValid = false;
+  }
+  if (c.travStats.p)
+    GVH_CHECK(hipMemset(c.travStats.p, 0, 6 * sizeof(unsigned long long)));
   hipEvent_t s0, s1;
   GVH_CHECK(hipEventCreate(&s0));
   GVH_CHECK(hipEventCreate(&s1));
@@ -763,6 +784,14 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
     if (!gvh::last_error()[0])
       set_error("gvizhip_force_profile failed");
     return -1;
+  }
+  if (c.travStats.p) {
+    unsigned long long t[6];
+    if (hipMemcpy(t, c.travStats.p, sizeof(t), hipMemcpyDeviceToHost) == hipSuccess)
+      fprintf(stderr,
+              "[gvizhip] traversal stats: steps %llu, deferred %llu (mask lanes %llu),"
+              " undecided %llu (mask lanes %llu), max stack %llu\n",
+              t[0], t[1], t[2], t[3], t[4], t[5]);
   }
   if (traversalMs)
     *traversalMs = trav;

@@ -101,9 +101,13 @@ struct ForceCommon {
   int64_t recordCap = 0;
   DevBuf<int32_t> interactions; // optional per-vertex repulsion count
   int wantInteractions = 0;
+  DevBuf<int32_t> heavyRows;            // vertices with degree > kHeavyDegree
+  int64_t nHeavy = 0;
+  DevBuf<unsigned char> attHeavy;       // their attraction (typed, n entries)
   // measurement hook: events around the traversal kernel (never captured)
   hipEvent_t travBegin = nullptr, travEnd = nullptr;
   int timeTraversal = 0;
+  DevBuf<unsigned long long> travStats; // null unless GVIZHIP_TRAVERSAL_STATS
   int64_t stepsSinceRecordBase = 0;
   int reductionBlocks = 0;
 

@@ -11,6 +11,8 @@
 #include "dd.hpp"
 #include "force_engine.hpp"
 
+#include <algorithm>
+
 namespace gvh {
 
 template <class T> struct V2;
@@ -20,6 +22,7 @@ template <> struct V2<double> { using type = double2; };
 constexpr int kTB = 256;            // threads per block
 constexpr int kTW = kTB / kWave;    // waves per block
 constexpr int kStack = 256;         // traversal stack entries per wave
+constexpr int kHeavyDegree = 32;    // rows above this get a wavefront each
 
 // Direction for exactly coincident points: splitmix64 finaliser over the two
 // vertex ids mapped to an angle. The reference hashes buffer addresses here
@@ -93,6 +96,18 @@ template <> struct Phys<double> {
     ax += s * dx;
     ay += s * dy;
   }
+  template <int MODEL>
+  __device__ __forceinline__ static void att_term(double dx, double dy, double k,
+                                                  double &cx, double &cy) {
+    cx = 0.0;
+    cy = 0.0;
+    double dist = sqrt(dx * dx + dy * dy);
+    if (dist < 1e-9)
+      return;
+    double s = MODEL == 1 ? log(1.0 + dist) / dist : dist / k;
+    cx = s * dx;
+    cy = s * dy;
+  }
   __device__ __forceinline__ static void gravity(double x, double y, double mag,
                                                  double &fx, double &fy) {
     double dist = sqrt(x * x + y * y);
@@ -161,6 +176,20 @@ template <> struct Phys<float> {
                          : dist * __builtin_amdgcn_rcpf(k);
     ax += s * dx;
     ay += s * dy;
+  }
+  template <int MODEL>
+  __device__ __forceinline__ static void att_term(float dx, float dy, float k,
+                                                  float &cx, float &cy) {
+    cx = 0.0f;
+    cy = 0.0f;
+    float d2 = dx * dx + dy * dy;
+    float dist = d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+    if (dist < 1e-9f)
+      return;
+    float s = MODEL == 1 ? log1pf(dist) * __builtin_amdgcn_rcpf(dist)
+                         : dist * __builtin_amdgcn_rcpf(k);
+    cx = s * dx;
+    cy = s * dy;
   }
   __device__ __forceinline__ static void gravity(float x, float y, float mag,
                                                  float &fx, float &fy) {
@@ -276,111 +305,294 @@ __global__ __launch_bounds__(kTB) void k_quad(
   nodes[b] = r;
 }
 
+// ---- lane broadcasts (v_readlane with a wave-uniform lane index)
+__device__ __forceinline__ int32_t rl(int32_t v, int l) {
+  return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ float rl(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint64_t rl(uint64_t v, int l) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double rl(double v, int l) {
+  return __longlong_as_double((long long)rl((uint64_t)__double_as_longlong(v), l));
+}
+
+constexpr int kTravThreads = 128;               // two wavefronts per block
+constexpr int kTravWaves = kTravThreads / kWave;
+constexpr int kTravStack = 768;                 // pending (node, mask) per wave
+constexpr int kTravDefer = 128;                 // deferred bodies per wave
+
 // Barnes-Hut repulsion, one wavefront per 64 consecutive targets in key
-// order. The wavefront walks the union of its lanes' trees with one shared
-// stack of (node, lane mask) entries: every lane takes its own opening
-// decision (the reference's per-vertex recursion, src/embedders/
-// gvizForceEmbedder.c:35-74), node records are fetched once per wavefront,
-// and children are popped in NW, NE, SW, SE order so each lane accumulates
-// in the reference's depth-first order.
+// order. Every lane ends up with exactly the interaction set of the
+// reference's per-vertex recursion (src/embedders/gvizForceEmbedder.c:35-74);
+// the wavefront walks the union of those sets with a shared stack of
+// (node, lane mask) entries, up to 64 entries per step, one per lane:
+//   * each lane fetches one entry's record and classifies it against the
+//     bounding box of the wavefront's targets: a cell that every target would
+//     accept (or open), with a relative margin far above the test's rounding,
+//     is accepted (or opened) for all lanes of its mask at once;
+//   * accepted cells and leaf points are deferred to a list that is
+//     evaluated densely: 64 bodies are fetched in parallel, then broadcast
+//     one by one to all lanes;
+//   * cells the box cannot decide are tested by each lane, as the reference
+//     does, and opened for the lanes that reject them.
 template <class Real, class PosT, int MODEL>
-__global__ __launch_bounds__(kTB) void k_traverse(
+__global__ __launch_bounds__(kTravThreads) void k_traverse(
     const NodeRec<Real, PosT> *__restrict__ nodes,
     const typename V2<PosT>::type *__restrict__ spos,
     const typename V2<Real>::type *__restrict__ smr,
     const uint32_t *__restrict__ sidx, int64_t n, Real theta, Real kSq, Real oc,
     typename V2<Real>::type *__restrict__ rep, int32_t *__restrict__ inter,
-    const double *__restrict__ r64, Scalars *__restrict__ sc) {
+    const double *__restrict__ r64, Scalars *__restrict__ sc,
+    unsigned long long *__restrict__ tstats) {
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
-  __shared__ int32_t sNode[kTW][kStack];
-  __shared__ uint64_t sMask[kTW][kStack];
+  __shared__ int32_t sNode[kTravWaves][kTravStack];
+  __shared__ uint64_t sMask[kTravWaves][kTravStack];
+  __shared__ int32_t dRef[kTravWaves][kTravDefer];
+  __shared__ uint64_t dMask[kTravWaves][kTravDefer];
   if (sc->stopped)
     return;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const int64_t s = ((int64_t)blockIdx.x * kTW + wave) * kWave + lane;
+  const int64_t s = ((int64_t)blockIdx.x * kTravWaves + wave) * kWave + lane;
   const bool valid = s < n;
   const uint64_t live = __ballot(valid);
   if (live == 0)
     return;
+  const uint64_t ltMask = lane ? (~0ull >> (kWave - lane)) : 0ull;
   typename V2<PosT>::type me = valid ? spos[s] : typename V2<PosT>::type{0, 0};
   typename V2<Real>::type mr = valid ? smr[s] : typename V2<Real>::type{0, 0};
   const int64_t selfId = valid ? (int64_t)sidx[s] : 0;
+  const double bx0 = wave_min(valid ? (double)me.x : INFINITY);
+  const double bx1 = wave_max(valid ? (double)me.x : -INFINITY);
+  const double by0 = wave_min(valid ? (double)me.y : INFINITY);
+  const double by1 = wave_max(valid ? (double)me.y : -INFINITY);
+  const double margin = sizeof(Real) == sizeof(double) ? 1e-12 : 1e-5;
+  const double thLo = (double)theta * (1.0 - margin);
+  const double thHi = (double)theta * (1.0 + margin);
   Real ax = 0, ay = 0;
   int32_t cnt = 0;
-  int sp = 0;
-  sNode[wave][0] = n >= 2 ? 0 : -1;
-  sMask[wave][0] = live;
-  sp = 1;
-  while (sp > 0) {
-    sp--;
-    const int32_t node = __builtin_amdgcn_readfirstlane(sNode[wave][sp]);
-    const uint64_t mask = sMask[wave][sp];
-    const bool in = (mask >> lane) & 1ull;
-    if (node < 0) {
-      const int32_t p = -node - 1;
-      if (in && p != s) {
-        typename V2<PosT>::type o = spos[p];
-        typename V2<Real>::type om = smr[p];
-        const PosT ex = o.x - me.x, ey = o.y - me.y;
-        Real dx = (Real)ex, dy = (Real)ey;
-        Real dist = Phys<Real>::dist(dx, dy);
-        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + om.y, mr.x,
-                                              om.x, kSq, oc, selfId,
-                                              (int64_t)sidx[p], ax, ay, ex, ey,
-                                              r64, n);
-        cnt++;
-      }
-      continue;
-    }
-    const NodeRec<Real, PosT> r = nodes[node];
-    if (r.count > 0) {
-      for (int32_t k = 0; k < r.count; k++) {
-        const int32_t p = r.first + k;
-        if (in && p != s) {
+  int nd = 0;
+  // optional counters (GVIZHIP_TRAVERSAL_STATS): steps, deferred bodies,
+  // their mask lanes, undecided cells, their mask lanes, max stack depth
+  unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+
+  // Evaluates deferred bodies: lane j fetches body j, then all lanes take
+  // each body in turn.
+  auto flush = [&]() {
+    for (int base = 0; base < nd; base += kWave) {
+      const int j = base + lane;
+      const bool has = j < nd;
+      const int32_t ref = has ? dRef[wave][j] : -1;
+      const uint64_t msk = has ? dMask[wave][j] : 0ull;
+      PosT bx = 0, by = 0;
+      Real bm = 0, br = 0;
+      int32_t bid = (int32_t)n, bpt = -1;
+      if (has) {
+        if (ref < 0) {
+          const int32_t p = -ref - 1;
           typename V2<PosT>::type o = spos[p];
           typename V2<Real>::type om = smr[p];
-          const PosT ex = o.x - me.x, ey = o.y - me.y;
+          bx = o.x;
+          by = o.y;
+          bm = om.x;
+          br = om.y;
+          bid = (int32_t)sidx[p];
+          bpt = p;
+        } else {
+          const NodeRec<Real, PosT> &r = nodes[ref];
+          bx = r.comx;
+          by = r.comy;
+          bm = r.mass;
+        }
+      }
+      const int len = nd - base < kWave ? nd - base : kWave;
+      if (tstats)
+        st[1] += len;
+      for (int t = 0; t < len; t++) {
+        const uint64_t m = rl(msk, t);
+        if (tstats)
+          st[2] += __popcll(m);
+        const int32_t pt = rl(bpt, t);
+        if (((m >> lane) & 1ull) && pt != s) {
+          const PosT ex = rl(bx, t) - me.x, ey = rl(by, t) - me.y;
           Real dx = (Real)ex, dy = (Real)ey;
           Real dist = Phys<Real>::dist(dx, dy);
-          Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + om.y, mr.x,
-                                                om.x, kSq, oc, selfId,
-                                                (int64_t)sidx[p], ax, ay, ex, ey,
+          Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + rl(br, t),
+                                                mr.x, rl(bm, t), kSq, oc, selfId,
+                                                (int64_t)rl(bid, t), ax, ay, ex, ey,
                                                 r64, n);
           cnt++;
         }
       }
-      continue;
     }
-    bool open = false;
-    if (in) {
-      const PosT ex = r.comx - me.x, ey = r.comy - me.y;
-      Real dx = (Real)ex, dy = (Real)ey;
-      Real dist = Phys<Real>::dist(dx, dy);
-      if (Phys<Real>::accept(r.twoh, dist, theta)) {
-        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + (Real)0, mr.x,
-                                              r.mass, kSq, oc, selfId, n, ax, ay,
-                                              ex, ey, r64, n);
-        cnt++;
+    nd = 0;
+  };
+
+  int sp = 1;
+  if (lane == 0) {
+    sNode[wave][0] = n >= 2 ? 0 : -1;
+    sMask[wave][0] = live;
+  }
+  enum { kDefer = 0, kOpen = 1, kAmbig = 2, kRange = 3 };
+  while (sp > 0) {
+    if (tstats) {
+      st[0]++;
+      st[5] = st[5] > (unsigned long long)sp ? st[5] : (unsigned long long)sp;
+    }
+    // entries the stack can take children for: each open entry adds <= 3
+    int take = sp < kWave ? sp : kWave;
+    const int room = (kTravStack - sp) / 3;
+    if (take > room)
+      take = room > 0 ? room : 1;
+    const int j = sp - 1 - lane; // lane 0 takes the top entry
+    const bool has = lane < take;
+    const int32_t node = has ? sNode[wave][j] : 0;
+    const uint64_t msk = has ? sMask[wave][j] : 0ull;
+    sp -= take;
+    int kind = kDefer;
+    NodeRec<Real, PosT> r;
+    r.child[0] = r.child[1] = r.child[2] = r.child[3] = kNone;
+    r.count = 0;
+    r.first = 0;
+    r.comx = r.comy = (PosT)0;
+    r.mass = r.twoh = (Real)0;
+    if (has && node >= 0) {
+      r = nodes[node];
+      if (r.count > 0) {
+        kind = kRange;
       } else {
-        open = true;
+        const double cx = (double)r.comx, cy = (double)r.comy;
+        const double gx = fmax(fmax(bx0 - cx, cx - bx1), 0.0);
+        const double gy = fmax(fmax(by0 - cy, cy - by1), 0.0);
+        const double fx = fmax(fabs(cx - bx0), fabs(cx - bx1));
+        const double fy = fmax(fabs(cy - by0), fabs(cy - by1));
+        const double twoh = (double)r.twoh;
+        if (twoh < thLo * sqrt(gx * gx + gy * gy))
+          kind = kDefer;
+        else if (twoh >= thHi * sqrt(fx * fx + fy * fy))
+          kind = kOpen;
+        else
+          kind = kAmbig;
       }
     }
-    const uint64_t om = __ballot(open);
-    if (om) {
-      if (sp + 4 > kStack) {
+    // deferred points and cells
+    {
+      const uint64_t dm = __ballot(has && kind == kDefer);
+      if (nd + (int)__popcll(dm) > kTravDefer)
+        flush();
+      if (has && kind == kDefer) {
+        const int at = nd + (int)__popcll(dm & ltMask);
+        dRef[wave][at] = node;
+        dMask[wave][at] = msk;
+      }
+      nd += (int)__popcll(dm);
+    }
+    // leaves holding several points (nodesPerCell > 1 or the depth limit)
+    uint64_t rm = __ballot(has && kind == kRange);
+    while (rm) {
+      const int l = __builtin_ctzll(rm);
+      rm &= rm - 1;
+      const int32_t first = rl(r.first, l), count = rl(r.count, l);
+      const uint64_t m = rl(msk, l);
+      for (int32_t k = 0; k < count; k++) {
+        if (nd == kTravDefer)
+          flush();
+        if (lane == 0) {
+          dRef[wave][nd] = -(first + k) - 1;
+          dMask[wave][nd] = m;
+        }
+        nd++;
+      }
+    }
+    // children of cells every target opens
+    {
+      const bool op = has && kind == kOpen;
+      int kids = 0;
+      if (op)
+        kids = (r.child[0] != kNone) + (r.child[1] != kNone) +
+               (r.child[2] != kNone) + (r.child[3] != kNone);
+      int incl = kids;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        int v = __shfl_up(incl, off, kWave);
+        if (lane >= off)
+          incl += v;
+      }
+      const int total = __shfl(incl, kWave - 1, kWave);
+      if (sp + total > kTravStack) {
         sc->traversalOverflow = 1;
         break;
       }
+      if (op) {
+        // children go on in reverse so NW is popped first
+        int at = sp + incl - 1;
 #pragma unroll
-      for (int c = 3; c >= 0; c--) {
-        if (r.child[c] != kNone) {
-          sNode[wave][sp] = r.child[c];
-          sMask[wave][sp] = om;
-          sp++;
+        for (int c = 0; c < 4; c++)
+          if (r.child[c] != kNone) {
+            sNode[wave][at] = r.child[c];
+            sMask[wave][at] = msk;
+            at--;
+          }
+      }
+      sp += total;
+    }
+    // cells the box cannot decide: every lane of the mask tests them
+    uint64_t am = __ballot(has && kind == kAmbig);
+    while (am) {
+      const int l = __builtin_ctzll(am);
+      am &= am - 1;
+      const uint64_t m = rl(msk, l);
+      if (tstats) {
+        st[3]++;
+        st[4] += __popcll(m);
+      }
+      const PosT cx = rl(r.comx, l), cy = rl(r.comy, l);
+      const Real twoh = rl(r.twoh, l), mass = rl(r.mass, l);
+      bool open = false;
+      if ((m >> lane) & 1ull) {
+        const PosT ex = cx - me.x, ey = cy - me.y;
+        Real dx = (Real)ex, dy = (Real)ey;
+        Real dist = Phys<Real>::dist(dx, dy);
+        if (Phys<Real>::accept(twoh, dist, theta)) {
+          Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + (Real)0, mr.x,
+                                                mass, kSq, oc, selfId, n, ax, ay,
+                                                ex, ey, r64, n);
+          cnt++;
+        } else {
+          open = true;
+        }
+      }
+      const uint64_t om = __ballot(open);
+      if (om) {
+        if (sp + 4 > kTravStack) {
+          sc->traversalOverflow = 1;
+          sp = 0;
+          am = 0;
+          break;
+        }
+        for (int c = 3; c >= 0; c--) {
+          const int32_t ch = rl(r.child[c], l);
+          if (ch != kNone) {
+            if (lane == 0) {
+              sNode[wave][sp] = ch;
+              sMask[wave][sp] = om;
+            }
+            sp++;
+          }
         }
       }
     }
+  }
+  flush();
+  if (tstats && lane == 0) {
+    for (int k = 0; k < 5; k++)
+      atomicAdd(&tstats[k], st[k]);
+    atomicMax(&tstats[5], st[5]);
   }
   if (valid) {
     rep[selfId] = {ax, ay};
@@ -441,6 +653,54 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
   }
 }
 
+// Attraction of high-degree vertices: one wavefront per row computes 64 edge
+// terms at a time and adds them to a running sum in adjacency order, so the
+// sum rounds exactly as the reference's sequential loop does.
+template <class Real, class PosT, int MODEL>
+__global__ __launch_bounds__(kTB) void k_att_heavy(
+    const typename V2<PosT>::type *__restrict__ pos,
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const int32_t *__restrict__ rows, int64_t nrows, Real edgeLength,
+    typename V2<Real>::type *__restrict__ out, const Scalars *__restrict__ sc) {
+  if (sc->stopped)
+    return;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t w = (int64_t)blockIdx.x * kTW + threadIdx.x / kWave; w < nrows;
+       w += (int64_t)gridDim.x * kTW) {
+    const int32_t v = rows[w];
+    const typename V2<PosT>::type pv = pos[v];
+    const int32_t e0 = off[v], e1 = off[v + 1];
+    Real ax = 0, ay = 0;
+    for (int32_t base = e0; base < e1; base += kWave) {
+      const int32_t e = base + lane;
+      Real cx = 0, cy = 0;
+      if (e < e1) {
+        typename V2<PosT>::type u = pos[adj[e]];
+        Phys<Real>::template att_term<MODEL>((Real)(u.x - pv.x), (Real)(u.y - pv.y),
+                                             edgeLength, cx, cy);
+      }
+      if (sizeof(Real) == sizeof(double)) {
+        // FP64: the reference's left-to-right sum
+        const int cnt = e1 - base < kWave ? e1 - base : kWave;
+        for (int l = 0; l < cnt; l++) {
+          ax += __shfl(cx, l, kWave);
+          ay += __shfl(cy, l, kWave);
+        }
+      } else {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          cx += __shfl_xor(cx, off, kWave);
+          cy += __shfl_xor(cy, off, kWave);
+        }
+        ax += cx;
+        ay += cy;
+      }
+    }
+    if (lane == 0)
+      out[v] = {ax, ay};
+  }
+}
+
 // Attraction along the adjacency in insertion order, then the per-vertex
 // epilogue of src/embedders/gvizForceEmbedder.c:117-124 and the swinging /
 // traction of :128-138; per-block FP64 partial sums of both go to partial.
@@ -453,7 +713,8 @@ __global__ __launch_bounds__(kTB) void k_forces(
     double gravityK, typename V2<Real>::type *__restrict__ sf,
     typename V2<Real>::type *__restrict__ disp, Real *__restrict__ swOut,
     Real *__restrict__ attMagOut, Real *__restrict__ repMagOut,
-    double *__restrict__ partial, const Scalars *__restrict__ sc) {
+    double *__restrict__ partial, const typename V2<Real>::type *__restrict__ heavy,
+    const Scalars *__restrict__ sc) {
   __shared__ double smem[2 * 16];
   if (sc->stopped)
     return;
@@ -462,11 +723,17 @@ __global__ __launch_bounds__(kTB) void k_forces(
        i += (int64_t)gridDim.x * kTB) {
     typename V2<PosT>::type v = pos[i];
     Real ax = 0, ay = 0;
-    const int32_t e1 = off[i + 1];
-    for (int32_t e = off[i]; e < e1; e++) {
-      typename V2<PosT>::type u = pos[adj[e]];
-      Phys<Real>::template att<MODEL>((Real)(u.x - v.x), (Real)(u.y - v.y),
-                                      edgeLength, ax, ay);
+    const int32_t e0 = off[i], e1 = off[i + 1];
+    if (e1 - e0 > kHeavyDegree) {
+      typename V2<Real>::type h = heavy[i];
+      ax = h.x;
+      ay = h.y;
+    } else {
+      for (int32_t e = e0; e < e1; e++) {
+        typename V2<PosT>::type u = pos[adj[e]];
+        Phys<Real>::template att<MODEL>((Real)(u.x - v.x), (Real)(u.y - v.y),
+                                        edgeLength, ax, ay);
+      }
     }
     typename V2<Real>::type rp = rep[i];
     Real fx = (Real)0, fy = (Real)0;
@@ -597,21 +864,21 @@ int typed_step(ForceCommon &c, int mode) {
       GVH_CHECK_LAUNCH();
       return 0;
     }
-    const unsigned tb = (unsigned)((n + kTB - 1) / kTB);
+    const unsigned tb = (unsigned)((n + kTravThreads - 1) / kTravThreads);
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
     if (model == 1)
-      hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
+      hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTravThreads), 0, st,
                          (const NodeRec<Real, PosT> *)c.nodes.p,
                          (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
                          theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
-                         c.sc.p);
+                         c.sc.p, c.travStats.p);
     else
-      hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTB), 0, st,
+      hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTravThreads), 0, st,
                          (const NodeRec<Real, PosT> *)c.nodes.p,
                          (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
                          theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
-                         c.sc.p);
+                         c.sc.p, c.travStats.p);
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
@@ -632,19 +899,35 @@ int typed_step(ForceCommon &c, int mode) {
                           ? grid_for(n, kTB)
                           : (unsigned)c.reductionBlocks;
   if (model == 1)
+  {
+    if (c.nHeavy > 0)
+      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 1>),
+                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
+                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
+                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
     hipLaunchKernelGGL((k_forces<Real, PosT, 1>), dim3(rb), dim3(kTB), 0, st,
                        posC, c.csrOff.p, c.csrAdj.p, c.degree.p,
                        (const Real *)c.massT.p, (const R2 *)c.rep.p, n,
                        (Real)c.prm.edgeLength, c.prm.gravityK, (R2 *)c.sf.p,
                        (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
-                       (Real *)c.repMag.p, c.partials.p, c.sc.p);
+                       (Real *)c.repMag.p, c.partials.p, (const R2 *)c.attHeavy.p,
+                       c.sc.p);
+  }
   else
+  {
+    if (c.nHeavy > 0)
+      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 0>),
+                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
+                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
+                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
     hipLaunchKernelGGL((k_forces<Real, PosT, 0>), dim3(rb), dim3(kTB), 0, st,
                        posC, c.csrOff.p, c.csrAdj.p, c.degree.p,
                        (const Real *)c.massT.p, (const R2 *)c.rep.p, n,
                        (Real)c.prm.edgeLength, c.prm.gravityK, (R2 *)c.sf.p,
                        (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
-                       (Real *)c.repMag.p, c.partials.p, c.sc.p);
+                       (Real *)c.repMag.p, c.partials.p, (const R2 *)c.attHeavy.p,
+                       c.sc.p);
+  }
   GVH_CHECK_LAUNCH();
   if (mode == kStepForcesOnly)
     return 0;

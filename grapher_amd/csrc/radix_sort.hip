@@ -28,37 +28,50 @@ __global__ __launch_bounds__(T) void k_hist(const uint64_t *__restrict__ keys,
   table[(int64_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of table[0..m) by one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t *__restrict__ table,
-                                               int64_t m) {
-  __shared__ uint32_t sums[1024];
-  const int64_t chunk = (m + 1023) / 1024;
-  const int64_t b = threadIdx.x * chunk;
-  const int64_t e = b + chunk < m ? b + chunk : m;
-  uint32_t s = 0;
-  for (int64_t i = b; i < e; i++)
-    s += table[i];
-  sums[threadIdx.x] = s;
+// Exclusive scan of one digit's row of per-tile counts (one block per
+// digit), in place; the row total goes to totals[digit].
+constexpr int kRowMax = 8192; // tiles per sort: up to 32M keys
+__global__ __launch_bounds__(T) void k_rowscan(uint32_t *__restrict__ table,
+                                               int tiles,
+                                               uint32_t *__restrict__ totals) {
+  __shared__ uint32_t row[kRowMax];
+  __shared__ uint32_t part[T];
+  uint32_t *r = table + (int64_t)blockIdx.x * tiles;
+  for (int i = threadIdx.x; i < tiles; i += T)
+    row[i] = r[i];
   __syncthreads();
-  // Hillis-Steele inclusive scan over 1024 partial sums.
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = threadIdx.x >= (unsigned)off ? sums[threadIdx.x - off] : 0u;
+  const int chunk = (tiles + T - 1) / T;
+  const int b = threadIdx.x * chunk;
+  const int e = b + chunk < tiles ? b + chunk : tiles;
+  uint32_t sum = 0;
+  for (int i = b; i < e; i++)
+    sum += row[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
     __syncthreads();
-    sums[threadIdx.x] += v;
+    part[threadIdx.x] += v;
     __syncthreads();
   }
-  uint32_t run = threadIdx.x ? sums[threadIdx.x - 1] : 0u;
-  for (int64_t i = b; i < e; i++) {
-    uint32_t v = table[i];
-    table[i] = run;
+  uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+  for (int i = b; i < e; i++) {
+    uint32_t v = row[i];
+    row[i] = run;
     run += v;
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tiles; i += T)
+    r[i] = row[i];
+  if (threadIdx.x == T - 1)
+    totals[blockIdx.x] = part[T - 1];
 }
 
 __global__ __launch_bounds__(T) void k_scatter(
     const uint64_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
     uint64_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut, int64_t n,
-    int shift, int tiles, const uint32_t *__restrict__ table) {
+    int shift, int tiles, const uint32_t *__restrict__ table,
+    const uint32_t *__restrict__ totals) {
   __shared__ uint64_t sKey[TILE];
   __shared__ uint32_t sVal[TILE];
   __shared__ uint32_t gOff[256];
@@ -73,13 +86,34 @@ __global__ __launch_bounds__(T) void k_scatter(
   const int tileLen = (int)((n - base) < TILE ? (n - base) : TILE);
 
   gOff[tid] = table[(int64_t)tid * tiles + blockIdx.x];
-  running[tid] = 0;
+  running[tid] = totals[tid];
   tStart[tid] = 0;
 #pragma unroll
   for (int w = 0; w < WAVES; w++)
     waveCnt[w][tid] = 0;
   __syncthreads();
 
+  // Digit bases: exclusive scan of the 256 digit totals (wave 0).
+  if (tid < kWave) {
+    uint32_t c0 = running[4 * tid], c1 = running[4 * tid + 1];
+    uint32_t c2 = running[4 * tid + 2], c3 = running[4 * tid + 3];
+    uint32_t s4 = c0 + c1 + c2 + c3;
+    uint32_t incl = s4;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      uint32_t v = __shfl_up(incl, off, kWave);
+      if (lane >= off)
+        incl += v;
+    }
+    uint32_t ex = incl - s4;
+    gOff[4 * tid] += ex;
+    gOff[4 * tid + 1] += ex + c0;
+    gOff[4 * tid + 2] += ex + c0 + c1;
+    gOff[4 * tid + 3] += ex + c0 + c1 + c2;
+  }
+  __syncthreads();
+  running[tid] = 0;
+  __syncthreads();
   // Tile histogram -> exclusive digit starts inside the tile.
   for (int r = 0; r < I; r++) {
     int li = r * T + tid;
@@ -166,7 +200,13 @@ int RadixSorter::reserve(int64_t n) {
   if (n <= capacity)
     return 0;
   int64_t tiles = (n + kTile - 1) / kTile;
-  if (keysAlt.alloc(n) || valsAlt.alloc(n) || table.alloc(256 * tiles))
+  if (tiles > kRowMax) {
+    set_error("radix sort: %lld keys exceed the %d-tile limit", (long long)n,
+              kRowMax);
+    return -1;
+  }
+  if (keysAlt.alloc(n) || valsAlt.alloc(n) || table.alloc(256 * tiles) ||
+      totals.alloc(256))
     return -1;
   capacity = n;
   return 0;
@@ -190,10 +230,10 @@ int RadixSorter::sort(uint64_t *keys, uint32_t *vals, int64_t n, int beginBit,
   for (int shift = beginBit; shift < endBit; shift += 8, passes++) {
     hipLaunchKernelGGL(k_hist, dim3(tiles), dim3(kThreads), 0, stream, kin, n,
                        shift, tiles, table.p);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, table.p,
-                       (int64_t)256 * tiles);
+    hipLaunchKernelGGL(k_rowscan, dim3(256), dim3(kThreads), 0, stream, table.p,
+                       tiles, totals.p);
     hipLaunchKernelGGL(k_scatter, dim3(tiles), dim3(kThreads), 0, stream, kin,
-                       vin, kout, vout, n, shift, tiles, table.p);
+                       vin, kout, vout, n, shift, tiles, table.p, totals.p);
     GVH_CHECK_LAUNCH();
     uint64_t *tk = kin;
     kin = kout;

@@ -1,8 +1,8 @@
 // radix_sort.hpp -- stable LSD radix sort of (uint64 key, uint32 value)
 // pairs for gfx950, 8-bit digits.
 //
-// Per pass: (1) per-tile 256-bin digit histograms, (2) one-block exclusive
-// scan over the digit-major [256 x tiles] table, (3) a stable scatter: each
+// Per pass: (1) per-tile 256-bin digit histograms, (2) one block per digit
+// scans that digit's row of the [256 x tiles] table, (3) a stable scatter: each
 // tile ranks its keys with wave ballots (stable: tile order = key order),
 // stages them digit-sorted in LDS and writes each digit's run contiguously.
 #pragma once
@@ -18,6 +18,7 @@ struct RadixSorter {
   DevBuf<uint64_t> keysAlt;
   DevBuf<uint32_t> valsAlt;
   DevBuf<uint32_t> table; // 256 * tiles
+  DevBuf<uint32_t> totals; // 256
   int64_t capacity = 0;
 
   int reserve(int64_t n);
