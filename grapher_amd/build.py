@@ -78,13 +78,28 @@ def build(verbose: bool = False) -> None:
 
     hs = host_sources()
     if hs:
-        cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall",
+        cmd = ["gcc", "-O2", "-std=gnu11", "-ffp-contract=off", "-fPIC", "-shared", "-Wall",
                "-Wno-unused-function", "-pthread",
                "-I", os.path.join(ROOT, "include"),
                "-o", HOST_LIB + ".tmp"] + [os.path.join(HOST, s) for s in hs] + [
                    "-L", HERE, "-lgvizhip", "-Wl,-rpath,$ORIGIN", "-lm"]
         _run(cmd)
         os.replace(HOST_LIB + ".tmp", HOST_LIB)
+        build_test_driver()
+
+
+TEST_DRIVER_SRC = os.path.join(ROOT, "tests", "host", "api_driver.c")
+TEST_DRIVER = os.path.join(ROOT, "tests", "host", "api_driver")
+
+
+def build_test_driver() -> None:
+    """The tests' C driver of the host layer (tests/host/api_driver.c)."""
+    if not os.path.exists(TEST_DRIVER_SRC):
+        return
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", os.path.join(ROOT, "include"),
+          "-o", TEST_DRIVER + ".tmp", TEST_DRIVER_SRC, "-L", HERE, "-lgviz",
+          "-lgvizhip", "-Wl,-rpath,$ORIGIN/../../grapher_amd", "-lm"])
+    os.replace(TEST_DRIVER + ".tmp", TEST_DRIVER)
 
 
 if __name__ == "__main__":

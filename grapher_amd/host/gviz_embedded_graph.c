@@ -1,0 +1,328 @@
+/*
+ * gviz_embedded_graph.c -- positions, action and stat registries, draw mask
+ * and text checkpoints shared by every embedder of the host layer
+ * (reference: src/embedders/gvizEmbeddedGraph.c).
+ */
+#include "gviz/gvizEmbeddedGraph.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+int gvizEmbeddedGraphInit(gvizEmbeddedGraph *e, gvizSubgraph subgraph,
+                          size_t dimension) {
+  if (!e || !subgraph.g || dimension == 0)
+    return -1;
+  memset(e, 0, sizeof(*e));
+  e->subgraph = subgraph;
+  e->embedding.dim = dimension;
+  size_t n = gvizGraphSize(subgraph.g);
+  e->embedding.vertexPositions = calloc((n ? n : 1) * dimension, sizeof(double));
+  e->visibleVertices = calloc(n ? n : 1, 1);
+  if (!e->embedding.vertexPositions || !e->visibleVertices)
+    return -1;
+  for (size_t u = 0; u < n; u++)
+    e->visibleVertices[u] = gvizSubgraphHasVertex(&subgraph, u) ? 1 : 0;
+  e->edgePolicy = GVIZ_DRAW_EDGES_ALL;
+  return 0;
+}
+
+void gvizEmbeddedGraphRelease(gvizEmbeddedGraph *e) {
+  if (!e)
+    return;
+  free(e->embedding.vertexPositions);
+  free(e->actions);
+  for (size_t i = 0; i < e->statCount; i++)
+    free(e->stats[i].samples);
+  free(e->stats);
+  free(e->visibleVertices);
+  gvizSubgraphRelease(&e->subgraph);
+  e->embedding.vertexPositions = NULL;
+  e->actions = NULL;
+  e->stats = NULL;
+  e->visibleVertices = NULL;
+  e->actionCount = e->statCount = 0;
+}
+
+size_t gvizEmbeddedGraphDim(const gvizEmbeddedGraph *e) { return e->embedding.dim; }
+size_t gvizEmbeddedGraphPositionCount(const gvizEmbeddedGraph *e) {
+  return gvizGraphSize(e->subgraph.g);
+}
+const double *gvizEmbeddedGraphPositions(const gvizEmbeddedGraph *e) {
+  return e->embedding.vertexPositions;
+}
+double *gvizEmbeddedGraphGetVPosition(gvizEmbeddedGraph *e, size_t u) {
+  return e->embedding.vertexPositions + u * e->embedding.dim;
+}
+void gvizEmbeddedGraphSetVPosition(gvizEmbeddedGraph *e, size_t u,
+                                   const double *pos) {
+  memcpy(e->embedding.vertexPositions + u * e->embedding.dim, pos,
+         sizeof(double) * e->embedding.dim);
+}
+void gvizEmbeddedGraphAddVPosition(gvizEmbeddedGraph *e, size_t u,
+                                   const double *delta) {
+  double *p = e->embedding.vertexPositions + u * e->embedding.dim;
+  for (size_t k = 0; k < e->embedding.dim; k++)
+    p[k] += delta[k];
+}
+
+/* src/embedders/gvizEmbeddedGraph.c:406-426 */
+void gvizEmbeddedGraphRandomizePositions(gvizEmbeddedGraph *e, double boxExtent,
+                                         unsigned int seed) {
+  if (!e)
+    return;
+  if (seed == 0)
+    seed = (unsigned int)time(NULL);
+  size_t dim = e->embedding.dim, n = gvizGraphSize(e->subgraph.g);
+  for (size_t u = 0; u < n; u++) {
+    if (!gvizSubgraphHasVertex(&e->subgraph, u))
+      continue;
+    double *p = gvizEmbeddedGraphGetVPosition(e, u);
+    for (size_t d = 0; d < dim; d++) {
+      double unit = (double)rand_r(&seed) / ((double)RAND_MAX + 1.0);
+      p[d] = boxExtent * (2.0 * unit - 1.0);
+    }
+  }
+}
+
+static gvizAction *find_action(const gvizEmbeddedGraph *e, const char *name) {
+  for (size_t i = 0; i < e->actionCount; i++)
+    if (strcmp(e->actions[i].name, name) == 0)
+      return &e->actions[i];
+  return NULL;
+}
+
+int gvizEmbeddedGraphAddAction(gvizEmbeddedGraph *e, const char *name,
+                               gvizActionHandler handler, void *userData) {
+  if (!e || !name || !handler)
+    return -1;
+  gvizAction *existing = find_action(e, name);
+  if (existing) {
+    existing->handler = handler;
+    existing->userData = userData;
+    return 0;
+  }
+  if (e->actionCount == e->actionCapacity) {
+    size_t cap = e->actionCapacity ? e->actionCapacity * 2 : 4;
+    gvizAction *grown = realloc(e->actions, cap * sizeof(gvizAction));
+    if (!grown)
+      return -1;
+    e->actions = grown;
+    e->actionCapacity = cap;
+  }
+  e->actions[e->actionCount++] = (gvizAction){name, handler, userData};
+  return 0;
+}
+
+int gvizEmbeddedGraphRemoveAction(gvizEmbeddedGraph *e, const char *name) {
+  if (!e || !name)
+    return -1;
+  gvizAction *a = find_action(e, name);
+  if (!a)
+    return -1;
+  *a = e->actions[--e->actionCount];
+  return 0;
+}
+
+const gvizAction *gvizEmbeddedGraphFindAction(const gvizEmbeddedGraph *e,
+                                              const char *name) {
+  if (!e || !name)
+    return NULL;
+  return find_action(e, name);
+}
+
+size_t gvizEmbeddedGraphActionCount(const gvizEmbeddedGraph *e) {
+  return e->actionCount;
+}
+
+const gvizAction *gvizEmbeddedGraphActionAt(const gvizEmbeddedGraph *e,
+                                            size_t idx) {
+  return e && idx < e->actionCount ? &e->actions[idx] : NULL;
+}
+
+int gvizEmbeddedGraphInvokeAction(gvizEmbeddedGraph *e, const char *name,
+                                  const gvizActionPayload *payload) {
+  const gvizAction *a = gvizEmbeddedGraphFindAction(e, name);
+  if (!a)
+    return -1;
+  gvizActionPayload zero = {0};
+  a->handler(e, a->userData, payload ? payload : &zero);
+  return 0;
+}
+
+static gvizStatSeries *find_series(const gvizEmbeddedGraph *e, const char *name) {
+  for (size_t i = 0; i < e->statCount; i++)
+    if (strcmp(e->stats[i].name, name) == 0)
+      return &e->stats[i];
+  return NULL;
+}
+
+gvizStatSeries *gvizEmbeddedGraphAddStatSeries(gvizEmbeddedGraph *e,
+                                               const char *name,
+                                               gvizStatChartKind kind) {
+  if (!e || !name)
+    return NULL;
+  gvizStatSeries *s = find_series(e, name);
+  if (s) {
+    s->kind = kind;
+    return s;
+  }
+  if (e->statCount == e->statCapacity) {
+    size_t cap = e->statCapacity ? e->statCapacity * 2 : 4;
+    gvizStatSeries *grown = realloc(e->stats, cap * sizeof(gvizStatSeries));
+    if (!grown)
+      return NULL;
+    e->stats = grown;
+    e->statCapacity = cap;
+  }
+  s = &e->stats[e->statCount++];
+  memset(s, 0, sizeof(*s));
+  s->name = name;
+  s->kind = kind;
+  return s;
+}
+
+int gvizEmbeddedGraphStatAppend(gvizEmbeddedGraph *e, const char *name,
+                                double value) {
+  if (!e || !name)
+    return -1;
+  gvizStatSeries *s = find_series(e, name);
+  if (!s)
+    s = gvizEmbeddedGraphAddStatSeries(e, name, GVIZ_STAT_CHART_LINE);
+  if (!s)
+    return -1;
+  if (s->count == s->capacity) {
+    size_t cap = s->capacity ? s->capacity * 2 : 64;
+    double *grown = realloc(s->samples, cap * sizeof(double));
+    if (!grown)
+      return -1;
+    s->samples = grown;
+    s->capacity = cap;
+  }
+  s->samples[s->count++] = value;
+  s->revision++;
+  return 0;
+}
+
+void gvizEmbeddedGraphStatClear(gvizEmbeddedGraph *e, const char *name) {
+  gvizStatSeries *s = e && name ? find_series(e, name) : NULL;
+  if (!s)
+    return;
+  s->count = 0;
+  s->revision++;
+}
+
+size_t gvizEmbeddedGraphStatSeriesCount(const gvizEmbeddedGraph *e) {
+  return e->statCount;
+}
+
+const gvizStatSeries *gvizEmbeddedGraphStatSeriesAt(const gvizEmbeddedGraph *e,
+                                                    size_t idx) {
+  return e && idx < e->statCount ? &e->stats[idx] : NULL;
+}
+
+const gvizStatSeries *gvizEmbeddedGraphFindStatSeries(const gvizEmbeddedGraph *e,
+                                                      const char *name) {
+  return e && name ? find_series(e, name) : NULL;
+}
+
+/* Draw mask (src/embedders/gvizEmbeddedGraph.c:74-122): vertex edits do not
+ * bump the revision; policy changes, resets and explicit notifications do. */
+void gvizEmbeddedGraphDrawMaskShowVertex(gvizEmbeddedGraph *e, size_t u) {
+  e->visibleVertices[u] = 1;
+}
+void gvizEmbeddedGraphDrawMaskHideVertex(gvizEmbeddedGraph *e, size_t u) {
+  e->visibleVertices[u] = 0;
+}
+void gvizEmbeddedGraphDrawMaskClearVertices(gvizEmbeddedGraph *e) {
+  memset(e->visibleVertices, 0, gvizGraphSize(e->subgraph.g));
+}
+void gvizEmbeddedGraphDrawMaskNotifyChanged(gvizEmbeddedGraph *e) {
+  e->drawMaskRevision++;
+}
+void gvizEmbeddedGraphResetDrawMask(gvizEmbeddedGraph *e) {
+  e->edgePolicy = GVIZ_DRAW_EDGES_ALL;
+  size_t n = gvizGraphSize(e->subgraph.g);
+  for (size_t u = 0; u < n; u++)
+    e->visibleVertices[u] = gvizSubgraphHasVertex(&e->subgraph, u) ? 1 : 0;
+  e->drawMaskRevision++;
+}
+uint64_t gvizEmbeddedGraphDrawMaskRevision(const gvizEmbeddedGraph *e) {
+  return e->drawMaskRevision;
+}
+void gvizEmbeddedGraphSetDrawMaskEdgePolicy(gvizEmbeddedGraph *e,
+                                            gvizDrawEdgePolicy policy) {
+  e->edgePolicy = policy;
+  e->drawMaskRevision++;
+}
+bool gvizEmbeddedGraphIsVertexVisible(const gvizEmbeddedGraph *e, size_t u) {
+  if (!gvizSubgraphHasVertex(&e->subgraph, u))
+    return false;
+  return e->visibleVertices[u] != 0;
+}
+bool gvizEmbeddedGraphIsEdgeVisible(const gvizEmbeddedGraph *e, size_t u,
+                                    size_t v) {
+  switch (e->edgePolicy) {
+  case GVIZ_DRAW_EDGES_NONE:
+    return false;
+  case GVIZ_DRAW_EDGES_IF_BOTH_VISIBLE:
+    return gvizEmbeddedGraphIsVertexVisible(e, u) &&
+           gvizEmbeddedGraphIsVertexVisible(e, v);
+  case GVIZ_DRAW_EDGES_ALL:
+  default: {
+    if (!gvizSubgraphHasVertex(&e->subgraph, u) ||
+        !gvizSubgraphHasVertex(&e->subgraph, v))
+      return false;
+    size_t c;
+    const size_t *nb = gvizGraphNeighbors(e->subgraph.g, u, &c);
+    for (size_t i = 0; i < c; i++)
+      if (nb[i] == v)
+        return true;
+    return false;
+  }
+  }
+}
+
+/* src/embedders/gvizEmbeddedGraph.c:428-448 */
+int gvizEmbeddedGraphSaveEmbedding(gvizEmbeddedGraph *e, const char *name,
+                                   const char *filename) {
+  FILE *f = fopen(filename, "w");
+  if (!f)
+    return -1;
+  fprintf(f, "%s\n", name);
+  size_t n = gvizGraphSize(e->subgraph.g);
+  fprintf(f, "%zu %zu\n", n, e->embedding.dim);
+  for (size_t i = 0; i < n; i++) {
+    double *p = gvizEmbeddedGraphGetVPosition(e, i);
+    for (size_t j = 0; j < e->embedding.dim; j++)
+      fprintf(f, "%f ", p[j]);
+    fprintf(f, "\n");
+  }
+  fclose(f);
+  return 0;
+}
+
+/* src/embedders/gvizEmbeddedGraph.c:450-485 */
+int gvizEmbeddedGraphLoadEmbedding(gvizEmbeddedGraph *e, const char *filename) {
+  FILE *f = fopen(filename, "r");
+  if (!f)
+    return -1;
+  char name[256];
+  size_t n, dim;
+  if (!fgets(name, sizeof(name), f) || fscanf(f, "%zu %zu", &n, &dim) != 2 ||
+      n != gvizGraphSize(e->subgraph.g) || dim != e->embedding.dim) {
+    fclose(f);
+    return -1;
+  }
+  for (size_t i = 0; i < n; i++) {
+    double *p = gvizEmbeddedGraphGetVPosition(e, i);
+    for (size_t j = 0; j < dim; j++)
+      if (fscanf(f, "%lf", &p[j]) != 1) {
+        fclose(f);
+        return -1;
+      }
+  }
+  fclose(f);
+  return 0;
+}

@@ -1,0 +1,508 @@
+/*
+ * gviz_force_embedder.c -- gvizForceEmbedder* on the gfx950 engine.
+ *
+ * The state keeps the reference's configuration fields and defaults
+ * (src/embedders/gvizForceEmbedder.c:219-325, 357-410); Begin chooses the
+ * initial positions on the host exactly as the reference does (rand_r over
+ * the subgraph in ascending vertex id, :412-443) and hands them to the
+ * device; Step and Run execute on the GPU (gvizhip_force_step / _run) and
+ * append the same five stat series per step (:445-510). There is no CPU
+ * path: without a GPU, Begin fails and Step returns a negative value.
+ */
+#include "gviz/gvizForceEmbedder.h"
+
+#include "gviz_hip.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------ force model table */
+
+/* Direction for coincident points: a hash of the two coordinate buffers'
+ * addresses mapped to an angle (include/core/gvizVec.h:385-394). */
+static void coincident_dir(const double *a, const double *b, double *ux,
+                           double *uy) {
+  uint64_t bits = (uint64_t)(uintptr_t)a ^
+                  ((uint64_t)(uintptr_t)b * 0x9E3779B97F4A7C15ULL);
+  bits ^= bits >> 33;
+  bits *= 0xff51afd7ed558ccdULL;
+  bits ^= bits >> 33;
+  double angle = (double)(bits % 1000000u) / 1000000.0 * 2.0 * M_PI;
+  *ux = cos(angle);
+  *uy = sin(angle);
+}
+
+static double norm_diff(int n, const double *v, const double *u, double *d) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) {
+    d[i] = u[i] - v[i];
+    s += d[i] * d[i];
+  }
+  return sqrt(s);
+}
+
+/* Shared shape of both repulsions (include/core/gvizVec.h:404-427, 552-575):
+ * magnitude over the surface gap, overlap constant once in contact. */
+static void push_away(int n, const double *v, const double *u, double num,
+                      double radiusSum, double overlapConstant, double weight,
+                      double *acc) {
+  double d[16];
+  if (n > 16)
+    return;
+  double dist = norm_diff(n, v, u, d);
+  double gap = dist - radiusSum;
+  double mag = gap > 0.0 ? num / gap : num * overlapConstant;
+  if (dist > 0.0) {
+    for (int i = 0; i < n; i++)
+      acc[i] -= weight * (mag * (d[i] / dist));
+  } else if (n == 2) {
+    double ux, uy;
+    coincident_dir(v, u, &ux, &uy);
+    acc[0] -= weight * (mag * ux);
+    acc[1] -= weight * (mag * uy);
+  }
+}
+
+static double fr_mass(size_t degree) {
+  (void)degree;
+  return 1.0;
+}
+
+static void fr_att(int n, double *v, double *u, double k, double *acc) {
+  double d[16];
+  if (n > 16)
+    return;
+  double dist = norm_diff(n, v, u, d);
+  if (dist < 1e-9)
+    return;
+  for (int i = 0; i < n; i++)
+    acc[i] += dist / k * d[i];
+}
+
+static void fr_rep(int n, double *v, double *other, double vMass,
+                   double otherMass, double vRadius, double otherRadius,
+                   double overlapConstant, double edgeLength, double *acc) {
+  (void)vMass;
+  push_away(n, v, other, edgeLength * edgeLength, vRadius + otherRadius,
+            overlapConstant, (double)(size_t)otherMass, acc);
+}
+
+static double linlog_mass(size_t degree) { return 1.0 + (double)degree; }
+
+static void linlog_att(int n, double *v, double *u, double k, double *acc) {
+  (void)k;
+  double d[16];
+  if (n > 16)
+    return;
+  double dist = norm_diff(n, v, u, d);
+  if (dist < 1e-9)
+    return;
+  for (int i = 0; i < n; i++)
+    acc[i] += log(1.0 + dist) / dist * d[i];
+}
+
+static void linlog_rep(int n, double *v, double *other, double vMass,
+                       double otherMass, double vRadius, double otherRadius,
+                       double overlapConstant, double edgeLength, double *acc) {
+  (void)edgeLength;
+  push_away(n, v, other, vMass * otherMass, vRadius + otherRadius,
+            overlapConstant, 1.0, acc);
+}
+
+static const gvizForceModel kModelFR = {fr_mass, fr_att, fr_rep};
+static const gvizForceModel kModelLinLog = {linlog_mass, linlog_att,
+                                            linlog_rep};
+
+/* src/embedders/gvizForceModel.c:57-65 */
+const gvizForceModel *gvizForceModelGet(gvizForceModelKind kind) {
+  return kind == GVIZ_FORCE_MODEL_LINLOG ? &kModelLinLog : &kModelFR;
+}
+
+/* ------------------------------------------------------------- embedder */
+
+static __thread const char *host_error = "";
+
+const char *gvizForceEmbedderLastError(void) {
+  return host_error[0] ? host_error : gvizhip_last_error();
+}
+
+/* src/embedders/gvizForceEmbedder.c:12-17 */
+static double default_box_extent(size_t vertexCount, double edgeLength) {
+  if (vertexCount == 0)
+    return edgeLength;
+  return 0.5 * sqrt((double)vertexCount) * edgeLength;
+}
+
+static void device_config(const gvizForceEmbedderState *s,
+                          gvizhip_force_config *c) {
+  memset(c, 0, sizeof(*c));
+  c->model = (int32_t)s->modelKind;
+  c->barnesHut = s->barnesHutEnabled ? 1 : 0;
+  c->preventOverlap = s->preventOverlap ? 1 : 0;
+  c->precision = (int32_t)s->precision;
+  c->nodesPerCell = (int64_t)s->nodesPerCell;
+  c->edgeLength = s->edgeLength;
+  c->theta = s->theta;
+  c->gravityK = s->gravityK;
+  c->overlapConstant = s->overlapConstant;
+  c->radiusBase = s->radiusBase;
+  c->radiusPerDegree = s->radiusPerDegree;
+  c->jitterTolerance = s->jitterTolerance;
+}
+
+/* The configuration last handed to the device, kept beside the handle so a
+ * setter call costs nothing until the next step (a change re-records the
+ * step's graph). */
+typedef struct device_box {
+  gvizhip_force *f;
+  gvizhip_force_config applied;
+} device_box;
+
+static int sync_config(gvizForceEmbedderState *s) {
+  device_box *box = (device_box *)s->device;
+  gvizhip_force_config c;
+  device_config(s, &c);
+  if (memcmp(&c, &box->applied, sizeof(c)) == 0)
+    return 0;
+  if (gvizhip_force_configure(box->f, &c) != 0)
+    return -1;
+  box->applied = c;
+  return 0;
+}
+
+static void gather_positions(gvizForceEmbedderState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  for (size_t i = 0; i < s->vertexCount; i++) {
+    const double *p = gvizEmbeddedGraphGetVPosition(e, s->vertices[i]);
+    s->positionsScratch[2 * i] = p[0];
+    s->positionsScratch[2 * i + 1] = p[1];
+  }
+}
+
+static void scatter_positions(gvizForceEmbedderState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  for (size_t i = 0; i < s->vertexCount; i++)
+    gvizEmbeddedGraphSetVPosition(e, s->vertices[i], s->positionsScratch + 2 * i);
+}
+
+static void record_step(gvizForceEmbedderState *s, const gvizhip_force_stats *r) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  s->globalSpeed = r->speed;
+  s->speedEfficiency = r->speedEfficiency;
+  s->iteration++;
+  s->lastMaxDisplacement = r->maxDisp;
+  gvizEmbeddedGraphStatAppend(e, "forceEmbedder.maxDisp", r->maxDisp);
+  gvizEmbeddedGraphStatAppend(e, "forceEmbedder.speed", r->speed);
+  gvizEmbeddedGraphStatAppend(e, "forceEmbedder.attractiveForce", r->meanAtt);
+  gvizEmbeddedGraphStatAppend(e, "forceEmbedder.repulsiveForce", r->meanRep);
+  gvizEmbeddedGraphStatAppend(e, "forceEmbedder.gravityForce", r->meanGrav);
+}
+
+static void action_step(gvizEmbeddedGraph *embedding, void *userData,
+                        const gvizActionPayload *payload) {
+  (void)userData;
+  (void)payload;
+  gvizForceEmbedderState *s = (gvizForceEmbedderState *)embedding;
+  if (s->begun)
+    gvizForceEmbedderStep(s);
+}
+
+static void action_toggle_overlap(gvizEmbeddedGraph *embedding, void *userData,
+                                  const gvizActionPayload *payload) {
+  (void)userData;
+  (void)payload;
+  gvizForceEmbedderState *s = (gvizForceEmbedderState *)embedding;
+  s->preventOverlap = !s->preventOverlap;
+}
+
+int gvizForceEmbedderInit(gvizForceEmbedderState *s, gvizSubgraph subgraph,
+                          size_t dimension, gvizForceModelKind model) {
+  memset(s, 0, sizeof(*s));
+  if (dimension != 2)
+    return -2;
+  int res = gvizEmbeddedGraphInit((gvizEmbeddedGraph *)s, subgraph, dimension);
+  if (res < 0)
+    return res;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  s->modelKind = model == GVIZ_FORCE_MODEL_LINLOG ? GVIZ_FORCE_MODEL_LINLOG
+                                                  : GVIZ_FORCE_MODEL_FRUCHTERMAN_REINGOLD;
+  s->model = gvizForceModelGet(s->modelKind);
+  if (gvizSubgraphToCompactCSR(&e->subgraph, &s->csrOffsets, &s->csrAdj,
+                               &s->vertices, &s->vertexCount) != 0)
+    return -1;
+  size_t n = s->vertexCount;
+  s->degree = malloc(sizeof(size_t) * (n ? n : 1));
+  s->mass = malloc(sizeof(double) * (n ? n : 1));
+  s->positionsScratch = malloc(sizeof(double) * 2 * (n ? n : 1));
+  if (!s->degree || !s->mass || !s->positionsScratch)
+    return -1;
+  double sumDegPlusOne = 0.0;
+  for (size_t i = 0; i < n; i++) {
+    s->degree[i] = (size_t)(s->csrOffsets[i + 1] - s->csrOffsets[i]);
+    s->mass[i] = s->model->vertexMass(s->degree[i]);
+    sumDegPlusOne += (double)(s->degree[i] + 1);
+  }
+  s->meanDegreePlusOne = n ? sumDegPlusOne / (double)n : 0.0;
+
+  s->edgeLength = GVIZ_FORCE_EMBEDDER_EDGE_LENGTH_DEFAULT;
+  s->boxExtent = default_box_extent(n, s->edgeLength);
+  s->jitterTolerance = GVIZ_FORCE_EMBEDDER_JITTER_TOLERANCE_DEFAULT;
+  s->theta = GVIZ_FORCE_EMBEDDER_THETA_DEFAULT;
+  s->nodesPerCell = GVIZ_QUADTREE_NODES_PER_CELL_DEFAULT;
+  s->gravityK = 0.0;
+  s->radiusBase = 0.0;
+  s->radiusPerDegree = 0.0;
+  s->overlapConstant = GVIZ_FORCE_EMBEDDER_OVERLAP_CONSTANT_DEFAULT;
+  s->preventOverlap = 0;
+  s->barnesHutEnabled = 1;
+  s->precision = GVIZ_FORCE_PRECISION_MIXED;
+  s->residentPositions = 0;
+
+  if (gvizEmbeddedGraphAddAction(e, "forceEmbedder.step", action_step, NULL) < 0)
+    return -1;
+  if (gvizEmbeddedGraphAddAction(e, "forceEmbedder.toggleOverlapPrevention",
+                                 action_toggle_overlap, NULL) < 0)
+    return -1;
+  static const char *const series[] = {
+      "forceEmbedder.maxDisp", "forceEmbedder.speed",
+      "forceEmbedder.attractiveForce", "forceEmbedder.repulsiveForce",
+      "forceEmbedder.gravityForce"};
+  for (size_t i = 0; i < sizeof(series) / sizeof(series[0]); i++)
+    if (!gvizEmbeddedGraphAddStatSeries(e, series[i], GVIZ_STAT_CHART_LINE_LOG))
+      return -1;
+  return 0;
+}
+
+void gvizForceEmbedderRelease(gvizForceEmbedderState *s) {
+  if (!s)
+    return;
+  if (s->device) {
+    device_box *box = (device_box *)s->device;
+    gvizhip_force_destroy(box->f);
+    free(box);
+    s->device = NULL;
+  }
+  gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
+  free(s->vertices);
+  free(s->degree);
+  free(s->mass);
+  free(s->positionsScratch);
+  free(s->csrOffsets);
+  free(s->csrAdj);
+  s->vertices = NULL;
+  s->degree = NULL;
+  s->mass = NULL;
+  s->positionsScratch = NULL;
+  s->csrOffsets = s->csrAdj = NULL;
+}
+
+/* Setters: src/embedders/gvizForceEmbedder.c:357-410 */
+void gvizForceEmbedderConfigure(gvizForceEmbedderState *s, double edgeLength,
+                                double boxExtent) {
+  if (edgeLength > 0.0)
+    s->edgeLength = edgeLength;
+  if (boxExtent > 0.0)
+    s->boxExtent = boxExtent;
+}
+
+void gvizForceEmbedderConfigureSpeed(gvizForceEmbedderState *s,
+                                     double tolerance) {
+  if (tolerance > 0.0)
+    s->jitterTolerance = tolerance;
+}
+
+void gvizForceEmbedderConfigureBarnesHut(gvizForceEmbedderState *s,
+                                         double theta, size_t nodesPerCell) {
+  if (theta > 0.0)
+    s->theta = theta;
+  if (nodesPerCell > 0)
+    s->nodesPerCell = nodesPerCell;
+}
+
+void gvizForceEmbedderSetBarnesHutEnabled(gvizForceEmbedderState *s,
+                                          int enabled) {
+  s->barnesHutEnabled = enabled;
+}
+
+void gvizForceEmbedderConfigureGravity(gvizForceEmbedderState *s, double k) {
+  s->gravityK = k;
+}
+
+double gvizForceEmbedderVertexRadius(const gvizForceEmbedderState *s,
+                                     size_t index) {
+  return s->radiusBase *
+         (1.0 + s->radiusPerDegree * sqrt((double)s->degree[index]));
+}
+
+void gvizForceEmbedderConfigureRadius(gvizForceEmbedderState *s, double base,
+                                      double perDegree) {
+  s->radiusBase = base;
+  s->radiusPerDegree = perDegree;
+}
+
+void gvizForceEmbedderConfigureOverlapPrevention(gvizForceEmbedderState *s,
+                                                 double constant) {
+  if (constant > 0.0)
+    s->overlapConstant = constant;
+}
+
+void gvizForceEmbedderSetPreventOverlapEnabled(gvizForceEmbedderState *s,
+                                               int enabled) {
+  s->preventOverlap = enabled;
+}
+
+void gvizForceEmbedderSetPrecision(gvizForceEmbedderState *s,
+                                   gvizForcePrecision precision) {
+  s->precision = precision;
+}
+
+void gvizForceEmbedderSetResidentPositions(gvizForceEmbedderState *s,
+                                           int enabled) {
+  if (s->residentPositions && !enabled && s->begun)
+    gvizForceEmbedderSyncPositions(s);
+  s->residentPositions = enabled;
+}
+
+int gvizForceEmbedderSyncPositions(gvizForceEmbedderState *s) {
+  if (!s->device)
+    return s->vertexCount == 0 ? 0 : -1;
+  device_box *box = (device_box *)s->device;
+  if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
+    return -1;
+  scatter_positions(s);
+  return 0;
+}
+
+static int ensure_device(gvizForceEmbedderState *s) {
+  if (s->device)
+    return sync_config(s);
+  device_box *box = calloc(1, sizeof(device_box));
+  if (!box) {
+    host_error = "gvizForceEmbedder: out of host memory";
+    return -1;
+  }
+  device_config(s, &box->applied);
+  if (gvizhip_force_create(&box->f, (int64_t)s->vertexCount, s->csrOffsets,
+                           s->csrAdj, &box->applied) != 0) {
+    free(box);
+    return -1;
+  }
+  s->device = box;
+  return 0;
+}
+
+/* src/embedders/gvizForceEmbedder.c:412-443 */
+int gvizForceEmbedderBegin(gvizForceEmbedderState *s, unsigned int seed) {
+  host_error = "";
+  gvizEmbeddedGraphRandomizePositions((gvizEmbeddedGraph *)s, s->boxExtent, seed);
+  s->globalSpeed = GVIZ_FORCE_EMBEDDER_SPEED_INITIAL;
+  s->speedEfficiency = GVIZ_FORCE_EMBEDDER_SPEED_EFFICIENCY_INITIAL;
+  s->iteration = 0;
+  s->lastMaxDisplacement = 0.0;
+  s->begun = 1;
+  if (s->vertexCount == 0)
+    return 0;
+  gather_positions(s);
+  if (ensure_device(s) != 0)
+    return -1;
+  return gvizhip_force_begin(((device_box *)s->device)->f, s->positionsScratch);
+}
+
+/* Appends the stats of a step on an empty subgraph (every mean is 0). */
+static double empty_step(gvizForceEmbedderState *s) {
+  gvizhip_force_stats r = {0};
+  r.speed = s->globalSpeed;
+  r.speedEfficiency = s->speedEfficiency;
+  r.meanGrav = s->gravityK * s->meanDegreePlusOne;
+  record_step(s, &r);
+  return 0.0;
+}
+
+/* src/embedders/gvizForceEmbedder.c:445-495 */
+double gvizForceEmbedderStep(gvizForceEmbedderState *s) {
+  host_error = "";
+  if (s->vertexCount == 0)
+    return empty_step(s);
+  if (!s->device) {
+    host_error = "gvizForceEmbedderStep: Begin has not created the GPU engine";
+    return -1.0;
+  }
+  device_box *box = (device_box *)s->device;
+  if (sync_config(s) != 0)
+    return -1.0;
+  if (!s->residentPositions) {
+    gather_positions(s);
+    if (gvizhip_force_upload_positions(box->f, s->positionsScratch) != 0)
+      return -1.0;
+  }
+  gvizhip_force_stats r;
+  if (gvizhip_force_step(box->f, &r) != 0)
+    return -1.0;
+  if (!s->residentPositions) {
+    if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
+      return -1.0;
+    scatter_positions(s);
+  }
+  record_step(s, &r);
+  return r.maxDisp;
+}
+
+/* src/embedders/gvizForceEmbedder.c:497-510, with the loop on the device:
+ * positions cross the bus once each way per call, and the stop test on
+ * maxDisp runs on the GPU. */
+int gvizForceEmbedderRun(gvizForceEmbedderState *s, size_t maxIters,
+                         double epsilon) {
+  host_error = "";
+  if (!s->begun)
+    return -1;
+  if (s->vertexCount == 0) {
+    size_t rounds = 0;
+    while (rounds < maxIters) {
+      double d = empty_step(s);
+      rounds++;
+      if (d < epsilon)
+        break;
+    }
+    return (int)rounds;
+  }
+  if (!s->device)
+    return -1;
+  device_box *box = (device_box *)s->device;
+  if (sync_config(s) != 0)
+    return -1;
+  if (!s->residentPositions) {
+    gather_positions(s);
+    if (gvizhip_force_upload_positions(box->f, s->positionsScratch) != 0)
+      return -1;
+  }
+  enum { kChunk = 4096 };
+  gvizhip_force_stats *rec = malloc(sizeof(gvizhip_force_stats) * kChunk);
+  if (!rec) {
+    host_error = "gvizForceEmbedderRun: out of host memory";
+    return -1;
+  }
+  size_t rounds = 0;
+  int failed = 0;
+  while (rounds < maxIters) {
+    size_t want = maxIters - rounds < kChunk ? maxIters - rounds : kChunk;
+    int64_t ran = gvizhip_force_run(box->f, (int64_t)want, epsilon, rec);
+    if (ran < 0) {
+      failed = 1;
+      break;
+    }
+    for (int64_t i = 0; i < ran; i++)
+      record_step(s, &rec[i]);
+    rounds += (size_t)ran;
+    if ((size_t)ran < want || (ran > 0 && rec[ran - 1].maxDisp < epsilon))
+      break;
+  }
+  free(rec);
+  if (!s->residentPositions) {
+    if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
+      return -1;
+    scatter_positions(s);
+  }
+  return failed ? -1 : (int)rounds;
+}

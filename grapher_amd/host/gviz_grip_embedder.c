@@ -1,0 +1,439 @@
+/*
+ * gviz_grip_embedder.c -- gvizGRIPEmbedder* with the multilevel schedule's
+ * heavy phases on the GPU (reference: src/embedders/gvizGRIPEmbedder.c).
+ *
+ * The host keeps the schedule (current layer, round, K policy, draw mask,
+ * stat series) exactly as the reference does; the MIS filtration, the kNN
+ * searches, the barycentric placement of each new layer and every
+ * refinement round run on the device (include/gviz_hip.h, GRIP section),
+ * which keeps positions, per-vertex state and kNN lists in HBM. Positions
+ * are uploaded before and read back after each device phase that uses
+ * them, unless residentPositions is set (then gvizGRIPEmbedderSyncPositions
+ * fetches them). Dimension 2 only.
+ */
+#include "gviz/gvizGRIPEmbedder.h"
+
+#include "gviz_hip.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define GRIP_EDGE_LENGTH 10.0              /* gvizGRIPEmbedder.c:31 */
+#define GRIP_KNN_CAPACITY_DEFAULT 256      /* :33 */
+#define GRIP_K_MAX_DEFAULT 128             /* :34 */
+#define GRIP_EMBED_MAX_ROUNDS_PER_LAYER 30 /* :1263 */
+#define GRIP_EMBED_CONVERGENCE_FACTOR 1e-3 /* :1264 */
+/* include/algorithms/search/gvizKNearest.h:9-13 */
+#define KNN_BATCH_VISIBLE_MAX 256
+#define KNN_BATCH_MIN_AVG_DEGREE 32.0
+#define KNN_BATCH_TARGET_RATIO 8
+#define MIS_MAX_LAYERS 128
+
+static size_t border_at(const gvizGRIPState *s, size_t i) {
+  return s->misBorder[i];
+}
+
+static size_t clamp_k(size_t k, size_t minK, size_t maxK) {
+  if (k < minK)
+    k = minK;
+  if (k > maxK)
+    k = maxK;
+  return k;
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:63-111 */
+static size_t compute_k(const gvizGRIPState *s, size_t maxK, int forPlacement) {
+  size_t minK = ((const gvizEmbeddedGraph *)s)->embedding.dim + 1;
+  size_t cap = s->knnCapacity;
+  if (maxK > cap)
+    maxK = cap;
+  gvizGRIPKPolicy policy = s->kPolicy;
+  if (forPlacement && policy == GVIZ_GRIP_K_PLACEMENT_DECAY)
+    policy = GVIZ_GRIP_K_LAYER_DECAY;
+  else if (!forPlacement && policy == GVIZ_GRIP_K_PLACEMENT_DECAY)
+    policy = GVIZ_GRIP_K_CONSTANT;
+  else if (forPlacement && policy == GVIZ_GRIP_K_BUDGET)
+    policy = GVIZ_GRIP_K_LAYER_DECAY;
+  size_t k;
+  switch (policy) {
+  case GVIZ_GRIP_K_LAYER_DECAY:
+    k = s->currLayer >= 8 * sizeof(size_t) ? 0 : maxK >> s->currLayer;
+    break;
+  case GVIZ_GRIP_K_LAYER_GROW: {
+    size_t depth = s->layerCount > 0 ? s->layerCount - 1 - s->currLayer : 0;
+    k = depth >= 8 * sizeof(size_t) ? 0 : maxK >> depth;
+    break;
+  }
+  case GVIZ_GRIP_K_BUDGET: {
+    size_t active = border_at(s, s->currLayer);
+    size_t finest = border_at(s, 0);
+    if (active == 0)
+      return clamp_k(maxK, minK, cap);
+    double scaled = (double)maxK * ((double)finest / (double)active);
+    k = scaled >= (double)cap ? cap : (size_t)scaled;
+    return clamp_k(k, minK, cap);
+  }
+  case GVIZ_GRIP_K_CONSTANT:
+  case GVIZ_GRIP_K_PLACEMENT_DECAY:
+  default:
+    k = maxK;
+    break;
+  }
+  return clamp_k(k, minK, maxK);
+}
+
+static size_t placement_k(const gvizGRIPState *s) {
+  return compute_k(s, s->placementKMax, 1);
+}
+
+static size_t refinement_k(const gvizGRIPState *s) {
+  return compute_k(s, s->refinementKMax, 0);
+}
+
+/* src/algorithms/search/gvizKNearest.c:149-166: whether one BFS per visible
+ * seed beats one BFS per target. */
+static int prefer_batch(const gvizGRIPState *s, size_t visibleCount,
+                        size_t targetCount) {
+  if (visibleCount == 0 || visibleCount >= targetCount ||
+      visibleCount > KNN_BATCH_VISIBLE_MAX)
+    return 0;
+  const gvizSubgraph *sg = &((const gvizEmbeddedGraph *)s)->subgraph;
+  size_t vc = gvizSubgraphVertexCount(sg);
+  if (vc == 0)
+    return 0;
+  double avgDeg = (2.0 * (double)gvizSubgraphEdgeCount(sg)) / (double)vc;
+  if (avgDeg > KNN_BATCH_MIN_AVG_DEGREE)
+    return 1;
+  return targetCount >= KNN_BATCH_TARGET_RATIO * visibleCount;
+}
+
+void gvizGRIPEmbedderConfigureStats(gvizGRIPState *s, bool enable) {
+  if (s)
+    s->statsRegisterMode = enable ? 0 : 1;
+}
+
+void gvizGRIPEmbedderConfigureKnnCapacity(gvizGRIPState *s, size_t knnCapacity) {
+  if (s)
+    s->knnCapacity = knnCapacity;
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:134-147 */
+void gvizGRIPEmbedderConfigureK(gvizGRIPState *s, size_t placementKMax,
+                                size_t refinementKMax, gvizGRIPKPolicy policy) {
+  if (!s)
+    return;
+  s->kPolicy = policy;
+  if (placementKMax > 0)
+    s->placementKMax = placementKMax > s->knnCapacity ? s->knnCapacity : placementKMax;
+  if (refinementKMax > 0)
+    s->refinementKMax =
+        refinementKMax > s->knnCapacity ? s->knnCapacity : refinementKMax;
+}
+
+gvizGRIPRoundStats gvizGRIPEmbedderLastRoundStats(const gvizGRIPState *s) {
+  return s ? s->lastRoundStats : (gvizGRIPRoundStats){0, 0, 0};
+}
+
+static void action_refine_round(gvizEmbeddedGraph *embedding, void *userData,
+                                const gvizActionPayload *payload) {
+  (void)userData;
+  (void)payload;
+  gvizGRIPState *s = (gvizGRIPState *)embedding;
+  if (s->layerCount == 0)
+    return;
+  gvizGRIPEmbedderRefineRound(s);
+}
+
+static void action_next_stage(gvizEmbeddedGraph *embedding, void *userData,
+                              const gvizActionPayload *payload) {
+  (void)userData;
+  (void)payload;
+  gvizGRIPState *s = (gvizGRIPState *)embedding;
+  if (s->layerCount == 0)
+    return;
+  gvizGRIPEmbedderNextStage(s);
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:194-332. The device engine (graph, kNN
+ * arrays of knnCapacity entries per vertex, per-vertex state) is created by
+ * the first Begin. */
+int gvizGRIPEmbedderInit(gvizGRIPState *s, gvizSubgraph subgraph,
+                         size_t diameter, size_t dimension) {
+  (void)diameter; /* sized the reference's border array; ours is fixed */
+  size_t activeCount = gvizSubgraphVertexCount(&subgraph);
+  if (activeCount < dimension + 1)
+    return -1;
+  if (dimension != 2)
+    return -2;
+  uint8_t statsRegisterMode = s->statsRegisterMode;
+  size_t knnCapacityRequested = s->knnCapacity;
+  memset(s, 0, sizeof(*s));
+  s->statsRegisterMode = statsRegisterMode;
+  s->placementKMax = GRIP_K_MAX_DEFAULT;
+  s->refinementKMax = GRIP_K_MAX_DEFAULT;
+  s->knnCapacity =
+      knnCapacityRequested > 0 ? knnCapacityRequested : GRIP_KNN_CAPACITY_DEFAULT;
+  s->kPolicy = GVIZ_GRIP_K_CONSTANT;
+  s->statsEnabled = s->statsRegisterMode != 1;
+
+  int res = gvizEmbeddedGraphInit((gvizEmbeddedGraph *)s, subgraph, dimension);
+  if (res < 0)
+    return res;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  s->misFiltration = calloc(activeCount, sizeof(size_t));
+  s->misBorder = calloc(MIS_MAX_LAYERS, sizeof(size_t));
+  if (!s->misFiltration || !s->misBorder)
+    return -1;
+  s->misBorder[0] = activeCount;
+  s->misBorderCount = 1;
+  gvizEmbeddedGraphDrawMaskClearVertices(e);
+
+  if (gvizEmbeddedGraphAddAction(e, "grip.refineRound", action_refine_round, NULL) < 0)
+    return -1;
+  if (gvizEmbeddedGraphAddAction(e, "grip.nextStage", action_next_stage, NULL) < 0)
+    return -1;
+  if (s->statsEnabled) {
+    if (!gvizEmbeddedGraphAddStatSeries(e, "grip.heat", GVIZ_STAT_CHART_LINE_LOG) ||
+        !gvizEmbeddedGraphAddStatSeries(e, "grip.meanDisp", GVIZ_STAT_CHART_LINE_LOG) ||
+        !gvizEmbeddedGraphAddStatSeries(e, "grip.maxDisp", GVIZ_STAT_CHART_LINE_LOG) ||
+        !gvizEmbeddedGraphAddStatSeries(e, "grip.meanForce", GVIZ_STAT_CHART_LINE_LOG))
+      return -1;
+  }
+  return 0;
+}
+
+void gvizGRIPEmbedderRelease(gvizGRIPState *s) {
+  if (!s)
+    return;
+  if (s->device)
+    gvizhip_grip_destroy(s->device);
+  s->device = NULL;
+  free(s->misFiltration);
+  free(s->misBorder);
+  s->misFiltration = NULL;
+  s->misBorder = NULL;
+  gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:335-420: n+1 vertices of a regular
+ * n-simplex centred at the origin with the given side length (component-wise
+ * construction on the unit sphere, pairwise dot product -1/n, then scaled). */
+static void regular_simplex(size_t n, double side, double *out) {
+  memset(out, 0, sizeof(double) * (n + 1) * n);
+  double c = -1.0 / (double)n;
+  out[0] = 1.0;
+  for (size_t k = 1; k <= n; k++) {
+    double *vk = out + k * n;
+    for (size_t j = 0; j < k; j++) {
+      const double *vj = out + j * n;
+      double dot = 0.0;
+      for (size_t m = 0; m < j; m++)
+        dot += vk[m] * vj[m];
+      vk[j] = (c - dot) / vj[j];
+    }
+    if (k < n) {
+      double sq = 0.0;
+      for (size_t m = 0; m < k; m++)
+        sq += vk[m] * vk[m];
+      double rem = 1.0 - sq;
+      vk[k] = rem > 0.0 ? sqrt(rem) : 0.0;
+    }
+  }
+  double edge = 0.0;
+  for (size_t m = 0; m < n; m++) {
+    double d = out[m] - out[n + m];
+    edge += d * d;
+  }
+  double scale = side / sqrt(edge);
+  for (size_t i = 0; i < (n + 1) * n; i++)
+    out[i] *= scale;
+}
+
+static double *host_positions(gvizGRIPState *s) {
+  return ((gvizEmbeddedGraph *)s)->embedding.vertexPositions;
+}
+
+static int push_positions(gvizGRIPState *s) {
+  if (s->residentPositions && s->devicePositionsFresh)
+    return 0;
+  if (gvizhip_grip_upload(s->device, host_positions(s), NULL, NULL, NULL) != 0)
+    return -1;
+  s->devicePositionsFresh = 1;
+  return 0;
+}
+
+static int pull_positions(gvizGRIPState *s) {
+  if (s->residentPositions)
+    return 0;
+  return gvizhip_grip_download(s->device, host_positions(s), NULL, NULL, NULL);
+}
+
+int gvizGRIPEmbedderSyncPositions(gvizGRIPState *s) {
+  if (!s || !s->device)
+    return -1;
+  return gvizhip_grip_download(s->device, host_positions(s), NULL, NULL, NULL);
+}
+
+/* clearDecorators then updateKNNs (src/embedders/gvizGRIPEmbedder.c:936-1006)
+ * for the current layer against the draw mask. */
+static int refresh_layer(gvizGRIPState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  size_t end = border_at(s, s->currLayer);
+  if (gvizhip_grip_clear_decorators(s->device, (int64_t)end) != 0)
+    return -1;
+  size_t n = gvizGraphSize(e->subgraph.g);
+  size_t visibleCount = gvizVertexSubsetCount(e->visibleVertices, n);
+  int batch = end > 0 && visibleCount > 0 && visibleCount < end &&
+              prefer_batch(s, visibleCount, end);
+  return gvizhip_grip_update_knn(s->device, (int64_t)end, e->visibleVertices,
+                                 (int64_t)refinement_k(s), batch);
+}
+
+/* gripSyncDrawMask (:186-192) */
+static void sync_draw_mask(gvizGRIPState *s) {
+  gvizEmbeddedGraphSetDrawMaskEdgePolicy(
+      (gvizEmbeddedGraph *)s,
+      s->currLayer == 0 ? GVIZ_DRAW_EDGES_IF_BOTH_VISIBLE : GVIZ_DRAW_EDGES_NONE);
+}
+
+static int ensure_device(gvizGRIPState *s) {
+  if (s->device)
+    return 0;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  int64_t *off = NULL, *adj = NULL;
+  if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
+    return -1;
+  int rc = gvizhip_grip_create(&s->device, (int64_t)gvizGraphSize(e->subgraph.g),
+                               off, adj, e->subgraph.vs, (int64_t)s->knnCapacity);
+  free(off);
+  free(adj);
+  if (rc != 0) {
+    s->device = NULL;
+    return -1;
+  }
+  return 0;
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:1008-1030. Without a usable GPU, or for
+ * a filtration the device cannot finish (a subgraph that is not connected),
+ * layerCount stays 0 (gvizhip_last_error() says why). */
+void gvizGRIPEmbedderBegin(gvizGRIPState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  size_t active = border_at(s, 0);
+  s->layerCount = 0;
+  if (ensure_device(s) != 0)
+    return;
+  int64_t *filt = malloc(sizeof(int64_t) * active);
+  int64_t borders[MIS_MAX_LAYERS];
+  if (!filt)
+    return;
+  int64_t layers = gvizhip_grip_mis_filtration(s->device, (int64_t)e->embedding.dim,
+                                               filt, borders);
+  if (layers <= 0 || layers > MIS_MAX_LAYERS) {
+    free(filt);
+    s->layerCount = 0;
+    return;
+  }
+  for (size_t i = 0; i < active; i++)
+    s->misFiltration[i] = (size_t)filt[i];
+  free(filt);
+  for (int64_t i = 0; i < layers; i++)
+    s->misBorder[i] = (size_t)borders[i];
+  s->misBorderCount = (size_t)layers;
+  s->layerCount = (size_t)layers;
+  s->currLayer = s->layerCount - 1;
+  s->currRound = 0;
+
+  size_t dim = e->embedding.dim;
+  double simplex[2 * 3];
+  regular_simplex(dim, GRIP_EDGE_LENGTH * 1000, simplex);
+  for (size_t j = 0; j < dim + 1; j++) {
+    gvizEmbeddedGraphDrawMaskShowVertex(e, s->misFiltration[j]);
+    gvizEmbeddedGraphSetVPosition(e, s->misFiltration[j], simplex + j * dim);
+  }
+  s->devicePositionsFresh = 0;
+  if (push_positions(s) != 0 || refresh_layer(s) != 0) {
+    s->layerCount = 0;
+    return;
+  }
+  sync_draw_mask(s);
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:1034-1069 with placeLayerVertices
+ * (:816-868): the new layer's vertices go to the barycentre of their
+ * placement-K nearest visible vertices, then become visible. */
+void gvizGRIPEmbedderNextStage(gvizGRIPState *s) {
+  if (s->currLayer == 0 || s->layerCount == 0)
+    return;
+  s->currLayer--;
+  s->currRound = 0;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  size_t begin = border_at(s, s->currLayer + 1);
+  size_t end = border_at(s, s->currLayer);
+  size_t placeCount = end - begin;
+  size_t n = gvizGraphSize(e->subgraph.g);
+  size_t visibleCount = gvizVertexSubsetCount(e->visibleVertices, n);
+  int batch = placeCount > 0 && visibleCount > 0 && visibleCount < placeCount &&
+              prefer_batch(s, visibleCount, placeCount);
+  if (push_positions(s) != 0)
+    return;
+  if (placeCount > 0 &&
+      gvizhip_grip_place(s->device, (int64_t)begin, (int64_t)end,
+                         e->visibleVertices, (int64_t)placement_k(s), batch) != 0)
+    return;
+  for (size_t i = begin; i < end; i++)
+    gvizEmbeddedGraphDrawMaskShowVertex(e, s->misFiltration[i]);
+  if (pull_positions(s) != 0)
+    return;
+  if (refresh_layer(s) != 0)
+    return;
+  sync_draw_mask(s);
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:1289-1333 */
+void gvizGRIPEmbedderRefineRound(gvizGRIPState *s) {
+  if (s->layerCount == 0)
+    return;
+  size_t layer = s->currLayer;
+  size_t active = border_at(s, layer);
+  if (push_positions(s) != 0)
+    return;
+  gvizhip_grip_round_stats st;
+  if (gvizhip_grip_refine_round(s->device, (int64_t)active, (int64_t)layer, &st) != 0)
+    return;
+  if (pull_positions(s) != 0)
+    return;
+  s->lastRoundStats.maxDisplacement = st.maxDisp;
+  s->lastRoundStats.meanDisplacement = st.meanDisp;
+  s->lastRoundStats.meanForce = st.meanForce;
+  if (s->statsEnabled) {
+    gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+    gvizEmbeddedGraphStatAppend(e, "grip.heat", st.meanHeat);
+    gvizEmbeddedGraphStatAppend(e, "grip.meanDisp", st.meanDisp);
+    gvizEmbeddedGraphStatAppend(e, "grip.maxDisp", st.maxDisp);
+    gvizEmbeddedGraphStatAppend(e, "grip.meanForce", st.meanForce);
+  }
+  s->currRound++;
+}
+
+/* src/embedders/gvizGRIPEmbedder.c:1266-1287 */
+int gvizGRIPEmbedderEmbed(gvizGRIPState *s) {
+  if (!s)
+    return -1;
+  gvizGRIPEmbedderBegin(s);
+  if (s->layerCount == 0)
+    return -1;
+  double epsilon = GRIP_EDGE_LENGTH * GRIP_EMBED_CONVERGENCE_FACTOR;
+  for (;;) {
+    for (size_t round = 0; round < GRIP_EMBED_MAX_ROUNDS_PER_LAYER; round++) {
+      gvizGRIPEmbedderRefineRound(s);
+      if (s->lastRoundStats.maxDisplacement < epsilon)
+        break;
+    }
+    if (s->currLayer == 0)
+      break;
+    gvizGRIPEmbedderNextStage(s);
+  }
+  return 0;
+}

@@ -1,0 +1,266 @@
+/*
+ * gviz_tutte_embedder.c -- gvizTutteEmbedder* with the Jacobi sweep on the
+ * GPU (reference: src/embedders/gvizTutteEmbedder.c).
+ *
+ * Boundary placement, interior seeding and the convex polygon are host
+ * bookkeeping done as the reference does them; every relaxation sweep runs
+ * on the device (gvizhip_tutte_sweeps), which keeps the live positions and
+ * the Jacobi snapshot in HBM. Positions cross the bus when the host changed
+ * them (boundary, seeding) and after each Step / Run. Gauss-Seidel sweeps
+ * and the planarity-based Begin / FixOuterFace are not part of this engine
+ * and return -1.
+ */
+#include "gviz/gvizTutteEmbedder.h"
+
+#include "gviz_hip.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static size_t num_vertices(const gvizTutteState *s) {
+  return gvizGraphSize(((const gvizEmbeddedGraph *)s)->subgraph.g);
+}
+
+static size_t dim_of(const gvizTutteState *s) {
+  return ((const gvizEmbeddedGraph *)s)->embedding.dim;
+}
+
+static void action_step(gvizEmbeddedGraph *embedding, void *userData,
+                        const gvizActionPayload *payload) {
+  (void)userData;
+  gvizTutteState *s = (gvizTutteState *)embedding;
+  if (!s->begun || !s->boundary)
+    return;
+  double dt = payload && payload->deltaTime > 0.0 ? payload->deltaTime : 1.0 / 60.0;
+  gvizTutteEmbedderStep(s, dt);
+}
+
+static void action_fix_outer_face(gvizEmbeddedGraph *embedding, void *userData,
+                                  const gvizActionPayload *payload) {
+  (void)userData;
+  (void)payload;
+  gvizTutteEmbedderFixOuterFace((gvizTutteState *)embedding);
+}
+
+/* src/embedders/gvizTutteEmbedder.c:55-103 */
+int gvizTutteEmbedderInit(gvizTutteState *s, gvizSubgraph subgraph,
+                          size_t dimension, double epsilon) {
+  memset(s, 0, sizeof(*s));
+  int res = gvizEmbeddedGraphInit((gvizEmbeddedGraph *)s, subgraph, dimension);
+  if (res < 0)
+    return res;
+  size_t n = gvizGraphSize(subgraph.g);
+  s->isBoundary = calloc(n ? n : 1, 1);
+  if (!s->isBoundary)
+    return -1;
+  s->epsilon = epsilon > 0.0 ? epsilon : GVIZ_TUTTE_DEFAULT_EPSILON;
+  s->relaxationRate = 5.0;
+  s->precision = GVIZHIP_PRECISION_FP64;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  if (gvizEmbeddedGraphAddAction(e, "tutte.step", action_step, NULL) < 0)
+    return -1;
+  if (gvizEmbeddedGraphAddAction(e, "tutte.fixOuterFace", action_fix_outer_face,
+                                 NULL) < 0)
+    return -1;
+  if (!gvizEmbeddedGraphAddStatSeries(e, "tutte.maxDelta", GVIZ_STAT_CHART_LINE_LOG))
+    return -1;
+  return 0;
+}
+
+void gvizTutteEmbedderRelease(gvizTutteState *s) {
+  if (!s)
+    return;
+  if (s->device)
+    gvizhip_tutte_destroy(s->device);
+  s->device = NULL;
+  free(s->boundary);
+  free(s->isBoundary);
+  s->boundary = NULL;
+  s->isBoundary = NULL;
+  gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
+}
+
+/* The reference's Begin embeds the graph in the plane (planarity test and
+ * rotation system, src/embedders/gvizTutteEmbedder.c:149-176) before fixing
+ * the largest face; that stage is not part of this engine. Callers fix a
+ * boundary with gvizTutteFixConvexPolygon or gvizTutteEmbedderSetBoundary. */
+int gvizTutteEmbedderBegin(gvizTutteState *s) {
+  (void)s;
+  return -1;
+}
+
+int gvizTutteEmbedderFixOuterFace(gvizTutteState *s) {
+  (void)s;
+  return -1;
+}
+
+/* src/embedders/gvizTutteEmbedder.c:178-205 */
+int gvizTutteEmbedderSetBoundary(gvizTutteState *s, const size_t *boundary,
+                                 size_t count, const double *positions) {
+  if (count < 3)
+    return -1;
+  size_t n = num_vertices(s), d = dim_of(s);
+  for (size_t i = 0; i < count; i++)
+    if (boundary[i] >= n)
+      return -1;
+  memset(s->isBoundary, 0, n);
+  free(s->boundary);
+  s->boundary = malloc(sizeof(size_t) * count);
+  if (!s->boundary) {
+    s->boundaryCount = 0;
+    return -1;
+  }
+  memcpy(s->boundary, boundary, sizeof(size_t) * count);
+  s->boundaryCount = count;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  for (size_t i = 0; i < count; i++) {
+    s->isBoundary[boundary[i]] = 1;
+    gvizEmbeddedGraphSetVPosition(e, boundary[i], positions + i * d);
+  }
+  s->devicePositionsFresh = 0;
+  return 0;
+}
+
+/* src/embedders/gvizTutteEmbedder.c:207-230: every non-boundary vertex at
+ * the boundary centroid. */
+void gvizTutteEmbedderSeedInterior(gvizTutteState *s) {
+  size_t n = num_vertices(s), d = dim_of(s);
+  double *centroid = calloc(d, sizeof(double));
+  if (!centroid)
+    return;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  for (size_t i = 0; i < s->boundaryCount; i++) {
+    const double *p = gvizEmbeddedGraphGetVPosition(e, s->boundary[i]);
+    for (size_t k = 0; k < d; k++)
+      centroid[k] += p[k];
+  }
+  if (s->boundaryCount > 0)
+    for (size_t k = 0; k < d; k++)
+      centroid[k] /= (double)s->boundaryCount;
+  for (size_t u = 0; u < n; u++)
+    if (!s->isBoundary[u])
+      gvizEmbeddedGraphSetVPosition(e, u, centroid);
+  free(centroid);
+  s->iteration = 0;
+  s->lastMaxDelta = 0.0;
+  s->converged = 0;
+  s->devicePositionsFresh = 0;
+}
+
+/* src/embedders/gvizTutteEmbedder.c:232-249 */
+void gvizTutteFixConvexPolygon(gvizTutteState *s, const size_t *boundary,
+                               size_t count, double radius) {
+  size_t d = dim_of(s);
+  double *positions = calloc(count > 0 ? count * d : 1, sizeof(double));
+  if (!positions)
+    return;
+  for (size_t k = 0; k < count; k++) {
+    double angle = 2.0 * M_PI * (double)k / (double)count;
+    positions[k * d] = radius * cos(angle);
+    if (d > 1)
+      positions[k * d + 1] = radius * sin(angle);
+  }
+  gvizTutteEmbedderSetBoundary(s, boundary, count, positions);
+  free(positions);
+}
+
+/* Device engine over parent ids with the subgraph's edges, created on the
+ * first sweep; positions re-uploaded whenever the host changed them. */
+static int ensure_device(gvizTutteState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  if (dim_of(s) != 2)
+    return -1;
+  if (!s->device) {
+    int64_t *off = NULL, *adj = NULL;
+    if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
+      return -1;
+    int rc = gvizhip_tutte_create(&s->device, (int64_t)num_vertices(s), off,
+                                  adj, s->precision);
+    free(off);
+    free(adj);
+    if (rc != 0) {
+      s->device = NULL;
+      return -1;
+    }
+    s->devicePositionsFresh = 0;
+  }
+  if (!s->devicePositionsFresh) {
+    if (gvizhip_tutte_upload(s->device, e->embedding.vertexPositions,
+                             s->isBoundary) != 0)
+      return -1;
+    s->devicePositionsFresh = 1;
+  }
+  return 0;
+}
+
+static void record_sweep(gvizTutteState *s, double maxDelta) {
+  s->iteration++;
+  s->lastMaxDelta = maxDelta;
+  if (maxDelta < s->epsilon)
+    s->converged = 1;
+  gvizEmbeddedGraphStatAppend((gvizEmbeddedGraph *)s, "tutte.maxDelta", maxDelta);
+}
+
+/* src/embedders/gvizTutteEmbedder.c:305-335 (Jacobi). Host positions are
+ * re-uploaded before the sweep (a caller may have moved vertices) and read
+ * back after it. */
+double gvizTutteEmbedderStep(gvizTutteState *s, double dt) {
+  double alpha = s->relaxationRate * dt;
+  if (alpha <= 0.0)
+    return 0.0;
+  if (alpha > 1.0)
+    alpha = 1.0;
+  if (s->useGaussSeidel)
+    return -1.0;
+  s->devicePositionsFresh = 0;
+  if (ensure_device(s) != 0)
+    return -1.0;
+  double maxDelta = 0.0;
+  if (gvizhip_tutte_sweeps(s->device, 1, alpha, -1.0, &maxDelta) != 1)
+    return -1.0;
+  if (gvizhip_tutte_download(s->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
+    return -1.0;
+  record_sweep(s, maxDelta);
+  return maxDelta;
+}
+
+/* src/embedders/gvizTutteEmbedder.c:337-345: sweeps with dt = 1 until the
+ * max delta drops below epsilon or the iteration count reaches maxIters; the
+ * loop and the stop test run on the device. Returns the iteration count. */
+int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters) {
+  if (!s || !s->boundary)
+    return -1;
+  if (s->converged || s->iteration >= maxIters)
+    return (int)s->iteration;
+  double alpha = s->relaxationRate;
+  if (alpha <= 0.0)
+    return (int)s->iteration; /* the reference's loop never advances here */
+  if (alpha > 1.0)
+    alpha = 1.0;
+  if (s->useGaussSeidel)
+    return -1;
+  s->devicePositionsFresh = 0;
+  if (ensure_device(s) != 0)
+    return -1;
+  enum { kChunk = 65536 };
+  double *deltas = malloc(sizeof(double) * kChunk);
+  if (!deltas)
+    return -1;
+  int failed = 0;
+  while (!s->converged && s->iteration < maxIters) {
+    size_t left = maxIters - s->iteration;
+    int64_t want = (int64_t)(left < kChunk ? left : kChunk);
+    int64_t ran = gvizhip_tutte_sweeps(s->device, want, alpha, s->epsilon, deltas);
+    if (ran <= 0) {
+      failed = 1;
+      break;
+    }
+    for (int64_t i = 0; i < ran; i++)
+      record_sweep(s, deltas[i]);
+  }
+  free(deltas);
+  if (gvizhip_tutte_download(s->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
+    failed = 1;
+  return failed ? -1 : (int)s->iteration;
+}

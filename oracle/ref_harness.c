@@ -385,6 +385,103 @@ static int mode_grip(void) {
   return 0;
 }
 
+/* ------------------------------------------------------- embedder API runs */
+static void write_series(gvizEmbeddedGraph *e, const char *name) {
+  const gvizStatSeries *s = gvizEmbeddedGraphFindStatSeries(e, name);
+  wr_i(s ? (int64_t)s->count : -1);
+  if (s)
+    for (size_t i = 0; i < s->count; i++)
+      wr_d(s->samples[i]);
+}
+
+/* Force embedder through its public API on a vertex-induced subgraph:
+ * Init, Configure*, Begin(seed) (the reference's own random placement),
+ * Run(maxIters, epsilon), then one more Step. */
+static int mode_api_force(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t nHidden = rd_i();
+  gvizVertexSubset vs = gvizVertexSubsetCreateEmpty(&g);
+  for (int64_t i = 0; i < n; i++)
+    gvizVertexSubsetShowVertex(vs, (size_t)i);
+  for (int64_t i = 0; i < nHidden; i++)
+    gvizVertexSubsetHideVertex(vs, (size_t)rd_i());
+  int64_t model = rd_i(), bh = rd_i(), overlap = rd_i(), npc = rd_i();
+  int64_t seed = rd_i(), runIters = rd_i();
+  double edgeLength = rd_d(), theta = rd_d(), gravityK = rd_d();
+  double overlapC = rd_d(), rBase = rd_d(), rPerDeg = rd_d(), jitter = rd_d();
+  double eps = rd_d();
+  gvizForceEmbedderState s;
+  if (gvizForceEmbedderInit(&s, gvizSubgraphCreateVertexInduced(&g, vs), 2,
+                            (gvizForceModelKind)model) != 0)
+    return 1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
+  gvizForceEmbedderConfigure(&s, edgeLength, 0.0);
+  gvizForceEmbedderConfigureSpeed(&s, jitter);
+  gvizForceEmbedderConfigureBarnesHut(&s, theta, (size_t)npc);
+  gvizForceEmbedderSetBarnesHutEnabled(&s, (int)bh);
+  gvizForceEmbedderConfigureGravity(&s, gravityK);
+  gvizForceEmbedderConfigureRadius(&s, rBase, rPerDeg);
+  gvizForceEmbedderConfigureOverlapPrevention(&s, overlapC);
+  gvizForceEmbedderSetPreventOverlapEnabled(&s, (int)overlap);
+  wr_d(s.boxExtent);
+  wr_i(gvizForceEmbedderBegin(&s, (unsigned int)seed));
+  const double *p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  wr_i(gvizForceEmbedderRun(&s, (size_t)runIters, eps));
+  static const char *const names[] = {
+      "forceEmbedder.maxDisp", "forceEmbedder.speed",
+      "forceEmbedder.attractiveForce", "forceEmbedder.repulsiveForce",
+      "forceEmbedder.gravityForce"};
+  for (int k = 0; k < 5; k++)
+    write_series(e, names[k]);
+  p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  wr_d(gvizForceEmbedderStep(&s));
+  p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  wr_d(s.globalSpeed);
+  wr_d(s.speedEfficiency);
+  wr_i((int64_t)s.iteration);
+  gvizForceEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
+/* gvizGRIPEmbedderEmbed end to end with a K policy. */
+static int mode_api_grip(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i();
+  gvizGRIPState st;
+  memset(&st, 0, sizeof(st));
+  gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+    return 1;
+  gvizGRIPEmbedderConfigureK(&st, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
+  wr_i(gvizGRIPEmbedderEmbed(&st));
+  wr_i((int64_t)st.layerCount);
+  for (size_t i = 0; i < st.layerCount; i++)
+    wr_i((int64_t)*(size_t *)gvizArrayAtIndex(&st.misBorder, i));
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
+  const double *p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  static const char *const names[] = {"grip.heat", "grip.meanDisp",
+                                      "grip.maxDisp", "grip.meanForce"};
+  for (int k = 0; k < 4; k++)
+    write_series(e, names[k]);
+  for (int64_t v = 0; v < n; v++)
+    wr_i(gvizEmbeddedGraphIsVertexVisible(e, (size_t)v) ? 1 : 0);
+  gvizGRIPEmbedderRelease(&st);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 4) {
     fprintf(stderr, "usage: %s mode in out\n", argv[0]);
@@ -407,6 +504,10 @@ int main(int argc, char **argv) {
     rc = mode_knn();
   else if (!strcmp(argv[1], "grip"))
     rc = mode_grip();
+  else if (!strcmp(argv[1], "api_force"))
+    rc = mode_api_force();
+  else if (!strcmp(argv[1], "api_grip"))
+    rc = mode_api_grip();
   fclose(fin);
   fclose(fout);
   return rc;

@@ -149,11 +149,44 @@ def grip_cases():
     grip_case("grip_ba200", n, e, 10, 1)
 
 
+def api_force_case(name, n, e, hidden, seed, run_iters, epsilon, **kw):
+    r = refio.api_force(n, e, hidden, seed=seed, run_iters=run_iters,
+                        epsilon=epsilon, **kw)
+    params = {"param_" + k: v for k, v in kw.items()}
+    save(name, n=n, edges=e, hidden=hidden, seed=seed, run_iters=run_iters,
+         epsilon=epsilon, **params, **r)
+
+
+def api_cases():
+    """Embedder API runs end to end (Begin's own random placement included)."""
+    n, e = G.barabasi_albert(400, 3, 21)
+    api_force_case("api_force_linlog_ba400", n, e,
+                   np.array([3, 17, 250, 399], dtype=np.int64), 777, 40, 1e-3,
+                   model=1, barnes_hut=1, prevent_overlap=1, npc=1,
+                   edge_length=10.0, theta=0.7, gravity=0.05,
+                   overlap_constant=100.0, radius_base=0.1,
+                   radius_per_degree=1.0, jitter=1.0)
+    n, e = G.grid(20, 20)
+    api_force_case("api_force_fr_grid20", n, e, np.zeros(0, np.int64), 4242, 300,
+                   0.5, model=0, barnes_hut=1, prevent_overlap=0, npc=1,
+                   edge_length=10.0, theta=1.0, gravity=0.0,
+                   overlap_constant=100.0, radius_base=0.0,
+                   radius_per_degree=0.0, jitter=1.0)
+    n, e, _ = G.random_geometric(400, 8.0, 5)
+    r = refio.api_grip(n, e, 12, 12, 12, 0)
+    save("api_grip_rgg400_constant", n=n, edges=e, kcap=12, placement_k=12,
+         refinement_k=12, policy=0, **r)
+    n, e = G.grid(16, 16)
+    r = refio.api_grip(n, e, 16, 16, 8, 1)
+    save("api_grip_grid16_decay", n=n, edges=e, kcap=16, placement_k=16,
+         refinement_k=8, policy=1, **r)
+
+
+CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
+             knn=knn_cases, grip=grip_cases, api=api_cases)
+
 if __name__ == "__main__":
     if not refio.harness_available():
         sys.exit("oracle/_ref/ref_harness is not built: make -C oracle ref")
-    quadtree_cases()
-    force_cases()
-    tutte_cases()
-    knn_cases()
-    grip_cases()
+    for name in (sys.argv[1:] or CASES):
+        CASES[name]()

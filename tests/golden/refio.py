@@ -72,13 +72,15 @@ class _R:
         return a
 
 
-def _run(mode, payload: bytes) -> bytes:
+def _run(mode, payload: bytes, exe=None) -> bytes:
+    """Runs `exe mode in out` (default: the reference harness); the same
+    framing drives tests/host/api_driver over this repository's library."""
     with tempfile.TemporaryDirectory() as td:
         fi = os.path.join(td, "in.bin")
         fo = os.path.join(td, "out.bin")
         with open(fi, "wb") as f:
             f.write(payload)
-        subprocess.run([HARNESS, mode, fi, fo], check=True)
+        subprocess.run([exe or HARNESS, mode, fi, fo], check=True)
         with open(fo, "rb") as f:
             return f.read()
 
@@ -142,13 +144,13 @@ def force(n, edges, pos, *, model, barnes_hut, prevent_overlap, npc, iters,
     return np.asarray(stats), dumps
 
 
-def tutte(n, edges, boundary, iters, radius, dt, gauss_seidel=False):
+def tutte(n, edges, boundary, iters, radius, dt, gauss_seidel=False, exe=None):
     w = _W()
     w.graph(n, edges)
     w.i(len(boundary), iters, 1 if gauss_seidel else 0)
     w.ia(np.asarray(boundary, dtype=np.int64))
     w.d(radius, dt)
-    r = _R(_run("tutte", w.bytes()))
+    r = _R(_run("tutte", w.bytes(), exe))
     p0 = r.da(2 * n).reshape(n, 2)
     deltas = r.da(iters)
     p1 = r.da(2 * n).reshape(n, 2)
@@ -208,3 +210,48 @@ def grip(n, edges, kcap, kmax, rounds_per_layer):
         rounds.append(dict(layer=layer, pos0=pos0, dec0=dec0, knn=knnl,
                            pos1=pos1, dec1=dec1, stats=st))
     return borders, filtration, rounds
+
+
+def api_force(n, edges, hidden, *, model, barnes_hut, prevent_overlap, npc, seed,
+              run_iters, edge_length, theta, gravity, overlap_constant,
+              radius_base, radius_per_degree, jitter, epsilon, exe=None):
+    """gvizForceEmbedderInit/Configure*/Begin(seed)/Run/Step on the subgraph
+    induced by every vertex but `hidden`."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(len(hidden))
+    w.ia(hidden)
+    w.i(model, barnes_hut, prevent_overlap, npc, seed, run_iters)
+    w.d(edge_length, theta, gravity, overlap_constant, radius_base,
+        radius_per_degree, jitter, epsilon)
+    r = _R(_run("api_force", w.bytes(), exe))
+    out = dict(box_extent=r.d(), begin_rc=r.i(), pos_begin=r.da(2 * n).reshape(n, 2))
+    out["run_rc"] = r.i()
+    for name in ("maxDisp", "speed", "attractiveForce", "repulsiveForce",
+                 "gravityForce"):
+        c = r.i()
+        out["series_" + name] = r.da(c)
+    out["pos_run"] = r.da(2 * n).reshape(n, 2)
+    out["step_ret"] = r.d()
+    out["pos_step"] = r.da(2 * n).reshape(n, 2)
+    out["speed"] = r.d()
+    out["efficiency"] = r.d()
+    out["iteration"] = r.i()
+    return out
+
+
+def api_grip(n, edges, kcap, placement_k, refinement_k, policy, exe=None):
+    """gvizGRIPEmbedderEmbed with ConfigureKnnCapacity / ConfigureK."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(kcap, placement_k, refinement_k, policy)
+    r = _R(_run("api_grip", w.bytes(), exe))
+    out = dict(embed_rc=r.i())
+    layers = r.i()
+    out["borders"] = r.ia(layers)
+    out["pos"] = r.da(2 * n).reshape(n, 2)
+    for name in ("heat", "meanDisp", "maxDisp", "meanForce"):
+        c = r.i()
+        out["series_" + name] = r.da(c)
+    out["visible"] = r.ia(n)
+    return out

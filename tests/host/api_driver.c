@@ -1,0 +1,556 @@
+/*
+ * api_driver.c -- drives this repository's C11 host layer (libgviz.so,
+ * include/gviz/*.h) for the tests.
+ *
+ *   api_driver unit                 host-side unit checks (no GPU needed)
+ *   api_driver <mode> <in> <out>    mode = api_force | api_grip | tutte
+ *
+ * The three run modes read and write the same framing as the reference-side
+ * harness that wrote the golden fixtures (tests/golden/refio.py), so a test
+ * feeds the fixture's inputs here and compares the outputs with the
+ * reference's. GVIZ_TEST_PRECISION = fp64 (default) | mixed | fp32 selects
+ * the force precision; GVIZ_TEST_RESIDENT=1 keeps force positions on the
+ * device between Begin and the end of Run.
+ */
+#include "gviz/gvizForceEmbedder.h"
+#include "gviz/gvizGRIPEmbedder.h"
+#include "gviz/gvizTutteEmbedder.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+static FILE *fin, *fout;
+
+static int64_t rd_i(void) {
+  int64_t v;
+  if (fread(&v, 8, 1, fin) != 1) {
+    fprintf(stderr, "short read\n");
+    exit(2);
+  }
+  return v;
+}
+static double rd_d(void) {
+  double v;
+  if (fread(&v, 8, 1, fin) != 1) {
+    fprintf(stderr, "short read\n");
+    exit(2);
+  }
+  return v;
+}
+static void wr_i(int64_t v) { fwrite(&v, 8, 1, fout); }
+static void wr_d(double v) { fwrite(&v, 8, 1, fout); }
+
+static void read_graph(gvizGraph *g, int64_t *nOut) {
+  int64_t n = rd_i(), m = rd_i();
+  gvizGraphInit(g, 0);
+  for (int64_t i = 0; i < n; i++)
+    gvizGraphAddVertex(g, NULL, NULL, NULL);
+  for (int64_t e = 0; e < m; e++) {
+    int64_t u = rd_i(), v = rd_i();
+    gvizGraphAddEdge(g, (size_t)u, (size_t)v);
+  }
+  *nOut = n;
+}
+
+static gvizForcePrecision env_precision(void) {
+  const char *p = getenv("GVIZ_TEST_PRECISION");
+  if (p && !strcmp(p, "mixed"))
+    return GVIZ_FORCE_PRECISION_MIXED;
+  if (p && !strcmp(p, "fp32"))
+    return GVIZ_FORCE_PRECISION_FP32;
+  return GVIZ_FORCE_PRECISION_FP64;
+}
+
+static void write_series(gvizEmbeddedGraph *e, const char *name) {
+  const gvizStatSeries *s = gvizEmbeddedGraphFindStatSeries(e, name);
+  wr_i(s ? (int64_t)s->count : -1);
+  if (s)
+    for (size_t i = 0; i < s->count; i++)
+      wr_d(s->samples[i]);
+}
+
+static int fail_device(const char *what) {
+  fprintf(stderr, "%s failed: %s\n", what, gvizForceEmbedderLastError());
+  return 1;
+}
+
+static int mode_api_force(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t nHidden = rd_i();
+  gvizVertexSubset vs = gvizVertexSubsetCreateEmpty(&g);
+  for (int64_t i = 0; i < n; i++)
+    gvizVertexSubsetShowVertex(vs, (size_t)i);
+  for (int64_t i = 0; i < nHidden; i++)
+    gvizVertexSubsetHideVertex(vs, (size_t)rd_i());
+  int64_t model = rd_i(), bh = rd_i(), overlap = rd_i(), npc = rd_i();
+  int64_t seed = rd_i(), runIters = rd_i();
+  double edgeLength = rd_d(), theta = rd_d(), gravityK = rd_d();
+  double overlapC = rd_d(), rBase = rd_d(), rPerDeg = rd_d(), jitter = rd_d();
+  double eps = rd_d();
+  gvizForceEmbedderState s;
+  if (gvizForceEmbedderInit(&s, gvizSubgraphCreateVertexInduced(&g, vs), 2,
+                            (gvizForceModelKind)model) != 0)
+    return 1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
+  gvizForceEmbedderSetPrecision(&s, env_precision());
+  int resident = getenv("GVIZ_TEST_RESIDENT") && atoi(getenv("GVIZ_TEST_RESIDENT"));
+  gvizForceEmbedderConfigure(&s, edgeLength, 0.0);
+  gvizForceEmbedderConfigureSpeed(&s, jitter);
+  gvizForceEmbedderConfigureBarnesHut(&s, theta, (size_t)npc);
+  gvizForceEmbedderSetBarnesHutEnabled(&s, (int)bh);
+  gvizForceEmbedderConfigureGravity(&s, gravityK);
+  gvizForceEmbedderConfigureRadius(&s, rBase, rPerDeg);
+  gvizForceEmbedderConfigureOverlapPrevention(&s, overlapC);
+  gvizForceEmbedderSetPreventOverlapEnabled(&s, (int)overlap);
+  wr_d(s.boxExtent);
+  int rc = gvizForceEmbedderBegin(&s, (unsigned int)seed);
+  if (rc != 0)
+    return fail_device("gvizForceEmbedderBegin");
+  wr_i(rc);
+  const double *p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  gvizForceEmbedderSetResidentPositions(&s, resident);
+  int rounds = gvizForceEmbedderRun(&s, (size_t)runIters, eps);
+  if (rounds < 0)
+    return fail_device("gvizForceEmbedderRun");
+  if (resident && gvizForceEmbedderSyncPositions(&s) != 0)
+    return fail_device("gvizForceEmbedderSyncPositions");
+  gvizForceEmbedderSetResidentPositions(&s, 0);
+  wr_i(rounds);
+  static const char *const names[] = {
+      "forceEmbedder.maxDisp", "forceEmbedder.speed",
+      "forceEmbedder.attractiveForce", "forceEmbedder.repulsiveForce",
+      "forceEmbedder.gravityForce"};
+  for (int k = 0; k < 5; k++)
+    write_series(e, names[k]);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  double d = gvizForceEmbedderStep(&s);
+  if (d < 0.0)
+    return fail_device("gvizForceEmbedderStep");
+  wr_d(d);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  wr_d(s.globalSpeed);
+  wr_d(s.speedEfficiency);
+  wr_i((int64_t)s.iteration);
+  gvizForceEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
+static int mode_api_grip(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i();
+  gvizGRIPState st;
+  memset(&st, 0, sizeof(st));
+  gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+    return 1;
+  gvizGRIPEmbedderConfigureK(&st, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
+  int rc = gvizGRIPEmbedderEmbed(&st);
+  if (rc != 0)
+    return fail_device("gvizGRIPEmbedderEmbed");
+  wr_i(rc);
+  wr_i((int64_t)st.layerCount);
+  for (size_t i = 0; i < st.layerCount; i++)
+    wr_i((int64_t)st.misBorder[i]);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
+  const double *p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  static const char *const names[] = {"grip.heat", "grip.meanDisp",
+                                      "grip.maxDisp", "grip.meanForce"};
+  for (int k = 0; k < 4; k++)
+    write_series(e, names[k]);
+  for (int64_t v = 0; v < n; v++)
+    wr_i(gvizEmbeddedGraphIsVertexVisible(e, (size_t)v) ? 1 : 0);
+  gvizGRIPEmbedderRelease(&st);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
+static int mode_tutte(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t nb = rd_i(), iters = rd_i(), gs = rd_i();
+  size_t *boundary = malloc(sizeof(size_t) * nb);
+  for (int64_t i = 0; i < nb; i++)
+    boundary[i] = (size_t)rd_i();
+  double radius = rd_d(), dt = rd_d();
+  gvizTutteState s;
+  if (gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, 1e-300) != 0)
+    return 1;
+  s.useGaussSeidel = (int)gs;
+  gvizTutteFixConvexPolygon(&s, boundary, (size_t)nb, radius);
+  gvizTutteEmbedderSeedInterior(&s);
+  const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  for (int64_t it = 0; it < iters; it++) {
+    double d = gvizTutteEmbedderStep(&s, dt);
+    if (d < 0.0)
+      return fail_device("gvizTutteEmbedderStep");
+    wr_d(d);
+  }
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  gvizTutteEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  free(boundary);
+  return 0;
+}
+
+/* ------------------------------------------------------------ unit checks */
+
+static int failures;
+#define CHECK(cond)                                                        \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      fprintf(stderr, "%s:%d: %s: check failed: %s\n", __FILE__, __LINE__, \
+              __func__, #cond);                                            \
+      failures++;                                                          \
+    }                                                                      \
+  } while (0)
+
+static void make_path(gvizGraph *g, size_t n) {
+  gvizGraphInit(g, 0);
+  for (size_t i = 0; i < n; i++)
+    gvizGraphAddVertex(g, NULL, NULL, NULL);
+  for (size_t i = 0; i + 1 < n; i++)
+    gvizGraphAddEdge(g, i, i + 1);
+}
+
+static void test_graph_insertionOrderAndCSR(void) {
+  gvizGraph g;
+  gvizGraphInit(&g, 0);
+  for (int i = 0; i < 5; i++)
+    gvizGraphAddVertex(&g, NULL, NULL, NULL);
+  gvizGraphAddEdge(&g, 0, 3);
+  gvizGraphAddEdge(&g, 0, 1);
+  gvizGraphAddEdge(&g, 2, 0);
+  gvizGraphAddEdge(&g, 3, 4);
+  CHECK(gvizGraphSize(&g) == 5);
+  CHECK(gvizGraphEdgeCount(&g) == 4);
+  size_t c;
+  const size_t *nb = gvizGraphNeighbors(&g, 0, &c);
+  CHECK(c == 3 && nb[0] == 3 && nb[1] == 1 && nb[2] == 2);
+  CHECK(gvizGraphAddEdge(&g, 0, 9) == -1);
+
+  gvizVertexSubset vs = gvizVertexSubsetCreateEmpty(&g);
+  gvizVertexSubsetShowVertex(vs, 0);
+  gvizVertexSubsetShowVertex(vs, 2);
+  gvizVertexSubsetShowVertex(vs, 3);
+  gvizSubgraph sg = gvizSubgraphCreateVertexInduced(&g, vs);
+  CHECK(gvizSubgraphVertexCount(&sg) == 3);
+  CHECK(gvizSubgraphDegree(&sg, 0) == 2);
+  CHECK(gvizSubgraphDegree(&sg, 1) == 0);
+  CHECK(gvizSubgraphEdgeCount(&sg) == 2);
+  int64_t *off, *adj;
+  size_t *compact, count;
+  CHECK(gvizSubgraphToCompactCSR(&sg, &off, &adj, &compact, &count) == 0);
+  CHECK(count == 3 && compact[0] == 0 && compact[1] == 2 && compact[2] == 3);
+  /* vertex 0 -> (3, 2) in insertion order: compact ids 2, 1 */
+  CHECK(off[0] == 0 && off[1] == 2 && adj[0] == 2 && adj[1] == 1);
+  CHECK(off[3] == 4);
+  free(off);
+  free(adj);
+  free(compact);
+  CHECK(gvizSubgraphToCSR(&sg, &off, &adj) == 0);
+  CHECK(off[1] - off[0] == 2 && off[2] == off[1] && off[5] == 4);
+  free(off);
+  free(adj);
+  gvizSubgraphRelease(&sg);
+  gvizGraphRelease(&g);
+}
+
+static int actionCalls;
+static void count_action(gvizEmbeddedGraph *e, void *u, const gvizActionPayload *p) {
+  (void)e;
+  actionCalls += *(int *)u + (int)p->iarg;
+}
+
+static void test_embeddedGraph_actionsStatsMask(void) {
+  gvizGraph g;
+  make_path(&g, 4);
+  gvizEmbeddedGraph e;
+  CHECK(gvizEmbeddedGraphInit(&e, gvizSubgraphCreateFull(&g), 2) == 0);
+  for (size_t u = 0; u < 4; u++) {
+    const double *p = gvizEmbeddedGraphGetVPosition(&e, u);
+    CHECK(p[0] == 0.0 && p[1] == 0.0);
+  }
+  int one = 1, ten = 10;
+  CHECK(gvizEmbeddedGraphAddAction(&e, "a", count_action, &one) == 0);
+  CHECK(gvizEmbeddedGraphAddAction(&e, "b", count_action, &one) == 0);
+  CHECK(gvizEmbeddedGraphAddAction(&e, "a", count_action, &ten) == 0);
+  CHECK(gvizEmbeddedGraphActionCount(&e) == 2);
+  CHECK(gvizEmbeddedGraphInvokeAction(&e, "a", NULL) == 0 && actionCalls == 10);
+  gvizActionPayload pl = {0};
+  pl.iarg = 5;
+  CHECK(gvizEmbeddedGraphInvokeAction(&e, "b", &pl) == 0 && actionCalls == 16);
+  CHECK(gvizEmbeddedGraphInvokeAction(&e, "zz", NULL) == -1);
+  CHECK(gvizEmbeddedGraphRemoveAction(&e, "a") == 0);
+  CHECK(gvizEmbeddedGraphRemoveAction(&e, "a") == -1);
+  CHECK(gvizEmbeddedGraphActionAt(&e, 0) && !gvizEmbeddedGraphActionAt(&e, 1));
+
+  CHECK(gvizEmbeddedGraphAddStatSeries(&e, "s", GVIZ_STAT_CHART_LINE_LOG) != NULL);
+  CHECK(gvizEmbeddedGraphStatAppend(&e, "s", 1.5) == 0);
+  CHECK(gvizEmbeddedGraphStatAppend(&e, "new", 2.5) == 0);
+  const gvizStatSeries *sn = gvizEmbeddedGraphFindStatSeries(&e, "new");
+  CHECK(sn && sn->kind == GVIZ_STAT_CHART_LINE && sn->count == 1 && sn->samples[0] == 2.5);
+  for (int i = 0; i < 1000; i++)
+    gvizEmbeddedGraphStatAppend(&e, "s", (double)i);
+  const gvizStatSeries *ss = gvizEmbeddedGraphFindStatSeries(&e, "s");
+  CHECK(ss->count == 1001 && ss->samples[1000] == 999.0);
+  gvizEmbeddedGraphStatClear(&e, "s");
+  CHECK(ss->count == 0 && gvizEmbeddedGraphStatSeriesCount(&e) == 2);
+
+  uint64_t r0 = gvizEmbeddedGraphDrawMaskRevision(&e);
+  CHECK(gvizEmbeddedGraphIsVertexVisible(&e, 3));
+  gvizEmbeddedGraphDrawMaskHideVertex(&e, 3);
+  CHECK(!gvizEmbeddedGraphIsVertexVisible(&e, 3));
+  CHECK(gvizEmbeddedGraphDrawMaskRevision(&e) == r0);
+  CHECK(gvizEmbeddedGraphIsEdgeVisible(&e, 2, 3));
+  gvizEmbeddedGraphSetDrawMaskEdgePolicy(&e, GVIZ_DRAW_EDGES_IF_BOTH_VISIBLE);
+  CHECK(gvizEmbeddedGraphDrawMaskRevision(&e) == r0 + 1);
+  CHECK(!gvizEmbeddedGraphIsEdgeVisible(&e, 2, 3));
+  CHECK(gvizEmbeddedGraphIsEdgeVisible(&e, 1, 2));
+  CHECK(!gvizEmbeddedGraphIsEdgeVisible(&e, 0, 3));
+  gvizEmbeddedGraphSetDrawMaskEdgePolicy(&e, GVIZ_DRAW_EDGES_NONE);
+  CHECK(!gvizEmbeddedGraphIsEdgeVisible(&e, 1, 2));
+  gvizEmbeddedGraphResetDrawMask(&e);
+  CHECK(gvizEmbeddedGraphIsVertexVisible(&e, 3) && gvizEmbeddedGraphIsEdgeVisible(&e, 0, 1));
+  CHECK(!gvizEmbeddedGraphIsEdgeVisible(&e, 0, 2));
+  gvizEmbeddedGraphRelease(&e);
+  gvizGraphRelease(&g);
+}
+
+static void test_embeddedGraph_randomizeSaveLoad(void) {
+  gvizGraph g;
+  make_path(&g, 50);
+  gvizEmbeddedGraph e, f;
+  gvizEmbeddedGraphInit(&e, gvizSubgraphCreateFull(&g), 2);
+  gvizEmbeddedGraphInit(&f, gvizSubgraphCreateFull(&g), 2);
+  gvizEmbeddedGraphRandomizePositions(&e, 7.0, 99);
+  gvizEmbeddedGraphRandomizePositions(&f, 7.0, 99);
+  int inBox = 1, same = 1, spread = 0;
+  for (size_t u = 0; u < 50; u++) {
+    const double *p = gvizEmbeddedGraphGetVPosition(&e, u);
+    const double *q = gvizEmbeddedGraphGetVPosition(&f, u);
+    inBox &= fabs(p[0]) <= 7.0 && fabs(p[1]) <= 7.0;
+    same &= p[0] == q[0] && p[1] == q[1];
+    spread |= fabs(p[0]) > 3.5;
+  }
+  CHECK(inBox && same && spread);
+  char path[] = "/tmp/gviz_embedding_XXXXXX";
+  int fd = mkstemp(path);
+  CHECK(fd >= 0);
+  if (fd >= 0)
+    close(fd);
+  CHECK(gvizEmbeddedGraphSaveEmbedding(&e, "layout", path) == 0);
+  gvizEmbeddedGraphRandomizePositions(&f, 1.0, 5);
+  CHECK(gvizEmbeddedGraphLoadEmbedding(&f, path) == 0);
+  double worst = 0.0;
+  for (size_t u = 0; u < 50; u++) {
+    const double *p = gvizEmbeddedGraphGetVPosition(&e, u);
+    const double *q = gvizEmbeddedGraphGetVPosition(&f, u);
+    worst = fmax(worst, fmax(fabs(p[0] - q[0]), fabs(p[1] - q[1])));
+  }
+  CHECK(worst <= 5e-7); /* "%f" keeps six decimals */
+  unlink(path);
+  gvizEmbeddedGraph h;
+  gvizGraph g2;
+  make_path(&g2, 3);
+  gvizEmbeddedGraphInit(&h, gvizSubgraphCreateFull(&g2), 2);
+  CHECK(gvizEmbeddedGraphSaveEmbedding(&e, "x", path) == 0);
+  CHECK(gvizEmbeddedGraphLoadEmbedding(&h, path) == -1);
+  unlink(path);
+  gvizEmbeddedGraphRelease(&h);
+  gvizEmbeddedGraphRelease(&e);
+  gvizEmbeddedGraphRelease(&f);
+  gvizGraphRelease(&g);
+  gvizGraphRelease(&g2);
+}
+
+/* Mirrors tests/embedders/gvizForceEmbedderTests.c (init, configure). */
+static void test_forceEmbedder_initConfigure(void) {
+  gvizGraph g;
+  make_path(&g, 10);
+  gvizForceEmbedderState s;
+  CHECK(gvizForceEmbedderInit(&s, gvizSubgraphCreateFull(&g), 3,
+                              GVIZ_FORCE_MODEL_FRUCHTERMAN_REINGOLD) == -2);
+  CHECK(gvizForceEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2,
+                              GVIZ_FORCE_MODEL_LINLOG) == 0);
+  CHECK(s.vertexCount == 10);
+  CHECK(s.edgeLength == GVIZ_FORCE_EMBEDDER_EDGE_LENGTH_DEFAULT);
+  CHECK(s.boxExtent == 0.5 * sqrt(10.0) * 10.0);
+  CHECK(s.theta == GVIZ_FORCE_EMBEDDER_THETA_DEFAULT && s.nodesPerCell == 1);
+  CHECK(s.barnesHutEnabled == 1 && s.preventOverlap == 0 && s.gravityK == 0.0);
+  CHECK(s.mass[0] == 2.0 && s.mass[5] == 3.0 && s.degree[5] == 2);
+  CHECK(s.meanDegreePlusOne == (2.0 * 2 + 3.0 * 8) / 10.0);
+  CHECK(s.precision == GVIZ_FORCE_PRECISION_MIXED && s.device == NULL);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
+  CHECK(gvizEmbeddedGraphFindAction(e, "forceEmbedder.step") != NULL);
+  CHECK(gvizEmbeddedGraphFindAction(e, "forceEmbedder.toggleOverlapPrevention") != NULL);
+  const char *series[] = {"forceEmbedder.maxDisp", "forceEmbedder.speed",
+                          "forceEmbedder.attractiveForce",
+                          "forceEmbedder.repulsiveForce",
+                          "forceEmbedder.gravityForce"};
+  for (int i = 0; i < 5; i++) {
+    const gvizStatSeries *ss = gvizEmbeddedGraphFindStatSeries(e, series[i]);
+    CHECK(ss && ss->kind == GVIZ_STAT_CHART_LINE_LOG && ss->count == 0);
+  }
+  gvizForceEmbedderConfigure(&s, 0.0, -1.0);
+  CHECK(s.edgeLength == 10.0 && s.boxExtent == 0.5 * sqrt(10.0) * 10.0);
+  gvizForceEmbedderConfigure(&s, 25.0, 100.0);
+  CHECK(s.edgeLength == 25.0 && s.boxExtent == 100.0);
+  gvizForceEmbedderConfigureSpeed(&s, 0.0);
+  CHECK(s.jitterTolerance == 1.0);
+  gvizForceEmbedderConfigureSpeed(&s, 2.5);
+  CHECK(s.jitterTolerance == 2.5);
+  gvizForceEmbedderConfigureBarnesHut(&s, 0.0, 0);
+  CHECK(s.theta == 1.0 && s.nodesPerCell == 1);
+  gvizForceEmbedderConfigureBarnesHut(&s, 0.3, 4);
+  CHECK(s.theta == 0.3 && s.nodesPerCell == 4);
+  gvizForceEmbedderConfigureOverlapPrevention(&s, -3.0);
+  CHECK(s.overlapConstant == 100.0);
+  gvizForceEmbedderConfigureRadius(&s, 0.5, 2.0);
+  CHECK(fabs(gvizForceEmbedderVertexRadius(&s, 5) - 0.5 * (1.0 + 2.0 * sqrt(2.0))) < 1e-15);
+  CHECK(gvizEmbeddedGraphInvokeAction(e, "forceEmbedder.toggleOverlapPrevention", NULL) == 0);
+  CHECK(s.preventOverlap == 1);
+  /* not begun: the step action is a no-op and Run refuses */
+  CHECK(gvizEmbeddedGraphInvokeAction(e, "forceEmbedder.step", NULL) == 0);
+  CHECK(gvizEmbeddedGraphFindStatSeries(e, "forceEmbedder.maxDisp")->count == 0);
+  CHECK(gvizForceEmbedderRun(&s, 10, 0.0) == -1);
+  gvizForceEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+}
+
+static void test_forceModel_table(void) {
+  const gvizForceModel *fr = gvizForceModelGet(GVIZ_FORCE_MODEL_FRUCHTERMAN_REINGOLD);
+  const gvizForceModel *ll = gvizForceModelGet(GVIZ_FORCE_MODEL_LINLOG);
+  CHECK(fr->vertexMass(7) == 1.0 && ll->vertexMass(7) == 8.0);
+  double v[2] = {0.0, 0.0}, u[2] = {3.0, 4.0}, acc[2] = {0.0, 0.0};
+  fr->attractive(2, v, u, 10.0, acc); /* (d / k) * delta = 0.5 * (3, 4) */
+  CHECK(fabs(acc[0] - 1.5) < 1e-15 && fabs(acc[1] - 2.0) < 1e-15);
+  acc[0] = acc[1] = 0.0;
+  fr->repulsive(2, v, u, 1.0, 2.0, 0.0, 0.0, 100.0, 10.0, acc);
+  /* weight 2 * k^2 / d along -(3, 4) / 5 */
+  CHECK(fabs(acc[0] + 2.0 * 100.0 / 5.0 * 0.6) < 1e-12);
+  CHECK(fabs(acc[1] + 2.0 * 100.0 / 5.0 * 0.8) < 1e-12);
+  acc[0] = acc[1] = 0.0;
+  ll->repulsive(2, v, u, 2.0, 3.0, 1.0, 1.0, 100.0, 10.0, acc);
+  /* mass product 6 over the gap 5 - 2 = 3 */
+  CHECK(fabs(acc[0] + 6.0 / 3.0 * 0.6) < 1e-12 && fabs(acc[1] + 6.0 / 3.0 * 0.8) < 1e-12);
+  acc[0] = acc[1] = 0.0;
+  ll->repulsive(2, v, u, 2.0, 3.0, 3.0, 3.0, 100.0, 10.0, acc);
+  CHECK(fabs(acc[0] + 600.0 * 0.6) < 1e-9); /* in contact: overlap constant */
+  acc[0] = acc[1] = 0.0;
+  ll->attractive(2, v, u, 10.0, acc);
+  CHECK(fabs(acc[0] - log(6.0) / 5.0 * 3.0) < 1e-15);
+}
+
+static void test_tutte_boundaryAndSeed(void) {
+  gvizGraph g;
+  make_path(&g, 6);
+  gvizGraphAddEdge(&g, 5, 0);
+  gvizTutteState s;
+  CHECK(gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, 0.0) == 0);
+  CHECK(s.epsilon == GVIZ_TUTTE_DEFAULT_EPSILON && s.relaxationRate == 5.0);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
+  CHECK(gvizEmbeddedGraphFindAction(e, "tutte.step") != NULL);
+  CHECK(gvizEmbeddedGraphFindAction(e, "tutte.fixOuterFace") != NULL);
+  CHECK(gvizEmbeddedGraphFindStatSeries(e, "tutte.maxDelta") != NULL);
+  CHECK(gvizTutteEmbedderRun(&s, 10) == -1);
+  size_t two[2] = {0, 1}, bad[3] = {0, 1, 6}, tri[4] = {0, 2, 4, 5};
+  double pos[8] = {0};
+  CHECK(gvizTutteEmbedderSetBoundary(&s, two, 2, pos) == -1);
+  CHECK(gvizTutteEmbedderSetBoundary(&s, bad, 3, pos) == -1);
+  gvizTutteFixConvexPolygon(&s, tri, 4, 10.0);
+  CHECK(s.boundaryCount == 4 && s.isBoundary[2] && !s.isBoundary[1]);
+  const double *p2 = gvizEmbeddedGraphGetVPosition(e, 2);
+  CHECK(fabs(p2[0]) < 1e-12 && fabs(p2[1] - 10.0) < 1e-12);
+  gvizTutteEmbedderSeedInterior(&s);
+  const double *p1 = gvizEmbeddedGraphGetVPosition(e, 1);
+  CHECK(fabs(p1[0]) < 1e-12 && fabs(p1[1]) < 1e-12);
+  CHECK(gvizTutteEmbedderStep(&s, 0.0) == 0.0 && s.iteration == 0);
+  CHECK(gvizTutteEmbedderBegin(&s) == -1);
+  gvizTutteEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+}
+
+static void test_grip_initAndK(void) {
+  gvizGraph g;
+  make_path(&g, 2);
+  gvizGRIPState s;
+  memset(&s, 0, sizeof(s));
+  CHECK(gvizGRIPEmbedderInit(&s, gvizSubgraphCreateFull(&g), 0, 2) == -1);
+  gvizGraphRelease(&g);
+  make_path(&g, 30);
+  memset(&s, 0, sizeof(s));
+  gvizGRIPEmbedderConfigureStats(&s, false);
+  gvizGRIPEmbedderConfigureKnnCapacity(&s, 20);
+  CHECK(gvizGRIPEmbedderInit(&s, gvizSubgraphCreateFull(&g), 0, 2) == 0);
+  CHECK(s.knnCapacity == 20 && !s.statsEnabled);
+  CHECK(s.placementKMax == 128 && s.refinementKMax == 128);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
+  CHECK(gvizEmbeddedGraphStatSeriesCount(e) == 0);
+  CHECK(gvizEmbeddedGraphFindAction(e, "grip.refineRound") != NULL);
+  CHECK(gvizEmbeddedGraphFindAction(e, "grip.nextStage") != NULL);
+  CHECK(!gvizEmbeddedGraphIsVertexVisible(e, 0));
+  gvizGRIPEmbedderConfigureK(&s, 64, 0, GVIZ_GRIP_K_LAYER_DECAY);
+  CHECK(s.placementKMax == 20 && s.refinementKMax == 128);
+  gvizGRIPEmbedderConfigureK(&s, 5, 7, GVIZ_GRIP_K_BUDGET);
+  CHECK(s.placementKMax == 5 && s.refinementKMax == 7 && s.kPolicy == GVIZ_GRIP_K_BUDGET);
+  /* before Begin the actions do nothing */
+  CHECK(gvizEmbeddedGraphInvokeAction(e, "grip.refineRound", NULL) == 0);
+  CHECK(s.currRound == 0);
+  gvizGRIPRoundStats st = gvizGRIPEmbedderLastRoundStats(&s);
+  CHECK(st.maxDisplacement == 0.0);
+  gvizGRIPEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+}
+
+static int mode_unit(void) {
+  test_graph_insertionOrderAndCSR();
+  test_embeddedGraph_actionsStatsMask();
+  test_embeddedGraph_randomizeSaveLoad();
+  test_forceEmbedder_initConfigure();
+  test_forceModel_table();
+  test_tutte_boundaryAndSeed();
+  test_grip_initAndK();
+  printf("%s (%d failed checks)\n", failures ? "FAIL" : "OK", failures);
+  return failures ? 1 : 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc == 2 && !strcmp(argv[1], "unit"))
+    return mode_unit();
+  if (argc != 4) {
+    fprintf(stderr, "usage: %s unit | %s <mode> <in> <out>\n", argv[0], argv[0]);
+    return 2;
+  }
+  fin = fopen(argv[2], "rb");
+  fout = fopen(argv[3], "wb");
+  if (!fin || !fout)
+    return 2;
+  int rc = 2;
+  if (!strcmp(argv[1], "api_force"))
+    rc = mode_api_force();
+  else if (!strcmp(argv[1], "api_grip"))
+    rc = mode_api_grip();
+  else if (!strcmp(argv[1], "tutte"))
+    rc = mode_tutte();
+  fclose(fin);
+  fclose(fout);
+  return rc;
+}

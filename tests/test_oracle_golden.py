@@ -148,3 +148,36 @@ def test_grip_filtration_and_rounds_exact(name):
         d1 = f[f"r{i}_dec1"]
         assert np.array_equal(dec["heat"], d1["heat"])
         assert (st.maxDisp, st.meanDisp, st.meanForce) == tuple(f[f"r{i}_stats"])
+
+
+@pytest.mark.parametrize("name", ["api_force_linlog_ba400", "api_force_fr_grid20"])
+def test_force_api_run_on_induced_subgraph_bit_exact(name):
+    """gvizForceEmbedderRun after Begin(seed) on the subgraph induced by the
+    non-hidden vertices: compact ids in ascending parent id, neighbours in
+    the parent's order, stop once maxDisp < epsilon."""
+    f = golden(name)
+    n = int(f["n"])
+    keep = np.ones(n, bool)
+    keep[f["hidden"]] = False
+    ids = np.flatnonzero(keep)
+    index = -np.ones(n, np.int64)
+    index[ids] = np.arange(ids.size)
+    e = f["edges"]
+    e = e[keep[e[:, 0]] & keep[e[:, 1]]]
+    off, adj = G.csr_from_edges(ids.size, index[e])
+    run = O.ForceRun(O.Graph(off, adj), _params_from(f), f["pos_begin"][ids])
+    ref = np.stack([f["series_" + s] for s in ("maxDisp", "speed", "attractiveForce",
+                                               "repulsiveForce", "gravityForce")], 1)
+    steps = 0
+    while steps < int(f["run_iters"]):
+        st = run.step()
+        assert (st.maxDisp, st.speed, st.meanAtt, st.meanRep,
+                st.meanGrav) == tuple(ref[steps]), steps
+        steps += 1
+        if st.maxDisp < float(f["epsilon"]):
+            break
+    assert steps == int(f["run_rc"])
+    assert np.array_equal(run.pos, f["pos_run"][ids])
+    st = run.step()
+    assert st.maxDisp == float(f["step_ret"])
+    assert np.array_equal(run.pos, f["pos_step"][ids])
