@@ -125,10 +125,11 @@ def knn_cases():
          batch_list=np.stack([b[2] for b in batch]))
 
 
-def grip_case(name, n, e, kcap, rounds):
-    borders, filt, rr = refio.grip(n, e, kcap, kcap, rounds)
-    out = dict(n=n, edges=e, kcap=kcap, borders=borders, filtration=filt,
-               rounds=len(rr))
+def grip_case(name, n, e, kcap, rounds, kmax=None):
+    kmax = kcap if kmax is None else kmax
+    borders, filt, rr = refio.grip(n, e, kcap, kmax, rounds)
+    out = dict(n=n, edges=e, kcap=kcap, kmax=kmax, borders=borders,
+               filtration=filt, rounds=len(rr))
     for i, r in enumerate(rr):
         out[f"r{i}_layer"] = r["layer"]
         out[f"r{i}_pos0"] = r["pos0"]
@@ -147,6 +148,13 @@ def grip_cases():
     grip_case("grip_grid10", n, e, 8, 2)
     n, e = G.barabasi_albert(200, 2, 1)
     grip_case("grip_ba200", n, e, 10, 1)
+
+
+def grip_chain_cases():
+    """Several rounds per layer with K below the kNN capacity: the rounds
+    chain through the device state and every NextStage is included."""
+    n, e = G.grid(16, 16)
+    grip_case("grip_grid16_k4", n, e, 16, 4, kmax=4)
 
 
 def api_force_case(name, n, e, hidden, seed, run_iters, epsilon, **kw):
@@ -183,7 +191,8 @@ def api_cases():
 
 
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
-             knn=knn_cases, grip=grip_cases, api=api_cases)
+             knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
+             api=api_cases)
 
 if __name__ == "__main__":
     if not refio.harness_available():

@@ -140,3 +140,79 @@ def test_update_knn_matches_reference_lists(gpu):
             assert dec["knnCount"][v] == c
             assert np.array_equal(kv[v, :c], ref[v, :c, 0])
             assert np.array_equal(kd[v, :c], ref[v, :c, 1])
+
+
+def _prefer_batch(n_vertices, n_edges, visible_count, target_count):
+    """src/algorithms/search/gvizKNearest.c:149-166 (constants from
+    include/algorithms/search/gvizKNearest.h:9-13)."""
+    if visible_count == 0 or visible_count >= target_count or visible_count > 256:
+        return False
+    if 2.0 * n_edges / n_vertices > 32.0:
+        return True
+    return target_count >= 8 * visible_count
+
+
+def _compare_state(i, when, pos, dec, kv, kd, ref_pos, ref_dec, ref_knn, active_ids):
+    scale = np.abs(ref_pos).max()
+    d = np.abs(pos - ref_pos).max()
+    assert d <= 1e-12 * scale, f"round {i} {when}: positions differ by {d:.3g}"
+    for k, a in (("dispX", ref_dec["disp"][:, 0]), ("dispY", ref_dec["disp"][:, 1]),
+                 ("oldDispX", ref_dec["oldDisp"][:, 0]),
+                 ("oldDispY", ref_dec["oldDisp"][:, 1]), ("heat", ref_dec["heat"]),
+                 ("oldCos", ref_dec["oldCos"]), ("lastForceMag", ref_dec["lastForceMag"])):
+        got = dec[k][active_ids]
+        want = a[active_ids]
+        tol = 1e-11 * max(np.abs(want).max(), 1e-300)
+        bad = np.flatnonzero(np.abs(got - want) > tol)
+        assert bad.size == 0, (f"round {i} {when}: {k} differs at vertex "
+                               f"{active_ids[bad[0]]}: {got[bad[0]]!r} vs {want[bad[0]]!r}")
+    for k in ("knnCount", "dispCalculated"):
+        got, want = dec[k][active_ids], ref_dec[k][active_ids]
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, (f"round {i} {when}: {k} differs at vertex "
+                               f"{active_ids[bad[0]]}: {got[bad[0]]} vs {want[bad[0]]}")
+    if ref_knn is not None:
+        for v in active_ids:
+            c = int(ref_dec["knnCount"][v])
+            assert np.array_equal(kv[v, :c], ref_knn[v, :c, 0]), (i, when, "knn", v)
+            assert np.array_equal(kd[v, :c], ref_knn[v, :c, 1]), (i, when, "dist", v)
+
+
+def test_grip_rounds_chain_through_device_state(gpu):
+    """The whole schedule from the state after Begin, with nothing re-uploaded:
+    every refinement round, NextStage placement, decorator clear and kNN
+    update runs on the device state the previous phase left, and every field
+    of every vertex is compared with the reference's after each phase."""
+    f = golden("grip_grid16_k4")
+    n = int(f["n"])
+    kcap, k = int(f["kcap"]), int(f["kmax"])
+    off, adj = G.csr_from_edges(n, f["edges"])
+    ne = f["edges"].shape[0]
+    eng = gpu.GripEngine(off, adj, kcap=kcap)
+    filt, borders = f["filtration"], f["borders"]
+    eng.set_filtration(filt)
+    knn0 = f["r0_knn"]
+    eng.upload(f["r0_pos0"], _dec_from(f["r0_dec0"], n), knn0[:, :, 0], knn0[:, :, 1])
+    prev = int(f["r0_layer"])
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(borders[layer])
+        ids = filt[:active]
+        if layer != prev:
+            begin, end = int(borders[layer + 1]), int(borders[layer])
+            vis = np.zeros(n, np.uint8)
+            vis[filt[:begin]] = 1
+            eng.place(begin, end, vis, k, _prefer_batch(n, ne, begin, end - begin))
+            vis[filt[:end]] = 1
+            eng.clear_decorators(end)
+            eng.update_knn(end, vis, k, _prefer_batch(n, ne, end, end))
+            pos, dec, kv, kd = eng.download()
+            _compare_state(i, "after NextStage", pos, dec, kv, kd, f[f"r{i}_pos0"],
+                           f[f"r{i}_dec0"], f[f"r{i}_knn"], ids)
+            prev = layer
+        st = eng.refine_round(active, layer)
+        pos, dec, kv, kd = eng.download()
+        _compare_state(i, "after the round", pos, dec, kv, kd, f[f"r{i}_pos1"],
+                       f[f"r{i}_dec1"], None, ids)
+        np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
+                                   f[f"r{i}_stats"], rtol=1e-11)

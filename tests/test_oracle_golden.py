@@ -181,3 +181,38 @@ def test_force_api_run_on_induced_subgraph_bit_exact(name):
     st = run.step()
     assert st.maxDisp == float(f["step_ret"])
     assert np.array_equal(run.pos, f["pos_step"][ids])
+
+
+def test_grip_rounds_chain_bit_exact():
+    """Rounds of one layer chained through the oracle's own per-vertex state
+    (the reference run serially): every field after every round is equal."""
+    f = golden("grip_grid16_k4")
+    g = _graph(f)
+    n, kcap = int(f["n"]), int(f["kcap"])
+    filt = f["filtration"]
+    prev, dec, pos = None, None, None
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        if layer != prev:
+            d0 = f[f"r{i}_dec0"]
+            dec = np.zeros(n, dtype=O.GRIP_VERTEX_DTYPE)
+            dec["dispX"], dec["dispY"] = d0["disp"][:, 0], d0["disp"][:, 1]
+            dec["oldDispX"], dec["oldDispY"] = d0["oldDisp"][:, 0], d0["oldDisp"][:, 1]
+            for k in ("heat", "oldCos", "lastForceMag", "knnCount", "dispCalculated"):
+                dec[k] = d0[k]
+            pos = f[f"r{i}_pos0"].copy()
+            prev = layer
+        else:
+            assert np.array_equal(pos, f[f"r{i}_pos0"])
+        active = int(f["borders"][layer])
+        st = O.grip_refine_round(g, filt, active, layer, f[f"r{i}_knn"], kcap, dec, pos)
+        d1 = f[f"r{i}_dec1"]
+        assert np.array_equal(pos, f[f"r{i}_pos1"]), i
+        ids = filt[:active]
+        for k, ref in (("dispX", d1["disp"][:, 0]), ("dispY", d1["disp"][:, 1]),
+                       ("oldDispX", d1["oldDisp"][:, 0]), ("oldDispY", d1["oldDisp"][:, 1]),
+                       ("heat", d1["heat"]), ("oldCos", d1["oldCos"]),
+                       ("lastForceMag", d1["lastForceMag"]),
+                       ("dispCalculated", d1["dispCalculated"])):
+            assert np.array_equal(dec[k][ids], ref[ids]), (i, k)
+        assert (st.maxDisp, st.meanDisp, st.meanForce) == tuple(f[f"r{i}_stats"])
