@@ -15,6 +15,12 @@ Barnes-Hut traversal), timed with HIP events on the engine's stream; the
 cpu_baseline object times the same step on the host cores (the reference
 library compiled from its own sources when it was built into oracle/_ref,
 else this repository's FP64 restatement of it) on a bounded sample.
+
+The other measured paths, one JSON line each:
+
+    python bench.py --workload allpairs_er65k    exact FR all-pairs step, ER n=65,536 avg deg 8
+    python bench.py --workload tutte_delaunay1m  Tutte Jacobi sweeps, Delaunay n=1,000,000
+    python bench.py --workload grip_rgg4m        GRIP schedule and finest-layer rounds, RGG n=4,000,000
 """
 from __future__ import annotations
 
@@ -117,10 +123,307 @@ def cpu_baseline(n, edges, pos, seconds_budget=20.0):
                       f"graph and initial positions, {threads} threads; mean {per:.3f} s/step"}
 
 
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def run_harness(mode, payload, timeout):
+    """Runs the reference library's timing harness (built from the
+    reference's sources into oracle/_ref) and returns its output bytes."""
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fi, "wb") as f:
+            f.write(payload)
+        subprocess.run([HARNESS, mode, fi, fo], check=True, timeout=timeout)
+        return open(fo, "rb").read()
+
+
+def graph_bytes(n, edges):
+    import struct
+    return struct.pack("<qq", n, edges.shape[0]) + \
+        np.ascontiguousarray(edges, dtype="<i8").tobytes()
+
+
+def emit(line):
+    print(json.dumps(line), flush=True)
+
+
+def base_line(metric, value, unit, args, ms, dtype, workload, extra_config,
+              higher=True):
+    return {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic (seeded generators in grapher_amd/graphs.py)",
+            "config": dict({"workload": workload, "parallelism": "single GPU"},
+                           **extra_config)}
+
+
+def bench_allpairs(args):
+    """Exact FR all-pairs step (Barnes-Hut off), ER n=65,536, avg degree 8."""
+    import struct
+    n, edges = G.erdos_renyi(65536, 8.0, 0)
+    off, adj = G.csr_from_edges(n, edges)
+    pos = G.uniform_positions(n, 0)
+    eng = native.ForceEngine(off, adj, precision=args.precision, model=native.MODEL_FR,
+                             barnes_hut=False)
+    eng.begin(pos)
+    steps = args.steps if args.steps_given else 50
+    args.steps = steps
+    if args.warmup:
+        eng.run(args.warmup, stats=False)
+    t0 = time.perf_counter()
+    _, ran = eng.run(steps, stats=False)
+    el = time.perf_counter() - t0
+    rep_ms, inter, _ = eng.profile(3)
+    launch_s = rep_ms / 1e3 / 3
+    flops = FLOP_PER_INTERACTION * inter / 3
+    achieved = flops / launch_s / 1e12
+    line = base_line("exact all-pairs FR force-iters/s at N=65,536", steps / el,
+                     "iters/s", args, 1e3 * el / steps,
+                     {"mixed": "f32 (f64 positions)", "fp32": "f32", "fp64": "f64"}[args.precision],
+                     "ER n=65,536 avg degree 8, FR, Barnes-Hut off (exact all-pairs), "
+                     "uniform [-1,1]^2 init",
+                     {"n_vertices": n, "n_edges": int(edges.shape[0]),
+                      "precision": args.precision})
+    line["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3),
+                        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                        "kernel": "k_allpairs",
+                        "note": "FP32 vector ALU (no matrix op on this path); 15 FLOP per "
+                                "pair evaluation, N(N-1) evaluations per launch",
+                        "launch_ms": round(launch_s * 1e3, 4),
+                        "interactions_per_launch": int(inter / 3)}
+    cpu = None
+    if not args.no_cpu_baseline and os.path.exists(HARNESS):
+        threads = min(16, len(os.sched_getaffinity(0)))
+        iters = 2
+        payload = graph_bytes(n, edges) + struct.pack("<qqqqqq", 0, 0, 0, 1, iters, threads - 1) \
+            + struct.pack("<ddddddd", 10.0, 1.0, 0.0, 100.0, 0.0, 0.0, 1.0) \
+            + np.ascontiguousarray(pos, dtype="<f8").tobytes()
+        times = np.frombuffer(run_harness("time_force", payload, 600), dtype="<f8")
+        per = float(np.mean(times))
+        cpu = {"value": 1.0 / per, "unit": "iters/s", "cores": threads, "kind": "reference",
+               "sample": f"{iters} gvizForceEmbedderStep calls (Barnes-Hut off) of the reference "
+                         f"library on the same graph and positions, pool of {threads} "
+                         f"threads; mean {per:.3f} s/step"}
+    line["cpu_baseline"] = cpu
+    if cpu:
+        line["speedup_vs_cpu"] = round(line["value"] / cpu["value"], 1)
+    emit(line)
+
+
+def tutte_setup(n, hull, radius=200.0):
+    """gvizTutteFixConvexPolygon + gvizTutteEmbedderSeedInterior
+    (src/embedders/gvizTutteEmbedder.c:207-247): hull on a circle, every other
+    vertex at the hull's centroid."""
+    k = np.arange(len(hull), dtype=np.float64)
+    ang = 2.0 * np.pi * k / len(hull)
+    ring = np.stack([radius * np.cos(ang), radius * np.sin(ang)], 1)
+    cx = 0.0
+    cy = 0.0
+    for x, y in ring:  # the reference's summation order
+        cx += x
+        cy += y
+    pos = np.empty((n, 2))
+    pos[:, 0] = cx / len(hull)
+    pos[:, 1] = cy / len(hull)
+    pos[hull] = ring
+    isb = np.zeros(n, np.uint8)
+    isb[hull] = 1
+    return pos, isb
+
+
+def bench_tutte(args):
+    """Tutte Jacobi sweeps, Delaunay triangulation of n=1,000,000 uniform
+    points, convex hull pinned on a circle of radius 200, alpha = 1."""
+    import struct
+    n, edges, _, hull = G.delaunay(1_000_000, 0)
+    off, adj = G.csr_from_edges(n, edges)
+    pos, isb = tutte_setup(n, hull)
+    prec = "fp32" if args.precision == "fp32" else "fp64"
+    eng = native.TutteEngine(off, adj, precision=prec)
+    eng.upload(pos, isb)
+    steps = args.steps if args.steps_given else 1000
+    args.steps = steps
+    if args.warmup:
+        eng.sweeps(args.warmup, record=False)
+    eng.upload(pos, isb)
+    t0 = time.perf_counter()
+    _, ran = eng.sweeps(steps, 1.0, -1.0, record=False)
+    el = time.perf_counter() - t0
+    assert ran == steps
+    sweep_ms, _ = eng.profile(20)
+    launch_s = sweep_ms / 1e3 / 20
+    nnz = int(adj.shape[0])
+    ps = 16 if prec == "fp64" else 8
+    bytes_per = 4 * (n + 1) + 4 * nnz + 2 * ps * n + n
+    achieved = bytes_per / launch_s / 1e9
+    line = base_line("Tutte Jacobi sweeps/s, Delaunay N=1,000,000", steps / el, "sweeps/s",
+                     args, 1e3 * el / steps, "f64" if prec == "fp64" else "f32",
+                     "Delaunay triangulation of 1,000,000 uniform points, convex hull "
+                     "pinned on a circle (radius 200), interior seeded at the centroid, "
+                     "Jacobi alpha = 1", {"n_vertices": n, "n_edges": int(edges.shape[0]),
+                                           "hull": int(len(hull)), "precision": prec})
+    line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(achieved / 8000.0, 5),
+                        "traffic": None, "kernel": "k_sweep",
+                        "note": "algorithmic bytes per sweep = 4(N+1) offsets + 4 nnz "
+                                "indices + N x (read + write position) + N boundary "
+                                "bytes; neighbour gathers are not counted",
+                        "launch_ms": round(launch_s * 1e3, 4),
+                        "bytes_per_launch": int(bytes_per)}
+    cpu = None
+    if not args.no_cpu_baseline and os.path.exists(HARNESS):
+        iters = 60
+        payload = graph_bytes(n, edges) + struct.pack("<qq", len(hull), iters) \
+            + np.ascontiguousarray(hull, dtype="<i8").tobytes() + struct.pack("<d", 200.0)
+        times = np.frombuffer(run_harness("time_tutte", payload, 600), dtype="<f8")
+        per = float(np.mean(times))
+        cpu = {"value": 1.0 / per, "unit": "sweeps/s", "cores": 1, "kind": "reference",
+               "sample": f"{iters} gvizTutteEmbedderStep(dt=1) calls of the reference "
+                         f"library on the same graph and boundary (its sweep is serial); "
+                         f"mean {per * 1e3:.2f} ms/sweep"}
+    line["cpu_baseline"] = cpu
+    if cpu:
+        line["speedup_vs_cpu"] = round(line["value"] / cpu["value"], 1)
+    emit(line)
+
+
+def regular_simplex2(side):
+    """Three vertices of an equilateral triangle centred at the origin, built
+    like makeRegularSimplex (src/embedders/gvizGRIPEmbedder.c:335-420) for
+    n = 2: unit vectors with pairwise dot product -1/2, scaled to `side`."""
+    v = np.zeros((3, 2))
+    v[0] = (1.0, 0.0)
+    v[1, 0] = -0.5
+    v[1, 1] = np.sqrt(1.0 - 0.25)
+    v[2, 0] = -0.5
+    v[2, 1] = (-0.5 - v[2, 0] * v[1, 0]) / v[1, 1]
+    edge = np.sqrt(((v[0] - v[1]) ** 2).sum())
+    return v * (side / edge)
+
+
+def prefer_batch(n_active, n_edges, visible, targets):
+    """src/algorithms/search/gvizKNearest.c:149-166."""
+    if visible == 0 or visible >= targets or visible > 256 or n_active == 0:
+        return False
+    if 2.0 * n_edges / n_active > 32.0:
+        return True
+    return targets >= 8 * visible
+
+
+def bench_grip(args):
+    """GRIP on an RGG with n=4,000,000 (largest component), k = 10, the
+    reference's Embed schedule (<= 30 rounds per layer, stop below 0.01);
+    value = refinement rounds/s at the finest layer."""
+    import struct
+    k = 10
+    n, edges, _ = G.random_geometric(4_000_000, 8.0, 3)
+    act, inside = G.largest_component(n, edges)
+    off, adj = G.csr_from_edges(n, inside)
+    na = int(act.sum())
+    ne = int(inside.shape[0])
+    eng = native.GripEngine(off, adj, active=act, kcap=k)
+    t_all = time.perf_counter()
+    filt, borders = eng.mis_filtration(2)
+    t_begin = time.perf_counter() - t_all
+    layers = len(borders)
+    pos = np.zeros((n, 2))
+    pos[filt[:3]] = regular_simplex2(10.0 * 1000)
+    eng.upload(pos)
+    vis = np.zeros(n, np.uint8)
+    vis[filt[:3]] = 1
+    layer = layers - 1
+    end = int(borders[layer])
+    eng.clear_decorators(end)
+    eng.update_knn(end, vis, k, prefer_batch(na, ne, 3, end))
+    rounds_total = 0
+    while True:
+        active = int(borders[layer])
+        for _ in range(30):
+            st = eng.refine_round(active, layer)
+            rounds_total += 1
+            if st.maxDisp < 10.0 * 1e-3:
+                break
+        if layer == 0:
+            break
+        layer -= 1
+        begin, end = int(borders[layer + 1]), int(borders[layer])
+        eng.place(begin, end, vis, k, prefer_batch(na, ne, begin, end - begin))
+        vis[filt[begin:end]] = 1
+        eng.clear_decorators(end)
+        eng.update_knn(end, vis, k, prefer_batch(na, ne, end, end))
+    t_embed = time.perf_counter() - t_all
+    steps = args.steps if args.steps_given else 20
+    args.steps = steps
+    for _ in range(args.warmup):
+        eng.refine_round(na, 0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.refine_round(na, 0)
+    el = time.perf_counter() - t0
+    f_ms, r_ms = eng.profile_rounds(na, 0, 5)
+    launch_s = f_ms / 1e3 / 5
+    deg_sum = int(adj.shape[0])
+    bytes_per = na * (16 + 8 + 8 + 16 + 8 + 8 + 8 + 16 + 16 + 8 + 8 + 8 + 8) \
+        + 4 * deg_sum + 16 * k * na
+    achieved = bytes_per / launch_s / 1e9
+    line = base_line("GRIP refinement rounds/s at the finest layer, RGG N=4,000,000, k=10",
+                     steps / el, "rounds/s", args, 1e3 * el / steps, "f64",
+                     "random geometric graph, 4,000,000 uniform points in the unit square, "
+                     "radius for average degree 8, largest component; GRIP dim 2, "
+                     "kNN capacity 10, K = 10 constant; the reference's Embed schedule",
+                     {"n_vertices": n, "active_vertices": na, "n_edges": ne,
+                      "layers": layers, "borders": [int(b) for b in borders]})
+    line["embed_s"] = round(t_embed, 3)
+    line["filtration_s"] = round(t_begin, 3)
+    line["schedule_rounds"] = rounds_total
+    line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(achieved / 8000.0, 5),
+                        "traffic": None, "kernel": "k_refine (layer 0)",
+                        "note": "algorithmic bytes per launch = per active vertex its "
+                                "position, CSR row, kNN list and per-vertex state read and "
+                                "written once; neighbour position gathers not counted",
+                        "launch_ms": round(launch_s * 1e3, 4),
+                        "round_ms": round(r_ms / 5, 4),
+                        "bytes_per_launch": int(bytes_per)}
+    cpu = None
+    if not args.no_cpu_baseline and os.path.exists(HARNESS):
+        threads = min(16, len(os.sched_getaffinity(0)))
+        sn, se, _ = G.random_geometric(500_000, 8.0, 3)
+        sact, _ = G.largest_component(sn, se)
+        payload = graph_bytes(sn, se) + np.ascontiguousarray(sact, dtype="<i8").tobytes() \
+            + struct.pack("<qqqq", k, 2, 64, threads - 1)
+        b = run_harness("time_grip", payload, 900)
+        off_b = 16
+        last = None
+        while True:
+            lay = struct.unpack_from("<q", b, off_b)[0]
+            off_b += 8
+            if lay == -1:
+                break
+            cnt = struct.unpack_from("<q", b, off_b)[0]
+            off_b += 16
+            rs = struct.unpack_from("<2d", b, off_b)
+            off_b += 16
+            last = (lay, cnt, rs)
+        per = float(np.mean(last[2])) * (na / last[1])
+        cpu = {"value": 1.0 / per, "unit": "rounds/s", "cores": threads, "kind": "reference",
+               "sample": f"the reference's GRIP schedule on an RGG of 500,000 points (same "
+                         f"generator and parameters, {last[1]} active vertices), 2 rounds per "
+                         f"layer with its pool of {threads} threads; its finest-layer round "
+                         f"({np.mean(last[2]):.3f} s) scaled linearly to {na} active vertices"}
+    line["cpu_baseline"] = cpu
+    if cpu:
+        line["speedup_vs_cpu"] = round(line["value"] / cpu["value"], 1)
+    emit(line)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--workload", default="force_ba1m",
+                    choices=["force_ba1m", "allpairs_er65k", "tutte_delaunay1m", "grip_rgg4m"])
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"])
@@ -147,6 +450,13 @@ def main():
     if native.device_count() <= 0:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    args.steps_given = args.steps is not None
+    if args.workload != "force_ba1m":
+        {"allpairs_er65k": bench_allpairs, "tutte_delaunay1m": bench_tutte,
+         "grip_rgg4m": bench_grip}[args.workload](args)
+        return
+    if args.steps is None:
+        args.steps = 200
 
     n, edges, off, adj, pos, gen_s = workload(args.n)
     print(f"# workload built in {gen_s:.1f}s: n={n} edges={edges.shape[0]}",

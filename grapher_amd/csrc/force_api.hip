@@ -750,7 +750,7 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
   std::vector<int32_t> cnt(c.n > 0 ? c.n : 1);
   int rc = 0;
   for (int64_t k = 0; k < steps && rc == 0; k++) {
-    c.timeTraversal = c.prm.barnesHut ? 1 : 0;
+    c.timeTraversal = 1;
     c.wantInteractions = 1;
     if (hipEventRecord(s0, c.stream) != hipSuccess)
       rc = -1;
@@ -763,7 +763,7 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
     if (rc == 0 && hipStreamSynchronize(c.stream) != hipSuccess)
       rc = -1;
     float ms = 0.0f;
-    if (rc == 0 && c.prm.barnesHut) {
+    if (rc == 0 && c.n > 0) {
       (void)hipEventElapsedTime(&ms, c.travBegin, c.travEnd);
       trav += ms;
     }

@@ -884,6 +884,8 @@ int typed_step(ForceCommon &c, int mode) {
       GVH_CHECK(hipEventRecord(c.travEnd, st));
   } else {
     const unsigned tb = (unsigned)((n + kTB - 1) / kTB);
+    if (c.timeTraversal)
+      GVH_CHECK(hipEventRecord(c.travBegin, st));
     if (model == 1)
       hipLaunchKernelGGL((k_allpairs<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
                          posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
@@ -893,6 +895,8 @@ int typed_step(ForceCommon &c, int mode) {
                          posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
                          kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p, c.sc.p);
     GVH_CHECK_LAUNCH();
+    if (c.timeTraversal)
+      GVH_CHECK(hipEventRecord(c.travEnd, st));
   }
 
   const unsigned rb = grid_for(n, kTB) < (unsigned)c.reductionBlocks

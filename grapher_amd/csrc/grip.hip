@@ -1356,6 +1356,46 @@ int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
   return 0;
 }
 
+} // extern "C"
+
+namespace {
+
+// One refinement round (src/embedders/gvizGRIPEmbedder.c:1289-1333) on the
+// engine's stream; f0/f1 (may be null) are recorded around the spring-force
+// kernel.
+int enqueue_round(gvizhip_grip *g, int64_t active, int64_t layer, unsigned *blocksOut,
+                  hipEvent_t f0, hipEvent_t f1) {
+  hipStream_t st = g->stream;
+  const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
+  if (f0)
+    GVH_CHECK(hipEventRecord(f0, st));
+  hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kB), 0, st, g->filt.p, active,
+                     layer, g->off.p, g->adj.p, g->pos.p, g->knnV.p, g->knnD.p,
+                     g->knnCount.p, g->kcap, g->disp.p, g->oldDisp.p, g->heat.p,
+                     g->oldCos.p, g->lastF.p, g->dispCalc.p, g->partial.p);
+  if (f1)
+    GVH_CHECK(hipEventRecord(f1, st));
+  hipLaunchKernelGGL(k_round_means, dim3(1), dim3(256), 0, st, g->partial.p,
+                     (int)blocks, active, g->round.p);
+  hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, g->filt.p,
+                     active, g->pos.p, g->disp.p, g->round.p, g->partial.p);
+  hipLaunchKernelGGL(k_round_omega, dim3(1), dim3(256), 0, st, g->partial.p,
+                     (int)blocks, active, g->round.p);
+  hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, g->filt.p,
+                     active, g->pos.p, g->disp.p, g->heat.p, g->lastF.p,
+                     g->round.p, g->partial.p);
+  hipLaunchKernelGGL(k_round_stats, dim3(1), dim3(256), 0, st, g->partial.p,
+                     (int)blocks, active, g->round.p);
+  GVH_CHECK_LAUNCH();
+  if (blocksOut)
+    *blocksOut = blocks;
+  return 0;
+}
+
+} // namespace
+
+extern "C" {
+
 int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
                               gvizhip_grip_round_stats *stats) {
   if (!g || active < 0 || active > g->filtCount) {
@@ -1365,23 +1405,8 @@ int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
   hipStream_t st = g->stream;
   GripRound r{};
   if (active > 0) {
-    const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
-    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kB), 0, st, g->filt.p, active,
-                       layer, g->off.p, g->adj.p, g->pos.p, g->knnV.p, g->knnD.p,
-                       g->knnCount.p, g->kcap, g->disp.p, g->oldDisp.p, g->heat.p,
-                       g->oldCos.p, g->lastF.p, g->dispCalc.p, g->partial.p);
-    hipLaunchKernelGGL(k_round_means, dim3(1), dim3(256), 0, st, g->partial.p,
-                       (int)blocks, active, g->round.p);
-    hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, g->filt.p,
-                       active, g->pos.p, g->disp.p, g->round.p, g->partial.p);
-    hipLaunchKernelGGL(k_round_omega, dim3(1), dim3(256), 0, st, g->partial.p,
-                       (int)blocks, active, g->round.p);
-    hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, g->filt.p,
-                       active, g->pos.p, g->disp.p, g->heat.p, g->lastF.p,
-                       g->round.p, g->partial.p);
-    hipLaunchKernelGGL(k_round_stats, dim3(1), dim3(256), 0, st, g->partial.p,
-                       (int)blocks, active, g->round.p);
-    GVH_CHECK_LAUNCH();
+    if (enqueue_round(g, active, layer, nullptr, nullptr, nullptr))
+      return -1;
     GVH_CHECK(hipMemcpyAsync(&r, g->round.p, sizeof(GripRound), hipMemcpyDeviceToHost,
                              st));
   }
@@ -1392,6 +1417,47 @@ int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
     stats->meanForce = active ? r.sumForce / (double)active : 0.0;
     stats->meanHeat = active ? r.sumHeat / (double)active : 0.0;
   }
+  return 0;
+}
+
+int gvizhip_grip_profile_rounds(gvizhip_grip *g, int64_t active, int64_t layer,
+                                int64_t rounds, double *forceMs, double *roundMs) {
+  if (!g || active < 0 || active > g->filtCount || rounds < 0) {
+    set_error("gvizhip_grip_profile_rounds: invalid arguments");
+    return -1;
+  }
+  double fsum = 0.0, rsum = 0.0;
+  if (active > 0 && rounds > 0) {
+    hipEvent_t e[4];
+    for (auto &x : e)
+      GVH_CHECK(hipEventCreate(&x));
+    int rc = 0;
+    for (int64_t k = 0; k < rounds && rc == 0; k++) {
+      if (hipEventRecord(e[2], g->stream) != hipSuccess ||
+          enqueue_round(g, active, layer, nullptr, e[0], e[1]) ||
+          hipEventRecord(e[3], g->stream) != hipSuccess ||
+          hipStreamSynchronize(g->stream) != hipSuccess) {
+        rc = -1;
+        break;
+      }
+      float a = 0.0f, b = 0.0f;
+      (void)hipEventElapsedTime(&a, e[0], e[1]);
+      (void)hipEventElapsedTime(&b, e[2], e[3]);
+      fsum += a;
+      rsum += b;
+    }
+    for (auto &x : e)
+      (void)hipEventDestroy(x);
+    if (rc) {
+      if (!gvizhip_last_error()[0])
+        set_error("gvizhip_grip_profile_rounds failed");
+      return -1;
+    }
+  }
+  if (forceMs)
+    *forceMs = fsum;
+  if (roundMs)
+    *roundMs = rsum;
   return 0;
 }
 

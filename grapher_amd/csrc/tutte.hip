@@ -387,6 +387,67 @@ int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
   return ran;
 }
 
+int gvizhip_tutte_profile(gvizhip_tutte *t, int64_t sweeps, double alpha,
+                          double *sweepMs, double *totalMs) {
+  if (!t || sweeps < 0 || !(alpha > 0.0) || alpha > 1.0) {
+    set_error("gvizhip_tutte_profile: invalid arguments");
+    return -1;
+  }
+  if (t->n == 0 || sweeps == 0) {
+    if (sweepMs)
+      *sweepMs = 0.0;
+    if (totalMs)
+      *totalMs = 0.0;
+    return 0;
+  }
+  TutteScalars s;
+  if (read_sc(t, &s))
+    return -1;
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, t->stream, t->sc.p, -1.0);
+  GVH_CHECK_LAUNCH();
+  std::vector<hipEvent_t> ev(2 * sweeps + 2, nullptr);
+  for (auto &e : ev)
+    GVH_CHECK(hipEventCreate(&e));
+  int cur = (int)(s.sweeps & 1);
+  const unsigned g = (unsigned)t->blocks;
+  GVH_CHECK(hipEventRecord(ev[2 * sweeps], t->stream));
+  for (int64_t k = 0; k < sweeps; k++) {
+    GVH_CHECK(hipEventRecord(ev[2 * k], t->stream));
+    if (t->precision == GVIZHIP_PRECISION_FP64)
+      hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+                         t->adj.p, t->boundary.p, (const double2 *)t->pos[cur].p,
+                         (double2 *)t->pos[1 - cur].p, t->n, alpha, t->partial.p,
+                         t->sc.p);
+    else
+      hipLaunchKernelGGL(k_sweep<float>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+                         t->adj.p, t->boundary.p, (const float2 *)t->pos[cur].p,
+                         (float2 *)t->pos[1 - cur].p, t->n, (float)alpha,
+                         t->partial.p, t->sc.p);
+    GVH_CHECK(hipEventRecord(ev[2 * k + 1], t->stream));
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, t->stream, t->partial.p,
+                       (int)g, t->sc.p, t->records.p, t->recordCap);
+    GVH_CHECK_LAUNCH();
+    cur = 1 - cur;
+  }
+  GVH_CHECK(hipEventRecord(ev[2 * sweeps + 1], t->stream));
+  GVH_CHECK(hipStreamSynchronize(t->stream));
+  double sum = 0.0;
+  for (int64_t k = 0; k < sweeps; k++) {
+    float ms = 0.0f;
+    GVH_CHECK(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+    sum += ms;
+  }
+  float total = 0.0f;
+  GVH_CHECK(hipEventElapsedTime(&total, ev[2 * sweeps], ev[2 * sweeps + 1]));
+  for (auto &e : ev)
+    (void)hipEventDestroy(e);
+  if (sweepMs)
+    *sweepMs = sum;
+  if (totalMs)
+    *totalMs = total;
+  return 0;
+}
+
 void gvizhip_tutte_destroy(gvizhip_tutte *t) {
   if (!t)
     return;

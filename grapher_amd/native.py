@@ -107,6 +107,7 @@ def lib():
         "gvizhip_tutte_upload": (C.c_int, [_vp, _vp, _vp]),
         "gvizhip_tutte_download": (C.c_int, [_vp, _vp]),
         "gvizhip_tutte_sweeps": (_i64, [_vp, _i64, C.c_double, C.c_double, _vp]),
+        "gvizhip_tutte_profile": (C.c_int, [_vp, _i64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gvizhip_tutte_destroy": (None, [_vp]),
         "gvizhip_grip_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
         "gvizhip_grip_mis_filtration": (_i64, [_vp, _i64, _vp, _vp]),
@@ -119,6 +120,7 @@ def lib():
         "gvizhip_grip_update_knn": (C.c_int, [_vp, _i64, _vp, _i64, C.c_int32]),
         "gvizhip_grip_place": (C.c_int, [_vp, _i64, _i64, _vp, _i64, C.c_int32]),
         "gvizhip_grip_refine_round": (C.c_int, [_vp, _i64, _i64, C.POINTER(GripRoundStats)]),
+        "gvizhip_grip_profile_rounds": (C.c_int, [_vp, _i64, _i64, _i64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gvizhip_grip_destroy": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
@@ -142,12 +144,13 @@ DECLARED = [
     "gvizhip_force_quadtree_leaves", "gvizhip_force_quadtree_cells",
     "gvizhip_force_profile", "gvizhip_force_destroy",
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
-    "gvizhip_tutte_sweeps", "gvizhip_tutte_destroy",
+    "gvizhip_tutte_sweeps", "gvizhip_tutte_profile", "gvizhip_tutte_destroy",
     "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
     "gvizhip_grip_set_filtration", "gvizhip_grip_knn", "gvizhip_grip_knn_batch",
     "gvizhip_grip_upload", "gvizhip_grip_download",
     "gvizhip_grip_clear_decorators", "gvizhip_grip_update_knn",
-    "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_destroy",
+    "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_profile_rounds",
+    "gvizhip_grip_destroy",
 ]
 
 
@@ -331,6 +334,13 @@ class TutteEngine:
             _err("gvizhip_tutte_sweeps")
         return (out[:r] if record else None), int(r)
 
+    def profile(self, sweeps, alpha=1.0):
+        """(summed sweep-kernel ms, total ms) over `sweeps` sweeps."""
+        a, b = C.c_double(), C.c_double()
+        if lib().gvizhip_tutte_profile(self.h, sweeps, alpha, C.byref(a), C.byref(b)) != 0:
+            _err("gvizhip_tutte_profile")
+        return a.value, b.value
+
     def close(self):
         if getattr(self, "h", None):
             lib().gvizhip_tutte_destroy(self.h)
@@ -425,6 +435,14 @@ class GripEngine:
         if lib().gvizhip_grip_refine_round(self.h, active, layer, C.byref(st)) != 0:
             _err("gvizhip_grip_refine_round")
         return st
+
+    def profile_rounds(self, active, layer, rounds):
+        """(summed spring-force kernel ms, summed round ms)."""
+        a, b = C.c_double(), C.c_double()
+        if lib().gvizhip_grip_profile_rounds(self.h, active, layer, rounds, C.byref(a),
+                                             C.byref(b)) != 0:
+            _err("gvizhip_grip_profile_rounds")
+        return a.value, b.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -120,9 +120,10 @@ int64_t gvizhip_force_quadtree_cells(gvizhip_force *f, int64_t cap,
                                      int64_t *count);
 /* Measurement hook (no reference counterpart): runs `steps` steps outside
  * the captured graph with HIP events on the engine's stream around the
- * Barnes-Hut traversal kernel and around the whole step. Outputs (may be
- * NULL): summed traversal kernel time, summed repulsion evaluations over all
- * vertices and steps, summed step time. */
+ * repulsion kernel (Barnes-Hut traversal, or all-pairs when Barnes-Hut is
+ * off) and around the whole step. Outputs (may be NULL): summed repulsion
+ * kernel time, summed repulsion evaluations over all vertices and steps,
+ * summed step time. */
 int gvizhip_force_profile(gvizhip_force *f, int64_t steps, double *traversalMs,
                           int64_t *interactions, double *stepMs);
 void gvizhip_force_destroy(gvizhip_force *f);
@@ -145,6 +146,11 @@ int gvizhip_tutte_download(gvizhip_tutte *t, double *positions);
  * max displacement. Returns the number of sweeps run. */
 int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
                              double epsilon, double *maxDeltas);
+/* Measurement hook (no reference counterpart): `sweeps` sweeps outside the
+ * captured graph, HIP events on the engine's stream around each sweep
+ * kernel. Outputs (may be NULL): summed sweep-kernel time, total time. */
+int gvizhip_tutte_profile(gvizhip_tutte *t, int64_t sweeps, double alpha,
+                          double *sweepMs, double *totalMs);
 void gvizhip_tutte_destroy(gvizhip_tutte *t);
 
 /* ------------------------------------------------------------------- GRIP */
@@ -211,6 +217,13 @@ int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
 /* gvizGRIPEmbedderRefineRound (:1289-1333) over filtration[0..active). */
 int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
                               gvizhip_grip_round_stats *stats);
+/* Measurement hook (no reference counterpart): `rounds` refinement rounds
+ * as gvizhip_grip_refine_round, with HIP events on the engine's stream
+ * around the spring-force kernel and around each whole round. Outputs (may
+ * be NULL): summed force-kernel time, summed round time. */
+int gvizhip_grip_profile_rounds(gvizhip_grip *g, int64_t active, int64_t layer,
+                                int64_t rounds, double *forceMs,
+                                double *roundMs);
 void gvizhip_grip_destroy(gvizhip_grip *g);
 
 #ifdef __cplusplus

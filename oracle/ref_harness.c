@@ -496,6 +496,87 @@ static int mode_api_grip(void) {
   return 0;
 }
 
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+/* CPU baseline for GRIP: the reference's own schedule on the subgraph
+ * induced by `active`, with its thread pool of `workers` threads. Writes
+ * the Begin time, then per stage (coarsest first) the NextStage time (0 for
+ * the first) and the times of `rounds` RefineRound calls; stops after
+ * `maxStages` stages. */
+static int mode_time_grip(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  gvizVertexSubset vs = gvizVertexSubsetCreateEmpty(&g);
+  for (int64_t i = 0; i < n; i++)
+    if (rd_i())
+      gvizVertexSubsetShowVertex(vs, (size_t)i);
+  int64_t kcap = rd_i(), rounds = rd_i(), maxStages = rd_i(), workers = rd_i();
+  gvizGRIPState st;
+  memset(&st, 0, sizeof(st));
+  gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateVertexInduced(&g, vs), 0, 2) != 0)
+    return 1;
+  gvizThreadPoolDestroy(st.pool);
+  st.pool = workers > 0 ? gvizThreadPoolCreate((size_t)workers) : NULL;
+  gvizGRIPEmbedderConfigureK(&st, (size_t)kcap, (size_t)kcap, GVIZ_GRIP_K_CONSTANT);
+  double t0 = now_s();
+  gvizGRIPEmbedderBegin(&st);
+  wr_d(now_s() - t0);
+  wr_i((int64_t)st.layerCount);
+  fflush(fout);
+  for (int64_t stage = 0; stage < maxStages; stage++) {
+    double tn = 0.0;
+    if (stage > 0) {
+      if (st.currLayer == 0)
+        break;
+      t0 = now_s();
+      gvizGRIPEmbedderNextStage(&st);
+      tn = now_s() - t0;
+    }
+    wr_i((int64_t)st.currLayer);
+    wr_i((int64_t)*(size_t *)gvizArrayAtIndex(&st.misBorder, st.currLayer));
+    wr_d(tn);
+    for (int64_t r = 0; r < rounds; r++) {
+      t0 = now_s();
+      gvizGRIPEmbedderRefineRound(&st);
+      wr_d(now_s() - t0);
+    }
+    fflush(fout);
+  }
+  wr_i(-1);
+  return 0;
+}
+
+/* CPU baseline for the Tutte sweep: `iters` Jacobi Step(1.0) calls after
+ * FixConvexPolygon + SeedInterior; writes each call's wall time. */
+static int mode_time_tutte(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t nb = rd_i(), iters = rd_i();
+  size_t *boundary = malloc(sizeof(size_t) * nb);
+  for (int64_t i = 0; i < nb; i++)
+    boundary[i] = (size_t)rd_i();
+  double radius = rd_d();
+  gvizTutteState s;
+  if (gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, 1e-300) != 0)
+    return 1;
+  gvizTutteFixConvexPolygon(&s, boundary, (size_t)nb, radius);
+  gvizTutteEmbedderSeedInterior(&s);
+  for (int64_t it = 0; it < iters; it++) {
+    double t0 = now_s();
+    gvizTutteEmbedderStep(&s, 1.0);
+    wr_d(now_s() - t0);
+    fflush(fout);
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 4) {
     fprintf(stderr, "usage: %s mode in out\n", argv[0]);
@@ -522,6 +603,10 @@ int main(int argc, char **argv) {
     rc = mode_api_force();
   else if (!strcmp(argv[1], "api_grip"))
     rc = mode_api_grip();
+  else if (!strcmp(argv[1], "time_grip"))
+    rc = mode_time_grip();
+  else if (!strcmp(argv[1], "time_tutte"))
+    rc = mode_time_tutte();
   fclose(fin);
   fclose(fout);
   return rc;
