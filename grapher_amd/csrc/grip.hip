@@ -55,13 +55,24 @@ struct BfsSlots {
 // Discovered vertices passing `filter` (and, when lowerOnly, with an id below
 // the source) are written to out[0..k) in discovery order with their depth.
 // kNN mode (stopAtK) ends the search at the k-th hit; ball mode explores the
-// whole radius and reports the total hit count (may exceed k).
+// whole radius and reports the total hit count (may exceed k). When the
+// filter admits `filterTotal` vertices (> 0), the search also ends once all
+// of them (but the source) are found: the rest of the graph holds no hit, so
+// the result is the same.
+//
+// The FIFO is walked as one stream of (queued vertex, adjacency entry)
+// pairs, 64 pairs per step: lane j takes the j-th pair after the cursor
+// (queue position `head`, entry `cur` of that vertex's row). The pairs keep
+// the sequential order (queue position, then adjacency order), so with
+// duplicates inside a step resolved in favour of the earlier lane the set of
+// fresh vertices and their order are exactly those of the one-vertex-at-a-time
+// search.
 __device__ int wave_bfs(const int32_t *__restrict__ off,
                         const int32_t *__restrict__ adj,
                         const uint8_t *__restrict__ filter, int32_t source, int k,
                         int maxDepth, bool lowerOnly, bool stopAtK,
-                        uint32_t *bits, int32_t *queue, int32_t *qdist,
-                        int32_t *outV, int32_t *outD) {
+                        int64_t filterTotal, uint32_t *bits, int32_t *queue,
+                        int32_t *qdist, int32_t *outV, int32_t *outD) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t lt = lane ? (~0ull >> (kWave - lane)) : 0ull;
   if (lane == 0) {
@@ -69,51 +80,94 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
     st_l2(queue, source);
     st_l2(qdist, 0);
   }
-  int head = 0, tail = 1, found = 0;
-  bool done = false;
+  const int64_t stopCount =
+      filterTotal > 0 ? filterTotal - ((filter && filter[source]) ? 1 : 0) : -1;
+  int head = 0, tail = 1, found = 0, cur = 0;
+  bool done = stopCount == 0;
   while (head < tail && !done) {
-    const int32_t u = __builtin_amdgcn_readfirstlane(ld_l2(queue + head));
-    const int32_t du = __builtin_amdgcn_readfirstlane(ld_l2(qdist + head));
-    head++;
-    if (maxDepth > 0 && du >= maxDepth)
-      continue;
-    const int32_t e0 = off[u], e1 = off[u + 1];
-    for (int32_t base = e0; base < e1 && !done; base += kWave) {
-      const int32_t e = base + lane;
-      const bool valid = e < e1;
-      const int32_t w = valid ? adj[e] : -1;
-      // a vertex listed twice in one adjacency: only its first entry counts
-      bool dup = false;
-      for (int o = 1; o < kWave; o++) {
-        int32_t other = __shfl_up(w, o, kWave);
-        if (lane >= o && other == w)
-          dup = true;
+    // this step's queued vertices: positions head .. head + avail - 1
+    const int avail = tail - head < kWave ? tail - head : kWave;
+    int32_t u = -1, du = 0, start = 0, deg = 0;
+    if (lane < avail) {
+      u = ld_l2(queue + head + lane);
+      du = ld_l2(qdist + head + lane);
+      if (!(maxDepth > 0 && du >= maxDepth)) {
+        start = off[u] + (lane == 0 ? cur : 0);
+        deg = off[u + 1] - start;
       }
-      bool fresh = false;
-      if (valid && !dup) {
-        const uint32_t bit = 1u << (w & 31);
-        fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
-      }
-      const uint64_t fm = __ballot(fresh);
-      const bool pass = fresh && (!filter || filter[w]) && (!lowerOnly || w < source);
-      const uint64_t pm = __ballot(pass);
-      if (pass) {
-        const int slot = found + (int)__popcll(pm & lt);
-        if (slot < k) {
-          outV[slot] = w;
-          outD[slot] = du + 1;
-        }
-      }
-      found += (int)__popcll(pm);
-      if (fresh) {
-        const int at = tail + (int)__popcll(fm & lt);
-        st_l2(queue + at, w);
-        st_l2(qdist + at, du + 1);
-      }
-      tail += (int)__popcll(fm);
-      if (stopAtK && found >= k)
-        done = true;
     }
+    // inclusive scan of the row lengths
+    int32_t incl = deg;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int32_t v = __shfl_up(incl, o, kWave);
+      if (lane >= o)
+        incl += v;
+    }
+    const int32_t total = __shfl(incl, kWave - 1, kWave);
+    const int pairs = total < kWave ? total : kWave;
+    // lane j < pairs: owner = first lane whose inclusive sum exceeds j
+    // (a fixed six-step search, every lane taking part in each shuffle)
+    int owner = 0;
+    for (int step = kWave / 2; step >= 1; step >>= 1) {
+      const int32_t v = __shfl(incl, owner + step - 1, kWave);
+      if (v <= lane)
+        owner += step;
+    }
+    const int32_t ownStart = __shfl(start, owner, kWave);
+    const int32_t ownExcl = __shfl(incl - deg, owner, kWave);
+    const int32_t ownDepth = __shfl(du, owner, kWave);
+    const bool valid = lane < pairs;
+    const int32_t w = valid ? adj[ownStart + (lane - ownExcl)] : -1;
+    // a vertex listed twice among this step's pairs: only its first counts
+    bool dup = false;
+    for (int o = 1; o < kWave; o++) {
+      const int32_t other = __shfl_up(w, o, kWave);
+      if (lane >= o && other == w)
+        dup = true;
+    }
+    bool fresh = false;
+    if (valid && !dup) {
+      const uint32_t bit = 1u << (w & 31);
+      fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
+    }
+    const uint64_t fm = __ballot(fresh);
+    const bool pass = fresh && (!filter || filter[w]) && (!lowerOnly || w < source);
+    const uint64_t pm = __ballot(pass);
+    if (pass) {
+      const int slot = found + (int)__popcll(pm & lt);
+      if (slot < k) {
+        outV[slot] = w;
+        outD[slot] = ownDepth + 1;
+      }
+    }
+    found += (int)__popcll(pm);
+    if (fresh) {
+      const int at = tail + (int)__popcll(fm & lt);
+      st_l2(queue + at, w);
+      st_l2(qdist + at, ownDepth + 1);
+    }
+    tail += (int)__popcll(fm);
+    // advance the cursor past the pairs taken
+    if (pairs == total) {
+      head += avail;
+      cur = 0;
+    } else {
+      const int last = __shfl(owner, pairs - 1, kWave);
+      const int32_t lastIncl = __shfl(incl, last, kWave);
+      const int32_t lastDeg = __shfl(deg, last, kWave);
+      const int32_t took = lastDeg - (lastIncl - pairs);
+      if (took == lastDeg) {
+        head += last + 1;
+        cur = 0;
+      } else {
+        cur = (last == 0 ? cur : 0) + took;
+        head += last;
+      }
+    }
+    if (stopAtK && found >= k)
+      done = true;
+    if (stopCount > 0 && found >= stopCount)
+      done = true;
   }
   // clear this query's bits: every visited vertex sits in the queue
   for (int i = lane; i < tail; i += kWave) {
@@ -128,8 +182,8 @@ __global__ __launch_bounds__(kB) void k_bfs_queries(
     const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
     const uint8_t *__restrict__ filter, const int32_t *__restrict__ sources,
     const int32_t *__restrict__ outIndex, int64_t count, int k, int64_t stride,
-    int maxDepth, int lowerOnly, int stopAtK, BfsSlots sl, int64_t slots,
-    int32_t *__restrict__ outV, int32_t *__restrict__ outD,
+    int maxDepth, int lowerOnly, int stopAtK, int64_t filterTotal, BfsSlots sl,
+    int64_t slots, int32_t *__restrict__ outV, int32_t *__restrict__ outD,
     int32_t *__restrict__ outCount, int32_t *__restrict__ overflow) {
   const int64_t slot = (int64_t)blockIdx.x * kWpb + threadIdx.x / kWave;
   if (slot >= slots)
@@ -141,7 +195,7 @@ __global__ __launch_bounds__(kB) void k_bfs_queries(
     const int32_t src = sources[qi];
     const int64_t o = (outIndex ? (int64_t)outIndex[qi] : qi) * stride;
     int found = wave_bfs(off, adj, filter, src, k, maxDepth, lowerOnly != 0,
-                         stopAtK != 0, bits, q, qd, outV + o, outD + o);
+                         stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o);
     if ((threadIdx.x & (kWave - 1)) == 0) {
       outCount[outIndex ? outIndex[qi] : qi] = found < k ? found : k;
       if (!stopAtK && found > k)
@@ -680,6 +734,16 @@ int ensure_slots(gvizhip_grip *g) {
   return 0;
 }
 
+// Number of vertices a host filter admits (0 for no filter: no early stop).
+int64_t count_set(const uint8_t *flags, int64_t n) {
+  if (!flags)
+    return 0;
+  int64_t c = 0;
+  for (int64_t v = 0; v < n; v++)
+    c += flags[v] != 0;
+  return c;
+}
+
 BfsSlots slot_view(gvizhip_grip *g) {
   BfsSlots v;
   v.bits = g->bits.p;
@@ -702,8 +766,8 @@ int upload_visible(gvizhip_grip *g, const uint8_t *visible) {
 // rows of `stride` selected by outIndex (device, may be null).
 int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
                 int64_t count, int k, int64_t stride, const uint8_t *filterDev,
-                int maxDepth, bool lowerOnly, bool stopAtK, int32_t *outV,
-                int32_t *outD, int32_t *outCount) {
+                int64_t filterTotal, int maxDepth, bool lowerOnly, bool stopAtK,
+                int32_t *outV, int32_t *outD, int32_t *outCount) {
   if (count == 0)
     return 0;
   if (ensure_slots(g))
@@ -713,8 +777,8 @@ int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
   hipLaunchKernelGGL(k_bfs_queries, dim3(blocks), dim3(kB), 0, g->stream,
                      g->off.p, g->adj.p, filterDev, srcDev, outIndex, count, k,
                      stride, maxDepth, lowerOnly ? 1 : 0, stopAtK ? 1 : 0,
-                     slot_view(g), (int64_t)blocks * kWpb, outV, outD, outCount,
-                     g->flags.p);
+                     filterTotal, slot_view(g), (int64_t)blocks * kWpb, outV, outD,
+                     outCount, g->flags.p);
   GVH_CHECK_LAUNCH();
   return 0;
 }
@@ -1006,7 +1070,7 @@ int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
         return -1;
       const int maxDepth = radius > INT32_MAX ? INT32_MAX : (int)radius;
       if (run_queries(g, candDev.p, nullptr, m, (int)stride, stride, candFlag.p,
-                      maxDepth, true, false, lists.p, listsD.p,
+                      0, maxDepth, true, false, lists.p, listsD.p,
                       listCount.p))
         return -1;
       int32_t ov = 0;
@@ -1120,8 +1184,8 @@ int gvizhip_grip_knn(gvizhip_grip *g, const uint8_t *visible, const int64_t *sou
     return 0;
   }
   if (run_queries(g, g->qSrc.p, nullptr, count, (int)k, k,
-                  visible ? g->visible.p : nullptr, 0, false, true, g->hitV.p,
-                  g->hitD.p, g->hitCount.p))
+                  visible ? g->visible.p : nullptr, count_set(visible, g->n), 0,
+                  false, true, g->hitV.p, g->hitD.p, g->hitCount.p))
     return -1;
   std::vector<int32_t> hv(count * k), hd(count * k), hc(count);
   GVH_CHECK(hipMemcpyAsync(hv.data(), g->hitV.p, sizeof(int32_t) * count * k,
@@ -1312,7 +1376,8 @@ int gvizhip_grip_update_knn(gvizhip_grip *g, int64_t active, const uint8_t *visi
                      g->knnV.p, g->knnD.p, g->knnCount.p);
   }
   return run_queries(g, g->filt.p, g->filt.p, active, (int)k, g->kcap, g->visible.p,
-                     0, false, true, g->knnV.p, g->knnD.p, g->knnCount.p);
+                     count_set(visible, g->n), 0, false, true, g->knnV.p, g->knnD.p,
+                     g->knnCount.p);
 }
 
 int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
@@ -1336,8 +1401,8 @@ int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
                   g->hitV.p, g->hitD.p, g->hitCount.p))
       return -1;
   } else if (run_queries(g, g->filt.p + begin, nullptr, count, (int)k, k,
-                         g->visible.p, 0, false, true, g->hitV.p, g->hitD.p,
-                         g->hitCount.p)) {
+                         g->visible.p, count_set(visible, g->n), 0, false, true,
+                         g->hitV.p, g->hitD.p, g->hitCount.p)) {
     return -1;
   }
   if (set_flag(g, 3, 0))
