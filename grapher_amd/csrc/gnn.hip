@@ -1,0 +1,704 @@
+// gnn.hip -- message-passing network that proposes initial layouts for the
+// force embedder (the learned warm start), with hand-written training and
+// inference on gfx950.
+//
+// Model (no reference counterpart; it sits in front of
+// gvizForceEmbedderBegin): input features x_v (16 wide: two random
+// coordinates, a degree term, a constant, zero padding), three layers
+//     h' = relu([h | mean_{u in N(v)} h_u] W + b)      (hidden width 64)
+// and a linear head y = h W4 + b4 (2 outputs, padded to 16). Weights and
+// activations are BF16 and every product accumulates in FP32 on the matrix
+// cores (v_mfma_f32_16x16x16_bf16); neighbour means, losses and the Adam
+// update are FP32. Training fits a force-embedder layout T up to rotation,
+// reflection and scale: with P and T centred and R, s the orthogonal
+// Procrustes fit of P to T,
+//     L = |s P R - T|^2 / |T|^2 = 1 - (sigma1 + sigma2)^2 / (|P|^2 |T|^2),
+// dL/dP = 2 s (s P R - T) R^T / |T|^2 (R and s optimal: envelope theorem).
+#include "../../include/gviz_hip.h"
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+using namespace gvh;
+
+namespace {
+
+constexpr int kF = 16;    // input feature width (padded)
+constexpr int kH = 64;    // hidden width
+constexpr int kO = 16;    // head width (2 used)
+constexpr int kB = 256;   // threads per block
+constexpr int kRows = 256; // rows per weight-gradient block
+
+// parameter layout (floats): W1 (2F x H), b1, W2 (2H x H), b2, W3, b3,
+// W4 (H x O), b4
+constexpr int64_t kOffW[4] = {0, 2 * kF * kH + kH,
+                              2 * kF * kH + kH + 2 * kH * kH + kH,
+                              2 * kF * kH + kH + 2 * (2 * kH * kH + kH)};
+constexpr int kK[4] = {2 * kF, 2 * kH, 2 * kH, kH}; // rows of W_l
+constexpr int kN[4] = {kH, kH, kH, kO};             // columns of W_l
+constexpr int64_t kParams = kOffW[3] + kH * kO + kO;
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) {
+  return __uint_as_float((uint32_t)b << 16);
+}
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// BF16 copies of the weights: Wt_l (N_l x K_l) for the forward product and
+// Wr_l (K_l x N_l) for the backward-data product.
+__global__ void k_weights_bf16(const float *__restrict__ p, uint16_t *__restrict__ wt,
+                               uint16_t *__restrict__ wr, int64_t offW, int K, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K * N)
+    return;
+  const int r = i / N, c = i % N; // W[r][c], row-major K x N
+  const uint16_t b = f2bf(p[offW + i]);
+  wr[i] = b;
+  wt[(int64_t)c * K + r] = b;
+}
+
+// M[v] = mean over neighbours of X (width F in {16, 64}); 64 / F vertices per
+// wavefront, one feature per lane.
+template <int F>
+__global__ __launch_bounds__(kB) void k_aggregate(const int32_t *__restrict__ off,
+                                                  const int32_t *__restrict__ adj,
+                                                  const float *__restrict__ invDeg,
+                                                  const uint16_t *__restrict__ X,
+                                                  uint16_t *__restrict__ M, int64_t n) {
+  constexpr int per = kWave / F;
+  const int64_t gt = (int64_t)blockIdx.x * kB + threadIdx.x;
+  const int64_t v = (gt / kWave) * per + (threadIdx.x & (kWave - 1)) / F;
+  const int f = threadIdx.x & (F - 1);
+  if (v >= n)
+    return;
+  float acc = 0.0f;
+  for (int32_t e = off[v]; e < off[v + 1]; e++)
+    acc += bf2f(X[(int64_t)adj[e] * F + f]);
+  M[v * F + f] = f2bf(acc * invDeg[v]);
+}
+
+// C = [A1 | A2] B with B given as Bt (Nc x K, BF16), one wavefront per 16
+// rows, NT column tiles of 16. Epilogue: +bias, optional relu, to BF16 (outB,
+// row stride Nc) and/or FP32 (outF, row stride Nc).
+template <int NT>
+__global__ __launch_bounds__(kB) void k_gemm(const uint16_t *__restrict__ A1,
+                                             const uint16_t *__restrict__ A2, int Ka,
+                                             int64_t n, const uint16_t *__restrict__ Bt,
+                                             int K, const float *__restrict__ bias,
+                                             int relu, uint16_t *__restrict__ outB,
+                                             float *__restrict__ outF) {
+  constexpr int Nc = 16 * NT;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t r0 = ((int64_t)blockIdx.x * (kB / kWave) + threadIdx.x / kWave) * 16;
+  if (r0 >= n)
+    return;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t row = r0 + li;
+  v4f acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+    acc[t] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+  for (int kb = 0; kb < K; kb += 16) {
+    const int kc = kb + 4 * lg;
+    v4s a = (v4s){0, 0, 0, 0};
+    if (row < n) {
+      const uint16_t *src = kc < Ka ? A1 + row * Ka + kc : A2 + row * Ka + (kc - Ka);
+      const uint2 raw = *(const uint2 *)src;
+      a = __builtin_bit_cast(v4s, raw);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const uint2 rb = *(const uint2 *)(Bt + (int64_t)(t * 16 + li) * K + kc);
+      const v4s b = __builtin_bit_cast(v4s, rb);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int col = t * 16 + li;
+    const float bcol = bias ? bias[col] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int64_t r = r0 + 4 * lg + i;
+      if (r >= n)
+        continue;
+      float z = acc[t][i] + bcol;
+      if (relu)
+        z = z > 0.0f ? z : 0.0f;
+      if (outB)
+        outB[r * Nc + col] = f2bf(z);
+      if (outF)
+        outF[r * Nc + col] = z;
+    }
+  }
+}
+
+// Per-block partial of dW = [A1 | A2]^T D over rows [b*kRows, (b+1)*kRows):
+// both operands staged in LDS, 16 x 16 output tiles spread over the
+// wavefronts, partial[b][K][Nc] in FP32.
+template <int NT>
+__global__ __launch_bounds__(kB) void k_wgrad(const uint16_t *__restrict__ A1,
+                                              const uint16_t *__restrict__ A2, int Ka,
+                                              int64_t n, const uint16_t *__restrict__ D,
+                                              int K, float *__restrict__ partial) {
+  constexpr int Nc = 16 * NT;
+  __shared__ uint16_t sA[kRows * 2 * kH];
+  __shared__ uint16_t sD[kRows * kH];
+  const int64_t rBase = (int64_t)blockIdx.x * kRows;
+  for (int i = threadIdx.x; i < kRows * K; i += kB) {
+    const int rr = i / K, c = i % K;
+    const int64_t r = rBase + rr;
+    uint16_t v = 0;
+    if (r < n)
+      v = c < Ka ? A1[r * Ka + c] : A2[r * Ka + (c - Ka)];
+    sA[i] = v;
+  }
+  for (int i = threadIdx.x; i < kRows * Nc; i += kB) {
+    const int64_t r = rBase + i / Nc;
+    sD[i] = r < n ? D[r * Nc + (i % Nc)] : (uint16_t)0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int li = lane & 15, lg = lane >> 4;
+  const int tilesM = K / 16;
+  for (int tile = wave; tile < tilesM * NT; tile += kB / kWave) {
+    const int mt = tile / NT, nt = tile % NT;
+    v4f acc = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+    for (int rs = 0; rs < kRows; rs += 16) {
+      v4s a, b;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int rr = rs + 4 * lg + j;
+        a[j] = (short)sA[rr * K + mt * 16 + li];
+        b[j] = (short)sD[rr * Nc + nt * 16 + li];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float *out = partial + (int64_t)blockIdx.x * K * Nc;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      out[(mt * 16 + 4 * lg + i) * Nc + nt * 16 + li] = acc[i];
+  }
+}
+
+// Per-block column sums of D (bias gradient), rows as k_wgrad.
+__global__ __launch_bounds__(kB) void k_colsum(const uint16_t *__restrict__ D, int Nc,
+                                               int64_t n, float *__restrict__ partial) {
+  const int64_t rBase = (int64_t)blockIdx.x * kRows;
+  for (int c = threadIdx.x; c < Nc; c += kB) {
+    float s = 0.0f;
+    for (int rr = 0; rr < kRows; rr++) {
+      const int64_t r = rBase + rr;
+      if (r < n)
+        s += bf2f(D[r * Nc + c]);
+    }
+    partial[(int64_t)blockIdx.x * Nc + c] = s;
+  }
+}
+
+// out[i] = sum over blocks of partial[b][i], in block order.
+__global__ void k_reduce_blocks(const float *__restrict__ partial, int blocks,
+                                int64_t count, float *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  float s = 0.0f;
+  for (int b = 0; b < blocks; b++)
+    s += partial[(int64_t)b * count + i];
+  out[i] = s;
+}
+
+// dZ = relu'(H) * dA for the head's input (widths 64).
+__global__ void k_mask(const float *__restrict__ dA, const uint16_t *__restrict__ H,
+                       int64_t count, uint16_t *__restrict__ dZ) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  dZ[i] = bf2f(H[i]) > 0.0f ? f2bf(dA[i]) : (uint16_t)0;
+}
+
+// Backward through [h | mean h]: dH[u] = dA[u][0:64] + sum over neighbours v
+// of dA[v][64:128] / deg(v) (the graph is symmetric), then the relu mask of
+// the layer that produced h. One wavefront per vertex.
+__global__ __launch_bounds__(kB) void k_bwd_combine(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const float *__restrict__ invDeg, const float *__restrict__ dA,
+    const uint16_t *__restrict__ H, int64_t n, uint16_t *__restrict__ dZ) {
+  const int64_t u = ((int64_t)blockIdx.x * kB + threadIdx.x) / kWave;
+  const int f = threadIdx.x & (kWave - 1);
+  if (u >= n)
+    return;
+  float s = dA[u * 2 * kH + f];
+  for (int32_t e = off[u]; e < off[u + 1]; e++) {
+    const int32_t v = adj[e];
+    s += dA[(int64_t)v * 2 * kH + kH + f] * invDeg[v];
+  }
+  const uint16_t h = H[u * kH + f];
+  dZ[u * kH + f] = bf2f(h) > 0.0f ? f2bf(s) : (uint16_t)0;
+}
+
+// Raw moments of the head output P = Y[:, 0:2] and the target T, FP64 per
+// block: sum p (2), sum t (2), sum p t^T (4), sum |p|^2, sum |t|^2.
+__global__ __launch_bounds__(kB) void k_head_moments(const float *__restrict__ Y,
+                                                     const float *__restrict__ T,
+                                                     int64_t n, double *__restrict__ part) {
+  __shared__ double sm[kB / kWave][10];
+  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kB) {
+    const double px = Y[v * kO], py = Y[v * kO + 1];
+    const double tx = T ? T[2 * v] : 0.0, ty = T ? T[2 * v + 1] : 0.0;
+    m[0] += px;
+    m[1] += py;
+    m[2] += tx;
+    m[3] += ty;
+    m[4] += px * tx;
+    m[5] += px * ty;
+    m[6] += py * tx;
+    m[7] += py * ty;
+    m[8] += px * px + py * py;
+    m[9] += tx * tx + ty * ty;
+  }
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < 10; j++) {
+    const double w = wave_sum(m[j]);
+    if (lane == 0)
+      sm[wave][j] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    double s = 0.0;
+    for (int w = 0; w < kB / kWave; w++)
+      s += sm[w][threadIdx.x];
+    part[blockIdx.x * 10 + threadIdx.x] = s;
+  }
+}
+
+struct LossFit {
+  float mpx, mpy, mtx, mty; // means
+  float r00, r01, r10, r11; // R
+  float s, scale;           // s, 2 s / |T|^2
+};
+
+// dY[v] = 2 s (s (p - mp) R - (t - mt)) R^T / |T|^2 in columns 0..1; zeros
+// elsewhere. (The row mean of this gradient is zero, so the centring of P
+// needs no correction.)
+__global__ void k_loss_grad(const float *__restrict__ Y, const float *__restrict__ T,
+                            int64_t n, LossFit fit, uint16_t *__restrict__ dY) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  const float px = Y[v * kO] - fit.mpx, py = Y[v * kO + 1] - fit.mpy;
+  const float tx = T[2 * v] - fit.mtx, ty = T[2 * v + 1] - fit.mty;
+  // e = s p R - t  (row vector times R)
+  const float ex = fit.s * (px * fit.r00 + py * fit.r10) - tx;
+  const float ey = fit.s * (px * fit.r01 + py * fit.r11) - ty;
+  // g = scale * e R^T
+  const float gx = fit.scale * (ex * fit.r00 + ey * fit.r01);
+  const float gy = fit.scale * (ex * fit.r10 + ey * fit.r11);
+  dY[v * kO] = f2bf(gx);
+  dY[v * kO + 1] = f2bf(gy);
+  for (int c = 2; c < kO; c++)
+    dY[v * kO + c] = 0;
+}
+
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, int64_t count,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count)
+    return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+} // namespace
+
+struct gvizhip_gnn {
+  int64_t cap = 0, capNnz = 0, n = 0;
+  hipStream_t stream = nullptr;
+  DevBuf<int32_t> off, adj;
+  DevBuf<float> invDeg, feat32;
+  DevBuf<uint16_t> X0, M0, H[3], M[2], dY, dZ;
+  DevBuf<float> Y, dA, target;
+  DevBuf<float> params, grads, adamM, adamV;
+  DevBuf<uint16_t> wt[4], wr[4];
+  DevBuf<float> partial;
+  DevBuf<double> moments;
+  int64_t step = 0;
+  bool weightsDirty = true;
+};
+
+namespace {
+
+int rows_blocks(int64_t n) { return (int)((n + kRows - 1) / kRows); }
+
+int refresh_weights(gvizhip_gnn *g) {
+  if (!g->weightsDirty)
+    return 0;
+  for (int l = 0; l < 4; l++) {
+    const int cnt = kK[l] * kN[l];
+    hipLaunchKernelGGL(k_weights_bf16, dim3(grid_for(cnt, kB)), dim3(kB), 0, g->stream,
+                       g->params.p, g->wt[l].p, g->wr[l].p, kOffW[l], kK[l], kN[l]);
+  }
+  GVH_CHECK_LAUNCH();
+  g->weightsDirty = false;
+  return 0;
+}
+
+template <int NT>
+void launch_gemm(gvizhip_gnn *g, const uint16_t *A1, const uint16_t *A2, int Ka,
+                 const uint16_t *Bt, int K, const float *bias, int relu, uint16_t *outB,
+                 float *outF) {
+  const int64_t waves = (g->n + 15) / 16;
+  const unsigned blocks = (unsigned)((waves + kB / kWave - 1) / (kB / kWave));
+  hipLaunchKernelGGL(k_gemm<NT>, dim3(blocks), dim3(kB), 0, g->stream, A1, A2, Ka, g->n,
+                     Bt, K, bias, relu, outB, outF);
+}
+
+int forward(gvizhip_gnn *g) {
+  if (refresh_weights(g))
+    return -1;
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  const float *b[4];
+  for (int l = 0; l < 4; l++)
+    b[l] = g->params.p + kOffW[l] + kK[l] * kN[l];
+  // layer 1
+  hipLaunchKernelGGL(k_aggregate<kF>, dim3(grid_for(n * kF, kB)), dim3(kB), 0, st,
+                     g->off.p, g->adj.p, g->invDeg.p, g->X0.p, g->M0.p, n);
+  launch_gemm<4>(g, g->X0.p, g->M0.p, kF, g->wt[0].p, 2 * kF, b[0], 1, g->H[0].p, nullptr);
+  // layers 2 and 3
+  for (int l = 1; l < 3; l++) {
+    hipLaunchKernelGGL(k_aggregate<kH>, dim3(grid_for(n * kH, kB)), dim3(kB), 0, st,
+                       g->off.p, g->adj.p, g->invDeg.p, g->H[l - 1].p, g->M[l - 1].p, n);
+    launch_gemm<4>(g, g->H[l - 1].p, g->M[l - 1].p, kH, g->wt[l].p, 2 * kH, b[l], 1,
+                   g->H[l].p, nullptr);
+  }
+  // head
+  launch_gemm<1>(g, g->H[2].p, g->H[2].p, kH, g->wt[3].p, kH, b[3], 0, nullptr, g->Y.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+// Procrustes fit of the head output to the target: loss and the epilogue
+// constants of k_loss_grad.
+int fit_loss(gvizhip_gnn *g, double *lossOut, LossFit *fitOut) {
+  const int blocks = (int)std::min<int64_t>(grid_for(g->n, kB), 256);
+  hipLaunchKernelGGL(k_head_moments, dim3(blocks), dim3(kB), 0, g->stream, g->Y.p,
+                     g->target.p, g->n, g->moments.p);
+  GVH_CHECK_LAUNCH();
+  std::vector<double> part(10 * blocks);
+  GVH_CHECK(hipMemcpyAsync(part.data(), g->moments.p, sizeof(double) * 10 * blocks,
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  double m[10] = {0};
+  for (int b = 0; b < blocks; b++)
+    for (int j = 0; j < 10; j++)
+      m[j] += part[b * 10 + j];
+  const double nn = (double)g->n;
+  const double mpx = m[0] / nn, mpy = m[1] / nn, mtx = m[2] / nn, mty = m[3] / nn;
+  // centred cross moments M = P^T T and squared norms
+  const double a = m[4] - nn * mpx * mtx, bq = m[5] - nn * mpx * mty;
+  const double c = m[6] - nn * mpy * mtx, d = m[7] - nn * mpy * mty;
+  const double pp = m[8] - nn * (mpx * mpx + mpy * mpy);
+  const double tt = m[9] - nn * (mtx * mtx + mty * mty);
+  // best rotation or reflection maximising tr(R^T M)
+  const double rotV = std::hypot(a + d, c - bq), refV = std::hypot(a - d, bq + c);
+  double r00, r01, r10, r11, tr;
+  if (rotV >= refV) {
+    const double th = std::atan2(c - bq, a + d);
+    r00 = std::cos(th), r01 = -std::sin(th), r10 = std::sin(th), r11 = std::cos(th);
+    tr = rotV;
+  } else {
+    const double th = std::atan2(bq + c, a - d);
+    r00 = std::cos(th), r01 = std::sin(th), r10 = std::sin(th), r11 = -std::cos(th);
+    tr = refV;
+  }
+  const double s = pp > 0.0 ? tr / pp : 0.0;
+  const double loss = (tt > 0.0 && pp > 0.0) ? 1.0 - tr * tr / (pp * tt) : 1.0;
+  if (lossOut)
+    *lossOut = loss;
+  if (fitOut) {
+    fitOut->mpx = (float)mpx, fitOut->mpy = (float)mpy;
+    fitOut->mtx = (float)mtx, fitOut->mty = (float)mty;
+    fitOut->r00 = (float)r00, fitOut->r01 = (float)r01;
+    fitOut->r10 = (float)r10, fitOut->r11 = (float)r11;
+    fitOut->s = (float)s;
+    fitOut->scale = tt > 0.0 ? (float)(2.0 * s / tt) : 0.0f;
+  }
+  return 0;
+}
+
+template <int NT>
+int weight_grad(gvizhip_gnn *g, const uint16_t *A1, const uint16_t *A2, int Ka, int K,
+                const uint16_t *D, int layer) {
+  constexpr int Nc = 16 * NT;
+  const int blocks = rows_blocks(g->n);
+  hipLaunchKernelGGL(k_wgrad<NT>, dim3(blocks), dim3(kB), 0, g->stream, A1, A2, Ka, g->n,
+                     D, K, g->partial.p);
+  hipLaunchKernelGGL(k_reduce_blocks, dim3(grid_for((int64_t)K * Nc, kB)), dim3(kB), 0,
+                     g->stream, g->partial.p, blocks, (int64_t)K * Nc,
+                     g->grads.p + kOffW[layer]);
+  hipLaunchKernelGGL(k_colsum, dim3(blocks), dim3(kB), 0, g->stream, D, Nc, g->n,
+                     g->partial.p);
+  hipLaunchKernelGGL(k_reduce_blocks, dim3(1), dim3(kB), 0, g->stream, g->partial.p,
+                     blocks, (int64_t)Nc, g->grads.p + kOffW[layer] + (int64_t)K * Nc);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+// Gradients of the loss with respect to every parameter (after forward and
+// fit_loss).
+int backward(gvizhip_gnn *g, const LossFit &fit) {
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  hipLaunchKernelGGL(k_loss_grad, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->Y.p,
+                     g->target.p, n, fit, g->dY.p);
+  // head: dW4 = H3^T dY, dH3 = dY W4^T
+  if (weight_grad<1>(g, g->H[2].p, g->H[2].p, kH, kH, g->dY.p, 3))
+    return -1;
+  launch_gemm<4>(g, g->dY.p, g->dY.p, kO, g->wr[3].p, kO, nullptr, 0, nullptr, g->dA.p);
+  hipLaunchKernelGGL(k_mask, dim3(grid_for(n * kH, kB)), dim3(kB), 0, st, g->dA.p,
+                     g->H[2].p, n * kH, g->dZ.p);
+  // layers 3 and 2: dW = [H | M]^T dZ, then dZ of the layer below
+  for (int l = 2; l >= 1; l--) {
+    if (weight_grad<4>(g, g->H[l - 1].p, g->M[l - 1].p, kH, 2 * kH, g->dZ.p, l))
+      return -1;
+    launch_gemm<8>(g, g->dZ.p, g->dZ.p, kH, g->wr[l].p, kH, nullptr, 0, nullptr, g->dA.p);
+    hipLaunchKernelGGL(k_bwd_combine, dim3(grid_for(n * kWave, kB)), dim3(kB), 0, st,
+                       g->off.p, g->adj.p, g->invDeg.p, g->dA.p, g->H[l - 1].p, n,
+                       g->dZ.p);
+  }
+  // layer 1
+  if (weight_grad<4>(g, g->X0.p, g->M0.p, kF, 2 * kF, g->dZ.p, 0))
+    return -1;
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int check_graph(gvizhip_gnn *g) {
+  if (!g || g->n <= 0) {
+    set_error("gvizhip_gnn: no graph set");
+    return -1;
+  }
+  return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int gvizhip_gnn_create(gvizhip_gnn **out, int64_t maxVertices, int64_t maxEntries) {
+  if (!out || maxVertices <= 0 || maxEntries < 0 || maxVertices > INT32_MAX ||
+      maxEntries > INT32_MAX) {
+    set_error("gvizhip_gnn_create: invalid arguments");
+    return -1;
+  }
+  *out = nullptr;
+  int dev = 0;
+  if (hipGetDeviceCount(&dev) != hipSuccess || dev <= 0) {
+    set_error("gvizhip_gnn_create: no GPU");
+    return -1;
+  }
+  auto *g = new gvizhip_gnn;
+  const int64_t n = maxVertices;
+  g->cap = n;
+  g->capNnz = maxEntries;
+  const int64_t rb = (n + kRows - 1) / kRows;
+  bool bad = hipStreamCreate(&g->stream) != hipSuccess || g->off.alloc(n + 1) ||
+             g->adj.alloc(maxEntries) || g->invDeg.alloc(n) || g->X0.alloc(n * kF) ||
+             g->M0.alloc(n * kF) || g->dY.alloc(n * kO) || g->dZ.alloc(n * kH) ||
+             g->Y.alloc(n * kO) || g->dA.alloc(n * 2 * kH) || g->target.alloc(2 * n) ||
+             g->feat32.alloc(1) || g->params.alloc(kParams) || g->grads.alloc(kParams) ||
+             g->adamM.alloc(kParams) || g->adamV.alloc(kParams) ||
+             g->partial.alloc(rb * 2 * kH * kH) || g->moments.alloc(10 * 256);
+  for (int l = 0; l < 3 && !bad; l++)
+    bad = g->H[l].alloc(n * kH) != 0;
+  for (int l = 0; l < 2 && !bad; l++)
+    bad = g->M[l].alloc(n * kH) != 0;
+  for (int l = 0; l < 4 && !bad; l++)
+    bad = g->wt[l].alloc(kK[l] * kN[l]) || g->wr[l].alloc(kK[l] * kN[l]);
+  if (bad) {
+    if (!gvizhip_last_error()[0])
+      set_error("gvizhip_gnn_create: allocation failed");
+    gvizhip_gnn_destroy(g);
+    return -1;
+  }
+  GVH_CHECK(hipMemsetAsync(g->params.p, 0, sizeof(float) * kParams, g->stream));
+  GVH_CHECK(hipMemsetAsync(g->adamM.p, 0, sizeof(float) * kParams, g->stream));
+  GVH_CHECK(hipMemsetAsync(g->adamV.p, 0, sizeof(float) * kParams, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  *out = g;
+  return 0;
+}
+
+int64_t gvizhip_gnn_param_count(void) { return kParams; }
+
+int gvizhip_gnn_set_params(gvizhip_gnn *g, const float *params) {
+  if (!g || !params) {
+    set_error("gvizhip_gnn_set_params: invalid arguments");
+    return -1;
+  }
+  GVH_CHECK(hipMemcpyAsync(g->params.p, params, sizeof(float) * kParams,
+                           hipMemcpyHostToDevice, g->stream));
+  GVH_CHECK(hipMemsetAsync(g->adamM.p, 0, sizeof(float) * kParams, g->stream));
+  GVH_CHECK(hipMemsetAsync(g->adamV.p, 0, sizeof(float) * kParams, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  g->step = 0;
+  g->weightsDirty = true;
+  return 0;
+}
+
+int gvizhip_gnn_get_params(gvizhip_gnn *g, float *params) {
+  if (!g || !params) {
+    set_error("gvizhip_gnn_get_params: invalid arguments");
+    return -1;
+  }
+  GVH_CHECK(hipMemcpyAsync(params, g->params.p, sizeof(float) * kParams,
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int gvizhip_gnn_set_graph(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
+                          const int64_t *adj, const float *features) {
+  if (!g || n <= 0 || n > g->cap || !offsets || !features ||
+      offsets[n] > g->capNnz || offsets[0] != 0 || (offsets[n] > 0 && !adj)) {
+    set_error("gvizhip_gnn_set_graph: invalid arguments");
+    return -1;
+  }
+  const int64_t m = offsets[n];
+  std::vector<int32_t> o(n + 1), a(m > 0 ? m : 1);
+  std::vector<float> inv(n);
+  std::vector<uint16_t> x(n * kF);
+  for (int64_t v = 0; v <= n; v++)
+    o[v] = (int32_t)offsets[v];
+  for (int64_t v = 0; v < n; v++) {
+    if (offsets[v + 1] < offsets[v]) {
+      set_error("gvizhip_gnn_set_graph: offsets are not monotone");
+      return -1;
+    }
+    const int64_t d = offsets[v + 1] - offsets[v];
+    inv[v] = d > 0 ? 1.0f / (float)d : 0.0f;
+  }
+  for (int64_t e = 0; e < m; e++) {
+    if (adj[e] < 0 || adj[e] >= n) {
+      set_error("gvizhip_gnn_set_graph: neighbour id out of range");
+      return -1;
+    }
+    a[e] = (int32_t)adj[e];
+  }
+  for (int64_t i = 0; i < n * kF; i++) {
+    const uint32_t u = [&] {
+      uint32_t b;
+      std::memcpy(&b, &features[i], 4);
+      return b;
+    }();
+    x[i] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+  }
+  g->n = n;
+  GVH_CHECK(hipMemcpyAsync(g->off.p, o.data(), sizeof(int32_t) * (n + 1),
+                           hipMemcpyHostToDevice, g->stream));
+  if (m > 0)
+    GVH_CHECK(hipMemcpyAsync(g->adj.p, a.data(), sizeof(int32_t) * m,
+                             hipMemcpyHostToDevice, g->stream));
+  GVH_CHECK(hipMemcpyAsync(g->invDeg.p, inv.data(), sizeof(float) * n,
+                           hipMemcpyHostToDevice, g->stream));
+  GVH_CHECK(hipMemcpyAsync(g->X0.p, x.data(), sizeof(uint16_t) * n * kF,
+                           hipMemcpyHostToDevice, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int gvizhip_gnn_forward(gvizhip_gnn *g, double *positions) {
+  if (check_graph(g) || forward(g))
+    return -1;
+  if (positions) {
+    std::vector<float> y(g->n * kO);
+    GVH_CHECK(hipMemcpyAsync(y.data(), g->Y.p, sizeof(float) * g->n * kO,
+                             hipMemcpyDeviceToHost, g->stream));
+    GVH_CHECK(hipStreamSynchronize(g->stream));
+    for (int64_t v = 0; v < g->n; v++) {
+      positions[2 * v] = y[v * kO];
+      positions[2 * v + 1] = y[v * kO + 1];
+    }
+  }
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+static int upload_target(gvizhip_gnn *g, const double *target) {
+  std::vector<float> t(2 * g->n);
+  for (int64_t i = 0; i < 2 * g->n; i++)
+    t[i] = (float)target[i];
+  GVH_CHECK(hipMemcpyAsync(g->target.p, t.data(), sizeof(float) * 2 * g->n,
+                           hipMemcpyHostToDevice, g->stream));
+  return 0;
+}
+
+int gvizhip_gnn_gradients(gvizhip_gnn *g, const double *target, double *loss,
+                          float *grads) {
+  if (check_graph(g) || !target) {
+    set_error("gvizhip_gnn_gradients: invalid arguments");
+    return -1;
+  }
+  LossFit fit;
+  if (upload_target(g, target) || forward(g) || fit_loss(g, loss, &fit) ||
+      backward(g, fit))
+    return -1;
+  if (grads) {
+    GVH_CHECK(hipMemcpyAsync(grads, g->grads.p, sizeof(float) * kParams,
+                             hipMemcpyDeviceToHost, g->stream));
+  }
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int gvizhip_gnn_train_step(gvizhip_gnn *g, const double *target, double lr,
+                           double *loss) {
+  if (check_graph(g) || !target || !(lr > 0.0)) {
+    set_error("gvizhip_gnn_train_step: invalid arguments");
+    return -1;
+  }
+  LossFit fit;
+  if (upload_target(g, target) || forward(g) || fit_loss(g, loss, &fit) ||
+      backward(g, fit))
+    return -1;
+  g->step++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)g->step);
+  const float c2 = 1.0f - std::pow(b2, (float)g->step);
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(kParams, kB)), dim3(kB), 0, g->stream,
+                     g->params.p, g->grads.p, g->adamM.p, g->adamV.p, kParams, (float)lr,
+                     b1, b2, 1e-8f, c1, c2);
+  GVH_CHECK_LAUNCH();
+  g->weightsDirty = true;
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+void gvizhip_gnn_destroy(gvizhip_gnn *g) {
+  if (!g)
+    return;
+  if (g->stream) {
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipStreamDestroy(g->stream);
+  }
+  delete g;
+}
+
+} // extern "C"

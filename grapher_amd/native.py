@@ -122,6 +122,18 @@ def lib():
         "gvizhip_grip_refine_round": (C.c_int, [_vp, _i64, _i64, C.POINTER(GripRoundStats)]),
         "gvizhip_grip_profile_rounds": (C.c_int, [_vp, _i64, _i64, _i64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gvizhip_grip_destroy": (None, [_vp]),
+        "gvizhip_stress_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
+        "gvizhip_stress_evaluate": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
+        "gvizhip_stress_destroy": (None, [_vp]),
+        "gvizhip_gnn_create": (C.c_int, [C.POINTER(_vp), _i64, _i64]),
+        "gvizhip_gnn_param_count": (_i64, []),
+        "gvizhip_gnn_set_params": (C.c_int, [_vp, _vp]),
+        "gvizhip_gnn_get_params": (C.c_int, [_vp, _vp]),
+        "gvizhip_gnn_set_graph": (C.c_int, [_vp, _i64, _vp, _vp, _vp]),
+        "gvizhip_gnn_forward": (C.c_int, [_vp, _vp]),
+        "gvizhip_gnn_gradients": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), _vp]),
+        "gvizhip_gnn_train_step": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
+        "gvizhip_gnn_destroy": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
@@ -151,6 +163,10 @@ DECLARED = [
     "gvizhip_grip_clear_decorators", "gvizhip_grip_update_knn",
     "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_profile_rounds",
     "gvizhip_grip_destroy",
+    "gvizhip_stress_create", "gvizhip_stress_evaluate", "gvizhip_stress_destroy",
+    "gvizhip_gnn_create", "gvizhip_gnn_param_count", "gvizhip_gnn_set_params",
+    "gvizhip_gnn_get_params", "gvizhip_gnn_set_graph", "gvizhip_gnn_forward",
+    "gvizhip_gnn_gradients", "gvizhip_gnn_train_step", "gvizhip_gnn_destroy",
 ]
 
 
@@ -447,6 +463,100 @@ class GripEngine:
     def close(self):
         if getattr(self, "h", None):
             lib().gvizhip_grip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class GnnEngine:
+    """The warm-start network on the device: gvizhip_gnn_*."""
+
+    def __init__(self, max_vertices, max_entries):
+        h = _vp()
+        if lib().gvizhip_gnn_create(C.byref(h), int(max_vertices), int(max_entries)) != 0:
+            _err("gvizhip_gnn_create")
+        self.h = h
+        self.count = int(lib().gvizhip_gnn_param_count())
+        self.n = 0
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.shape == (self.count,)
+        if lib().gvizhip_gnn_set_params(self.h, _p(t)) != 0:
+            _err("gvizhip_gnn_set_params")
+
+    def params(self):
+        t = np.zeros(self.count, dtype=np.float32)
+        if lib().gvizhip_gnn_get_params(self.h, _p(t)) != 0:
+            _err("gvizhip_gnn_get_params")
+        return t
+
+    def set_graph(self, offsets, adj, features):
+        o = np.ascontiguousarray(offsets, dtype=np.int64)
+        a = np.ascontiguousarray(adj, dtype=np.int64)
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        n = o.shape[0] - 1
+        assert f.shape == (n, 16)
+        if lib().gvizhip_gnn_set_graph(self.h, n, _p(o), _p(a) if a.size else None,
+                                       _p(f)) != 0:
+            _err("gvizhip_gnn_set_graph")
+        self.n = n
+
+    def forward(self):
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_gnn_forward(self.h, _p(out)) != 0:
+            _err("gvizhip_gnn_forward")
+        return out
+
+    def gradients(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        loss = C.c_double()
+        g = np.zeros(self.count, dtype=np.float32)
+        if lib().gvizhip_gnn_gradients(self.h, _p(t), C.byref(loss), _p(g)) != 0:
+            _err("gvizhip_gnn_gradients")
+        return loss.value, g
+
+    def train_step(self, target, lr):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        loss = C.c_double()
+        if lib().gvizhip_gnn_train_step(self.h, _p(t), float(lr), C.byref(loss)) != 0:
+            _err("gvizhip_gnn_train_step")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class StressMeter:
+    """Pivot-based normalised stress on the device: gvizhip_stress_*."""
+
+    def __init__(self, offsets, adj, pivots):
+        o = np.ascontiguousarray(offsets, dtype=np.int64)
+        a = np.ascontiguousarray(adj, dtype=np.int64)
+        p = np.ascontiguousarray(pivots, dtype=np.int64)
+        self.n = o.shape[0] - 1
+        h = _vp()
+        if lib().gvizhip_stress_create(C.byref(h), self.n, _p(o), _p(a) if a.size else None,
+                                       _p(p), p.shape[0]) != 0:
+            _err("gvizhip_stress_create")
+        self.h = h
+
+    def __call__(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 2)
+        out = C.c_double()
+        if lib().gvizhip_stress_evaluate(self.h, _p(pos), C.byref(out)) != 0:
+            _err("gvizhip_stress_evaluate")
+        return out.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_stress_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -226,6 +226,51 @@ int gvizhip_grip_profile_rounds(gvizhip_grip *g, int64_t active, int64_t layer,
                                 double *roundMs);
 void gvizhip_grip_destroy(gvizhip_grip *g);
 
+/* ---------------------------------------------------------------- metrics */
+/* No reference counterpart: layout quality for judging runs. Normalised
+ * stress over BFS hop distances from `count` pivot vertices:
+ * 1 - A^2 / (B C), A = sum D/d, B = sum D^2/d^2, C = pairs, over every pivot
+ * s and every vertex j it reaches (D = |x_j - x_s|, d = hops). Distances are
+ * computed once at create. */
+typedef struct gvizhip_stress gvizhip_stress;
+int gvizhip_stress_create(gvizhip_stress **out, int64_t n, const int64_t *offsets,
+                          const int64_t *adj, const int64_t *pivots,
+                          int64_t count);
+int gvizhip_stress_evaluate(gvizhip_stress *s, const double *positions,
+                            double *stress);
+void gvizhip_stress_destroy(gvizhip_stress *s);
+
+/* -------------------------------------------------------- learned start */
+/* No reference counterpart: a message-passing network whose output replaces
+ * the random positions of gvizForceEmbedderBegin
+ * (src/embedders/gvizForceEmbedder.c:412-443) as the force embedder's start.
+ * Three layers h' = relu([h | mean of neighbours' h] W + b), hidden width
+ * 64, BF16 weights and activations on the matrix cores with FP32
+ * accumulation, a linear head to 2 coordinates. Training fits layouts up to
+ * rotation, reflection and scale (Procrustes-aligned squared error divided
+ * by |T|^2). Parameters are one FP32 vector of gvizhip_gnn_param_count()
+ * entries: W1 (32x64), b1, W2 (128x64), b2, W3 (128x64), b3, W4 (64x16),
+ * b4 (16); rows of W are inputs, columns outputs, row-major. */
+typedef struct gvizhip_gnn gvizhip_gnn;
+int gvizhip_gnn_create(gvizhip_gnn **out, int64_t maxVertices,
+                       int64_t maxEntries);
+int64_t gvizhip_gnn_param_count(void);
+int gvizhip_gnn_set_params(gvizhip_gnn *g, const float *params);
+int gvizhip_gnn_get_params(gvizhip_gnn *g, float *params);
+/* Graph (CSR over 0..n-1, symmetric) and input features (n x 16 FP32, row
+ * major; rounded to BF16). */
+int gvizhip_gnn_set_graph(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
+                          const int64_t *adj, const float *features);
+/* Network output (n x 2, interleaved) for the current graph. */
+int gvizhip_gnn_forward(gvizhip_gnn *g, double *positions);
+/* Loss against target (n x 2) and the gradient of every parameter. */
+int gvizhip_gnn_gradients(gvizhip_gnn *g, const double *target, double *loss,
+                          float *grads);
+/* One Adam step (beta 0.9 / 0.999) on the loss against target. */
+int gvizhip_gnn_train_step(gvizhip_gnn *g, const double *target, double lr,
+                           double *loss);
+void gvizhip_gnn_destroy(gvizhip_gnn *g);
+
 #ifdef __cplusplus
 }
 #endif

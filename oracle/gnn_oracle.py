@@ -1,0 +1,151 @@
+"""NumPy restatement of the warm-start network's forward and backward passes.
+
+TEST INFRASTRUCTURE ONLY: the checker for grapher_amd/csrc/gnn.hip (tests,
+bench cpu_baseline). Nothing in the product imports it.
+
+The model has no counterpart in the reference (it proposes the start that
+gvizForceEmbedderBegin, src/embedders/gvizForceEmbedder.c:412-443, would
+otherwise draw at random):
+
+    x (n x 16) -> h1 = relu([x  | A x ] W1 + b1)
+               -> h2 = relu([h1 | A h1] W2 + b2)
+               -> h3 = relu([h2 | A h2] W3 + b3)
+               -> y  = h3 W4 + b4,   P = y[:, :2]
+
+with A the row-normalised adjacency (mean over neighbours; 0 for isolated
+vertices). Loss: L = |s Pc R - Tc|^2 / |Tc|^2 with Pc, Tc centred and (R, s)
+the orthogonal Procrustes fit (rotation or reflection) of Pc to Tc.
+
+`bf16=True` rounds weights, inputs, activations and back-propagated signals
+to bfloat16 at the points where the GPU kernels store them, so the GPU's
+gradients can be compared closely; `bf16=False` is exact FP64, the form the
+finite-difference checks pin.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+F, H, O = 16, 64, 16
+SHAPES = [(2 * F, H), (2 * H, H), (2 * H, H), (H, O)]
+PARAMS = sum(k * n + n for k, n in SHAPES)
+
+
+def unpack(theta):
+    """Split the flat parameter vector into [(W, b)] per layer."""
+    out, o = [], 0
+    for k, n in SHAPES:
+        W = theta[o:o + k * n].reshape(k, n)
+        o += k * n
+        b = theta[o:o + n]
+        o += n
+        out.append((W, b))
+    return out
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even) and back, as float64."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def mean_agg(off, adj, X):
+    n = len(off) - 1
+    deg = np.diff(off)
+    S = np.zeros_like(X)
+    np.add.at(S, np.repeat(np.arange(n), deg), X[adj])
+    inv = np.where(deg > 0, 1.0 / np.maximum(deg, 1), 0.0)
+    return S * inv[:, None], inv
+
+
+def mean_agg_T(off, adj, D, inv):
+    """Transpose of mean_agg for a symmetric graph: out[u] = sum over
+    neighbours v of D[v] / deg(v)."""
+    n = len(off) - 1
+    deg = np.diff(off)
+    S = np.zeros_like(D)
+    np.add.at(S, np.repeat(np.arange(n), deg), (D * inv[:, None])[adj])
+    return S
+
+
+def forward(theta, off, adj, X, rb=False):
+    r = bf16 if rb else (lambda a: np.asarray(a, dtype=np.float64))
+    layers = unpack(np.asarray(theta, dtype=np.float64))
+    h = r(X)
+    cache = []
+    for li, (W, b) in enumerate(layers[:3]):
+        m, inv = mean_agg(off, adj, h)
+        m = r(m)
+        a = np.concatenate([h, m], 1)
+        z = a @ r(W) + b
+        hn = r(np.maximum(z, 0.0))
+        cache.append((a, hn, inv))
+        h = hn
+    W4, b4 = layers[3]
+    y = h @ r(W4) + b4
+    return y, cache
+
+
+def procrustes(P, T):
+    """(loss, R, s, Pc, Tc) of the orthogonal Procrustes fit of P to T."""
+    Pc = P - P.mean(0)
+    Tc = T - T.mean(0)
+    M = Pc.T @ Tc
+    U, S, Vt = np.linalg.svd(M)
+    R = U @ Vt
+    pp = (Pc ** 2).sum()
+    tt = (Tc ** 2).sum()
+    tr = S.sum()
+    s = tr / pp
+    loss = 1.0 - tr * tr / (pp * tt)
+    return loss, R, s, Pc, Tc
+
+
+def loss_and_grads(theta, off, adj, X, T, rb=False):
+    """Loss and the gradient of every parameter (flat, like theta)."""
+    r = bf16 if rb else (lambda a: np.asarray(a, dtype=np.float64))
+    theta = np.asarray(theta, dtype=np.float64)
+    layers = unpack(theta)
+    y, cache = forward(theta, off, adj, X, rb)
+    P = y[:, :2]
+    loss, R, s, Pc, Tc = procrustes(P, T)
+    tt = (Tc ** 2).sum()
+    dP = 2.0 * s * (s * Pc @ R - Tc) @ R.T / tt
+    dY = np.zeros_like(y)
+    dY[:, :2] = dP
+    dY = r(dY)
+    grads = []
+    h3 = cache[2][1]
+    W4 = r(layers[3][0])
+    grads.append((h3.T @ dY, dY.sum(0)))
+    dh = dY @ W4.T
+    dz = r(np.where(h3 > 0, dh, 0.0))
+    for li in (2, 1, 0):
+        a, hn, inv = cache[li]
+        W = r(layers[li][0])
+        grads.append((a.T @ dz, dz.sum(0)))
+        if li == 0:
+            break
+        da = dz @ W.T
+        k = a.shape[1] // 2
+        dh = da[:, :k] + mean_agg_T(off, adj, da[:, k:], inv)
+        hprev = cache[li - 1][1]
+        dz = r(np.where(hprev > 0, dh, 0.0))
+    grads = grads[::-1]  # layer order 1..4
+    flat = np.concatenate([np.concatenate([g[0].ravel(), g[1]]) for g in grads])
+    return loss, flat
+
+
+def init_params(seed=0):
+    """He-scaled normal weights, zero biases; head columns 2..15 zero."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for li, (k, n) in enumerate(SHAPES):
+        W = rng.normal(0.0, np.sqrt(2.0 / k), size=(k, n))
+        if li == 3:
+            W[:, 2:] = 0.0
+            W *= 0.5
+        parts.append(W.ravel())
+        parts.append(np.zeros(n))
+    return np.concatenate(parts)

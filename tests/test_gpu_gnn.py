@@ -1,0 +1,95 @@
+"""The warm-start network on the GPU against its CPU restatement
+(oracle/gnn_oracle.py, itself pinned by finite differences), the device
+stress meter against the oracle's stress, and training on force-embedder
+layouts.
+
+Tolerances: weights and activations are BF16 on both sides (the oracle
+rounds at the same points), products accumulate in FP32 on the GPU and in
+FP64 in the oracle, so a rounding boundary can fall differently; outputs
+agree to 1e-2 normwise and gradients to 3e-2 normwise per parameter block."""
+import numpy as np
+import pytest
+
+import gnn_oracle as N
+import pyoracle as O
+from grapher_amd import graphs as G
+from grapher_amd import warmstart as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed=0, n=600, blocks=6):
+    n, e, _ = W.sbm(n, blocks, seed)
+    off, adj = G.csr_from_edges(n, e)
+    x = W.features(off, seed + 11)
+    return n, off, adj, x
+
+
+def _theta(seed):
+    t = N.init_params(seed)
+    t[N.PARAMS - N.O:N.PARAMS - N.O + 2] = [0.05, -0.03]
+    return t.astype(np.float32)
+
+
+def _norm_err(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_matches_oracle(gpu):
+    n, off, adj, x = _case()
+    th = _theta(1)
+    eng = gpu.GnnEngine(n, len(adj))
+    eng.set_params(th)
+    eng.set_graph(off, adj, x)
+    y_gpu = eng.forward()
+    y_ref = N.forward(th.astype(np.float64), off, adj, x.astype(np.float64), rb=True)[0][:, :2]
+    assert _norm_err(y_gpu, y_ref) <= 1e-2, _norm_err(y_gpu, y_ref)
+
+
+def test_gradients_match_oracle(gpu):
+    n, off, adj, x = _case(2, 900, 9)
+    th = _theta(3)
+    target = np.random.default_rng(5).normal(size=(n, 2)) * 40.0
+    eng = gpu.GnnEngine(n, len(adj))
+    eng.set_params(th)
+    eng.set_graph(off, adj, x)
+    loss, g = eng.gradients(target)
+    loss_ref, g_ref = N.loss_and_grads(th.astype(np.float64), off, adj,
+                                       x.astype(np.float64), target, rb=True)
+    assert abs(loss - loss_ref) <= 1e-3 * max(abs(loss_ref), 1e-3)
+    o = 0
+    for k, m in N.SHAPES:
+        for name, size in (("W", k * m), ("b", m)):
+            a, b = g[o:o + size], g_ref[o:o + size]
+            o += size
+            if np.linalg.norm(b) == 0:
+                assert np.abs(a).max() == 0.0
+                continue
+            assert _norm_err(a, b) <= 3e-2, (k, m, name, _norm_err(a, b))
+
+
+def test_training_on_force_layouts_reduces_loss(gpu):
+    data = []
+    for i in range(6):
+        n, e, _ = W.sbm(400 + 150 * i, 3 + i, 100 + i)
+        off, adj = G.csr_from_edges(n, e)
+        data.append((off, adj, W.force_layout(off, adj, seed=i, iters=150)))
+    ws, hist = W.train(data, epochs=25, lr=3e-3, seed=0, log=None)
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < 0.8 * hist[0], hist
+    p = ws.positions(*data[0][:2], seed=3)
+    assert np.all(np.isfinite(p)) and p.shape == (len(data[0][0]) - 1, 2)
+
+
+def test_stress_meter_matches_oracle(gpu):
+    n, e, _ = W.sbm(3000, 5, 7)
+    off, adj = G.csr_from_edges(n, e)
+    pos = G.uniform_positions(n, 3) * 17.0
+    piv = np.random.default_rng(0).choice(n, 32, replace=False)
+    meter = gpu.StressMeter(off, adj, piv)
+    ref = O.stress(O.Graph(off, adj), pos, 10.0, piv, O.Pool(4))
+    assert abs(meter(pos) - ref) <= 1e-9 * max(abs(ref), 1e-12)
+    # invariant under scale, rotation and translation
+    th = 1.1
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    assert abs(meter(3.0 * pos @ R + 5.0) - meter(pos)) <= 1e-9
