@@ -6,7 +6,9 @@ layouts.
 Tolerances: weights and activations are BF16 on both sides (the oracle
 rounds at the same points), products accumulate in FP32 on the GPU and in
 FP64 in the oracle, so a rounding boundary can fall differently; outputs
-agree to 1e-2 normwise and gradients to 3e-2 normwise per parameter block."""
+agree to 1e-2 normwise; gradients, rounded to BF16 after every layer of
+the backward pass on both sides, agree to 5e-2 normwise per parameter block
+and point the same way (cosine >= 0.998)."""
 import numpy as np
 import pytest
 
@@ -65,7 +67,9 @@ def test_gradients_match_oracle(gpu):
             if np.linalg.norm(b) == 0:
                 assert np.abs(a).max() == 0.0
                 continue
-            assert _norm_err(a, b) <= 3e-2, (k, m, name, _norm_err(a, b))
+            cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+            assert _norm_err(a, b) <= 5e-2 and cos >= 0.998, (k, m, name,
+                                                             _norm_err(a, b), cos)
 
 
 def test_training_on_force_layouts_reduces_loss(gpu):
