@@ -60,10 +60,16 @@ def test_gradients_match_oracle(gpu):
                                        x.astype(np.float64), target, rb=True)
     assert abs(loss - loss_ref) <= 1e-3 * max(abs(loss_ref), 1e-3)
     o = 0
-    for k, m in N.SHAPES:
+    for li, (k, m) in enumerate(N.SHAPES):
         for name, size in (("W", k * m), ("b", m)):
             a, b = g[o:o + size], g_ref[o:o + size]
             o += size
+            if li == 3 and name == "b":
+                # the head bias gradient is the column sum of dL/dP, which is
+                # exactly zero (P is centred); both sides hold rounding noise
+                w4 = np.linalg.norm(g_ref[o - size - k * m:o - size])
+                assert np.abs(a).max() <= 1e-2 * w4 and np.abs(b).max() <= 1e-2 * w4
+                continue
             if np.linalg.norm(b) == 0:
                 assert np.abs(a).max() == 0.0
                 continue
