@@ -325,6 +325,83 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g,
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+
+// One lazy random-walk step of a 2-column signal: y = (x + mean of the
+// neighbours' x) / 2 (isolated vertices keep x).
+__global__ __launch_bounds__(kB) void k_diffuse(const int32_t *__restrict__ off,
+                                                const int32_t *__restrict__ adj,
+                                                const float *__restrict__ invDeg,
+                                                const float2 *__restrict__ x, int64_t n,
+                                                float2 *__restrict__ y) {
+  const int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (v >= n)
+    return;
+  const int32_t e0 = off[v], e1 = off[v + 1];
+  const float2 xv = x[v];
+  if (e0 == e1) {
+    y[v] = xv;
+    return;
+  }
+  float sx = 0.0f, sy = 0.0f;
+  for (int32_t e = e0; e < e1; e++) {
+    const float2 u = x[adj[e]];
+    sx += u.x;
+    sy += u.y;
+  }
+  y[v] = make_float2(0.5f * (xv.x + sx * invDeg[v]), 0.5f * (xv.y + sy * invDeg[v]));
+}
+
+// Per-block sums of x, y, x^2, y^2 (FP64).
+__global__ __launch_bounds__(kB) void k_moments2(const float2 *__restrict__ x, int64_t n,
+                                                 double *__restrict__ part) {
+  __shared__ double sm[kB / kWave][4];
+  double m[4] = {0, 0, 0, 0};
+  for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kB) {
+    const double a = x[v].x, b = x[v].y;
+    m[0] += a;
+    m[1] += b;
+    m[2] += a * a;
+    m[3] += b * b;
+  }
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double w = wave_sum(m[j]);
+    if (lane == 0)
+      sm[wave][j] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < kB / kWave; w++)
+      t += sm[w][threadIdx.x];
+    part[blockIdx.x * 4 + threadIdx.x] = t;
+  }
+}
+
+// Feature columns c, c+1 = (x - mean) * invStd, per coordinate, as BF16.
+__global__ void k_put_feature(const float2 *__restrict__ x, int64_t n, int c, float mx,
+                              float my, float sx, float sy, uint16_t *__restrict__ X0) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  X0[v * kF + c] = f2bf((x[v].x - mx) * sx);
+  X0[v * kF + c + 1] = f2bf((x[v].y - my) * sy);
+}
+
+// Degree term, constant and zero padding (columns 10..15).
+__global__ void k_put_tail(const int32_t *__restrict__ off, int64_t n,
+                           uint16_t *__restrict__ X0) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  X0[v * kF + 10] = f2bf(logf(1.0f + (float)(off[v + 1] - off[v])) * 0.25f);
+  X0[v * kF + 11] = f2bf(1.0f);
+  for (int c = 12; c < kF; c++)
+    X0[v * kF + c] = 0;
+}
+
 } // namespace
 
 struct gvizhip_gnn {
@@ -338,6 +415,7 @@ struct gvizhip_gnn {
   DevBuf<uint16_t> wt[4], wr[4];
   DevBuf<float> partial;
   DevBuf<double> moments;
+  DevBuf<float> diff[2];
   int64_t step = 0;
   bool weightsDirty = true;
 };
@@ -523,7 +601,8 @@ int gvizhip_gnn_create(gvizhip_gnn **out, int64_t maxVertices, int64_t maxEntrie
              g->adj.alloc(maxEntries) || g->invDeg.alloc(n) || g->X0.alloc(n * kF) ||
              g->M0.alloc(n * kF) || g->dY.alloc(n * kO) || g->dZ.alloc(n * kH) ||
              g->Y.alloc(n * kO) || g->dA.alloc(n * 2 * kH) || g->target.alloc(2 * n) ||
-             g->feat32.alloc(1) || g->params.alloc(kParams) || g->grads.alloc(kParams) ||
+             g->feat32.alloc(1) || g->diff[0].alloc(2 * n) || g->diff[1].alloc(2 * n) ||
+             g->params.alloc(kParams) || g->grads.alloc(kParams) ||
              g->adamM.alloc(kParams) || g->adamV.alloc(kParams) ||
              g->partial.alloc(rb * 2 * kH * kH) || g->moments.alloc(10 * 256);
   for (int l = 0; l < 3 && !bad; l++)
@@ -620,6 +699,62 @@ int gvizhip_gnn_set_graph(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
                            hipMemcpyHostToDevice, g->stream));
   GVH_CHECK(hipMemcpyAsync(g->X0.p, x.data(), sizeof(uint16_t) * n * kF,
                            hipMemcpyHostToDevice, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+
+// Uploads the graph like gvizhip_gnn_set_graph and builds the features on
+// the device from start (n x 2): columns 0-1 the start, 2-3 / 4-5 / 6-7 / 8-9
+// the start after 4 / 16 / 64 / 256 lazy random-walk steps (each pair
+// centred and scaled to unit variance per coordinate), 10 log(1 + degree) /
+// 4, 11 one, the rest zero.
+int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
+                                   const int64_t *adj, const double *start) {
+  if (!g || !start || n <= 0 || n > g->cap) {
+    set_error("gvizhip_gnn_set_graph_diffused: invalid arguments");
+    return -1;
+  }
+  std::vector<float> zeros((size_t)n * kF, 0.0f);
+  if (gvizhip_gnn_set_graph(g, n, offsets, adj, zeros.data()))
+    return -1;
+  std::vector<float> x(2 * n);
+  for (int64_t i = 0; i < 2 * n; i++)
+    x[i] = (float)start[i];
+  GVH_CHECK(hipMemcpyAsync(g->diff[0].p, x.data(), sizeof(float) * 2 * n,
+                           hipMemcpyHostToDevice, g->stream));
+  const int marks[5] = {0, 4, 16, 64, 256};
+  int cur = 0, done = 0;
+  const int blocks = (int)std::min<int64_t>(grid_for(n, kB), 256);
+  std::vector<double> part(4 * blocks);
+  for (int f = 0; f < 5; f++) {
+    for (; done < marks[f]; done++) {
+      hipLaunchKernelGGL(k_diffuse, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream,
+                         g->off.p, g->adj.p, g->invDeg.p, (const float2 *)g->diff[cur].p,
+                         n, (float2 *)g->diff[1 - cur].p);
+      cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_moments2, dim3(blocks), dim3(kB), 0, g->stream,
+                       (const float2 *)g->diff[cur].p, n, g->moments.p);
+    GVH_CHECK_LAUNCH();
+    GVH_CHECK(hipMemcpyAsync(part.data(), g->moments.p, sizeof(double) * 4 * blocks,
+                             hipMemcpyDeviceToHost, g->stream));
+    GVH_CHECK(hipStreamSynchronize(g->stream));
+    double m[4] = {0, 0, 0, 0};
+    for (int b = 0; b < blocks; b++)
+      for (int j = 0; j < 4; j++)
+        m[j] += part[4 * b + j];
+    const double mx = m[0] / n, my = m[1] / n;
+    const double vx = m[2] / n - mx * mx, vy = m[3] / n - my * my;
+    const float sx = vx > 1e-30 ? (float)(1.0 / std::sqrt(vx)) : 0.0f;
+    const float sy = vy > 1e-30 ? (float)(1.0 / std::sqrt(vy)) : 0.0f;
+    hipLaunchKernelGGL(k_put_feature, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream,
+                       (const float2 *)g->diff[cur].p, n, 2 * f, (float)mx, (float)my, sx,
+                       sy, g->X0.p);
+  }
+  hipLaunchKernelGGL(k_put_tail, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream, g->off.p,
+                     n, g->X0.p);
+  GVH_CHECK_LAUNCH();
   GVH_CHECK(hipStreamSynchronize(g->stream));
   return 0;
 }

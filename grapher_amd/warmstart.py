@@ -1,8 +1,9 @@
 """Learned warm start for the force embedder.
 
 A three-layer message-passing network (grapher_amd/csrc/gnn.hip, BF16 on
-the matrix cores) maps per-vertex inputs -- two random coordinates, a degree
-term and a constant -- to 2-D positions. It is trained on layouts the force
+the matrix cores) maps per-vertex inputs -- random start coordinates, the
+same coordinates after 4, 16, 64 and 256 lazy random-walk steps (built on
+the device), a degree term and a constant -- to 2-D positions. It is trained on layouts the force
 embedder itself produced in FP64 (LinLog, gravity 0.05, Barnes-Hut theta
 0.7) for stochastic block model graphs, with a loss that ignores rotation,
 reflection and scale. Its output, rescaled to the size such layouts have,
@@ -64,8 +65,8 @@ def sbm(n: int, blocks: int, seed: int = 0, deg_in: float = 8.0, deg_out: float 
 
 
 def features(offsets, seed: int):
-    """Network inputs: uniform [-1, 1] coordinates (the plain method's random
-    start), log(1 + degree) / 4, a constant 1, zero padding to 16."""
+    """Explicit inputs (gvizhip_gnn_set_graph): uniform [-1, 1] coordinates,
+    log(1 + degree) / 4, a constant 1, zero padding to 16."""
     n = len(offsets) - 1
     x = np.zeros((n, F_IN), dtype=np.float32)
     x[:, :2] = G.uniform_positions(n, seed)
@@ -117,7 +118,7 @@ class WarmStart:
             self.engine = native.GnnEngine(max(n, 1), max(len(adj), 1))
             self.engine_cap = (n, len(adj))
             self.engine.set_params(self.params)
-        self.engine.set_graph(offsets, adj, features(offsets, seed))
+        self.engine.set_graph_diffused(offsets, adj, G.uniform_positions(n, seed))
         p = self.engine.forward()
         p -= p.mean(0)
         r = rms_radius(p)
@@ -156,7 +157,8 @@ def train(data, epochs, lr=2e-3, seed=0, log=print, init=None):
         losses = []
         for gi in rng.permutation(len(data)):
             off, adj, t = data[gi]
-            eng.set_graph(off, adj, features(off, seed=int(rng.integers(1 << 30))))
+            start = G.uniform_positions(len(off) - 1, int(rng.integers(1 << 30)))
+            eng.set_graph_diffused(off, adj, start)
             losses.append(eng.train_step(t, lr))
         history.append(float(np.mean(losses)))
         if log:

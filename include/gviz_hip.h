@@ -261,6 +261,14 @@ int gvizhip_gnn_get_params(gvizhip_gnn *g, float *params);
  * major; rounded to BF16). */
 int gvizhip_gnn_set_graph(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
                           const int64_t *adj, const float *features);
+/* The same graph upload with the features built on the device from a start
+ * layout (n x 2): columns 0-1 the start, 2-3 / 4-5 / 6-7 / 8-9 the start
+ * after 4 / 16 / 64 / 256 lazy random-walk steps x <- (x + mean of the
+ * neighbours' x) / 2, each pair centred and scaled to unit variance per
+ * coordinate; column 10 log(1 + degree) / 4, column 11 one, the rest 0. */
+int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n,
+                                   const int64_t *offsets, const int64_t *adj,
+                                   const double *start);
 /* Network output (n x 2, interleaved) for the current graph. */
 int gvizhip_gnn_forward(gvizhip_gnn *g, double *positions);
 /* Loss against target (n x 2) and the gradient of every parameter. */

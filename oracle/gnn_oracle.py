@@ -69,6 +69,29 @@ def mean_agg_T(off, adj, D, inv):
     return S
 
 
+def diffused_features(off, adj, start):
+    """The device-built inputs (gvizhip_gnn_set_graph_diffused): the start and
+    its images after 4, 16, 64, 256 lazy random-walk steps
+    x <- (x + mean of neighbours' x) / 2, each pair centred and scaled to unit
+    variance per coordinate; log(1 + degree) / 4; a constant one."""
+    n = len(off) - 1
+    deg = np.diff(off)
+    x = np.asarray(start, dtype=np.float64).reshape(n, 2).copy()
+    X = np.zeros((n, F))
+    done = 0
+    for f, mark in enumerate((0, 4, 16, 64, 256)):
+        while done < mark:
+            m, _ = mean_agg(off, adj, x)
+            x = np.where(deg[:, None] > 0, 0.5 * (x + m), x)
+            done += 1
+        c = x - x.mean(0)
+        sd = np.sqrt((c * c).mean(0))
+        X[:, 2 * f:2 * f + 2] = c / np.where(sd > 1e-15, sd, np.inf)
+    X[:, 10] = np.log1p(deg) / 4.0
+    X[:, 11] = 1.0
+    return X
+
+
 def forward(theta, off, adj, X, rb=False):
     r = bf16 if rb else (lambda a: np.asarray(a, dtype=np.float64))
     layers = unpack(np.asarray(theta, dtype=np.float64))

@@ -48,6 +48,21 @@ def test_forward_matches_oracle(gpu):
     assert _norm_err(y_gpu, y_ref) <= 1e-2, _norm_err(y_gpu, y_ref)
 
 
+def test_device_features_match_oracle(gpu):
+    """Features built on the device (random-walk diffusion in FP32) against
+    the oracle's FP64 construction, seen through the network's output."""
+    n, off, adj, _ = _case(4, 800, 5)
+    start = G.uniform_positions(n, 9)
+    th = _theta(6)
+    eng = gpu.GnnEngine(n, len(adj))
+    eng.set_params(th)
+    eng.set_graph_diffused(off, adj, start)
+    y_gpu = eng.forward()
+    X = N.diffused_features(off, adj, start)
+    y_ref = N.forward(th.astype(np.float64), off, adj, X, rb=True)[0][:, :2]
+    assert _norm_err(y_gpu, y_ref) <= 2e-2, _norm_err(y_gpu, y_ref)
+
+
 def test_gradients_match_oracle(gpu):
     n, off, adj, x = _case(2, 900, 9)
     th = _theta(3)
