@@ -515,3 +515,61 @@ class GnnEngine:
             _err("gvizhip_gnn_set_graph_diffused")
         self.n = n
 
+    def forward(self):
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_gnn_forward(self.h, _p(out)) != 0:
+            _err("gvizhip_gnn_forward")
+        return out
+
+    def gradients(self, target):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        loss = C.c_double()
+        g = np.zeros(self.count, dtype=np.float32)
+        if lib().gvizhip_gnn_gradients(self.h, _p(t), C.byref(loss), _p(g)) != 0:
+            _err("gvizhip_gnn_gradients")
+        return loss.value, g
+
+    def train_step(self, target, lr):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        loss = C.c_double()
+        if lib().gvizhip_gnn_train_step(self.h, _p(t), float(lr), C.byref(loss)) != 0:
+            _err("gvizhip_gnn_train_step")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class StressMeter:
+    """Pivot-based normalised stress on the device: gvizhip_stress_*."""
+
+    def __init__(self, offsets, adj, pivots):
+        o = np.ascontiguousarray(offsets, dtype=np.int64)
+        a = np.ascontiguousarray(adj, dtype=np.int64)
+        p = np.ascontiguousarray(pivots, dtype=np.int64)
+        self.n = o.shape[0] - 1
+        h = _vp()
+        if lib().gvizhip_stress_create(C.byref(h), self.n, _p(o), _p(a) if a.size else None,
+                                       _p(p), p.shape[0]) != 0:
+            _err("gvizhip_stress_create")
+        self.h = h
+
+    def __call__(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 2)
+        out = C.c_double()
+        if lib().gvizhip_stress_evaluate(self.h, _p(pos), C.byref(out)) != 0:
+            _err("gvizhip_stress_evaluate")
+        return out.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_stress_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
