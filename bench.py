@@ -418,6 +418,30 @@ def bench_grip(args):
     emit(line)
 
 
+def warm_start_line(graphs=3, n=200_000):
+    """The metric's second half: wall time to the plain method's final
+    stress (within 1%) from the learned start versus from the random start,
+    on unseen connected SBM graphs of ~200k vertices (FR, Barnes-Hut 0.7,
+    mixed precision; the plain method's final stress is its stress after 500
+    steps)."""
+    from grapher_amd import warmstart as WS
+    if not os.path.exists(WS.WEIGHTS):
+        return {"error": f"{WS.WEIGHTS} missing"}
+    ws = WS.WarmStart.load()
+    rng = np.random.default_rng(2024)
+    rows = []
+    for i in range(graphs):
+        rows.append(WS.evaluate(ws, n, int(rng.integers(2, 33)), 7000 + i))
+    ok = [r for r in rows if r["speedup"] is not None]
+    return {"workload": f"{graphs} unseen connected SBM graphs, ~{n} vertices, 2-32 blocks, "
+                        "p_in = 8/size, p_out = 0.2/n; FR Barnes-Hut 0.7, mixed precision",
+            "target": "stress within 1% of the plain method's stress after 500 steps",
+            "plain_s_mean": round(float(np.mean([r["plain_s"] for r in rows])), 4),
+            "warm_s_mean": round(float(np.mean([r["warm_s"] for r in ok])), 4) if ok else None,
+            "speedup_mean": round(float(np.mean([r["speedup"] for r in ok])), 2) if ok else None,
+            "reached": f"{len(ok)}/{len(rows)}", "per_graph": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,6 +453,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"])
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE readings)")
@@ -493,6 +518,10 @@ def main():
                 "interactions_per_launch": int(inter / ps),
                 "step_ms_unCaptured": round(step_ms / ps, 4)}
 
+    warm = None
+    if not args.no_warm_start:
+        warm = warm_start_line()
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -514,6 +543,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "warm_start": warm,
     }
     if cpu and cpu.get("value"):
         line["speedup_vs_cpu"] = round(ips / cpu["value"], 1)

@@ -35,6 +35,10 @@ WEIGHTS = os.path.join(HERE, "weights", "warmstart_sbm.npz")
 FORCE = dict(model=native.MODEL_FR, barnes_hut=True, theta=0.7, gravity=0.0,
              edge_length=10.0, nodes_per_cell=1, jitter=1.0)
 F_IN = 16
+# the proposal is placed at this fraction of the RMS radius converged
+# layouts of its size have: repulsion then spreads it out, which settles
+# faster than attraction pulling an oversized start together
+START_SCALE = 0.5
 
 
 def sbm(n: int, blocks: int, seed: int = 0, deg_in: float = 8.0, deg_out: float = 0.2):
@@ -125,8 +129,8 @@ class WarmStart:
         return float(np.exp(self.scale_a + self.scale_b * np.log(n)))
 
     def positions(self, offsets, adj, seed=0):
-        """Proposed start: the network output, centred and scaled to the RMS
-        radius layouts of this size have."""
+        """Proposed start: the network output, centred and scaled to
+        START_SCALE times the RMS radius layouts of this size have."""
         n = len(offsets) - 1
         if self.engine is None or self.engine_cap[0] < n or self.engine_cap[1] < len(adj):
             self.engine = native.GnnEngine(max(n, 1), max(len(adj), 1))
@@ -137,7 +141,7 @@ class WarmStart:
         p -= p.mean(0)
         r = rms_radius(p)
         if r > 0:
-            p *= self.radius(n) / r
+            p *= START_SCALE * self.radius(n) / r
         return p
 
 

@@ -118,3 +118,26 @@ def test_stress_meter_matches_oracle(gpu):
     th = 1.1
     R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
     assert abs(meter(3.0 * pos @ R + 5.0) - meter(pos)) <= 1e-9
+
+
+def test_trained_warm_start_gains_over_random_start(gpu):
+    """The committed network (trained on the GPU on FP64 FR layouts of SBM
+    graphs) against the method without it, on unseen connected SBM graphs:
+    after the same number of refinement steps the warm start's layout has
+    the lower stress."""
+    from grapher_amd import native
+    ws = W.WarmStart.load()
+    wins = 0
+    for seed, blocks in ((501, 7), (502, 19)):
+        m, e = W.sbm_connected(50000, blocks, seed)
+        off, adj = G.csr_from_edges(m, e)
+        meter = native.StressMeter(off, adj,
+                                   np.random.default_rng(seed).choice(m, 64, replace=False))
+        _, _, plain = W.time_to_stress(off, adj, G.uniform_positions(m, seed), meter,
+                                       None, 20, chunk=10)
+        _, _, warm = W.time_to_stress(off, adj, ws.positions(off, adj, seed=seed), meter,
+                                      None, 20, chunk=10)
+        print(f"SBM {m} vertices, {blocks} blocks: stress after 10/20 steps "
+              f"plain {plain[1][1]:.4f}/{plain[2][1]:.4f} warm {warm[1][1]:.4f}/{warm[2][1]:.4f}")
+        wins += warm[2][1] < plain[2][1]
+    assert wins == 2
