@@ -26,7 +26,8 @@ ARCH = os.environ.get("GVIZ_OFFLOAD_ARCH", "gfx950")
 
 # Translation units that must round like the reference (FP64 paths, the
 # double-double prefix sums): no FMA contraction.
-NO_CONTRACT = {"force_common.hip", "force_f64.hip", "tutte.hip", "grip.hip"}
+NO_CONTRACT = {"force_common.hip", "force_f64.hip", "tutte.hip", "grip.hip",
+               "batch_api.hip"}
 
 
 def hipcc():

@@ -5,6 +5,7 @@
 // recurrence and the FA2 scalar update round exactly like the reference.
 #include "dd.hpp"
 #include "force_engine.hpp"
+#include "step_tail.hpp"
 
 #include <cstdarg>
 
@@ -356,95 +357,121 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(
 }
 
 // -------------------------------------------------------- FA2 speed update
-// src/embedders/gvizForceEmbedder.c:140-185, with S and T reduced from the
-// per-block FP64 partials of the force kernel.
 __global__ void k_fa2(const double *__restrict__ partial, int blocks, int64_t n,
                       double jitterTolerance, Scalars *__restrict__ sc,
                       StepRecord *__restrict__ rec, int64_t recCap) {
   __shared__ double smem[2 * 16];
-  if (sc->stopped)
-    return;
-  double v[2] = {0.0, 0.0};
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-    v[0] += partial[2 * b];
-    v[1] += partial[2 * b + 1];
-  }
-  const bool isMax[2] = {false, false};
-  block_reduce<2>(v, isMax, smem);
-  if (threadIdx.x != 0)
-    return;
-  double totalSwinging = v[0], totalEffectiveTraction = v[1];
-  int64_t slot = sc->iteration - sc->recordBase;
-  if (slot >= 0 && slot < recCap) {
-    rec[slot].totalSwinging = totalSwinging;
-    rec[slot].totalTraction = totalEffectiveTraction;
-  }
-  if (totalEffectiveTraction < 1e-7)
-    return;
-  double gs = sc->globalSpeed, eff = sc->speedEfficiency;
-  double nn = (double)n;
-  double estimatedOptimalJT = 0.05 * sqrt(nn);
-  double minJT = sqrt(estimatedOptimalJT);
-  double maxJT = 10.0;
-  double jt = jitterTolerance *
-              fmax(minJT, fmin(maxJT, estimatedOptimalJT *
-                                          totalEffectiveTraction / (nn * nn)));
-  if (totalSwinging / totalEffectiveTraction > 2.0) {
-    if (eff > 0.05)
-      eff *= 0.5;
-    jt = fmax(jt, jitterTolerance);
-  }
-  double targetSpeed = totalSwinging > 1e-7
-                           ? jt * eff * totalEffectiveTraction / totalSwinging
-                           : gs;
-  if (totalSwinging > jt * totalEffectiveTraction) {
-    if (eff > 0.05)
-      eff *= 0.7;
-  } else if (gs < 1000.0) {
-    eff *= 1.3;
-  }
-  if (eff < 0.05)
-    eff = 0.05;
-  else if (eff > 1.0)
-    eff = 1.0;
-  gs += fmin(targetSpeed - gs, 0.5 * gs);
-  sc->globalSpeed = gs;
-  sc->speedEfficiency = eff;
+  fa2_tail(partial, blocks, n, jitterTolerance, sc, rec, recCap, smem);
 }
 
-// Step tail (src/embedders/gvizForceEmbedder.c:467-494): max |applied|, mean
-// |att|, mean |rep|, mean gravity; advances the iteration and, for Run,
-// raises the stop flag once maxDisp < epsilon.
 __global__ void k_stats(const double *__restrict__ partial, int blocks,
                         int64_t n, double gravityK, Scalars *__restrict__ sc,
                         StepRecord *__restrict__ rec, int64_t recCap) {
   __shared__ double smem[3 * 16];
+  stats_tail(partial, blocks, n, gravityK, sc, rec, recCap, nullptr, smem);
+}
+
+
+// ------------------------------------------------------------ batch stages
+// Bounding box and root cell of every graph of a batch: one block per graph,
+// the same root formulas as k_root.
+__global__ __launch_bounds__(kBlock) void k_bbox_root_g(const double2 *__restrict__ pos,
+                                                        const int64_t *__restrict__ gStart,
+                                                        Scalars *__restrict__ scG) {
+  __shared__ double sm[4 * 16];
+  const int g = blockIdx.x;
+  Scalars *sc = scG + g;
   if (sc->stopped)
     return;
-  double v[3] = {0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-    v[0] = fmax(v[0], partial[3 * b]);
-    v[1] += partial[3 * b + 1];
-    v[2] += partial[3 * b + 2];
+  const int64_t b = gStart[g], e = gStart[g + 1];
+  double v[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    double2 p = pos[i];
+    v[0] = fmax(v[0], -p.x);
+    v[1] = fmax(v[1], p.x);
+    v[2] = fmax(v[2], -p.y);
+    v[3] = fmax(v[3], p.y);
   }
-  const bool isMax[3] = {true, false, false};
-  block_reduce<3>(v, isMax, smem);
-  if (threadIdx.x != 0)
+  const bool isMax[4] = {true, true, true, true};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    double minX = -v[0], maxX = v[1], minY = -v[2], maxY = v[3];
+    double sx = maxX - minX, sy = maxY - minY;
+    double span = sx > sy ? sx : sy;
+    double half = span > 0.0 ? span / 2.0 : 1.0;
+    sc->rootCx = (minX + maxX) / 2.0;
+    sc->rootCy = (minY + maxY) / 2.0;
+    sc->rootHalf = half;
+    int d = 0;
+    double h = half;
+    while (!(h <= 1e-9) && d < 64) {
+      h = h / 2.0;
+      d++;
+    }
+    sc->maxDepth = d;
+    sc->rootRef = e - b >= 2 ? 0 : -1;
+    sc->traversalOverflow = d > kBatchLevels ? 2 : 0;
+  }
+}
+
+// Batch keys: graph id in bits 63..52 of keyHi, then levels 1..26 of the
+// quadrant path in keyHi and levels 27..58 in keyLo (the single-graph layout
+// shifted by 12 bits, so the longest common prefix of two keys of one graph
+// is 12 + the single-graph one).
+__global__ __launch_bounds__(kBlock) void k_keys_g(const double2 *__restrict__ pos,
+                                                   int64_t n,
+                                                   const int32_t *__restrict__ gid,
+                                                   const Scalars *__restrict__ scG,
+                                                   uint64_t *__restrict__ keyHi,
+                                                   uint64_t *__restrict__ keyLo,
+                                                   uint32_t *__restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
     return;
-  double maxDisp = v[0];
-  int64_t slot = sc->iteration - sc->recordBase;
-  if (slot >= 0 && slot < recCap) {
-    StepRecord &r = rec[slot];
-    r.maxDisp = maxDisp;
-    r.speed = sc->globalSpeed;
-    r.meanAtt = n ? v[1] / (double)n : 0.0;
-    r.meanRep = n ? v[2] / (double)n : 0.0;
-    r.meanGrav = gravityK * sc->meanDegreePlusOne;
-    r.speedEfficiency = sc->speedEfficiency;
+  const int32_t g = gid[i];
+  const Scalars &sc = scG[g];
+  double2 p = pos[i];
+  double cx = sc.rootCx, cy = sc.rootCy, h = sc.rootHalf;
+  const int levels = sc.maxDepth < kBatchLevels ? sc.maxDepth : kBatchLevels;
+  uint64_t hi = (uint64_t)g << 52, lo = 0;
+  for (int level = 1; level <= levels; level++) {
+    bool east = p.x >= cx;
+    bool north = p.y >= cy;
+    uint64_t digit = (north ? 0u : 2u) | (east ? 1u : 0u);
+    if (level <= 26)
+      hi |= digit << (2 * (26 - level));
+    else
+      lo |= digit << (2 * (58 - level));
+    double quarter = h / 2.0;
+    cx = cx + (east ? quarter : -quarter);
+    cy = cy + (north ? quarter : -quarter);
+    h = quarter;
   }
-  sc->iteration++;
-  if (sc->epsilon >= 0.0 && maxDisp < sc->epsilon)
-    sc->stopped = 1;
+  keyHi[i] = hi;
+  keyLo[i] = lo;
+  vals[i] = (uint32_t)i;
+}
+
+__global__ void k_fa2_g(const double *__restrict__ partial,
+                        const int64_t *__restrict__ gStart,
+                        const int32_t *__restrict__ gBlocks,
+                        const int64_t *__restrict__ gPart, double jitterTolerance,
+                        Scalars *__restrict__ scG) {
+  __shared__ double smem[2 * 16];
+  const int g = blockIdx.x;
+  fa2_tail(partial + 2 * gPart[g], gBlocks[g], gStart[g + 1] - gStart[g],
+           jitterTolerance, scG + g, nullptr, 0, smem);
+}
+
+__global__ void k_stats_g(const double *__restrict__ partial,
+                          const int64_t *__restrict__ gStart,
+                          const int32_t *__restrict__ gBlocks,
+                          const int64_t *__restrict__ gPart, double gravityK,
+                          Scalars *__restrict__ scG, StepRecord *__restrict__ last) {
+  __shared__ double smem[3 * 16];
+  const int g = blockIdx.x;
+  stats_tail(partial + 3 * gPart[g], gBlocks[g], gStart[g + 1] - gStart[g], gravityK,
+             scG + g, nullptr, 0, last + g, smem);
 }
 
 } // namespace
@@ -501,6 +528,39 @@ int enqueue_keys_and_sort(ForceCommon &c) {
     return -1;
   hipLaunchKernelGGL(k_fix_runs, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
                      c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b) {
+  const int64_t n = c.n;
+  hipStream_t st = c.stream;
+  hipLaunchKernelGGL(k_bbox_root_g, dim3((unsigned)b.graphs), dim3(kBlock), 0, st,
+                     (const double2 *)c.pos64.p, b.gStart.p, b.scG.p);
+  hipLaunchKernelGGL(k_keys_g, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     (const double2 *)c.pos64.p, n, b.gid.p, b.scG.p, c.keyHi.p,
+                     c.keyLo.p, c.vals.p);
+  GVH_CHECK_LAUNCH();
+  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 0, 64, st))
+    return -1;
+  hipLaunchKernelGGL(k_fix_runs, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int enqueue_batch_fa2(ForceCommon &c, BatchState &b) {
+  hipLaunchKernelGGL(k_fa2_g, dim3((unsigned)b.graphs), dim3(256), 0, c.stream,
+                     b.partials.p, b.gStart.p, b.gBlocks.p, b.gPart.p,
+                     c.prm.jitterTolerance, b.scG.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+int enqueue_batch_stats(ForceCommon &c, BatchState &b) {
+  hipLaunchKernelGGL(k_stats_g, dim3((unsigned)b.graphs), dim3(256), 0, c.stream,
+                     b.partials.p, b.gStart.p, b.gBlocks.p, b.gPart.p, c.prm.gravityK,
+                     b.scG.p, b.last.p);
   GVH_CHECK_LAUNCH();
   return 0;
 }

@@ -61,6 +61,25 @@ struct Scalars {
 };
 
 constexpr int32_t kNone = INT32_MIN;
+// quadrant-path levels a batch key holds (12 bits go to the graph id)
+constexpr int kBatchLevels = 58;
+
+// Segments of a batch of independent graphs laid out together (all null for
+// a single graph). Graph g owns compact (and sorted) positions
+// [gStart[g], gStart[g+1]); its reductions use the same block decomposition a
+// run of that graph alone uses (gBlocks[g] blocks, partial slots from
+// gPart[g]); its traversal waves start at multiples of 64 of its own range.
+struct SegView {
+  const int32_t *vbGraph = nullptr;   // reduction block -> graph
+  const int32_t *vbIdx = nullptr;     // reduction block -> index within graph
+  const int64_t *gStart = nullptr;    // G + 1
+  const int32_t *gBlocks = nullptr;   // G
+  const int64_t *gPart = nullptr;     // G
+  const int32_t *waveGraph = nullptr; // traversal wave -> graph
+  const int64_t *waveBase = nullptr;  // traversal wave -> first sorted position
+  const int32_t *rootOf = nullptr;    // graph -> root quad record (-1: none)
+  int64_t waves = 0;
+};
 
 template <class Real, class PosT> struct NodeRec {
   PosT comx, comy;
@@ -114,9 +133,49 @@ struct ForceCommon {
   int alloc(int64_t nVertices, int64_t nEntries);
 };
 
+// A batch of independent graphs laid out together (graph g owns compact
+// positions [gStart[g], gStart[g+1]); up to 4096 graphs, whose ids take the
+// top 12 bits of the sort key). Device tables for SegView plus per-graph
+// scalars, last step records and reduction partials.
+struct BatchState {
+  int64_t graphs = 0, vblocks = 0, waves = 0;
+  DevBuf<int32_t> gid;                  // n
+  DevBuf<int64_t> gStart;               // G + 1
+  DevBuf<int32_t> gBlocks;              // G
+  DevBuf<int64_t> gPart;                // G
+  DevBuf<int32_t> vbGraph, vbIdx;       // vblocks
+  DevBuf<int32_t> waveGraph;            // waves
+  DevBuf<int64_t> waveBase;             // waves
+  DevBuf<int32_t> rootOf;               // G
+  DevBuf<Scalars> scG;                  // G
+  DevBuf<StepRecord> last;              // G
+  DevBuf<double> partials;              // 3 * vblocks
+  SegView view() const {
+    SegView v;
+    v.vbGraph = vbGraph.p;
+    v.vbIdx = vbIdx.p;
+    v.gStart = gStart.p;
+    v.gBlocks = gBlocks.p;
+    v.gPart = gPart.p;
+    v.waveGraph = waveGraph.p;
+    v.waveBase = waveBase.p;
+    v.rootOf = rootOf.p;
+    v.waves = waves;
+    return v;
+  }
+};
+
 // Step modes: the whole step, the force evaluation without the speed update
 // and position update, or only the quadtree build.
 enum StepMode : int { kStepFull = 0, kStepForcesOnly = 1, kStepTreeOnly = 2 };
+
+// One step of every graph of a batch (Barnes-Hut only); c.sc must hold
+// neutral scalars (not stopped, depth limit 64). Instantiated next to
+// typed_step, so each precision is compiled exactly as its single-graph step.
+template <class Real, class PosT> int typed_batch_step(ForceCommon &c, BatchState &b);
+int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b);
+int enqueue_batch_fa2(ForceCommon &c, BatchState &b);
+int enqueue_batch_stats(ForceCommon &c, BatchState &b);
 
 // Enqueues one step on c.stream. Instantiated in force_f64.hip
 // (<double,double>) and force_f32.hip (<float,float>, <float,double>).

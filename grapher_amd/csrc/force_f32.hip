@@ -6,4 +6,6 @@ namespace gvh {
 template int typed_step<float, float>(ForceCommon &, int);
 template int typed_step<float, double>(ForceCommon &, int);
 template int typed_upload<float>(ForceCommon &);
+template int typed_batch_step<float, float>(ForceCommon &, BatchState &);
+template int typed_batch_step<float, double>(ForceCommon &, BatchState &);
 } // namespace gvh

@@ -5,4 +5,5 @@
 namespace gvh {
 template int typed_step<double, double>(ForceCommon &, int);
 template int typed_upload<double>(ForceCommon &);
+template int typed_batch_step<double, double>(ForceCommon &, BatchState &);
 } // namespace gvh

@@ -348,18 +348,37 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
     const uint32_t *__restrict__ sidx, int64_t n, Real theta, Real kSq, Real oc,
     typename V2<Real>::type *__restrict__ rep, int32_t *__restrict__ inter,
     const double *__restrict__ r64, Scalars *__restrict__ sc,
-    unsigned long long *__restrict__ tstats) {
+    unsigned long long *__restrict__ tstats, SegView sv) {
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
   __shared__ int32_t sNode[kTravWaves][kTravStack];
   __shared__ uint64_t sMask[kTravWaves][kTravStack];
   __shared__ int32_t dRef[kTravWaves][kTravDefer];
   __shared__ uint64_t dMask[kTravWaves][kTravDefer];
-  if (sc->stopped)
-    return;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  const int64_t s = ((int64_t)blockIdx.x * kTravWaves + wave) * kWave + lane;
-  const bool valid = s < n;
+  int64_t s;
+  bool valid;
+  int32_t root;
+  if (sv.waveGraph) {
+    // batch: this wave serves 64 consecutive positions of one graph
+    const int64_t w = (int64_t)blockIdx.x * kTravWaves + wave;
+    if (w >= sv.waves)
+      return;
+    const int32_t g = sv.waveGraph[w];
+    sc += g;
+    s = sv.waveBase[w] + lane;
+    valid = s < sv.gStart[g + 1];
+    root = sv.rootOf[g];
+    // a one-vertex graph has no cell: its only body is the target itself
+    if (root == -1)
+      root = -(int32_t)(sv.gStart[g] + 1);
+  } else {
+    s = ((int64_t)blockIdx.x * kTravWaves + wave) * kWave + lane;
+    valid = s < n;
+    root = n >= 2 ? 0 : -1;
+  }
+  if (sc->stopped)
+    return;
   const uint64_t live = __ballot(valid);
   if (live == 0)
     return;
@@ -435,7 +454,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
 
   int sp = 1;
   if (lane == 0) {
-    sNode[wave][0] = n >= 2 ? 0 : -1;
+    sNode[wave][0] = root;
     sMask[wave][0] = live;
   }
   enum { kDefer = 0, kOpen = 1, kAmbig = 2, kRange = 3 };
@@ -714,13 +733,24 @@ __global__ __launch_bounds__(kTB) void k_forces(
     typename V2<Real>::type *__restrict__ disp, Real *__restrict__ swOut,
     Real *__restrict__ attMagOut, Real *__restrict__ repMagOut,
     double *__restrict__ partial, const typename V2<Real>::type *__restrict__ heavy,
-    const Scalars *__restrict__ sc) {
+    const Scalars *__restrict__ sc, SegView sv) {
   __shared__ double smem[2 * 16];
+  int64_t start = 0, len = n, part = 0;
+  int64_t blk = blockIdx.x, nblk = gridDim.x;
+  if (sv.vbGraph) {
+    const int32_t g = sv.vbGraph[blockIdx.x];
+    blk = sv.vbIdx[blockIdx.x];
+    start = sv.gStart[g];
+    len = sv.gStart[g + 1] - start;
+    nblk = sv.gBlocks[g];
+    part = sv.gPart[g];
+    sc += g;
+  }
   if (sc->stopped)
     return;
   double accS = 0.0, accT = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kTB) {
+  for (int64_t li = blk * kTB + threadIdx.x; li < len; li += nblk * kTB) {
+    const int64_t i = start + li;
     typename V2<PosT>::type v = pos[i];
     Real ax = 0, ay = 0;
     const int32_t e0 = off[i], e1 = off[i + 1];
@@ -759,8 +789,8 @@ __global__ __launch_bounds__(kTB) void k_forces(
   const bool isMax[2] = {false, false};
   block_reduce<2>(vals, isMax, smem);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = vals[0];
-    partial[2 * blockIdx.x + 1] = vals[1];
+    partial[2 * (part + blk)] = vals[0];
+    partial[2 * (part + blk) + 1] = vals[1];
   }
 }
 
@@ -771,14 +801,25 @@ __global__ __launch_bounds__(kTB) void k_apply(
     const typename V2<Real>::type *__restrict__ disp,
     const Real *__restrict__ sw, const Real *__restrict__ attMag,
     const Real *__restrict__ repMag, int64_t n, double2 *__restrict__ pos64,
-    double *__restrict__ partial, const Scalars *__restrict__ sc) {
+    double *__restrict__ partial, const Scalars *__restrict__ sc, SegView sv) {
   __shared__ double smem[3 * 16];
+  int64_t start = 0, len = n, part = 0;
+  int64_t blk = blockIdx.x, nblk = gridDim.x;
+  if (sv.vbGraph) {
+    const int32_t g = sv.vbGraph[blockIdx.x];
+    blk = sv.vbIdx[blockIdx.x];
+    start = sv.gStart[g];
+    len = sv.gStart[g + 1] - start;
+    nblk = sv.gBlocks[g];
+    part = sv.gPart[g];
+    sc += g;
+  }
   if (sc->stopped)
     return;
   const double gs = sc->globalSpeed;
   double vmax = 0.0, sa = 0.0, sr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kTB) {
+  for (int64_t li = blk * kTB + threadIdx.x; li < len; li += nblk * kTB) {
+    const int64_t i = start + li;
     double factor = gs / (1.0 + sqrt(gs * (double)sw[i]));
     typename V2<Real>::type d = disp[i];
     double o0 = (double)d.x * factor;
@@ -796,9 +837,9 @@ __global__ __launch_bounds__(kTB) void k_apply(
   const bool isMax[3] = {true, false, false};
   block_reduce<3>(vals, isMax, smem);
   if (threadIdx.x == 0) {
-    partial[3 * blockIdx.x] = vals[0];
-    partial[3 * blockIdx.x + 1] = vals[1];
-    partial[3 * blockIdx.x + 2] = vals[2];
+    partial[3 * (part + blk)] = vals[0];
+    partial[3 * (part + blk) + 1] = vals[1];
+    partial[3 * (part + blk) + 2] = vals[2];
   }
 }
 
@@ -872,13 +913,13 @@ int typed_step(ForceCommon &c, int mode) {
                          (const NodeRec<Real, PosT> *)c.nodes.p,
                          (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
                          theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
-                         c.sc.p, c.travStats.p);
+                         c.sc.p, c.travStats.p, SegView{});
     else
       hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTravThreads), 0, st,
                          (const NodeRec<Real, PosT> *)c.nodes.p,
                          (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
                          theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
-                         c.sc.p, c.travStats.p);
+                         c.sc.p, c.travStats.p, SegView{});
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
@@ -915,7 +956,7 @@ int typed_step(ForceCommon &c, int mode) {
                        (Real)c.prm.edgeLength, c.prm.gravityK, (R2 *)c.sf.p,
                        (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
                        (Real *)c.repMag.p, c.partials.p, (const R2 *)c.attHeavy.p,
-                       c.sc.p);
+                       c.sc.p, SegView{});
   }
   else
   {
@@ -930,7 +971,7 @@ int typed_step(ForceCommon &c, int mode) {
                        (Real)c.prm.edgeLength, c.prm.gravityK, (R2 *)c.sf.p,
                        (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
                        (Real *)c.repMag.p, c.partials.p, (const R2 *)c.attHeavy.p,
-                       c.sc.p);
+                       c.sc.p, SegView{});
   }
   GVH_CHECK_LAUNCH();
   if (mode == kStepForcesOnly)
@@ -940,9 +981,175 @@ int typed_step(ForceCommon &c, int mode) {
   hipLaunchKernelGGL(k_apply<Real>, dim3(rb), dim3(kTB), 0, st,
                      (const R2 *)c.disp.p, (const Real *)c.sw.p,
                      (const Real *)c.attMag.p, (const Real *)c.repMag.p, n,
-                     (double2 *)c.pos64.p, c.partials.p, c.sc.p);
+                     (double2 *)c.pos64.p, c.partials.p, c.sc.p, SegView{});
   GVH_CHECK_LAUNCH();
   return enqueue_stats(c, (int)rb);
+}
+
+
+// ------------------------------------------------------------ batch stages
+template <int Unused = 0>
+__global__ void k_reset_roots(int32_t *__restrict__ rootOf, int64_t graphs) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < graphs)
+    rootOf[g] = -1;
+}
+
+// k_quad for a batch: radix nodes whose common prefix is shorter than the 12
+// graph-id bits span several graphs and are skipped; within a graph the quad
+// depth is (delta - 12) / 2 and the graph's root record is the head whose
+// parent spans several graphs.
+template <class Real, class PosT>
+__global__ __launch_bounds__(kTB) void k_quad_g(
+    const int32_t *__restrict__ left, const int32_t *__restrict__ right,
+    const int32_t *__restrict__ parent, const int32_t *__restrict__ first,
+    const int32_t *__restrict__ last, const int32_t *__restrict__ delta,
+    const double *__restrict__ pm, const double *__restrict__ pxh,
+    const double *__restrict__ pxl, const double *__restrict__ pyh,
+    const double *__restrict__ pyl, int64_t n, int64_t npc,
+    NodeRec<Real, PosT> *__restrict__ nodes, const int32_t *__restrict__ gid,
+    const Scalars *__restrict__ scG, int32_t *__restrict__ rootOf) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n - 1)
+    return;
+  const int dl = delta[b];
+  if (dl < 12)
+    return;
+  const int32_t f = first[b], l = last[b];
+  const int32_t g = gid[f];
+  const Scalars &sc = scG[g];
+  if (sc.stopped)
+    return;
+  const int maxDepth = sc.maxDepth < kBatchLevels ? sc.maxDepth : kBatchLevels;
+  const int q = quad_depth(dl - 12, maxDepth);
+  const int32_t p = parent[b];
+  const bool top = p < 0 || delta[p] < 12;
+  if (!top && quad_depth(delta[p] - 12, maxDepth) >= q)
+    return;
+  if (top)
+    rootOf[g] = (int32_t)b;
+  NodeRec<Real, PosT> r;
+  const int32_t count = l - f + 1;
+  r.first = f;
+  r.child[0] = r.child[1] = r.child[2] = r.child[3] = kNone;
+  if (count <= npc || q >= maxDepth) {
+    r.count = count;
+    r.comx = r.comy = (PosT)0;
+    r.mass = r.twoh = (Real)0;
+    nodes[b] = r;
+    return;
+  }
+  r.count = 0;
+  double M = pm[l + 1] - pm[f];
+  dd sx = dd_add({pxh[l + 1], pxl[l + 1]}, {-pxh[f], -pxl[f]});
+  dd sy = dd_add({pyh[l + 1], pyl[l + 1]}, {-pyh[f], -pyl[f]});
+  r.comx = (PosT)((sx.hi + sx.lo) / M);
+  r.comy = (PosT)((sy.hi + sy.lo) / M);
+  r.mass = (Real)M;
+  r.twoh = (Real)(2.0 * ldexp(sc.rootHalf, -q));
+  int k = 0;
+  const int32_t kids[2] = {left[b], right[b]};
+  for (int c = 0; c < 2; c++) {
+    int32_t ch = kids[c];
+    if (ch < 0 || quad_depth(delta[ch] - 12, maxDepth) > q) {
+      r.child[k++] = ch;
+    } else {
+      r.child[k++] = left[ch];
+      r.child[k++] = right[ch];
+    }
+  }
+  nodes[b] = r;
+}
+
+template <class Real, class PosT>
+int typed_batch_step(ForceCommon &c, BatchState &bs) {
+  using P2 = typename V2<PosT>::type;
+  using R2 = typename V2<Real>::type;
+  const int64_t n = c.n;
+  hipStream_t st = c.stream;
+  if (n == 0)
+    return 0;
+  const SegView sv = bs.view();
+  const Real kSq = (Real)(c.prm.edgeLength * c.prm.edgeLength);
+  const Real oc = (Real)c.prm.overlapConstant;
+  const Real theta = (Real)c.prm.theta;
+  const P2 *posC;
+  if (sizeof(PosT) == sizeof(double)) {
+    posC = (const P2 *)c.pos64.p;
+  } else {
+    hipLaunchKernelGGL(k_post<PosT>, dim3(grid_for(n, kTB)), dim3(kTB), 0, st,
+                       (const double2 *)c.pos64.p, n, (P2 *)c.posT.p, c.sc.p);
+    posC = (const P2 *)c.posT.p;
+  }
+  const int model = c.prm.model == 1 ? 1 : 0;
+  if (enqueue_batch_keys_and_sort(c, bs))
+    return -1;
+  hipLaunchKernelGGL((k_gather<Real, PosT>), dim3(grid_for(n, kTB)), dim3(kTB), 0, st,
+                     (const double2 *)c.pos64.p, c.mass64.p, c.radius64.p, c.vals.p,
+                     c.keyLo.p, n, (double2 *)c.spos64.p, (P2 *)c.spos.p,
+                     (typename V2<Real>::type *)c.smr.p, c.keyLoS.p, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  if (enqueue_radix_tree(c))
+    return -1;
+  hipLaunchKernelGGL(k_reset_roots<0>, dim3(grid_for(bs.graphs, kTB)), dim3(kTB), 0, st,
+                     bs.rootOf.p, bs.graphs);
+  if (n >= 2)
+    hipLaunchKernelGGL((k_quad_g<Real, PosT>), dim3(grid_for(n - 1, kTB)), dim3(kTB), 0,
+                       st, c.left.p, c.right.p, c.parent.p, c.first.p, c.last.p,
+                       c.delta.p, c.pm.p, c.pmxHi.p, c.pmxLo.p, c.pmyHi.p, c.pmyLo.p, n,
+                       c.prm.nodesPerCell > 0 ? c.prm.nodesPerCell : 1,
+                       (NodeRec<Real, PosT> *)c.nodes.p, bs.gid.p, bs.scG.p,
+                       bs.rootOf.p);
+  GVH_CHECK_LAUNCH();
+  const unsigned tb = (unsigned)((bs.waves + kTravWaves - 1) / kTravWaves);
+  if (model == 1)
+    hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTravThreads), 0, st,
+                       (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,
+                       (const R2 *)c.smr.p, c.vals.p, n, theta, kSq, oc, (R2 *)c.rep.p,
+                       (int32_t *)nullptr, c.radius64.p, bs.scG.p,
+                       (unsigned long long *)nullptr, sv);
+  else
+    hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTravThreads), 0, st,
+                       (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,
+                       (const R2 *)c.smr.p, c.vals.p, n, theta, kSq, oc, (R2 *)c.rep.p,
+                       (int32_t *)nullptr, c.radius64.p, bs.scG.p,
+                       (unsigned long long *)nullptr, sv);
+  GVH_CHECK_LAUNCH();
+  const unsigned vb = (unsigned)bs.vblocks;
+  if (model == 1) {
+    if (c.nHeavy > 0)
+      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 1>),
+                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
+                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
+                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
+    hipLaunchKernelGGL((k_forces<Real, PosT, 1>), dim3(vb), dim3(kTB), 0, st, posC,
+                       c.csrOff.p, c.csrAdj.p, c.degree.p, (const Real *)c.massT.p,
+                       (const R2 *)c.rep.p, n, (Real)c.prm.edgeLength, c.prm.gravityK,
+                       (R2 *)c.sf.p, (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
+                       (Real *)c.repMag.p, bs.partials.p, (const R2 *)c.attHeavy.p,
+                       bs.scG.p, sv);
+  } else {
+    if (c.nHeavy > 0)
+      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 0>),
+                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
+                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
+                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
+    hipLaunchKernelGGL((k_forces<Real, PosT, 0>), dim3(vb), dim3(kTB), 0, st, posC,
+                       c.csrOff.p, c.csrAdj.p, c.degree.p, (const Real *)c.massT.p,
+                       (const R2 *)c.rep.p, n, (Real)c.prm.edgeLength, c.prm.gravityK,
+                       (R2 *)c.sf.p, (R2 *)c.disp.p, (Real *)c.sw.p, (Real *)c.attMag.p,
+                       (Real *)c.repMag.p, bs.partials.p, (const R2 *)c.attHeavy.p,
+                       bs.scG.p, sv);
+  }
+  GVH_CHECK_LAUNCH();
+  if (enqueue_batch_fa2(c, bs))
+    return -1;
+  hipLaunchKernelGGL(k_apply<Real>, dim3(vb), dim3(kTB), 0, st, (const R2 *)c.disp.p,
+                     (const Real *)c.sw.p, (const Real *)c.attMag.p,
+                     (const Real *)c.repMag.p, n, (double2 *)c.pos64.p, bs.partials.p,
+                     bs.scG.p, sv);
+  GVH_CHECK_LAUNCH();
+  return enqueue_batch_stats(c, bs);
 }
 
 template <class Real>

@@ -122,6 +122,13 @@ def lib():
         "gvizhip_grip_refine_round": (C.c_int, [_vp, _i64, _i64, C.POINTER(GripRoundStats)]),
         "gvizhip_grip_profile_rounds": (C.c_int, [_vp, _i64, _i64, _i64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gvizhip_grip_destroy": (None, [_vp]),
+        "gvizhip_batch_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, C.POINTER(ForceConfig)]),
+        "gvizhip_batch_begin": (C.c_int, [_vp, _vp]),
+        "gvizhip_batch_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+        "gvizhip_batch_autotune": (C.c_int, [_vp, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "gvizhip_batch_run": (_i64, [_vp, _i64, C.c_double, _vp, _vp, _vp]),
+        "gvizhip_batch_download_positions": (C.c_int, [_vp, _vp]),
+        "gvizhip_batch_destroy": (None, [_vp]),
         "gvizhip_stress_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
         "gvizhip_stress_evaluate": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
         "gvizhip_stress_destroy": (None, [_vp]),
@@ -164,6 +171,9 @@ DECLARED = [
     "gvizhip_grip_clear_decorators", "gvizhip_grip_update_knn",
     "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_profile_rounds",
     "gvizhip_grip_destroy",
+    "gvizhip_batch_create", "gvizhip_batch_begin", "gvizhip_batch_schedule",
+    "gvizhip_batch_autotune", "gvizhip_batch_run", "gvizhip_batch_download_positions",
+    "gvizhip_batch_destroy",
     "gvizhip_stress_create", "gvizhip_stress_evaluate", "gvizhip_stress_destroy",
     "gvizhip_gnn_create", "gvizhip_gnn_param_count", "gvizhip_gnn_set_params",
     "gvizhip_gnn_get_params", "gvizhip_gnn_set_graph", "gvizhip_gnn_set_graph_diffused",
@@ -569,6 +579,80 @@ class StressMeter:
     def close(self):
         if getattr(self, "h", None):
             lib().gvizhip_stress_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class BatchEngine:
+    """Many independent graphs laid out together: gvizhip_batch_*.
+
+    graphs: list of (offsets, adj) CSR pairs over each graph's own ids; they
+    are concatenated in order (graph g's vertex i becomes start[g] + i)."""
+
+    def __init__(self, graphs, *, model=MODEL_FR, prevent_overlap=False,
+                 precision="fp32", nodes_per_cell=1, edge_length=10.0, theta=1.0,
+                 gravity=0.0, overlap_constant=100.0, radius_base=0.0,
+                 radius_per_degree=0.0, jitter=1.0):
+        sizes = [len(o) - 1 for o, _ in graphs]
+        self.start = np.zeros(len(graphs) + 1, dtype=np.int64)
+        self.start[1:] = np.cumsum(sizes)
+        offs, adjs, base_e = [np.zeros(1, np.int64)], [], 0
+        for (o, a), s in zip(graphs, self.start[:-1]):
+            o = np.asarray(o, dtype=np.int64)
+            offs.append(o[1:] + base_e)
+            adjs.append(np.asarray(a, dtype=np.int64) + s)
+            base_e += int(o[-1])
+        self._o = np.ascontiguousarray(np.concatenate(offs))
+        self._a = np.ascontiguousarray(np.concatenate(adjs) if adjs else np.zeros(0, np.int64))
+        self.n = int(self.start[-1])
+        self.G = len(graphs)
+        self.cfg = ForceConfig(model, 1, int(prevent_overlap),
+                               PRECISIONS.get(precision, precision), nodes_per_cell,
+                               edge_length, theta, gravity, overlap_constant,
+                               radius_base, radius_per_degree, jitter)
+        h = _vp()
+        if lib().gvizhip_batch_create(C.byref(h), self.G, _p(self.start), _p(self._o),
+                                      _p(self._a) if self._a.size else None,
+                                      C.byref(self.cfg)) != 0:
+            _err("gvizhip_batch_create")
+        self.h = h
+
+    def begin(self, positions):
+        p = np.ascontiguousarray(positions, dtype=np.float64).reshape(self.n, 2)
+        if lib().gvizhip_batch_begin(self.h, _p(p)) != 0:
+            _err("gvizhip_batch_begin")
+
+    def schedule(self, wave_order=0, steps_per_launch=4):
+        if lib().gvizhip_batch_schedule(self.h, wave_order, steps_per_launch) != 0:
+            _err("gvizhip_batch_schedule")
+
+    def autotune(self, trial_steps=8):
+        o, s = C.c_int32(), C.c_int32()
+        if lib().gvizhip_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(s)) != 0:
+            _err("gvizhip_batch_autotune")
+        return o.value, s.value
+
+    def run(self, iters, epsilon=-1.0):
+        """(steps launched, per-graph steps, per-graph latency ms, last maxDisp)."""
+        it = np.zeros(self.G, dtype=np.int64)
+        lat = np.zeros(self.G)
+        md = np.zeros(self.G)
+        r = lib().gvizhip_batch_run(self.h, iters, epsilon, _p(it), _p(lat), _p(md))
+        if r < 0:
+            _err("gvizhip_batch_run")
+        return int(r), it, lat, md
+
+    def positions(self):
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_batch_download_positions(self.h, _p(out)) != 0:
+            _err("gvizhip_batch_download_positions")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gvizhip_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -226,6 +226,40 @@ int gvizhip_grip_profile_rounds(gvizhip_grip *g, int64_t active, int64_t layer,
                                 double *roundMs);
 void gvizhip_grip_destroy(gvizhip_grip *g);
 
+/* ------------------------------------------------------------------ batch */
+/* No reference counterpart (the reference lays out one graph per embedder
+ * state): up to 4096 independent graphs laid out together. Graph g owns
+ * vertices [vertexStart[g], vertexStart[g+1]) of one CSR (offsets[n+1], adj
+ * over those global ids; no edge may leave its graph). Every stage of
+ * gvizForceEmbedderStep runs once for the whole batch; each graph follows
+ * exactly the trajectory gvizhip_force gives it alone from the same start
+ * (same keys, tree, interaction sets and summation order; the direction
+ * between exactly coincident points hashes batch-wide ids instead).
+ * Barnes-Hut only; a graph may need at most 58 quadtree levels. */
+typedef struct gvizhip_batch gvizhip_batch;
+int gvizhip_batch_create(gvizhip_batch **out, int64_t graphs,
+                         const int64_t *vertexStart, const int64_t *offsets,
+                         const int64_t *adj, const gvizhip_force_config *cfg);
+/* Positions (2n) for every graph; speed 1, efficiency 1, forces zero. */
+int gvizhip_batch_begin(gvizhip_batch *b, const double *positions);
+/* Schedule knobs that never change results: traversal wave order (0: graph
+ * order, 1: largest graphs first) and steps per captured launch (1..64). */
+int gvizhip_batch_schedule(gvizhip_batch *b, int32_t waveOrder,
+                           int32_t stepsPerLaunch);
+/* Times every schedule on `trialSteps` steps from the current state, puts
+ * the state back and keeps the fastest schedule. */
+int gvizhip_batch_autotune(gvizhip_batch *b, int64_t trialSteps,
+                           int32_t *waveOrder, int32_t *stepsPerLaunch);
+/* Steps every graph until its maxDisp < epsilon (epsilon >= 0) or maxIters
+ * steps. Per graph (any may be NULL): steps taken, milliseconds from the
+ * call to the check that saw it stop (the call's total when it ran out of
+ * steps), last maxDisp. Returns the steps launched. */
+int64_t gvizhip_batch_run(gvizhip_batch *b, int64_t maxIters, double epsilon,
+                          int64_t *iterations, double *latencyMs,
+                          double *maxDisp);
+int gvizhip_batch_download_positions(gvizhip_batch *b, double *positions);
+void gvizhip_batch_destroy(gvizhip_batch *b);
+
 /* ---------------------------------------------------------------- metrics */
 /* No reference counterpart: layout quality for judging runs. Normalised
  * stress over BFS hop distances from `count` pivot vertices:
