@@ -21,6 +21,7 @@ The other measured paths, one JSON line each:
     python bench.py --workload allpairs_er65k    exact FR all-pairs step, ER n=65,536 avg deg 8
     python bench.py --workload tutte_delaunay1m  Tutte Jacobi sweeps, Delaunay n=1,000,000
     python bench.py --workload grip_rgg4m        GRIP schedule and finest-layer rounds, RGG n=4,000,000
+    python bench.py --workload batch4096         4,096 graphs (50..20,000 vertices) laid out together
 """
 from __future__ import annotations
 
@@ -442,11 +443,116 @@ def warm_start_line(graphs=3, n=200_000):
             "reached": f"{len(ok)}/{len(rows)}", "per_graph": rows}
 
 
+def batch_graphs(count=4096, seed=0, n_min=50, n_max=20000):
+    """Random trees, 2-D grids and Erdos-Renyi graphs (average degree 4),
+    vertex counts log-uniform in [n_min, n_max], seeded."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), count)).astype(np.int64)
+    kinds = rng.integers(0, 3, count)
+    out = []
+    for i, (size, kind) in enumerate(zip(sizes.tolist(), kinds.tolist())):
+        if kind == 0:
+            n, e = G.random_tree(size, seed * 10007 + i)
+        elif kind == 1:
+            w = max(1, int(round(np.sqrt(size))))
+            n, e = G.grid(w, max(1, int(round(size / w))))
+        else:
+            n, e = G.erdos_renyi_fast(size, 4.0, seed * 10007 + i)
+        out.append(G.csr_from_edges(n, e))
+    return out, kinds
+
+
+def bench_batch(args):
+    """4,096 graphs, FR Barnes-Hut (theta 1.0, the reference's default) in
+    FP32, laid out together: every graph runs until its maxDisp < 0.5 or 500
+    steps. value = vertex-steps per second over the batch; per-graph latency
+    = time from the start of the run to the check that saw it stop."""
+    import struct
+    t0 = time.time()
+    graphs, kinds = batch_graphs()
+    sizes = np.array([len(o) - 1 for o, _ in graphs])
+    print(f"# batch of {len(graphs)} graphs, {sizes.sum()} vertices built in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    starts = [G.uniform_positions(int(n), 1000 + i) for i, n in enumerate(sizes)]
+    start = np.concatenate(starts)
+    cfg = dict(model=native.MODEL_FR, precision="fp32", theta=1.0, edge_length=10.0,
+               gravity=0.0)
+    b = native.BatchEngine(graphs, **cfg)
+    b.begin(start)
+    order, spl = b.autotune(4)
+    b.begin(start)
+    max_steps, eps = 500, 0.5
+    t1 = time.perf_counter()
+    launched, its, lat, _ = b.run(max_steps, epsilon=eps)
+    el = time.perf_counter() - t1
+    vsteps = float((its * sizes).sum())
+    args.steps = int(launched)
+    b.begin(start)
+    trav_ms, inter, step_ms = b.profile(3)
+    launch_s = trav_ms / 1e3 / 3
+    achieved = FLOP_PER_INTERACTION * inter / 3 / launch_s / 1e12
+    line = base_line("batched layouts: vertex-steps/s over 4,096 graphs", vsteps / el,
+                     "vertex-steps/s", args, 1e3 * el / max(launched, 1), "f32",
+                     "4,096 graphs: random trees, 2-D grids, Erdos-Renyi (average degree "
+                     "4), vertex counts log-uniform in [50, 20,000]; FR Barnes-Hut theta 1.0 "
+                     "FP32; each graph until maxDisp < 0.5 or 500 steps",
+                     {"graphs": len(graphs), "vertices": int(sizes.sum()),
+                      "edges": int(sum(len(a) for _, a in graphs) // 2),
+                      "schedule": {"wave_order": order, "steps_per_launch": spl}})
+    line["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2),
+                          "p99": round(float(np.percentile(lat, 99)), 2),
+                          "max": round(float(lat.max()), 2)}
+    line["graphs_per_s"] = round(len(graphs) / el, 1)
+    line["steps_per_graph"] = {"p50": int(np.percentile(its, 50)),
+                               "max": int(its.max()),
+                               "stopped": int((its < max_steps).sum())}
+    line["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3),
+                        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                        "kernel": "k_traverse (batch)",
+                        "note": "FP32 vector ALU; 15 FLOP per repulsion evaluation over "
+                                "all graphs of one step",
+                        "launch_ms": round(launch_s * 1e3, 4),
+                        "interactions_per_launch": int(inter / 3),
+                        "step_ms_uncaptured": round(step_ms / 3, 4)}
+    cpu = None
+    if not args.no_cpu_baseline and os.path.exists(HARNESS):
+        threads = min(16, len(os.sched_getaffinity(0)))
+        order_idx = np.argsort(sizes)
+        pick = order_idx[np.linspace(0, len(sizes) - 1, 24).astype(int)]
+        tot_t, tot_vs = 0.0, 0.0
+        for gi in pick.tolist():
+            o, a = graphs[gi]
+            n = len(o) - 1
+            src = np.repeat(np.arange(n), np.diff(o))
+            e = np.stack([src, a], 1)
+            e = e[e[:, 0] < e[:, 1]]
+            steps = int(min(its[gi], 10))
+            payload = graph_bytes(n, e) + struct.pack("<qqqqqq", 0, 1, 0, 1, steps,
+                                                      threads - 1) \
+                + struct.pack("<ddddddd", 10.0, 1.0, 0.0, 100.0, 0.0, 0.0, 1.0) \
+                + np.ascontiguousarray(starts[gi], dtype="<f8").tobytes()
+            t = np.frombuffer(run_harness("time_force", payload, 600), dtype="<f8")
+            tot_t += float(t.sum())
+            tot_vs += steps * n
+        cpu = {"value": tot_vs / tot_t, "unit": "vertex-steps/s", "cores": threads,
+               "kind": "reference",
+               "sample": f"24 graphs of the batch spread over its size range, up to 10 "
+                         f"gvizForceEmbedderStep calls each (the reference lays out one "
+                         f"graph at a time), pool of {threads} threads; "
+                         f"{tot_vs:.0f} vertex-steps in {tot_t:.1f} s"}
+    line["cpu_baseline"] = cpu
+    if cpu:
+        line["speedup_vs_cpu"] = round(line["value"] / cpu["value"], 1)
+    emit(line)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="force_ba1m",
-                    choices=["force_ba1m", "allpairs_er65k", "tutte_delaunay1m", "grip_rgg4m"])
+                    choices=["force_ba1m", "allpairs_er65k", "tutte_delaunay1m", "grip_rgg4m",
+                             "batch4096"])
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -478,7 +584,7 @@ def main():
     args.steps_given = args.steps is not None
     if args.workload != "force_ba1m":
         {"allpairs_er65k": bench_allpairs, "tutte_delaunay1m": bench_tutte,
-         "grip_rgg4m": bench_grip}[args.workload](args)
+         "grip_rgg4m": bench_grip, "batch4096": bench_batch}[args.workload](args)
         return
     if args.steps is None:
         args.steps = 200

@@ -466,6 +466,64 @@ int gvizhip_batch_autotune(gvizhip_batch *bt, int64_t trialSteps, int32_t *waveO
   return 0;
 }
 
+int gvizhip_batch_profile(gvizhip_batch *bt, int64_t steps, double *traversalMs,
+                          int64_t *interactions, double *stepMs) {
+  if (!bt || steps < 0) {
+    set_error("gvizhip_batch_profile: invalid arguments");
+    return -1;
+  }
+  ForceCommon &c = bt->c;
+  if (!c.travBegin) {
+    GVH_CHECK(hipEventCreate(&c.travBegin));
+    GVH_CHECK(hipEventCreate(&c.travEnd));
+  }
+  hipEvent_t s0, s1;
+  GVH_CHECK(hipEventCreate(&s0));
+  GVH_CHECK(hipEventCreate(&s1));
+  std::vector<int32_t> cnt(c.n);
+  double trav = 0.0, step = 0.0;
+  int64_t inter = 0;
+  int rc = 0;
+  for (int64_t k = 0; k < steps && rc == 0; k++) {
+    GVH_CHECK(hipMemsetAsync(c.interactions.p, 0, sizeof(int32_t) * c.n, c.stream));
+    c.timeTraversal = 1;
+    c.wantInteractions = 1;
+    if (hipEventRecord(s0, c.stream) != hipSuccess || batch_step(bt) ||
+        hipEventRecord(s1, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess)
+      rc = -1;
+    c.timeTraversal = 0;
+    c.wantInteractions = 0;
+    float ms = 0.0f;
+    if (rc == 0) {
+      (void)hipEventElapsedTime(&ms, c.travBegin, c.travEnd);
+      trav += ms;
+      (void)hipEventElapsedTime(&ms, s0, s1);
+      step += ms;
+      if (hipMemcpy(cnt.data(), c.interactions.p, sizeof(int32_t) * c.n,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        rc = -1;
+      for (int64_t i = 0; rc == 0 && i < c.n; i++)
+        inter += cnt[i];
+    }
+  }
+  (void)hipEventDestroy(s0);
+  (void)hipEventDestroy(s1);
+  drop_graph(bt);
+  if (rc) {
+    if (!gvizhip_last_error()[0])
+      set_error("gvizhip_batch_profile failed");
+    return -1;
+  }
+  if (traversalMs)
+    *traversalMs = trav;
+  if (interactions)
+    *interactions = inter;
+  if (stepMs)
+    *stepMs = step;
+  return 0;
+}
+
 int gvizhip_batch_download_positions(gvizhip_batch *bt, double *positions) {
   if (!bt || !positions) {
     set_error("gvizhip_batch_download_positions: invalid arguments");
@@ -481,6 +539,10 @@ void gvizhip_batch_destroy(gvizhip_batch *bt) {
   if (!bt)
     return;
   drop_graph(bt);
+  if (bt->c.travBegin)
+    (void)hipEventDestroy(bt->c.travBegin);
+  if (bt->c.travEnd)
+    (void)hipEventDestroy(bt->c.travEnd);
   if (bt->c.stream) {
     (void)hipStreamSynchronize(bt->c.stream);
     (void)hipStreamDestroy(bt->c.stream);

@@ -1102,19 +1102,22 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
                        bs.rootOf.p);
   GVH_CHECK_LAUNCH();
   const unsigned tb = (unsigned)((bs.waves + kTravWaves - 1) / kTravWaves);
+  int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
+  if (c.timeTraversal)
+    GVH_CHECK(hipEventRecord(c.travBegin, st));
   if (model == 1)
     hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTravThreads), 0, st,
                        (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,
                        (const R2 *)c.smr.p, c.vals.p, n, theta, kSq, oc, (R2 *)c.rep.p,
-                       (int32_t *)nullptr, c.radius64.p, bs.scG.p,
-                       (unsigned long long *)nullptr, sv);
+                       inter, c.radius64.p, bs.scG.p, (unsigned long long *)nullptr, sv);
   else
     hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTravThreads), 0, st,
                        (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,
                        (const R2 *)c.smr.p, c.vals.p, n, theta, kSq, oc, (R2 *)c.rep.p,
-                       (int32_t *)nullptr, c.radius64.p, bs.scG.p,
-                       (unsigned long long *)nullptr, sv);
+                       inter, c.radius64.p, bs.scG.p, (unsigned long long *)nullptr, sv);
   GVH_CHECK_LAUNCH();
+  if (c.timeTraversal)
+    GVH_CHECK(hipEventRecord(c.travEnd, st));
   const unsigned vb = (unsigned)bs.vblocks;
   if (model == 1) {
     if (c.nHeavy > 0)

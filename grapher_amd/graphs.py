@@ -114,6 +114,30 @@ def erdos_renyi(n: int, avg_degree: float, seed: int = 0):
     return n, np.asarray(edges, dtype=np.int64).reshape(-1, 2)
 
 
+def erdos_renyi_fast(n: int, avg_degree: float, seed: int = 0):
+    """G(n, M) with M = round(n * avg_degree / 2) distinct undirected edges
+    drawn with numpy (rejection of loops and repeats); edges sorted (u < v).
+    A vectorised companion of erdos_renyi for large batches; the two draw
+    different graphs from the same seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m = int(round(n * avg_degree / 2.0))
+    if n < 2 or m == 0:
+        return n, np.zeros((0, 2), dtype=np.int64)
+    m = min(m, n * (n - 1) // 2)
+    keys = np.zeros(0, dtype=np.int64)
+    while keys.size < m:
+        need = m - keys.size
+        u = rng.integers(0, n, size=2 * need + 16)
+        v = rng.integers(0, n, size=2 * need + 16)
+        ok = u != v
+        a = np.minimum(u[ok], v[ok])
+        b = np.maximum(u[ok], v[ok])
+        keys = np.unique(np.concatenate([keys, a * n + b]))
+    keys = keys[:m] if keys.size == m else np.sort(
+        rng.choice(keys, size=m, replace=False))
+    return n, np.stack([keys // n, keys % n], axis=1)
+
+
 def random_tree(n: int, seed: int = 0):
     """Vertex v > 0 attaches to a uniform earlier vertex."""
     rng = np.random.Generator(np.random.PCG64(seed))

@@ -127,6 +127,7 @@ def lib():
         "gvizhip_batch_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32]),
         "gvizhip_batch_autotune": (C.c_int, [_vp, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "gvizhip_batch_run": (_i64, [_vp, _i64, C.c_double, _vp, _vp, _vp]),
+        "gvizhip_batch_profile": (C.c_int, [_vp, _i64, C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(C.c_double)]),
         "gvizhip_batch_download_positions": (C.c_int, [_vp, _vp]),
         "gvizhip_batch_destroy": (None, [_vp]),
         "gvizhip_stress_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
@@ -172,7 +173,8 @@ DECLARED = [
     "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_profile_rounds",
     "gvizhip_grip_destroy",
     "gvizhip_batch_create", "gvizhip_batch_begin", "gvizhip_batch_schedule",
-    "gvizhip_batch_autotune", "gvizhip_batch_run", "gvizhip_batch_download_positions",
+    "gvizhip_batch_autotune", "gvizhip_batch_run", "gvizhip_batch_profile",
+    "gvizhip_batch_download_positions",
     "gvizhip_batch_destroy",
     "gvizhip_stress_create", "gvizhip_stress_evaluate", "gvizhip_stress_destroy",
     "gvizhip_gnn_create", "gvizhip_gnn_param_count", "gvizhip_gnn_set_params",
@@ -643,6 +645,14 @@ class BatchEngine:
         if r < 0:
             _err("gvizhip_batch_run")
         return int(r), it, lat, md
+
+    def profile(self, steps):
+        """(traversal ms, repulsion evaluations, step ms) summed over steps."""
+        t, i, s = C.c_double(), _i64(), C.c_double()
+        if lib().gvizhip_batch_profile(self.h, steps, C.byref(t), C.byref(i),
+                                       C.byref(s)) != 0:
+            _err("gvizhip_batch_profile")
+        return t.value, i.value, s.value
 
     def positions(self):
         out = np.zeros((self.n, 2))

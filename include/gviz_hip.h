@@ -257,6 +257,11 @@ int gvizhip_batch_autotune(gvizhip_batch *b, int64_t trialSteps,
 int64_t gvizhip_batch_run(gvizhip_batch *b, int64_t maxIters, double epsilon,
                           int64_t *iterations, double *latencyMs,
                           double *maxDisp);
+/* Measurement hook: `steps` uncaptured steps with HIP events around the
+ * traversal kernel and the whole step; summed times and repulsion
+ * evaluations (outputs may be NULL). */
+int gvizhip_batch_profile(gvizhip_batch *b, int64_t steps, double *traversalMs,
+                          int64_t *interactions, double *stepMs);
 int gvizhip_batch_download_positions(gvizhip_batch *b, double *positions);
 void gvizhip_batch_destroy(gvizhip_batch *b);
 
