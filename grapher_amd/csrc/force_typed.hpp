@@ -27,15 +27,15 @@ constexpr int kHeavyDegree = 32;    // rows above this get a wavefront each
 // Direction for exactly coincident points: splitmix64 finaliser over the two
 // vertex ids mapped to an angle. The reference hashes buffer addresses here
 // (include/core/gvizVec.h:385-394), which nothing else can reproduce.
-static __device__ __noinline__ void coincident_dir(int64_t selfId, int64_t otherId,
-                                            double &ux, double &uy) {
+// Returned by value: an out-parameter would put the call's results in
+// scratch memory in every kernel that inlines a force evaluation.
+static __device__ __noinline__ double2 coincident_dir(int64_t selfId, int64_t otherId) {
   uint64_t bits = (uint64_t)selfId ^ ((uint64_t)otherId * 0x9E3779B97F4A7C15ull);
   bits ^= bits >> 33;
   bits *= 0xff51afd7ed558ccdull;
   bits ^= bits >> 33;
   double angle = (double)(bits % 1000000u) / 1000000.0 * 2.0 * M_PI;
-  ux = cos(angle);
-  uy = sin(angle);
+  return double2{cos(angle), sin(angle)};
 }
 
 // ------------------------------------------------------- pairwise physics
@@ -62,7 +62,9 @@ template <> struct Phys<double> {
       ux = dx / dist;
       uy = dy / dist;
     } else {
-      coincident_dir(selfId, otherId, ux, uy);
+      const double2 u = coincident_dir(selfId, otherId);
+      ux = u.x;
+      uy = u.y;
     }
     if (MODEL == 1) {
       double mp = mSelf * mOther;
@@ -147,10 +149,9 @@ template <> struct Phys<float> {
       ux = dx * inv;
       uy = dy * inv;
     } else {
-      double dux, duy;
-      coincident_dir(selfId, otherId, dux, duy);
-      ux = (float)dux;
-      uy = (float)duy;
+      const double2 u = coincident_dir(selfId, otherId);
+      ux = (float)u.x;
+      uy = (float)u.y;
     }
     float num = MODEL == 1 ? mSelf * mOther : kSq * truncf(mOther);
     float mag = gap > 0.0f ? num * __builtin_amdgcn_rcpf(gap) : num * oc;
@@ -321,10 +322,38 @@ __device__ __forceinline__ double rl(double v, int l) {
   return __longlong_as_double((long long)rl((uint64_t)__double_as_longlong(v), l));
 }
 
-constexpr int kTravThreads = 128;               // two wavefronts per block
+#ifndef GVH_TRAV_STACK
+#define GVH_TRAV_STACK 512
+#endif
+#ifndef GVH_TRAV_DEFER
+#define GVH_TRAV_DEFER 128
+#endif
+#ifndef GVH_TRAV_MINBLOCKS
+#define GVH_TRAV_MINBLOCKS 1
+#endif
+#ifndef GVH_TRAV_THREADS
+#define GVH_TRAV_THREADS 128
+#endif
+constexpr int kTravThreads = GVH_TRAV_THREADS;  // wavefronts per block x 64
 constexpr int kTravWaves = kTravThreads / kWave;
-constexpr int kTravStack = 768;                 // pending (node, mask) per wave
-constexpr int kTravDefer = 128;                 // deferred bodies per wave
+constexpr int kTravStack = GVH_TRAV_STACK;      // pending (node, mask) per wave
+constexpr int kTravDefer = GVH_TRAV_DEFER;      // deferred bodies per wave
+// Stack entries held back from the wide phase. Once the stack holds more than
+// kTravStack - kTravReserve - 3 entries the wavefront takes one entry per
+// step, which is a depth-first walk: each popped cell adds at most 3 net
+// entries per level it descends, and a cell lies at most 64 levels deep, so
+// the walk needs at most 3 * 64 + 3 slots above that point and the stack can
+// never overflow.
+constexpr int kTravReserve = 3 * 64 + 4;
+static_assert(kTravStack - kTravReserve >= 3 * kWave, "traversal stack too small");
+
+// wave-uniform double held in scalar registers
+__device__ __forceinline__ double uniform(double v) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 // Barnes-Hut repulsion, one wavefront per 64 consecutive targets in key
 // order. Every lane ends up with exactly the interaction set of the
@@ -341,7 +370,7 @@ constexpr int kTravDefer = 128;                 // deferred bodies per wave
 //   * cells the box cannot decide are tested by each lane, as the reference
 //     does, and opened for the lanes that reject them.
 template <class Real, class PosT, int MODEL>
-__global__ __launch_bounds__(kTravThreads) void k_traverse(
+__global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     const NodeRec<Real, PosT> *__restrict__ nodes,
     const typename V2<PosT>::type *__restrict__ spos,
     const typename V2<Real>::type *__restrict__ smr,
@@ -383,13 +412,20 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
   if (live == 0)
     return;
   const uint64_t ltMask = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  // A point never acts on itself: its own lane, if it is one of this wave's
+  // targets, is cleared from the mask when the point is deferred.
+  const int64_t base = (int64_t)rl((uint64_t)(s - lane), 0);
+  auto notSelf = [&](int64_t p, uint64_t m) -> uint64_t {
+    const int64_t rel = p - base;
+    return (rel >= 0 && rel < kWave) ? (m & ~(1ull << rel)) : m;
+  };
   typename V2<PosT>::type me = valid ? spos[s] : typename V2<PosT>::type{0, 0};
   typename V2<Real>::type mr = valid ? smr[s] : typename V2<Real>::type{0, 0};
   const int64_t selfId = valid ? (int64_t)sidx[s] : 0;
-  const double bx0 = wave_min(valid ? (double)me.x : INFINITY);
-  const double bx1 = wave_max(valid ? (double)me.x : -INFINITY);
-  const double by0 = wave_min(valid ? (double)me.y : INFINITY);
-  const double by1 = wave_max(valid ? (double)me.y : -INFINITY);
+  const double bx0 = uniform(wave_min(valid ? (double)me.x : INFINITY));
+  const double bx1 = uniform(wave_max(valid ? (double)me.x : -INFINITY));
+  const double by0 = uniform(wave_min(valid ? (double)me.y : INFINITY));
+  const double by1 = uniform(wave_max(valid ? (double)me.y : -INFINITY));
   const double margin = sizeof(Real) == sizeof(double) ? 1e-12 : 1e-5;
   const double thLo = (double)theta * (1.0 - margin);
   const double thHi = (double)theta * (1.0 + margin);
@@ -410,7 +446,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
       const uint64_t msk = has ? dMask[wave][j] : 0ull;
       PosT bx = 0, by = 0;
       Real bm = 0, br = 0;
-      int32_t bid = (int32_t)n, bpt = -1;
+      int32_t bid = (int32_t)n;
       if (has) {
         if (ref < 0) {
           const int32_t p = -ref - 1;
@@ -421,7 +457,6 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
           bm = om.x;
           br = om.y;
           bid = (int32_t)sidx[p];
-          bpt = p;
         } else {
           const NodeRec<Real, PosT> &r = nodes[ref];
           bx = r.comx;
@@ -436,8 +471,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
         const uint64_t m = rl(msk, t);
         if (tstats)
           st[2] += __popcll(m);
-        const int32_t pt = rl(bpt, t);
-        if (((m >> lane) & 1ull) && pt != s) {
+        if ((m >> lane) & 1ull) {
           const PosT ex = rl(bx, t) - me.x, ey = rl(by, t) - me.y;
           Real dx = (Real)ex, dy = (Real)ey;
           Real dist = Phys<Real>::dist(dx, dy);
@@ -465,7 +499,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
     }
     // entries the stack can take children for: each open entry adds <= 3
     int take = sp < kWave ? sp : kWave;
-    const int room = (kTravStack - sp) / 3;
+    const int room = (kTravStack - kTravReserve - sp) / 3;
     if (take > room)
       take = room > 0 ? room : 1;
     const int j = sp - 1 - lane; // lane 0 takes the top entry
@@ -501,13 +535,15 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
     }
     // deferred points and cells
     {
-      const uint64_t dm = __ballot(has && kind == kDefer);
+      const uint64_t dmsk = (has && node < 0) ? notSelf(-(int64_t)node - 1, msk) : msk;
+      const bool dflag = has && kind == kDefer && dmsk != 0;
+      const uint64_t dm = __ballot(dflag);
       if (nd + (int)__popcll(dm) > kTravDefer)
         flush();
-      if (has && kind == kDefer) {
+      if (dflag) {
         const int at = nd + (int)__popcll(dm & ltMask);
         dRef[wave][at] = node;
-        dMask[wave][at] = msk;
+        dMask[wave][at] = dmsk;
       }
       nd += (int)__popcll(dm);
     }
@@ -519,11 +555,14 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
       const int32_t first = rl(r.first, l), count = rl(r.count, l);
       const uint64_t m = rl(msk, l);
       for (int32_t k = 0; k < count; k++) {
+        const uint64_t mk = notSelf((int64_t)first + k, m);
+        if (mk == 0)
+          continue;
         if (nd == kTravDefer)
           flush();
         if (lane == 0) {
           dRef[wave][nd] = -(first + k) - 1;
-          dMask[wave][nd] = m;
+          dMask[wave][nd] = mk;
         }
         nd++;
       }
@@ -542,7 +581,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
         if (lane >= off)
           incl += v;
       }
-      const int total = __shfl(incl, kWave - 1, kWave);
+      const int total = rl(incl, kWave - 1);
       if (sp + total > kTravStack) {
         sc->traversalOverflow = 1;
         break;
@@ -594,6 +633,7 @@ __global__ __launch_bounds__(kTravThreads) void k_traverse(
           am = 0;
           break;
         }
+#pragma unroll
         for (int c = 3; c >= 0; c--) {
           const int32_t ch = rl(r.child[c], l);
           if (ch != kNone) {
