@@ -323,18 +323,18 @@ __device__ __forceinline__ double rl(double v, int l) {
 }
 
 #ifndef GVH_TRAV_STACK
-#define GVH_TRAV_STACK 512
+#define GVH_TRAV_STACK 384
 #endif
 #ifndef GVH_TRAV_DEFER
-#define GVH_TRAV_DEFER 128
+#define GVH_TRAV_DEFER 96
 #endif
 #ifndef GVH_TRAV_MINBLOCKS
 #define GVH_TRAV_MINBLOCKS 1
 #endif
 #ifndef GVH_TRAV_THREADS
-#define GVH_TRAV_THREADS 128
+#define GVH_TRAV_THREADS 64
 #endif
-constexpr int kTravThreads = GVH_TRAV_THREADS;  // wavefronts per block x 64
+constexpr int kTravThreads = GVH_TRAV_THREADS;  // one wavefront per block
 constexpr int kTravWaves = kTravThreads / kWave;
 constexpr int kTravStack = GVH_TRAV_STACK;      // pending (node, mask) per wave
 constexpr int kTravDefer = GVH_TRAV_DEFER;      // deferred bodies per wave
@@ -345,7 +345,16 @@ constexpr int kTravDefer = GVH_TRAV_DEFER;      // deferred bodies per wave
 // the walk needs at most 3 * 64 + 3 slots above that point and the stack can
 // never overflow.
 constexpr int kTravReserve = 3 * 64 + 4;
-static_assert(kTravStack - kTravReserve >= 3 * kWave, "traversal stack too small");
+static_assert(kTravStack - kTravReserve >= kWave, "traversal stack too small");
+
+// A deferred body as every lane reads it back from LDS: position, the lanes
+// it acts on, mass, radius (0 for a cell) and vertex id (n for a cell).
+template <class Real, class PosT> struct alignas(16) TravBody {
+  PosT x, y;
+  uint64_t mask;
+  Real m, r;
+  int32_t id;
+};
 
 // wave-uniform double held in scalar registers
 __device__ __forceinline__ double uniform(double v) {
@@ -364,9 +373,9 @@ __device__ __forceinline__ double uniform(double v) {
 //     bounding box of the wavefront's targets: a cell that every target would
 //     accept (or open), with a relative margin far above the test's rounding,
 //     is accepted (or opened) for all lanes of its mask at once;
-//   * accepted cells and leaf points are deferred to a list that is
-//     evaluated densely: 64 bodies are fetched in parallel, then broadcast
-//     one by one to all lanes;
+//   * accepted cells and leaf points are deferred: the lane that meets one
+//     writes its body to a per-wave LDS list, which is later evaluated
+//     densely, every lane reading each body with one broadcast LDS load;
 //   * cells the box cannot decide are tested by each lane, as the reference
 //     does, and opened for the lanes that reject them.
 template <class Real, class PosT, int MODEL>
@@ -381,8 +390,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
   __shared__ int32_t sNode[kTravWaves][kTravStack];
   __shared__ uint64_t sMask[kTravWaves][kTravStack];
-  __shared__ int32_t dRef[kTravWaves][kTravDefer];
-  __shared__ uint64_t dMask[kTravWaves][kTravDefer];
+  __shared__ TravBody<Real, PosT> dBody[kTravWaves][kTravDefer];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   int64_t s;
@@ -427,8 +435,11 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   const double by0 = uniform(wave_min(valid ? (double)me.y : INFINITY));
   const double by1 = uniform(wave_max(valid ? (double)me.y : -INFINITY));
   const double margin = sizeof(Real) == sizeof(double) ? 1e-12 : 1e-5;
+  // squared thresholds: the box tests compare squares (no square roots); the
+  // margin is far above the rounding of either form
   const double thLo = (double)theta * (1.0 - margin);
   const double thHi = (double)theta * (1.0 + margin);
+  const double thLo2 = thLo * thLo, thHi2 = thHi * thHi;
   Real ax = 0, ay = 0;
   int32_t cnt = 0;
   int nd = 0;
@@ -436,54 +447,39 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   // their mask lanes, undecided cells, their mask lanes, max stack depth
   unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
 
-  // Evaluates deferred bodies: lane j fetches body j, then all lanes take
-  // each body in turn.
+  // Evaluates the deferred bodies in the order they were deferred.
   auto flush = [&]() {
-    for (int base = 0; base < nd; base += kWave) {
-      const int j = base + lane;
-      const bool has = j < nd;
-      const int32_t ref = has ? dRef[wave][j] : -1;
-      const uint64_t msk = has ? dMask[wave][j] : 0ull;
-      PosT bx = 0, by = 0;
-      Real bm = 0, br = 0;
-      int32_t bid = (int32_t)n;
-      if (has) {
-        if (ref < 0) {
-          const int32_t p = -ref - 1;
-          typename V2<PosT>::type o = spos[p];
-          typename V2<Real>::type om = smr[p];
-          bx = o.x;
-          by = o.y;
-          bm = om.x;
-          br = om.y;
-          bid = (int32_t)sidx[p];
-        } else {
-          const NodeRec<Real, PosT> &r = nodes[ref];
-          bx = r.comx;
-          by = r.comy;
-          bm = r.mass;
-        }
-      }
-      const int len = nd - base < kWave ? nd - base : kWave;
+    if (tstats)
+      st[1] += nd;
+#pragma unroll 2
+    for (int t = 0; t < nd; t++) {
+      const TravBody<Real, PosT> b = dBody[wave][t];
       if (tstats)
-        st[1] += len;
-      for (int t = 0; t < len; t++) {
-        const uint64_t m = rl(msk, t);
-        if (tstats)
-          st[2] += __popcll(m);
-        if ((m >> lane) & 1ull) {
-          const PosT ex = rl(bx, t) - me.x, ey = rl(by, t) - me.y;
-          Real dx = (Real)ex, dy = (Real)ey;
-          Real dist = Phys<Real>::dist(dx, dy);
-          Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + rl(br, t),
-                                                mr.x, rl(bm, t), kSq, oc, selfId,
-                                                (int64_t)rl(bid, t), ax, ay, ex, ey,
-                                                r64, n);
-          cnt++;
-        }
+        st[2] += __popcll(rl(b.mask, 0));
+      if ((b.mask >> lane) & 1ull) {
+        const PosT ex = b.x - me.x, ey = b.y - me.y;
+        Real dx = (Real)ex, dy = (Real)ey;
+        Real dist = Phys<Real>::dist(dx, dy);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + b.r, mr.x, b.m, kSq,
+                                              oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
+                                              r64, n);
+        cnt++;
       }
     }
     nd = 0;
+  };
+  // Writes point p's body at list slot `at`.
+  auto putPoint = [&](int at, int32_t p, uint64_t mk) {
+    const typename V2<PosT>::type o = spos[p];
+    const typename V2<Real>::type om = smr[p];
+    TravBody<Real, PosT> b;
+    b.x = o.x;
+    b.y = o.y;
+    b.mask = mk;
+    b.m = om.x;
+    b.r = om.y;
+    b.id = (int32_t)sidx[p];
+    dBody[wave][at] = b;
   };
 
   int sp = 1;
@@ -525,9 +521,9 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         const double fx = fmax(fabs(cx - bx0), fabs(cx - bx1));
         const double fy = fmax(fabs(cy - by0), fabs(cy - by1));
         const double twoh = (double)r.twoh;
-        if (twoh < thLo * sqrt(gx * gx + gy * gy))
+        if (twoh * twoh < thLo2 * (gx * gx + gy * gy))
           kind = kDefer;
-        else if (twoh >= thHi * sqrt(fx * fx + fy * fy))
+        else if (twoh * twoh >= thHi2 * (fx * fx + fy * fy))
           kind = kOpen;
         else
           kind = kAmbig;
@@ -542,8 +538,18 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         flush();
       if (dflag) {
         const int at = nd + (int)__popcll(dm & ltMask);
-        dRef[wave][at] = node;
-        dMask[wave][at] = dmsk;
+        if (node < 0) {
+          putPoint(at, -node - 1, dmsk);
+        } else {
+          TravBody<Real, PosT> b;
+          b.x = r.comx;
+          b.y = r.comy;
+          b.mask = dmsk;
+          b.m = r.mass;
+          b.r = (Real)0;
+          b.id = (int32_t)n;
+          dBody[wave][at] = b;
+        }
       }
       nd += (int)__popcll(dm);
     }
@@ -554,17 +560,16 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
       rm &= rm - 1;
       const int32_t first = rl(r.first, l), count = rl(r.count, l);
       const uint64_t m = rl(msk, l);
-      for (int32_t k = 0; k < count; k++) {
-        const uint64_t mk = notSelf((int64_t)first + k, m);
-        if (mk == 0)
-          continue;
-        if (nd == kTravDefer)
+      // the leaf's points, up to 64 at a time, one per lane
+      for (int32_t k0 = 0; k0 < count; k0 += kWave) {
+        const int32_t p = first + k0 + lane;
+        const uint64_t mk = k0 + lane < count ? notSelf(p, m) : 0ull;
+        const uint64_t pm = __ballot(mk != 0);
+        if (nd + (int)__popcll(pm) > kTravDefer)
           flush();
-        if (lane == 0) {
-          dRef[wave][nd] = -(first + k) - 1;
-          dMask[wave][nd] = mk;
-        }
-        nd++;
+        if (mk != 0)
+          putPoint(nd + (int)__popcll(pm & ltMask), p, mk);
+        nd += (int)__popcll(pm);
       }
     }
     // children of cells every target opens

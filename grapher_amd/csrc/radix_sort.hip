@@ -8,12 +8,14 @@ constexpr int T = RadixSorter::kThreads;
 constexpr int I = RadixSorter::kItems;
 constexpr int TILE = RadixSorter::kTile;
 constexpr int WAVES = T / kWave;
+static_assert(T % 256 == 0, "one or more threads per digit");
 
 __global__ __launch_bounds__(T) void k_hist(const uint64_t *__restrict__ keys,
                                             int64_t n, int shift, int tiles,
                                             uint32_t *__restrict__ table) {
   __shared__ uint32_t hist[256];
-  hist[threadIdx.x] = 0;
+  if (threadIdx.x < 256)
+    hist[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * TILE;
 #pragma unroll 4
@@ -25,27 +27,24 @@ __global__ __launch_bounds__(T) void k_hist(const uint64_t *__restrict__ keys,
     }
   }
   __syncthreads();
-  table[(int64_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+  if (threadIdx.x < 256)
+    table[(int64_t)threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
 // Exclusive scan of one digit's row of per-tile counts (one block per
-// digit), in place; the row total goes to totals[digit].
-constexpr int kRowMax = 8192; // tiles per sort: up to 32M keys
+// digit), in place; the row total goes to totals[digit]. Each thread owns a
+// contiguous chunk of the row, so the row length is not bounded by LDS.
 __global__ __launch_bounds__(T) void k_rowscan(uint32_t *__restrict__ table,
                                                int tiles,
                                                uint32_t *__restrict__ totals) {
-  __shared__ uint32_t row[kRowMax];
   __shared__ uint32_t part[T];
   uint32_t *r = table + (int64_t)blockIdx.x * tiles;
-  for (int i = threadIdx.x; i < tiles; i += T)
-    row[i] = r[i];
-  __syncthreads();
   const int chunk = (tiles + T - 1) / T;
-  const int b = threadIdx.x * chunk;
-  const int e = b + chunk < tiles ? b + chunk : tiles;
+  const int64_t b = (int64_t)threadIdx.x * chunk;
+  const int64_t e = b + chunk < tiles ? b + chunk : tiles;
   uint32_t sum = 0;
-  for (int i = b; i < e; i++)
-    sum += row[i];
+  for (int64_t i = b; i < e; i++)
+    sum += r[i];
   part[threadIdx.x] = sum;
   __syncthreads();
   for (int off = 1; off < T; off <<= 1) {
@@ -55,14 +54,11 @@ __global__ __launch_bounds__(T) void k_rowscan(uint32_t *__restrict__ table,
     __syncthreads();
   }
   uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-  for (int i = b; i < e; i++) {
-    uint32_t v = row[i];
-    row[i] = run;
+  for (int64_t i = b; i < e; i++) {
+    uint32_t v = r[i];
+    r[i] = run;
     run += v;
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < tiles; i += T)
-    r[i] = row[i];
   if (threadIdx.x == T - 1)
     totals[blockIdx.x] = part[T - 1];
 }
@@ -85,12 +81,13 @@ __global__ __launch_bounds__(T) void k_scatter(
   const int64_t base = (int64_t)blockIdx.x * TILE;
   const int tileLen = (int)((n - base) < TILE ? (n - base) : TILE);
 
-  gOff[tid] = table[(int64_t)tid * tiles + blockIdx.x];
-  running[tid] = totals[tid];
-  tStart[tid] = 0;
-#pragma unroll
-  for (int w = 0; w < WAVES; w++)
-    waveCnt[w][tid] = 0;
+  if (tid < 256) {
+    gOff[tid] = table[(int64_t)tid * tiles + blockIdx.x];
+    running[tid] = totals[tid];
+    tStart[tid] = 0;
+  }
+  for (int i = tid; i < WAVES * 256; i += T)
+    waveCnt[i / 256][i % 256] = 0;
   __syncthreads();
 
   // Digit bases: exclusive scan of the 256 digit totals (wave 0).
@@ -112,15 +109,12 @@ __global__ __launch_bounds__(T) void k_scatter(
     gOff[4 * tid + 3] += ex + c0 + c1 + c2;
   }
   __syncthreads();
-  running[tid] = 0;
-  __syncthreads();
-  // Tile histogram -> exclusive digit starts inside the tile.
-  for (int r = 0; r < I; r++) {
-    int li = r * T + tid;
-    if (li < tileLen) {
-      uint32_t d = (uint32_t)(keysIn[base + li] >> shift) & 255u;
-      atomicAdd(&running[d], 1u);
-    }
+  // This tile's digit counts, read back from the scanned table (the next
+  // tile's offset minus this one's) -> exclusive digit starts in the tile.
+  if (tid < 256) {
+    const int64_t at = (int64_t)tid * tiles + blockIdx.x;
+    const uint32_t next = (int)blockIdx.x + 1 < tiles ? table[at + 1] : totals[tid];
+    running[tid] = next - table[at];
   }
   __syncthreads();
   if (tid < kWave) {
@@ -142,7 +136,8 @@ __global__ __launch_bounds__(T) void k_scatter(
     tStart[4 * tid + 3] = ex + c0 + c1 + c2;
   }
   __syncthreads();
-  running[tid] = 0;
+  if (tid < 256)
+    running[tid] = 0;
   __syncthreads();
 
   const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
@@ -173,7 +168,7 @@ __global__ __launch_bounds__(T) void k_scatter(
       sVal[slot] = val;
     }
     __syncthreads();
-    {
+    if (tid < 256) {
       uint32_t add = 0;
 #pragma unroll
       for (int w = 0; w < WAVES; w++) {
@@ -200,11 +195,6 @@ int RadixSorter::reserve(int64_t n) {
   if (n <= capacity)
     return 0;
   int64_t tiles = (n + kTile - 1) / kTile;
-  if (tiles > kRowMax) {
-    set_error("radix sort: %lld keys exceed the %d-tile limit", (long long)n,
-              kRowMax);
-    return -1;
-  }
   if (keysAlt.alloc(n) || valsAlt.alloc(n) || table.alloc(256 * tiles) ||
       totals.alloc(256))
     return -1;

@@ -11,10 +11,17 @@
 
 namespace gvh {
 
+#ifndef GVH_SORT_THREADS
+#define GVH_SORT_THREADS 512
+#endif
+#ifndef GVH_SORT_ITEMS
+#define GVH_SORT_ITEMS 4
+#endif
+
 struct RadixSorter {
-  static constexpr int kThreads = 256;
-  static constexpr int kItems = 16;                  // keys per thread
-  static constexpr int kTile = kThreads * kItems;    // 4096 keys per tile
+  static constexpr int kThreads = GVH_SORT_THREADS;  // a multiple of 256
+  static constexpr int kItems = GVH_SORT_ITEMS;      // keys per thread
+  static constexpr int kTile = kThreads * kItems;    // keys per tile
   DevBuf<uint64_t> keysAlt;
   DevBuf<uint32_t> valsAlt;
   DevBuf<uint32_t> table; // 256 * tiles
