@@ -20,6 +20,7 @@ __global__ void k_begin_run(Scalars *sc, double epsilon) {
   sc->recordBase = sc->iteration;
   sc->epsilon = epsilon;
   sc->stopped = 0;
+  sc->traversalOverflow = 0; // set by any step of the run, read after it
 }
 } // namespace gvh
 
@@ -442,6 +443,7 @@ int gvizhip_force_evaluate(gvizhip_force *f, double *disp, double *structForce,
     return -1;
   int stopped = s.stopped;
   s.stopped = 0;
+  s.traversalOverflow = 0;
   if (write_scalars(f, s))
     return -1;
   c.wantInteractions = interactions != nullptr;

@@ -85,7 +85,6 @@ __global__ void k_root(const double *__restrict__ partial, int blocks,
     }
     sc->maxDepth = d;
     sc->rootRef = n >= 2 ? 0 : -1;
-    sc->traversalOverflow = 0;
   }
 }
 
@@ -410,7 +409,8 @@ __global__ __launch_bounds__(kBlock) void k_bbox_root_g(const double2 *__restric
     }
     sc->maxDepth = d;
     sc->rootRef = e - b >= 2 ? 0 : -1;
-    sc->traversalOverflow = d > kBatchLevels ? 2 : 0;
+    if (d > kBatchLevels)
+      sc->traversalOverflow = 2;
   }
 }
 

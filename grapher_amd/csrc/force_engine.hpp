@@ -55,7 +55,7 @@ struct Scalars {
   int64_t iteration; // steps taken since Begin
   int64_t recordBase;// iteration whose record lands in records[0]
   int32_t stopped;
-  int32_t traversalOverflow;
+  int32_t traversalOverflow; // 1 stack overflow, 2 tree too deep
   double meanDegreePlusOne;
   double pad[4];
 };

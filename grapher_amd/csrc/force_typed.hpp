@@ -50,7 +50,7 @@ template <> struct Phys<double> {
   // rounded, and this TU is compiled without FMA contraction.
   template <int MODEL, bool NEAR>
   __device__ __forceinline__ static void rep(double dx, double dy, double dist,
-                                             double radiusSum, double mSelf,
+                                             double /*inv*/, double radiusSum, double mSelf,
                                              double mOther, double kSq,
                                              double oc, int64_t selfId,
                                              int64_t otherId, double &ax,
@@ -81,6 +81,11 @@ template <> struct Phys<double> {
     }
   }
   __device__ __forceinline__ static double dist(double dx, double dy) {
+    return sqrt(dx * dx + dy * dy);
+  }
+  // distance; `inv` is not used by this precision
+  __device__ __forceinline__ static double dist_inv(double dx, double dy, double &inv) {
+    inv = 0.0;
     return sqrt(dx * dx + dy * dy);
   }
   // opening test of src/embedders/gvizForceEmbedder.c:58-63
@@ -130,7 +135,7 @@ template <> struct Phys<float> {
   // magnitude is ill-conditioned, so the gap is recomputed from the FP64
   // position difference and FP64 radii.
   template <int MODEL, bool NEAR>
-  __device__ __forceinline__ static void rep(float dx, float dy, float dist,
+  __device__ __forceinline__ static void rep(float dx, float dy, float dist, float inv,
                                              float radiusSum, float mSelf,
                                              float mOther, float kSq, float oc,
                                              int64_t selfId, int64_t otherId,
@@ -145,7 +150,6 @@ template <> struct Phys<float> {
     }
     float ux, uy;
     if (dist > 0.0f) {
-      float inv = __builtin_amdgcn_rcpf(dist);
       ux = dx * inv;
       uy = dy * inv;
     } else {
@@ -161,6 +165,12 @@ template <> struct Phys<float> {
   __device__ __forceinline__ static float dist(float dx, float dy) {
     float d2 = dx * dx + dy * dy;
     return d2 > 0.0f ? d2 * __builtin_amdgcn_rsqf(d2) : 0.0f;
+  }
+  // distance and its reciprocal from one reciprocal square root
+  __device__ __forceinline__ static float dist_inv(float dx, float dy, float &inv) {
+    float d2 = dx * dx + dy * dy;
+    inv = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    return d2 * inv;
   }
   __device__ __forceinline__ static bool accept(float twoh, float dist,
                                                 float theta) {
@@ -459,8 +469,9 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
       if ((b.mask >> lane) & 1ull) {
         const PosT ex = b.x - me.x, ey = b.y - me.y;
         Real dx = (Real)ex, dy = (Real)ey;
-        Real dist = Phys<Real>::dist(dx, dy);
-        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + b.r, mr.x, b.m, kSq,
+        Real inv;
+        Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + b.r, mr.x, b.m, kSq,
                                               oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
                                               r64, n);
         cnt++;
@@ -604,49 +615,65 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
       }
       sp += total;
     }
-    // cells the box cannot decide: every lane of the mask tests them
-    uint64_t am = __ballot(has && kind == kAmbig);
-    while (am) {
-      const int l = __builtin_ctzll(am);
-      am &= am - 1;
-      const uint64_t m = rl(msk, l);
-      if (tstats) {
-        st[3]++;
-        st[4] += __popcll(m);
+    // cells the box cannot decide: staged in the free tail of the body list
+    // (radius slot = cell size, id = node), then every lane of each cell's
+    // mask tests it, as the reference does
+    {
+      const bool amb = has && kind == kAmbig;
+      uint64_t am = __ballot(amb);
+      const int na = (int)__popcll(am);
+      if (na != 0 && nd + na > kTravDefer)
+        flush();
+      if (amb) {
+        TravBody<Real, PosT> b;
+        b.x = r.comx;
+        b.y = r.comy;
+        b.mask = msk;
+        b.m = r.mass;
+        b.r = r.twoh;
+        b.id = node;
+        dBody[wave][nd + (int)__popcll(am & ltMask)] = b;
       }
-      const PosT cx = rl(r.comx, l), cy = rl(r.comy, l);
-      const Real twoh = rl(r.twoh, l), mass = rl(r.mass, l);
-      bool open = false;
-      if ((m >> lane) & 1ull) {
-        const PosT ex = cx - me.x, ey = cy - me.y;
-        Real dx = (Real)ex, dy = (Real)ey;
-        Real dist = Phys<Real>::dist(dx, dy);
-        if (Phys<Real>::accept(twoh, dist, theta)) {
-          Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, mr.y + (Real)0, mr.x,
-                                                mass, kSq, oc, selfId, n, ax, ay,
-                                                ex, ey, r64, n);
-          cnt++;
-        } else {
-          open = true;
+      for (int k = 0; am; k++) {
+        const int l = __builtin_ctzll(am);
+        am &= am - 1;
+        const TravBody<Real, PosT> b = dBody[wave][nd + k];
+        if (tstats) {
+          st[3]++;
+          st[4] += __popcll(rl(b.mask, 0));
         }
-      }
-      const uint64_t om = __ballot(open);
-      if (om) {
-        if (sp + 4 > kTravStack) {
-          sc->traversalOverflow = 1;
-          sp = 0;
-          am = 0;
-          break;
+        bool open = false;
+        if ((b.mask >> lane) & 1ull) {
+          const PosT ex = b.x - me.x, ey = b.y - me.y;
+          Real dx = (Real)ex, dy = (Real)ey;
+          Real inv;
+          Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+          if (Phys<Real>::accept(b.r, dist, theta)) {
+            Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + (Real)0, mr.x,
+                                                  b.m, kSq, oc, selfId, n, ax, ay, ex, ey,
+                                                  r64, n);
+            cnt++;
+          } else {
+            open = true;
+          }
         }
+        const uint64_t om = __ballot(open);
+        if (om) {
+          if (sp + 4 > kTravStack) {
+            sc->traversalOverflow = 1;
+            sp = 0;
+            break;
+          }
 #pragma unroll
-        for (int c = 3; c >= 0; c--) {
-          const int32_t ch = rl(r.child[c], l);
-          if (ch != kNone) {
-            if (lane == 0) {
-              sNode[wave][sp] = ch;
-              sMask[wave][sp] = om;
+          for (int c = 3; c >= 0; c--) {
+            const int32_t ch = rl(r.child[c], l);
+            if (ch != kNone) {
+              if (lane == 0) {
+                sNode[wave][sp] = ch;
+                sMask[wave][sp] = om;
+              }
+              sp++;
             }
-            sp++;
           }
         }
       }
@@ -702,8 +729,9 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
           continue;
         const PosT ex = tPos[t].x - me.x, ey = tPos[t].y - me.y;
         Real dx = (Real)ex, dy = (Real)ey;
-        Real dist = Phys<Real>::dist(dx, dy);
-        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, rv + tRad[t], mv,
+        Real inv;
+        Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, rv + tRad[t], mv,
                                               tMass[t], kSq, oc, i, jj, ax, ay, ex,
                                               ey, r64, n);
       }
