@@ -205,8 +205,7 @@ int gvizhip_batch_create(gvizhip_batch **out, int64_t graphs, const int64_t *ver
   c.prm = *(const ForceParams *)cfg;
   bs.graphs = graphs;
   bt->start.assign(vertexStart, vertexStart + graphs + 1);
-  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
-      c.alloc(n, m2)) {
+  if (c.open_streams() || c.alloc(n, m2)) {
     if (!gvizhip_last_error()[0])
       set_error("gvizhip_batch_create: allocation failed");
     gvizhip_batch_destroy(bt);
@@ -543,10 +542,7 @@ void gvizhip_batch_destroy(gvizhip_batch *bt) {
     (void)hipEventDestroy(bt->c.travBegin);
   if (bt->c.travEnd)
     (void)hipEventDestroy(bt->c.travEnd);
-  if (bt->c.stream) {
-    (void)hipStreamSynchronize(bt->c.stream);
-    (void)hipStreamDestroy(bt->c.stream);
-  }
+  bt->c.close_streams();
   delete bt;
 }
 

@@ -221,7 +221,7 @@ int gvizhip_force_create(gvizhip_force **out, int64_t n, const int64_t *offsets,
   gvizhip_force *f = new gvizhip_force();
   ForceCommon &c = f->c;
   c.prm = *(const ForceParams *)cfg;
-  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+  if (c.open_streams() ||
       hipHostMalloc((void **)&f->pinned, sizeof(Scalars), hipHostMallocDefault) !=
           hipSuccess) {
     set_error("gvizhip_force_create: stream or pinned allocation failed");
@@ -812,11 +812,8 @@ extern "C" void gvizhip_force_destroy(gvizhip_force *f) {
     (void)hipEventDestroy(f->c.travBegin);
   if (f->c.travEnd)
     (void)hipEventDestroy(f->c.travEnd);
-  if (f->c.stream)
-    (void)hipStreamSynchronize(f->c.stream);
+  f->c.close_streams();
   if (f->pinned)
     (void)hipHostFree(f->pinned);
-  if (f->c.stream)
-    (void)hipStreamDestroy(f->c.stream);
   delete f;
 }

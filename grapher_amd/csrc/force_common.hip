@@ -510,6 +510,35 @@ int ForceCommon::alloc(int64_t nv, int64_t ne) {
   return 0;
 }
 
+int ForceCommon::open_streams() {
+  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&evJoin, hipEventDisableTiming) != hipSuccess) {
+    set_error("stream or event creation failed");
+    return -1;
+  }
+  return 0;
+}
+
+void ForceCommon::close_streams() {
+  if (side) {
+    (void)hipStreamSynchronize(side);
+    (void)hipStreamDestroy(side);
+    side = nullptr;
+  }
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+  if (evFork)
+    (void)hipEventDestroy(evFork);
+  if (evJoin)
+    (void)hipEventDestroy(evJoin);
+  evFork = evJoin = nullptr;
+}
+
 // Bounding box, keys, sort and radix tree (everything before the typed
 // gather). Typed gather + prefix sums + quad records follow in the typed TU.
 int enqueue_keys_and_sort(ForceCommon &c) {

@@ -122,7 +122,11 @@ struct ForceCommon {
   int wantInteractions = 0;
   DevBuf<int32_t> heavyRows;            // vertices with degree > kHeavyDegree
   int64_t nHeavy = 0;
-  DevBuf<unsigned char> attHeavy;       // their attraction (typed, n entries)
+  DevBuf<unsigned char> attHeavy;       // attraction of every vertex (typed, n)
+  // attraction runs on a side stream, forked after the positions are ready
+  // and joined before the force sums, so it overlaps the tree and traversal
+  hipStream_t side = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   // measurement hook: events around the traversal kernel (never captured)
   hipEvent_t travBegin = nullptr, travEnd = nullptr;
   int timeTraversal = 0;
@@ -131,6 +135,8 @@ struct ForceCommon {
   int reductionBlocks = 0;
 
   int alloc(int64_t nVertices, int64_t nEntries);
+  int open_streams();   // main stream, side stream and fork/join events
+  void close_streams(); // synchronizes and releases them
 };
 
 // A batch of independent graphs laid out together (graph g owns compact

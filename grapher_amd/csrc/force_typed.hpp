@@ -796,6 +796,29 @@ __global__ __launch_bounds__(kTB) void k_att_heavy(
 // Attraction along the adjacency in insertion order, then the per-vertex
 // epilogue of src/embedders/gvizForceEmbedder.c:117-124 and the swinging /
 // traction of :128-138; per-block FP64 partial sums of both go to partial.
+// Attraction of the vertices with at most kHeavyDegree neighbours, one
+// thread each, summed in adjacency order (k_att_heavy takes the others).
+template <class Real, class PosT, int MODEL>
+__global__ __launch_bounds__(kTB) void k_att_light(
+    const typename V2<PosT>::type *__restrict__ pos, const int32_t *__restrict__ off,
+    const int32_t *__restrict__ adj, int64_t n, Real edgeLength,
+    typename V2<Real>::type *__restrict__ out, const Scalars *__restrict__ sc) {
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i >= n || sc->stopped)
+    return;
+  const int32_t e0 = off[i], e1 = off[i + 1];
+  if (e1 - e0 > kHeavyDegree)
+    return;
+  const typename V2<PosT>::type v = pos[i];
+  Real ax = 0, ay = 0;
+  for (int32_t e = e0; e < e1; e++) {
+    const typename V2<PosT>::type u = pos[adj[e]];
+    Phys<Real>::template att<MODEL>((Real)(u.x - v.x), (Real)(u.y - v.y), edgeLength, ax,
+                                    ay);
+  }
+  out[i] = {ax, ay};
+}
+
 template <class Real, class PosT, int MODEL>
 __global__ __launch_bounds__(kTB) void k_forces(
     const typename V2<PosT>::type *__restrict__ pos,
@@ -805,7 +828,7 @@ __global__ __launch_bounds__(kTB) void k_forces(
     double gravityK, typename V2<Real>::type *__restrict__ sf,
     typename V2<Real>::type *__restrict__ disp, Real *__restrict__ swOut,
     Real *__restrict__ attMagOut, Real *__restrict__ repMagOut,
-    double *__restrict__ partial, const typename V2<Real>::type *__restrict__ heavy,
+    double *__restrict__ partial, const typename V2<Real>::type *__restrict__ att,
     const Scalars *__restrict__ sc, SegView sv) {
   __shared__ double smem[2 * 16];
   int64_t start = 0, len = n, part = 0;
@@ -825,19 +848,8 @@ __global__ __launch_bounds__(kTB) void k_forces(
   for (int64_t li = blk * kTB + threadIdx.x; li < len; li += nblk * kTB) {
     const int64_t i = start + li;
     typename V2<PosT>::type v = pos[i];
-    Real ax = 0, ay = 0;
-    const int32_t e0 = off[i], e1 = off[i + 1];
-    if (e1 - e0 > kHeavyDegree) {
-      typename V2<Real>::type h = heavy[i];
-      ax = h.x;
-      ay = h.y;
-    } else {
-      for (int32_t e = e0; e < e1; e++) {
-        typename V2<PosT>::type u = pos[adj[e]];
-        Phys<Real>::template att<MODEL>((Real)(u.x - v.x), (Real)(u.y - v.y),
-                                        edgeLength, ax, ay);
-      }
-    }
+    const typename V2<Real>::type a = att[i];
+    const Real ax = a.x, ay = a.y;
     typename V2<Real>::type rp = rep[i];
     Real fx = (Real)0, fy = (Real)0;
     fx += (Real)1 * ax;
@@ -935,6 +947,37 @@ int enqueue_radix_tree(ForceCommon &c);
 int enqueue_fa2(ForceCommon &c, int blocks);
 int enqueue_stats(ForceCommon &c, int blocks);
 
+// Attraction of every vertex into c.attHeavy on the side stream: forked from
+// the main stream once the positions are final, joined before the force sums.
+template <class Real, class PosT>
+int fork_attraction(ForceCommon &c, const typename V2<PosT>::type *posC, int model) {
+  using R2 = typename V2<Real>::type;
+  hipStream_t s = c.side;
+  const int64_t n = c.n;
+  GVH_CHECK(hipEventRecord(c.evFork, c.stream));
+  GVH_CHECK(hipStreamWaitEvent(s, c.evFork, 0));
+  const unsigned hb = (unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048);
+  const Real el = (Real)c.prm.edgeLength;
+  if (model == 1) {
+    if (c.nHeavy > 0)
+      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 1>), dim3(hb), dim3(kTB), 0, s, posC,
+                         c.csrOff.p, c.csrAdj.p, c.heavyRows.p, c.nHeavy, el,
+                         (R2 *)c.attHeavy.p, c.sc.p);
+    hipLaunchKernelGGL((k_att_light<Real, PosT, 1>), dim3(grid_for(n, kTB)), dim3(kTB), 0,
+                       s, posC, c.csrOff.p, c.csrAdj.p, n, el, (R2 *)c.attHeavy.p, c.sc.p);
+  } else {
+    if (c.nHeavy > 0)
+      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 0>), dim3(hb), dim3(kTB), 0, s, posC,
+                         c.csrOff.p, c.csrAdj.p, c.heavyRows.p, c.nHeavy, el,
+                         (R2 *)c.attHeavy.p, c.sc.p);
+    hipLaunchKernelGGL((k_att_light<Real, PosT, 0>), dim3(grid_for(n, kTB)), dim3(kTB), 0,
+                       s, posC, c.csrOff.p, c.csrAdj.p, n, el, (R2 *)c.attHeavy.p, c.sc.p);
+  }
+  GVH_CHECK_LAUNCH();
+  GVH_CHECK(hipEventRecord(c.evJoin, s));
+  return 0;
+}
+
 template <class Real, class PosT>
 int typed_step(ForceCommon &c, int mode) {
   using P2 = typename V2<PosT>::type;
@@ -956,6 +999,8 @@ int typed_step(ForceCommon &c, int mode) {
     posC = (const P2 *)c.posT.p;
   }
   const int model = c.prm.model == 1 ? 1 : 0;
+  if (mode != kStepTreeOnly && fork_attraction<Real, PosT>(c, posC, model))
+    return -1;
   if (c.prm.barnesHut || mode == kStepTreeOnly) {
     if (enqueue_keys_and_sort(c))
       return -1;
@@ -1016,13 +1061,9 @@ int typed_step(ForceCommon &c, int mode) {
   const unsigned rb = grid_for(n, kTB) < (unsigned)c.reductionBlocks
                           ? grid_for(n, kTB)
                           : (unsigned)c.reductionBlocks;
+  GVH_CHECK(hipStreamWaitEvent(st, c.evJoin, 0));
   if (model == 1)
   {
-    if (c.nHeavy > 0)
-      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 1>),
-                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
-                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
-                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
     hipLaunchKernelGGL((k_forces<Real, PosT, 1>), dim3(rb), dim3(kTB), 0, st,
                        posC, c.csrOff.p, c.csrAdj.p, c.degree.p,
                        (const Real *)c.massT.p, (const R2 *)c.rep.p, n,
@@ -1033,11 +1074,6 @@ int typed_step(ForceCommon &c, int mode) {
   }
   else
   {
-    if (c.nHeavy > 0)
-      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 0>),
-                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
-                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
-                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
     hipLaunchKernelGGL((k_forces<Real, PosT, 0>), dim3(rb), dim3(kTB), 0, st,
                        posC, c.csrOff.p, c.csrAdj.p, c.degree.p,
                        (const Real *)c.massT.p, (const R2 *)c.rep.p, n,
@@ -1155,6 +1191,8 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
     posC = (const P2 *)c.posT.p;
   }
   const int model = c.prm.model == 1 ? 1 : 0;
+  if (fork_attraction<Real, PosT>(c, posC, model))
+    return -1;
   if (enqueue_batch_keys_and_sort(c, bs))
     return -1;
   hipLaunchKernelGGL((k_gather<Real, PosT>), dim3(grid_for(n, kTB)), dim3(kTB), 0, st,
@@ -1192,12 +1230,8 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travEnd, st));
   const unsigned vb = (unsigned)bs.vblocks;
+  GVH_CHECK(hipStreamWaitEvent(st, c.evJoin, 0));
   if (model == 1) {
-    if (c.nHeavy > 0)
-      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 1>),
-                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
-                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
-                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
     hipLaunchKernelGGL((k_forces<Real, PosT, 1>), dim3(vb), dim3(kTB), 0, st, posC,
                        c.csrOff.p, c.csrAdj.p, c.degree.p, (const Real *)c.massT.p,
                        (const R2 *)c.rep.p, n, (Real)c.prm.edgeLength, c.prm.gravityK,
@@ -1205,11 +1239,6 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
                        (Real *)c.repMag.p, bs.partials.p, (const R2 *)c.attHeavy.p,
                        bs.scG.p, sv);
   } else {
-    if (c.nHeavy > 0)
-      hipLaunchKernelGGL((k_att_heavy<Real, PosT, 0>),
-                         dim3((unsigned)std::min<int64_t>((c.nHeavy + kTW - 1) / kTW, 2048)),
-                         dim3(kTB), 0, st, posC, c.csrOff.p, c.csrAdj.p, c.heavyRows.p,
-                         c.nHeavy, (Real)c.prm.edgeLength, (R2 *)c.attHeavy.p, c.sc.p);
     hipLaunchKernelGGL((k_forces<Real, PosT, 0>), dim3(vb), dim3(kTB), 0, st, posC,
                        c.csrOff.p, c.csrAdj.p, c.degree.p, (const Real *)c.massT.p,
                        (const R2 *)c.rep.p, n, (Real)c.prm.edgeLength, c.prm.gravityK,
