@@ -228,10 +228,9 @@ __device__ __forceinline__ Acc acc_add(const Acc &a, const Acc &b) {
 }
 
 __device__ __forceinline__ Acc load_elem(const double *__restrict__ spos64,
-                                         const double *__restrict__ mass64,
-                                         const uint32_t *__restrict__ vals,
+                                         const double *__restrict__ smass64,
                                          int64_t s) {
-  double m = mass64[vals[s]];
+  double m = smass64[s];
   return {m, two_prod(m, spos64[2 * s]), two_prod(m, spos64[2 * s + 1])};
 }
 
@@ -253,8 +252,8 @@ __device__ Acc block_inclusive_scan(Acc v, Acc *sm) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(
-    const double *__restrict__ spos64, const double *__restrict__ mass64,
-    const uint32_t *__restrict__ vals, int64_t n, double *__restrict__ part,
+    const double *__restrict__ spos64, const double *__restrict__ smass64,
+    int64_t n, double *__restrict__ part,
     const Scalars *__restrict__ sc) {
   __shared__ Acc sm[kBlock];
   if (sc->stopped)
@@ -263,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(
   Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
   for (int k = 0; k < kScanItems; k++)
     if (base + k < n)
-      a = acc_add(a, load_elem(spos64, mass64, vals, base + k));
+      a = acc_add(a, load_elem(spos64, smass64, base + k));
   a = block_inclusive_scan(a, sm);
   if (threadIdx.x == kBlock - 1) {
     double *p = part + 5 * blockIdx.x;
@@ -316,8 +315,8 @@ __global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ par
 
 // prefix[s + 1] = sum over sorted points 0..s; prefix[0] = 0.
 __global__ __launch_bounds__(kBlock) void k_scan_final(
-    const double *__restrict__ spos64, const double *__restrict__ mass64,
-    const uint32_t *__restrict__ vals, int64_t n, const double *__restrict__ part,
+    const double *__restrict__ spos64, const double *__restrict__ smass64,
+    int64_t n, const double *__restrict__ part,
     double *__restrict__ pm, double *__restrict__ pxh, double *__restrict__ pxl,
     double *__restrict__ pyh, double *__restrict__ pyl,
     const Scalars *__restrict__ sc) {
@@ -329,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(
   Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
   for (int k = 0; k < kScanItems; k++) {
     if (base + k < n)
-      a = acc_add(a, load_elem(spos64, mass64, vals, base + k));
+      a = acc_add(a, load_elem(spos64, smass64, base + k));
     loc[k] = a;
   }
   Acc incl = block_inclusive_scan(a, sm);
@@ -481,7 +480,8 @@ int ForceCommon::alloc(int64_t nv, int64_t ne) {
   m2 = ne;
   const int64_t n1 = nv > 0 ? nv : 1;
   if (pos64.alloc(2 * n1) || csrOff.alloc(n1 + 1) || csrAdj.alloc(ne > 0 ? ne : 1) ||
-      degree.alloc(n1) || mass64.alloc(n1) || radius64.alloc(n1))
+      degree.alloc(n1) || mass64.alloc(n1) || radius64.alloc(n1) || mr64.alloc(2 * n1) ||
+      smass64.alloc(n1))
     return -1;
   if (keyHi.alloc(n1) || keyLo.alloc(n1) || keyLoS.alloc(n1) || vals.alloc(n1) ||
       sorter.reserve(n1))
@@ -603,12 +603,12 @@ int enqueue_radix_tree(ForceCommon &c) {
                        c.parent.p, c.first.p, c.last.p, c.delta.p, c.sc.p);
   const int tiles = (int)((n + kScanTile - 1) / kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(tiles > 0 ? tiles : 1), dim3(kBlock), 0,
-                     st, c.spos64.p, c.mass64.p, c.vals.p, n, c.scanPartial.p,
+                     st, c.spos64.p, c.smass64.p, n, c.scanPartial.p,
                      c.sc.p);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, c.scanPartial.p,
                      tiles, c.sc.p);
   hipLaunchKernelGGL(k_scan_final, dim3(tiles > 0 ? tiles : 1), dim3(kBlock), 0,
-                     st, c.spos64.p, c.mass64.p, c.vals.p, n, c.scanPartial.p,
+                     st, c.spos64.p, c.smass64.p, n, c.scanPartial.p,
                      c.pm.p, c.pmxHi.p, c.pmxLo.p, c.pmyHi.p, c.pmyLo.p, c.sc.p);
   GVH_CHECK_LAUNCH();
   return 0;

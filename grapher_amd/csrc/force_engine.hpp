@@ -100,6 +100,8 @@ struct ForceCommon {
   DevBuf<int32_t> degree;     // n
   DevBuf<double> mass64;      // n  (exact: 1 or 1+deg)
   DevBuf<double> radius64;    // n  (0 when overlap prevention is off)
+  DevBuf<double> mr64;        // 2n (mass, radius) pairs, one load per vertex
+  DevBuf<double> smass64;     // n  masses in key order (prefix sums)
 
   DevBuf<uint64_t> keyHi, keyLo, keyLoS;
   DevBuf<uint32_t> vals;

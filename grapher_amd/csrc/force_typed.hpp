@@ -224,8 +224,8 @@ template <> struct Phys<float> {
 // (mass, radius) for the traversal, and the low key word for the tree.
 template <class Real, class PosT>
 __global__ __launch_bounds__(kTB) void k_gather(
-    const double2 *__restrict__ pos64, const double *__restrict__ mass64,
-    const double *__restrict__ radius64, const uint32_t *__restrict__ vals,
+    const double2 *__restrict__ pos64, const double2 *__restrict__ mr64,
+    double *__restrict__ smass64, const uint32_t *__restrict__ vals,
     const uint64_t *__restrict__ keyLo, int64_t n, double2 *__restrict__ spos64,
     typename V2<PosT>::type *__restrict__ spos,
     typename V2<Real>::type *__restrict__ smr, uint64_t *__restrict__ keyLoS,
@@ -237,7 +237,9 @@ __global__ __launch_bounds__(kTB) void k_gather(
   double2 p = pos64[i];
   spos64[s] = p;
   spos[s] = {(PosT)p.x, (PosT)p.y};
-  smr[s] = {(Real)mass64[i], (Real)radius64[i]};
+  const double2 mr = mr64[i];
+  smr[s] = {(Real)mr.x, (Real)mr.y};
+  smass64[s] = mr.x;
   keyLoS[s] = keyLo[i];
 }
 
@@ -933,12 +935,14 @@ template <class Real>
 __global__ __launch_bounds__(kTB) void k_typed_mr(const double *__restrict__ m64,
                                                   const double *__restrict__ r64,
                                                   int64_t n, Real *__restrict__ m,
-                                                  Real *__restrict__ r) {
+                                                  Real *__restrict__ r,
+                                                  double2 *__restrict__ mr) {
   int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (i >= n)
     return;
   m[i] = (Real)m64[i];
   r[i] = (Real)r64[i];
+  mr[i] = {m64[i], r64[i]};
 }
 
 // ------------------------------------------------------------ host glue
@@ -999,14 +1003,12 @@ int typed_step(ForceCommon &c, int mode) {
     posC = (const P2 *)c.posT.p;
   }
   const int model = c.prm.model == 1 ? 1 : 0;
-  if (mode != kStepTreeOnly && fork_attraction<Real, PosT>(c, posC, model))
-    return -1;
   if (c.prm.barnesHut || mode == kStepTreeOnly) {
     if (enqueue_keys_and_sort(c))
       return -1;
     hipLaunchKernelGGL((k_gather<Real, PosT>), dim3(grid_for(n, kTB)), dim3(kTB),
-                       0, st, (const double2 *)c.pos64.p, c.mass64.p,
-                       c.radius64.p, c.vals.p, c.keyLo.p, n,
+                       0, st, (const double2 *)c.pos64.p, (const double2 *)c.mr64.p,
+                       c.smass64.p, c.vals.p, c.keyLo.p, n,
                        (double2 *)c.spos64.p, (P2 *)c.spos.p, (R2 *)c.smr.p,
                        c.keyLoS.p, c.sc.p);
     GVH_CHECK_LAUNCH();
@@ -1023,6 +1025,9 @@ int typed_step(ForceCommon &c, int mode) {
       GVH_CHECK_LAUNCH();
       return 0;
     }
+    // attraction (memory-bound) overlaps the traversal (ALU-bound)
+    if (fork_attraction<Real, PosT>(c, posC, model))
+      return -1;
     const unsigned tb = (unsigned)((n + kTravThreads - 1) / kTravThreads);
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
@@ -1042,6 +1047,8 @@ int typed_step(ForceCommon &c, int mode) {
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
   } else {
+    if (fork_attraction<Real, PosT>(c, posC, model))
+      return -1;
     const unsigned tb = (unsigned)((n + kTB - 1) / kTB);
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
@@ -1191,12 +1198,11 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
     posC = (const P2 *)c.posT.p;
   }
   const int model = c.prm.model == 1 ? 1 : 0;
-  if (fork_attraction<Real, PosT>(c, posC, model))
-    return -1;
   if (enqueue_batch_keys_and_sort(c, bs))
     return -1;
   hipLaunchKernelGGL((k_gather<Real, PosT>), dim3(grid_for(n, kTB)), dim3(kTB), 0, st,
-                     (const double2 *)c.pos64.p, c.mass64.p, c.radius64.p, c.vals.p,
+                     (const double2 *)c.pos64.p, (const double2 *)c.mr64.p, c.smass64.p,
+                     c.vals.p,
                      c.keyLo.p, n, (double2 *)c.spos64.p, (P2 *)c.spos.p,
                      (typename V2<Real>::type *)c.smr.p, c.keyLoS.p, c.sc.p);
   GVH_CHECK_LAUNCH();
@@ -1212,6 +1218,9 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
                        (NodeRec<Real, PosT> *)c.nodes.p, bs.gid.p, bs.scG.p,
                        bs.rootOf.p);
   GVH_CHECK_LAUNCH();
+  // attraction (memory-bound) overlaps the traversal (ALU-bound)
+  if (fork_attraction<Real, PosT>(c, posC, model))
+    return -1;
   const unsigned tb = (unsigned)((bs.waves + kTravWaves - 1) / kTravWaves);
   int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
   if (c.timeTraversal)
@@ -1263,7 +1272,7 @@ int typed_upload(ForceCommon &c) {
     return 0;
   hipLaunchKernelGGL(k_typed_mr<Real>, dim3(grid_for(c.n, kTB)), dim3(kTB), 0,
                      c.stream, c.mass64.p, c.radius64.p, c.n, (Real *)c.massT.p,
-                     (Real *)c.radT.p);
+                     (Real *)c.radT.p, (double2 *)c.mr64.p);
   GVH_CHECK_LAUNCH();
   return 0;
 }
