@@ -38,6 +38,18 @@ static __device__ __noinline__ double2 coincident_dir(int64_t selfId, int64_t ot
   return double2{cos(angle), sin(angle)};
 }
 
+// The same direction for the FP32 force math, inline: the hardware cosine
+// and sine take the angle in revolutions (|error| ~1e-6, far below FP32
+// force tolerances), so no library call sits in the force loops.
+__device__ __forceinline__ float2 coincident_dir_f(int64_t selfId, int64_t otherId) {
+  uint64_t bits = (uint64_t)selfId ^ ((uint64_t)otherId * 0x9E3779B97F4A7C15ull);
+  bits ^= bits >> 33;
+  bits *= 0xff51afd7ed558ccdull;
+  bits ^= bits >> 33;
+  const float rev = (float)(uint32_t)(bits % 1000000u) * 1e-6f;
+  return float2{__builtin_amdgcn_cosf(rev), __builtin_amdgcn_sinf(rev)};
+}
+
 // ------------------------------------------------------- pairwise physics
 // MODEL 0 = Fruchterman-Reingold, 1 = LinLog. `mOther` is the other body's
 // mass (a pseudo-body's aggregated mass); FR scales by its integer part
@@ -148,19 +160,17 @@ template <> struct Phys<float> {
       double rs = r64[selfId] + (otherId < n ? r64[otherId] : 0.0);
       gap = (float)(sqrt(dxd * dxd + dyd * dyd) - rs);
     }
-    float ux, uy;
-    if (dist > 0.0f) {
-      ux = dx * inv;
-      uy = dy * inv;
-    } else {
-      const double2 u = coincident_dir(selfId, otherId);
-      ux = (float)u.x;
-      uy = (float)u.y;
-    }
     float num = MODEL == 1 ? mSelf * mOther : kSq * truncf(mOther);
     float mag = gap > 0.0f ? num * __builtin_amdgcn_rcpf(gap) : num * oc;
-    ax -= mag * ux;
-    ay -= mag * uy;
+    if (dist > 0.0f) {
+      const float s = mag * inv; // mag times the unit vector (dx, dy) / dist
+      ax -= s * dx;
+      ay -= s * dy;
+    } else {
+      const float2 u = coincident_dir_f(selfId, otherId);
+      ax -= mag * u.x;
+      ay -= mag * u.y;
+    }
   }
   __device__ __forceinline__ static float dist(float dx, float dy) {
     float d2 = dx * dx + dy * dy;
@@ -338,7 +348,7 @@ __device__ __forceinline__ double rl(double v, int l) {
 #define GVH_TRAV_STACK 384
 #endif
 #ifndef GVH_TRAV_DEFER
-#define GVH_TRAV_DEFER 96
+#define GVH_TRAV_DEFER 72
 #endif
 #ifndef GVH_TRAV_MINBLOCKS
 #define GVH_TRAV_MINBLOCKS 1
