@@ -407,6 +407,20 @@ static void write_series(gvizEmbeddedGraph *e, const char *name) {
       wr_d(s->samples[i]);
 }
 
+/* Multiplies every coordinate by 1 + amount * u, u uniform in [-1, 1) from a
+ * 64-bit LCG: the same start up to rounding-size noise, to measure how far a
+ * run's chaotic FA2 dynamics carry such differences. amount 0: no change. */
+static void perturb_positions(double *p, int64_t count, double amount, int64_t seed) {
+  if (!(amount > 0.0))
+    return;
+  uint64_t st = (uint64_t)seed * 2654435761u + 1u;
+  for (int64_t i = 0; i < count; i++) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    const double u = (double)(st >> 11) * 0x1p-53 * 2.0 - 1.0;
+    p[i] *= 1.0 + amount * u;
+  }
+}
+
 /* Force embedder through its public API on a vertex-induced subgraph:
  * Init, Configure*, Begin(seed) (the reference's own random placement),
  * Run(maxIters, epsilon), then one more Step. */
@@ -425,6 +439,8 @@ static int mode_api_force(void) {
   double edgeLength = rd_d(), theta = rd_d(), gravityK = rd_d();
   double overlapC = rd_d(), rBase = rd_d(), rPerDeg = rd_d(), jitter = rd_d();
   double eps = rd_d();
+  double perturb = rd_d();
+  int64_t perturbSeed = rd_i();
   gvizForceEmbedderState s;
   if (gvizForceEmbedderInit(&s, gvizSubgraphCreateVertexInduced(&g, vs), 2,
                             (gvizForceModelKind)model) != 0)
@@ -443,6 +459,7 @@ static int mode_api_force(void) {
   const double *p = gvizEmbeddedGraphPositions(e);
   for (int64_t i = 0; i < 2 * n; i++)
     wr_d(p[i]);
+  perturb_positions(e->embedding.vertexPositions, 2 * n, perturb, perturbSeed);
   wr_i(gvizForceEmbedderRun(&s, (size_t)runIters, eps));
   static const char *const names[] = {
       "forceEmbedder.maxDisp", "forceEmbedder.speed",

@@ -17,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "..", ".."))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
 
 import refio  # noqa: E402
 from grapher_amd import graphs as G  # noqa: E402
@@ -157,12 +158,47 @@ def grip_chain_cases():
     grip_case("grip_grid16_k4", n, e, 16, 4, kmax=4)
 
 
+def layout_quality(n, e, hidden, pos, edge_length):
+    """Stress and edge energy of the layout of the subgraph induced by every
+    vertex but `hidden` (compact ids in ascending parent id)."""
+    import pyoracle as O
+    keep = np.ones(n, bool)
+    keep[hidden] = False
+    ids = np.flatnonzero(keep)
+    index = -np.ones(n, np.int64)
+    index[ids] = np.arange(ids.size)
+    ee = e[keep[e[:, 0]] & keep[e[:, 1]]]
+    off, adj = G.csr_from_edges(ids.size, index[ee])
+    g = O.Graph(off, adj)
+    pool = O.Pool(4)
+    return O.stress(g, pos[ids], edge_length, None, pool), O.edge_energy(g, pos[ids])
+
+
+SPREAD_RUNS = 8
+SPREAD_NOISE = 1e-12
+
+
 def api_force_case(name, n, e, hidden, seed, run_iters, epsilon, **kw):
+    """The reference's run, plus SPREAD_RUNS runs of the reference from the
+    same start with every coordinate scaled by 1 +- 1e-12: the spread of
+    their final stress / edge energy / run length is how far rounding-size
+    differences carry in this chaotic FA2 run, the yardstick for any other
+    summation order (the engine's included)."""
     r = refio.api_force(n, e, hidden, seed=seed, run_iters=run_iters,
                         epsilon=epsilon, **kw)
     params = {"param_" + k: v for k, v in kw.items()}
+    st, en, rc = [], [], []
+    for k in range(SPREAD_RUNS):
+        q = refio.api_force(n, e, hidden, seed=seed, run_iters=run_iters,
+                            epsilon=epsilon, perturb=SPREAD_NOISE, perturb_seed=k + 1, **kw)
+        a, b = layout_quality(n, e, hidden, q["pos_step"], kw["edge_length"])
+        st.append(a)
+        en.append(b)
+        rc.append(q["run_rc"])
     save(name, n=n, edges=e, hidden=hidden, seed=seed, run_iters=run_iters,
-         epsilon=epsilon, **params, **r)
+         epsilon=epsilon, **params, **r, spread_stress=np.array(st),
+         spread_energy=np.array(en), spread_run_rc=np.array(rc, np.int64),
+         spread_noise=SPREAD_NOISE)
 
 
 def api_cases():

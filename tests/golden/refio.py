@@ -214,16 +214,19 @@ def grip(n, edges, kcap, kmax, rounds_per_layer):
 
 def api_force(n, edges, hidden, *, model, barnes_hut, prevent_overlap, npc, seed,
               run_iters, edge_length, theta, gravity, overlap_constant,
-              radius_base, radius_per_degree, jitter, epsilon, exe=None):
+              radius_base, radius_per_degree, jitter, epsilon, perturb=0.0,
+              perturb_seed=0, exe=None):
     """gvizForceEmbedderInit/Configure*/Begin(seed)/Run/Step on the subgraph
-    induced by every vertex but `hidden`."""
+    induced by every vertex but `hidden`; `perturb` > 0 multiplies each
+    coordinate after Begin by 1 + perturb * u (u uniform in [-1, 1), seeded)."""
     w = _W()
     w.graph(n, edges)
     w.i(len(hidden))
     w.ia(hidden)
     w.i(model, barnes_hut, prevent_overlap, npc, seed, run_iters)
     w.d(edge_length, theta, gravity, overlap_constant, radius_base,
-        radius_per_degree, jitter, epsilon)
+        radius_per_degree, jitter, epsilon, perturb)
+    w.i(perturb_seed)
     r = _R(_run("api_force", w.bytes(), exe))
     out = dict(box_extent=r.d(), begin_rc=r.i(), pos_begin=r.da(2 * n).reshape(n, 2))
     out["run_rc"] = r.i()

@@ -77,6 +77,18 @@ static int fail_device(const char *what) {
   return 1;
 }
 
+/* Same noise as the reference-side harness (coordinate *= 1 + amount * u). */
+static void perturb_positions(double *p, int64_t count, double amount, int64_t seed) {
+  if (!(amount > 0.0))
+    return;
+  uint64_t st = (uint64_t)seed * 2654435761u + 1u;
+  for (int64_t i = 0; i < count; i++) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    const double u = (double)(st >> 11) * 0x1p-53 * 2.0 - 1.0;
+    p[i] *= 1.0 + amount * u;
+  }
+}
+
 static int mode_api_force(void) {
   gvizGraph g;
   int64_t n;
@@ -92,6 +104,8 @@ static int mode_api_force(void) {
   double edgeLength = rd_d(), theta = rd_d(), gravityK = rd_d();
   double overlapC = rd_d(), rBase = rd_d(), rPerDeg = rd_d(), jitter = rd_d();
   double eps = rd_d();
+  double perturb = rd_d();
+  int64_t perturbSeed = rd_i();
   gvizForceEmbedderState s;
   if (gvizForceEmbedderInit(&s, gvizSubgraphCreateVertexInduced(&g, vs), 2,
                             (gvizForceModelKind)model) != 0)
@@ -115,6 +129,7 @@ static int mode_api_force(void) {
   const double *p = gvizEmbeddedGraphPositions(e);
   for (int64_t i = 0; i < 2 * n; i++)
     wr_d(p[i]);
+  perturb_positions(e->embedding.vertexPositions, 2 * n, perturb, perturbSeed);
   gvizForceEmbedderSetResidentPositions(&s, resident);
   int rounds = gvizForceEmbedderRun(&s, (size_t)runIters, eps);
   if (rounds < 0)

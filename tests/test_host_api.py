@@ -133,7 +133,10 @@ def test_force_api_fp64_matches_reference(gpu, name):
     each step's forces agree with the reference to ~1e-11 (the Barnes-Hut
     sums run in another order), so the first steps agree to 1e-8 and the
     chaotic FA2 dynamics then amplify the difference: the run is checked for
-    its length and the final layout's stress and edge energy."""
+    its length and the final layout's stress and edge energy against the
+    band the reference itself spans when its start is scaled by 1 +- 1e-12
+    (eight such reference runs are stored in the fixture), widened by 2 %
+    (3 % or 2 steps for the run length)."""
     f = golden(name)
     print(f"host api force {name}", flush=True)
     r = _api_force(f, {"GVIZ_TEST_PRECISION": "fp64", "GVIZ_TEST_RESIDENT": "0"})
@@ -147,16 +150,24 @@ def test_force_api_fp64_matches_reference(gpu, name):
         np.testing.assert_allclose(r["series_" + s][:8], f["series_" + s][:8],
                                    rtol=1e-8, atol=1e-12, err_msg=s)
     ref_rc = int(f["run_rc"])
-    assert abs(r["run_rc"] - ref_rc) <= max(2, 0.03 * ref_rc)
+    rcs = np.append(f["spread_run_rc"], ref_rc)
+    slack = max(2, 0.03 * ref_rc)
+    print(f"  run length {r['run_rc']} vs {ref_rc} (perturbed reference runs "
+          f"{rcs.min()}..{rcs.max()})")
+    assert rcs.min() - slack <= r["run_rc"] <= rcs.max() + slack
     act = _active(f)
     assert np.all(r["pos_run"][~act] == 0.0)
     assert r["iteration"] == r["run_rc"] + 1
     g, ids = _compact(f)
     s_ref, e_ref = _layout_quality(g, f["pos_step"][ids])
     s_gpu, e_gpu = _layout_quality(g, r["pos_step"][ids])
-    print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g}; energy {e_gpu:.6g} vs {e_ref:.6g}")
-    assert abs(s_gpu - s_ref) <= 0.02 * abs(s_ref)
-    assert abs(e_gpu - e_ref) <= 0.02 * abs(e_ref)
+    ss = np.append(f["spread_stress"], s_ref)
+    es = np.append(f["spread_energy"], e_ref)
+    print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g} (perturbed reference runs "
+          f"{ss.min():.6g}..{ss.max():.6g}); energy {e_gpu:.6g} vs {e_ref:.6g} "
+          f"({es.min():.6g}..{es.max():.6g})")
+    assert ss.min() - 0.02 * abs(s_ref) <= s_gpu <= ss.max() + 0.02 * abs(s_ref)
+    assert es.min() - 0.02 * abs(e_ref) <= e_gpu <= es.max() + 0.02 * abs(e_ref)
 
 
 @pytest.mark.gpu
