@@ -215,8 +215,12 @@ __global__ __launch_bounds__(kBlock) void k_karras(
 }
 
 // ----------------------------------------------- mass prefix sums (dd)
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kBlock * kScanItems;
+// One wavefront per chunk of 512 points in key order, taken as 8 rows of 64:
+// every load and store is a coalesced row, and the chunk's offset comes from
+// the scanned per-chunk totals, so no block-wide synchronisation is needed.
+constexpr int kScanRows = 8;
+constexpr int kScanTile = kScanRows * kWave; // points per chunk (one wave)
+constexpr int kScanWaves = kBlock / kWave;   // chunks per block
 
 struct Acc {
   double m;
@@ -227,6 +231,8 @@ __device__ __forceinline__ Acc acc_add(const Acc &a, const Acc &b) {
   return {a.m + b.m, dd_add(a.x, b.x), dd_add(a.y, b.y)};
 }
 
+__device__ __forceinline__ Acc acc_zero() { return Acc{0.0, {0.0, 0.0}, {0.0, 0.0}}; }
+
 __device__ __forceinline__ Acc load_elem(const double *__restrict__ spos64,
                                          const double *__restrict__ smass64,
                                          int64_t s) {
@@ -234,38 +240,48 @@ __device__ __forceinline__ Acc load_elem(const double *__restrict__ spos64,
   return {m, two_prod(m, spos64[2 * s]), two_prod(m, spos64[2 * s + 1])};
 }
 
-__device__ Acc block_inclusive_scan(Acc v, Acc *sm) {
-  sm[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    Acc o = {0.0, {0.0, 0.0}, {0.0, 0.0}};
-    if (threadIdx.x >= (unsigned)off)
-      o = sm[threadIdx.x - off];
-    __syncthreads();
-    if (threadIdx.x >= (unsigned)off) {
-      v = acc_add(o, v);
-      sm[threadIdx.x] = v;
-    }
-    __syncthreads();
-  }
-  return v;
+__device__ __forceinline__ Acc acc_shfl_up(const Acc &a, int off) {
+  return {__shfl_up(a.m, off, kWave),
+          {__shfl_up(a.x.hi, off, kWave), __shfl_up(a.x.lo, off, kWave)},
+          {__shfl_up(a.y.hi, off, kWave), __shfl_up(a.y.lo, off, kWave)}};
 }
 
+__device__ __forceinline__ Acc acc_shfl_xor(const Acc &a, int off) {
+  return {__shfl_xor(a.m, off, kWave),
+          {__shfl_xor(a.x.hi, off, kWave), __shfl_xor(a.x.lo, off, kWave)},
+          {__shfl_xor(a.y.hi, off, kWave), __shfl_xor(a.y.lo, off, kWave)}};
+}
+
+__device__ __forceinline__ Acc acc_lane(const Acc &a, int l) {
+  return {__shfl(a.m, l, kWave),
+          {__shfl(a.x.hi, l, kWave), __shfl(a.x.lo, l, kWave)},
+          {__shfl(a.y.hi, l, kWave), __shfl(a.y.lo, l, kWave)}};
+}
+
+// Per-chunk totals: each lane sums its column of the chunk's 8 rows, then
+// the wave adds the 64 column sums in a fixed butterfly order.
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(
     const double *__restrict__ spos64, const double *__restrict__ smass64,
-    int64_t n, double *__restrict__ part,
-    const Scalars *__restrict__ sc) {
-  __shared__ Acc sm[kBlock];
+    int64_t n, double *__restrict__ part, const Scalars *__restrict__ sc) {
   if (sc->stopped)
     return;
-  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
-  for (int k = 0; k < kScanItems; k++)
-    if (base + k < n)
-      a = acc_add(a, load_elem(spos64, smass64, base + k));
-  a = block_inclusive_scan(a, sm);
-  if (threadIdx.x == kBlock - 1) {
-    double *p = part + 5 * blockIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t chunk = (int64_t)blockIdx.x * kScanWaves + threadIdx.x / kWave;
+  const int64_t base = chunk * kScanTile;
+  if (base >= n)
+    return;
+  Acc a = acc_zero();
+#pragma unroll
+  for (int r = 0; r < kScanRows; r++) {
+    const int64_t s = base + r * kWave + lane;
+    if (s < n)
+      a = acc_add(a, load_elem(spos64, smass64, s));
+  }
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1)
+    a = acc_add(a, acc_shfl_xor(a, off));
+  if (lane == 0) {
+    double *p = part + 5 * chunk;
     p[0] = a.m;
     p[1] = a.x.hi;
     p[2] = a.x.lo;
@@ -274,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(
   }
 }
 
-// Exclusive scan of the per-tile totals, in place (one block of 1024).
+// Exclusive scan of the per-chunk totals, in place (one block of 1024).
 __global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ part,
                                                         int tiles,
                                                         const Scalars *__restrict__ sc) {
@@ -284,7 +300,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ par
   const int chunk = (tiles + 1023) / 1024;
   const int b = threadIdx.x * chunk;
   const int e = b + chunk < tiles ? b + chunk : tiles;
-  Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+  Acc a = acc_zero();
   for (int t = b; t < e; t++) {
     const double *p = part + 5 * t;
     a = acc_add(a, Acc{p[0], {p[1], p[2]}, {p[3], p[4]}});
@@ -292,7 +308,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ par
   sm[threadIdx.x] = a;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    Acc o = {0.0, {0.0, 0.0}, {0.0, 0.0}};
+    Acc o = acc_zero();
     if (threadIdx.x >= (unsigned)off)
       o = sm[threadIdx.x - off];
     __syncthreads();
@@ -300,7 +316,7 @@ __global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ par
       sm[threadIdx.x] = acc_add(o, sm[threadIdx.x]);
     __syncthreads();
   }
-  Acc run = threadIdx.x ? sm[threadIdx.x - 1] : Acc{0.0, {0.0, 0.0}, {0.0, 0.0}};
+  Acc run = threadIdx.x ? sm[threadIdx.x - 1] : acc_zero();
   for (int t = b; t < e; t++) {
     double *p = part + 5 * t;
     Acc v = {p[0], {p[1], p[2]}, {p[3], p[4]}};
@@ -313,42 +329,46 @@ __global__ __launch_bounds__(1024) void k_scan_partials(double *__restrict__ par
   }
 }
 
-// prefix[s + 1] = sum over sorted points 0..s; prefix[0] = 0.
+// prefix[s + 1] = sum over sorted points 0..s; prefix[0] = 0. Row by row:
+// a wave-wide inclusive scan of the row, plus the running carry.
 __global__ __launch_bounds__(kBlock) void k_scan_final(
     const double *__restrict__ spos64, const double *__restrict__ smass64,
     int64_t n, const double *__restrict__ part,
     double *__restrict__ pm, double *__restrict__ pxh, double *__restrict__ pxl,
     double *__restrict__ pyh, double *__restrict__ pyl,
     const Scalars *__restrict__ sc) {
-  __shared__ Acc sm[kBlock];
   if (sc->stopped)
     return;
-  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  Acc loc[kScanItems];
-  Acc a = {0.0, {0.0, 0.0}, {0.0, 0.0}};
-  for (int k = 0; k < kScanItems; k++) {
-    if (base + k < n)
-      a = acc_add(a, load_elem(spos64, smass64, base + k));
-    loc[k] = a;
-  }
-  Acc incl = block_inclusive_scan(a, sm);
-  const double *p = part + 5 * blockIdx.x;
-  Acc off = {p[0], {p[1], p[2]}, {p[3], p[4]}};
-  Acc ex = threadIdx.x ? sm[threadIdx.x - 1] : Acc{0.0, {0.0, 0.0}, {0.0, 0.0}};
-  (void)incl;
-  off = acc_add(off, ex);
-  for (int k = 0; k < kScanItems; k++) {
-    int64_t s = base + k;
-    if (s < n) {
-      Acc v = acc_add(off, loc[k]);
-      pm[s + 1] = v.m;
-      pxh[s + 1] = v.x.hi;
-      pxl[s + 1] = v.x.lo;
-      pyh[s + 1] = v.y.hi;
-      pyl[s + 1] = v.y.lo;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t chunk = (int64_t)blockIdx.x * kScanWaves + threadIdx.x / kWave;
+  const int64_t base = chunk * kScanTile;
+  if (base >= n)
+    return;
+  const double *p = part + 5 * chunk;
+  Acc carry = {p[0], {p[1], p[2]}, {p[3], p[4]}};
+  for (int r = 0; r < kScanRows; r++) {
+    const int64_t row = base + r * kWave;
+    if (row >= n)
+      break;
+    const int64_t s = row + lane;
+    Acc v = s < n ? load_elem(spos64, smass64, s) : acc_zero();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const Acc o = acc_shfl_up(v, off);
+      if (lane >= off)
+        v = acc_add(o, v);
     }
+    const Acc out = acc_add(carry, v);
+    if (s < n) {
+      pm[s + 1] = out.m;
+      pxh[s + 1] = out.x.hi;
+      pxl[s + 1] = out.x.lo;
+      pyh[s + 1] = out.y.hi;
+      pyl[s + 1] = out.y.lo;
+    }
+    carry = acc_add(carry, acc_lane(v, kWave - 1));
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (chunk == 0 && lane == 0) {
     pm[0] = 0.0;
     pxh[0] = pxl[0] = pyh[0] = pyl[0] = 0.0;
   }
@@ -602,12 +622,13 @@ int enqueue_radix_tree(ForceCommon &c) {
                        st, c.keyHi.p, c.keyLoS.p, n, c.left.p, c.right.p,
                        c.parent.p, c.first.p, c.last.p, c.delta.p, c.sc.p);
   const int tiles = (int)((n + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(tiles > 0 ? tiles : 1), dim3(kBlock), 0,
+  const unsigned sb = (unsigned)((tiles + kScanWaves - 1) / kScanWaves);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(sb > 0 ? sb : 1), dim3(kBlock), 0,
                      st, c.spos64.p, c.smass64.p, n, c.scanPartial.p,
                      c.sc.p);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, c.scanPartial.p,
                      tiles, c.sc.p);
-  hipLaunchKernelGGL(k_scan_final, dim3(tiles > 0 ? tiles : 1), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_scan_final, dim3(sb > 0 ? sb : 1), dim3(kBlock), 0,
                      st, c.spos64.p, c.smass64.p, n, c.scanPartial.p,
                      c.pm.p, c.pmxHi.p, c.pmxLo.p, c.pmyHi.p, c.pmyLo.p, c.sc.p);
   GVH_CHECK_LAUNCH();
