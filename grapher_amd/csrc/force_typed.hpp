@@ -470,25 +470,34 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
 
   // Evaluates the deferred bodies in the order they were deferred.
+  // Evaluates the deferred bodies in the order they were deferred, two per
+  // iteration so the second body's LDS loads overlap the first's math.
+  auto body = [&](const TravBody<Real, PosT> &b) {
+    if (tstats)
+      st[2] += __popcll(rl(b.mask, 0));
+    if ((b.mask >> lane) & 1ull) {
+      const PosT ex = b.x - me.x, ey = b.y - me.y;
+      Real dx = (Real)ex, dy = (Real)ey;
+      Real inv;
+      Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+      Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + b.r, mr.x, b.m, kSq,
+                                            oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
+                                            r64, n);
+      cnt++;
+    }
+  };
   auto flush = [&]() {
     if (tstats)
       st[1] += nd;
-#pragma unroll 2
-    for (int t = 0; t < nd; t++) {
-      const TravBody<Real, PosT> b = dBody[wave][t];
-      if (tstats)
-        st[2] += __popcll(rl(b.mask, 0));
-      if ((b.mask >> lane) & 1ull) {
-        const PosT ex = b.x - me.x, ey = b.y - me.y;
-        Real dx = (Real)ex, dy = (Real)ey;
-        Real inv;
-        Real dist = Phys<Real>::dist_inv(dx, dy, inv);
-        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + b.r, mr.x, b.m, kSq,
-                                              oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
-                                              r64, n);
-        cnt++;
-      }
+    int t = 0;
+    for (; t + 2 <= nd; t += 2) {
+      const TravBody<Real, PosT> b0 = dBody[wave][t];
+      const TravBody<Real, PosT> b1 = dBody[wave][t + 1];
+      body(b0);
+      body(b1);
     }
+    if (t < nd)
+      body(dBody[wave][t]);
     nd = 0;
   };
   // Writes point p's body at list slot `at`.
