@@ -419,7 +419,7 @@ def bench_grip(args):
     emit(line)
 
 
-def warm_start_line(graphs=3, n=200_000):
+def warm_start_line(graphs=8, n=200_000):
     """The metric's second half: wall time to the plain method's final
     stress (within 1%) from the learned start versus from the random start,
     on unseen connected SBM graphs of ~200k vertices (FR, Barnes-Hut 0.7,

@@ -402,6 +402,270 @@ __global__ void k_put_tail(const int32_t *__restrict__ off, int64_t n,
     X0[v * kF + c] = 0;
 }
 
+// ---- pivot MDS (Brandes & Pich): classical MDS restricted to the columns of
+// k <= 64 pivot vertices. D = BFS hop distances (n x k), C = -1/2 of D^2
+// double-centred, axes = C v_i / mu_i^(1/4) for the top two eigenpairs
+// (v_i, mu_i) of C^T C (so each axis carries sqrt of the corresponding
+// classical-MDS eigenvalue, as in full MDS).
+constexpr int kPiv = 64;
+constexpr uint16_t kFar = 0xFFFF;
+
+// dist = kFar everywhere; visited/frontier = the pivots' bits.
+__global__ void k_piv_init(uint16_t *__restrict__ dist, uint64_t *__restrict__ vis,
+                           uint64_t *__restrict__ front, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n * kPiv)
+    dist[i] = kFar;
+  if (i < n) {
+    vis[i] = 0;
+    front[i] = 0;
+  }
+}
+
+__global__ void k_piv_seed(const int32_t *__restrict__ piv, int k, uint16_t *__restrict__ dist,
+                           uint64_t *__restrict__ vis, uint64_t *__restrict__ front) {
+  // one thread: pivots may repeat, so their bits are merged serially
+  if (blockIdx.x != 0 || threadIdx.x != 0)
+    return;
+  for (int j = 0; j < k; j++) {
+    const int32_t v = piv[j];
+    dist[(int64_t)v * kPiv + j] = 0;
+    vis[v] |= 1ull << j;
+    front[v] |= 1ull << j;
+  }
+}
+
+// One BFS level for all pivots at once: a vertex joins pivot j's next
+// frontier when a neighbour is on j's frontier and v has not been reached
+// from j. Reads only `front` and writes only `next`, `vis[v]` and v's row.
+__global__ __launch_bounds__(kB) void k_piv_level(const int32_t *__restrict__ off,
+                                                  const int32_t *__restrict__ adj,
+                                                  const uint64_t *__restrict__ front,
+                                                  uint64_t *__restrict__ vis,
+                                                  uint64_t *__restrict__ next,
+                                                  uint16_t *__restrict__ dist, int64_t n,
+                                                  int level, int32_t *__restrict__ changed) {
+  const int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (v >= n)
+    return;
+  uint64_t m = 0;
+  for (int32_t e = off[v]; e < off[v + 1]; e++)
+    m |= front[adj[e]];
+  m &= ~vis[v];
+  next[v] = m;
+  if (m) {
+    vis[v] |= m;
+    const uint16_t d = (uint16_t)(level + 1);
+    for (uint64_t b = m; b; b &= b - 1)
+      dist[v * kPiv + __builtin_ctzll(b)] = d;
+    *changed = 1;
+  }
+}
+
+__device__ __forceinline__ double piv_d2(uint16_t d, uint16_t far) {
+  const double x = (double)(d == kFar ? far : d);
+  return x * x;
+}
+
+// Row means of D^2 per vertex and per-block column sums (fixed order).
+__global__ __launch_bounds__(kB) void k_piv_moments(const uint16_t *__restrict__ dist,
+                                                    int64_t n, int k, uint16_t far,
+                                                    double *__restrict__ rowMean,
+                                                    double *__restrict__ colPart) {
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
+  for (int64_t v = b0 + threadIdx.x; v < b1; v += kB) {
+    double r = 0.0;
+    for (int j = 0; j < k; j++)
+      r += piv_d2(dist[v * kPiv + j], far);
+    rowMean[v] = r / k;
+  }
+  if (threadIdx.x < k) {
+    double c = 0.0;
+    for (int64_t v = b0; v < b1; v++)
+      c += piv_d2(dist[v * kPiv + threadIdx.x], far);
+    colPart[blockIdx.x * kPiv + threadIdx.x] = c;
+  }
+}
+
+__device__ __forceinline__ double piv_c(const uint16_t *__restrict__ dist, int64_t v, int j,
+                                        uint16_t far, const double *__restrict__ rowMean,
+                                        const double *__restrict__ colMean, double all) {
+  return -0.5 * (piv_d2(dist[v * kPiv + j], far) - rowMean[v] - colMean[j] + all);
+}
+
+// Per-block partial Gram matrix C^T C (k x k, FP64), rows staged in LDS.
+constexpr int kGramRows = 32;
+__global__ __launch_bounds__(kB) void k_piv_gram(const uint16_t *__restrict__ dist, int64_t n,
+                                                 int k, uint16_t far,
+                                                 const double *__restrict__ rowMean,
+                                                 const double *__restrict__ colMean,
+                                                 double all, double *__restrict__ gramPart) {
+  __shared__ double cs[kGramRows][kPiv];
+  double acc[kPiv * kPiv / kB];
+#pragma unroll
+  for (int q = 0; q < kPiv * kPiv / kB; q++)
+    acc[q] = 0.0;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = blockIdx.x * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
+  for (int64_t base = b0; base < b1; base += kGramRows) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kGramRows * kPiv; i += kB) {
+      const int r = i / kPiv, j = i % kPiv;
+      const int64_t v = base + r;
+      cs[r][j] = (v < b1 && j < k) ? piv_c(dist, v, j, far, rowMean, colMean, all) : 0.0;
+    }
+    __syncthreads();
+    const int rows = b1 - base < kGramRows ? (int)(b1 - base) : kGramRows;
+    for (int r = 0; r < rows; r++) {
+#pragma unroll
+      for (int q = 0; q < kPiv * kPiv / kB; q++) {
+        const int e = q * kB + threadIdx.x;
+        acc[q] += cs[r][e / kPiv] * cs[r][e % kPiv];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPiv * kPiv / kB; q++)
+    gramPart[(int64_t)blockIdx.x * kPiv * kPiv + q * kB + threadIdx.x] = acc[q];
+}
+
+// Sums `blocks` partial k x k (padded 64 x 64) matrices in block order.
+__global__ void k_piv_gram_sum(const double *__restrict__ part, int blocks,
+                               double *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kPiv * kPiv)
+    return;
+  double s = 0.0;
+  for (int b = 0; b < blocks; b++)
+    s += part[(int64_t)b * kPiv * kPiv + e];
+  out[e] = s;
+}
+
+// Coordinates = C [e1 e2] (axes pre-scaled on the host), FP64 accumulation.
+__global__ __launch_bounds__(kB) void k_piv_project(const uint16_t *__restrict__ dist,
+                                                    int64_t n, int k, uint16_t far,
+                                                    const double *__restrict__ rowMean,
+                                                    const double *__restrict__ colMean,
+                                                    double all, const double *__restrict__ axes,
+                                                    double2 *__restrict__ out) {
+  const int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (v >= n)
+    return;
+  double x = 0.0, y = 0.0;
+  for (int j = 0; j < k; j++) {
+    const double c = piv_c(dist, v, j, far, rowMean, colMean, all);
+    x += c * axes[j];
+    y += c * axes[kPiv + j];
+  }
+  out[v] = make_double2(x, y);
+}
+
+// Per-block sums of x, y, x^2 + y^2 of the raw coordinates (fixed order).
+__global__ __launch_bounds__(kB) void k_piv_stats(const double2 *__restrict__ raw, int64_t n,
+                                                  double *__restrict__ part) {
+  __shared__ double sm[kB / kWave][3];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kB) {
+    const double2 p = raw[v];
+    a += p.x;
+    b += p.y;
+    c += p.x * p.x + p.y * p.y;
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0) {
+    sm[wave][0] = a;
+    sm[wave][1] = b;
+    sm[wave][2] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+    for (int w = 0; w < kB / kWave; w++)
+      t += sm[w][threadIdx.x];
+    part[blockIdx.x * 3 + threadIdx.x] = t;
+  }
+}
+
+// Skip term: centred, scaled to RMS radius 1 (one scale for both axes);
+// FP32 copy for the head and BF16 feature columns 12-13.
+__global__ void k_piv_finish(const double2 *__restrict__ raw, int64_t n, double mx, double my,
+                             double inv, float2 *__restrict__ skip, uint16_t *__restrict__ X0) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  const float x = (float)((raw[v].x - mx) * inv), y = (float)((raw[v].y - my) * inv);
+  skip[v] = make_float2(x, y);
+  X0[v * kF + 12] = f2bf(x);
+  X0[v * kF + 13] = f2bf(y);
+}
+
+// Head outputs 0-1 of every row, packed.
+__global__ void k_head_xy(const float *__restrict__ Y, int64_t n, float2 *__restrict__ out) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n)
+    out[v] = make_float2(Y[v * kO], Y[v * kO + 1]);
+}
+
+// Head outputs 0-1 += skip (after the head product).
+__global__ void k_add_skip(const float2 *__restrict__ skip, int64_t n, float *__restrict__ Y) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  Y[v * kO] += skip[v].x;
+  Y[v * kO + 1] += skip[v].y;
+}
+
+// Eigenpairs of a symmetric m x m matrix (row-major, m <= 64) by cyclic
+// Jacobi rotations; returns eigenvalues in w and eigenvectors as columns of V.
+void jacobi_eigen(std::vector<double> A, int m, std::vector<double> &w,
+                  std::vector<double> &V) {
+  V.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; i++)
+    V[(size_t)i * m + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, diag = 0.0;
+    for (int p = 0; p < m; p++)
+      for (int q = 0; q < m; q++)
+        (p == q ? diag : off) += A[(size_t)p * m + q] * A[(size_t)p * m + q];
+    if (off <= 1e-30 * (diag > 0.0 ? diag : 1.0))
+      break;
+    for (int p = 0; p < m - 1; p++)
+      for (int q = p + 1; q < m; q++) {
+        const double apq = A[(size_t)p * m + q];
+        if (apq == 0.0)
+          continue;
+        const double app = A[(size_t)p * m + p], aqq = A[(size_t)q * m + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) /
+                         (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int r = 0; r < m; r++) {
+          const double arp = A[(size_t)r * m + p], arq = A[(size_t)r * m + q];
+          A[(size_t)r * m + p] = c * arp - sn * arq;
+          A[(size_t)r * m + q] = sn * arp + c * arq;
+        }
+        for (int r = 0; r < m; r++) {
+          const double apr = A[(size_t)p * m + r], aqr = A[(size_t)q * m + r];
+          A[(size_t)p * m + r] = c * apr - sn * aqr;
+          A[(size_t)q * m + r] = sn * apr + c * aqr;
+        }
+        for (int r = 0; r < m; r++) {
+          const double vrp = V[(size_t)r * m + p], vrq = V[(size_t)r * m + q];
+          V[(size_t)r * m + p] = c * vrp - sn * vrq;
+          V[(size_t)r * m + q] = sn * vrp + c * vrq;
+        }
+      }
+  }
+  w.resize(m);
+  for (int i = 0; i < m; i++)
+    w[i] = A[(size_t)i * m + i];
+}
+
 } // namespace
 
 struct gvizhip_gnn {
@@ -416,6 +680,13 @@ struct gvizhip_gnn {
   DevBuf<float> partial;
   DevBuf<double> moments;
   DevBuf<float> diff[2];
+  // pivot MDS
+  DevBuf<int32_t> piv, changed;
+  DevBuf<uint16_t> pdist;
+  DevBuf<uint64_t> pvis, pfront[2];
+  DevBuf<double> prow, pcol, pgram, paxes, praw;
+  DevBuf<float> skip;
+  bool hasSkip = false;
   int64_t step = 0;
   bool weightsDirty = true;
 };
@@ -466,8 +737,11 @@ int forward(gvizhip_gnn *g) {
     launch_gemm<4>(g, g->H[l - 1].p, g->M[l - 1].p, kH, g->wt[l].p, 2 * kH, b[l], 1,
                    g->H[l].p, nullptr);
   }
-  // head
+  // head (+ the pivot-MDS skip term when one is set)
   launch_gemm<1>(g, g->H[2].p, g->H[2].p, kH, g->wt[3].p, kH, b[3], 0, nullptr, g->Y.p);
+  if (g->hasSkip)
+    hipLaunchKernelGGL(k_add_skip, dim3(grid_for(n, kB)), dim3(kB), 0, st,
+                       (const float2 *)g->skip.p, n, g->Y.p);
   GVH_CHECK_LAUNCH();
   return 0;
 }
@@ -690,6 +964,7 @@ int gvizhip_gnn_set_graph(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
     x[i] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
   }
   g->n = n;
+  g->hasSkip = false;
   GVH_CHECK(hipMemcpyAsync(g->off.p, o.data(), sizeof(int32_t) * (n + 1),
                            hipMemcpyHostToDevice, g->stream));
   if (m > 0)
@@ -718,6 +993,19 @@ int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n, const int64_t *off
   std::vector<float> zeros((size_t)n * kF, 0.0f);
   if (gvizhip_gnn_set_graph(g, n, offsets, adj, zeros.data()))
     return -1;
+  return gvizhip_gnn_set_start(g, start);
+}
+
+// Rebuilds the diffused features of the current graph from a new start
+// (columns 0-11; 12-15 zero, the skip term removed).
+int gvizhip_gnn_set_start(gvizhip_gnn *g, const double *start) {
+  if (check_graph(g) || !start) {
+    if (g && !start)
+      set_error("gvizhip_gnn_set_start: invalid arguments");
+    return -1;
+  }
+  const int64_t n = g->n;
+  g->hasSkip = false;
   std::vector<float> x(2 * n);
   for (int64_t i = 0; i < 2 * n; i++)
     x[i] = (float)start[i];
@@ -759,18 +1047,177 @@ int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n, const int64_t *off
   return 0;
 }
 
+// Pivot MDS of the current graph from BFS distances to `count` pivots (see
+// k_piv_* above): feature columns 12-13 and the skip term added to the head.
+int gvizhip_gnn_set_pivots(gvizhip_gnn *g, const int64_t *pivots, int64_t count,
+                           double *coords) {
+  if (check_graph(g))
+    return -1;
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  if (count == 0) {
+    g->hasSkip = false;
+    return 0;
+  }
+  if (!pivots || count < 0 || count > kPiv) {
+    set_error("gvizhip_gnn_set_pivots: 0 to %d pivots", kPiv);
+    return -1;
+  }
+  std::vector<int32_t> pv(count);
+  for (int64_t j = 0; j < count; j++) {
+    if (pivots[j] < 0 || pivots[j] >= n) {
+      set_error("gvizhip_gnn_set_pivots: pivot out of range");
+      return -1;
+    }
+    pv[j] = (int32_t)pivots[j];
+  }
+  const int64_t cap = g->cap;
+  constexpr int kParts = 256;
+  if (g->piv.alloc(kPiv) || g->changed.alloc(1) || g->pdist.alloc(cap * kPiv) ||
+      g->pvis.alloc(cap) || g->pfront[0].alloc(cap) || g->pfront[1].alloc(cap) ||
+      g->prow.alloc(cap) || g->pcol.alloc(kParts * kPiv + kPiv) ||
+      g->pgram.alloc((int64_t)kParts * kPiv * kPiv + kPiv * kPiv) ||
+      g->paxes.alloc(2 * kPiv) || g->praw.alloc(2 * cap) || g->skip.alloc(2 * cap))
+    return -1;
+  const int k = (int)count;
+  GVH_CHECK(hipMemcpyAsync(g->piv.p, pv.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice,
+                           st));
+  hipLaunchKernelGGL(k_piv_init, dim3(grid_for(n * kPiv, kB)), dim3(kB), 0, st, g->pdist.p,
+                     g->pvis.p, g->pfront[0].p, n);
+  hipLaunchKernelGGL(k_piv_seed, dim3(1), dim3(1), 0, st, g->piv.p, k, g->pdist.p, g->pvis.p,
+                     g->pfront[0].p);
+  GVH_CHECK_LAUNCH();
+  int cur = 0, level = 0;
+  for (; level < (int)kFar - 2; level++) {
+    int32_t zero = 0, changed = 0;
+    GVH_CHECK(hipMemcpyAsync(g->changed.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_piv_level, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->off.p,
+                       g->adj.p, g->pfront[cur].p, g->pvis.p, g->pfront[1 - cur].p,
+                       g->pdist.p, n, level, g->changed.p);
+    GVH_CHECK_LAUNCH();
+    GVH_CHECK(hipMemcpyAsync(&changed, g->changed.p, sizeof(int32_t), hipMemcpyDeviceToHost,
+                             st));
+    GVH_CHECK(hipStreamSynchronize(st));
+    cur = 1 - cur;
+    if (!changed)
+      break;
+  }
+  // vertices a pivot never reaches count as one hop beyond the farthest
+  const uint16_t far = (uint16_t)(level + 1);
+  const int parts = (int)std::min<int64_t>(kParts, (n + kB - 1) / kB);
+  hipLaunchKernelGGL(k_piv_moments, dim3(parts), dim3(kB), 0, st, g->pdist.p, n, k, far,
+                     g->prow.p, g->pcol.p);
+  GVH_CHECK_LAUNCH();
+  std::vector<double> colPart((size_t)parts * kPiv);
+  GVH_CHECK(hipMemcpyAsync(colPart.data(), g->pcol.p, sizeof(double) * parts * kPiv,
+                           hipMemcpyDeviceToHost, st));
+  GVH_CHECK(hipStreamSynchronize(st));
+  std::vector<double> colMean(kPiv, 0.0);
+  double all = 0.0;
+  for (int j = 0; j < k; j++) {
+    double c = 0.0;
+    for (int b = 0; b < parts; b++)
+      c += colPart[(size_t)b * kPiv + j];
+    colMean[j] = c / (double)n;
+    all += colMean[j];
+  }
+  all /= k;
+  double *colMeanDev = g->pcol.p + kParts * kPiv;
+  GVH_CHECK(hipMemcpyAsync(colMeanDev, colMean.data(), sizeof(double) * kPiv,
+                           hipMemcpyHostToDevice, st));
+  const int gparts = (int)std::min<int64_t>(kParts, (n + kGramRows - 1) / kGramRows);
+  hipLaunchKernelGGL(k_piv_gram, dim3(gparts), dim3(kB), 0, st, g->pdist.p, n, k, far,
+                     g->prow.p, colMeanDev, all, g->pgram.p);
+  double *gramDev = g->pgram.p + (int64_t)kParts * kPiv * kPiv;
+  hipLaunchKernelGGL(k_piv_gram_sum, dim3(kPiv * kPiv / kB), dim3(kB), 0, st, g->pgram.p,
+                     gparts, gramDev);
+  GVH_CHECK_LAUNCH();
+  std::vector<double> gram(kPiv * kPiv);
+  GVH_CHECK(hipMemcpyAsync(gram.data(), gramDev, sizeof(double) * kPiv * kPiv,
+                           hipMemcpyDeviceToHost, st));
+  GVH_CHECK(hipStreamSynchronize(st));
+  std::vector<double> A((size_t)k * k), w, V;
+  for (int a = 0; a < k; a++)
+    for (int b = 0; b < k; b++)
+      A[(size_t)a * k + b] = gram[(size_t)a * kPiv + b];
+  jacobi_eigen(A, k, w, V);
+  int top[2] = {-1, -1};
+  for (int i = 0; i < k; i++) {
+    if (top[0] < 0 || w[i] > w[top[0]]) {
+      top[1] = top[0];
+      top[0] = i;
+    } else if (top[1] < 0 || w[i] > w[top[1]]) {
+      top[1] = i;
+    }
+  }
+  std::vector<double> axes(2 * kPiv, 0.0);
+  for (int a = 0; a < 2; a++) {
+    const int c = top[a];
+    if (c < 0 || !(w[c] > 0.0))
+      continue;
+    // sign: the largest-magnitude component is positive
+    int big = 0;
+    for (int j = 1; j < k; j++)
+      if (std::fabs(V[(size_t)j * k + c]) > std::fabs(V[(size_t)big * k + c]))
+        big = j;
+    const double sgn = V[(size_t)big * k + c] < 0.0 ? -1.0 : 1.0;
+    const double sc = sgn / std::pow(w[c], 0.25);
+    for (int j = 0; j < k; j++)
+      axes[a * kPiv + j] = V[(size_t)j * k + c] * sc;
+  }
+  GVH_CHECK(hipMemcpyAsync(g->paxes.p, axes.data(), sizeof(double) * 2 * kPiv,
+                           hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_piv_project, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->pdist.p, n, k,
+                     far, g->prow.p, colMeanDev, all, g->paxes.p, (double2 *)g->praw.p);
+  GVH_CHECK_LAUNCH();
+  const int sparts = (int)std::min<int64_t>(kParts, grid_for(n, kB));
+  hipLaunchKernelGGL(k_piv_stats, dim3(sparts), dim3(kB), 0, st, (const double2 *)g->praw.p,
+                     n, g->pcol.p);
+  GVH_CHECK_LAUNCH();
+  std::vector<double> sp(3 * sparts);
+  GVH_CHECK(hipMemcpyAsync(sp.data(), g->pcol.p, sizeof(double) * 3 * sparts,
+                           hipMemcpyDeviceToHost, st));
+  GVH_CHECK(hipStreamSynchronize(st));
+  double sx = 0.0, sy = 0.0, s2 = 0.0;
+  for (int b = 0; b < sparts; b++) {
+    sx += sp[3 * b];
+    sy += sp[3 * b + 1];
+    s2 += sp[3 * b + 2];
+  }
+  const double mx = sx / n, my = sy / n;
+  const double var = s2 / n - (mx * mx + my * my);
+  const double rms = var > 0.0 ? std::sqrt(var) : 0.0;
+  const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
+  hipLaunchKernelGGL(k_piv_finish, dim3(grid_for(n, kB)), dim3(kB), 0, st,
+                     (const double2 *)g->praw.p, n, mx, my, inv, (float2 *)g->skip.p, g->X0.p);
+  GVH_CHECK_LAUNCH();
+  if (coords) {
+    std::vector<float> c(2 * n);
+    GVH_CHECK(hipMemcpyAsync(c.data(), g->skip.p, sizeof(float) * 2 * n,
+                             hipMemcpyDeviceToHost, st));
+    GVH_CHECK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < 2 * n; i++)
+      coords[i] = c[i];
+  }
+  GVH_CHECK(hipStreamSynchronize(st));
+  g->hasSkip = true;
+  return 0;
+}
+
 int gvizhip_gnn_forward(gvizhip_gnn *g, double *positions) {
   if (check_graph(g) || forward(g))
     return -1;
   if (positions) {
-    std::vector<float> y(g->n * kO);
-    GVH_CHECK(hipMemcpyAsync(y.data(), g->Y.p, sizeof(float) * g->n * kO,
+    // outputs 0-1 of every row, packed on the device before the copy
+    hipLaunchKernelGGL(k_head_xy, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream, g->Y.p,
+                       g->n, (float2 *)g->diff[0].p);
+    GVH_CHECK_LAUNCH();
+    std::vector<float> y(2 * g->n);
+    GVH_CHECK(hipMemcpyAsync(y.data(), g->diff[0].p, sizeof(float) * 2 * g->n,
                              hipMemcpyDeviceToHost, g->stream));
     GVH_CHECK(hipStreamSynchronize(g->stream));
-    for (int64_t v = 0; v < g->n; v++) {
-      positions[2 * v] = y[v * kO];
-      positions[2 * v + 1] = y[v * kO + 1];
-    }
+    for (int64_t i = 0; i < 2 * g->n; i++)
+      positions[i] = y[i];
   }
   GVH_CHECK(hipStreamSynchronize(g->stream));
   return 0;

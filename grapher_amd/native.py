@@ -139,6 +139,8 @@ def lib():
         "gvizhip_gnn_get_params": (C.c_int, [_vp, _vp]),
         "gvizhip_gnn_set_graph": (C.c_int, [_vp, _i64, _vp, _vp, _vp]),
         "gvizhip_gnn_set_graph_diffused": (C.c_int, [_vp, _i64, _vp, _vp, _vp]),
+        "gvizhip_gnn_set_pivots": (C.c_int, [_vp, _vp, _i64, _vp]),
+        "gvizhip_gnn_set_start": (C.c_int, [_vp, _vp]),
         "gvizhip_gnn_forward": (C.c_int, [_vp, _vp]),
         "gvizhip_gnn_gradients": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), _vp]),
         "gvizhip_gnn_train_step": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
@@ -179,7 +181,7 @@ DECLARED = [
     "gvizhip_stress_create", "gvizhip_stress_evaluate", "gvizhip_stress_destroy",
     "gvizhip_gnn_create", "gvizhip_gnn_param_count", "gvizhip_gnn_set_params",
     "gvizhip_gnn_get_params", "gvizhip_gnn_set_graph", "gvizhip_gnn_set_graph_diffused",
-    "gvizhip_gnn_forward",
+    "gvizhip_gnn_set_pivots", "gvizhip_gnn_set_start", "gvizhip_gnn_forward",
     "gvizhip_gnn_gradients", "gvizhip_gnn_train_step", "gvizhip_gnn_destroy",
 ]
 
@@ -526,6 +528,23 @@ class GnnEngine:
                                                 _p(st)) != 0:
             _err("gvizhip_gnn_set_graph_diffused")
         self.n = n
+
+    def set_start(self, start):
+        """New diffused features of the current graph from `start` (n x 2)."""
+        st = np.ascontiguousarray(start, dtype=np.float64).reshape(self.n, 2)
+        if lib().gvizhip_gnn_set_start(self.h, _p(st)) != 0:
+            _err("gvizhip_gnn_set_start")
+
+    def set_pivots(self, pivots):
+        """Pivot-MDS features and skip term from BFS distances to `pivots`
+        (at most 64 vertex ids; empty removes the skip). Returns the
+        normalised coordinates (n x 2)."""
+        p = np.ascontiguousarray(pivots, dtype=np.int64)
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_gnn_set_pivots(self.h, _p(p) if p.size else None, int(p.size),
+                                        _p(out)) != 0:
+            _err("gvizhip_gnn_set_pivots")
+        return out
 
     def forward(self):
         out = np.zeros((self.n, 2))

@@ -39,6 +39,8 @@ F_IN = 16
 # layouts of its size have: repulsion then spreads it out, which settles
 # faster than attraction pulling an oversized start together
 START_SCALE = 0.5
+# pivots of the pivot-MDS input and skip term (gvizhip_gnn_set_pivots)
+PIVOTS = 64
 
 
 def sbm(n: int, blocks: int, seed: int = 0, deg_in: float = 8.0, deg_out: float = 0.2):
@@ -93,11 +95,22 @@ def features(offsets, seed: int):
     return x
 
 
-def force_layout(offsets, adj, seed: int, iters: int, precision: str = "fp64"):
+def force_layout(offsets, adj, seed: int, iters: int, precision: str = "fp64", start=None):
+    """The force embedder's layout after `iters` steps from `start` (uniform
+    [-1, 1]^2 from `seed` when None)."""
     eng = native.ForceEngine(offsets, adj, precision=precision, **FORCE)
-    eng.begin(G.uniform_positions(len(offsets) - 1, seed))
+    eng.begin(G.uniform_positions(len(offsets) - 1, seed) if start is None else start)
     eng.run(iters, stats=False)
     return eng.positions()
+
+
+def pmds_start(offsets, adj, seed: int, engine=None):
+    """Pivot-MDS coordinates (device) scaled to START_SCALE x 6 sqrt(n), a
+    rough converged radius at edge length 10."""
+    n = len(offsets) - 1
+    eng = engine or native.GnnEngine(n, max(len(adj), 1))
+    eng.set_graph_diffused(offsets, adj, np.zeros((n, 2)))
+    return eng.set_pivots(pick_pivots(n, seed)) * (START_SCALE * 6.0 * np.sqrt(n))
 
 
 def rms_radius(pos):
@@ -108,35 +121,48 @@ def rms_radius(pos):
 class WarmStart:
     """Trained network + the size model that rescales its output."""
 
-    def __init__(self, params, scale_a, scale_b):
+    def __init__(self, params, scale_a, scale_b, pivots=0):
         self.params = np.asarray(params, dtype=np.float32)
         self.scale_a = float(scale_a)
         self.scale_b = float(scale_b)
+        # > 0: the network also takes pivot-MDS coordinates from this many
+        # pivots as inputs and as a skip term on its output
+        self.pivots = int(pivots)
         self.engine = None
         self.engine_cap = (0, 0)
 
     @classmethod
     def load(cls, path=WEIGHTS):
         z = np.load(path, allow_pickle=False)
-        return cls(z["params"], z["scale_a"], z["scale_b"])
+        return cls(z["params"], z["scale_a"], z["scale_b"],
+                   int(z["pivots"]) if "pivots" in z.files else 0)
 
     def save(self, path=WEIGHTS, **extra):
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, params=self.params, scale_a=self.scale_a, scale_b=self.scale_b,
-                 **extra)
+                 pivots=self.pivots, **extra)
 
     def radius(self, n):
         return float(np.exp(self.scale_a + self.scale_b * np.log(n)))
 
-    def positions(self, offsets, adj, seed=0):
-        """Proposed start: the network output, centred and scaled to
-        START_SCALE times the RMS radius layouts of this size have."""
+    def prepare(self, offsets, adj):
+        """Uploads the graph to the network's engine (sized on first use)."""
         n = len(offsets) - 1
         if self.engine is None or self.engine_cap[0] < n or self.engine_cap[1] < len(adj):
             self.engine = native.GnnEngine(max(n, 1), max(len(adj), 1))
             self.engine_cap = (n, len(adj))
             self.engine.set_params(self.params)
-        self.engine.set_graph_diffused(offsets, adj, G.uniform_positions(n, seed))
+        self.engine.set_graph_diffused(offsets, adj, np.zeros((n, 2)))
+
+    def propose(self, seed=0):
+        """Proposed start for the graph prepared last: device features from a
+        random start, pivot MDS, the network's forward pass; centred and
+        scaled to START_SCALE times the RMS radius layouts of this size
+        have."""
+        n = self.engine.n
+        self.engine.set_start(G.uniform_positions(n, seed))
+        if self.pivots:
+            self.engine.set_pivots(pick_pivots(n, seed, self.pivots))
         p = self.engine.forward()
         p -= p.mean(0)
         r = rms_radius(p)
@@ -144,30 +170,50 @@ class WarmStart:
             p *= START_SCALE * self.radius(n) / r
         return p
 
+    def positions(self, offsets, adj, seed=0):
+        """prepare() then propose()."""
+        self.prepare(offsets, adj)
+        return self.propose(seed)
 
-def training_set(count, seed, n_min=1000, n_max=20000, iters=400, log=print):
+
+def training_set(count, seed, n_min=1000, n_max=20000, iters=400, log=print,
+                 start="random"):
+    """(offsets, adj, target) for `count` connected SBM graphs; the target is
+    the FP64 force layout after `iters` steps from a uniform random start, or
+    (start="pmds") from the graph's pivot-MDS start, which keeps that start's
+    global arrangement."""
     rng = np.random.default_rng(seed)
     out = []
+    eng = native.GnnEngine(n_max, 64 * n_max) if start == "pmds" else None
     for i in range(count):
         n = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
         blocks = int(rng.integers(2, 33))
         n, e = sbm_connected(n, blocks, seed * 100000 + i)
         off, adj = G.csr_from_edges(n, e)
-        t = force_layout(off, adj, seed=i + 7, iters=iters)
+        s0 = pmds_start(off, adj, i + 7, eng) if start == "pmds" else None
+        t = force_layout(off, adj, seed=i + 7, iters=iters, start=s0)
         out.append((off, adj, t))
         if log and (i + 1) % 20 == 0:
             log(f"# training layouts: {i + 1}/{count}")
     return out
 
 
-def train(data, epochs, lr=2e-3, seed=0, log=print, init=None):
+def pick_pivots(n, seed, count=PIVOTS):
+    """`count` distinct random vertices (all of them when n <= count)."""
+    if n <= count:
+        return np.arange(n, dtype=np.int64)
+    return np.sort(np.random.default_rng(seed).choice(n, count, replace=False))
+
+
+def train(data, epochs, lr=2e-3, seed=0, log=print, init=None, pivots=0):
     """Adam on the Procrustes loss, one graph per step, fresh random inputs
-    every epoch. Returns (WarmStart, per-epoch mean loss)."""
+    (and pivots, when `pivots` > 0) every epoch. Returns (WarmStart,
+    per-epoch mean loss)."""
     nmax = max(len(off) - 1 for off, _, _ in data)
     emax = max(len(adj) for _, adj, _ in data)
     eng = native.GnnEngine(nmax, emax)
     if init is None:
-        init = init_params(seed)
+        init = init_params(seed, head=0.0 if pivots else 0.5)
     eng.set_params(init)
     rng = np.random.default_rng(seed)
     history = []
@@ -175,8 +221,11 @@ def train(data, epochs, lr=2e-3, seed=0, log=print, init=None):
         losses = []
         for gi in rng.permutation(len(data)):
             off, adj, t = data[gi]
-            start = G.uniform_positions(len(off) - 1, int(rng.integers(1 << 30)))
+            n = len(off) - 1
+            start = G.uniform_positions(n, int(rng.integers(1 << 30)))
             eng.set_graph_diffused(off, adj, start)
+            if pivots:
+                eng.set_pivots(pick_pivots(n, int(rng.integers(1 << 30)), pivots))
             losses.append(eng.train_step(t, lr))
         history.append(float(np.mean(losses)))
         if log:
@@ -184,12 +233,14 @@ def train(data, epochs, lr=2e-3, seed=0, log=print, init=None):
     radii = np.array([rms_radius(t) for _, _, t in data])
     ns = np.array([len(off) - 1 for off, _, _ in data], dtype=np.float64)
     b, a = np.polyfit(np.log(ns), np.log(radii), 1)
-    return WarmStart(eng.params(), a, b), history
+    return WarmStart(eng.params(), a, b, pivots), history
 
 
-def init_params(seed=0):
-    """He-scaled normal weights, zero biases, head columns beyond the two
-    outputs zero (they stay zero: their gradients are zero)."""
+def init_params(seed=0, head=0.5):
+    """He-scaled normal weights, zero biases, the head scaled by `head`
+    (0 with a pivot-MDS skip term: the first proposals are the skip itself
+    and training learns the correction); head columns beyond the two
+    outputs stay zero (their gradients are zero)."""
     shapes = [(32, 64), (128, 64), (128, 64), (64, 16)]
     rng = np.random.default_rng(seed)
     parts = []
@@ -197,7 +248,7 @@ def init_params(seed=0):
         W = rng.normal(0.0, np.sqrt(2.0 / k), size=(k, n))
         if li == 3:
             W[:, 2:] = 0.0
-            W *= 0.5
+            W *= head
         parts.append(W.ravel())
         parts.append(np.zeros(n))
     return np.concatenate(parts).astype(np.float32)
@@ -231,8 +282,8 @@ def evaluate(ws, n, blocks, seed, plain_iters=500, chunk=10, pivots=64,
     The plain method starts from uniform [-1, 1]^2 positions and runs
     `plain_iters` steps; its stress then is the target (within 1%). Reported:
     the steps and embedder seconds each start needs to reach it, and the
-    network's own seconds (device features + forward + rescale), which the
-    warm start's wall time includes."""
+    network's own seconds (device features, pivot MDS, forward, rescale),
+    which the warm start's wall time includes."""
     m, e = sbm_connected(n, blocks, seed)
     off, adj = G.csr_from_edges(m, e)
     piv = np.random.default_rng(seed).choice(m, min(pivots, m), replace=False)
@@ -244,9 +295,10 @@ def evaluate(ws, n, blocks, seed, plain_iters=500, chunk=10, pivots=64,
     target = 1.01 * final
     reach_it, reach_sec = next((a, sec_p * a / max(it_p, 1)) for a, b in traj_p
                                if b <= target)
+    # the graph upload is not timed, as the embedder's own creation is not
     ws.positions(off, adj, seed=seed + 1)  # allocate and warm the engine
     t0 = time.perf_counter()
-    start = ws.positions(off, adj, seed=seed + 1)
+    start = ws.propose(seed=seed + 1)
     net_sec = time.perf_counter() - t0
     it_w, sec_w, traj_w = time_to_stress(off, adj, start, meter, target, plain_iters,
                                          chunk, precision)
@@ -271,6 +323,9 @@ def main():
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--pivots", type=int, default=0,
+                   help="pivot-MDS inputs and skip term from this many pivots (0: none)")
+    t.add_argument("--targets-from", choices=["random", "pmds"], default="random")
     ev = sub.add_parser("eval")
     ev.add_argument("--n", type=int, default=200000)
     ev.add_argument("--graphs", type=int, default=5)
@@ -286,10 +341,11 @@ def main():
         return
     if args.cmd == "train":
         t0 = time.time()
-        data = training_set(args.graphs, args.seed, iters=args.layout_iters)
+        data = training_set(args.graphs, args.seed, iters=args.layout_iters,
+                            start=args.targets_from)
         t1 = time.time()
         print(f"# {len(data)} FP64 layouts in {t1 - t0:.1f}s", flush=True)
-        ws, hist = train(data, args.epochs, lr=args.lr, seed=args.seed)
+        ws, hist = train(data, args.epochs, lr=args.lr, seed=args.seed, pivots=args.pivots)
         print(f"# trained in {time.time() - t1:.1f}s; loss {hist[0]:.4f} -> {hist[-1]:.4f}",
               flush=True)
         ws.save(args.out, history=np.array(hist), graphs=args.graphs, epochs=args.epochs,

@@ -13,7 +13,7 @@ otherwise draw at random):
                -> y  = h3 W4 + b4,   P = y[:, :2]
 
 with A the row-normalised adjacency (mean over neighbours; 0 for isolated
-vertices). Loss: L = |s Pc R - Tc|^2 / |Tc|^2 with Pc, Tc centred and (R, s)
+vertices); with pivots set, P = y[:, :2] + the pivot-MDS coordinates. Loss: L = |s Pc R - Tc|^2 / |Tc|^2 with Pc, Tc centred and (R, s)
 the orthogonal Procrustes fit (rotation or reflection) of Pc to Tc.
 
 `bf16=True` rounds weights, inputs, activations and back-propagated signals
@@ -92,7 +92,48 @@ def diffused_features(off, adj, start):
     return X
 
 
-def forward(theta, off, adj, X, rb=False):
+def pivot_mds(off, adj, pivots):
+    """Pivot MDS (Brandes & Pich) as gvizhip_gnn_set_pivots computes it: BFS
+    hop distances D to the pivots (unreached = farthest + 1), C = -1/2 D^2
+    double-centred (row means, column means, mean of the column means), the
+    top two eigenpairs (v, mu) of C^T C with the largest-magnitude component
+    of v positive, axes C v / mu^(1/4), centred and scaled to RMS radius 1."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+    n = len(off) - 1
+    A = sp.csr_matrix((np.ones(len(adj)), np.asarray(adj), np.asarray(off)), shape=(n, n))
+    D = cg.shortest_path(A, unweighted=True, indices=np.asarray(pivots)).T
+    fin = np.isfinite(D)
+    D[~fin] = (D[fin].max() if fin.any() else 0.0) + 1.0
+    D2 = D * D
+    colm = D2.mean(0)
+    C = -0.5 * (D2 - D2.mean(1)[:, None] - colm[None, :] + colm.mean())
+    w, V = np.linalg.eigh(C.T @ C)
+    X = np.zeros((n, 2))
+    for a, c in enumerate(np.argsort(w)[::-1][:2]):
+        if not w[c] > 0:
+            continue
+        v = V[:, c]
+        v = v if v[np.argmax(np.abs(v))] > 0 else -v
+        X[:, a] = C @ (v / w[c] ** 0.25)
+    X -= X.mean(0)
+    rms = np.sqrt((X * X).sum(1).mean())
+    return X / rms if rms > 0 else X
+
+
+def pivot_features(off, adj, start, pivots):
+    """gvizhip_gnn_set_graph_diffused then gvizhip_gnn_set_pivots: the
+    diffused features with the pivot-MDS coordinates in columns 12-13, and
+    those coordinates (the skip term)."""
+    X = diffused_features(off, adj, start)
+    P = pivot_mds(off, adj, pivots)
+    X[:, 12:14] = P
+    return X, P
+
+
+def forward(theta, off, adj, X, rb=False, skip=None):
+    """Head output y (n x 16); `skip` (n x 2, the pivot-MDS coordinates) is
+    added to outputs 0-1 after the head, as the device does."""
     r = bf16 if rb else (lambda a: np.asarray(a, dtype=np.float64))
     layers = unpack(np.asarray(theta, dtype=np.float64))
     h = r(X)
@@ -107,6 +148,8 @@ def forward(theta, off, adj, X, rb=False):
         h = hn
     W4, b4 = layers[3]
     y = h @ r(W4) + b4
+    if skip is not None:
+        y[:, :2] += skip
     return y, cache
 
 
@@ -125,12 +168,12 @@ def procrustes(P, T):
     return loss, R, s, Pc, Tc
 
 
-def loss_and_grads(theta, off, adj, X, T, rb=False):
+def loss_and_grads(theta, off, adj, X, T, rb=False, skip=None):
     """Loss and the gradient of every parameter (flat, like theta)."""
     r = bf16 if rb else (lambda a: np.asarray(a, dtype=np.float64))
     theta = np.asarray(theta, dtype=np.float64)
     layers = unpack(theta)
-    y, cache = forward(theta, off, adj, X, rb)
+    y, cache = forward(theta, off, adj, X, rb, skip)
     P = y[:, :2]
     loss, R, s, Pc, Tc = procrustes(P, T)
     tt = (Tc ** 2).sum()
@@ -160,15 +203,16 @@ def loss_and_grads(theta, off, adj, X, T, rb=False):
     return loss, flat
 
 
-def init_params(seed=0):
-    """He-scaled normal weights, zero biases; head columns 2..15 zero."""
+def init_params(seed=0, head=0.5):
+    """He-scaled normal weights, zero biases; head columns 2..15 zero and
+    the head scaled by `head` (0: the output starts as the skip term)."""
     rng = np.random.default_rng(seed)
     parts = []
     for li, (k, n) in enumerate(SHAPES):
         W = rng.normal(0.0, np.sqrt(2.0 / k), size=(k, n))
         if li == 3:
             W[:, 2:] = 0.0
-            W *= 0.5
+            W *= head
         parts.append(W.ravel())
         parts.append(np.zeros(n))
     return np.concatenate(parts)

@@ -68,3 +68,73 @@ def test_bf16_rounding_is_nearest_even():
     assert r[2] == 1.0 + 2 ** -7
     assert r[3] == -2.5
     assert abs(r[4] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+def _classical_mds(off, adj):
+    """Textbook classical MDS of all-pairs hop distances: B = -1/2 J D^2 J,
+    axes sqrt(lambda) u for the top two eigenpairs, centred, RMS radius 1."""
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+    n = len(off) - 1
+    A = sp.csr_matrix((np.ones(len(adj)), adj, off), shape=(n, n))
+    D = cg.shortest_path(A, unweighted=True)
+    J = np.eye(n) - 1.0 / n
+    B = -0.5 * J @ (D * D) @ J
+    w, U = np.linalg.eigh(B)
+    X = U[:, [-1, -2]] * np.sqrt(w[[-1, -2]])
+    X -= X.mean(0)
+    return X / np.sqrt((X * X).sum(1).mean())
+
+
+def test_pivot_mds_with_every_vertex_is_classical_mds():
+    """With all vertices as pivots C is the double-centred distance matrix
+    B, C^T C = B^2, and C v / mu^(1/4) = sqrt(lambda) u: classical MDS,
+    axis by axis up to sign."""
+    n, e = G.erdos_renyi(30, 4.0, 7)
+    act, inside = G.largest_component(n, e)
+    ids = np.flatnonzero(act)
+    idx = -np.ones(n, np.int64)
+    idx[ids] = np.arange(ids.size)
+    off, adj = G.csr_from_edges(ids.size, idx[inside])
+    P = N.pivot_mds(off, adj, np.arange(ids.size))
+    M = _classical_mds(off, adj)
+    for a in range(2):
+        s = np.sign(P[:, a] @ M[:, a])
+        assert np.abs(P[:, a] - s * M[:, a]).max() <= 1e-9
+
+
+def test_pivot_mds_of_a_cycle_is_a_regular_polygon():
+    """Hop distances of a 12-cycle embed classically as a regular 12-gon:
+    every radius is 1 (RMS radius 1) and neighbours sit 30 degrees apart."""
+    n = 12
+    e = np.array([[i, (i + 1) % n] for i in range(n)])
+    off, adj = G.csr_from_edges(n, e)
+    P = N.pivot_mds(off, adj, np.arange(n))
+    r = np.sqrt((P * P).sum(1))
+    assert np.abs(r - 1.0).max() <= 1e-9
+    ang = np.unwrap(np.arctan2(P[:, 1], P[:, 0]))
+    step = np.abs(np.diff(ang))
+    assert np.abs(step - 2 * np.pi / n).max() <= 1e-9
+
+
+def test_skip_term_enters_output_and_gradients():
+    """The skip is added to the head's outputs; finite differences still
+    match the backward pass with it."""
+    off, adj, X, T = _small(6)
+    skip = np.random.default_rng(8).normal(size=(len(off) - 1, 2))
+    theta = N.init_params(4)
+    y0 = N.forward(theta, off, adj, X)[0]
+    y1 = N.forward(theta, off, adj, X, skip=skip)[0]
+    assert np.abs(y1[:, :2] - y0[:, :2] - skip).max() <= 1e-12
+    assert np.array_equal(y1[:, 2:], y0[:, 2:])
+    loss, grad = N.loss_and_grads(theta, off, adj, X, T, skip=skip)
+    rng = np.random.default_rng(2)
+    live = np.flatnonzero(np.abs(grad) > 0)
+    h = 1e-6
+    for i in rng.choice(live, size=30, replace=False):
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (N.loss_and_grads(tp, off, adj, X, T, skip=skip)[0] -
+              N.loss_and_grads(tm, off, adj, X, T, skip=skip)[0]) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, grad[i])

@@ -141,3 +141,55 @@ def test_trained_warm_start_gains_over_random_start(gpu):
               f"plain {plain[1][1]:.4f}/{plain[2][1]:.4f} warm {warm[1][1]:.4f}/{warm[2][1]:.4f}")
         wins += warm[2][1] < plain[2][1]
     assert wins == 2
+
+
+def _connected_sbm(n, blocks, seed):
+    m, e = W.sbm_connected(n, blocks, seed)
+    off, adj = G.csr_from_edges(m, e)
+    return m, off, adj
+
+
+@pytest.mark.parametrize("shape", ["grid", "sbm"])
+def test_pivot_mds_matches_oracle(gpu, shape):
+    """Device BFS from 64 pivots, double centring, Gram matrix and Jacobi
+    eigenpairs against the oracle's FP64 restatement (coordinates are
+    returned in FP32)."""
+    if shape == "grid":
+        n, e = G.grid(30, 45)
+        off, adj = G.csr_from_edges(n, e)
+    else:
+        n, off, adj = _connected_sbm(3000, 3, 21)
+    piv = W.pick_pivots(n, 5)
+    eng = gpu.GnnEngine(n, len(adj))
+    eng.set_graph_diffused(off, adj, G.uniform_positions(n, 1))
+    P = eng.set_pivots(piv)
+    R = N.pivot_mds(off, adj, piv)
+    print(f"{shape}: max |device - oracle| {np.abs(P - R).max():.2e}")
+    assert np.abs(P - R).max() <= 1e-4
+
+
+def test_forward_and_gradients_with_skip_match_oracle(gpu):
+    n, off, adj = _connected_sbm(1500, 4, 8)
+    start = G.uniform_positions(n, 2)
+    piv = W.pick_pivots(n, 3)
+    th = _theta(7)
+    eng = gpu.GnnEngine(n, len(adj))
+    eng.set_params(th)
+    eng.set_graph_diffused(off, adj, start)
+    P = eng.set_pivots(piv)
+    X = N.diffused_features(off, adj, start)
+    X[:, 12:14] = P
+    y_gpu = eng.forward()
+    y_ref = N.forward(th.astype(np.float64), off, adj, X, rb=True, skip=P)[0][:, :2]
+    assert _norm_err(y_gpu, y_ref) <= 2e-2, _norm_err(y_gpu, y_ref)
+    target = W.force_layout(off, adj, seed=1, iters=60)
+    loss, g = eng.gradients(target)
+    loss_ref, g_ref = N.loss_and_grads(th.astype(np.float64), off, adj, X, target,
+                                       rb=True, skip=P)
+    assert abs(loss - loss_ref) <= 2e-2 * max(abs(loss_ref), 1e-3), (loss, loss_ref)
+    cos = float(g @ g_ref / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
+    assert cos >= 0.99, cos
+    # removing the pivots removes the skip
+    eng.set_pivots([])
+    y0 = eng.forward()
+    assert _norm_err(y_gpu - y0, P) <= 1e-2
