@@ -23,6 +23,8 @@ constexpr int kTB = 256;            // threads per block
 constexpr int kTW = kTB / kWave;    // waves per block
 constexpr int kStack = 256;         // traversal stack entries per wave
 constexpr int kHeavyDegree = 32;    // rows above this get a wavefront each
+constexpr int kAllPairsRows = 16;   // most source partitions of the all-pairs launch
+constexpr int64_t kAllPairsWaves = 16384; // wavefronts the all-pairs launch aims for
 
 // Direction for exactly coincident points: splitmix64 finaliser over the two
 // vertex ids mapped to an angle. The reference hashes buffer addresses here
@@ -714,12 +716,17 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
 }
 
 // Exact all-pairs repulsion (src/embedders/gvizForceEmbedder.c:104-115):
-// sources stream through LDS in tiles of 256, in ascending id order.
+// sources stream through LDS in tiles of 256, in ascending id order. Block
+// row y takes the sources [y * span, (y + 1) * span): with one row (FP64)
+// every target sums all sources in the reference's order straight into
+// `out`; with several (FP32 force math) each row writes its partial sum to
+// out + y * n and k_allpairs_sum adds the rows in order, which gives the
+// launch gridDim.y times more wavefronts than there are target tiles.
 template <class Real, class PosT, int MODEL>
 __global__ __launch_bounds__(kTB) void k_allpairs(
     const typename V2<PosT>::type *__restrict__ pos,
-    const Real *__restrict__ massT, const Real *__restrict__ radT, int64_t n,
-    Real kSq, Real oc, typename V2<Real>::type *__restrict__ rep,
+    const Real *__restrict__ massT, const Real *__restrict__ radT, int64_t n, int64_t span,
+    Real kSq, Real oc, typename V2<Real>::type *__restrict__ out,
     int32_t *__restrict__ inter, const double *__restrict__ r64,
     const Scalars *__restrict__ sc) {
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
@@ -734,15 +741,17 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
   const Real mv = valid ? massT[i] : (Real)0;
   const Real rv = valid ? radT[i] : (Real)0;
   Real ax = 0, ay = 0;
-  for (int64_t jb = 0; jb < n; jb += kTB) {
+  const int64_t j0 = (int64_t)blockIdx.y * span;
+  const int64_t j1 = j0 + span < n ? j0 + span : n;
+  for (int64_t jb = j0; jb < j1; jb += kTB) {
     const int64_t j = jb + threadIdx.x;
-    if (j < n) {
+    if (j < j1) {
       tPos[threadIdx.x] = pos[j];
       tMass[threadIdx.x] = massT[j];
       tRad[threadIdx.x] = radT[j];
     }
     __syncthreads();
-    const int len = (int)((n - jb) < kTB ? (n - jb) : kTB);
+    const int len = (int)((j1 - jb) < kTB ? (j1 - jb) : kTB);
     if (valid) {
       for (int t = 0; t < len; t++) {
         const int64_t jj = jb + t;
@@ -760,15 +769,29 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
     __syncthreads();
   }
   if (valid) {
-    rep[i] = {ax, ay};
-    if (inter)
+    out[(int64_t)blockIdx.y * n + i] = {ax, ay};
+    if (inter && blockIdx.y == 0)
       inter[i] = (int32_t)(n - 1);
   }
 }
 
-// Attraction of high-degree vertices: one wavefront per row computes 64 edge
-// terms at a time and adds them to a running sum in adjacency order, so the
-// sum rounds exactly as the reference's sequential loop does.
+// rep[i] = sum of the `parts` partial rows, in row order.
+template <class Real>
+__global__ __launch_bounds__(kTB) void k_allpairs_sum(
+    const typename V2<Real>::type *__restrict__ part, int64_t n, int parts,
+    typename V2<Real>::type *__restrict__ rep, const Scalars *__restrict__ sc) {
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i >= n || sc->stopped)
+    return;
+  Real ax = 0, ay = 0;
+  for (int p = 0; p < parts; p++) {
+    const typename V2<Real>::type v = part[(int64_t)p * n + i];
+    ax += v.x;
+    ay += v.y;
+  }
+  rep[i] = {ax, ay};
+}
+
 template <class Real, class PosT, int MODEL>
 __global__ __launch_bounds__(kTB) void k_att_heavy(
     const typename V2<PosT>::type *__restrict__ pos,
@@ -1069,16 +1092,31 @@ int typed_step(ForceCommon &c, int mode) {
     if (fork_attraction<Real, PosT>(c, posC, model))
       return -1;
     const unsigned tb = (unsigned)((n + kTB - 1) / kTB);
+    // FP32 force math: split the sources over up to kAllPairsRows block rows
+    // (partials in the quadtree record buffer, which this mode leaves
+    // unused), so small target counts still fill the chip; FP64 keeps one
+    // row and the reference's summation order.
+    int rows = 1;
+    if (sizeof(Real) == sizeof(float)) {
+      const int64_t fit = (int64_t)(c.nodes.n / (sizeof(R2) * (size_t)n));
+      const int64_t want = (kAllPairsWaves + (int64_t)tb * kTW - 1) / ((int64_t)tb * kTW);
+      rows = (int)std::max<int64_t>(1, std::min<int64_t>({want, fit, kAllPairsRows}));
+    }
+    const int64_t span = ((n + rows - 1) / rows + kTB - 1) / kTB * kTB;
+    R2 *apOut = rows == 1 ? (R2 *)c.rep.p : (R2 *)c.nodes.p;
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
     if (model == 1)
-      hipLaunchKernelGGL((k_allpairs<Real, PosT, 1>), dim3(tb), dim3(kTB), 0, st,
-                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
-                         kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p, c.sc.p);
+      hipLaunchKernelGGL((k_allpairs<Real, PosT, 1>), dim3(tb, rows), dim3(kTB), 0, st,
+                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n, span,
+                         kSq, oc, apOut, inter, c.radius64.p, c.sc.p);
     else
-      hipLaunchKernelGGL((k_allpairs<Real, PosT, 0>), dim3(tb), dim3(kTB), 0, st,
-                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n,
-                         kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p, c.sc.p);
+      hipLaunchKernelGGL((k_allpairs<Real, PosT, 0>), dim3(tb, rows), dim3(kTB), 0, st,
+                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n, span,
+                         kSq, oc, apOut, inter, c.radius64.p, c.sc.p);
+    if (rows > 1)
+      hipLaunchKernelGGL(k_allpairs_sum<Real>, dim3(tb), dim3(kTB), 0, st,
+                         (const R2 *)c.nodes.p, n, rows, (R2 *)c.rep.p, c.sc.p);
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
