@@ -8,6 +8,11 @@
 // Each sweep's max displacement is reduced per block and finished by a
 // one-block kernel that records it and raises the stop flag. Compiled with
 // -ffp-contract=off: in FP64 every sum and product rounds as the reference's.
+//
+// Vertices are stored in breadth-first order (relabelled at create, mapped
+// back at upload/download), so the neighbours a thread gathers sit close in
+// memory. Each row keeps its insertion order, and the sweep's only
+// reduction is a max, so the results are those of the original order.
 #include "../../include/gviz_hip.h"
 #include "common.hpp"
 
@@ -136,6 +141,7 @@ struct gvizhip_tutte {
   hipGraph_t graph = nullptr;
   double graphAlpha = -1.0;
   TutteScalars *pinned = nullptr;
+  std::vector<int32_t> perm; // stored position -> caller's vertex id
 };
 
 namespace {
@@ -225,18 +231,54 @@ int gvizhip_tutte_create(gvizhip_tutte **out, int64_t n, const int64_t *offsets,
     set_error("gvizhip_tutte_create: graph exceeds 2^31 vertices or entries");
     return -1;
   }
-  std::vector<int32_t> o32(n + 1), a32(m2 > 0 ? m2 : 1);
-  for (int64_t i = 0; i <= n; i++)
-    o32[i] = n > 0 ? (int32_t)offsets[i] : 0;
   for (int64_t e = 0; e < m2; e++) {
     if (adj[e] < 0 || adj[e] >= n) {
       set_error("gvizhip_tutte_create: adjacency entry %lld out of range",
                 (long long)e);
       return -1;
     }
-    a32[e] = (int32_t)adj[e];
+  }
+  for (int64_t v = 0; v < n; v++) {
+    if (offsets[v + 1] < offsets[v]) {
+      set_error("gvizhip_tutte_create: offsets are not monotone");
+      return -1;
+    }
+  }
+  // breadth-first order over the components in ascending id of their first
+  // vertex; neighbours are enqueued in row order
+  std::vector<int32_t> perm(n), inv(n, -1);
+  {
+    int64_t head = 0, tail = 0;
+    for (int64_t r = 0; r < n; r++) {
+      if (inv[r] >= 0)
+        continue;
+      inv[r] = (int32_t)tail;
+      perm[tail++] = (int32_t)r;
+      while (head < tail) {
+        const int32_t u = perm[head++];
+        for (int64_t e = offsets[u]; e < offsets[u + 1]; e++) {
+          const int64_t w = adj[e];
+          if (inv[w] < 0) {
+            inv[w] = (int32_t)tail;
+            perm[tail++] = (int32_t)w;
+          }
+        }
+      }
+    }
+  }
+  // relabelled CSR: stored vertex i is the caller's perm[i], its row in the
+  // original entry order with every neighbour renamed
+  std::vector<int32_t> o32(n + 1), a32(m2 > 0 ? m2 : 1);
+  o32[0] = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t u = perm[i];
+    int64_t at = o32[i];
+    for (int64_t e = offsets[u]; e < offsets[u + 1]; e++)
+      a32[at++] = inv[adj[e]];
+    o32[i + 1] = (int32_t)at;
   }
   gvizhip_tutte *t = new gvizhip_tutte();
+  t->perm = std::move(perm);
   t->n = n;
   t->m2 = m2;
   t->precision = precision == GVIZHIP_PRECISION_FP64 ? GVIZHIP_PRECISION_FP64
@@ -281,20 +323,32 @@ int gvizhip_tutte_upload(gvizhip_tutte *t, const double *positions,
   const int64_t n = t->n;
   if (n == 0)
     return 0;
+  const std::vector<int32_t> &perm = t->perm;
   if (positions) {
     if (t->precision == GVIZHIP_PRECISION_FP64) {
-      GVH_CHECK(hipMemcpy(t->pos[0].p, positions, sizeof(double) * 2 * n,
+      std::vector<double> d(2 * n);
+      for (int64_t i = 0; i < n; i++) {
+        d[2 * i] = positions[2 * (int64_t)perm[i]];
+        d[2 * i + 1] = positions[2 * (int64_t)perm[i] + 1];
+      }
+      GVH_CHECK(hipMemcpy(t->pos[0].p, d.data(), sizeof(double) * 2 * n,
                           hipMemcpyHostToDevice));
     } else {
       std::vector<float> f(2 * n);
-      for (int64_t i = 0; i < 2 * n; i++)
-        f[i] = (float)positions[i];
+      for (int64_t i = 0; i < n; i++) {
+        f[2 * i] = (float)positions[2 * (int64_t)perm[i]];
+        f[2 * i + 1] = (float)positions[2 * (int64_t)perm[i] + 1];
+      }
       GVH_CHECK(hipMemcpy(t->pos[0].p, f.data(), sizeof(float) * 2 * n,
                           hipMemcpyHostToDevice));
     }
   }
-  if (isBoundary)
-    GVH_CHECK(hipMemcpy(t->boundary.p, isBoundary, n, hipMemcpyHostToDevice));
+  if (isBoundary) {
+    std::vector<uint8_t> b(n);
+    for (int64_t i = 0; i < n; i++)
+      b[i] = isBoundary[perm[i]];
+    GVH_CHECK(hipMemcpy(t->boundary.p, b.data(), n, hipMemcpyHostToDevice));
+  }
   TutteScalars s{};
   s.epsilon = -1.0;
   GVH_CHECK(hipMemcpy(t->sc.p, &s, sizeof(s), hipMemcpyHostToDevice));
@@ -313,15 +367,23 @@ int gvizhip_tutte_download(gvizhip_tutte *t, double *positions) {
   if (read_sc(t, &s))
     return -1;
   const int cur = (int)(s.sweeps & 1);
+  const std::vector<int32_t> &perm = t->perm;
   if (t->precision == GVIZHIP_PRECISION_FP64) {
-    GVH_CHECK(hipMemcpy(positions, t->pos[cur].p, sizeof(double) * 2 * n,
+    std::vector<double> d(2 * n);
+    GVH_CHECK(hipMemcpy(d.data(), t->pos[cur].p, sizeof(double) * 2 * n,
                         hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      positions[2 * (int64_t)perm[i]] = d[2 * i];
+      positions[2 * (int64_t)perm[i] + 1] = d[2 * i + 1];
+    }
   } else {
     std::vector<float> f(2 * n);
     GVH_CHECK(hipMemcpy(f.data(), t->pos[cur].p, sizeof(float) * 2 * n,
                         hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < 2 * n; i++)
-      positions[i] = (double)f[i];
+    for (int64_t i = 0; i < n; i++) {
+      positions[2 * (int64_t)perm[i]] = (double)f[2 * i];
+      positions[2 * (int64_t)perm[i] + 1] = (double)f[2 * i + 1];
+    }
   }
   return 0;
 }
