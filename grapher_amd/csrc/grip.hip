@@ -19,6 +19,7 @@
 #include "radix_sort.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <deque>
 #include <vector>
 
@@ -175,6 +176,202 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
     atomicAnd(&bits[w >> 5], 0u);
   }
   return found;
+}
+
+// ---- block-cooperative search for calls with few queries: one block of
+// kBB threads takes kBB (queued vertex, adjacency entry) pairs per step of
+// the same FIFO stream wave_bfs walks 64 at a time. A vertex is fresh at its
+// first pair of the step (the smallest pair index, found through an LDS hash
+// table) when no earlier step visited it, so the fresh vertices, their queue
+// order and the hits are exactly wave_bfs's; only the step size differs.
+constexpr int kBB = 512;
+constexpr int kBW = kBB / kWave;
+constexpr int kHash = 2 * kBB;
+
+// Inclusive block scan of one int per thread; `total` gets the block sum.
+__device__ __forceinline__ int block_scan(int v, int *sWave, int &total) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int x = __shfl_up(incl, o, kWave);
+    if (lane >= o)
+      incl += x;
+  }
+  if (lane == kWave - 1)
+    sWave[wave] = incl;
+  __syncthreads();
+  int pre = 0, sum = 0;
+#pragma unroll
+  for (int w = 0; w < kBW; w++) {
+    const int x = sWave[w];
+    pre += w < wave ? x : 0;
+    sum += x;
+  }
+  __syncthreads();
+  total = sum;
+  return incl + pre;
+}
+
+__device__ int block_bfs(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                         const uint8_t *__restrict__ filter, int32_t source, int k,
+                         int maxDepth, bool lowerOnly, bool stopAtK, int64_t filterTotal,
+                         uint32_t *bits, int32_t *queue, int32_t *qdist, int32_t *outV,
+                         int32_t *outD) {
+  __shared__ int32_t sIncl[kBB], sStart[kBB], sDepth[kBB];
+  __shared__ int32_t hKey[kHash], hVal[kHash];
+  __shared__ int sWave[kBW];
+  __shared__ int sLast;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    atomicOr(&bits[source >> 5], 1u << (source & 31));
+    st_l2(queue, source);
+    st_l2(qdist, 0);
+  }
+  __syncthreads();
+  const int64_t stopCount =
+      filterTotal > 0 ? filterTotal - ((filter && filter[source]) ? 1 : 0) : -1;
+  int head = 0, tail = 1, found = 0, cur = 0;
+  bool done = stopCount == 0;
+  while (head < tail && !done) {
+    const int avail = tail - head < kBB ? tail - head : kBB;
+    int32_t start = 0, deg = 0, du = 0;
+    if (t < avail) {
+      const int32_t u = ld_l2(queue + head + t);
+      du = ld_l2(qdist + head + t);
+      if (!(maxDepth > 0 && du >= maxDepth)) {
+        start = off[u] + (t == 0 ? cur : 0);
+        deg = off[u + 1] - start;
+      }
+    }
+    int total;
+    const int incl = block_scan(deg, sWave, total);
+    sIncl[t] = incl;
+    sStart[t] = start;
+    sDepth[t] = du;
+    for (int i = t; i < kHash; i += kBB) {
+      hKey[i] = -1;
+      hVal[i] = INT32_MAX;
+    }
+    __syncthreads();
+    const int pairs = total < kBB ? total : kBB;
+    const bool valid = t < pairs;
+    int32_t w = -1, ownDepth = 0;
+    int owner = 0;
+    if (valid) {
+      // owner = first queued vertex whose inclusive sum exceeds t
+      int lo = 0, hi = avail - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sIncl[mid] > t)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      owner = lo;
+      const int excl = owner ? sIncl[owner - 1] : 0;
+      w = adj[sStart[owner] + (t - excl)];
+      ownDepth = sDepth[owner];
+      // smallest pair index per vertex (open addressing)
+      uint32_t h = ((uint32_t)w * 2654435761u) & (kHash - 1);
+      while (true) {
+        const int32_t prev = atomicCAS(&hKey[h], -1, w);
+        if (prev == -1 || prev == w) {
+          atomicMin(&hVal[h], t);
+          break;
+        }
+        h = (h + 1) & (kHash - 1);
+      }
+    }
+    if (t == pairs - 1)
+      sLast = owner;
+    __syncthreads();
+    bool fresh = false;
+    if (valid) {
+      uint32_t h = ((uint32_t)w * 2654435761u) & (kHash - 1);
+      while (hKey[h] != w)
+        h = (h + 1) & (kHash - 1);
+      if (hVal[h] == t) {
+        const uint32_t bit = 1u << (w & 31);
+        fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
+      }
+    }
+    const bool pass = fresh && (!filter || filter[w]) && (!lowerOnly || w < source);
+    int counts;
+    const int packed = block_scan((fresh ? 1 : 0) | (pass ? 1 << 16 : 0), sWave, counts);
+    const int exFresh = (packed & 0xFFFF) - (fresh ? 1 : 0);
+    const int exPass = (packed >> 16) - (pass ? 1 : 0);
+    if (pass) {
+      const int slot = found + exPass;
+      if (slot < k) {
+        outV[slot] = w;
+        outD[slot] = ownDepth + 1;
+      }
+    }
+    if (fresh) {
+      st_l2(queue + tail + exFresh, w);
+      st_l2(qdist + tail + exFresh, ownDepth + 1);
+    }
+    found += counts >> 16;
+    tail += counts & 0xFFFF;
+    // advance the cursor past the pairs taken
+    if (pairs == total) {
+      head += avail;
+      cur = 0;
+    } else {
+      const int last = sLast;
+      const int32_t lastIncl = sIncl[last];
+      const int32_t lastDeg = lastIncl - (last ? sIncl[last - 1] : 0);
+      const int32_t took = lastDeg - (lastIncl - pairs);
+      if (took == lastDeg) {
+        head += last + 1;
+        cur = 0;
+      } else {
+        cur = (last == 0 ? cur : 0) + took;
+        head += last;
+      }
+    }
+    if (stopAtK && found >= k)
+      done = true;
+    if (stopCount > 0 && found >= stopCount)
+      done = true;
+    __syncthreads();
+  }
+  // clear this query's bits: every visited vertex sits in the queue
+  for (int i = t; i < tail; i += kBB) {
+    const int32_t w = ld_l2(queue + i);
+    atomicAnd(&bits[w >> 5], 0u);
+  }
+  __syncthreads();
+  return found;
+}
+
+// Persistent kernel, one block per slot: block `slot` serves queries slot,
+// slot + S, ... with block_bfs.
+__global__ __launch_bounds__(kBB) void k_bfs_queries_block(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const uint8_t *__restrict__ filter, const int32_t *__restrict__ sources,
+    const int32_t *__restrict__ outIndex, int64_t count, int k, int64_t stride,
+    int maxDepth, int lowerOnly, int stopAtK, int64_t filterTotal, BfsSlots sl,
+    int64_t slots, int32_t *__restrict__ outV, int32_t *__restrict__ outD,
+    int32_t *__restrict__ outCount, int32_t *__restrict__ overflow) {
+  const int64_t slot = blockIdx.x;
+  if (slot >= slots)
+    return;
+  uint32_t *bits = sl.bits + slot * sl.words;
+  int32_t *q = sl.queue + slot * sl.cap;
+  int32_t *qd = sl.qdist + slot * sl.cap;
+  for (int64_t qi = slot; qi < count; qi += slots) {
+    const int32_t src = sources[qi];
+    const int64_t o = (outIndex ? (int64_t)outIndex[qi] : qi) * stride;
+    const int found = block_bfs(off, adj, filter, src, k, maxDepth, lowerOnly != 0,
+                                stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o);
+    if (threadIdx.x == 0) {
+      outCount[outIndex ? outIndex[qi] : qi] = found < k ? found : k;
+      if (!stopAtK && found > k)
+        atomicOr(overflow, 1);
+    }
+  }
 }
 
 // Persistent kernel: wavefront `slot` serves queries slot, slot + S, ...
@@ -712,11 +909,21 @@ struct gvizhip_grip {
   RadixSorter sorter;
   int64_t slots = 0;
   int64_t filtCount = 0;
+  int64_t blockQueries = -1; // see kBlockQueries
 };
 
 namespace {
 
+// calls with fewer searches than this give each search a block (the
+// environment variable GVIZHIP_GRIP_BLOCK_QUERIES overrides it per engine;
+// 0 keeps every search on one wavefront)
+constexpr int64_t kBlockQueries = 2048;
+
 int ensure_slots(gvizhip_grip *g) {
+  if (g->blockQueries < 0) {
+    const char *e = getenv("GVIZHIP_GRIP_BLOCK_QUERIES");
+    g->blockQueries = e ? std::max<int64_t>(0, atoll(e)) : kBlockQueries;
+  }
   if (g->slots)
     return 0;
   const int64_t words = (g->n + 31) / 32 + 1;
@@ -764,6 +971,7 @@ int upload_visible(gvizhip_grip *g, const uint8_t *visible) {
 
 // BFS queries on the device: sources (device int32), results into outV/outD
 // rows of `stride` selected by outIndex (device, may be null).
+
 int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
                 int64_t count, int k, int64_t stride, const uint8_t *filterDev,
                 int64_t filterTotal, int maxDepth, bool lowerOnly, bool stopAtK,
@@ -772,6 +980,16 @@ int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
     return 0;
   if (ensure_slots(g))
     return -1;
+  if (count < g->blockQueries) {
+    // few searches (the coarse layers): a block per search
+    const int64_t nb = std::min<int64_t>(g->slots, count);
+    hipLaunchKernelGGL(k_bfs_queries_block, dim3((unsigned)nb), dim3(kBB), 0, g->stream,
+                       g->off.p, g->adj.p, filterDev, srcDev, outIndex, count, k, stride,
+                       maxDepth, lowerOnly ? 1 : 0, stopAtK ? 1 : 0, filterTotal,
+                       slot_view(g), nb, outV, outD, outCount, g->flags.p);
+    GVH_CHECK_LAUNCH();
+    return 0;
+  }
   const int64_t waves = std::min<int64_t>(g->slots, count);
   const unsigned blocks = (unsigned)((waves + kWpb - 1) / kWpb);
   hipLaunchKernelGGL(k_bfs_queries, dim3(blocks), dim3(kB), 0, g->stream,

@@ -11,6 +11,17 @@ from grapher_amd import graphs as G
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["block", "wave"])
+def search_mode(request, monkeypatch):
+    """Every test runs twice: searches of calls with few queries on one block
+    each (the default), and every search on one wavefront."""
+    if request.param == "wave":
+        monkeypatch.setenv("GVIZHIP_GRIP_BLOCK_QUERIES", "0")
+    else:
+        monkeypatch.delenv("GVIZHIP_GRIP_BLOCK_QUERIES", raising=False)
+    return request.param
+
+
 def _dec_from(d0, n):
     from grapher_amd import native
     dec = np.zeros(n, dtype=native.GRIP_VERTEX_DTYPE)
