@@ -184,7 +184,10 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
 // first pair of the step (the smallest pair index, found through an LDS hash
 // table) when no earlier step visited it, so the fresh vertices, their queue
 // order and the hits are exactly wave_bfs's; only the step size differs.
-constexpr int kBB = 512;
+#ifndef GVH_GRIP_BB
+#define GVH_GRIP_BB 1024
+#endif
+constexpr int kBB = GVH_GRIP_BB;
 constexpr int kBW = kBB / kWave;
 constexpr int kHash = 2 * kBB;
 
@@ -929,7 +932,11 @@ int ensure_slots(gvizhip_grip *g) {
   const int64_t words = (g->n + 31) / 32 + 1;
   const int64_t cap = g->n + 1;
   const double perSlot = (double)words * 4 + (double)cap * 8;
-  const double budget = 12.0 * (1ull << 30);
+  // search scratch: up to a quarter of the free device memory, at most 48 GiB
+  size_t freeB = 0, totalB = 0;
+  double budget = 12.0 * (1ull << 30);
+  if (hipMemGetInfo(&freeB, &totalB) == hipSuccess && freeB > 0)
+    budget = std::min(48.0 * (1ull << 30), 0.25 * (double)freeB);
   int64_t s = (int64_t)std::min(1024.0, std::max(4.0, budget / perSlot));
   s = (s / kWpb) * kWpb;
   if (s < kWpb)
