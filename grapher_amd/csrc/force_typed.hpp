@@ -722,7 +722,13 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
 // `out`; with several (FP32 force math) each row writes its partial sum to
 // out + y * n and k_allpairs_sum adds the rows in order, which gives the
 // launch gridDim.y times more wavefronts than there are target tiles.
-template <class Real, class PosT, int MODEL>
+// Only the tile holding the block's own targets tests for the self pair.
+// OVERLAP false (every radius 0) with FP32 force math: the gap is the
+// distance, so mag times the unit vector is num * d / |d|^2, one reciprocal
+// per pair; num is formed once per source as it enters LDS (FR:
+// k^2 * trunc(mass); LinLog: the source's mass, the target's own mass
+// multiplied in once at the end).
+template <class Real, class PosT, int MODEL, bool OVERLAP>
 __global__ __launch_bounds__(kTB) void k_allpairs(
     const typename V2<PosT>::type *__restrict__ pos,
     const Real *__restrict__ massT, const Real *__restrict__ radT, int64_t n, int64_t span,
@@ -730,9 +736,10 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
     int32_t *__restrict__ inter, const double *__restrict__ r64,
     const Scalars *__restrict__ sc) {
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
+  constexpr bool SHORT = !OVERLAP && sizeof(Real) == sizeof(float);
   __shared__ typename V2<PosT>::type tPos[kTB];
   __shared__ Real tMass[kTB];
-  __shared__ Real tRad[kTB];
+  __shared__ Real tRad[SHORT ? 1 : kTB];
   if (sc->stopped)
     return;
   const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
@@ -741,34 +748,87 @@ __global__ __launch_bounds__(kTB) void k_allpairs(
   const Real mv = valid ? massT[i] : (Real)0;
   const Real rv = valid ? radT[i] : (Real)0;
   Real ax = 0, ay = 0;
+  auto pair = [&](int t, int64_t jj) {
+    const PosT ex = tPos[t].x - me.x, ey = tPos[t].y - me.y;
+    Real dx = (Real)ex, dy = (Real)ey;
+    Real inv;
+    Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+    Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, rv + tRad[t], mv, tMass[t], kSq,
+                                          oc, i, jj, ax, ay, ex, ey, r64, n);
+  };
+  // SHORT: d2 of source t. The main loop has no test: a pair with d2 == 0
+  // (the target itself, exactly coincident points) gives inf * 0 = NaN, so
+  // a lane whose tile sum is not finite redoes that tile pair by pair from
+  // the sums before it, with the coincident rule for d2 <= 0.
+  auto d2_of = [&](int t, Real &dx, Real &dy) {
+    const PosT ex = tPos[t].x - me.x, ey = tPos[t].y - me.y;
+    dx = (Real)ex;
+    dy = (Real)ey;
+    return dx * dx + dy * dy;
+  };
+  const int64_t own = (int64_t)blockIdx.x * kTB;
   const int64_t j0 = (int64_t)blockIdx.y * span;
   const int64_t j1 = j0 + span < n ? j0 + span : n;
   for (int64_t jb = j0; jb < j1; jb += kTB) {
     const int64_t j = jb + threadIdx.x;
     if (j < j1) {
       tPos[threadIdx.x] = pos[j];
-      tMass[threadIdx.x] = massT[j];
-      tRad[threadIdx.x] = radT[j];
+      if constexpr (SHORT) {
+        tMass[threadIdx.x] = MODEL == 1 ? massT[j] : kSq * truncf(massT[j]);
+      } else {
+        tMass[threadIdx.x] = massT[j];
+        tRad[threadIdx.x] = radT[j];
+      }
     }
     __syncthreads();
     const int len = (int)((j1 - jb) < kTB ? (j1 - jb) : kTB);
     if (valid) {
-      for (int t = 0; t < len; t++) {
-        const int64_t jj = jb + t;
-        if (jj == i)
-          continue;
-        const PosT ex = tPos[t].x - me.x, ey = tPos[t].y - me.y;
-        Real dx = (Real)ex, dy = (Real)ey;
-        Real inv;
-        Real dist = Phys<Real>::dist_inv(dx, dy, inv);
-        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, rv + tRad[t], mv,
-                                              tMass[t], kSq, oc, i, jj, ax, ay, ex,
-                                              ey, r64, n);
+      if constexpr (SHORT) {
+        const Real bx = ax, by = ay;
+#pragma unroll 8
+        for (int t = 0; t < len; t++) {
+          Real dx, dy;
+          const Real d2 = d2_of(t, dx, dy);
+          const Real s = tMass[t] * __builtin_amdgcn_rcpf(d2);
+          ax -= s * dx;
+          ay -= s * dy;
+        }
+        if (!__builtin_isfinite(ax + ay)) {
+          ax = bx;
+          ay = by;
+          for (int t = 0; t < len; t++) {
+            Real dx, dy;
+            const Real d2 = d2_of(t, dx, dy);
+            if (d2 > (Real)0) {
+              const Real s = tMass[t] * __builtin_amdgcn_rcpf(d2);
+              ax -= s * dx;
+              ay -= s * dy;
+            } else if (jb + t != i) {
+              const float2 u = coincident_dir_f(i, jb + t);
+              const Real mag = tMass[t] * oc;
+              ax -= mag * u.x;
+              ay -= mag * u.y;
+            }
+          }
+        }
+      } else if (jb == own) {
+        for (int t = 0; t < len; t++) {
+          if (jb + t == i)
+            continue;
+          pair(t, jb + t);
+        }
+      } else {
+        for (int t = 0; t < len; t++)
+          pair(t, jb + t);
       }
     }
     __syncthreads();
   }
   if (valid) {
+    if (SHORT && MODEL == 1) {
+      ax *= mv;
+      ay *= mv;
+    }
     out[(int64_t)blockIdx.y * n + i] = {ax, ay};
     if (inter && blockIdx.y == 0)
       inter[i] = (int32_t)(n - 1);
@@ -1106,14 +1166,25 @@ int typed_step(ForceCommon &c, int mode) {
     R2 *apOut = rows == 1 ? (R2 *)c.rep.p : (R2 *)c.nodes.p;
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
-    if (model == 1)
-      hipLaunchKernelGGL((k_allpairs<Real, PosT, 1>), dim3(tb, rows), dim3(kTB), 0, st,
-                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n, span,
-                         kSq, oc, apOut, inter, c.radius64.p, c.sc.p);
-    else
-      hipLaunchKernelGGL((k_allpairs<Real, PosT, 0>), dim3(tb, rows), dim3(kTB), 0, st,
-                         posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n, span,
-                         kSq, oc, apOut, inter, c.radius64.p, c.sc.p);
+    // radii are all 0 without overlap prevention; FP64 keeps the
+    // reference's formulas either way
+    const bool ov = c.prm.preventOverlap != 0 || sizeof(Real) == sizeof(double);
+#define GVH_ALLPAIRS(M, OV)                                                      \
+  hipLaunchKernelGGL((k_allpairs<Real, PosT, M, OV>), dim3(tb, rows), dim3(kTB), 0, st, \
+                     posC, (const Real *)c.massT.p, (const Real *)c.radT.p, n, span,   \
+                     kSq, oc, apOut, inter, c.radius64.p, c.sc.p)
+    if (model == 1) {
+      if (ov)
+        GVH_ALLPAIRS(1, true);
+      else
+        GVH_ALLPAIRS(1, false);
+    } else {
+      if (ov)
+        GVH_ALLPAIRS(0, true);
+      else
+        GVH_ALLPAIRS(0, false);
+    }
+#undef GVH_ALLPAIRS
     if (rows > 1)
       hipLaunchKernelGGL(k_allpairs_sum<Real>, dim3(tb), dim3(kTB), 0, st,
                          (const R2 *)c.nodes.p, n, rows, (R2 *)c.rep.p, c.sc.p);
