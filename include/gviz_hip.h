@@ -312,7 +312,6 @@ int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n,
  * gvizhip_gnn_set_graph_diffused does, without uploading the graph again;
  * removes the pivot-MDS skip term. */
 int gvizhip_gnn_set_start(gvizhip_gnn *g, const double *start);
-/* Network output (n x 2, interleaved) for the current graph. */
 /* Pivot MDS (Brandes & Pich) of the graph set last: BFS hop distances to
  * `count` (<= 64) pivot vertices, D^2 double-centred into C, axes C v / mu^(1/4)
  * for the top two eigenpairs (v, mu) of C^T C, centred and scaled to RMS
@@ -323,6 +322,7 @@ int gvizhip_gnn_set_start(gvizhip_gnn *g, const double *start);
  * gvizForceEmbedderBegin (src/embedders/gvizForceEmbedder.c:412-443). */
 int gvizhip_gnn_set_pivots(gvizhip_gnn *g, const int64_t *pivots, int64_t count,
                            double *coords);
+/* Network output (n x 2, interleaved) for the current graph. */
 int gvizhip_gnn_forward(gvizhip_gnn *g, double *positions);
 /* Loss against target (n x 2) and the gradient of every parameter. */
 int gvizhip_gnn_gradients(gvizhip_gnn *g, const double *target, double *loss,
