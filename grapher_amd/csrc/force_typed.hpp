@@ -380,6 +380,56 @@ template <class Real, class PosT> struct alignas(16) TravBody {
   int32_t id;
 };
 
+// FP32 pair term of the traversal with the target's own factor left out
+// (LinLog: the target's mass, multiplied in once at the end) and no guard
+// for d2 == 0. num is the source's part of the numerator (FR:
+// k^2 * trunc(mass), LinLog: its mass), formed once when the body is listed.
+// A pair with d2 == 0 makes the term non-finite; rep_pair_guarded gives it
+// the reference's treatment (coincident direction, magnitude num * oc).
+// Without overlap prevention every radius is 0, the gap is the distance and
+// the term is num * d / |d|^2. With it, a gap below 1e-3 of the distance is
+// recomputed from the FP64 difference and radii when positions are FP64
+// (a zero radius sum never meets that test).
+template <bool OVERLAP, bool NEAR>
+__device__ __forceinline__ void rep_pair(float dx, float dy, float d2, float num,
+                                         float radiusSum, float oc, double dxd, double dyd,
+                                         const double *__restrict__ r64, int64_t selfId,
+                                         int64_t otherId, int64_t n, float &ax, float &ay) {
+  if constexpr (!OVERLAP) {
+    const float s = num * __builtin_amdgcn_rcpf(d2);
+    ax -= s * dx;
+    ay -= s * dy;
+  } else {
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float dist = d2 * inv;
+    float gap = dist - radiusSum;
+    if (NEAR && fabsf(gap) < 1e-3f * dist) {
+      const double rs = r64[selfId] + (otherId < n ? r64[otherId] : 0.0);
+      gap = (float)(sqrt(dxd * dxd + dyd * dyd) - rs);
+    }
+    const float s = num * (gap > 0.0f ? __builtin_amdgcn_rcpf(gap) : oc) * inv;
+    ax -= s * dx;
+    ay -= s * dy;
+  }
+}
+
+template <bool OVERLAP, bool NEAR>
+__device__ __forceinline__ void rep_pair_guarded(float dx, float dy, float d2, float num,
+                                                 float radiusSum, float oc, double dxd,
+                                                 double dyd, const double *__restrict__ r64,
+                                                 int64_t selfId, int64_t otherId, int64_t n,
+                                                 float &ax, float &ay) {
+  if (d2 > 0.0f) {
+    rep_pair<OVERLAP, NEAR>(dx, dy, d2, num, radiusSum, oc, dxd, dyd, r64, selfId, otherId,
+                            n, ax, ay);
+  } else {
+    const float2 u = coincident_dir_f(selfId, otherId);
+    const float mag = num * oc;
+    ax -= mag * u.x;
+    ay -= mag * u.y;
+  }
+}
+
 // wave-uniform double held in scalar registers
 __device__ __forceinline__ double uniform(double v) {
   const uint64_t u = (uint64_t)__double_as_longlong(v);
@@ -402,7 +452,7 @@ __device__ __forceinline__ double uniform(double v) {
 //     densely, every lane reading each body with one broadcast LDS load;
 //   * cells the box cannot decide are tested by each lane, as the reference
 //     does, and opened for the lanes that reject them.
-template <class Real, class PosT, int MODEL>
+template <class Real, class PosT, int MODEL, bool OVERLAP>
 __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     const NodeRec<Real, PosT> *__restrict__ nodes,
     const typename V2<PosT>::type *__restrict__ spos,
@@ -412,6 +462,9 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     const double *__restrict__ r64, Scalars *__restrict__ sc,
     unsigned long long *__restrict__ tstats, SegView sv) {
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
+  // FP32 force math: pair terms from rep_pair, flushes replayed with guards
+  // when their sum is not finite; FP64 keeps the reference's formulas
+  constexpr bool FAST = sizeof(Real) == sizeof(float);
   __shared__ int32_t sNode[kTravWaves][kTravStack];
   __shared__ uint64_t sMask[kTravWaves][kTravStack];
   __shared__ TravBody<Real, PosT> dBody[kTravWaves][kTravDefer];
@@ -471,35 +524,66 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   // their mask lanes, undecided cells, their mask lanes, max stack depth
   unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
 
-  // Evaluates the deferred bodies in the order they were deferred.
-  // Evaluates the deferred bodies in the order they were deferred, two per
-  // iteration so the second body's LDS loads overlap the first's math.
-  auto body = [&](const TravBody<Real, PosT> &b) {
-    if (tstats)
-      st[2] += __popcll(rl(b.mask, 0));
+  // The source's numerator as listed in a body (rep_pair); FP64 lists the mass.
+  auto numOf = [&](Real m) -> Real {
+    if constexpr (FAST)
+      return MODEL == 1 ? m : kSq * truncf(m);
+    else
+      return m;
+  };
+  // Deferred body b against this lane's target, if b's mask holds the lane.
+  auto body = [&](const TravBody<Real, PosT> &b, bool guarded) {
     if ((b.mask >> lane) & 1ull) {
       const PosT ex = b.x - me.x, ey = b.y - me.y;
       Real dx = (Real)ex, dy = (Real)ey;
-      Real inv;
-      Real dist = Phys<Real>::dist_inv(dx, dy, inv);
-      Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + b.r, mr.x, b.m, kSq,
-                                            oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
-                                            r64, n);
+      if constexpr (FAST) {
+        const Real d2 = dx * dx + dy * dy;
+        if (guarded)
+          rep_pair_guarded<OVERLAP, NEAR>(dx, dy, d2, b.m, mr.y + b.r, oc, ex, ey, r64,
+                                          selfId, (int64_t)b.id, n, ax, ay);
+        else
+          rep_pair<OVERLAP, NEAR>(dx, dy, d2, b.m, mr.y + b.r, oc, ex, ey, r64, selfId,
+                                  (int64_t)b.id, n, ax, ay);
+      } else {
+        Real inv;
+        Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+        Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + b.r, mr.x, b.m, kSq,
+                                              oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
+                                              r64, n);
+      }
       cnt++;
     }
   };
+  // Evaluates the deferred bodies in the order they were deferred, two per
+  // iteration so the second body's LDS loads overlap the first's math. In
+  // FP32 a lane whose sums are no longer finite (a pair with d2 == 0)
+  // replays the list from the sums before it with guarded terms.
   auto flush = [&]() {
-    if (tstats)
+    if (tstats) {
       st[1] += nd;
+      for (int t = 0; t < nd; t++)
+        st[2] += __popcll(rl(dBody[wave][t].mask, 0));
+    }
+    const Real ax0 = ax, ay0 = ay;
+    const int32_t c0 = cnt;
     int t = 0;
     for (; t + 2 <= nd; t += 2) {
       const TravBody<Real, PosT> b0 = dBody[wave][t];
       const TravBody<Real, PosT> b1 = dBody[wave][t + 1];
-      body(b0);
-      body(b1);
+      body(b0, false);
+      body(b1, false);
     }
     if (t < nd)
-      body(dBody[wave][t]);
+      body(dBody[wave][t], false);
+    if constexpr (FAST) {
+      if (!__builtin_isfinite(ax + ay)) {
+        ax = ax0;
+        ay = ay0;
+        cnt = c0;
+        for (int u = 0; u < nd; u++)
+          body(dBody[wave][u], true);
+      }
+    }
     nd = 0;
   };
   // Writes point p's body at list slot `at`.
@@ -510,7 +594,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     b.x = o.x;
     b.y = o.y;
     b.mask = mk;
-    b.m = om.x;
+    b.m = numOf(om.x);
     b.r = om.y;
     b.id = (int32_t)sidx[p];
     dBody[wave][at] = b;
@@ -579,7 +663,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
           b.x = r.comx;
           b.y = r.comy;
           b.mask = dmsk;
-          b.m = r.mass;
+          b.m = numOf(r.mass);
           b.r = (Real)0;
           b.id = (int32_t)n;
           dBody[wave][at] = b;
@@ -652,7 +736,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         b.x = r.comx;
         b.y = r.comy;
         b.mask = msk;
-        b.m = r.mass;
+        b.m = numOf(r.mass);
         b.r = r.twoh;
         b.id = node;
         dBody[wave][nd + (int)__popcll(am & ltMask)] = b;
@@ -669,15 +753,29 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         if ((b.mask >> lane) & 1ull) {
           const PosT ex = b.x - me.x, ey = b.y - me.y;
           Real dx = (Real)ex, dy = (Real)ey;
-          Real inv;
-          Real dist = Phys<Real>::dist_inv(dx, dy, inv);
-          if (Phys<Real>::accept(b.r, dist, theta)) {
-            Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + (Real)0, mr.x,
-                                                  b.m, kSq, oc, selfId, n, ax, ay, ex, ey,
-                                                  r64, n);
-            cnt++;
+          if constexpr (FAST) {
+            // d2 == 0 gives a NaN distance, which is not accepted: the cell
+            // is opened, as the reference opens it at distance 0
+            const Real d2 = dx * dx + dy * dy;
+            const Real dist = d2 * __builtin_amdgcn_rsqf(d2);
+            if (Phys<Real>::accept(b.r, dist, theta)) {
+              rep_pair<OVERLAP, NEAR>(dx, dy, d2, b.m, mr.y, oc, ex, ey, r64, selfId, n, n,
+                                      ax, ay);
+              cnt++;
+            } else {
+              open = true;
+            }
           } else {
-            open = true;
+            Real inv;
+            Real dist = Phys<Real>::dist_inv(dx, dy, inv);
+            if (Phys<Real>::accept(b.r, dist, theta)) {
+              Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + (Real)0, mr.x,
+                                                    b.m, kSq, oc, selfId, n, ax, ay, ex, ey,
+                                                    r64, n);
+              cnt++;
+            } else {
+              open = true;
+            }
           }
         }
         const uint64_t om = __ballot(open);
@@ -709,6 +807,10 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     atomicMax(&tstats[5], st[5]);
   }
   if (valid) {
+    if (FAST && MODEL == 1) {
+      ax *= mr.x;
+      ay *= mr.x;
+    }
     rep[selfId] = {ax, ay};
     if (inter)
       inter[selfId] = cnt;
@@ -1053,6 +1155,35 @@ int enqueue_radix_tree(ForceCommon &c);
 int enqueue_fa2(ForceCommon &c, int blocks);
 int enqueue_stats(ForceCommon &c, int blocks);
 
+// k_traverse for the configured model; the overlap-free FP32 form when no
+// vertex has a radius (FP64 always takes the reference's formulas).
+template <class Real, class PosT>
+void launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int32_t *inter,
+                     Real theta, Real kSq, Real oc, Scalars *sc,
+                     unsigned long long *tstats, SegView sv) {
+  using P2 = typename V2<PosT>::type;
+  using R2 = typename V2<Real>::type;
+  constexpr bool F64 = sizeof(Real) == sizeof(double);
+  const bool ov = F64 || c.prm.preventOverlap != 0;
+#define GVH_TRAVERSE(M, OV)                                                              \
+  hipLaunchKernelGGL((k_traverse<Real, PosT, M, OV>), dim3(tb), dim3(kTravThreads), 0, st, \
+                     (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,        \
+                     (const R2 *)c.smr.p, c.vals.p, c.n, theta, kSq, oc, (R2 *)c.rep.p,   \
+                     inter, c.radius64.p, sc, tstats, sv)
+  if (model == 1) {
+    if (ov)
+      GVH_TRAVERSE(1, true);
+    else
+      GVH_TRAVERSE(1, F64);
+  } else {
+    if (ov)
+      GVH_TRAVERSE(0, true);
+    else
+      GVH_TRAVERSE(0, F64);
+  }
+#undef GVH_TRAVERSE
+}
+
 // Attraction of every vertex into c.attHeavy on the side stream: forked from
 // the main stream once the positions are final, joined before the force sums.
 template <class Real, class PosT>
@@ -1133,18 +1264,8 @@ int typed_step(ForceCommon &c, int mode) {
     const unsigned tb = (unsigned)((n + kTravThreads - 1) / kTravThreads);
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
-    if (model == 1)
-      hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTravThreads), 0, st,
-                         (const NodeRec<Real, PosT> *)c.nodes.p,
-                         (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
-                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
-                         c.sc.p, c.travStats.p, SegView{});
-    else
-      hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTravThreads), 0, st,
-                         (const NodeRec<Real, PosT> *)c.nodes.p,
-                         (const P2 *)c.spos.p, (const R2 *)c.smr.p, c.vals.p, n,
-                         theta, kSq, oc, (R2 *)c.rep.p, inter, c.radius64.p,
-                         c.sc.p, c.travStats.p, SegView{});
+    launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, c.sc.p,
+                                c.travStats.p, SegView{});
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
@@ -1353,16 +1474,8 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
   int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travBegin, st));
-  if (model == 1)
-    hipLaunchKernelGGL((k_traverse<Real, PosT, 1>), dim3(tb), dim3(kTravThreads), 0, st,
-                       (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,
-                       (const R2 *)c.smr.p, c.vals.p, n, theta, kSq, oc, (R2 *)c.rep.p,
-                       inter, c.radius64.p, bs.scG.p, (unsigned long long *)nullptr, sv);
-  else
-    hipLaunchKernelGGL((k_traverse<Real, PosT, 0>), dim3(tb), dim3(kTravThreads), 0, st,
-                       (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,
-                       (const R2 *)c.smr.p, c.vals.p, n, theta, kSq, oc, (R2 *)c.rep.p,
-                       inter, c.radius64.p, bs.scG.p, (unsigned long long *)nullptr, sv);
+  launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, bs.scG.p,
+                              (unsigned long long *)nullptr, sv);
   GVH_CHECK_LAUNCH();
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travEnd, st));
