@@ -352,6 +352,8 @@ __device__ __forceinline__ double rl(double v, int l) {
 #ifndef GVH_TRAV_DEFER
 #define GVH_TRAV_DEFER 72
 #endif
+// stack entries encode points and tagged cells as negative int32
+constexpr int64_t kTravMaxVertices = int64_t(1) << 29;
 #ifndef GVH_TRAV_MINBLOCKS
 #define GVH_TRAV_MINBLOCKS 1
 #endif
@@ -452,6 +454,9 @@ __device__ __forceinline__ double uniform(double v) {
 //     densely, every lane reading each body with one broadcast LDS load;
 //   * cells the box cannot decide are tested by each lane, as the reference
 //     does, and opened for the lanes that reject them.
+// Stack entries: a cell c >= 0; a point p as -1 - p; a cell to be opened
+// for the entry's lanes (put back by the undecided test) as -1 - n - c,
+// which keeps every entry an int32 for n < 2^29 vertices.
 template <class Real, class PosT, int MODEL, bool OVERLAP>
 __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     const NodeRec<Real, PosT> *__restrict__ nodes,
@@ -646,6 +651,10 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         else
           kind = kAmbig;
       }
+    } else if (has && -1 - (int64_t)node >= n) {
+      // a cell put back by the lanes of its mask that rejected it
+      r = nodes[-1 - (int64_t)node - n];
+      kind = kOpen;
     }
     // deferred points and cells
     {
@@ -780,22 +789,18 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         }
         const uint64_t om = __ballot(open);
         if (om) {
-          if (sp + 4 > kTravStack) {
+          // the cell goes back on the stack once, tagged, and is opened for
+          // the lanes of om (in parallel with other entries) when taken
+          if (sp + 1 > kTravStack) {
             sc->traversalOverflow = 1;
             sp = 0;
             break;
           }
-#pragma unroll
-          for (int c = 3; c >= 0; c--) {
-            const int32_t ch = rl(r.child[c], l);
-            if (ch != kNone) {
-              if (lane == 0) {
-                sNode[wave][sp] = ch;
-                sMask[wave][sp] = om;
-              }
-              sp++;
-            }
+          if (lane == 0) {
+            sNode[wave][sp] = (int32_t)(-1 - n - (int64_t)b.id);
+            sMask[wave][sp] = om;
           }
+          sp++;
         }
       }
     }
@@ -1158,11 +1163,16 @@ int enqueue_stats(ForceCommon &c, int blocks);
 // k_traverse for the configured model; the overlap-free FP32 form when no
 // vertex has a radius (FP64 always takes the reference's formulas).
 template <class Real, class PosT>
-void launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int32_t *inter,
-                     Real theta, Real kSq, Real oc, Scalars *sc,
-                     unsigned long long *tstats, SegView sv) {
+int launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int32_t *inter,
+                    Real theta, Real kSq, Real oc, Scalars *sc,
+                    unsigned long long *tstats, SegView sv) {
   using P2 = typename V2<PosT>::type;
   using R2 = typename V2<Real>::type;
+  if (c.n >= kTravMaxVertices) {
+    set_error("Barnes-Hut traversal: %lld vertices exceed its limit of %lld",
+              (long long)c.n, (long long)kTravMaxVertices);
+    return -1;
+  }
   constexpr bool F64 = sizeof(Real) == sizeof(double);
   const bool ov = F64 || c.prm.preventOverlap != 0;
 #define GVH_TRAVERSE(M, OV)                                                              \
@@ -1182,6 +1192,7 @@ void launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int
       GVH_TRAVERSE(0, F64);
   }
 #undef GVH_TRAVERSE
+  return 0;
 }
 
 // Attraction of every vertex into c.attHeavy on the side stream: forked from
@@ -1264,8 +1275,9 @@ int typed_step(ForceCommon &c, int mode) {
     const unsigned tb = (unsigned)((n + kTravThreads - 1) / kTravThreads);
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
-    launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, c.sc.p,
-                                c.travStats.p, SegView{});
+    if (launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, c.sc.p,
+                                    c.travStats.p, SegView{}))
+      return -1;
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
@@ -1474,8 +1486,9 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
   int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travBegin, st));
-  launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, bs.scG.p,
-                              (unsigned long long *)nullptr, sv);
+  if (launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, bs.scG.p,
+                                  (unsigned long long *)nullptr, sv))
+    return -1;
   GVH_CHECK_LAUNCH();
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travEnd, st));
