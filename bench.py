@@ -63,6 +63,29 @@ def traversal_traffic_bytes():
     return int((float(m.group(1)) + float(m.group(2))) * 1e6)
 
 
+def valu_issue(profile):
+    """Steady-state VALU issue rate and lane use of a kernel from the
+    committed rocprofv3 --pmc table of `profile` (median over dispatches
+    after the first)."""
+    import re
+    import statistics
+    try:
+        text = open(os.path.join(ROOT, "profiles", profile)).read()
+    except OSError:
+        return None
+    sec = text.split("## Vector ALU issue", 1)
+    if len(sec) < 2:
+        return None
+    rows = re.findall(r"^\| \d+ \|.*\| ([0-9.]+) \| ([0-9.]+) \|$", sec[1], re.M)
+    if len(rows) < 2:
+        return None
+    rows = rows[1:]
+    return {"issue_frac": round(statistics.median(float(a) for a, _ in rows), 3),
+            "lane_use": round(statistics.median(float(b) for _, b in rows), 3),
+            "source": f"profiles/{profile} (pmc: SQ_INSTS_VALU, SQ_THREAD_CYCLES_VALU, "
+                      "SQ_ACTIVE_INST_VALU, GRBM_GUI_ACTIVE)"}
+
+
 def workload(n):
     g_t0 = time.time()
     n, edges = G.barabasi_albert(n, 3, 0)
@@ -192,7 +215,8 @@ def bench_allpairs(args):
                         "note": "FP32 vector ALU (no matrix op on this path); 15 FLOP per "
                                 "pair evaluation, N(N-1) evaluations per launch",
                         "launch_ms": round(launch_s * 1e3, 4),
-                        "interactions_per_launch": int(inter / 3)}
+                        "interactions_per_launch": int(inter / 3),
+                        "valu": valu_issue("allpairs_er65k.md")}
     cpu = None
     if not args.no_cpu_baseline and os.path.exists(HARNESS):
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -628,7 +652,8 @@ def main():
                         "repulsion evaluations per launch / mean launch time",
                 "launch_ms": round(trav_avg_s * 1e3, 4),
                 "interactions_per_launch": int(inter / ps),
-                "step_ms_unCaptured": round(step_ms / ps, 4)}
+                "step_ms_unCaptured": round(step_ms / ps, 4),
+                "valu": valu_issue("force_step_ba1m_linlog_mixed.md")}
 
     warm = None
     if not args.no_warm_start:
