@@ -590,6 +590,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--edges-file", default=None,
+                    help="force_ba1m settings on the graph of a .edges file (Network "
+                         "Repository format, undirected) instead of the BA graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy (the known-size "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE readings)")
@@ -619,7 +622,21 @@ def main():
     if args.steps is None:
         args.steps = 200
 
-    n, edges, off, adj, pos, gen_s = workload(args.n)
+    if args.edges_file:
+        from grapher_amd import edges_file
+        g_t0 = time.time()
+        off, adj = edges_file.load_csr(args.edges_file)
+        n = len(off) - 1
+        # edge list for the CPU baseline's graph: each undirected edge once
+        # (self loops left out)
+        src = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+        keep = src < adj
+        edges = np.stack([src[keep], adj[keep]], 1)
+        pos = G.uniform_positions(n, 0)
+        gen_s = time.time() - g_t0
+        args.n = n
+    else:
+        n, edges, off, adj, pos, gen_s = workload(args.n)
     print(f"# workload built in {gen_s:.1f}s: n={n} edges={edges.shape[0]}",
           file=sys.stderr, flush=True)
     eng = native.ForceEngine(off, adj, precision=args.precision, **PARAMS)
@@ -644,7 +661,8 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                "traffic": traversal_traffic_bytes() if args.n == 1_000_000 else None,
+                "traffic": (traversal_traffic_bytes()
+                            if args.n == 1_000_000 and not args.edges_file else None),
                 "traffic_source": "profiles/force_step_ba1m_linlog_mixed.md (pmc)",
                 "kernel": "k_traverse (Barnes-Hut repulsion)",
                 "note": "compute-bound on the FP32 vector ALU (no matrix op on this "
@@ -672,8 +690,12 @@ def main():
         "ms_per_step": round(1e3 * elapsed / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"mixed": "f32 (f64 positions)", "fp32": "f32", "fp64": "f64"}[args.precision],
-        "data": "synthetic (seeded Barabasi-Albert graph, seeded uniform positions)",
-        "config": {"workload": "BA n=1,000,000 m=3 LinLog gravity 0.05 prevent-overlap "
+        "data": ("synthetic (seeded Barabasi-Albert graph, seeded uniform positions)"
+                 if not args.edges_file else
+                 f"graph from {os.path.basename(args.edges_file)}, seeded uniform positions"),
+        "config": {"workload": ("BA n=1,000,000 m=3" if not args.edges_file else
+                                os.path.basename(args.edges_file)) +
+                               " LinLog gravity 0.05 prevent-overlap "
                                "(r=0.1(1+sqrt(deg))) Barnes-Hut theta 0.7, uniform "
                                "[-1,1]^2 init", "n_vertices": n,
                    "n_edges": int(edges.shape[0]), "precision": args.precision,

@@ -19,6 +19,7 @@
 #include "embedders/gvizGRIPInternal.h"
 #include "embedders/gvizTutteEmbedder.h"
 #include "core/gvizThreadPool.h"
+#include "utils/graphLoader.h"
 
 #include <stdio.h>
 #include <stdlib.h>
@@ -594,6 +595,36 @@ static int mode_time_tutte(void) {
   return 0;
 }
 
+/* The reference's .edges loader on a file named in the input: directed,
+ * skip_header, path length and path; writes the return code and, on
+ * success, the vertex count and every adjacency list in order. */
+static int mode_edges_file(void) {
+  gvizEdgesFileOptions opts;
+  gvizEdgesFileOptionsInit(&opts);
+  opts.directed = (int)rd_i();
+  opts.skip_header = (int)rd_i();
+  const int64_t len = rd_i();
+  char *path = calloc((size_t)len + 1, 1);
+  if (fread(path, 1, (size_t)len, fin) != (size_t)len)
+    return 2;
+  gvizGraph g;
+  const int rc = gvizGraphLoadFromEdgesFile(path, &opts, &g);
+  free(path);
+  wr_i(rc);
+  if (rc != 0)
+    return 0;
+  const size_t n = gvizGraphSize(&g);
+  wr_i((int64_t)n);
+  for (size_t u = 0; u < n; u++) {
+    gvizArray *nb = gvizGraphGetVertexNeighbors(&g, u);
+    wr_i((int64_t)nb->count);
+    for (size_t k = 0; k < nb->count; k++)
+      wr_i((int64_t) * (size_t *)gvizArrayAtIndex(nb, k));
+  }
+  gvizGraphRelease(&g);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 4) {
     fprintf(stderr, "usage: %s mode in out\n", argv[0]);
@@ -624,6 +655,8 @@ int main(int argc, char **argv) {
     rc = mode_time_grip();
   else if (!strcmp(argv[1], "time_tutte"))
     rc = mode_time_tutte();
+  else if (!strcmp(argv[1], "edges_file"))
+    rc = mode_edges_file();
   fclose(fin);
   fclose(fout);
   return rc;

@@ -226,9 +226,53 @@ def api_cases():
          refinement_k=8, policy=1, **r)
 
 
+def edges_text_cases():
+    """(name, text, directed, skip_header) for the .edges loader: the
+    reference's own test files (tests/data/*.edges) and files that exercise
+    each parsing rule."""
+    rng = np.random.default_rng(5)
+    big = "".join(f"{int(a)} {int(b)}\n" for a, b in rng.integers(0, 5000, size=(2000, 2)))
+    return [
+        ("sample", "% Small undirected test graph (1-indexed vertex IDs)\n%     4\n"
+                   "%    /\n%   3\n%  / \\\n% 1---2\n1 2\n1 3\n2 3\n3 4\n", 0, 0),
+        ("malformed", "% malformed edges file: second data line is non-numeric\n"
+                      "1 2\nfoo bar\n2 3\n", 0, 0),
+        ("directed", "% Small directed test graph (1-indexed vertex IDs)\n1 2\n2 3\n", 1, 0),
+        ("gapped", "% Vertex IDs with gaps, to test id compaction\n10 20\n20 30\n", 0, 0),
+        ("rules", "% header next\n5 7\n\n  % indented comment\n1 2 0.5\n\t3\t1\r\n"
+                  "   4 2 weight\n3 3\n1 2\n+9 4\n1000000 6abc\n\r\n12 1000000", 0, 1),
+        ("rules_directed", "1 2\n2 1\n2 2\n7 1\n1 2\n", 1, 0),
+        ("zero_based", "0 1\n1 2\n2 0\n", 0, 0),
+        ("header_not_numeric", "% c\nnodes edges\n1 2\n", 0, 1),
+        ("negative", "1 2\n-3 4\n", 0, 0),
+        ("comments_only", "% nothing\n\n%\n", 0, 0),
+        ("single_number", "1 2\n5\n", 0, 0),
+        ("random2000", big, 0, 0),
+    ]
+
+
+def edges_cases():
+    import tempfile
+    out = {}
+    names = []
+    with tempfile.TemporaryDirectory() as td:
+        for i, (name, text, directed, skip) in enumerate(edges_text_cases()):
+            path = os.path.join(td, name + ".edges")
+            with open(path, "w", newline="") as f:
+                f.write(text)
+            rc, off, adj = refio.edges_file(path, directed, skip)
+            names.append(name)
+            out[f"c{i}_text"] = np.frombuffer(text.encode(), np.uint8)
+            out[f"c{i}_opts"] = np.array([directed, skip], np.int64)
+            out[f"c{i}_rc"] = np.array(rc, np.int64)
+            out[f"c{i}_offsets"] = off if off is not None else np.zeros(0, np.int64)
+            out[f"c{i}_adj"] = adj if adj is not None else np.zeros(0, np.int64)
+    save("edges_files", names=np.array(names), **out)
+
+
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
              knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
-             api=api_cases)
+             api=api_cases, edges=edges_cases)
 
 if __name__ == "__main__":
     if not refio.harness_available():

@@ -258,3 +258,37 @@ def api_grip(n, edges, kcap, placement_k, refinement_k, policy, exe=None):
         out["series_" + name] = r.da(c)
     out["visible"] = r.ia(n)
     return out
+
+
+def edges_file(path, directed=0, skip_header=0, exe=None, csr=False):
+    """The .edges loader on `path`: (rc, offsets, adj) of the adjacency lists
+    in order (offsets/adj None when rc != 0). With csr=True (this
+    repository's driver) also the CSR loader's (rc, offsets, adj)."""
+    w = _W()
+    raw = os.fsencode(path)
+    w.i(directed, skip_header, len(raw))
+    r = _R(_run("edges_file", w.bytes() + raw, exe))
+
+    def lists():
+        rc = r.i()
+        if rc != 0:
+            return rc, None, None
+        n = r.i()
+        off = [0]
+        adj = []
+        for _ in range(n):
+            c = r.i()
+            adj.extend(r.ia(c).tolist())
+            off.append(off[-1] + c)
+        return rc, np.array(off, np.int64), np.array(adj, np.int64)
+
+    first = lists()
+    if not csr:
+        return first
+    rc = r.i()
+    if rc != 0:
+        return first, (rc, None, None)
+    n = r.i()
+    off = r.ia(n + 1)
+    adj = r.ia(int(off[-1]))
+    return first, (rc, off, adj)

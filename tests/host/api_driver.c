@@ -15,6 +15,7 @@
 #include "gviz/gvizForceEmbedder.h"
 #include "gviz/gvizGRIPEmbedder.h"
 #include "gviz/gvizTutteEmbedder.h"
+#include "gviz/gvizGraphLoader.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -547,6 +548,50 @@ static int mode_unit(void) {
   return failures ? 1 : 0;
 }
 
+/* The .edges loader on a file named in the input (directed, skip_header,
+ * path length, path), in the reference harness's framing: return code, then
+ * the vertex count and every adjacency list; then the CSR loader's return
+ * code, vertex count, offsets and adjacency. */
+static int mode_edges_file(void) {
+  gvizEdgesFileOptions opts;
+  gvizEdgesFileOptionsInit(&opts);
+  opts.directed = (int)rd_i();
+  opts.skip_header = (int)rd_i();
+  const int64_t len = rd_i();
+  char *path = calloc((size_t)len + 1, 1);
+  if (fread(path, 1, (size_t)len, fin) != (size_t)len)
+    return 2;
+  gvizGraph g;
+  const int rc = gvizGraphLoadFromEdgesFile(path, &opts, &g);
+  wr_i(rc);
+  if (rc == 0) {
+    const size_t n = gvizGraphSize(&g);
+    wr_i((int64_t)n);
+    for (size_t u = 0; u < n; u++) {
+      size_t cnt = 0;
+      const size_t *nb = gvizGraphNeighbors(&g, u, &cnt);
+      wr_i((int64_t)cnt);
+      for (size_t k = 0; k < cnt; k++)
+        wr_i((int64_t)nb[k]);
+    }
+    gvizGraphRelease(&g);
+  }
+  int64_t n = 0, *off = NULL, *adj = NULL;
+  const int rc2 = gvizGraphLoadEdgesFileCSR(path, &opts, &n, &off, &adj);
+  free(path);
+  wr_i(rc2);
+  if (rc2 == 0) {
+    wr_i(n);
+    for (int64_t i = 0; i <= n; i++)
+      wr_i(off[i]);
+    for (int64_t e = 0; e < off[n]; e++)
+      wr_i(adj[e]);
+    free(off);
+    free(adj);
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc == 2 && !strcmp(argv[1], "unit"))
     return mode_unit();
@@ -565,6 +610,8 @@ int main(int argc, char **argv) {
     rc = mode_api_grip();
   else if (!strcmp(argv[1], "tutte"))
     rc = mode_tutte();
+  else if (!strcmp(argv[1], "edges_file"))
+    rc = mode_edges_file();
   fclose(fin);
   fclose(fout);
   return rc;
