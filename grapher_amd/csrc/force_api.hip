@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 using namespace gvh;
@@ -502,6 +503,105 @@ int gvizhip_force_set_speed(gvizhip_force *f, double globalSpeed,
     return -1;
   s.globalSpeed = globalSpeed;
   s.speedEfficiency = speedEfficiency;
+  return write_scalars(f, s);
+}
+
+// Exact checkpoint: a header, the FP64 positions and the previous step's
+// structForce in the engine's precision.
+namespace {
+struct StateHeader {
+  char magic[8];
+  int64_t n;
+  int32_t realSize;
+  int32_t reserved;
+  double globalSpeed, speedEfficiency;
+  int64_t iteration;
+};
+constexpr char kStateMagic[8] = {'G', 'V', 'H', 'F', 'S', 'T', '0', '1'};
+} // namespace
+
+int64_t gvizhip_force_state_bytes(gvizhip_force *f) {
+  if (!f) {
+    set_error("gvizhip_force_state_bytes: null context");
+    return -1;
+  }
+  return (int64_t)sizeof(StateHeader) + 2 * f->c.n * (8 + (int64_t)real_size(f->c));
+}
+
+int gvizhip_force_save_state(gvizhip_force *f, void *buf, int64_t bytes) {
+  if (!f || !buf || bytes < gvizhip_force_state_bytes(f)) {
+    set_error("gvizhip_force_save_state: null argument or buffer below "
+              "gvizhip_force_state_bytes");
+    return -1;
+  }
+  ForceCommon &c = f->c;
+  Scalars s;
+  if (read_scalars(f, &s))
+    return -1;
+  StateHeader h;
+  memcpy(h.magic, kStateMagic, 8);
+  h.n = c.n;
+  h.realSize = (int32_t)real_size(c);
+  h.reserved = 0;
+  h.globalSpeed = s.globalSpeed;
+  h.speedEfficiency = s.speedEfficiency;
+  h.iteration = s.iteration;
+  unsigned char *p = (unsigned char *)buf;
+  memcpy(p, &h, sizeof h);
+  p += sizeof h;
+  if (c.n > 0) {
+    GVH_CHECK(hipMemcpy(p, c.pos64.p, 16 * c.n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(p + 16 * c.n, c.sf.p, 2 * c.n * h.realSize,
+                        hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int gvizhip_force_load_state(gvizhip_force *f, const void *buf, int64_t bytes) {
+  StateHeader h;
+  if (!f || !buf || bytes < (int64_t)sizeof h) {
+    set_error("gvizhip_force_load_state: invalid arguments");
+    return -1;
+  }
+  memcpy(&h, buf, sizeof h);
+  ForceCommon &c = f->c;
+  if (memcmp(h.magic, kStateMagic, 8) != 0 || h.n != c.n ||
+      (h.realSize != 4 && h.realSize != 8) ||
+      bytes < (int64_t)sizeof h + 2 * h.n * (8 + (int64_t)h.realSize)) {
+    set_error("gvizhip_force_load_state: not a state of an engine with %lld vertices",
+              (long long)c.n);
+    return -1;
+  }
+  const unsigned char *p = (const unsigned char *)buf + sizeof h;
+  if (c.n > 0) {
+    GVH_CHECK(hipMemcpy(c.pos64.p, p, 16 * c.n, hipMemcpyHostToDevice));
+    const size_t rs = real_size(c);
+    const int64_t cnt = 2 * c.n;
+    if ((size_t)h.realSize == rs) {
+      GVH_CHECK(hipMemcpy(c.sf.p, p + 16 * c.n, cnt * rs, hipMemcpyHostToDevice));
+    } else {
+      // saved in the other precision: converted through FP64
+      std::vector<double> d(cnt);
+      to_double(p + 16 * c.n, (size_t)h.realSize, cnt, d.data());
+      if (rs == 8) {
+        GVH_CHECK(hipMemcpy(c.sf.p, d.data(), 8 * cnt, hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> fl(cnt);
+        for (int64_t i = 0; i < cnt; i++)
+          fl[i] = (float)d[i];
+        GVH_CHECK(hipMemcpy(c.sf.p, fl.data(), 4 * cnt, hipMemcpyHostToDevice));
+      }
+    }
+  }
+  Scalars s;
+  if (read_scalars(f, &s))
+    return -1;
+  s.globalSpeed = h.globalSpeed;
+  s.speedEfficiency = h.speedEfficiency;
+  s.iteration = h.iteration;
+  s.recordBase = h.iteration;
+  s.stopped = 0;
+  s.epsilon = -1.0;
   return write_scalars(f, s);
 }
 

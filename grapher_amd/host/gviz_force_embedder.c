@@ -14,6 +14,7 @@
 #include "gviz_hip.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -372,6 +373,103 @@ int gvizForceEmbedderSyncPositions(gvizForceEmbedderState *s) {
   if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
     return -1;
   scatter_positions(s);
+  return 0;
+}
+
+/* Binary checkpoint file: magic, the host's step count and last maxDisp,
+ * then the device state (gvizhip_force_save_state). */
+static const char kStateFileMagic[8] = {'G', 'V', 'Z', 'F', 'E', 'S', '0', '1'};
+
+int gvizForceEmbedderSaveState(gvizForceEmbedderState *s, const char *filename) {
+  host_error = "";
+  if (!s->begun || (!s->device && s->vertexCount > 0) || !filename) {
+    host_error = "gvizForceEmbedderSaveState: not begun or no file name";
+    return -1;
+  }
+  device_box *box = (device_box *)s->device;
+  int64_t bytes = 0;
+  void *blob = NULL;
+  if (box) {
+    if (!s->residentPositions) {
+      gather_positions(s);
+      if (gvizhip_force_upload_positions(box->f, s->positionsScratch) != 0)
+        return -1;
+    }
+    bytes = gvizhip_force_state_bytes(box->f);
+    blob = bytes > 0 ? malloc((size_t)bytes) : NULL;
+    if (!blob || gvizhip_force_save_state(box->f, blob, bytes) != 0) {
+      free(blob);
+      host_error = "gvizForceEmbedderSaveState: device state not read";
+      return -1;
+    }
+  }
+  FILE *f = fopen(filename, "wb");
+  if (!f) {
+    free(blob);
+    host_error = "gvizForceEmbedderSaveState: cannot open the file";
+    return -1;
+  }
+  const int64_t it = (int64_t)s->iteration;
+  int ok = fwrite(kStateFileMagic, 1, 8, f) == 8 && fwrite(&it, 8, 1, f) == 1 &&
+           fwrite(&s->lastMaxDisplacement, 8, 1, f) == 1 && fwrite(&bytes, 8, 1, f) == 1 &&
+           (bytes == 0 || fwrite(blob, 1, (size_t)bytes, f) == (size_t)bytes);
+  ok = (fclose(f) == 0) && ok;
+  free(blob);
+  if (!ok) {
+    host_error = "gvizForceEmbedderSaveState: write failed";
+    return -1;
+  }
+  return 0;
+}
+
+int gvizForceEmbedderLoadState(gvizForceEmbedderState *s, const char *filename) {
+  host_error = "";
+  if (!s->begun || (!s->device && s->vertexCount > 0) || !filename) {
+    host_error = "gvizForceEmbedderLoadState: Begin the embedder on the same graph first";
+    return -1;
+  }
+  FILE *f = fopen(filename, "rb");
+  if (!f) {
+    host_error = "gvizForceEmbedderLoadState: cannot open the file";
+    return -1;
+  }
+  char magic[8];
+  int64_t it = 0, bytes = -1;
+  double last = 0.0;
+  int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kStateFileMagic, 8) == 0 &&
+           fread(&it, 8, 1, f) == 1 && fread(&last, 8, 1, f) == 1 &&
+           fread(&bytes, 8, 1, f) == 1 && bytes >= 0 && it >= 0;
+  void *blob = NULL;
+  if (ok && bytes > 0) {
+    blob = malloc((size_t)bytes);
+    ok = blob && fread(blob, 1, (size_t)bytes, f) == (size_t)bytes;
+  }
+  fclose(f);
+  device_box *box = (device_box *)s->device;
+  if (ok && box && sync_config(s) != 0)
+    ok = 0;
+  if (ok && box) {
+    ok = gvizhip_force_load_state(box->f, blob, bytes) == 0;
+    double gs = 0.0, eff = 0.0;
+    int64_t dit = 0;
+    if (ok)
+      ok = gvizhip_force_get_speed(box->f, &gs, &eff, &dit) == 0;
+    if (ok) {
+      s->globalSpeed = gs;
+      s->speedEfficiency = eff;
+    }
+    if (ok && !s->residentPositions)
+      ok = gvizForceEmbedderSyncPositions(s) == 0;
+  } else if (ok && bytes != 0) {
+    ok = 0;
+  }
+  free(blob);
+  if (!ok) {
+    host_error = "gvizForceEmbedderLoadState: not a state of this embedder's graph";
+    return -1;
+  }
+  s->iteration = (size_t)it;
+  s->lastMaxDisplacement = last;
   return 0;
 }
 

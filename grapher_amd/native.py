@@ -99,6 +99,9 @@ def lib():
         "gvizhip_force_evaluate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "gvizhip_force_get_speed": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_i64)]),
         "gvizhip_force_set_speed": (C.c_int, [_vp, C.c_double, C.c_double]),
+        "gvizhip_force_state_bytes": (_i64, [_vp]),
+        "gvizhip_force_save_state": (C.c_int, [_vp, _vp, _i64]),
+        "gvizhip_force_load_state": (C.c_int, [_vp, _vp, _i64]),
         "gvizhip_force_quadtree_leaves": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "gvizhip_force_quadtree_cells": (_i64, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "gvizhip_force_profile": (C.c_int, [_vp, _i64, C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(C.c_double)]),
@@ -164,6 +167,7 @@ DECLARED = [
     "gvizhip_force_upload_positions", "gvizhip_force_download_positions",
     "gvizhip_force_step", "gvizhip_force_run", "gvizhip_force_evaluate",
     "gvizhip_force_get_speed", "gvizhip_force_set_speed",
+    "gvizhip_force_state_bytes", "gvizhip_force_save_state", "gvizhip_force_load_state",
     "gvizhip_force_quadtree_leaves", "gvizhip_force_quadtree_cells",
     "gvizhip_force_profile", "gvizhip_force_destroy",
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
@@ -290,6 +294,22 @@ class ForceEngine:
     def set_speed(self, gs, eff):
         if lib().gvizhip_force_set_speed(self.h, gs, eff) != 0:
             _err("gvizhip_force_set_speed")
+
+    def save_state(self):
+        """Exact checkpoint (positions, previous structForce, speed state,
+        step count) as bytes."""
+        size = lib().gvizhip_force_state_bytes(self.h)
+        if size < 0:
+            _err("gvizhip_force_state_bytes")
+        buf = C.create_string_buffer(int(size))
+        if lib().gvizhip_force_save_state(self.h, buf, size) != 0:
+            _err("gvizhip_force_save_state")
+        return buf.raw
+
+    def load_state(self, state):
+        buf = C.create_string_buffer(bytes(state), len(state))
+        if lib().gvizhip_force_load_state(self.h, buf, len(state)) != 0:
+            _err("gvizhip_force_load_state")
 
     def quadtree_leaves(self):
         n = self.n

@@ -129,6 +129,14 @@ void gvizForceEmbedderSetPrecision(gvizForceEmbedderState *state,
 void gvizForceEmbedderSetResidentPositions(gvizForceEmbedderState *state,
                                            int enabled);
 int gvizForceEmbedderSyncPositions(gvizForceEmbedderState *state);
+/* Exact binary checkpoint of a begun embedder: positions, the previous
+ * step's structForce, global speed, speed efficiency, step count and last
+ * maxDisp. LoadState needs Begin on the same graph first; the run then
+ * continues bit for bit as if never interrupted (the reference's text
+ * checkpoint, gvizEmbeddedGraphSaveEmbedding, keeps positions only, as
+ * "%f"). 0 or -1. */
+int gvizForceEmbedderSaveState(gvizForceEmbedderState *state, const char *filename);
+int gvizForceEmbedderLoadState(gvizForceEmbedderState *state, const char *filename);
 const char *gvizForceEmbedderLastError(void);
 
 #ifdef __cplusplus

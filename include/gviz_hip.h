@@ -104,6 +104,17 @@ int gvizhip_force_get_speed(gvizhip_force *f, double *globalSpeed,
                             double *speedEfficiency, int64_t *iteration);
 int gvizhip_force_set_speed(gvizhip_force *f, double globalSpeed,
                             double speedEfficiency);
+/* Exact checkpoint of the embedder's evolving state: FP64 positions, the
+ * previous step's structForce (in the engine's precision), global speed,
+ * speed efficiency and the step count. No reference counterpart: the
+ * reference's checkpoint (src/embedders/gvizEmbeddedGraph.c:428-485) keeps
+ * positions as "%f" text and no speed state, so a run resumed from it
+ * differs; one resumed from this state continues bit for bit. `bytes` is
+ * gvizhip_force_state_bytes(); load takes a state of an engine with the same
+ * vertex count (structForce saved in the other precision is converted). */
+int64_t gvizhip_force_state_bytes(gvizhip_force *f);
+int gvizhip_force_save_state(gvizhip_force *f, void *buf, int64_t bytes);
+int gvizhip_force_load_state(gvizhip_force *f, const void *buf, int64_t bytes);
 /* The quadtree of the current positions (gvizQuadtreeBuild,
  * src/ds/gvizQuadtree.c:159-192): per vertex the depth and quadrant-path key
  * of its leaf; root[4] = centre x, centre y, half size, depth limit. */

@@ -292,3 +292,19 @@ def edges_file(path, directed=0, skip_header=0, exe=None, csr=False):
     off = r.ia(n + 1)
     adj = r.ia(int(off[-1]))
     return first, (rc, off, adj)
+
+
+def force_checkpoint(n, edges, model, seed, k1, k2, path, exe):
+    """This repository's driver: embedder A runs k1 steps, saves its state to
+    `path` and runs k2 more; embedder B loads the state and runs k2. Returns
+    [(positions, steps, globalSpeed)] for A and B."""
+    w = _W()
+    w.graph(n, edges)
+    raw = os.fsencode(path)
+    w.i(model, seed, k1, k2, len(raw))
+    r = _R(_run("force_checkpoint", w.bytes() + raw, exe))
+    out = []
+    for _ in range(2):
+        pos = r.da(2 * n).reshape(n, 2)
+        out.append((pos, r.i(), r.d()))
+    return out

@@ -548,6 +548,55 @@ static int mode_unit(void) {
   return failures ? 1 : 0;
 }
 
+/* Binary checkpoint through the host API: embedder A runs k1 steps from
+ * Begin(seed), saves its state to `path`, runs k2 more; embedder B (same
+ * graph, Begin(seed + 1)) loads the state and runs k2 steps. Writes both
+ * position arrays, both step counts and both global speeds. Input: graph,
+ * model, seed, k1, k2, path length, path. */
+static int mode_force_checkpoint(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  const int64_t model = rd_i(), seed = rd_i(), k1 = rd_i(), k2 = rd_i();
+  const int64_t len = rd_i();
+  char *path = calloc((size_t)len + 1, 1);
+  if (fread(path, 1, (size_t)len, fin) != (size_t)len)
+    return 2;
+  gvizForceEmbedderState a, b;
+  if (gvizForceEmbedderInit(&a, gvizSubgraphCreateFull(&g), 2, (gvizForceModelKind)model) ||
+      gvizForceEmbedderInit(&b, gvizSubgraphCreateFull(&g), 2, (gvizForceModelKind)model))
+    return 1;
+  gvizForceEmbedderSetPrecision(&a, env_precision());
+  gvizForceEmbedderSetPrecision(&b, env_precision());
+  gvizForceEmbedderConfigureGravity(&a, 0.05);
+  gvizForceEmbedderConfigureGravity(&b, 0.05);
+  if (gvizForceEmbedderBegin(&a, (unsigned int)seed) ||
+      gvizForceEmbedderBegin(&b, (unsigned int)seed + 1))
+    return fail_device("gvizForceEmbedderBegin");
+  if (gvizForceEmbedderRun(&a, (size_t)k1, 0.0) < 0)
+    return fail_device("gvizForceEmbedderRun");
+  if (gvizForceEmbedderSaveState(&a, path) != 0)
+    return fail_device("gvizForceEmbedderSaveState");
+  if (gvizForceEmbedderRun(&a, (size_t)k2, 0.0) < 0)
+    return fail_device("gvizForceEmbedderRun");
+  if (gvizForceEmbedderLoadState(&b, path) != 0)
+    return fail_device("gvizForceEmbedderLoadState");
+  if (gvizForceEmbedderRun(&b, (size_t)k2, 0.0) < 0)
+    return fail_device("gvizForceEmbedderRun");
+  free(path);
+  gvizForceEmbedderState *st[2] = {&a, &b};
+  for (int k = 0; k < 2; k++) {
+    const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)st[k]);
+    for (int64_t i = 0; i < 2 * n; i++)
+      wr_d(p[i]);
+    wr_i((int64_t)st[k]->iteration);
+    wr_d(st[k]->globalSpeed);
+  }
+  gvizForceEmbedderRelease(&a);
+  gvizForceEmbedderRelease(&b);
+  return 0;
+}
+
 /* The .edges loader on a file named in the input (directed, skip_header,
  * path length, path), in the reference harness's framing: return code, then
  * the vertex count and every adjacency list; then the CSR loader's return
@@ -612,6 +661,8 @@ int main(int argc, char **argv) {
     rc = mode_tutte();
   else if (!strcmp(argv[1], "edges_file"))
     rc = mode_edges_file();
+  else if (!strcmp(argv[1], "force_checkpoint"))
+    rc = mode_force_checkpoint();
   fclose(fin);
   fclose(fout);
   return rc;

@@ -230,3 +230,33 @@ def test_exact_all_pairs_against_oracle(gpu, model, overlap, precision):
     # the stacked vertices are pushed apart along their hashed directions
     for v in (0, 10, 19, 300, 1000):
         assert np.linalg.norm(got[v] - ref[v]) <= tol * np.linalg.norm(ref[v]), v
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_checkpoint_resume_is_bit_identical(gpu, precision):
+    """gvizhip_force_save_state / load_state: a run resumed in a second engine
+    from the state after 20 steps matches the uninterrupted run bit for bit
+    over the next 30 steps (positions, speed, step count); a state of another
+    vertex count is refused."""
+    n, e = G.barabasi_albert(3000, 3, 4)
+    off, adj = G.csr_from_edges(n, e)
+    kw = dict(model=1, theta=0.7, gravity=0.05, prevent_overlap=True, radius_base=0.01,
+              radius_per_degree=0.5, precision=precision)
+    a = gpu.ForceEngine(off, adj, **kw)
+    a.begin(G.uniform_positions(n, 1))
+    a.run(20)
+    state = a.save_state()
+    stats_a, _ = a.run(30)
+    b = gpu.ForceEngine(off, adj, **kw)
+    b.begin(G.uniform_positions(n, 2))
+    b.load_state(state)
+    stats_b, _ = b.run(30)
+    np.testing.assert_array_equal(a.positions(), b.positions())
+    np.testing.assert_array_equal(stats_a["maxDisp"], stats_b["maxDisp"])
+    assert a.speed() == b.speed()
+    n2, e2 = G.barabasi_albert(2000, 3, 4)
+    o2, a2 = G.csr_from_edges(n2, e2)
+    c = gpu.ForceEngine(o2, a2, **kw)
+    c.begin(G.uniform_positions(n2, 1))
+    with pytest.raises(RuntimeError):
+        c.load_state(state)

@@ -248,3 +248,18 @@ def test_tutte_api_bit_exact(gpu, name):
     assert np.array_equal(p0, f["p0"])
     assert np.array_equal(deltas, f["deltas"])
     assert np.array_equal(p1, f["p1"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_force_state_checkpoint_resumes_exactly(gpu, tmp_path, monkeypatch, precision):
+    """gvizForceEmbedderSaveState / LoadState: a second embedder (begun from
+    another seed) that loads the state saved after 15 Run steps continues
+    bit for bit with the first over the next 25 steps."""
+    from grapher_amd import graphs as G
+    n, e = G.barabasi_albert(800, 2, 3)
+    monkeypatch.setenv("GVIZ_TEST_PRECISION", precision)
+    (pa, ia, ga), (pb, ib, gb) = refio.force_checkpoint(
+        n, e, 1, 7, 15, 25, str(tmp_path / "state.bin"), DRIVER)
+    np.testing.assert_array_equal(pa, pb)
+    assert ia == ib == 40 and ga == gb
