@@ -1,0 +1,283 @@
+/*
+ * gviz_spring_tutte_embedder.c -- gvizSpringTutteEmbedder* with the step on
+ * the GPU (reference: src/embedders/gvizSpringTutteEmbedder.c).
+ *
+ * Boundary placement, seeding and the convex polygon are host bookkeeping
+ * done as the reference does them; every step runs on the device
+ * (gvizhip_tutte_spring_steps), which keeps positions, the Jacobi snapshot
+ * and the velocities in HBM. Positions and velocities are uploaded when the
+ * host changed them and read back after each Step / Run.
+ */
+#include "gviz/gvizSpringTutteEmbedder.h"
+
+#include "gviz_hip.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static size_t num_vertices(const gvizSpringTutteState *s) {
+  return gvizGraphSize(((const gvizEmbeddedGraph *)s)->subgraph.g);
+}
+
+static size_t dim_of(const gvizSpringTutteState *s) {
+  return ((const gvizEmbeddedGraph *)s)->embedding.dim;
+}
+
+static void action_step(gvizEmbeddedGraph *embedding, void *userData,
+                        const gvizActionPayload *payload) {
+  (void)userData;
+  gvizSpringTutteState *s = (gvizSpringTutteState *)embedding;
+  if (!s->begun || !s->boundary)
+    return;
+  double dt = payload && payload->deltaTime > 0.0 ? payload->deltaTime : 1.0 / 60.0;
+  gvizSpringTutteEmbedderStep(s, dt);
+}
+
+static void action_fix_outer_face(gvizEmbeddedGraph *embedding, void *userData,
+                                  const gvizActionPayload *payload) {
+  (void)userData;
+  (void)payload;
+  gvizSpringTutteEmbedderFixOuterFace((gvizSpringTutteState *)embedding);
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:56-104 */
+int gvizSpringTutteEmbedderInit(gvizSpringTutteState *s, gvizSubgraph subgraph,
+                                size_t dimension, double epsilon) {
+  memset(s, 0, sizeof(*s));
+  int res = gvizEmbeddedGraphInit((gvizEmbeddedGraph *)s, subgraph, dimension);
+  if (res < 0)
+    return res;
+  const size_t n = gvizGraphSize(subgraph.g);
+  s->isBoundary = calloc(n ? n : 1, 1);
+  s->velocity = calloc(n * dimension ? n * dimension : 1, sizeof(double));
+  if (!s->isBoundary || !s->velocity)
+    return -1;
+  s->epsilon = epsilon > 0.0 ? epsilon : GVIZ_SPRING_TUTTE_DEFAULT_EPSILON;
+  s->stiffness = GVIZ_SPRING_TUTTE_DEFAULT_STIFFNESS;
+  s->damping = GVIZ_SPRING_TUTTE_DEFAULT_DAMPING;
+  s->precision = GVIZHIP_PRECISION_FP64;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  if (gvizEmbeddedGraphAddAction(e, "springTutte.step", action_step, NULL) < 0)
+    return -1;
+  if (gvizEmbeddedGraphAddAction(e, "springTutte.fixOuterFace", action_fix_outer_face,
+                                 NULL) < 0)
+    return -1;
+  if (!gvizEmbeddedGraphAddStatSeries(e, "springTutte.maxDelta", GVIZ_STAT_CHART_LINE_LOG))
+    return -1;
+  return 0;
+}
+
+void gvizSpringTutteEmbedderRelease(gvizSpringTutteState *s) {
+  if (!s)
+    return;
+  if (s->device)
+    gvizhip_tutte_destroy(s->device);
+  s->device = NULL;
+  free(s->boundary);
+  free(s->isBoundary);
+  free(s->velocity);
+  s->boundary = NULL;
+  s->isBoundary = NULL;
+  s->velocity = NULL;
+  gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:118-124 */
+void gvizSpringTutteEmbedderConfigure(gvizSpringTutteState *s, double stiffness,
+                                      double damping) {
+  if (stiffness > 0.0)
+    s->stiffness = stiffness;
+  if (damping > 0.0)
+    s->damping = damping;
+}
+
+/* The reference's Begin embeds the graph in the plane (planarity test,
+ * rotation system, largest face: :164-191); that stage is not part of this
+ * engine. */
+int gvizSpringTutteEmbedderBegin(gvizSpringTutteState *s) {
+  (void)s;
+  return -1;
+}
+
+int gvizSpringTutteEmbedderFixOuterFace(gvizSpringTutteState *s) {
+  (void)s;
+  return -1;
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:193-223: pins the vertices at the
+ * given positions and zeroes their velocity. */
+int gvizSpringTutteEmbedderSetBoundary(gvizSpringTutteState *s, const size_t *boundary,
+                                       size_t count, const double *positions) {
+  if (count < 3)
+    return -1;
+  const size_t n = num_vertices(s), d = dim_of(s);
+  for (size_t i = 0; i < count; i++)
+    if (boundary[i] >= n)
+      return -1;
+  memset(s->isBoundary, 0, n);
+  free(s->boundary);
+  s->boundary = malloc(sizeof(size_t) * count);
+  if (!s->boundary) {
+    s->boundaryCount = 0;
+    return -1;
+  }
+  memcpy(s->boundary, boundary, sizeof(size_t) * count);
+  s->boundaryCount = count;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  for (size_t i = 0; i < count; i++) {
+    s->isBoundary[boundary[i]] = 1;
+    gvizEmbeddedGraphSetVPosition(e, boundary[i], positions + i * d);
+    memset(s->velocity + boundary[i] * d, 0, sizeof(double) * d);
+  }
+  s->deviceFresh = 0;
+  return 0;
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:225-250: interior vertices at the
+ * boundary centroid, every velocity zero, counters reset. */
+void gvizSpringTutteEmbedderSeedInterior(gvizSpringTutteState *s) {
+  const size_t n = num_vertices(s), d = dim_of(s);
+  double *centroid = calloc(d, sizeof(double));
+  if (!centroid)
+    return;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  for (size_t i = 0; i < s->boundaryCount; i++) {
+    const double *p = gvizEmbeddedGraphGetVPosition(e, s->boundary[i]);
+    for (size_t k = 0; k < d; k++)
+      centroid[k] += p[k];
+  }
+  if (s->boundaryCount > 0)
+    for (size_t k = 0; k < d; k++)
+      centroid[k] /= (double)s->boundaryCount;
+  for (size_t u = 0; u < n; u++)
+    if (!s->isBoundary[u])
+      gvizEmbeddedGraphSetVPosition(e, u, centroid);
+  free(centroid);
+  memset(s->velocity, 0, sizeof(double) * n * d);
+  s->iteration = 0;
+  s->lastMaxDelta = 0.0;
+  s->converged = 0;
+  s->deviceFresh = 0;
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:252-270 */
+void gvizSpringTutteFixConvexPolygon(gvizSpringTutteState *s, const size_t *boundary,
+                                     size_t count, double radius) {
+  const size_t d = dim_of(s);
+  double *positions = calloc(count > 0 ? count * d : 1, sizeof(double));
+  if (!positions)
+    return;
+  for (size_t k = 0; k < count; k++) {
+    double angle = 2.0 * M_PI * (double)k / (double)count;
+    positions[k * d] = radius * cos(angle);
+    if (d > 1)
+      positions[k * d + 1] = radius * sin(angle);
+  }
+  gvizSpringTutteEmbedderSetBoundary(s, boundary, count, positions);
+  free(positions);
+}
+
+/* Device engine over parent ids with the subgraph's edges, created on the
+ * first step; positions and velocities re-uploaded when the host changed
+ * them. */
+static int ensure_device(gvizSpringTutteState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  if (dim_of(s) != 2)
+    return -1;
+  if (!s->device) {
+    int64_t *off = NULL, *adj = NULL;
+    if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
+      return -1;
+    int rc = gvizhip_tutte_create(&s->device, (int64_t)num_vertices(s), off, adj,
+                                  s->precision);
+    free(off);
+    free(adj);
+    if (rc != 0) {
+      s->device = NULL;
+      return -1;
+    }
+    s->deviceFresh = 0;
+  }
+  if (!s->deviceFresh) {
+    if (gvizhip_tutte_upload(s->device, e->embedding.vertexPositions, s->isBoundary) != 0 ||
+        gvizhip_tutte_set_velocity(s->device, s->velocity) != 0)
+      return -1;
+    s->deviceFresh = 1;
+  }
+  return 0;
+}
+
+static int read_back(gvizSpringTutteState *s) {
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  if (gvizhip_tutte_download(s->device, e->embedding.vertexPositions) != 0 ||
+      gvizhip_tutte_get_velocity(s->device, s->velocity) != 0)
+    return -1;
+  return 0;
+}
+
+static void record_step(gvizSpringTutteState *s, double maxDelta) {
+  s->iteration++;
+  s->lastMaxDelta = maxDelta;
+  if (maxDelta < s->epsilon)
+    s->converged = 1;
+  gvizEmbeddedGraphStatAppend((gvizEmbeddedGraph *)s, "springTutte.maxDelta", maxDelta);
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:333-358. Host positions are
+ * uploaded before the step (a caller may have moved vertices) and read back
+ * after it, with the velocities. */
+double gvizSpringTutteEmbedderStep(gvizSpringTutteState *s, double dt) {
+  if (dt <= 0.0)
+    return 0.0;
+  s->deviceFresh = 0;
+  if (ensure_device(s) != 0)
+    return -1.0;
+  double maxDelta = 0.0;
+  if (gvizhip_tutte_spring_steps(s->device, 1, dt, s->stiffness, s->damping, -1.0,
+                                 &maxDelta) != 1)
+    return -1.0;
+  if (read_back(s) != 0)
+    return -1.0;
+  record_step(s, maxDelta);
+  return maxDelta;
+}
+
+/* src/embedders/gvizSpringTutteEmbedder.c:360-369: steps until converged or
+ * the iteration count reaches maxIters, the loop and the stop test on the
+ * device. */
+int gvizSpringTutteEmbedderRun(gvizSpringTutteState *s, size_t maxIters, double dt) {
+  if (!s || !s->boundary)
+    return -1;
+  if (s->converged || s->iteration >= maxIters)
+    return (int)s->iteration;
+  if (dt <= 0.0) {
+    /* the reference's Step returns at once and its loop never ends; here
+     * the call returns without stepping */
+    return (int)s->iteration;
+  }
+  s->deviceFresh = 0;
+  if (ensure_device(s) != 0)
+    return -1;
+  enum { kChunk = 65536 };
+  double *deltas = malloc(sizeof(double) * kChunk);
+  if (!deltas)
+    return -1;
+  int failed = 0;
+  while (!s->converged && s->iteration < maxIters) {
+    const size_t left = maxIters - s->iteration;
+    const int64_t want = (int64_t)(left < kChunk ? left : kChunk);
+    const int64_t ran = gvizhip_tutte_spring_steps(s->device, want, dt, s->stiffness,
+                                                   s->damping, s->epsilon, deltas);
+    if (ran <= 0) {
+      failed = 1;
+      break;
+    }
+    for (int64_t i = 0; i < ran; i++)
+      record_step(s, deltas[i]);
+  }
+  free(deltas);
+  if (read_back(s) != 0)
+    failed = 1;
+  return failed ? -1 : (int)s->iteration;
+}

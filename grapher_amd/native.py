@@ -110,6 +110,10 @@ def lib():
         "gvizhip_tutte_upload": (C.c_int, [_vp, _vp, _vp]),
         "gvizhip_tutte_download": (C.c_int, [_vp, _vp]),
         "gvizhip_tutte_sweeps": (_i64, [_vp, _i64, C.c_double, C.c_double, _vp]),
+        "gvizhip_tutte_spring_steps": (_i64, [_vp, _i64, C.c_double, C.c_double, C.c_double,
+                                              C.c_double, _vp]),
+        "gvizhip_tutte_set_velocity": (C.c_int, [_vp, _vp]),
+        "gvizhip_tutte_get_velocity": (C.c_int, [_vp, _vp]),
         "gvizhip_tutte_profile": (C.c_int, [_vp, _i64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gvizhip_tutte_destroy": (None, [_vp]),
         "gvizhip_grip_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
@@ -172,6 +176,7 @@ DECLARED = [
     "gvizhip_force_profile", "gvizhip_force_destroy",
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
     "gvizhip_tutte_sweeps", "gvizhip_tutte_profile", "gvizhip_tutte_destroy",
+    "gvizhip_tutte_spring_steps", "gvizhip_tutte_set_velocity", "gvizhip_tutte_get_velocity",
     "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
     "gvizhip_grip_set_filtration", "gvizhip_grip_knn", "gvizhip_grip_knn_batch",
     "gvizhip_grip_upload", "gvizhip_grip_download",
@@ -385,6 +390,28 @@ class TutteEngine:
         if r < 0:
             _err("gvizhip_tutte_sweeps")
         return (out[:r] if record else None), int(r)
+
+    def spring_steps(self, iters, dt, stiffness=30.0, damping=6.0, epsilon=-1.0,
+                     record=True):
+        """Spring-driven steps (gvizhip_tutte_spring_steps): (max deltas, steps)."""
+        out = np.zeros(max(iters, 1)) if record else None
+        r = lib().gvizhip_tutte_spring_steps(self.h, iters, dt, stiffness, damping, epsilon,
+                                             _p(out))
+        if r < 0:
+            _err("gvizhip_tutte_spring_steps")
+        return (out[:r] if record else None), int(r)
+
+    def set_velocity(self, velocity=None):
+        v = None if velocity is None else np.ascontiguousarray(
+            velocity, dtype=np.float64).reshape(self.n, 2)
+        if lib().gvizhip_tutte_set_velocity(self.h, _p(v)) != 0:
+            _err("gvizhip_tutte_set_velocity")
+
+    def velocity(self):
+        out = np.zeros((self.n, 2))
+        if lib().gvizhip_tutte_get_velocity(self.h, _p(out)) != 0:
+            _err("gvizhip_tutte_get_velocity")
+        return out
 
     def profile(self, sweeps, alpha=1.0):
         """(summed sweep-kernel ms, total ms) over `sweeps` sweeps."""

@@ -157,6 +157,20 @@ int gvizhip_tutte_download(gvizhip_tutte *t, double *positions);
  * max displacement. Returns the number of sweeps run. */
 int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
                              double epsilon, double *maxDeltas);
+/* Spring variant (replaces the loop of gvizSpringTutteEmbedderStep / Run,
+ * src/embedders/gvizSpringTutteEmbedder.c:313-358): `iters` steps of
+ * a = stiffness (barycentre - p) - damping v, v += a dt, p += v dt for every
+ * interior vertex with neighbours, on the same positions and boundary as
+ * the sweeps; the velocity (n x 2, caller's ids, zero after create) is set
+ * with gvizhip_tutte_set_velocity (NULL: zero) and read with
+ * gvizhip_tutte_get_velocity. Stops after the first step whose max delta is
+ * below epsilon (epsilon >= 0); returns the steps taken, maxDeltas
+ * (optional) receives each one's max delta. */
+int64_t gvizhip_tutte_spring_steps(gvizhip_tutte *t, int64_t iters, double dt,
+                                   double stiffness, double damping, double epsilon,
+                                   double *maxDeltas);
+int gvizhip_tutte_set_velocity(gvizhip_tutte *t, const double *velocity);
+int gvizhip_tutte_get_velocity(gvizhip_tutte *t, double *velocity);
 /* Measurement hook (no reference counterpart): `sweeps` sweeps outside the
  * captured graph, HIP events on the engine's stream around each sweep
  * kernel. Outputs (may be NULL): summed sweep-kernel time, total time. */

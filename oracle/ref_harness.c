@@ -18,6 +18,7 @@
 #include "embedders/gvizGRIPEmbedder.h"
 #include "embedders/gvizGRIPInternal.h"
 #include "embedders/gvizTutteEmbedder.h"
+#include "embedders/gvizSpringTutteEmbedder.h"
 #include "core/gvizThreadPool.h"
 #include "utils/graphLoader.h"
 
@@ -625,6 +626,50 @@ static int mode_edges_file(void) {
   return 0;
 }
 
+/* Spring-driven Tutte: FixConvexPolygon + SeedInterior (+ Configure), then
+ * `steps` Step(dt) calls and one Run(runTo, dt). Writes the seeded
+ * positions, each step's return, positions and velocities after the steps,
+ * Run's return, positions and velocities after it, and the
+ * "springTutte.maxDelta" series. Input: graph, boundary count, steps,
+ * runTo, boundary, radius, dt, stiffness, damping, epsilon. */
+static int mode_spring_tutte(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  const int64_t nb = rd_i(), steps = rd_i(), runTo = rd_i();
+  size_t *boundary = malloc(sizeof(size_t) * nb);
+  for (int64_t i = 0; i < nb; i++)
+    boundary[i] = (size_t)rd_i();
+  const double radius = rd_d(), dt = rd_d(), stiffness = rd_d(), damping = rd_d();
+  const double epsilon = rd_d();
+  gvizSpringTutteState s;
+  if (gvizSpringTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, epsilon) != 0)
+    return 1;
+  gvizSpringTutteEmbedderConfigure(&s, stiffness, damping);
+  gvizSpringTutteFixConvexPolygon(&s, boundary, (size_t)nb, radius);
+  gvizSpringTutteEmbedderSeedInterior(&s);
+  s.begun = 1;
+  const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  for (int64_t it = 0; it < steps; it++)
+    wr_d(gvizSpringTutteEmbedderStep(&s, dt));
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(s.velocity[i]);
+  wr_i(gvizSpringTutteEmbedderRun(&s, (size_t)runTo, dt));
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(s.velocity[i]);
+  write_series((gvizEmbeddedGraph *)&s, "springTutte.maxDelta");
+  gvizSpringTutteEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  free(boundary);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 4) {
     fprintf(stderr, "usage: %s mode in out\n", argv[0]);
@@ -657,6 +702,8 @@ int main(int argc, char **argv) {
     rc = mode_time_tutte();
   else if (!strcmp(argv[1], "edges_file"))
     rc = mode_edges_file();
+  else if (!strcmp(argv[1], "spring_tutte"))
+    rc = mode_spring_tutte();
   fclose(fin);
   fclose(fout);
   return rc;

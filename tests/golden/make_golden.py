@@ -270,9 +270,27 @@ def edges_cases():
     save("edges_files", names=np.array(names), **out)
 
 
+def spring_tutte_cases():
+    """Spring-driven Tutte (src/embedders/gvizSpringTutteEmbedder.c): steps
+    and a Run to convergence, default and underdamped settings."""
+    n, e = G.grid(6, 6)
+    ring = [0, 1, 2, 3, 4, 5, 11, 17, 23, 29, 35, 34, 33, 32, 31, 30, 24, 18, 12, 6]
+    cases = [("spring_tutte_grid6", n, e, ring, 12, 400, 1.0 / 60.0, 30.0, 6.0, 1e-5)]
+    n2, e2, _, hull = G.delaunay(300, 3)
+    cases.append(("spring_tutte_delaunay300", n2, e2, hull, 8, 3000, 1.0 / 60.0, 30.0, 6.0,
+                  1e-6))
+    cases.append(("spring_tutte_delaunay300_under", n2, e2, hull, 20, 120, 0.05, 80.0, 2.0,
+                  1e-5))
+    for name, nn, ee, bd, steps, run_to, dt, k, c, eps in cases:
+        r = refio.spring_tutte(nn, ee, bd, steps, run_to, 200.0, dt, k, c, eps)
+        save(name, n=nn, edges=ee, boundary=np.asarray(bd), steps=steps, run_to=run_to,
+             radius=200.0, dt=dt, stiffness=k, damping=c, epsilon=eps,
+             **{key: (v if v is not None else np.zeros(0)) for key, v in r.items()})
+
+
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
              knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
-             api=api_cases, edges=edges_cases)
+             api=api_cases, edges=edges_cases, spring_tutte=spring_tutte_cases)
 
 if __name__ == "__main__":
     if not refio.harness_available():

@@ -308,3 +308,22 @@ def force_checkpoint(n, edges, model, seed, k1, k2, path, exe):
         pos = r.da(2 * n).reshape(n, 2)
         out.append((pos, r.i(), r.d()))
     return out
+
+
+def spring_tutte(n, edges, boundary, steps, run_to, radius, dt, stiffness, damping,
+                 epsilon, exe=None):
+    """Spring-driven Tutte through the reference library (or `exe`): seeded
+    positions, Step returns, positions/velocities after the steps, Run's
+    return, positions/velocities after it, the maxDelta series."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(len(boundary), steps, run_to)
+    w.ia(np.asarray(boundary, dtype=np.int64))
+    w.d(radius, dt, stiffness, damping, epsilon)
+    r = _R(_run("spring_tutte", w.bytes(), exe))
+    out = {"p0": r.da(2 * n).reshape(n, 2), "step": r.da(steps),
+           "p1": r.da(2 * n).reshape(n, 2), "v1": r.da(2 * n).reshape(n, 2),
+           "run": r.i(), "p2": r.da(2 * n).reshape(n, 2), "v2": r.da(2 * n).reshape(n, 2)}
+    c = r.i()
+    out["series"] = r.da(c) if c >= 0 else None
+    return out
