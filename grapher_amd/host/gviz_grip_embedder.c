@@ -16,6 +16,7 @@
 #include "gviz_hip.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -273,6 +274,108 @@ int gvizGRIPEmbedderSyncPositions(gvizGRIPState *s) {
   if (!s || !s->device)
     return -1;
   return gvizhip_grip_download(s->device, host_positions(s), NULL, NULL, NULL);
+}
+
+/* Binary checkpoint: magic, n, kNN capacity, layer count, current layer and
+ * round, last round stats, the filtration (checked on load), the draw mask,
+ * positions, the per-vertex decorators and the kNN lists. */
+static void sync_draw_mask(gvizGRIPState *s);
+
+static const char kGripStateMagic[8] = {'G', 'V', 'Z', 'G', 'R', 'S', '0', '1'};
+
+int gvizGRIPEmbedderSaveState(gvizGRIPState *s, const char *filename) {
+  if (!s || !s->device || !filename || s->layerCount == 0)
+    return -1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  const int64_t n = (int64_t)gvizGraphSize(e->subgraph.g);
+  const int64_t kcap = (int64_t)s->knnCapacity, active = (int64_t)border_at(s, 0);
+  if (s->residentPositions && gvizGRIPEmbedderSyncPositions(s) != 0)
+    return -1;
+  gvizhip_grip_vertex *dec = malloc(sizeof(gvizhip_grip_vertex) * (size_t)(n ? n : 1));
+  int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
+  int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
+  int ok = dec && kv && kd && gvizhip_grip_download(s->device, NULL, dec, kv, kd) == 0;
+  FILE *f = ok ? fopen(filename, "wb") : NULL;
+  ok = ok && f;
+  if (ok) {
+    const int64_t head[6] = {n, kcap, (int64_t)s->layerCount, (int64_t)s->currLayer,
+                             (int64_t)s->currRound, active};
+    const double st[3] = {s->lastRoundStats.maxDisplacement,
+                          s->lastRoundStats.meanDisplacement, s->lastRoundStats.meanForce};
+    ok = fwrite(kGripStateMagic, 1, 8, f) == 8 && fwrite(head, 8, 6, f) == 6 &&
+         fwrite(st, 8, 3, f) == 3;
+    for (int64_t i = 0; ok && i < active; i++) {
+      const int64_t v = (int64_t)s->misFiltration[i];
+      ok = fwrite(&v, 8, 1, f) == 1;
+    }
+    ok = ok && fwrite(e->visibleVertices, 1, (size_t)n, f) == (size_t)n &&
+         fwrite(host_positions(s), sizeof(double), (size_t)(2 * n), f) == (size_t)(2 * n) &&
+         fwrite(dec, sizeof(gvizhip_grip_vertex), (size_t)n, f) == (size_t)n &&
+         fwrite(kv, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap) &&
+         fwrite(kd, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap);
+  }
+  if (f && fclose(f) != 0)
+    ok = 0;
+  free(dec);
+  free(kv);
+  free(kd);
+  return ok ? 0 : -1;
+}
+
+int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
+  if (!s || !s->device || !filename || s->layerCount == 0)
+    return -1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  const int64_t n = (int64_t)gvizGraphSize(e->subgraph.g);
+  FILE *f = fopen(filename, "rb");
+  if (!f)
+    return -1;
+  char magic[8];
+  int64_t head[6];
+  double st[3];
+  int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kGripStateMagic, 8) == 0 &&
+           fread(head, 8, 6, f) == 6 && fread(st, 8, 3, f) == 3 && head[0] == n &&
+           head[1] == (int64_t)s->knnCapacity && head[2] == (int64_t)s->layerCount &&
+           head[3] >= 0 && head[3] < head[2] && head[5] == (int64_t)border_at(s, 0);
+  /* the same graph and configuration give the same filtration */
+  for (int64_t i = 0; ok && i < head[5]; i++) {
+    int64_t v = -1;
+    ok = fread(&v, 8, 1, f) == 1 && v == (int64_t)s->misFiltration[i];
+  }
+  const int64_t kcap = ok ? head[1] : 0;
+  uint8_t *vis = malloc((size_t)(n ? n : 1));
+  double *pos = malloc(sizeof(double) * (size_t)(2 * n ? 2 * n : 1));
+  gvizhip_grip_vertex *dec = malloc(sizeof(gvizhip_grip_vertex) * (size_t)(n ? n : 1));
+  int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
+  int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
+  ok = ok && vis && pos && dec && kv && kd &&
+       fread(vis, 1, (size_t)n, f) == (size_t)n &&
+       fread(pos, sizeof(double), (size_t)(2 * n), f) == (size_t)(2 * n) &&
+       fread(dec, sizeof(gvizhip_grip_vertex), (size_t)n, f) == (size_t)n &&
+       fread(kv, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap) &&
+       fread(kd, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap);
+  fclose(f);
+  if (ok) {
+    memcpy(host_positions(s), pos, sizeof(double) * (size_t)(2 * n));
+    ok = gvizhip_grip_upload(s->device, pos, dec, kv, kd) == 0;
+  }
+  if (ok) {
+    memcpy(e->visibleVertices, vis, (size_t)n);
+    gvizEmbeddedGraphDrawMaskNotifyChanged(e);
+    s->currLayer = (size_t)head[3];
+    s->currRound = (size_t)head[4];
+    s->lastRoundStats.maxDisplacement = st[0];
+    s->lastRoundStats.meanDisplacement = st[1];
+    s->lastRoundStats.meanForce = st[2];
+    s->devicePositionsFresh = 1;
+    sync_draw_mask(s);
+  }
+  free(vis);
+  free(pos);
+  free(dec);
+  free(kv);
+  free(kd);
+  return ok ? 0 : -1;
 }
 
 /* clearDecorators then updateKNNs (src/embedders/gvizGRIPEmbedder.c:936-1006)

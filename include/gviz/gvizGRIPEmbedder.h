@@ -64,6 +64,13 @@ void gvizGRIPEmbedderNextStage(gvizGRIPState *state);
 void gvizGRIPEmbedderRefineRound(gvizGRIPState *state);
 /* Positions computed on the device are copied to the host positions. */
 int gvizGRIPEmbedderSyncPositions(gvizGRIPState *state);
+/* Exact binary checkpoint (no reference counterpart): layer, round, last
+ * round stats, draw mask, positions, every vertex's displacement / heat /
+ * cosine / kNN state and the kNN lists. LoadState needs Begin on the same
+ * graph and configuration first (the filtration is checked); NextStage and
+ * RefineRound then continue bit for bit. 0 or -1. */
+int gvizGRIPEmbedderSaveState(gvizGRIPState *state, const char *filename);
+int gvizGRIPEmbedderLoadState(gvizGRIPState *state, const char *filename);
 
 #ifdef __cplusplus
 }

@@ -327,3 +327,21 @@ def spring_tutte(n, edges, boundary, steps, run_to, radius, dt, stiffness, dampi
     c = r.i()
     out["series"] = r.da(c) if c >= 0 else None
     return out
+
+
+def grip_checkpoint(n, edges, kcap, pk, rk, policy, stages, rounds, path, exe):
+    """This repository's driver: a GRIP run saved mid-schedule and the same
+    continuation in the saved embedder and in one that loaded the state.
+    Returns [(current layer, positions, visibility)] for both."""
+    w = _W()
+    w.graph(n, edges)
+    raw = os.fsencode(path)
+    w.i(kcap, pk, rk, policy, stages, rounds, len(raw))
+    r = _R(_run("grip_checkpoint", w.bytes() + raw, exe))
+    out = []
+    for _ in range(2):
+        layer = r.i()
+        pos = r.da(2 * n).reshape(n, 2)
+        vis = r.ia(n)
+        out.append((layer, pos, vis))
+    return out

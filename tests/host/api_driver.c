@@ -598,6 +598,71 @@ static int mode_force_checkpoint(void) {
   return 0;
 }
 
+/* GRIP checkpoint through the host API: embedder A (Begin, `stages`
+ * NextStage calls, `rounds` RefineRound calls) saves its state to `path`;
+ * then A and a second embedder B that Begins on the same graph and loads the
+ * state run the same continuation (`rounds` rounds, then up to `stages`
+ * times NextStage + `rounds` rounds). Writes, for A then B: current layer,
+ * positions, visibility. Input: graph, kcap, placement K, refinement K,
+ * policy, stages, rounds, path length, path. */
+static void grip_continue(gvizGRIPState *st, int64_t stages, int64_t rounds) {
+  for (int64_t r = 0; r < rounds; r++)
+    gvizGRIPEmbedderRefineRound(st);
+  for (int64_t k = 0; k < stages && st->currLayer > 0; k++) {
+    gvizGRIPEmbedderNextStage(st);
+    for (int64_t r = 0; r < rounds; r++)
+      gvizGRIPEmbedderRefineRound(st);
+  }
+}
+
+static int mode_grip_checkpoint(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  const int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i();
+  const int64_t stages = rd_i(), rounds = rd_i(), len = rd_i();
+  char *path = calloc((size_t)len + 1, 1);
+  if (fread(path, 1, (size_t)len, fin) != (size_t)len)
+    return 2;
+  gvizGRIPState a, b;
+  memset(&a, 0, sizeof(a));
+  memset(&b, 0, sizeof(b));
+  gvizGRIPEmbedderConfigureKnnCapacity(&a, (size_t)kcap);
+  gvizGRIPEmbedderConfigureKnnCapacity(&b, (size_t)kcap);
+  if (gvizGRIPEmbedderInit(&a, gvizSubgraphCreateFull(&g), 0, 2) != 0 ||
+      gvizGRIPEmbedderInit(&b, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+    return 1;
+  gvizGRIPEmbedderConfigureK(&a, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
+  gvizGRIPEmbedderConfigureK(&b, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
+  gvizGRIPEmbedderBegin(&a);
+  for (int64_t k = 0; k < stages && a.currLayer > 0; k++)
+    gvizGRIPEmbedderNextStage(&a);
+  for (int64_t r = 0; r < rounds; r++)
+    gvizGRIPEmbedderRefineRound(&a);
+  if (gvizGRIPEmbedderSaveState(&a, path) != 0)
+    return fail_device("gvizGRIPEmbedderSaveState");
+  grip_continue(&a, stages, rounds);
+  gvizGRIPEmbedderBegin(&b);
+  if (gvizGRIPEmbedderLoadState(&b, path) != 0)
+    return fail_device("gvizGRIPEmbedderLoadState");
+  grip_continue(&b, stages, rounds);
+  free(path);
+  gvizGRIPState *st[2] = {&a, &b};
+  for (int k = 0; k < 2; k++) {
+    gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)st[k];
+    wr_i((int64_t)st[k]->currLayer);
+    const double *p = gvizEmbeddedGraphPositions(e);
+    for (int64_t i = 0; i < 2 * n; i++)
+      wr_d(p[i]);
+    for (int64_t v = 0; v < n; v++)
+      wr_i(gvizEmbeddedGraphIsVertexVisible(e, (size_t)v) ? 1 : 0);
+  }
+  gvizGRIPEmbedderRelease(&a);
+  gvizGRIPEmbedderRelease(&b);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
 /* The .edges loader on a file named in the input (directed, skip_header,
  * path length, path), in the reference harness's framing: return code, then
  * the vertex count and every adjacency list; then the CSR loader's return
@@ -708,6 +773,8 @@ int main(int argc, char **argv) {
     rc = mode_edges_file();
   else if (!strcmp(argv[1], "spring_tutte"))
     rc = mode_spring_tutte();
+  else if (!strcmp(argv[1], "grip_checkpoint"))
+    rc = mode_grip_checkpoint();
   else if (!strcmp(argv[1], "force_checkpoint"))
     rc = mode_force_checkpoint();
   fclose(fin);

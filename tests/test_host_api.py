@@ -263,3 +263,19 @@ def test_force_state_checkpoint_resumes_exactly(gpu, tmp_path, monkeypatch, prec
         n, e, 1, 7, 15, 25, str(tmp_path / "state.bin"), DRIVER)
     np.testing.assert_array_equal(pa, pb)
     assert ia == ib == 40 and ga == gb
+
+
+@pytest.mark.gpu
+def test_grip_state_checkpoint_resumes_exactly(gpu, tmp_path):
+    """gvizGRIPEmbedderSaveState / LoadState: saved after 3 NextStage calls and
+    4 rounds, the continuation (4 rounds, then NextStage + 4 rounds up to 3
+    times) is bit-identical in the saved embedder and in a second one that
+    Begins on the same graph and loads the state."""
+    from grapher_amd import graphs as G
+    n, e = G.grid(50, 50)
+    (la, pa, va), (lb, pb, vb) = refio.grip_checkpoint(
+        n, e, 10, 10, 10, 0, 3, 4, str(tmp_path / "grip.bin"), DRIVER)
+    assert la == lb
+    np.testing.assert_array_equal(va, vb)
+    np.testing.assert_array_equal(pa, pb)
+    assert np.all(np.isfinite(pa[va == 1]))
