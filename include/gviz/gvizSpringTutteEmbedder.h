@@ -9,7 +9,8 @@
  * -1; callers pin a boundary with gvizSpringTutteFixConvexPolygon or
  * gvizSpringTutteEmbedderSetBoundary. The Jacobi snapshot lives on the
  * device, so the state has no scratchPos; velocity is mirrored on the host
- * after every Step / Run.
+ * after every Step / Run. The device step is two-dimensional: Step and Run
+ * fail (negative / -1) for another dimension.
  */
 #ifndef GVIZ_HOST_SPRING_TUTTE_EMBEDDER_H
 #define GVIZ_HOST_SPRING_TUTTE_EMBEDDER_H
