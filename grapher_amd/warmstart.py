@@ -37,8 +37,10 @@ FORCE = dict(model=native.MODEL_FR, barnes_hut=True, theta=0.7, gravity=0.0,
 F_IN = 16
 # the proposal is placed at this fraction of the RMS radius converged
 # layouts of its size have: repulsion then spreads it out, which settles
-# faster than attraction pulling an oversized start together
-START_SCALE = 0.5
+# faster than attraction pulling an oversized start together. Measured on 8
+# unseen 200k SBM graphs: 0.05 gave a mean speedup of 1.66x (median 0.99x),
+# 0.2 1.33x, 0.5 1.32x, 1.0 (with a lower initial speed) 1.11x
+START_SCALE = 0.05
 # pivots of the pivot-MDS input and skip term (gvizhip_gnn_set_pivots)
 PIVOTS = 64
 
