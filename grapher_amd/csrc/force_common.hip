@@ -150,6 +150,83 @@ __global__ __launch_bounds__(kBlock) void k_fix_runs(
   }
 }
 
+// The single-graph sort orders by the top 32 bits of keyHi (the first 16
+// levels) only. A run of equal top halves -- points that share their first 16
+// levels: rare, and long only in tight clusters -- is put in full
+// (keyHi, keyLo, id) order here by the thread at its head, which is the
+// order a stable sort of keyHi followed by k_fix_runs gives: insertion sort
+// up to 32 entries, heap sort (O(L log L), every key distinct once the id is
+// included) beyond.
+__global__ __launch_bounds__(kBlock) void k_fix_runs32(
+    uint64_t *__restrict__ keyHi, const uint64_t *__restrict__ keyLo,
+    uint32_t *__restrict__ vals, int64_t n, const Scalars *__restrict__ sc) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || sc->stopped)
+    return;
+  const uint32_t top = (uint32_t)(keyHi[s] >> 32);
+  if (s > 0 && (uint32_t)(keyHi[s - 1] >> 32) == top)
+    return;
+  int64_t e = s + 1;
+  while (e < n && (uint32_t)(keyHi[e] >> 32) == top)
+    e++;
+  const int64_t len = e - s;
+  if (len < 2)
+    return;
+  uint64_t *kh = keyHi + s;
+  uint32_t *vv = vals + s;
+  auto less = [&](uint64_t ha, uint32_t va, uint64_t hb, uint32_t vb) {
+    if (ha != hb)
+      return ha < hb;
+    const uint64_t la = keyLo[va], lb = keyLo[vb];
+    if (la != lb)
+      return la < lb;
+    return va < vb;
+  };
+  if (len <= 32) {
+    for (int64_t a = 1; a < len; a++) {
+      const uint64_t h = kh[a];
+      const uint32_t v = vv[a];
+      int64_t b = a - 1;
+      while (b >= 0 && less(h, v, kh[b], vv[b])) {
+        kh[b + 1] = kh[b];
+        vv[b + 1] = vv[b];
+        b--;
+      }
+      kh[b + 1] = h;
+      vv[b + 1] = v;
+    }
+    return;
+  }
+  // heap sort: max-heap on the same order, then repeated extraction
+  auto sift = [&](int64_t root, int64_t end) {
+    while (2 * root + 1 < end) {
+      int64_t c = 2 * root + 1;
+      if (c + 1 < end && less(kh[c], vv[c], kh[c + 1], vv[c + 1]))
+        c++;
+      if (!less(kh[root], vv[root], kh[c], vv[c]))
+        return;
+      const uint64_t th = kh[root];
+      kh[root] = kh[c];
+      kh[c] = th;
+      const uint32_t tv = vv[root];
+      vv[root] = vv[c];
+      vv[c] = tv;
+      root = c;
+    }
+  };
+  for (int64_t r = len / 2 - 1; r >= 0; r--)
+    sift(r, len);
+  for (int64_t end = len - 1; end > 0; end--) {
+    const uint64_t th = kh[0];
+    kh[0] = kh[end];
+    kh[end] = th;
+    const uint32_t tv = vv[0];
+    vv[0] = vv[end];
+    vv[end] = tv;
+    sift(0, end);
+  }
+}
+
 // Longest common prefix in bits of the 128-bit keys at sorted positions i, j;
 // identical keys compare their positions (Karras 2012), so delta is unique.
 __device__ __forceinline__ int lcp(const uint64_t *__restrict__ hi,
@@ -573,9 +650,9 @@ int enqueue_keys_and_sort(ForceCommon &c) {
                      (const double2 *)c.pos64.p, n, c.sc.p, c.keyHi.p, c.keyLo.p,
                      c.vals.p);
   GVH_CHECK_LAUNCH();
-  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 0, 64, st))
+  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 32, 64, st))
     return -1;
-  hipLaunchKernelGGL(k_fix_runs, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+  hipLaunchKernelGGL(k_fix_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
                      c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
   GVH_CHECK_LAUNCH();
   return 0;

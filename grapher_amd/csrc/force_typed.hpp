@@ -280,6 +280,43 @@ __device__ __forceinline__ int quad_depth(int delta, int maxDepth) {
 // reference's branching cells (chains of single-child cells collapse onto
 // the deepest one, which carries the same mass and centre of mass and a
 // smaller size, so it takes the same opening decisions).
+// Leaf records (count > 0). A head holding <= npc points is a leaf of the
+// reference directly under its branching parent. A head at the depth limit
+// holding more (an overflow leaf) hangs, when its parent cell lies above
+// depth maxDepth - 1, under a chain of single-child cells that hold exactly
+// its points; the reference's walk tests each of them
+// (src/embedders/gvizForceEmbedder.c:50-63), and the deepest (twice the
+// leaf's size) accepts whenever any does, so the record carries that cell's
+// size, mass and centre of mass (twoh > 0): a target that accepts it takes
+// one pseudo-body, one that rejects it takes the points. twoh = 0: the
+// points always. Returns false for a branching head.
+template <class Real, class PosT>
+__device__ __forceinline__ bool leaf_record(NodeRec<Real, PosT> &r, int32_t count,
+                                            int64_t npc, int q, int qp, int maxDepth,
+                                            int32_t f, int32_t l,
+                                            const double *__restrict__ pm,
+                                            const double *__restrict__ pxh,
+                                            const double *__restrict__ pxl,
+                                            const double *__restrict__ pyh,
+                                            const double *__restrict__ pyl,
+                                            double rootHalf) {
+  if (count > npc && q < maxDepth)
+    return false;
+  r.count = count;
+  r.comx = r.comy = (PosT)0;
+  r.mass = r.twoh = (Real)0;
+  if (count > npc && qp <= maxDepth - 2) {
+    const double M = pm[l + 1] - pm[f];
+    const dd sx = dd_add({pxh[l + 1], pxl[l + 1]}, {-pxh[f], -pxl[f]});
+    const dd sy = dd_add({pyh[l + 1], pyl[l + 1]}, {-pyh[f], -pyl[f]});
+    r.comx = (PosT)((sx.hi + sx.lo) / M);
+    r.comy = (PosT)((sy.hi + sy.lo) / M);
+    r.mass = (Real)M;
+    r.twoh = (Real)(2.0 * ldexp(rootHalf, -(maxDepth - 1)));
+  }
+  return true;
+}
+
 template <class Real, class PosT>
 __global__ __launch_bounds__(kTB) void k_quad(
     const int32_t *__restrict__ left, const int32_t *__restrict__ right,
@@ -301,10 +338,9 @@ __global__ __launch_bounds__(kTB) void k_quad(
   const int32_t count = l - f + 1;
   r.first = f;
   r.child[0] = r.child[1] = r.child[2] = r.child[3] = kNone;
-  if (count <= npc || q >= maxDepth) {
-    r.count = count;
-    r.comx = r.comy = (PosT)0;
-    r.mass = r.twoh = (Real)0;
+  const int qp = b == 0 ? -1 : quad_depth(delta[parent[b]], maxDepth);
+  if (leaf_record(r, count, npc, q, qp, maxDepth, f, l, pm, pxh, pxl, pyh, pyl,
+                  sc->rootHalf)) {
     nodes[b] = r;
     return;
   }
@@ -635,7 +671,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     r.mass = r.twoh = (Real)0;
     if (has && node >= 0) {
       r = nodes[node];
-      if (r.count > 0) {
+      if (r.count > 0 && r.twoh == (Real)0) {
         kind = kRange;
       } else {
         const double cx = (double)r.comx, cy = (double)r.comy;
@@ -650,11 +686,14 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
           kind = kOpen;
         else
           kind = kAmbig;
+        // an overflow leaf its targets open: its points
+        if (kind == kOpen && r.count > 0)
+          kind = kRange;
       }
     } else if (has && -1 - (int64_t)node >= n) {
       // a cell put back by the lanes of its mask that rejected it
       r = nodes[-1 - (int64_t)node - n];
-      kind = kOpen;
+      kind = r.count > 0 ? kRange : kOpen;
     }
     // deferred points and cells
     {
@@ -1409,10 +1448,9 @@ __global__ __launch_bounds__(kTB) void k_quad_g(
   const int32_t count = l - f + 1;
   r.first = f;
   r.child[0] = r.child[1] = r.child[2] = r.child[3] = kNone;
-  if (count <= npc || q >= maxDepth) {
-    r.count = count;
-    r.comx = r.comy = (PosT)0;
-    r.mass = r.twoh = (Real)0;
+  const int qp = top ? -1 : quad_depth(delta[p] - 12, maxDepth);
+  if (leaf_record(r, count, npc, q, qp, maxDepth, f, l, pm, pxh, pxl, pyh, pyl,
+                  sc.rootHalf)) {
     nodes[b] = r;
     return;
   }

@@ -260,3 +260,38 @@ def test_checkpoint_resume_is_bit_identical(gpu, precision):
     c.begin(G.uniform_positions(n2, 1))
     with pytest.raises(RuntimeError):
         c.load_state(state)
+
+
+@pytest.mark.parametrize("npc", [1, 3])
+def test_clustered_points_sort_and_tree_match_oracle(gpu, npc):
+    """Points that share their first 16 quadrant levels reach the sort's run
+    repair: 3,000 inside one 1e-7 square (heap-sort path), 20 inside a 1e-6
+    square (insertion path), 50 exact duplicates and 10 near duplicates,
+    among 5,000 uniform points. Every point's leaf (depth and quadrant path)
+    equals the oracle's quadtree, and the FP64 repulsion interaction counts
+    equal the oracle's: leaves that hold several points at the depth limit
+    are accepted as one body wherever the reference accepts the chain of
+    single-child cells above them."""
+    rng = np.random.default_rng(31)
+    parts = [rng.uniform(-1, 1, (5000, 2)),
+             np.array([0.3, 0.3]) + rng.uniform(0, 1e-7, (3000, 2)),
+             np.array([-0.61, 0.47]) + rng.uniform(0, 1e-6, (20, 2)),
+             np.tile([[0.125, -0.5]], (50, 1)),
+             np.array([0.7, 0.7]) + rng.uniform(0, 1e-13, (10, 2))]
+    for with_tight in (True, False):
+        pts = np.concatenate(parts if with_tight else parts[:1] + parts[2:])
+        pts = pts[rng.permutation(len(pts))]
+        n = len(pts)
+        o = O.quadtree(pts, np.ones(n), npc)
+        eng = gpu.ForceEngine(np.zeros(n + 1, np.int64), np.zeros(0, np.int64),
+                              precision="fp64", nodes_per_cell=npc)
+        eng.set_positions(pts)
+        depth, kh, kl, _ = eng.quadtree_leaves()
+        np.testing.assert_array_equal(depth, o["leaf_depth"])
+        np.testing.assert_array_equal(kh, o["leaf_keyHi"])
+        np.testing.assert_array_equal(kl, o["leaf_keyLo"])
+        if True:
+            got = eng.evaluate(interactions=True)
+            g = O.Graph(np.zeros(n + 1, np.int64), np.zeros(0, np.int64))
+            ref = O.force_compute(g, pts, O.params(npc=npc))
+            np.testing.assert_array_equal(got["interactions"], ref["interactions"])
