@@ -123,40 +123,12 @@ __global__ __launch_bounds__(kBlock) void k_keys(const double2 *__restrict__ pos
   vals[i] = (uint32_t)i;
 }
 
-// The sort orders by keyHi only; a run of equal keyHi (two points sharing
-// their first 32 levels) is put in (keyLo, id) order here. Runs start in id
-// order (the sort is stable), so this is an insertion sort on keyLo.
-__global__ __launch_bounds__(kBlock) void k_fix_runs(
-    const uint64_t *__restrict__ keyHi, const uint64_t *__restrict__ keyLo,
-    uint32_t *__restrict__ vals, int64_t n, const Scalars *__restrict__ sc) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || sc->stopped || sc->maxDepth <= 32)
-    return;
-  uint64_t k = keyHi[s];
-  if (s > 0 && keyHi[s - 1] == k)
-    return;
-  int64_t e = s + 1;
-  while (e < n && keyHi[e] == k)
-    e++;
-  for (int64_t a = s + 1; a < e; a++) {
-    uint32_t v = vals[a];
-    uint64_t kl = keyLo[v];
-    int64_t b = a - 1;
-    while (b >= s && keyLo[vals[b]] > kl) {
-      vals[b + 1] = vals[b];
-      b--;
-    }
-    vals[b + 1] = v;
-  }
-}
-
-// The single-graph sort orders by the top 32 bits of keyHi (the first 16
-// levels) only. A run of equal top halves -- points that share their first 16
-// levels: rare, and long only in tight clusters -- is put in full
-// (keyHi, keyLo, id) order here by the thread at its head, which is the
-// order a stable sort of keyHi followed by k_fix_runs gives: insertion sort
-// up to 32 entries, heap sort (O(L log L), every key distinct once the id is
-// included) beyond.
+// The radix sort orders by the top 32 bits of keyHi only (one graph: the
+// first 16 levels; a batch: the graph id and 10 levels), stable, so equal
+// top halves stay in id order. A run of equal top halves -- rare, and long
+// only in tight clusters -- is put in full (keyHi, keyLo, id) order here by
+// the thread at its head: insertion sort up to 32 entries, heap sort
+// (O(L log L); every key is distinct once the id is included) beyond.
 __global__ __launch_bounds__(kBlock) void k_fix_runs32(
     uint64_t *__restrict__ keyHi, const uint64_t *__restrict__ keyLo,
     uint32_t *__restrict__ vals, int64_t n, const Scalars *__restrict__ sc) {
@@ -667,9 +639,11 @@ int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b) {
                      (const double2 *)c.pos64.p, n, b.gid.p, b.scG.p, c.keyHi.p,
                      c.keyLo.p, c.vals.p);
   GVH_CHECK_LAUNCH();
-  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 0, 64, st))
+  // graph id and the first 10 levels order the keys; runs that share them
+  // are put in full order as for one graph
+  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 32, 64, st))
     return -1;
-  hipLaunchKernelGGL(k_fix_runs, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+  hipLaunchKernelGGL(k_fix_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
                      c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
   GVH_CHECK_LAUNCH();
   return 0;
