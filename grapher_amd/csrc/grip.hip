@@ -28,6 +28,7 @@ using namespace gvh;
 namespace {
 
 constexpr int kB = 256;
+constexpr int kGather = 4; // neighbour loads in flight per thread in k_refine
 constexpr int kWpb = kB / kWave;
 constexpr double kEdge = 10.0;      // GVIZ_EDGE_LENGTH (gvizGRIPEmbedder.c:31)
 constexpr double kGripR = 0.15;     // gvizGRIPr (:45)
@@ -500,6 +501,25 @@ struct GripRound {
 
 // refinementPass1Range + calculateSpringForces + updateLocalTemp
 // (gvizGRIPEmbedder.c:1071-1099, 892-934, 870-889).
+__global__ __launch_bounds__(kB) void k_order_keys(const int32_t *__restrict__ filt,
+                                                   int64_t active,
+                                                   uint64_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < active) {
+    keys[i] = (uint64_t)(uint32_t)filt[i];
+    vals[i] = (uint32_t)i;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_order_ids(const uint64_t *__restrict__ keys,
+                                                  int64_t active,
+                                                  int32_t *__restrict__ order) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < active)
+    order[i] = (int32_t)keys[i];
+}
+
 __global__ __launch_bounds__(kB) void k_refine(
     const int32_t *__restrict__ filt, int64_t active, int64_t layer,
     const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
@@ -520,38 +540,69 @@ __global__ __launch_bounds__(kB) void k_refine(
     double f0 = 0.0, f1 = 0.0;
     const int32_t kc = knnCount[v];
     const int64_t kb = (int64_t)v * kcap;
+    // neighbour positions are gathered kGather at a time ahead of the
+    // (in-order) sums, so each thread keeps several loads in flight
+    double2 pu[kGather];
     if (layer > 0) {
-      for (int32_t j = 0; j < kc; j++) {
-        const double2 pu = pos[knnV[kb + j]];
-        const double L = (double)knnD[kb + j] * kEdge;
-        if (L < 1e-9)
-          continue;
-        double dx = pu.x - pv.x, dy = pu.y - pv.y;
-        double s = sqrt(dx * dx + dy * dy) / L - 1.0;
-        f0 += s * dx;
-        f1 += s * dy;
+      for (int32_t j0 = 0; j0 < kc; j0 += kGather) {
+        const int m = kc - j0 < kGather ? kc - j0 : kGather;
+#pragma unroll
+        for (int k = 0; k < kGather; k++)
+          if (k < m)
+            pu[k] = pos[knnV[kb + j0 + k]];
+#pragma unroll
+        for (int k = 0; k < kGather; k++) {
+          if (k >= m)
+            break;
+          const double L = (double)knnD[kb + j0 + k] * kEdge;
+          if (L < 1e-9)
+            continue;
+          double dx = pu[k].x - pv.x, dy = pu[k].y - pv.y;
+          double s = sqrt(dx * dx + dy * dy) / L - 1.0;
+          f0 += s * dx;
+          f1 += s * dy;
+        }
       }
     } else {
       const double eSq = kEdge * kEdge;
-      for (int32_t e = off[v]; e < off[v + 1]; e++) {
-        const double2 pu = pos[adj[e]];
-        double dx = pu.x - pv.x, dy = pu.y - pv.y;
-        double dsq = dx * dx + dy * dy;
-        if (dsq < 1e-18)
-          continue;
-        double s = dsq / eSq;
-        f0 += s * dx;
-        f1 += s * dy;
+      const int32_t e1 = off[v + 1];
+      for (int32_t e0 = off[v]; e0 < e1; e0 += kGather) {
+        const int m = e1 - e0 < kGather ? e1 - e0 : kGather;
+#pragma unroll
+        for (int k = 0; k < kGather; k++)
+          if (k < m)
+            pu[k] = pos[adj[e0 + k]];
+#pragma unroll
+        for (int k = 0; k < kGather; k++) {
+          if (k >= m)
+            break;
+          double dx = pu[k].x - pv.x, dy = pu[k].y - pv.y;
+          double dsq = dx * dx + dy * dy;
+          if (dsq < 1e-18)
+            continue;
+          double s = dsq / eSq;
+          f0 += s * dx;
+          f1 += s * dy;
+        }
       }
-      for (int32_t j = 0; j < kc; j++) {
-        const double2 pu = pos[knnV[kb + j]];
-        double dx = pv.x - pu.x, dy = pv.y - pu.y;
-        double dsq = dx * dx + dy * dy;
-        if (dsq < 1e-18)
-          continue;
-        double s = kFrScale * eSq / dsq;
-        f0 += s * dx;
-        f1 += s * dy;
+      for (int32_t j0 = 0; j0 < kc; j0 += kGather) {
+        const int m = kc - j0 < kGather ? kc - j0 : kGather;
+#pragma unroll
+        for (int k = 0; k < kGather; k++)
+          if (k < m)
+            pu[k] = pos[knnV[kb + j0 + k]];
+#pragma unroll
+        for (int k = 0; k < kGather; k++) {
+          if (k >= m)
+            break;
+          double dx = pv.x - pu[k].x, dy = pv.y - pu[k].y;
+          double dsq = dx * dx + dy * dy;
+          if (dsq < 1e-18)
+            continue;
+          double s = kFrScale * eSq / dsq;
+          f0 += s * dx;
+          f1 += s * dy;
+        }
       }
     }
     double h = heat[v];
@@ -910,6 +961,10 @@ struct gvizhip_grip {
   DevBuf<uint64_t> keys;
   DevBuf<uint32_t> vals;
   RadixSorter sorter;
+  // the active vertices of the current layer in ascending id: the order the
+  // round kernels walk them in, so per-vertex state is read in sequence
+  DevBuf<int32_t> order;
+  int64_t orderActive = -1;
   int64_t slots = 0;
   int64_t filtCount = 0;
   int64_t blockQueries = -1; // see kBlockQueries
@@ -1164,7 +1219,7 @@ int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
       g->pos.alloc(n) || g->disp.alloc(n) || g->oldDisp.alloc(n) ||
       g->heat.alloc(n) || g->oldCos.alloc(n) || g->lastF.alloc(n) ||
       g->dispCalc.alloc(n) || g->knnV.alloc(n * kcap) || g->knnD.alloc(n * kcap) ||
-      g->knnCount.alloc(n) || g->partial.alloc(4 * 1024) || g->flags.alloc(8) ||
+      g->knnCount.alloc(n) || g->partial.alloc(4 * std::max<int64_t>(1024, (int64_t)grid_for(n > 0 ? n : 1, kB))) || g->flags.alloc(8) ||
       g->round.alloc(1) || g->qSrc.alloc(n) || g->qOut.alloc(n) ||
       g->hitCount.alloc(n)) {
     gvizhip_grip_destroy(g);
@@ -1207,6 +1262,7 @@ int gvizhip_grip_set_filtration(gvizhip_grip *g, const int64_t *filtration,
     GVH_CHECK(hipMemcpy(g->filt.p, f32.data(), sizeof(int32_t) * count,
                         hipMemcpyHostToDevice));
   g->filtCount = count;
+  g->orderActive = -1;
   return 0;
 }
 
@@ -1653,25 +1709,52 @@ namespace {
 // One refinement round (src/embedders/gvizGRIPEmbedder.c:1289-1333) on the
 // engine's stream; f0/f1 (may be null) are recorded around the spring-force
 // kernel.
+// The layer's active vertices (filtration[0..active)) in ascending id, built
+// once per layer with the radix sort. Every vertex's round update reads only
+// the previous state, so the walk order changes nothing but the order of the
+// round's sums (means, rotation terms, statistics).
+int build_order(gvizhip_grip *g, int64_t active) {
+  if (g->orderActive == active)
+    return 0;
+  hipStream_t st = g->stream;
+  if (g->order.alloc((size_t)active) || g->keys.alloc((size_t)active) ||
+      g->vals.alloc((size_t)active))
+    return -1;
+  const unsigned b = grid_for(active, kB);
+  hipLaunchKernelGGL(k_order_keys, dim3(b), dim3(kB), 0, st, g->filt.p, active, g->keys.p,
+                     g->vals.p);
+  GVH_CHECK_LAUNCH();
+  if (g->sorter.sort(g->keys.p, g->vals.p, active, 0, 32, st))
+    return -1;
+  hipLaunchKernelGGL(k_order_ids, dim3(b), dim3(kB), 0, st, g->keys.p, active, g->order.p);
+  GVH_CHECK_LAUNCH();
+  g->orderActive = active;
+  return 0;
+}
+
 int enqueue_round(gvizhip_grip *g, int64_t active, int64_t layer, unsigned *blocksOut,
                   hipEvent_t f0, hipEvent_t f1) {
   hipStream_t st = g->stream;
+  if (build_order(g, active))
+    return -1;
   const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
   if (f0)
     GVH_CHECK(hipEventRecord(f0, st));
-  hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kB), 0, st, g->filt.p, active,
+  // one thread per vertex for the gathers: every CU's wave slots filled
+  const unsigned rblocks = grid_for(active, kB);
+  hipLaunchKernelGGL(k_refine, dim3(rblocks), dim3(kB), 0, st, g->order.p, active,
                      layer, g->off.p, g->adj.p, g->pos.p, g->knnV.p, g->knnD.p,
                      g->knnCount.p, g->kcap, g->disp.p, g->oldDisp.p, g->heat.p,
                      g->oldCos.p, g->lastF.p, g->dispCalc.p, g->partial.p);
   if (f1)
     GVH_CHECK(hipEventRecord(f1, st));
   hipLaunchKernelGGL(k_round_means, dim3(1), dim3(256), 0, st, g->partial.p,
-                     (int)blocks, active, g->round.p);
-  hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, g->filt.p,
+                     (int)rblocks, active, g->round.p);
+  hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, g->order.p,
                      active, g->pos.p, g->disp.p, g->round.p, g->partial.p);
   hipLaunchKernelGGL(k_round_omega, dim3(1), dim3(256), 0, st, g->partial.p,
                      (int)blocks, active, g->round.p);
-  hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, g->filt.p,
+  hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, g->order.p,
                      active, g->pos.p, g->disp.p, g->heat.p, g->lastF.p,
                      g->round.p, g->partial.p);
   hipLaunchKernelGGL(k_round_stats, dim3(1), dim3(256), 0, st, g->partial.p,
