@@ -123,6 +123,11 @@ __global__ __launch_bounds__(kBlock) void k_keys(const double2 *__restrict__ pos
   vals[i] = (uint32_t)i;
 }
 
+#ifndef GVH_SORT_FROM_BIT
+#define GVH_SORT_FROM_BIT 32
+#endif
+constexpr int kSortFrom = GVH_SORT_FROM_BIT; // keyHi bits below this are not radix-sorted
+
 // The radix sort orders by the top 32 bits of keyHi only (one graph: the
 // first 16 levels; a batch: the graph id and 10 levels), stable, so equal
 // top halves stay in id order. A run of equal top halves -- rare, and long
@@ -135,11 +140,11 @@ __global__ __launch_bounds__(kBlock) void k_fix_runs32(
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n || sc->stopped)
     return;
-  const uint32_t top = (uint32_t)(keyHi[s] >> 32);
-  if (s > 0 && (uint32_t)(keyHi[s - 1] >> 32) == top)
+  const uint64_t top = keyHi[s] >> kSortFrom;
+  if (s > 0 && (keyHi[s - 1] >> kSortFrom) == top)
     return;
   int64_t e = s + 1;
-  while (e < n && (uint32_t)(keyHi[e] >> 32) == top)
+  while (e < n && (keyHi[e] >> kSortFrom) == top)
     e++;
   const int64_t len = e - s;
   if (len < 2)
@@ -622,7 +627,7 @@ int enqueue_keys_and_sort(ForceCommon &c) {
                      (const double2 *)c.pos64.p, n, c.sc.p, c.keyHi.p, c.keyLo.p,
                      c.vals.p);
   GVH_CHECK_LAUNCH();
-  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 32, 64, st))
+  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, kSortFrom, 64, st))
     return -1;
   hipLaunchKernelGGL(k_fix_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
                      c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
@@ -641,7 +646,7 @@ int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b) {
   GVH_CHECK_LAUNCH();
   // graph id and the first 10 levels order the keys; runs that share them
   // are put in full order as for one graph
-  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, 32, 64, st))
+  if (c.sorter.sort(c.keyHi.p, c.vals.p, n, kSortFrom, 64, st))
     return -1;
   hipLaunchKernelGGL(k_fix_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
                      c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
