@@ -43,6 +43,33 @@ template <class T> struct V2;
 template <> struct V2<float> { using type = float2; };
 template <> struct V2<double> { using type = double2; };
 
+// Sum of the neighbours' positions over one row, in row order; the
+// positions are loaded kGather at a time ahead of the adds so several
+// gathers are in flight per thread.
+#ifndef GVH_TUTTE_GATHER
+#define GVH_TUTTE_GATHER 8
+#endif
+constexpr int kGather = GVH_TUTTE_GATHER;
+template <class Real>
+__device__ __forceinline__ void row_sum(const int32_t *__restrict__ adj,
+                                        const typename V2<Real>::type *__restrict__ in,
+                                        int32_t b, int32_t e, Real &bx, Real &by) {
+  typename V2<Real>::type p[kGather];
+  for (int32_t k0 = b; k0 < e; k0 += kGather) {
+    const int m = e - k0 < kGather ? e - k0 : kGather;
+#pragma unroll
+    for (int k = 0; k < kGather; k++)
+      if (k < m)
+        p[k] = in[adj[k0 + k]];
+#pragma unroll
+    for (int k = 0; k < kGather; k++)
+      if (k < m) {
+        bx += p[k].x;
+        by += p[k].y;
+      }
+  }
+}
+
 template <class Real>
 __global__ __launch_bounds__(kB) void k_sweep(
     const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
@@ -63,11 +90,7 @@ __global__ __launch_bounds__(kB) void k_sweep(
       continue;
     }
     Real bx = 0, by = 0;
-    for (int32_t k = b; k < e; k++) {
-      typename V2<Real>::type p = in[adj[k]];
-      bx += p.x;
-      by += p.y;
-    }
+    row_sum<Real>(adj, in, b, e, bx, by);
     const Real cnt = (Real)(e - b);
     bx /= cnt;
     by /= cnt;
@@ -119,11 +142,7 @@ __global__ __launch_bounds__(kB) void k_spring(
       continue;
     }
     Real bx = 0, by = 0;
-    for (int32_t k = b; k < e; k++) {
-      typename V2<Real>::type p = in[adj[k]];
-      bx += p.x;
-      by += p.y;
-    }
+    row_sum<Real>(adj, in, b, e, bx, by);
     const Real cnt = (Real)(e - b);
     bx /= cnt;
     by /= cnt;
