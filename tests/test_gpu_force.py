@@ -295,3 +295,34 @@ def test_clustered_points_sort_and_tree_match_oracle(gpu, npc):
             g = O.Graph(np.zeros(n + 1, np.int64), np.zeros(0, np.int64))
             ref = O.force_compute(g, pts, O.params(npc=npc))
             np.testing.assert_array_equal(got["interactions"], ref["interactions"])
+
+
+@pytest.mark.parametrize("barnes_hut", [True, False])
+@pytest.mark.parametrize("model", [0, 1])
+def test_tiny_and_degenerate_graphs_match_oracle(gpu, model, barnes_hut):
+    """FP64 forces of graphs the tree and the traversal treat specially
+    against the oracle: one vertex (gravity only), two vertices, two
+    coincident vertices (the coincident direction rule), three vertices of
+    which one is isolated, and a graph with isolated vertices at one point;
+    with and without overlap prevention and gravity."""
+    cases = [
+        (1, np.zeros((0, 2), np.int64), np.array([[0.3, -0.2]])),
+        (2, np.array([[0, 1]]), np.array([[0.0, 0.0], [1.5, -0.5]])),
+        (2, np.array([[0, 1]]), np.array([[0.25, 0.25], [0.25, 0.25]])),
+        (3, np.array([[0, 1]]), np.array([[0.0, 1.0], [1.0, 0.0], [-2.0, 0.5]])),
+        (6, np.array([[0, 1], [1, 2]]),
+         np.array([[0.0, 0.0], [1.0, 1.0], [2.0, 0.0], [5.0, 5.0], [5.0, 5.0], [5.0, 5.0]])),
+    ]
+    for n, e, pos in cases:
+        off, adj = G.csr_from_edges(n, e)
+        for overlap in (False, True):
+            kw = dict(model=model, barnes_hut=barnes_hut, prevent_overlap=overlap,
+                      gravity=0.05, radius_base=0.1 if overlap else 0.0,
+                      radius_per_degree=0.5 if overlap else 0.0)
+            eng = gpu.ForceEngine(off, adj, precision="fp64", **kw)
+            eng.set_positions(pos)
+            got = eng.evaluate()["disp"]
+            ref = O.force_compute(O.Graph(off, adj), pos, O.params(**kw))["disp"]
+            assert np.all(np.isfinite(got)), (n, overlap)
+            assert np.linalg.norm(got - ref) <= 1e-11 * max(np.linalg.norm(ref), 1e-300), \
+                (n, overlap, got, ref)
