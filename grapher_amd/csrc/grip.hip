@@ -769,6 +769,478 @@ __global__ void k_round_stats(const double *__restrict__ partial, int blocks,
   }
 }
 
+// ------------------------------------ refinement in three and four dimensions
+// The same round as above on positions of D = 3 or 4 coordinates, stored
+// D-strided per vertex. Sums of squares, dot products and the pairwise
+// forces run over the coordinates left to right as gvizVecDot and the
+// gvizVecAcc*Force cases for n = 3, 4 do (include/core/gvizVec.h:42-57,
+// 138-290).
+struct GripRoundN {
+  double mean[4], com[4];
+  double omega[6]; // D = 3: the solved rotation; D = 4: one per coordinate plane
+  int32_t rotate[6];
+  double maxDisp, sumDisp, sumHeat, sumForce;
+};
+
+template <int D> struct PtN {
+  double c[D];
+};
+
+template <int D>
+__device__ __forceinline__ PtN<D> ld_pt(const double *__restrict__ a, int64_t v) {
+  PtN<D> p;
+#pragma unroll
+  for (int k = 0; k < D; k++)
+    p.c[k] = a[v * D + k];
+  return p;
+}
+
+template <int D>
+__device__ __forceinline__ void st_pt(double *__restrict__ a, int64_t v, const PtN<D> &p) {
+#pragma unroll
+  for (int k = 0; k < D; k++)
+    a[v * D + k] = p.c[k];
+}
+
+template <int D> __device__ __forceinline__ double sq_sum(const double (&d)[D]) {
+  double s = d[0] * d[0];
+#pragma unroll
+  for (int k = 1; k < D; k++)
+    s += d[k] * d[k];
+  return s;
+}
+
+// refinementPass1Range, calculateSpringForces and updateLocalTemp
+// (gvizGRIPEmbedder.c:1071-1099, 892-934, 870-889) for D coordinates.
+template <int D>
+__global__ __launch_bounds__(kB) void k_refine_nd(
+    const int32_t *__restrict__ order, int64_t active, int64_t layer,
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const double *__restrict__ pos, const int32_t *__restrict__ knnV,
+    const int32_t *__restrict__ knnD, const int32_t *__restrict__ knnCount,
+    int64_t kcap, double *__restrict__ disp, double *__restrict__ oldDisp,
+    double *__restrict__ heat, double *__restrict__ oldCos,
+    double *__restrict__ lastF, uint8_t *__restrict__ dispCalc,
+    double *__restrict__ partial) {
+  __shared__ double sm[2 * D * 16];
+  double acc[2 * D];
+#pragma unroll
+  for (int k = 0; k < 2 * D; k++)
+    acc[k] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = order[i];
+    const PtN<D> d0 = ld_pt<D>(disp, v);
+    st_pt<D>(oldDisp, v, d0);
+    const PtN<D> pv = ld_pt<D>(pos, v);
+    double f[D], dd[D];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+      f[k] = 0.0;
+    const int32_t kc = knnCount[v];
+    const int64_t kb = (int64_t)v * kcap;
+    if (layer > 0) {
+      for (int32_t j = 0; j < kc; j++) {
+        const double L = (double)knnD[kb + j] * kEdge;
+        if (L < 1e-9)
+          continue;
+        const PtN<D> pu = ld_pt<D>(pos, knnV[kb + j]);
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          dd[k] = pu.c[k] - pv.c[k];
+        const double s = sqrt(sq_sum<D>(dd)) / L - 1.0;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          f[k] += s * dd[k];
+      }
+    } else {
+      const double eSq = kEdge * kEdge;
+      for (int32_t e = off[v]; e < off[v + 1]; e++) {
+        const PtN<D> pu = ld_pt<D>(pos, adj[e]);
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          dd[k] = pu.c[k] - pv.c[k];
+        const double dsq = sq_sum<D>(dd);
+        if (dsq < 1e-18)
+          continue;
+        const double s = dsq / eSq;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          f[k] += s * dd[k];
+      }
+      for (int32_t j = 0; j < kc; j++) {
+        const PtN<D> pu = ld_pt<D>(pos, knnV[kb + j]);
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          dd[k] = pv.c[k] - pu.c[k];
+        const double dsq = sq_sum<D>(dd);
+        if (dsq < 1e-18)
+          continue;
+        const double s = kFrScale * eSq / dsq;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+          f[k] += s * dd[k];
+      }
+    }
+    double h = heat[v];
+    if (!dispCalc[v]) {
+      dispCalc[v] = 1;
+      h = kEdge / 6.0;
+    } else {
+      const double nrm = sqrt(sq_sum<D>(f));
+      const double oldNrm = sqrt(sq_sum<D>(d0.c));
+      if (!(fabs(nrm) < kEps || fabs(oldNrm) < kEps)) {
+        double dot = f[0] * d0.c[0];
+#pragma unroll
+        for (int k = 1; k < D; k++)
+          dot += f[k] * d0.c[k];
+        const double cs = dot / (nrm * oldNrm);
+        if (cs > 0 && oldCos[v] > 0)
+          h *= (1 + cs * kGripR * kGripS);
+        else
+          h *= (1 + cs * kGripR);
+        oldCos[v] = cs;
+      }
+    }
+    h = fmin(h, kEdge * 10);
+    h = fmax(h, kEdge * 1e-4);
+    heat[v] = h;
+    const double nrm = sqrt(sq_sum<D>(f));
+    lastF[v] = nrm;
+    if (nrm > kEps) {
+      const double s = h / nrm;
+#pragma unroll
+      for (int k = 0; k < D; k++)
+        f[k] *= s;
+    }
+    PtN<D> out;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      out.c[k] = f[k];
+      acc[k] += f[k];
+      acc[D + k] += pv.c[k];
+    }
+    st_pt<D>(disp, v, out);
+  }
+  bool isMax[2 * D];
+#pragma unroll
+  for (int k = 0; k < 2 * D; k++)
+    isMax[k] = false;
+  block_reduce<2 * D>(acc, isMax, sm);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 2 * D; k++)
+      partial[2 * D * blockIdx.x + k] = acc[k];
+}
+
+// removeNetTranslation's mean (gvizGRIPEmbedder.c:1106-1121) and the centre
+// of removeNetRotation (:1198-1206).
+template <int D>
+__global__ void k_round_means_nd(const double *__restrict__ partial, int blocks,
+                                 int64_t active, GripRoundN *__restrict__ r) {
+  __shared__ double sm[2 * D * 16];
+  double v[2 * D];
+  bool isMax[2 * D];
+#pragma unroll
+  for (int k = 0; k < 2 * D; k++) {
+    v[k] = 0.0;
+    isMax[k] = false;
+  }
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 2 * D; k++)
+      v[k] += partial[2 * D * b + k];
+  block_reduce<2 * D>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    const double inv = 1.0 / (double)active;
+    for (int k = 0; k < D; k++) {
+      r->mean[k] = v[k] * inv;
+      r->com[k] = v[D + k] / (double)active;
+    }
+  }
+}
+
+// Translation removal and the sums of the 3-D branch of removeNetRotation
+// (gvizGRIPEmbedder.c:1216-1235): the angular momentum L and the inertia
+// tensor, whose off-diagonal entries are symmetric term by term.
+__global__ __launch_bounds__(kB) void k_rotation3_sums(
+    const int32_t *__restrict__ order, int64_t active, const double *__restrict__ pos,
+    double *__restrict__ disp, const GripRoundN *__restrict__ r,
+    double *__restrict__ partial) {
+  __shared__ double sm[9 * 16];
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double m0 = r->mean[0], m1 = r->mean[1], m2 = r->mean[2];
+  const double c0 = r->com[0], c1 = r->com[1], c2 = r->com[2];
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = order[i];
+    PtN<3> d = ld_pt<3>(disp, v);
+    d.c[0] += -1.0 * m0;
+    d.c[1] += -1.0 * m1;
+    d.c[2] += -1.0 * m2;
+    st_pt<3>(disp, v, d);
+    const PtN<3> p = ld_pt<3>(pos, v);
+    const double r0 = p.c[0] - c0, r1 = p.c[1] - c1, r2 = p.c[2] - c2;
+    acc[0] += r1 * d.c[2] - r2 * d.c[1];
+    acc[1] += r2 * d.c[0] - r0 * d.c[2];
+    acc[2] += r0 * d.c[1] - r1 * d.c[0];
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    acc[3] += rr - r0 * r0;   // I00
+    acc[4] += 0.0 - r0 * r1;  // I01 = I10
+    acc[5] += 0.0 - r0 * r2;  // I02 = I20
+    acc[6] += rr - r1 * r1;   // I11
+    acc[7] += 0.0 - r1 * r2;  // I12 = I21
+    acc[8] += rr - r2 * r2;   // I22
+  }
+  const bool isMax[9] = {false, false, false, false, false, false, false, false, false};
+  block_reduce<9>(acc, isMax, sm);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 9; k++)
+      partial[9 * blockIdx.x + k] = acc[k];
+}
+
+// solve3x3 (gvizGRIPEmbedder.c:1122-1152): Gaussian elimination with the
+// row of largest magnitude as pivot, rows swapped through an index table;
+// -1 when a pivot is below gvizNumericEpsilon.
+__device__ int solve3(double (&a)[3][3], const double (&b)[3], double (&x)[3]) {
+  int idx[3] = {0, 1, 2};
+  double rhs[3] = {b[0], b[1], b[2]};
+  for (int col = 0; col < 3; col++) {
+    int piv = col;
+    for (int row = col + 1; row < 3; row++)
+      if (fabs(a[idx[row]][col]) > fabs(a[idx[piv]][col]))
+        piv = row;
+    if (fabs(a[idx[piv]][col]) < kEps)
+      return -1;
+    const int t = idx[col];
+    idx[col] = idx[piv];
+    idx[piv] = t;
+    for (int row = col + 1; row < 3; row++) {
+      const double f = a[idx[row]][col] / a[idx[col]][col];
+      for (int j = col; j < 3; j++)
+        a[idx[row]][j] -= f * a[idx[col]][j];
+      rhs[idx[row]] -= f * rhs[idx[col]];
+    }
+  }
+  for (int col = 2; col >= 0; col--) {
+    double s = rhs[idx[col]];
+    for (int j = col + 1; j < 3; j++)
+      s -= a[idx[col]][j] * x[j];
+    x[col] = s / a[idx[col]][col];
+  }
+  return 0;
+}
+
+__global__ void k_rotation3_solve(const double *__restrict__ partial, int blocks,
+                                  int64_t active, GripRoundN *__restrict__ r) {
+  __shared__ double sm[9 * 16];
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+    for (int k = 0; k < 9; k++)
+      v[k] += partial[9 * b + k];
+  const bool isMax[9] = {false, false, false, false, false, false, false, false, false};
+  block_reduce<9>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    double a[3][3] = {{v[3], v[4], v[5]}, {v[4], v[6], v[7]}, {v[5], v[7], v[8]}};
+    const double L[3] = {v[0], v[1], v[2]};
+    double w[3] = {0.0, 0.0, 0.0};
+    const int ok = active >= 3 && solve3(a, L, w) == 0;
+    r->rotate[0] = ok;
+    for (int k = 0; k < 3; k++)
+      r->omega[k] = ok ? w[k] : 0.0;
+  }
+}
+
+// The six coordinate planes of the 4-D branch, removed one after another
+// (removeNetRotationPlane, gvizGRIPEmbedder.c:1156-1185, called from
+// :1208-1215).
+__constant__ int kPlaneA[6] = {0, 0, 0, 1, 1, 2};
+__constant__ int kPlaneB[6] = {1, 2, 3, 2, 3, 3};
+
+// Plane `plane`'s sums over the displacements that the previous plane's
+// removal (or, for the first plane, the translation removal) leaves.
+__global__ __launch_bounds__(kB) void k_plane_sums(
+    const int32_t *__restrict__ order, int64_t active, const double *__restrict__ pos,
+    double *__restrict__ disp, const GripRoundN *__restrict__ r, int plane,
+    double *__restrict__ partial) {
+  __shared__ double sm[2 * 16];
+  double acc[2] = {0.0, 0.0};
+  const int a = kPlaneA[plane], b = kPlaneB[plane];
+  const bool first = plane == 0;
+  const bool prevRot = !first && r->rotate[plane - 1] != 0;
+  const int pa = first ? 0 : kPlaneA[plane - 1], pb = first ? 0 : kPlaneB[plane - 1];
+  const double pw = first ? 0.0 : r->omega[plane - 1];
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = order[i];
+    PtN<4> d = ld_pt<4>(disp, v);
+    const PtN<4> p = ld_pt<4>(pos, v);
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        d.c[k] += -1.0 * r->mean[k];
+    } else if (prevRot) {
+      const double ra = p.c[pa] - r->com[pa], rb = p.c[pb] - r->com[pb];
+      d.c[pa] += pw * rb;
+      d.c[pb] -= pw * ra;
+    }
+    if (first || prevRot)
+      st_pt<4>(disp, v, d);
+    const double ra = p.c[a] - r->com[a], rb = p.c[b] - r->com[b];
+    acc[0] += ra * d.c[b] - rb * d.c[a];
+    acc[1] += ra * ra + rb * rb;
+  }
+  const bool isMax[2] = {false, false};
+  block_reduce<2>(acc, isMax, sm);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = acc[0];
+    partial[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+__global__ void k_plane_omega(const double *__restrict__ partial, int blocks,
+                              int64_t active, GripRoundN *__restrict__ r, int plane) {
+  __shared__ double sm[2 * 16];
+  double v[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] += partial[2 * b];
+    v[1] += partial[2 * b + 1];
+  }
+  const bool isMax[2] = {false, false};
+  block_reduce<2>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    const int ok = active >= 3 && !(v[1] < kEps);
+    r->rotate[plane] = ok;
+    r->omega[plane] = ok ? v[0] / v[1] : 0.0;
+  }
+}
+
+// The last rotation fix-up (the solved 3-D rotation, :1246-1260, or the
+// last 4-D plane), the position update and the round statistics
+// (:1299-1318).
+template <int D>
+__global__ __launch_bounds__(kB) void k_round_apply_nd(
+    const int32_t *__restrict__ order, int64_t active, double *__restrict__ pos,
+    double *__restrict__ disp, const double *__restrict__ heat,
+    const double *__restrict__ lastF, const GripRoundN *__restrict__ r,
+    double *__restrict__ partial) {
+  __shared__ double sm[4 * 16];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool rot = D == 3 ? r->rotate[0] != 0 : r->rotate[5] != 0;
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const int32_t v = order[i];
+    PtN<D> d = ld_pt<D>(disp, v);
+    PtN<D> p = ld_pt<D>(pos, v);
+    if (rot) {
+      if constexpr (D == 3) {
+        const double w0 = r->omega[0], w1 = r->omega[1], w2 = r->omega[2];
+        const double r0 = p.c[0] - r->com[0], r1 = p.c[1] - r->com[1],
+                     r2 = p.c[2] - r->com[2];
+        d.c[0] -= w1 * r2 - w2 * r1;
+        d.c[1] -= w2 * r0 - w0 * r2;
+        d.c[2] -= w0 * r1 - w1 * r0;
+      } else {
+        const double w = r->omega[5];
+        const double ra = p.c[2] - r->com[2], rb = p.c[3] - r->com[3];
+        d.c[2] += w * rb;
+        d.c[3] -= w * ra;
+      }
+      st_pt<D>(disp, v, d);
+    }
+    const double nrm = sqrt(sq_sum<D>(d.c));
+    acc[0] = nrm > acc[0] ? nrm : acc[0];
+    acc[1] += nrm;
+    acc[2] += heat[v];
+    acc[3] += lastF[v];
+#pragma unroll
+    for (int k = 0; k < D; k++)
+      p.c[k] += d.c[k];
+    st_pt<D>(pos, v, p);
+  }
+  const bool isMax[4] = {true, false, false, false};
+  block_reduce<4>(acc, isMax, sm);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++)
+      partial[4 * blockIdx.x + k] = acc[k];
+}
+
+__global__ void k_round_stats_nd(const double *__restrict__ partial, int blocks,
+                                 GripRoundN *__restrict__ r) {
+  __shared__ double sm[4 * 16];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    v[0] = fmax(v[0], partial[4 * b]);
+    for (int k = 1; k < 4; k++)
+      v[k] += partial[4 * b + k];
+  }
+  const bool isMax[4] = {true, false, false, false};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    r->maxDisp = v[0];
+    r->sumDisp = v[1];
+    r->sumHeat = v[2];
+    r->sumForce = v[3];
+  }
+}
+
+// clearDecorators (gvizGRIPEmbedder.c:936-952) for D coordinates.
+template <int D>
+__global__ __launch_bounds__(kB) void k_clear_nd(const int32_t *__restrict__ filt,
+                                                 int64_t active, int64_t n, double *disp,
+                                                 double *oldDisp, double *heat,
+                                                 double *oldCos, int32_t *knnCount,
+                                                 uint8_t *dispCalc) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n)
+    dispCalc[i] = 0;
+  if (i < active) {
+    const int32_t v = filt[i];
+    heat[v] = 0.0;
+    oldCos[v] = 0.0;
+    knnCount[v] = 0;
+    PtN<D> z;
+#pragma unroll
+    for (int k = 0; k < D; k++)
+      z.c[k] = 0.0;
+    st_pt<D>(disp, v, z);
+    st_pt<D>(oldDisp, v, z);
+  }
+}
+
+// gripBarycenter (gvizGRIPEmbedder.c:21-27) for D coordinates.
+template <int D>
+__global__ __launch_bounds__(kB) void k_place_nd(
+    const int32_t *__restrict__ filt, int64_t begin, int64_t end,
+    const int32_t *__restrict__ hitV, const int32_t *__restrict__ hitCount,
+    int64_t stride, double *__restrict__ pos, int32_t *__restrict__ empty) {
+  const int64_t i = begin + (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= end)
+    return;
+  const int64_t q = i - begin;
+  const int32_t c = hitCount[q];
+  if (c <= 0) {
+    atomicOr(empty, 1);
+    return;
+  }
+  double o[D];
+#pragma unroll
+  for (int k = 0; k < D; k++)
+    o[k] = 0.0;
+  for (int32_t j = 0; j < c; j++) {
+    const PtN<D> p = ld_pt<D>(pos, hitV[q * stride + j]);
+#pragma unroll
+    for (int k = 0; k < D; k++)
+      o[k] += 1.0 * p.c[k];
+  }
+  const double s = 1.0 / (double)c;
+  PtN<D> out;
+#pragma unroll
+  for (int k = 0; k < D; k++)
+    out.c[k] = o[k] * s;
+  st_pt<D>(pos, filt[i], out);
+}
+
 // clearDecorators (gvizGRIPEmbedder.c:936-952)
 __global__ __launch_bounds__(kB) void k_clear(const int32_t *__restrict__ filt,
                                               int64_t active, int64_t n,
@@ -958,6 +1430,11 @@ struct gvizhip_grip {
   DevBuf<int32_t> queue, qdist;
   DevBuf<int32_t> table, targetMap;
   DevBuf<GripRound> round;
+  // three- and four-dimensional layouts: dim-strided positions and
+  // displacements (the double2 buffers above serve dim == 2)
+  int64_t dim = 2;
+  DevBuf<double> posN, dispN, oldDispN;
+  DevBuf<GripRoundN> roundN;
   DevBuf<uint64_t> keys;
   DevBuf<uint32_t> vals;
   RadixSorter sorter;
@@ -1527,6 +2004,134 @@ int gvizhip_grip_knn_batch(gvizhip_grip *g, const uint8_t *visible,
   return 0;
 }
 
+int gvizhip_grip_set_dimension(gvizhip_grip *g, int64_t dim) {
+  if (!g || dim < 2 || dim > 4) {
+    set_error("gvizhip_grip_set_dimension: dimension must be 2, 3 or 4");
+    return -1;
+  }
+  g->dim = dim;
+  if (dim == 2)
+    return 0;
+  const int64_t n = g->n;
+  const size_t nd = (size_t)(n * dim);
+  const int64_t parts =
+      std::max<int64_t>(9 * 1024, 2 * dim * (int64_t)grid_for(n > 0 ? n : 1, kB));
+  if (g->posN.alloc(nd) || g->dispN.alloc(nd) || g->oldDispN.alloc(nd) ||
+      g->roundN.alloc(1) || g->partial.alloc((size_t)parts))
+    return -1;
+  GVH_CHECK(hipMemsetAsync(g->posN.p, 0, sizeof(double) * nd, g->stream));
+  GVH_CHECK(hipMemsetAsync(g->dispN.p, 0, sizeof(double) * nd, g->stream));
+  GVH_CHECK(hipMemsetAsync(g->oldDispN.p, 0, sizeof(double) * nd, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int gvizhip_grip_upload_nd(gvizhip_grip *g, const double *positions,
+                           const gvizhip_grip_vertex_nd *dec, const int64_t *knnVertex,
+                           const int64_t *knnDist) {
+  if (!g) {
+    set_error("gvizhip_grip_upload_nd: null context");
+    return -1;
+  }
+  const int64_t n = g->n, D = g->dim;
+  if (D == 2) {
+    std::vector<gvizhip_grip_vertex> d2(dec ? n : 0);
+    for (int64_t v = 0; dec && v < n; v++)
+      d2[v] = {dec[v].disp[0], dec[v].disp[1], dec[v].oldDisp[0], dec[v].oldDisp[1],
+               dec[v].heat, dec[v].oldCos, dec[v].lastForceMag, dec[v].knnCount,
+               dec[v].dispCalculated};
+    return gvizhip_grip_upload(g, positions, dec ? d2.data() : nullptr, knnVertex,
+                               knnDist);
+  }
+  if (positions)
+    GVH_CHECK(hipMemcpy(g->posN.p, positions, sizeof(double) * n * D,
+                        hipMemcpyHostToDevice));
+  if (dec) {
+    std::vector<double> d(n * D), od(n * D), h(n), oc(n), lf(n);
+    std::vector<uint8_t> dc(n);
+    std::vector<int32_t> kc(n);
+    for (int64_t v = 0; v < n; v++) {
+      for (int64_t k = 0; k < D; k++) {
+        d[v * D + k] = dec[v].disp[k];
+        od[v * D + k] = dec[v].oldDisp[k];
+      }
+      h[v] = dec[v].heat;
+      oc[v] = dec[v].oldCos;
+      lf[v] = dec[v].lastForceMag;
+      dc[v] = dec[v].dispCalculated ? 1 : 0;
+      kc[v] = (int32_t)dec[v].knnCount;
+    }
+    GVH_CHECK(hipMemcpy(g->dispN.p, d.data(), sizeof(double) * n * D, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->oldDispN.p, od.data(), sizeof(double) * n * D,
+                        hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->heat.p, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->oldCos.p, oc.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->lastF.p, lf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->dispCalc.p, dc.data(), n, hipMemcpyHostToDevice));
+    GVH_CHECK(hipMemcpy(g->knnCount.p, kc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  }
+  return gvizhip_grip_upload(g, nullptr, nullptr, knnVertex, knnDist);
+}
+
+int gvizhip_grip_download_nd(gvizhip_grip *g, double *positions,
+                             gvizhip_grip_vertex_nd *dec, int64_t *knnVertex,
+                             int64_t *knnDist) {
+  if (!g) {
+    set_error("gvizhip_grip_download_nd: null context");
+    return -1;
+  }
+  const int64_t n = g->n, D = g->dim;
+  if (D == 2) {
+    std::vector<gvizhip_grip_vertex> d2(dec ? n : 0);
+    if (gvizhip_grip_download(g, positions, dec ? d2.data() : nullptr, knnVertex,
+                              knnDist))
+      return -1;
+    for (int64_t v = 0; dec && v < n; v++) {
+      dec[v] = gvizhip_grip_vertex_nd{};
+      dec[v].disp[0] = d2[v].dispX;
+      dec[v].disp[1] = d2[v].dispY;
+      dec[v].oldDisp[0] = d2[v].oldDispX;
+      dec[v].oldDisp[1] = d2[v].oldDispY;
+      dec[v].heat = d2[v].heat;
+      dec[v].oldCos = d2[v].oldCos;
+      dec[v].lastForceMag = d2[v].lastForceMag;
+      dec[v].knnCount = d2[v].knnCount;
+      dec[v].dispCalculated = d2[v].dispCalculated;
+    }
+    return 0;
+  }
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  if (positions)
+    GVH_CHECK(hipMemcpy(positions, g->posN.p, sizeof(double) * n * D,
+                        hipMemcpyDeviceToHost));
+  if (dec) {
+    std::vector<double> d(n * D), od(n * D), h(n), oc(n), lf(n);
+    std::vector<uint8_t> dc(n);
+    std::vector<int32_t> kc(n);
+    GVH_CHECK(hipMemcpy(d.data(), g->dispN.p, sizeof(double) * n * D, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(od.data(), g->oldDispN.p, sizeof(double) * n * D,
+                        hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(h.data(), g->heat.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(oc.data(), g->oldCos.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(lf.data(), g->lastF.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(dc.data(), g->dispCalc.p, n, hipMemcpyDeviceToHost));
+    GVH_CHECK(hipMemcpy(kc.data(), g->knnCount.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    for (int64_t v = 0; v < n; v++) {
+      dec[v] = gvizhip_grip_vertex_nd{};
+      for (int64_t k = 0; k < D; k++) {
+        dec[v].disp[k] = d[v * D + k];
+        dec[v].oldDisp[k] = od[v * D + k];
+      }
+      dec[v].heat = h[v];
+      dec[v].oldCos = oc[v];
+      dec[v].lastForceMag = lf[v];
+      dec[v].dispCalculated = dc[v];
+      dec[v].knnCount = kc[v];
+    }
+  }
+  return gvizhip_grip_download(g, nullptr, nullptr, knnVertex, knnDist);
+}
+
 int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
                         const gvizhip_grip_vertex *dec, const int64_t *knnVertex,
                         const int64_t *knnDist) {
@@ -1535,6 +2140,16 @@ int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
     return -1;
   }
   const int64_t n = g->n;
+  if (g->dim != 2 && (positions || dec)) {
+    if (dec) {
+      set_error("gvizhip_grip_upload: the vertex state of a %lld-dimensional layout "
+                "goes through gvizhip_grip_upload_nd", (long long)g->dim);
+      return -1;
+    }
+    GVH_CHECK(hipMemcpy(g->posN.p, positions, sizeof(double) * n * g->dim,
+                        hipMemcpyHostToDevice));
+    positions = nullptr;
+  }
   if (positions)
     GVH_CHECK(hipMemcpy(g->pos.p, positions, sizeof(double2) * n,
                         hipMemcpyHostToDevice));
@@ -1584,6 +2199,16 @@ int gvizhip_grip_download(gvizhip_grip *g, double *positions, gvizhip_grip_verte
   }
   const int64_t n = g->n;
   GVH_CHECK(hipStreamSynchronize(g->stream));
+  if (g->dim != 2 && (positions || dec)) {
+    if (dec) {
+      set_error("gvizhip_grip_download: the vertex state of a %lld-dimensional layout "
+                "goes through gvizhip_grip_download_nd", (long long)g->dim);
+      return -1;
+    }
+    GVH_CHECK(hipMemcpy(positions, g->posN.p, sizeof(double) * n * g->dim,
+                        hipMemcpyDeviceToHost));
+    positions = nullptr;
+  }
   if (positions)
     GVH_CHECK(hipMemcpy(positions, g->pos.p, sizeof(double2) * n,
                         hipMemcpyDeviceToHost));
@@ -1631,9 +2256,18 @@ int gvizhip_grip_clear_decorators(gvizhip_grip *g, int64_t active) {
     set_error("gvizhip_grip_clear_decorators: invalid arguments");
     return -1;
   }
-  hipLaunchKernelGGL(k_clear, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
-                     g->filt.p, active, g->n, g->disp.p, g->oldDisp.p, g->heat.p,
-                     g->oldCos.p, g->knnCount.p, g->dispCalc.p);
+  if (g->dim == 3)
+    hipLaunchKernelGGL(k_clear_nd<3>, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
+                       g->filt.p, active, g->n, g->dispN.p, g->oldDispN.p, g->heat.p,
+                       g->oldCos.p, g->knnCount.p, g->dispCalc.p);
+  else if (g->dim == 4)
+    hipLaunchKernelGGL(k_clear_nd<4>, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
+                       g->filt.p, active, g->n, g->dispN.p, g->oldDispN.p, g->heat.p,
+                       g->oldCos.p, g->knnCount.p, g->dispCalc.p);
+  else
+    hipLaunchKernelGGL(k_clear, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
+                       g->filt.p, active, g->n, g->disp.p, g->oldDisp.p, g->heat.p,
+                       g->oldCos.p, g->knnCount.p, g->dispCalc.p);
   GVH_CHECK_LAUNCH();
   return 0;
 }
@@ -1688,9 +2322,18 @@ int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
   }
   if (set_flag(g, 3, 0))
     return -1;
-  hipLaunchKernelGGL(k_place, dim3(grid_for(count, kB)), dim3(kB), 0, g->stream,
-                     g->filt.p, begin, end, g->hitV.p, g->hitCount.p, k, g->pos.p,
-                     g->flags.p + 3);
+  if (g->dim == 3)
+    hipLaunchKernelGGL(k_place_nd<3>, dim3(grid_for(count, kB)), dim3(kB), 0, g->stream,
+                       g->filt.p, begin, end, g->hitV.p, g->hitCount.p, k, g->posN.p,
+                       g->flags.p + 3);
+  else if (g->dim == 4)
+    hipLaunchKernelGGL(k_place_nd<4>, dim3(grid_for(count, kB)), dim3(kB), 0, g->stream,
+                       g->filt.p, begin, end, g->hitV.p, g->hitCount.p, k, g->posN.p,
+                       g->flags.p + 3);
+  else
+    hipLaunchKernelGGL(k_place, dim3(grid_for(count, kB)), dim3(kB), 0, g->stream,
+                       g->filt.p, begin, end, g->hitV.p, g->hitCount.p, k, g->pos.p,
+                       g->flags.p + 3);
   GVH_CHECK_LAUNCH();
   int32_t empty = 0;
   if (read_flag(g, 3, &empty))
@@ -1732,11 +2375,58 @@ int build_order(gvizhip_grip *g, int64_t active) {
   return 0;
 }
 
+// The round for D = 3 or 4 coordinates: the refinement pass, the means,
+// the rotation removal (one solved 3-D rotation, or the six 4-D coordinate
+// planes in turn), then the position update and statistics.
+template <int D>
+int enqueue_round_nd(gvizhip_grip *g, int64_t active, int64_t layer, hipEvent_t f0,
+                     hipEvent_t f1) {
+  hipStream_t st = g->stream;
+  const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
+  const unsigned rblocks = grid_for(active, kB);
+  if (f0)
+    GVH_CHECK(hipEventRecord(f0, st));
+  hipLaunchKernelGGL(k_refine_nd<D>, dim3(rblocks), dim3(kB), 0, st, g->order.p, active,
+                     layer, g->off.p, g->adj.p, g->posN.p, g->knnV.p, g->knnD.p,
+                     g->knnCount.p, g->kcap, g->dispN.p, g->oldDispN.p, g->heat.p,
+                     g->oldCos.p, g->lastF.p, g->dispCalc.p, g->partial.p);
+  if (f1)
+    GVH_CHECK(hipEventRecord(f1, st));
+  hipLaunchKernelGGL(k_round_means_nd<D>, dim3(1), dim3(256), 0, st, g->partial.p,
+                     (int)rblocks, active, g->roundN.p);
+  if (D == 3) {
+    hipLaunchKernelGGL(k_rotation3_sums, dim3(blocks), dim3(kB), 0, st, g->order.p,
+                       active, g->posN.p, g->dispN.p, g->roundN.p, g->partial.p);
+    hipLaunchKernelGGL(k_rotation3_solve, dim3(1), dim3(256), 0, st, g->partial.p,
+                       (int)blocks, active, g->roundN.p);
+  } else {
+    for (int plane = 0; plane < 6; plane++) {
+      hipLaunchKernelGGL(k_plane_sums, dim3(blocks), dim3(kB), 0, st, g->order.p,
+                         active, g->posN.p, g->dispN.p, g->roundN.p, plane, g->partial.p);
+      hipLaunchKernelGGL(k_plane_omega, dim3(1), dim3(256), 0, st, g->partial.p,
+                         (int)blocks, active, g->roundN.p, plane);
+    }
+  }
+  hipLaunchKernelGGL(k_round_apply_nd<D>, dim3(blocks), dim3(kB), 0, st, g->order.p,
+                     active, g->posN.p, g->dispN.p, g->heat.p, g->lastF.p, g->roundN.p,
+                     g->partial.p);
+  hipLaunchKernelGGL(k_round_stats_nd, dim3(1), dim3(256), 0, st, g->partial.p,
+                     (int)blocks, g->roundN.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
 int enqueue_round(gvizhip_grip *g, int64_t active, int64_t layer, unsigned *blocksOut,
                   hipEvent_t f0, hipEvent_t f1) {
   hipStream_t st = g->stream;
   if (build_order(g, active))
     return -1;
+  if (g->dim == 3 || g->dim == 4) {
+    if (blocksOut)
+      *blocksOut = std::min<unsigned>(grid_for(active, kB), 1024);
+    return g->dim == 3 ? enqueue_round_nd<3>(g, active, layer, f0, f1)
+                       : enqueue_round_nd<4>(g, active, layer, f0, f1);
+  }
   const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
   if (f0)
     GVH_CHECK(hipEventRecord(f0, st));
@@ -1777,13 +2467,24 @@ int gvizhip_grip_refine_round(gvizhip_grip *g, int64_t active, int64_t layer,
   }
   hipStream_t st = g->stream;
   GripRound r{};
+  GripRoundN rn{};
   if (active > 0) {
     if (enqueue_round(g, active, layer, nullptr, nullptr, nullptr))
       return -1;
-    GVH_CHECK(hipMemcpyAsync(&r, g->round.p, sizeof(GripRound), hipMemcpyDeviceToHost,
-                             st));
+    if (g->dim == 2)
+      GVH_CHECK(hipMemcpyAsync(&r, g->round.p, sizeof(GripRound), hipMemcpyDeviceToHost,
+                               st));
+    else
+      GVH_CHECK(hipMemcpyAsync(&rn, g->roundN.p, sizeof(GripRoundN),
+                               hipMemcpyDeviceToHost, st));
   }
   GVH_CHECK(hipStreamSynchronize(st));
+  if (g->dim != 2) {
+    r.maxDisp = rn.maxDisp;
+    r.sumDisp = rn.sumDisp;
+    r.sumHeat = rn.sumHeat;
+    r.sumForce = rn.sumForce;
+  }
   if (stats) {
     stats->maxDisp = r.maxDisp;
     stats->meanDisp = active ? r.sumDisp / (double)active : 0.0;

@@ -9,7 +9,9 @@
  * which keeps positions, per-vertex state and kNN lists in HBM. Positions
  * are uploaded before and read back after each device phase that uses
  * them, unless residentPositions is set (then gvizGRIPEmbedderSyncPositions
- * fetches them). Dimension 2 only.
+ * fetches them). Dimensions 2, 3 and 4 (the ones whose rotation removal
+ * the reference defines, gvizGRIPEmbedder.c:1192-1261); Init returns -2
+ * for another.
  */
 #include "gviz/gvizGRIPEmbedder.h"
 
@@ -165,7 +167,7 @@ int gvizGRIPEmbedderInit(gvizGRIPState *s, gvizSubgraph subgraph,
   size_t activeCount = gvizSubgraphVertexCount(&subgraph);
   if (activeCount < dimension + 1)
     return -1;
-  if (dimension != 2)
+  if (dimension < 2 || dimension > 4)
     return -2;
   uint8_t statsRegisterMode = s->statsRegisterMode;
   size_t knnCapacityRequested = s->knnCapacity;
@@ -281,7 +283,7 @@ int gvizGRIPEmbedderSyncPositions(gvizGRIPState *s) {
  * positions, the per-vertex decorators and the kNN lists. */
 static void sync_draw_mask(gvizGRIPState *s);
 
-static const char kGripStateMagic[8] = {'G', 'V', 'Z', 'G', 'R', 'S', '0', '1'};
+static const char kGripStateMagic[8] = {'G', 'V', 'Z', 'G', 'R', 'S', '0', '2'};
 
 int gvizGRIPEmbedderSaveState(gvizGRIPState *s, const char *filename) {
   if (!s || !s->device || !filename || s->layerCount == 0)
@@ -289,28 +291,31 @@ int gvizGRIPEmbedderSaveState(gvizGRIPState *s, const char *filename) {
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   const int64_t n = (int64_t)gvizGraphSize(e->subgraph.g);
   const int64_t kcap = (int64_t)s->knnCapacity, active = (int64_t)border_at(s, 0);
+  const int64_t dim = (int64_t)e->embedding.dim;
   if (s->residentPositions && gvizGRIPEmbedderSyncPositions(s) != 0)
     return -1;
-  gvizhip_grip_vertex *dec = malloc(sizeof(gvizhip_grip_vertex) * (size_t)(n ? n : 1));
+  gvizhip_grip_vertex_nd *dec =
+      malloc(sizeof(gvizhip_grip_vertex_nd) * (size_t)(n ? n : 1));
   int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
   int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
-  int ok = dec && kv && kd && gvizhip_grip_download(s->device, NULL, dec, kv, kd) == 0;
+  int ok = dec && kv && kd && gvizhip_grip_download_nd(s->device, NULL, dec, kv, kd) == 0;
   FILE *f = ok ? fopen(filename, "wb") : NULL;
   ok = ok && f;
   if (ok) {
-    const int64_t head[6] = {n, kcap, (int64_t)s->layerCount, (int64_t)s->currLayer,
-                             (int64_t)s->currRound, active};
+    const int64_t head[7] = {n, kcap, (int64_t)s->layerCount, (int64_t)s->currLayer,
+                             (int64_t)s->currRound, active, dim};
     const double st[3] = {s->lastRoundStats.maxDisplacement,
                           s->lastRoundStats.meanDisplacement, s->lastRoundStats.meanForce};
-    ok = fwrite(kGripStateMagic, 1, 8, f) == 8 && fwrite(head, 8, 6, f) == 6 &&
+    ok = fwrite(kGripStateMagic, 1, 8, f) == 8 && fwrite(head, 8, 7, f) == 7 &&
          fwrite(st, 8, 3, f) == 3;
     for (int64_t i = 0; ok && i < active; i++) {
       const int64_t v = (int64_t)s->misFiltration[i];
       ok = fwrite(&v, 8, 1, f) == 1;
     }
     ok = ok && fwrite(e->visibleVertices, 1, (size_t)n, f) == (size_t)n &&
-         fwrite(host_positions(s), sizeof(double), (size_t)(2 * n), f) == (size_t)(2 * n) &&
-         fwrite(dec, sizeof(gvizhip_grip_vertex), (size_t)n, f) == (size_t)n &&
+         fwrite(host_positions(s), sizeof(double), (size_t)(dim * n), f) ==
+             (size_t)(dim * n) &&
+         fwrite(dec, sizeof(gvizhip_grip_vertex_nd), (size_t)n, f) == (size_t)n &&
          fwrite(kv, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap) &&
          fwrite(kd, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap);
   }
@@ -330,13 +335,15 @@ int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
   FILE *f = fopen(filename, "rb");
   if (!f)
     return -1;
+  const int64_t dim = (int64_t)e->embedding.dim;
   char magic[8];
-  int64_t head[6];
+  int64_t head[7];
   double st[3];
   int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kGripStateMagic, 8) == 0 &&
-           fread(head, 8, 6, f) == 6 && fread(st, 8, 3, f) == 3 && head[0] == n &&
+           fread(head, 8, 7, f) == 7 && fread(st, 8, 3, f) == 3 && head[0] == n &&
            head[1] == (int64_t)s->knnCapacity && head[2] == (int64_t)s->layerCount &&
-           head[3] >= 0 && head[3] < head[2] && head[5] == (int64_t)border_at(s, 0);
+           head[3] >= 0 && head[3] < head[2] && head[5] == (int64_t)border_at(s, 0) &&
+           head[6] == dim;
   /* the same graph and configuration give the same filtration */
   for (int64_t i = 0; ok && i < head[5]; i++) {
     int64_t v = -1;
@@ -344,20 +351,21 @@ int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
   }
   const int64_t kcap = ok ? head[1] : 0;
   uint8_t *vis = malloc((size_t)(n ? n : 1));
-  double *pos = malloc(sizeof(double) * (size_t)(2 * n ? 2 * n : 1));
-  gvizhip_grip_vertex *dec = malloc(sizeof(gvizhip_grip_vertex) * (size_t)(n ? n : 1));
+  double *pos = malloc(sizeof(double) * (size_t)(dim * n ? dim * n : 1));
+  gvizhip_grip_vertex_nd *dec =
+      malloc(sizeof(gvizhip_grip_vertex_nd) * (size_t)(n ? n : 1));
   int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
   int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
   ok = ok && vis && pos && dec && kv && kd &&
        fread(vis, 1, (size_t)n, f) == (size_t)n &&
-       fread(pos, sizeof(double), (size_t)(2 * n), f) == (size_t)(2 * n) &&
-       fread(dec, sizeof(gvizhip_grip_vertex), (size_t)n, f) == (size_t)n &&
+       fread(pos, sizeof(double), (size_t)(dim * n), f) == (size_t)(dim * n) &&
+       fread(dec, sizeof(gvizhip_grip_vertex_nd), (size_t)n, f) == (size_t)n &&
        fread(kv, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap) &&
        fread(kd, 8, (size_t)(n * kcap), f) == (size_t)(n * kcap);
   fclose(f);
   if (ok) {
-    memcpy(host_positions(s), pos, sizeof(double) * (size_t)(2 * n));
-    ok = gvizhip_grip_upload(s->device, pos, dec, kv, kd) == 0;
+    memcpy(host_positions(s), pos, sizeof(double) * (size_t)(dim * n));
+    ok = gvizhip_grip_upload_nd(s->device, pos, dec, kv, kd) == 0;
   }
   if (ok) {
     memcpy(e->visibleVertices, vis, (size_t)n);
@@ -415,6 +423,11 @@ static int ensure_device(gvizGRIPState *s) {
     s->device = NULL;
     return -1;
   }
+  if (gvizhip_grip_set_dimension(s->device, (int64_t)e->embedding.dim) != 0) {
+    gvizhip_grip_destroy(s->device);
+    s->device = NULL;
+    return -1;
+  }
   return 0;
 }
 
@@ -449,7 +462,7 @@ void gvizGRIPEmbedderBegin(gvizGRIPState *s) {
   s->currRound = 0;
 
   size_t dim = e->embedding.dim;
-  double simplex[2 * 3];
+  double simplex[4 * 5];
   regular_simplex(dim, GRIP_EDGE_LENGTH * 1000, simplex);
   for (size_t j = 0; j < dim + 1; j++) {
     gvizEmbeddedGraphDrawMaskShowVertex(e, s->misFiltration[j]);

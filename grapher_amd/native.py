@@ -62,6 +62,10 @@ GRIP_VERTEX_DTYPE = np.dtype([
     ("dispX", "f8"), ("dispY", "f8"), ("oldDispX", "f8"), ("oldDispY", "f8"),
     ("heat", "f8"), ("oldCos", "f8"), ("lastForceMag", "f8"),
     ("knnCount", "i8"), ("dispCalculated", "i8")])
+# gvizhip_grip_vertex_nd: disp / oldDisp of up to four coordinates
+GRIP_VERTEX_ND_DTYPE = np.dtype([
+    ("disp", "f8", 4), ("oldDisp", "f8", 4), ("heat", "f8"), ("oldCos", "f8"),
+    ("lastForceMag", "f8"), ("knnCount", "i8"), ("dispCalculated", "i8")])
 
 
 class GripRoundStats(C.Structure):
@@ -123,6 +127,9 @@ def lib():
         "gvizhip_grip_knn_batch": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
         "gvizhip_grip_upload": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "gvizhip_grip_download": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_grip_set_dimension": (C.c_int, [_vp, C.c_int64]),
+        "gvizhip_grip_upload_nd": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_grip_download_nd": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "gvizhip_grip_clear_decorators": (C.c_int, [_vp, _i64]),
         "gvizhip_grip_update_knn": (C.c_int, [_vp, _i64, _vp, _i64, C.c_int32]),
         "gvizhip_grip_place": (C.c_int, [_vp, _i64, _i64, _vp, _i64, C.c_int32]),
@@ -179,7 +186,8 @@ DECLARED = [
     "gvizhip_tutte_spring_steps", "gvizhip_tutte_set_velocity", "gvizhip_tutte_get_velocity",
     "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
     "gvizhip_grip_set_filtration", "gvizhip_grip_knn", "gvizhip_grip_knn_batch",
-    "gvizhip_grip_upload", "gvizhip_grip_download",
+    "gvizhip_grip_upload", "gvizhip_grip_download", "gvizhip_grip_set_dimension",
+    "gvizhip_grip_upload_nd", "gvizhip_grip_download_nd",
     "gvizhip_grip_clear_decorators", "gvizhip_grip_update_knn",
     "gvizhip_grip_place", "gvizhip_grip_refine_round", "gvizhip_grip_profile_rounds",
     "gvizhip_grip_destroy",
@@ -432,16 +440,19 @@ class TutteEngine:
 class GripEngine:
     """Device GRIP state: gvizhip_grip_*."""
 
-    def __init__(self, offsets, adj, active=None, kcap=10):
+    def __init__(self, offsets, adj, active=None, kcap=10, dim=2):
         self._o, self._a = _csr(offsets, adj)
         self.n = self._o.shape[0] - 1
         self.kcap = kcap
+        self.dim = dim
         act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
         h = _vp()
         if lib().gvizhip_grip_create(C.byref(h), self.n, _p(self._o), _p(self._a),
                                      _p(act), kcap) != 0:
             _err("gvizhip_grip_create")
         self.h = h
+        if dim != 2 and lib().gvizhip_grip_set_dimension(h, dim) != 0:
+            _err("gvizhip_grip_set_dimension")
 
     def mis_filtration(self, dim=2):
         filt = np.zeros(self.n, dtype=np.int64)
@@ -479,18 +490,30 @@ class GripEngine:
         return ov[: len(t) * k].reshape(len(t), k), od[: len(t) * k].reshape(len(t), k), cnt[: len(t)]
 
     def upload(self, positions=None, dec=None, knn_vertex=None, knn_dist=None):
+        """Positions (n, dim); dec in GRIP_VERTEX_DTYPE for dim 2, in
+        GRIP_VERTEX_ND_DTYPE for dim 3 and 4."""
         pos = None if positions is None else np.ascontiguousarray(positions, dtype=np.float64)
-        d = None if dec is None else np.ascontiguousarray(dec, dtype=GRIP_VERTEX_DTYPE)
         kv = None if knn_vertex is None else np.ascontiguousarray(knn_vertex, dtype=np.int64)
         kd = None if knn_dist is None else np.ascontiguousarray(knn_dist, dtype=np.int64)
+        if self.dim != 2:
+            d = None if dec is None else np.ascontiguousarray(dec, dtype=GRIP_VERTEX_ND_DTYPE)
+            if lib().gvizhip_grip_upload_nd(self.h, _p(pos), _p(d), _p(kv), _p(kd)) != 0:
+                _err("gvizhip_grip_upload_nd")
+            return
+        d = None if dec is None else np.ascontiguousarray(dec, dtype=GRIP_VERTEX_DTYPE)
         if lib().gvizhip_grip_upload(self.h, _p(pos), _p(d), _p(kv), _p(kd)) != 0:
             _err("gvizhip_grip_upload")
 
     def download(self):
-        pos = np.zeros((self.n, 2))
-        dec = np.zeros(self.n, dtype=GRIP_VERTEX_DTYPE)
+        pos = np.zeros((self.n, self.dim))
         kv = np.zeros((self.n, self.kcap), dtype=np.int64)
         kd = np.zeros((self.n, self.kcap), dtype=np.int64)
+        if self.dim != 2:
+            dec = np.zeros(self.n, dtype=GRIP_VERTEX_ND_DTYPE)
+            if lib().gvizhip_grip_download_nd(self.h, _p(pos), _p(dec), _p(kv), _p(kd)) != 0:
+                _err("gvizhip_grip_download_nd")
+            return pos, dec, kv, kd
+        dec = np.zeros(self.n, dtype=GRIP_VERTEX_DTYPE)
         if lib().gvizhip_grip_download(self.h, _p(pos), _p(dec), _p(kv), _p(kd)) != 0:
             _err("gvizhip_grip_download")
         return pos, dec, kv, kd

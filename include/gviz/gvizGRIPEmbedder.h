@@ -2,7 +2,7 @@
  * gvizGRIPEmbedder.h -- the reference's GRIP multilevel embedder API
  * (include/embedders/gvizGRIPEmbedder.h, src/embedders/gvizGRIPInternal.h)
  * with the MIS filtration, kNN searches, placement and refinement rounds on
- * the GPU. Dimension 2 only.
+ * the GPU. Dimensions 2, 3 and 4; Init returns -2 for another.
  */
 #ifndef GVIZ_HOST_GRIP_EMBEDDER_H
 #define GVIZ_HOST_GRIP_EMBEDDER_H

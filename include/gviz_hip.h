@@ -221,7 +221,7 @@ typedef struct gvizhip_grip_vertex {
 typedef struct gvizhip_grip_round_stats {
   double maxDisp, meanDisp, meanForce, meanHeat;
 } gvizhip_grip_round_stats;
-/* Host <-> device copies of positions (2n), vertex state (n) and kNN lists
+/* Host <-> device copies of positions (n*dim, 2n by default), vertex state (n) and kNN lists
  * (n*kcap vertex and dist entries); any pointer may be NULL. */
 int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
                         const gvizhip_grip_vertex *dec,
@@ -229,6 +229,25 @@ int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
 int gvizhip_grip_download(gvizhip_grip *g, double *positions,
                           gvizhip_grip_vertex *dec, int64_t *knnVertex,
                           int64_t *knnDist);
+/* Layouts of 3 or 4 coordinates (gvizGRIPEmbedderInit's dimension,
+ * src/embedders/gvizGRIPEmbedder.c:1192-1261 for the rotation removal of
+ * each): call once after create, before any upload. Positions passed to
+ * upload/download are then n*dim doubles; the per-vertex state goes
+ * through the _nd pair, whose disp/oldDisp hold dim coordinates (the rest
+ * zero). The _nd pair also serves dim 2. 0 or -1. */
+int gvizhip_grip_set_dimension(gvizhip_grip *g, int64_t dim);
+typedef struct gvizhip_grip_vertex_nd {
+  double disp[4], oldDisp[4];
+  double heat, oldCos, lastForceMag;
+  int64_t knnCount;
+  int64_t dispCalculated;
+} gvizhip_grip_vertex_nd;
+int gvizhip_grip_upload_nd(gvizhip_grip *g, const double *positions,
+                           const gvizhip_grip_vertex_nd *dec,
+                           const int64_t *knnVertex, const int64_t *knnDist);
+int gvizhip_grip_download_nd(gvizhip_grip *g, double *positions,
+                             gvizhip_grip_vertex_nd *dec, int64_t *knnVertex,
+                             int64_t *knnDist);
 /* clearDecorators (:936-952) for filtration[0..active). */
 int gvizhip_grip_clear_decorators(gvizhip_grip *g, int64_t active);
 /* updateKNNs (:954-1006): kNN lists of filtration[0..active) against the

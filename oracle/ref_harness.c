@@ -332,16 +332,16 @@ static void serial_grip(gvizGRIPState *st) {
   st->pool = NULL;
 }
 
-static void dump_grip_state(gvizGRIPState *st, int64_t n) {
+static void dump_grip_state(gvizGRIPState *st, int64_t n, int64_t dim) {
   const double *p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)st);
-  for (int64_t i = 0; i < 2 * n; i++)
+  for (int64_t i = 0; i < dim * n; i++)
     wr_d(p[i]);
   for (int64_t v = 0; v < n; v++) {
     gvizGRIPDecorators *d = &st->dec[v];
-    wr_d(d->disp[0]);
-    wr_d(d->disp[1]);
-    wr_d(d->oldDisp[0]);
-    wr_d(d->oldDisp[1]);
+    for (int64_t k = 0; k < dim; k++)
+      wr_d(d->disp[k]);
+    for (int64_t k = 0; k < dim; k++)
+      wr_d(d->oldDisp[k]);
     wr_d(d->heat);
     wr_d(d->oldCos);
     wr_d(d->lastForceMag);
@@ -354,11 +354,11 @@ static int mode_grip(void) {
   gvizGraph g;
   int64_t n;
   read_graph(&g, &n);
-  int64_t kcap = rd_i(), kmax = rd_i(), roundsPerLayer = rd_i();
+  int64_t kcap = rd_i(), kmax = rd_i(), roundsPerLayer = rd_i(), dim = rd_i();
   gvizGRIPState st;
   memset(&st, 0, sizeof(st));
   gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
-  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, (size_t)dim) != 0)
     return 1;
   serial_grip(&st);
   gvizGRIPEmbedderConfigureK(&st, (size_t)kmax, (size_t)kmax,
@@ -373,7 +373,7 @@ static int mode_grip(void) {
   for (;;) {
     for (int64_t r = 0; r < roundsPerLayer; r++) {
       wr_i((int64_t)st.currLayer);
-      dump_grip_state(&st, n);
+      dump_grip_state(&st, n, dim);
       /* kNN lists of every vertex, kcap slots */
       for (int64_t v = 0; v < n; v++) {
         gvizGRIPDecorators *d = &st.dec[v];
@@ -385,7 +385,7 @@ static int mode_grip(void) {
         }
       }
       gvizGRIPEmbedderRefineRound(&st);
-      dump_grip_state(&st, n);
+      dump_grip_state(&st, n, dim);
       gvizGRIPRoundStats rs = gvizGRIPEmbedderLastRoundStats(&st);
       wr_d(rs.maxDisplacement);
       wr_d(rs.meanDisplacement);
@@ -489,11 +489,11 @@ static int mode_api_grip(void) {
   gvizGraph g;
   int64_t n;
   read_graph(&g, &n);
-  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i();
+  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i(), dim = rd_i();
   gvizGRIPState st;
   memset(&st, 0, sizeof(st));
   gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
-  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, (size_t)dim) != 0)
     return 1;
   serial_grip(&st);
   gvizGRIPEmbedderConfigureK(&st, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
@@ -503,7 +503,7 @@ static int mode_api_grip(void) {
     wr_i((int64_t)*(size_t *)gvizArrayAtIndex(&st.misBorder, i));
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
   const double *p = gvizEmbeddedGraphPositions(e);
-  for (int64_t i = 0; i < 2 * n; i++)
+  for (int64_t i = 0; i < dim * n; i++)
     wr_d(p[i]);
   static const char *const names[] = {"grip.heat", "grip.meanDisp",
                                       "grip.maxDisp", "grip.meanForce"};

@@ -126,11 +126,11 @@ def knn_cases():
          batch_list=np.stack([b[2] for b in batch]))
 
 
-def grip_case(name, n, e, kcap, rounds, kmax=None):
+def grip_case(name, n, e, kcap, rounds, kmax=None, dim=2):
     kmax = kcap if kmax is None else kmax
-    borders, filt, rr = refio.grip(n, e, kcap, kmax, rounds)
+    borders, filt, rr = refio.grip(n, e, kcap, kmax, rounds, dim)
     out = dict(n=n, edges=e, kcap=kcap, kmax=kmax, borders=borders,
-               filtration=filt, rounds=len(rr))
+               filtration=filt, rounds=len(rr), dim=dim)
     for i, r in enumerate(rr):
         out[f"r{i}_layer"] = r["layer"]
         out[f"r{i}_pos0"] = r["pos0"]
@@ -156,6 +156,23 @@ def grip_chain_cases():
     chain through the device state and every NextStage is included."""
     n, e = G.grid(16, 16)
     grip_case("grip_grid16_k4", n, e, 16, 4, kmax=4)
+
+
+def grip_nd_cases():
+    """Refinement rounds, placement, decorator clears and kNN updates in
+    three and four dimensions (the 3-D rotation solve and the six 4-D
+    coordinate planes), and gvizGRIPEmbedderEmbed end to end."""
+    n, e, _ = G.random_geometric(300, 8.0, 2)
+    grip_case("grip_rgg300_d3", n, e, 10, 2, dim=3)
+    grip_case("grip_rgg300_d4", n, e, 10, 2, dim=4)
+    n, e = G.grid(16, 16)
+    grip_case("grip_grid16_k4_d3", n, e, 16, 4, kmax=4, dim=3)
+    grip_case("grip_grid16_k4_d4", n, e, 16, 4, kmax=4, dim=4)
+    n, e, _ = G.random_geometric(400, 8.0, 5)
+    for dim in (3, 4):
+        r = refio.api_grip(n, e, 12, 12, 12, 0, dim=dim)
+        save(f"api_grip_rgg400_constant_d{dim}", n=n, edges=e, kcap=12, placement_k=12,
+             refinement_k=12, policy=0, dim=dim, **r)
 
 
 def layout_quality(n, e, hidden, pos, edge_length):
@@ -290,6 +307,7 @@ def spring_tutte_cases():
 
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
              knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
+             grip_nd=grip_nd_cases,
              api=api_cases, edges=edges_cases, spring_tutte=spring_tutte_cases)
 
 if __name__ == "__main__":

@@ -180,22 +180,23 @@ def knn(n, edges, visible, k, sources):
     return per, rc, batch
 
 
-def grip(n, edges, kcap, kmax, rounds_per_layer):
+def grip(n, edges, kcap, kmax, rounds_per_layer, dim=2):
     w = _W()
     w.graph(n, edges)
-    w.i(kcap, kmax, rounds_per_layer)
+    w.i(kcap, kmax, rounds_per_layer, dim)
     r = _R(_run("grip", w.bytes()))
     layers = r.i()
     borders = r.ia(layers)
     filtration = r.ia(int(borders[0]))
 
     def state():
-        pos = r.da(2 * n).reshape(n, 2)
-        dec = np.frombuffer(r.buf, dtype=np.dtype([
-            ("disp", "<f8", 2), ("oldDisp", "<f8", 2), ("heat", "<f8"),
+        pos = r.da(dim * n).reshape(n, dim)
+        dt = np.dtype([
+            ("disp", "<f8", dim), ("oldDisp", "<f8", dim), ("heat", "<f8"),
             ("oldCos", "<f8"), ("lastForceMag", "<f8"), ("knnCount", "<i8"),
-            ("dispCalculated", "<i8")]), count=n, offset=r.off).copy()
-        r.off += n * 72
+            ("dispCalculated", "<i8")])
+        dec = np.frombuffer(r.buf, dtype=dt, count=n, offset=r.off).copy()
+        r.off += n * dt.itemsize
         return pos, dec
 
     rounds = []
@@ -243,16 +244,17 @@ def api_force(n, edges, hidden, *, model, barnes_hut, prevent_overlap, npc, seed
     return out
 
 
-def api_grip(n, edges, kcap, placement_k, refinement_k, policy, exe=None):
-    """gvizGRIPEmbedderEmbed with ConfigureKnnCapacity / ConfigureK."""
+def api_grip(n, edges, kcap, placement_k, refinement_k, policy, exe=None, dim=2):
+    """gvizGRIPEmbedderEmbed with ConfigureKnnCapacity / ConfigureK in `dim`
+    dimensions."""
     w = _W()
     w.graph(n, edges)
-    w.i(kcap, placement_k, refinement_k, policy)
+    w.i(kcap, placement_k, refinement_k, policy, dim)
     r = _R(_run("api_grip", w.bytes(), exe))
     out = dict(embed_rc=r.i())
     layers = r.i()
     out["borders"] = r.ia(layers)
-    out["pos"] = r.da(2 * n).reshape(n, 2)
+    out["pos"] = r.da(dim * n).reshape(n, dim)
     for name in ("heat", "meanDisp", "maxDisp", "meanForce"):
         c = r.i()
         out["series_" + name] = r.da(c)

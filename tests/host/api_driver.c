@@ -166,11 +166,11 @@ static int mode_api_grip(void) {
   gvizGraph g;
   int64_t n;
   read_graph(&g, &n);
-  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i();
+  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i(), dim = rd_i();
   gvizGRIPState st;
   memset(&st, 0, sizeof(st));
   gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
-  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, 2) != 0)
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, (size_t)dim) != 0)
     return 1;
   gvizGRIPEmbedderConfigureK(&st, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
   int rc = gvizGRIPEmbedderEmbed(&st);
@@ -182,7 +182,7 @@ static int mode_api_grip(void) {
     wr_i((int64_t)st.misBorder[i]);
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
   const double *p = gvizEmbeddedGraphPositions(e);
-  for (int64_t i = 0; i < 2 * n; i++)
+  for (int64_t i = 0; i < dim * n; i++)
     wr_d(p[i]);
   static const char *const names[] = {"grip.heat", "grip.meanDisp",
                                       "grip.maxDisp", "grip.meanForce"};

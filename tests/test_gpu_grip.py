@@ -227,3 +227,103 @@ def test_grip_rounds_chain_through_device_state(gpu):
                        f[f"r{i}_dec1"], None, ids)
         np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
                                    f[f"r{i}_stats"], rtol=1e-11)
+
+
+def _dec_nd_from(d0, n, dim):
+    from grapher_amd import native
+    dec = np.zeros(n, dtype=native.GRIP_VERTEX_ND_DTYPE)
+    dec["disp"][:, :dim] = d0["disp"]
+    dec["oldDisp"][:, :dim] = d0["oldDisp"]
+    for k in ("heat", "oldCos", "lastForceMag", "knnCount", "dispCalculated"):
+        dec[k] = d0[k]
+    return dec
+
+
+def _compare_state_nd(i, when, dim, pos, dec, kv, kd, ref_pos, ref_dec, ref_knn, ids):
+    scale = np.abs(ref_pos).max()
+    d = np.abs(pos - ref_pos).max()
+    assert d <= 1e-12 * scale, f"round {i} {when}: positions differ by {d:.3g}"
+    for k in ("disp", "oldDisp"):
+        got, want = dec[k][ids, :dim], ref_dec[k][ids]
+        tol = 1e-11 * max(np.abs(want).max(), 1e-300)
+        assert np.abs(got - want).max() <= tol, (i, when, k)
+        assert not np.any(dec[k][ids, dim:]), (i, when, k, "coordinates beyond dim")
+    for k in ("heat", "oldCos", "lastForceMag"):
+        got, want = dec[k][ids], ref_dec[k][ids]
+        tol = 1e-11 * max(np.abs(want).max(), 1e-300)
+        assert np.abs(got - want).max() <= tol, (i, when, k)
+    for k in ("knnCount", "dispCalculated"):
+        assert np.array_equal(dec[k][ids], ref_dec[k][ids]), (i, when, k)
+    if ref_knn is not None:
+        for v in ids:
+            c = int(ref_dec["knnCount"][v])
+            assert np.array_equal(kv[v, :c], ref_knn[v, :c, 0]), (i, when, "knn", v)
+            assert np.array_equal(kd[v, :c], ref_knn[v, :c, 1]), (i, when, "dist", v)
+
+
+@pytest.mark.parametrize("name", ["grip_rgg300_d3", "grip_rgg300_d4"])
+def test_refine_rounds_match_reference_nd(gpu, name):
+    """Refinement rounds in three and four dimensions from the reference's
+    state: the 3-D round removes one rotation solved from the inertia tensor
+    (solve3x3), the 4-D round the six coordinate-plane rotations in turn."""
+    f = golden(name)
+    n, dim = int(f["n"]), int(f["dim"])
+    kcap = int(f["kcap"])
+    off, adj = G.csr_from_edges(n, f["edges"])
+    eng = gpu.GripEngine(off, adj, kcap=kcap, dim=dim)
+    eng.set_filtration(f["filtration"])
+    print(f"grip {name}: {int(f['rounds'])} rounds in {dim}-D", flush=True)
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(f["borders"][layer])
+        knn = f[f"r{i}_knn"]
+        eng.upload(f[f"r{i}_pos0"], _dec_nd_from(f[f"r{i}_dec0"], n, dim), knn[:, :, 0],
+                   knn[:, :, 1])
+        st = eng.refine_round(active, layer)
+        pos, dec, _, _ = eng.download()
+        _compare_state_nd(i, "after the round", dim, pos, dec, None, None,
+                          f[f"r{i}_pos1"], f[f"r{i}_dec1"], None, f["filtration"][:active])
+        np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
+                                   f[f"r{i}_stats"], rtol=1e-11)
+
+
+@pytest.mark.parametrize("name", ["grip_grid16_k4_d3", "grip_grid16_k4_d4"])
+def test_grip_rounds_chain_through_device_state_nd(gpu, name):
+    """The whole 3-D / 4-D schedule on the device state, compared with the
+    reference after every NextStage (placement, decorator clear, kNN
+    update) and every round."""
+    f = golden(name)
+    n, dim = int(f["n"]), int(f["dim"])
+    kcap, k = int(f["kcap"]), int(f["kmax"])
+    off, adj = G.csr_from_edges(n, f["edges"])
+    ne = f["edges"].shape[0]
+    eng = gpu.GripEngine(off, adj, kcap=kcap, dim=dim)
+    filt, borders = f["filtration"], f["borders"]
+    eng.set_filtration(filt)
+    knn0 = f["r0_knn"]
+    eng.upload(f["r0_pos0"], _dec_nd_from(f["r0_dec0"], n, dim), knn0[:, :, 0],
+               knn0[:, :, 1])
+    prev = int(f["r0_layer"])
+    print(f"grip chain {name}: {int(f['rounds'])} rounds in {dim}-D", flush=True)
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(borders[layer])
+        ids = filt[:active]
+        if layer != prev:
+            begin, end = int(borders[layer + 1]), int(borders[layer])
+            vis = np.zeros(n, np.uint8)
+            vis[filt[:begin]] = 1
+            eng.place(begin, end, vis, k, _prefer_batch(n, ne, begin, end - begin))
+            vis[filt[:end]] = 1
+            eng.clear_decorators(end)
+            eng.update_knn(end, vis, k, _prefer_batch(n, ne, end, end))
+            pos, dec, kv, kd = eng.download()
+            _compare_state_nd(i, "after NextStage", dim, pos, dec, kv, kd, f[f"r{i}_pos0"],
+                              f[f"r{i}_dec0"], f[f"r{i}_knn"], ids)
+            prev = layer
+        st = eng.refine_round(active, layer)
+        pos, dec, kv, kd = eng.download()
+        _compare_state_nd(i, "after the round", dim, pos, dec, kv, kd, f[f"r{i}_pos1"],
+                          f[f"r{i}_dec1"], None, ids)
+        np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
+                                   f[f"r{i}_stats"], rtol=1e-11)

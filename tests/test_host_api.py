@@ -279,3 +279,47 @@ def test_grip_state_checkpoint_resumes_exactly(gpu, tmp_path):
     np.testing.assert_array_equal(va, vb)
     np.testing.assert_array_equal(pa, pb)
     assert np.all(np.isfinite(pa[va == 1]))
+
+
+def _stress_nd(n, edges, pos, edge_length=10.0):
+    """Kamada-Kawai stress of a layout in any dimension: the sum over pairs
+    in one component of (|p_u - p_v| - L d_uv)^2 / (L d_uv)^2 with d the
+    BFS distance."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import shortest_path
+    a = sp.coo_matrix((np.ones(len(edges)), (edges[:, 0], edges[:, 1])), shape=(n, n))
+    dist = shortest_path(a, directed=False, unweighted=True)
+    iu = np.triu_indices(n, 1)
+    d = dist[iu]
+    keep = np.isfinite(d)  # pairs in one component
+    ideal = edge_length * d[keep]
+    real = np.linalg.norm(pos[iu[0][keep]] - pos[iu[1][keep]], axis=1)
+    return float(np.sum((real - ideal) ** 2 / ideal ** 2))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["api_grip_rgg400_constant_d3", "api_grip_rgg400_constant_d4"])
+def test_grip_embed_matches_reference_nd(gpu, name):
+    """gvizGRIPEmbedderEmbed in three and four dimensions end to end, by the
+    bars of the 2-D run: exact filtration, draw mask and schedule, the
+    per-round series to 1e-6 over the first 60 rounds, and the final
+    layout's stress within 5 %."""
+    f = golden(name)
+    n, dim = int(f["n"]), int(f["dim"])
+    print(f"host api grip {name}", flush=True)
+    r = refio.api_grip(n, f["edges"], int(f["kcap"]), int(f["placement_k"]),
+                       int(f["refinement_k"]), int(f["policy"]), exe=DRIVER, dim=dim)
+    assert r["embed_rc"] == 0
+    assert r["pos"].shape == (n, dim)
+    assert np.array_equal(r["borders"], f["borders"])
+    assert np.array_equal(r["visible"], f["visible"])
+    ref_rounds = len(f["series_maxDisp"])
+    assert abs(len(r["series_maxDisp"]) - ref_rounds) <= max(2, 0.05 * ref_rounds)
+    head = min(60, ref_rounds, len(r["series_maxDisp"]))
+    for s in ("heat", "meanDisp", "maxDisp", "meanForce"):
+        np.testing.assert_allclose(r["series_" + s][:head], f["series_" + s][:head],
+                                   rtol=1e-6, atol=1e-12, err_msg=s)
+    s_ref = _stress_nd(n, f["edges"], f["pos"])
+    s_gpu = _stress_nd(n, f["edges"], r["pos"])
+    print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g}")
+    assert abs(s_gpu - s_ref) <= 0.05 * abs(s_ref)
