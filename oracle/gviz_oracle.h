@@ -219,6 +219,21 @@ void oc_grip_refine_round(const oc_csr *g, const int64_t *filtration,
                           int64_t active, int64_t layer, const oc_found *knn,
                           int64_t kcap, oc_grip_vertex *dec, double *pos,
                           oc_grip_round_stats *stats);
+
+/* The same round for dim = 2, 3 or 4 coordinates (pos: dim*n). dec holds
+ * disp / oldDisp of dim coordinates; the 3-D rotation removal solves the
+ * inertia system, the 4-D one removes the six coordinate-plane rotations
+ * in turn. */
+typedef struct oc_grip_vertex_nd {
+  double disp[4], oldDisp[4];
+  double heat, oldCos, lastForceMag;
+  int64_t knnCount;
+  int64_t dispCalculated;
+} oc_grip_vertex_nd;
+void oc_grip_refine_round_nd(const oc_csr *g, const int64_t *filtration,
+                             int64_t active, int64_t layer, const oc_found *knn,
+                             int64_t kcap, int64_t dim, oc_grip_vertex_nd *dec,
+                             double *pos, oc_grip_round_stats *stats);
 void oc_regular_simplex(int64_t n, double sideLength, double *out);
 
 #ifdef __cplusplus

@@ -490,6 +490,241 @@ void oc_grip_refine_round(const oc_csr *g, const int64_t *filtration,
   }
 }
 
+/* ------------------------------------ refinement in 2, 3 or 4 dimensions -- */
+
+/* gvizVecDot (include/core/gvizVec.h:42-57): products summed left to right */
+static double nd_dot(int64_t dim, const double *a, const double *b) {
+  double s = a[0] * b[0];
+  for (int64_t k = 1; k < dim; k++)
+    s += a[k] * b[k];
+  return s;
+}
+
+/* updateLocalTemp (gvizGRIPEmbedder.c:870-889) */
+static void update_local_temp_nd(int64_t dim, oc_grip_vertex_nd *d) {
+  double nrm = sqrt(nd_dot(dim, d->disp, d->disp));
+  double oldNrm = sqrt(nd_dot(dim, d->oldDisp, d->oldDisp));
+  if (fabs(nrm) < OC_NUMERIC_EPS || fabs(oldNrm) < OC_NUMERIC_EPS)
+    return;
+  double cs = nd_dot(dim, d->disp, d->oldDisp) / (nrm * oldNrm);
+  if (cs > 0 && d->oldCos > 0)
+    d->heat *= (1 + cs * GRIP_R * GRIP_S);
+  else
+    d->heat *= (1 + cs * GRIP_R);
+  d->oldCos = cs;
+}
+
+/* refinementPass1Range and calculateSpringForces (gvizGRIPEmbedder.c:
+ * 1071-1099, 892-934) with the force cases of gvizVecAccKKForce,
+ * gvizVecAccGRIPFRRepForce and gvizVecAccGRIPFRAttForce for n = 2..4
+ * (include/core/gvizVec.h:138-290). */
+static void refine_vertex_nd(const oc_csr *g, int64_t v, int64_t layer,
+                             const oc_found *knn, int64_t kcap, int64_t dim,
+                             oc_grip_vertex_nd *d, const double *pos) {
+  memcpy(d->oldDisp, d->disp, sizeof(double) * (size_t)dim);
+  double f[4] = {0.0, 0.0, 0.0, 0.0}, diff[4];
+  const double *vp = pos + dim * v;
+  const oc_found *list = knn + v * kcap;
+  const double eSq = GRIP_EDGE_LENGTH * GRIP_EDGE_LENGTH;
+  if (layer > 0) {
+    for (int64_t i = 0; i < d->knnCount; i++) {
+      double L = (double)list[i].dist * GRIP_EDGE_LENGTH;
+      if (L < 1e-9)
+        continue;
+      const double *up = pos + dim * list[i].v;
+      for (int64_t k = 0; k < dim; k++)
+        diff[k] = up[k] - vp[k];
+      double s = sqrt(nd_dot(dim, diff, diff)) / L - 1.0;
+      for (int64_t k = 0; k < dim; k++)
+        f[k] += s * diff[k];
+    }
+  } else {
+    for (int64_t e = g->offsets[v]; e < g->offsets[v + 1]; e++) {
+      const double *up = pos + dim * g->adj[e];
+      for (int64_t k = 0; k < dim; k++)
+        diff[k] = up[k] - vp[k];
+      double dsq = nd_dot(dim, diff, diff);
+      if (dsq < 1e-18)
+        continue;
+      double s = dsq / eSq;
+      for (int64_t k = 0; k < dim; k++)
+        f[k] += s * diff[k];
+    }
+    for (int64_t i = 0; i < d->knnCount; i++) {
+      const double *up = pos + dim * list[i].v;
+      for (int64_t k = 0; k < dim; k++)
+        diff[k] = vp[k] - up[k];
+      double dsq = nd_dot(dim, diff, diff);
+      if (dsq < 1e-18)
+        continue;
+      double s = GRIP_FR_SCALE * eSq / dsq;
+      for (int64_t k = 0; k < dim; k++)
+        f[k] += s * diff[k];
+    }
+  }
+  memcpy(d->disp, f, sizeof(double) * (size_t)dim);
+  if (!d->dispCalculated) {
+    d->dispCalculated = 1;
+    d->heat = GRIP_EDGE_LENGTH / 6.0;
+  } else {
+    update_local_temp_nd(dim, d);
+  }
+  d->heat = fmin(d->heat, GRIP_EDGE_LENGTH * 10);
+  d->heat = fmax(d->heat, GRIP_EDGE_LENGTH * 1e-4);
+  double nrm = sqrt(nd_dot(dim, d->disp, d->disp));
+  d->lastForceMag = nrm;
+  if (nrm > OC_NUMERIC_EPS) {
+    double s = d->heat / nrm;
+    for (int64_t k = 0; k < dim; k++)
+      d->disp[k] *= s;
+  }
+}
+
+/* solve3x3 (gvizGRIPEmbedder.c:1122-1152): elimination with the row of
+ * largest magnitude as pivot, kept through an index table. */
+static int solve3(double a[3][3], const double *b, double *x) {
+  int idx[3] = {0, 1, 2};
+  double rhs[3] = {b[0], b[1], b[2]};
+  for (int col = 0; col < 3; col++) {
+    int piv = col;
+    for (int row = col + 1; row < 3; row++)
+      if (fabs(a[idx[row]][col]) > fabs(a[idx[piv]][col]))
+        piv = row;
+    if (fabs(a[idx[piv]][col]) < OC_NUMERIC_EPS)
+      return -1;
+    int t = idx[col];
+    idx[col] = idx[piv];
+    idx[piv] = t;
+    for (int row = col + 1; row < 3; row++) {
+      double fct = a[idx[row]][col] / a[idx[col]][col];
+      for (int j = col; j < 3; j++)
+        a[idx[row]][j] -= fct * a[idx[col]][j];
+      rhs[idx[row]] -= fct * rhs[idx[col]];
+    }
+  }
+  for (int col = 2; col >= 0; col--) {
+    double s = rhs[idx[col]];
+    for (int j = col + 1; j < 3; j++)
+      s -= a[idx[col]][j] * x[j];
+    x[col] = s / a[idx[col]][col];
+  }
+  return 0;
+}
+
+/* removeNetRotationPlane (gvizGRIPEmbedder.c:1156-1185) */
+static void remove_plane_rotation(const int64_t *filtration, int64_t active, int64_t dim,
+                                  int a, int b, const double *com,
+                                  oc_grip_vertex_nd *dec, const double *pos) {
+  double L = 0.0, inertia = 0.0;
+  for (int64_t i = 0; i < active; i++) {
+    const double *p = pos + dim * filtration[i];
+    const double *d = dec[filtration[i]].disp;
+    double ra = p[a] - com[a], rb = p[b] - com[b];
+    L += ra * d[b] - rb * d[a];
+    inertia += ra * ra + rb * rb;
+  }
+  if (inertia < OC_NUMERIC_EPS)
+    return;
+  double omega = L / inertia;
+  for (int64_t i = 0; i < active; i++) {
+    const double *p = pos + dim * filtration[i];
+    double *d = dec[filtration[i]].disp;
+    double ra = p[a] - com[a], rb = p[b] - com[b];
+    d[a] += omega * rb;
+    d[b] -= omega * ra;
+  }
+}
+
+/* gvizGRIPEmbedderRefineRound (gvizGRIPEmbedder.c:1289-1333) with
+ * removeNetTranslation (:1106-1121) and removeNetRotation (:1192-1261). */
+void oc_grip_refine_round_nd(const oc_csr *g, const int64_t *filtration,
+                             int64_t active, int64_t layer, const oc_found *knn,
+                             int64_t kcap, int64_t dim, oc_grip_vertex_nd *dec,
+                             double *pos, oc_grip_round_stats *stats) {
+  for (int64_t i = 0; i < active; i++)
+    refine_vertex_nd(g, filtration[i], layer, knn, kcap, dim, &dec[filtration[i]], pos);
+  if (active > 0) {
+    double mean[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < active; i++)
+      for (int64_t k = 0; k < dim; k++)
+        mean[k] += 1.0 * dec[filtration[i]].disp[k];
+    double inv = 1.0 / (double)active;
+    for (int64_t k = 0; k < dim; k++)
+      mean[k] *= inv;
+    for (int64_t i = 0; i < active; i++)
+      for (int64_t k = 0; k < dim; k++)
+        dec[filtration[i]].disp[k] += -1.0 * mean[k];
+  }
+  if (active >= 3) {
+    double com[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < active; i++)
+      for (int64_t k = 0; k < dim; k++)
+        com[k] += pos[dim * filtration[i] + k];
+    for (int64_t k = 0; k < dim; k++)
+      com[k] /= (double)active;
+    if (dim == 4) {
+      static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+      for (int q = 0; q < 6; q++)
+        remove_plane_rotation(filtration, active, dim, pa[q], pb[q], com, dec, pos);
+    } else {
+      /* 2-D and 3-D: omega = I^-1 L about the centre, 2-D through I22 */
+      const int three = dim == 3;
+      double L[3] = {0, 0, 0}, I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int64_t i = 0; i < active; i++) {
+        const double *p = pos + dim * filtration[i];
+        const double *d = dec[filtration[i]].disp;
+        double r[3] = {p[0] - com[0], p[1] - com[1], three ? p[2] - com[2] : 0.0};
+        double dz = three ? d[2] : 0.0;
+        L[0] += r[1] * dz - r[2] * d[1];
+        L[1] += r[2] * d[0] - r[0] * dz;
+        L[2] += r[0] * d[1] - r[1] * d[0];
+        double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++)
+            I[a][b] += (a == b ? r2 : 0.0) - r[a] * r[b];
+      }
+      double w[3] = {0.0, 0.0, 0.0};
+      int ok;
+      if (!three) {
+        ok = !(fabs(I[2][2]) < OC_NUMERIC_EPS);
+        if (ok)
+          w[2] = L[2] / I[2][2];
+      } else {
+        ok = solve3(I, L, w) == 0;
+      }
+      if (ok)
+        for (int64_t i = 0; i < active; i++) {
+          const double *p = pos + dim * filtration[i];
+          double *d = dec[filtration[i]].disp;
+          double r[3] = {p[0] - com[0], p[1] - com[1], three ? p[2] - com[2] : 0.0};
+          d[0] -= w[1] * r[2] - w[2] * r[1];
+          d[1] -= w[2] * r[0] - w[0] * r[2];
+          if (three)
+            d[2] -= w[0] * r[1] - w[1] * r[0];
+        }
+    }
+  }
+  double maxDisp = 0.0, sumDisp = 0.0, sumHeat = 0.0, sumForce = 0.0;
+  for (int64_t i = 0; i < active; i++) {
+    int64_t v = filtration[i];
+    oc_grip_vertex_nd *d = &dec[v];
+    double nrm = sqrt(nd_dot(dim, d->disp, d->disp));
+    if (nrm > maxDisp)
+      maxDisp = nrm;
+    sumDisp += nrm;
+    sumHeat += d->heat;
+    sumForce += d->lastForceMag;
+    for (int64_t k = 0; k < dim; k++)
+      pos[dim * v + k] += d->disp[k];
+  }
+  if (stats) {
+    stats->maxDisp = maxDisp;
+    stats->meanDisp = active ? sumDisp / (double)active : 0.0;
+    stats->meanForce = active ? sumForce / (double)active : 0.0;
+    stats->meanHeat = active ? sumHeat / (double)active : 0.0;
+  }
+}
+
 /* src/embedders/gvizGRIPEmbedder.c:335-420 */
 void oc_regular_simplex(int64_t n, double sideLength, double *out) {
   memset(out, 0, sizeof(double) * (size_t)((n + 1) * n));

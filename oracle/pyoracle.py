@@ -149,6 +149,9 @@ def lib():
         L.oc_grip_refine_round.argtypes = [
             C.POINTER(CSR), _i64p, C.c_int64, C.c_int64, _i64p, C.c_int64,
             C.c_void_p, _f64p, C.POINTER(GripRoundStats)]
+        L.oc_grip_refine_round_nd.argtypes = [
+            C.POINTER(CSR), _i64p, C.c_int64, C.c_int64, _i64p, C.c_int64, C.c_int64,
+            C.c_void_p, _f64p, C.POINTER(GripRoundStats)]
         L.oc_regular_simplex.argtypes = [C.c_int64, C.c_double, _f64p]
         _lib = L
     return _lib
@@ -367,6 +370,26 @@ def grip_refine_round(g: Graph, filtration, active, layer, knn, kcap, dec, pos):
                                np.ascontiguousarray(filtration, dtype=np.int64),
                                active, layer, knn.reshape(-1), kcap,
                                dec.ctypes.data, pos.reshape(-1), C.byref(st))
+    return st
+
+
+# oc_grip_vertex_nd
+GRIP_VERTEX_ND_DTYPE = np.dtype([
+    ("disp", "f8", 4), ("oldDisp", "f8", 4), ("heat", "f8"), ("oldCos", "f8"),
+    ("lastForceMag", "f8"), ("knnCount", "i8"), ("dispCalculated", "i8")])
+
+
+def grip_refine_round_nd(g: Graph, filtration, active, layer, knn, kcap, dim, dec, pos):
+    """The round in dim = 2, 3 or 4 coordinates: dec a GRIP_VERTEX_ND_DTYPE
+    array (n,) and pos (n, dim), both updated in place."""
+    assert dec.dtype == GRIP_VERTEX_ND_DTYPE and pos.shape == (g.n, dim)
+    assert pos.flags.c_contiguous and pos.dtype == np.float64
+    st = GripRoundStats()
+    knn = np.ascontiguousarray(knn, dtype=np.int64)
+    lib().oc_grip_refine_round_nd(C.byref(g.c),
+                                  np.ascontiguousarray(filtration, dtype=np.int64),
+                                  active, layer, knn.reshape(-1), kcap, dim,
+                                  dec.ctypes.data, pos.reshape(-1), C.byref(st))
     return st
 
 

@@ -216,3 +216,68 @@ def test_grip_rounds_chain_bit_exact():
                        ("dispCalculated", d1["dispCalculated"])):
             assert np.array_equal(dec[k][ids], ref[ids]), (i, k)
         assert (st.maxDisp, st.meanDisp, st.meanForce) == tuple(f[f"r{i}_stats"])
+
+
+def _nd_state(f, i, when, n, dim):
+    d0 = f[f"r{i}_dec{when}"]
+    dec = np.zeros(n, dtype=O.GRIP_VERTEX_ND_DTYPE)
+    dec["disp"][:, :dim] = d0["disp"]
+    dec["oldDisp"][:, :dim] = d0["oldDisp"]
+    for k in ("heat", "oldCos", "lastForceMag", "knnCount", "dispCalculated"):
+        dec[k] = d0[k]
+    return dec
+
+
+@pytest.mark.parametrize("name", ["grip_rgg300_d3", "grip_rgg300_d4",
+                                  "grip_grid16_k4_d3", "grip_grid16_k4_d4"])
+def test_grip_rounds_nd_bit_exact(name):
+    """Every round of the reference's 3-D and 4-D runs from its own state:
+    positions, displacements, heat and statistics bit for bit."""
+    f = golden(name)
+    g = _graph(f)
+    n, dim, kcap = int(f["n"]), int(f["dim"]), int(f["kcap"])
+    filt = f["filtration"]
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(f["borders"][layer])
+        dec = _nd_state(f, i, 0, n, dim)
+        pos = f[f"r{i}_pos0"].copy()
+        st = O.grip_refine_round_nd(g, filt, active, layer, f[f"r{i}_knn"], kcap, dim,
+                                    dec, pos)
+        assert np.array_equal(pos, f[f"r{i}_pos1"]), (i, layer)
+        d1 = f[f"r{i}_dec1"]
+        assert np.array_equal(dec["disp"][:, :dim], d1["disp"]), i
+        assert np.array_equal(dec["oldDisp"][:, :dim], d1["oldDisp"]), i
+        for k in ("heat", "oldCos", "lastForceMag"):
+            assert np.array_equal(dec[k], d1[k]), (i, k)
+        assert (st.maxDisp, st.meanDisp, st.meanForce) == tuple(f[f"r{i}_stats"])
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4])
+def test_grip_round_nd_with_zero_coordinates_is_the_2d_round(dim):
+    """A layout whose coordinates beyond the first two are zero stays in the
+    plane and moves exactly as the 2-D round moves it: the 3-D inertia
+    system decouples (its z row holds the 2-D moment) and of the six 4-D
+    planes only (0, 1) carries a rotation. Checked against the reference's
+    2-D fixtures."""
+    f = golden("grip_rgg300")
+    g = _graph(f)
+    n, kcap = int(f["n"]), int(f["kcap"])
+    filt = f["filtration"]
+    for i in range(int(f["rounds"])):
+        layer = int(f[f"r{i}_layer"])
+        active = int(f["borders"][layer])
+        d0 = f[f"r{i}_dec0"]
+        dec = np.zeros(n, dtype=O.GRIP_VERTEX_ND_DTYPE)
+        dec["disp"][:, :2] = d0["disp"]
+        dec["oldDisp"][:, :2] = d0["oldDisp"]
+        for k in ("heat", "oldCos", "lastForceMag", "knnCount", "dispCalculated"):
+            dec[k] = d0[k]
+        pos = np.zeros((n, dim))
+        pos[:, :2] = f[f"r{i}_pos0"]
+        st = O.grip_refine_round_nd(g, filt, active, layer, f[f"r{i}_knn"], kcap, dim,
+                                    dec, pos)
+        assert np.array_equal(pos[:, :2], f[f"r{i}_pos1"]), (dim, i)
+        assert not np.any(pos[:, 2:]), (dim, i)
+        assert np.array_equal(dec["heat"], f[f"r{i}_dec1"]["heat"])
+        assert (st.maxDisp, st.meanDisp, st.meanForce) == tuple(f[f"r{i}_stats"])
