@@ -11,6 +11,7 @@
 #include "gviz/gvizSpringTutteEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_planar.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -92,12 +93,35 @@ void gvizSpringTutteEmbedderConfigure(gvizSpringTutteState *s, double stiffness,
     s->damping = damping;
 }
 
-/* The reference's Begin embeds the graph in the plane (planarity test,
- * rotation system, largest face: :164-191); that stage is not part of this
- * engine. */
+/* src/embedders/gvizSpringTutteEmbedder.c:164-191: planarity test and rotation system, then the largest face
+ * (gviz_planar.c) fixed on a convex polygon of radius 200 and the interior
+ * seeded at its centroid. -2 for a graph that is not planar, -1 for another
+ * dimension than 2 or a failure. The reference's planarity suite installs
+ * its rotation into the parent's adjacency order; this engine keeps the
+ * insertion order (only the order of each barycentre's sum differs). */
 int gvizSpringTutteEmbedderBegin(gvizSpringTutteState *s) {
-  (void)s;
-  return -1;
+  if (!s || dim_of(s) != 2)
+    return -1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  int64_t *off = NULL, *adj = NULL;
+  if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
+    return -1;
+  const size_t N = num_vertices(s);
+  size_t *boundary = malloc(sizeof(size_t) * (N ? N : 1));
+  size_t count = 0;
+  int rc = boundary ? gviz_planar_largest_face((int64_t)N, off, adj, boundary, N, &count)
+                    : -1;
+  free(off);
+  free(adj);
+  if (rc != 0) {
+    free(boundary);
+    return rc == -2 ? -2 : -1;
+  }
+  gvizSpringTutteFixConvexPolygon(s, boundary, count, 200.0);
+  free(boundary);
+  gvizSpringTutteEmbedderSeedInterior(s);
+  s->begun = 1;
+  return 0;
 }
 
 int gvizSpringTutteEmbedderFixOuterFace(gvizSpringTutteState *s) {

@@ -6,13 +6,15 @@
  * bookkeeping done as the reference does them; every relaxation sweep runs
  * on the device (gvizhip_tutte_sweeps), which keeps the live positions and
  * the Jacobi snapshot in HBM. Positions cross the bus when the host changed
- * them (boundary, seeding) and after each Step / Run. Gauss-Seidel sweeps
- * and the planarity-based Begin / FixOuterFace are not part of this engine
+ * them (boundary, seeding) and after each Step / Run. Begin's planarity
+ * test and face search run on the host (gviz_planar.c). Gauss-Seidel sweeps
+ * and FixOuterFace (a face picked in a viewer) are not part of this engine
  * and return -1.
  */
 #include "gviz/gvizTutteEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_planar.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -81,13 +83,35 @@ void gvizTutteEmbedderRelease(gvizTutteState *s) {
   gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
 }
 
-/* The reference's Begin embeds the graph in the plane (planarity test and
- * rotation system, src/embedders/gvizTutteEmbedder.c:149-176) before fixing
- * the largest face; that stage is not part of this engine. Callers fix a
- * boundary with gvizTutteFixConvexPolygon or gvizTutteEmbedderSetBoundary. */
+/* src/embedders/gvizTutteEmbedder.c:145-173: planarity test and rotation system, then the largest face
+ * (gviz_planar.c) fixed on a convex polygon of radius 200 and the interior
+ * seeded at its centroid. -2 for a graph that is not planar, -1 for another
+ * dimension than 2 or a failure. The reference's planarity suite installs
+ * its rotation into the parent's adjacency order; this engine keeps the
+ * insertion order (only the order of each barycentre's sum differs). */
 int gvizTutteEmbedderBegin(gvizTutteState *s) {
-  (void)s;
-  return -1;
+  if (!s || dim_of(s) != 2)
+    return -1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  int64_t *off = NULL, *adj = NULL;
+  if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
+    return -1;
+  const size_t N = num_vertices(s);
+  size_t *boundary = malloc(sizeof(size_t) * (N ? N : 1));
+  size_t count = 0;
+  int rc = boundary ? gviz_planar_largest_face((int64_t)N, off, adj, boundary, N, &count)
+                    : -1;
+  free(off);
+  free(adj);
+  if (rc != 0) {
+    free(boundary);
+    return rc == -2 ? -2 : -1;
+  }
+  gvizTutteFixConvexPolygon(s, boundary, count, 200.0);
+  free(boundary);
+  gvizTutteEmbedderSeedInterior(s);
+  s->begun = 1;
+  return 0;
 }
 
 int gvizTutteEmbedderFixOuterFace(gvizTutteState *s) {

@@ -5,9 +5,10 @@
  * a = stiffness (bary - P) - damping v, v += a dt, P += v dt, from a Jacobi
  * snapshot of the positions. Same fixed point as the Tutte sweep.
  *
- * Planarity-based Begin / FixOuterFace are outside this engine and return
- * -1; callers pin a boundary with gvizSpringTutteFixConvexPolygon or
- * gvizSpringTutteEmbedderSetBoundary. The Jacobi snapshot lives on the
+ * Begin tests planarity on the host (left-right test) and fixes the
+ * largest face on a convex polygon (0, or -2 for a graph that is not
+ * planar); FixOuterFace (a face picked in a viewer) is outside this engine
+ * and returns -1. The Jacobi snapshot lives on the
  * device, so the state has no scratchPos; velocity is mirrored on the host
  * after every Step / Run. The device step is two-dimensional: Step and Run
  * fail (negative / -1) for another dimension.

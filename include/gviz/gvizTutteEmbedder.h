@@ -1,7 +1,9 @@
 /*
  * gvizTutteEmbedder.h -- the reference's Tutte barycentric embedder API
  * (include/embedders/gvizTutteEmbedder.h) with the Jacobi sweep on the GPU.
- * Gauss-Seidel sweeps and planarity-based Begin / FixOuterFace are outside
+ * Begin tests planarity on the host (left-right test) and fixes the largest
+ * face on a convex polygon: 0, -2 for a graph that is not planar. Gauss-
+ * Seidel sweeps and FixOuterFace (a face picked in a viewer) are outside
  * this engine: they return -1 (Step: a negative value).
  */
 #ifndef GVIZ_HOST_TUTTE_EMBEDDER_H

@@ -632,6 +632,55 @@ static int mode_edges_file(void) {
  * Run's return, positions and velocities after it, and the
  * "springTutte.maxDelta" series. Input: graph, boundary count, steps,
  * runTo, boundary, radius, dt, stiffness, damping, epsilon. */
+/* Tutte and spring-Tutte Begin (planarity, largest face, convex polygon,
+ * seeded interior), then Run to convergence. */
+static int mode_tutte_begin(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t spring = rd_i(), iters = rd_i();
+  double eps = rd_d();
+  const double *p;
+  int rc, it;
+  if (!spring) {
+    gvizTutteState s;
+    if (gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, eps) != 0)
+      return 1;
+    rc = gvizTutteEmbedderBegin(&s);
+    wr_i(rc);
+    if (rc == 0) {
+      wr_i((int64_t)s.boundaryCount);
+      for (size_t i = 0; i < s.boundaryCount; i++)
+        wr_i((int64_t)s.boundary[i]);
+      it = gvizTutteEmbedderRun(&s, (size_t)iters);
+      wr_i(it);
+      p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(p[i]);
+    }
+    gvizTutteEmbedderRelease(&s);
+  } else {
+    gvizSpringTutteState s;
+    if (gvizSpringTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, eps) != 0)
+      return 1;
+    rc = gvizSpringTutteEmbedderBegin(&s);
+    wr_i(rc);
+    if (rc == 0) {
+      wr_i((int64_t)s.boundaryCount);
+      for (size_t i = 0; i < s.boundaryCount; i++)
+        wr_i((int64_t)s.boundary[i]);
+      it = gvizSpringTutteEmbedderRun(&s, (size_t)iters, 1.0 / 60.0);
+      wr_i(it);
+      p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(p[i]);
+    }
+    gvizSpringTutteEmbedderRelease(&s);
+  }
+  gvizGraphRelease(&g);
+  return 0;
+}
+
 static int mode_spring_tutte(void) {
   gvizGraph g;
   int64_t n;
@@ -704,6 +753,8 @@ int main(int argc, char **argv) {
     rc = mode_edges_file();
   else if (!strcmp(argv[1], "spring_tutte"))
     rc = mode_spring_tutte();
+  else if (!strcmp(argv[1], "tutte_begin"))
+    rc = mode_tutte_begin();
   fclose(fin);
   fclose(fout);
   return rc;

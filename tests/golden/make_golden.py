@@ -305,9 +305,35 @@ def spring_tutte_cases():
              **{key: (v if v is not None else np.zeros(0)) for key, v in r.items()})
 
 
+def tutte_begin_cases():
+    """Begin through the planarity suite on 3-connected planar graphs (a
+    Delaunay triangulation, a wheel-like grid) and a non-planar graph, then
+    Run to convergence, for the Tutte and the spring-driven Tutte embedder."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(11)
+    pts = rng.random((400, 2))
+    e = set()
+    for t in Delaunay(pts).simplices:
+        for a in range(3):
+            u, v = int(t[a]), int(t[(a + 1) % 3])
+            e.add((min(u, v), max(u, v)))
+    e = np.array(sorted(e), np.int64)
+    n2, e2 = G.grid(10, 10)
+    # close the grid into a 3-connected graph: the diagonals of every cell
+    d = [(10 * r + c, 10 * (r + 1) + c + 1) for r in range(9) for c in range(9)]
+    e2 = np.concatenate([np.asarray(e2, np.int64), np.array(d, np.int64)])
+    k33 = np.array([(i, 3 + j) for i in range(3) for j in range(3)], np.int64)
+    for name, n, ee in (("delaunay400", 400, e), ("trigrid10", n2, e2), ("k33", 6, k33)):
+        for spring in (0, 1):
+            r = refio.tutte_begin(n, ee, spring, 20000, 1e-9)
+            save(f"tutte_begin_{name}_{'spring' if spring else 'jacobi'}", n=n, edges=ee,
+                 spring=spring, iters=20000, epsilon=1e-9,
+                 **{k: np.asarray(v) for k, v in r.items()})
+
+
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
              knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
-             grip_nd=grip_nd_cases,
+             grip_nd=grip_nd_cases, tutte_begin=tutte_begin_cases,
              api=api_cases, edges=edges_cases, spring_tutte=spring_tutte_cases)
 
 if __name__ == "__main__":

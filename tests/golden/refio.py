@@ -347,3 +347,21 @@ def grip_checkpoint(n, edges, kcap, pk, rk, policy, stages, rounds, path, exe):
         vis = r.ia(n)
         out.append((layer, pos, vis))
     return out
+
+
+def tutte_begin(n, edges, spring, iters, epsilon, exe=None):
+    """Tutte (spring=0) or spring-Tutte (spring=1) Begin then Run(iters):
+    Begin's return, and when 0 the boundary cycle, Run's return and the
+    positions (n, 2)."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(spring, iters)
+    w.d(epsilon)
+    r = _R(_run("tutte_begin", w.bytes(), exe))
+    out = dict(begin_rc=r.i())
+    if out["begin_rc"] == 0:
+        c = r.i()
+        out["boundary"] = r.ia(c)
+        out["run"] = r.i()
+        out["pos"] = r.da(2 * n).reshape(n, 2)
+    return out

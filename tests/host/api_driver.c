@@ -500,9 +500,22 @@ static void test_tutte_boundaryAndSeed(void) {
   const double *p1 = gvizEmbeddedGraphGetVPosition(e, 1);
   CHECK(fabs(p1[0]) < 1e-12 && fabs(p1[1]) < 1e-12);
   CHECK(gvizTutteEmbedderStep(&s, 0.0) == 0.0 && s.iteration == 0);
-  CHECK(gvizTutteEmbedderBegin(&s) == -1);
+  /* Begin on the 6-cycle: planar, both faces are the cycle */
+  CHECK(gvizTutteEmbedderBegin(&s) == 0 && s.boundaryCount == 6 && s.begun);
   gvizTutteEmbedderRelease(&s);
   gvizGraphRelease(&g);
+  /* K5 is not planar */
+  gvizGraph k5;
+  gvizGraphInit(&k5, 0);
+  for (size_t i = 0; i < 5; i++)
+    gvizGraphAddVertex(&k5, NULL, NULL, NULL);
+  for (size_t i = 0; i < 5; i++)
+    for (size_t j = i + 1; j < 5; j++)
+      gvizGraphAddEdge(&k5, i, j);
+  CHECK(gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&k5), 2, 0.0) == 0);
+  CHECK(gvizTutteEmbedderBegin(&s) == -2 && !s.begun);
+  gvizTutteEmbedderRelease(&s);
+  gvizGraphRelease(&k5);
 }
 
 static void test_grip_initAndK(void) {
@@ -713,6 +726,55 @@ static int mode_edges_file(void) {
  * Run's return, positions and velocities after it, and the
  * "springTutte.maxDelta" series. Input: graph, boundary count, steps,
  * runTo, boundary, radius, dt, stiffness, damping, epsilon. */
+/* Tutte and spring-Tutte Begin (planarity, largest face, convex polygon,
+ * seeded interior), then Run to convergence. */
+static int mode_tutte_begin(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t spring = rd_i(), iters = rd_i();
+  double eps = rd_d();
+  const double *p;
+  int rc, it;
+  if (!spring) {
+    gvizTutteState s;
+    if (gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, eps) != 0)
+      return 1;
+    rc = gvizTutteEmbedderBegin(&s);
+    wr_i(rc);
+    if (rc == 0) {
+      wr_i((int64_t)s.boundaryCount);
+      for (size_t i = 0; i < s.boundaryCount; i++)
+        wr_i((int64_t)s.boundary[i]);
+      it = gvizTutteEmbedderRun(&s, (size_t)iters);
+      wr_i(it);
+      p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(p[i]);
+    }
+    gvizTutteEmbedderRelease(&s);
+  } else {
+    gvizSpringTutteState s;
+    if (gvizSpringTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, eps) != 0)
+      return 1;
+    rc = gvizSpringTutteEmbedderBegin(&s);
+    wr_i(rc);
+    if (rc == 0) {
+      wr_i((int64_t)s.boundaryCount);
+      for (size_t i = 0; i < s.boundaryCount; i++)
+        wr_i((int64_t)s.boundary[i]);
+      it = gvizSpringTutteEmbedderRun(&s, (size_t)iters, 1.0 / 60.0);
+      wr_i(it);
+      p = gvizEmbeddedGraphPositions((gvizEmbeddedGraph *)&s);
+      for (int64_t i = 0; i < 2 * n; i++)
+        wr_d(p[i]);
+    }
+    gvizSpringTutteEmbedderRelease(&s);
+  }
+  gvizGraphRelease(&g);
+  return 0;
+}
+
 static int mode_spring_tutte(void) {
   gvizGraph g;
   int64_t n;
@@ -773,6 +835,8 @@ int main(int argc, char **argv) {
     rc = mode_edges_file();
   else if (!strcmp(argv[1], "spring_tutte"))
     rc = mode_spring_tutte();
+  else if (!strcmp(argv[1], "tutte_begin"))
+    rc = mode_tutte_begin();
   else if (!strcmp(argv[1], "grip_checkpoint"))
     rc = mode_grip_checkpoint();
   else if (!strcmp(argv[1], "force_checkpoint"))

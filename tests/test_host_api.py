@@ -323,3 +323,52 @@ def test_grip_embed_matches_reference_nd(gpu, name):
     s_gpu = _stress_nd(n, f["edges"], r["pos"])
     print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g}")
     assert abs(s_gpu - s_ref) <= 0.05 * abs(s_ref)
+
+
+def _same_cycle_shift(a, b):
+    """(shift, reversed) with a == roll(b or reversed b), or None."""
+    a, b = list(a), list(b)
+    if len(a) != len(b) or sorted(a) != sorted(b):
+        return None
+    for rev in (False, True):
+        c = list(reversed(b)) if rev else b
+        k = c.index(a[0])
+        if c[k:] + c[:k] == a:
+            return k, rev
+    return None
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["delaunay400_jacobi", "delaunay400_spring",
+                                  "trigrid10_jacobi", "trigrid10_spring",
+                                  "k33_jacobi", "k33_spring"])
+def test_tutte_begin_matches_reference_up_to_polygon_symmetry(gpu, name):
+    """Tutte / spring-Tutte Begin (planarity test, largest face, convex
+    polygon, seeded interior) and Run to convergence. The reference embeds
+    with the Boyer-Myrvold suite, this engine with the left-right test; on
+    these 3-connected graphs the largest face is the same cycle, which may
+    start at another vertex or run the other way round, so the converged
+    layout equals the reference's under that symmetry of the polygon (an
+    orthogonal map fitted on the boundary). A non-planar graph returns -2
+    from both."""
+    f = golden("tutte_begin_" + name)
+    n = int(f["n"])
+    print(f"tutte begin {name}", flush=True)
+    r = refio.tutte_begin(n, f["edges"], int(f["spring"]), int(f["iters"]),
+                          float(f["epsilon"]), exe=DRIVER)
+    assert r["begin_rc"] == int(f["begin_rc"])
+    if r["begin_rc"] != 0:
+        return
+    sym = _same_cycle_shift(r["boundary"], f["boundary"])
+    assert sym is not None, (r["boundary"], f["boundary"])
+    ref_run = int(f["run"])
+    assert abs(r["run"] - ref_run) <= max(2, 0.01 * ref_run), (r["run"], ref_run)
+    b = np.asarray(f["boundary"])
+    # orthogonal Q with pos_ref[b] @ Q ~ pos[b] (both polygons are centred)
+    u, _, vt = np.linalg.svd(f["pos"][b].T @ r["pos"][b])
+    q = u @ vt
+    assert np.allclose(q @ q.T, np.eye(2), atol=1e-12)
+    err = np.abs(f["pos"] @ q - r["pos"]).max()
+    print(f"  boundary {len(b)} (shift {sym[0]}, reversed {sym[1]}), run {r['run']} vs "
+          f"{ref_run}, max deviation {err:.3g}", flush=True)
+    assert err <= 1e-6 * 200.0
