@@ -327,3 +327,48 @@ def test_grip_rounds_chain_through_device_state_nd(gpu, name):
                           f[f"r{i}_dec1"], None, ids)
         np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
                                    f[f"r{i}_stats"], rtol=1e-11)
+
+
+@pytest.mark.parametrize("dim", [3, 4])
+@pytest.mark.parametrize("layer", [0, 1])
+def test_refine_round_nd_matches_oracle_on_larger_graph(gpu, dim, layer):
+    """One 3-D / 4-D round on a 20,000-vertex graph from a random state
+    (positions, displacements, heat, cosines, half the vertices already
+    computed, full kNN lists with graph distances 1..5), device against the
+    oracle's serial round: layer 0 runs the FR-like edge and kNN forces,
+    layer 1 the Kamada-Kawai kNN springs."""
+    n, kcap = 20000, 8
+    _, e, _ = G.random_geometric(n, 8.0, 13)
+    off, adj = G.csr_from_edges(n, e)
+    rng = np.random.default_rng(100 * dim + layer)
+    filt = rng.permutation(n).astype(np.int64)
+    active = n - 1000 * layer  # layer 1: a prefix of the filtration
+    pos = rng.uniform(-100.0, 100.0, (n, dim))
+    kv = rng.integers(0, n, (n, kcap)).astype(np.int64)
+    kd = rng.integers(1, 6, (n, kcap)).astype(np.int64)
+    dec = np.zeros(n, dtype=O.GRIP_VERTEX_ND_DTYPE)
+    dec["disp"][:, :dim] = rng.normal(0.0, 2.0, (n, dim))
+    dec["oldDisp"][:, :dim] = rng.normal(0.0, 2.0, (n, dim))
+    dec["heat"] = rng.uniform(0.1, 5.0, n)
+    dec["oldCos"] = rng.uniform(-1.0, 1.0, n)
+    dec["knnCount"] = kcap
+    dec["dispCalculated"] = rng.integers(0, 2, n)
+    eng = gpu.GripEngine(off, adj, kcap=kcap, dim=dim)
+    eng.set_filtration(filt)
+    eng.upload(pos, dec, kv, kd)
+    st = eng.refine_round(active, layer)
+    got_pos, got_dec, _, _ = eng.download()
+    knn = np.stack([kv, kd], axis=2)
+    ref_dec = dec.copy()
+    ref_pos = np.ascontiguousarray(pos.copy())
+    ref = O.grip_refine_round_nd(O.Graph(off, adj), filt, active, layer, knn, kcap, dim,
+                                 ref_dec, ref_pos)
+    ids = filt[:active]
+    scale = np.abs(ref_pos).max()
+    assert np.abs(got_pos - ref_pos).max() <= 1e-12 * scale
+    for k in ("heat", "oldCos", "lastForceMag"):
+        np.testing.assert_allclose(got_dec[k][ids], ref_dec[k][ids], rtol=1e-12, err_msg=k)
+    d = np.abs(got_dec["disp"][ids] - ref_dec["disp"][ids]).max()
+    assert d <= 1e-11 * np.abs(ref_dec["disp"][ids]).max()
+    np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
+                               [ref.maxDisp, ref.meanDisp, ref.meanForce], rtol=1e-11)
