@@ -21,6 +21,7 @@ The other measured paths, one JSON line each:
     python bench.py --workload allpairs_er65k    exact FR all-pairs step, ER n=65,536 avg deg 8
     python bench.py --workload tutte_delaunay1m  Tutte Jacobi sweeps, Delaunay n=1,000,000
     python bench.py --workload grip_rgg4m        GRIP schedule and finest-layer rounds, RGG n=4,000,000
+                    [--grip-dim 3|4]             the same in three or four dimensions
     python bench.py --workload batch4096         4,096 graphs (50..20,000 vertices) laid out together
 """
 from __future__ import annotations
@@ -327,6 +328,23 @@ def regular_simplex2(side):
     return v * (side / edge)
 
 
+def regular_simplex(dim, side):
+    """dim + 1 unit vectors with pairwise dot product -1/dim (the
+    construction of makeRegularSimplex, src/embedders/gvizGRIPEmbedder.c:
+    335-420), scaled so that an edge is `side` long."""
+    if dim == 2:
+        return regular_simplex2(side)
+    v = np.zeros((dim + 1, dim))
+    c = -1.0 / dim
+    v[0, 0] = 1.0
+    for k in range(1, dim + 1):
+        for j in range(k):
+            v[k, j] = (c - v[k, :j] @ v[j, :j]) / v[j, j]
+        if k < dim:
+            v[k, k] = np.sqrt(max(1.0 - v[k, :k] @ v[k, :k], 0.0))
+    return v * (side / np.sqrt(((v[0] - v[1]) ** 2).sum()))
+
+
 def prefer_batch(n_active, n_edges, visible, targets):
     """src/algorithms/search/gvizKNearest.c:149-166."""
     if visible == 0 or visible >= targets or visible > 256 or n_active == 0:
@@ -342,25 +360,26 @@ def bench_grip(args):
     value = refinement rounds/s at the finest layer."""
     import struct
     k = 10
+    dim = args.grip_dim
     n, edges, _ = G.random_geometric(4_000_000, 8.0, 3)
     act, inside = G.largest_component(n, edges)
     off, adj = G.csr_from_edges(n, inside)
     na = int(act.sum())
     ne = int(inside.shape[0])
-    eng = native.GripEngine(off, adj, active=act, kcap=k)
+    eng = native.GripEngine(off, adj, active=act, kcap=k, dim=dim)
     t_all = time.perf_counter()
-    filt, borders = eng.mis_filtration(2)
+    filt, borders = eng.mis_filtration(dim)
     t_begin = time.perf_counter() - t_all
     layers = len(borders)
-    pos = np.zeros((n, 2))
-    pos[filt[:3]] = regular_simplex2(10.0 * 1000)
+    pos = np.zeros((n, dim))
+    pos[filt[:dim + 1]] = regular_simplex(dim, 10.0 * 1000)
     eng.upload(pos)
     vis = np.zeros(n, np.uint8)
-    vis[filt[:3]] = 1
+    vis[filt[:dim + 1]] = 1
     layer = layers - 1
     end = int(borders[layer])
     eng.clear_decorators(end)
-    eng.update_knn(end, vis, k, prefer_batch(na, ne, 3, end))
+    eng.update_knn(end, vis, k, prefer_batch(na, ne, dim + 1, end))
     rounds_total = 0
     while True:
         active = int(borders[layer])
@@ -389,13 +408,17 @@ def bench_grip(args):
     f_ms, r_ms = eng.profile_rounds(na, 0, 5)
     launch_s = f_ms / 1e3 / 5
     deg_sum = int(adj.shape[0])
-    bytes_per = na * (16 + 8 + 8 + 16 + 8 + 8 + 8 + 16 + 16 + 8 + 8 + 8 + 8) \
-        + 4 * deg_sum + 16 * k * na
+    # per active vertex: four dim-vectors (position read, displacement read
+    # and written, previous displacement written) and nine 8-byte scalars
+    bytes_per = na * (4 * 8 * dim + 9 * 8) + 4 * deg_sum + 16 * k * na
     achieved = bytes_per / launch_s / 1e9
-    line = base_line("GRIP refinement rounds/s at the finest layer, RGG N=4,000,000, k=10",
+    metric = "GRIP refinement rounds/s at the finest layer, RGG N=4,000,000, k=10"
+    if dim != 2:
+        metric += f", dim {dim}"
+    line = base_line(metric,
                      steps / el, "rounds/s", args, 1e3 * el / steps, "f64",
                      "random geometric graph, 4,000,000 uniform points in the unit square, "
-                     "radius for average degree 8, largest component; GRIP dim 2, "
+                     f"radius for average degree 8, largest component; GRIP dim {dim}, "
                      "kNN capacity 10, K = 10 constant; the reference's Embed schedule",
                      {"n_vertices": n, "active_vertices": na, "n_edges": ne,
                       "layers": layers, "borders": [int(b) for b in borders]})
@@ -404,7 +427,9 @@ def bench_grip(args):
     line["schedule_rounds"] = rounds_total
     line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
                         "unit": "GB/s", "frac": round(achieved / 8000.0, 5),
-                        "traffic": None, "kernel": "k_refine (layer 0)",
+                        "traffic": None,
+                        "kernel": "k_refine (layer 0)" if dim == 2 else
+                                  f"k_refine_nd<{dim}> (layer 0)",
                         "note": "algorithmic bytes per launch = per active vertex its "
                                 "position, CSR row, kNN list and per-vertex state read and "
                                 "written once; neighbour position gathers not counted",
@@ -412,7 +437,7 @@ def bench_grip(args):
                         "round_ms": round(r_ms / 5, 4),
                         "bytes_per_launch": int(bytes_per)}
     cpu = None
-    if not args.no_cpu_baseline and os.path.exists(HARNESS):
+    if dim == 2 and not args.no_cpu_baseline and os.path.exists(HARNESS):
         threads = min(16, len(os.sched_getaffinity(0)))
         sn, se, _ = G.random_geometric(500_000, 8.0, 3)
         sact, _ = G.largest_component(sn, se)
@@ -589,6 +614,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"])
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--grip-dim", type=int, default=2, choices=[2, 3, 4],
+                    help="layout dimension of the grip_rgg4m workload")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--edges-file", default=None,
                     help="force_ba1m settings on the graph of a .edges file (Network "
