@@ -839,47 +839,78 @@ __global__ __launch_bounds__(kB) void k_refine_nd(
       f[k] = 0.0;
     const int32_t kc = knnCount[v];
     const int64_t kb = (int64_t)v * kcap;
+    // neighbour positions are gathered kGather at a time ahead of the
+    // (in-order) sums, as in the 2-D kernel
+    PtN<D> pu[kGather];
     if (layer > 0) {
-      for (int32_t j = 0; j < kc; j++) {
-        const double L = (double)knnD[kb + j] * kEdge;
-        if (L < 1e-9)
-          continue;
-        const PtN<D> pu = ld_pt<D>(pos, knnV[kb + j]);
+      for (int32_t j0 = 0; j0 < kc; j0 += kGather) {
+        const int m = kc - j0 < kGather ? kc - j0 : kGather;
 #pragma unroll
-        for (int k = 0; k < D; k++)
-          dd[k] = pu.c[k] - pv.c[k];
-        const double s = sqrt(sq_sum<D>(dd)) / L - 1.0;
+        for (int k = 0; k < kGather; k++)
+          if (k < m)
+            pu[k] = ld_pt<D>(pos, knnV[kb + j0 + k]);
 #pragma unroll
-        for (int k = 0; k < D; k++)
-          f[k] += s * dd[k];
+        for (int k = 0; k < kGather; k++) {
+          if (k >= m)
+            break;
+          const double L = (double)knnD[kb + j0 + k] * kEdge;
+          if (L < 1e-9)
+            continue;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            dd[c] = pu[k].c[c] - pv.c[c];
+          const double s = sqrt(sq_sum<D>(dd)) / L - 1.0;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            f[c] += s * dd[c];
+        }
       }
     } else {
       const double eSq = kEdge * kEdge;
-      for (int32_t e = off[v]; e < off[v + 1]; e++) {
-        const PtN<D> pu = ld_pt<D>(pos, adj[e]);
+      const int32_t e1 = off[v + 1];
+      for (int32_t e0 = off[v]; e0 < e1; e0 += kGather) {
+        const int m = e1 - e0 < kGather ? e1 - e0 : kGather;
 #pragma unroll
-        for (int k = 0; k < D; k++)
-          dd[k] = pu.c[k] - pv.c[k];
-        const double dsq = sq_sum<D>(dd);
-        if (dsq < 1e-18)
-          continue;
-        const double s = dsq / eSq;
+        for (int k = 0; k < kGather; k++)
+          if (k < m)
+            pu[k] = ld_pt<D>(pos, adj[e0 + k]);
 #pragma unroll
-        for (int k = 0; k < D; k++)
-          f[k] += s * dd[k];
+        for (int k = 0; k < kGather; k++) {
+          if (k >= m)
+            break;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            dd[c] = pu[k].c[c] - pv.c[c];
+          const double dsq = sq_sum<D>(dd);
+          if (dsq < 1e-18)
+            continue;
+          const double s = dsq / eSq;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            f[c] += s * dd[c];
+        }
       }
-      for (int32_t j = 0; j < kc; j++) {
-        const PtN<D> pu = ld_pt<D>(pos, knnV[kb + j]);
+      for (int32_t j0 = 0; j0 < kc; j0 += kGather) {
+        const int m = kc - j0 < kGather ? kc - j0 : kGather;
 #pragma unroll
-        for (int k = 0; k < D; k++)
-          dd[k] = pv.c[k] - pu.c[k];
-        const double dsq = sq_sum<D>(dd);
-        if (dsq < 1e-18)
-          continue;
-        const double s = kFrScale * eSq / dsq;
+        for (int k = 0; k < kGather; k++)
+          if (k < m)
+            pu[k] = ld_pt<D>(pos, knnV[kb + j0 + k]);
 #pragma unroll
-        for (int k = 0; k < D; k++)
-          f[k] += s * dd[k];
+        for (int k = 0; k < kGather; k++) {
+          if (k >= m)
+            break;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            dd[c] = pv.c[c] - pu[k].c[c];
+          const double dsq = sq_sum<D>(dd);
+          if (dsq < 1e-18)
+            continue;
+          const double s = kFrScale * eSq / dsq;
+#pragma unroll
+          for (int c = 0; c < D; c++)
+            f[c] += s * dd[c];
+        }
       }
     }
     double h = heat[v];
