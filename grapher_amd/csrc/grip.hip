@@ -1088,35 +1088,39 @@ __global__ void k_rotation3_solve(const double *__restrict__ partial, int blocks
 __constant__ int kPlaneA[6] = {0, 0, 0, 1, 1, 2};
 __constant__ int kPlaneB[6] = {1, 2, 3, 2, 3, 3};
 
-// Plane `plane`'s sums over the displacements that the previous plane's
-// removal (or, for the first plane, the translation removal) leaves.
+// The translation removal and the removals of planes [0, count) applied to
+// one vertex's displacement, in the reference's order.
+__device__ __forceinline__ void apply_planes(PtN<4> &d, const PtN<4> &p,
+                                             const GripRoundN *__restrict__ r, int count) {
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    d.c[k] += -1.0 * r->mean[k];
+  for (int q = 0; q < count; q++) {
+    if (!r->rotate[q])
+      continue;
+    const int qa = kPlaneA[q], qb = kPlaneB[q];
+    const double ra = p.c[qa] - r->com[qa], rb = p.c[qb] - r->com[qb];
+    d.c[qa] += r->omega[q] * rb;
+    d.c[qb] -= r->omega[q] * ra;
+  }
+}
+
+// Plane `plane`'s sums over the displacements the earlier removals leave.
+// Those are recomputed per vertex from the refinement's output (the same
+// operations in the same order), so no pass but the last writes them.
 __global__ __launch_bounds__(kB) void k_plane_sums(
     const int32_t *__restrict__ order, int64_t active, const double *__restrict__ pos,
-    double *__restrict__ disp, const GripRoundN *__restrict__ r, int plane,
+    const double *__restrict__ disp, const GripRoundN *__restrict__ r, int plane,
     double *__restrict__ partial) {
   __shared__ double sm[2 * 16];
   double acc[2] = {0.0, 0.0};
   const int a = kPlaneA[plane], b = kPlaneB[plane];
-  const bool first = plane == 0;
-  const bool prevRot = !first && r->rotate[plane - 1] != 0;
-  const int pa = first ? 0 : kPlaneA[plane - 1], pb = first ? 0 : kPlaneB[plane - 1];
-  const double pw = first ? 0.0 : r->omega[plane - 1];
   for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
        i += (int64_t)gridDim.x * kB) {
     const int32_t v = order[i];
     PtN<4> d = ld_pt<4>(disp, v);
     const PtN<4> p = ld_pt<4>(pos, v);
-    if (first) {
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        d.c[k] += -1.0 * r->mean[k];
-    } else if (prevRot) {
-      const double ra = p.c[pa] - r->com[pa], rb = p.c[pb] - r->com[pb];
-      d.c[pa] += pw * rb;
-      d.c[pb] -= pw * ra;
-    }
-    if (first || prevRot)
-      st_pt<4>(disp, v, d);
+    apply_planes(d, p, r, plane);
     const double ra = p.c[a] - r->com[a], rb = p.c[b] - r->com[b];
     acc[0] += ra * d.c[b] - rb * d.c[a];
     acc[1] += ra * ra + rb * rb;
@@ -1146,9 +1150,9 @@ __global__ void k_plane_omega(const double *__restrict__ partial, int blocks,
   }
 }
 
-// The last rotation fix-up (the solved 3-D rotation, :1246-1260, or the
-// last 4-D plane), the position update and the round statistics
-// (:1299-1318).
+// The last rotation fix-up (the solved 3-D rotation, :1246-1260, or for
+// 4-D the translation and all six planes on the refinement's output), the
+// position update and the round statistics (:1299-1318).
 template <int D>
 __global__ __launch_bounds__(kB) void k_round_apply_nd(
     const int32_t *__restrict__ order, int64_t active, double *__restrict__ pos,
@@ -1157,26 +1161,24 @@ __global__ __launch_bounds__(kB) void k_round_apply_nd(
     double *__restrict__ partial) {
   __shared__ double sm[4 * 16];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const bool rot = D == 3 ? r->rotate[0] != 0 : r->rotate[5] != 0;
+  const bool rot = r->rotate[0] != 0;
   for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
        i += (int64_t)gridDim.x * kB) {
     const int32_t v = order[i];
     PtN<D> d = ld_pt<D>(disp, v);
     PtN<D> p = ld_pt<D>(pos, v);
-    if (rot) {
-      if constexpr (D == 3) {
+    if constexpr (D == 3) {
+      if (rot) {
         const double w0 = r->omega[0], w1 = r->omega[1], w2 = r->omega[2];
         const double r0 = p.c[0] - r->com[0], r1 = p.c[1] - r->com[1],
                      r2 = p.c[2] - r->com[2];
         d.c[0] -= w1 * r2 - w2 * r1;
         d.c[1] -= w2 * r0 - w0 * r2;
         d.c[2] -= w0 * r1 - w1 * r0;
-      } else {
-        const double w = r->omega[5];
-        const double ra = p.c[2] - r->com[2], rb = p.c[3] - r->com[3];
-        d.c[2] += w * rb;
-        d.c[3] -= w * ra;
+        st_pt<D>(disp, v, d);
       }
+    } else {
+      apply_planes(d, p, r, 6); // translation and all six planes
       st_pt<D>(disp, v, d);
     }
     const double nrm = sqrt(sq_sum<D>(d.c));
