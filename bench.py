@@ -492,25 +492,6 @@ def warm_start_line(graphs=8, n=200_000):
             "reached": f"{len(ok)}/{len(rows)}", "per_graph": rows}
 
 
-def batch_graphs(count=4096, seed=0, n_min=50, n_max=20000):
-    """Random trees, 2-D grids and Erdos-Renyi graphs (average degree 4),
-    vertex counts log-uniform in [n_min, n_max], seeded."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), count)).astype(np.int64)
-    kinds = rng.integers(0, 3, count)
-    out = []
-    for i, (size, kind) in enumerate(zip(sizes.tolist(), kinds.tolist())):
-        if kind == 0:
-            n, e = G.random_tree(size, seed * 10007 + i)
-        elif kind == 1:
-            w = max(1, int(round(np.sqrt(size))))
-            n, e = G.grid(w, max(1, int(round(size / w))))
-        else:
-            n, e = G.erdos_renyi_fast(size, 4.0, seed * 10007 + i)
-        out.append(G.csr_from_edges(n, e))
-    return out, kinds
-
-
 def bench_batch(args):
     """4,096 graphs, FR Barnes-Hut (theta 1.0, the reference's default) in
     FP32, laid out together: every graph runs until its maxDisp < 1.0 (a
@@ -518,7 +499,7 @@ def bench_batch(args):
     = time from the start of the run to the check that saw it stop."""
     import struct
     t0 = time.time()
-    graphs, kinds = batch_graphs()
+    graphs, kinds = G.batch_mix()
     sizes = np.array([len(o) - 1 for o, _ in graphs])
     print(f"# batch of {len(graphs)} graphs, {sizes.sum()} vertices built in "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)

@@ -1768,6 +1768,7 @@ int gvizhip_grip_set_filtration(gvizhip_grip *g, const int64_t *filtration,
   std::vector<int32_t> f32(count > 0 ? count : 1);
   for (int64_t i = 0; i < count; i++)
     f32[i] = (int32_t)filtration[i];
+  GVH_CHECK(hipStreamSynchronize(g->stream));
   if (count > 0)
     GVH_CHECK(hipMemcpy(g->filt.p, f32.data(), sizeof(int32_t) * count,
                         hipMemcpyHostToDevice));
@@ -2066,6 +2067,9 @@ int gvizhip_grip_upload_nd(gvizhip_grip *g, const double *positions,
     set_error("gvizhip_grip_upload_nd: null context");
     return -1;
   }
+  // The copies below run on the null stream, which does not order against
+  // the engine's non-blocking stream: let queued kernels finish first.
+  GVH_CHECK(hipStreamSynchronize(g->stream));
   const int64_t n = g->n, D = g->dim;
   if (D == 2) {
     std::vector<gvizhip_grip_vertex> d2(dec ? n : 0);
@@ -2172,6 +2176,8 @@ int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
     set_error("gvizhip_grip_upload: null context");
     return -1;
   }
+  // Null-stream copies: wait for kernels queued on the engine's stream.
+  GVH_CHECK(hipStreamSynchronize(g->stream));
   const int64_t n = g->n;
   if (g->dim != 2 && (positions || dec)) {
     if (dec) {

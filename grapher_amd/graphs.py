@@ -240,3 +240,23 @@ def largest_component(n: int, edges: np.ndarray):
     active = (labels == big).astype(np.uint8)
     keep = active[edges[:, 0]].astype(bool) & active[edges[:, 1]].astype(bool)
     return active, edges[keep]
+
+
+def batch_mix(count=4096, seed=0, n_min=50, n_max=20000):
+    """The batched-layout workload: random trees, 2-D grids and Erdos-Renyi
+    graphs (average degree 4), vertex counts log-uniform in [n_min, n_max],
+    seeded. Returns ([(offsets, adj)], kinds) with kind 0 tree, 1 grid, 2 ER."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), count)).astype(np.int64)
+    kinds = rng.integers(0, 3, count)
+    out = []
+    for i, (size, kind) in enumerate(zip(sizes.tolist(), kinds.tolist())):
+        if kind == 0:
+            n, e = random_tree(size, seed * 10007 + i)
+        elif kind == 1:
+            w = max(1, int(round(np.sqrt(size))))
+            n, e = grid(w, max(1, int(round(size / w))))
+        else:
+            n, e = erdos_renyi_fast(size, 4.0, seed * 10007 + i)
+        out.append(csr_from_edges(n, e))
+    return out, kinds

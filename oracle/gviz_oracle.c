@@ -456,6 +456,7 @@ typedef struct force_ctx {
   const oc_quadtree *tree;
   double *disp, *structForce, *attMag, *repMag;
   int64_t *interactions;
+  double *termSum; /* optional: per vertex, sum of |term| over every term */
 } force_ctx;
 
 static inline double radius_if(const force_ctx *c, int64_t idx) {
@@ -464,10 +465,19 @@ static inline double radius_if(const force_ctx *c, int64_t idx) {
   return oc_vertex_radius(c->p, c->degree[idx]);
 }
 
+/* Adds one pair term computed into a zeroed temporary t. The sum rounds
+ * exactly as accumulating in place (0 + x and a + (-b) are exact rewrites of
+ * the in-place updates), and |t| goes to *termAcc. */
+static inline void add_term(double *acc, const double *t, double *termAcc) {
+  acc[0] += t[0];
+  acc[1] += t[1];
+  *termAcc += sqrt(t[0] * t[0] + t[1] * t[1]);
+}
+
 /* src/embedders/gvizForceEmbedder.c:35-74 (recursion made explicit; the
  * stack pops children in NW, NE, SW, SE order, as the recursion does). */
 static int64_t bh_accumulate(const force_ctx *c, int64_t self, double vRadius,
-                             const double *vPos, double *acc) {
+                             const double *vPos, double *acc, double *termAcc) {
   const oc_quadtree *t = c->tree;
   if (t->root < 0)
     return 0;
@@ -485,10 +495,14 @@ static int64_t bh_accumulate(const force_ctx *c, int64_t self, double vRadius,
         int64_t idx = t->leafPts[nd->ptStart + k];
         if (idx == self)
           continue;
+        double t[2] = {0.0, 0.0};
+        double *dst = termAcc ? t : acc;
         model_repulsive(c->p->model, vPos, c->pos + 2 * idx, c->mass[self],
                         c->mass[idx], vRadius, radius_if(c, idx),
                         c->p->overlapConstant, c->p->edgeLength, self, idx,
-                        acc);
+                        dst);
+        if (termAcc)
+          add_term(acc, t, termAcc);
         count++;
       }
       continue;
@@ -499,9 +513,13 @@ static int64_t bh_accumulate(const force_ctx *c, int64_t self, double vRadius,
     double ratio = (2.0 * nd->halfSize) / dist;
     if (ratio < c->p->theta) {
       double com[2] = {nd->comX, nd->comY};
+      double t[2] = {0.0, 0.0};
+      double *dst = termAcc ? t : acc;
       model_repulsive(c->p->model, vPos, com, c->mass[self], nd->mass, vRadius,
                       0.0, c->p->overlapConstant, c->p->edgeLength, self,
-                      c->g->n, acc);
+                      c->g->n, dst);
+      if (termAcc)
+        add_term(acc, t, termAcc);
       count++;
       continue;
     }
@@ -520,21 +538,33 @@ static void force_range(void *vctx, int64_t begin, int64_t end) {
     double attF[2] = {0.0, 0.0};
     double repF[2] = {0.0, 0.0};
     const double *vPos = c->pos + 2 * i;
+    double termSum = 0.0;
+    double *termAcc = c->termSum ? &termSum : NULL;
     for (int64_t e = c->g->offsets[i]; e < c->g->offsets[i + 1]; e++) {
       int64_t u = c->g->adj[e];
-      model_attractive(p->model, vPos, c->pos + 2 * u, p->edgeLength, attF);
+      if (termAcc) {
+        double t[2] = {0.0, 0.0};
+        model_attractive(p->model, vPos, c->pos + 2 * u, p->edgeLength, t);
+        add_term(attF, t, termAcc);
+      } else {
+        model_attractive(p->model, vPos, c->pos + 2 * u, p->edgeLength, attF);
+      }
     }
     double vRadius = radius_if(c, i);
     int64_t count = 0;
     if (p->barnesHut) {
-      count = bh_accumulate(c, i, vRadius, vPos, repF);
+      count = bh_accumulate(c, i, vRadius, vPos, repF, termAcc);
     } else {
       for (int64_t j = 0; j < c->g->n; j++) {
         if (j == i)
           continue;
+        double t[2] = {0.0, 0.0};
+        double *dst = termAcc ? t : repF;
         model_repulsive(p->model, vPos, c->pos + 2 * j, c->mass[i], c->mass[j],
                         vRadius, radius_if(c, j), p->overlapConstant,
-                        p->edgeLength, i, j, repF);
+                        p->edgeLength, i, j, dst);
+        if (termAcc)
+          add_term(repF, t, termAcc);
       }
       count = c->g->n - 1;
     }
@@ -551,7 +581,14 @@ static void force_range(void *vctx, int64_t begin, int64_t end) {
       c->structForce[2 * i + 1] = f[1];
     }
     double gravityMag = p->gravityK * (double)(c->degree[i] + 1);
-    oc_gravity(vPos, gravityMag, f);
+    if (termAcc) {
+      double t[2] = {0.0, 0.0};
+      oc_gravity(vPos, gravityMag, t);
+      add_term(f, t, termAcc);
+      c->termSum[i] = termSum;
+    } else {
+      oc_gravity(vPos, gravityMag, f);
+    }
     if (c->disp) {
       c->disp[2 * i] = f[0];
       c->disp[2 * i + 1] = f[1];
@@ -572,9 +609,22 @@ void oc_force_compute(const oc_csr *g, const double *pos, const double *mass,
     tree = own;
   }
   force_ctx c = {g,    pos,         mass,   degree, p,      tree,
-                 disp, structForce, attMag, repMag, interactions};
+                 disp, structForce, attMag, repMag, interactions, NULL};
   oc_pool_for_range(pool, 0, g->n, 1, force_range, &c);
   oc_quadtree_free(own);
+}
+
+void oc_force_term_sums(const oc_csr *g, const double *pos, const double *mass,
+                        const int64_t *degree, const oc_force_params *p,
+                        oc_pool *pool, double *disp, int64_t *interactions,
+                        double *termSum) {
+  oc_quadtree *tree = NULL;
+  if (p->barnesHut)
+    tree = oc_quadtree_build(pos, mass, g->n, p->nodesPerCell);
+  force_ctx c = {g,    pos,  mass, degree, p,            tree,
+                 disp, NULL, NULL, NULL,   interactions, termSum};
+  oc_pool_for_range(pool, 0, g->n, 1, force_range, &c);
+  oc_quadtree_free(tree);
 }
 
 /* src/embedders/gvizForceEmbedder.c:140-185 */

@@ -123,6 +123,16 @@ void oc_force_compute(const oc_csr *g, const double *pos, const double *mass,
                       double *structForce, double *attMag, double *repMag,
                       int64_t *interactions);
 
+/* The same evaluation, also writing termSum[i] = sum over every term that
+ * makes up vertex i's force (each attraction entry, each repulsion leaf point
+ * and pseudo-body, gravity) of that term's magnitude: the scale of the
+ * per-vertex FP32 error bound |dF_i| <= tol * sum_j |f_ij|. disp and
+ * interactions (optional) equal oc_force_compute's bit for bit. */
+void oc_force_term_sums(const oc_csr *g, const double *pos, const double *mass,
+                        const int64_t *degree, const oc_force_params *p,
+                        oc_pool *pool, double *disp, int64_t *interactions,
+                        double *termSum);
+
 /* Full-step state (the per-embedder fields Step reads and writes). */
 typedef struct oc_force_state {
   int64_t n;

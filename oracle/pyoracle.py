@@ -111,6 +111,9 @@ def lib():
             C.POINTER(CSR), _f64p, _f64p, _i64p, C.POINTER(ForceParams),
             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
             C.c_void_p, C.c_void_p]
+        L.oc_force_term_sums.argtypes = [
+            C.POINTER(CSR), _f64p, _f64p, _i64p, C.POINTER(ForceParams),
+            C.c_void_p, C.c_void_p, C.c_void_p, _f64p]
         L.oc_force_state_create.restype = C.c_void_p
         L.oc_force_state_create.argtypes = [C.POINTER(CSR),
                                             C.POINTER(ForceParams)]
@@ -153,8 +156,89 @@ def lib():
             C.POINTER(CSR), _i64p, C.c_int64, C.c_int64, _i64p, C.c_int64, C.c_int64,
             C.c_void_p, _f64p, C.POINTER(GripRoundStats)]
         L.oc_regular_simplex.argtypes = [C.c_int64, C.c_double, _f64p]
+        _d = C.c_double
+        _dp = C.POINTER(C.c_double)
+        L.oc_fr_att.argtypes = [_f64p, _f64p, _d, _f64p]
+        L.oc_linlog_att.argtypes = [_f64p, _f64p, _f64p]
+        L.oc_fr_rep.argtypes = [_f64p, _f64p, _d, _d, _d, C.c_int64, C.c_int64,
+                                _f64p]
+        L.oc_linlog_rep.argtypes = [_f64p, _f64p, _d, _d, _d, _d, C.c_int64,
+                                    C.c_int64, _f64p]
+        L.oc_gravity.argtypes = [_f64p, _d, _f64p]
+        L.oc_kk.argtypes = [_f64p, _f64p, _d, _f64p]
+        L.oc_grip_fr_rep.argtypes = [_f64p, _f64p, _d, _f64p]
+        L.oc_grip_fr_att.argtypes = [_f64p, _f64p, _d, _d, _f64p]
+        L.oc_coincident_dir.argtypes = [C.c_int64, C.c_int64, _dp, _dp]
         _lib = L
     return _lib
+
+
+def _v2(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(2)
+
+
+def _acc(acc):
+    return np.zeros(2) if acc is None else np.ascontiguousarray(acc, dtype=np.float64).copy()
+
+
+# Pairwise force terms (include/core/gvizVec.h, n = 2). Each adds into a copy
+# of `acc` (zero by default) and returns it, as the reference's gvizVecAcc*
+# functions accumulate into their last argument.
+def fr_att(v, u, k, acc=None):
+    a = _acc(acc)
+    lib().oc_fr_att(_v2(v), _v2(u), k, a)
+    return a
+
+
+def linlog_att(v, u, acc=None):
+    a = _acc(acc)
+    lib().oc_linlog_att(_v2(v), _v2(u), a)
+    return a
+
+
+def fr_rep(v, u, k, radius_sum, overlap_constant, self_id=0, other_id=1, acc=None):
+    a = _acc(acc)
+    lib().oc_fr_rep(_v2(v), _v2(u), k, radius_sum, overlap_constant, self_id,
+                    other_id, a)
+    return a
+
+
+def linlog_rep(v, u, v_mass, other_mass, radius_sum, overlap_constant,
+               self_id=0, other_id=1, acc=None):
+    a = _acc(acc)
+    lib().oc_linlog_rep(_v2(v), _v2(u), v_mass, other_mass, radius_sum,
+                        overlap_constant, self_id, other_id, a)
+    return a
+
+
+def gravity(v, magnitude, acc=None):
+    a = _acc(acc)
+    lib().oc_gravity(_v2(v), magnitude, a)
+    return a
+
+
+def kk(v, u, length, acc=None):
+    a = _acc(acc)
+    lib().oc_kk(_v2(v), _v2(u), length, a)
+    return a
+
+
+def grip_fr_rep(v, u, edge_length, acc=None):
+    a = _acc(acc)
+    lib().oc_grip_fr_rep(_v2(v), _v2(u), edge_length, a)
+    return a
+
+
+def grip_fr_att(v, u, edge_length, fr_scale, acc=None):
+    a = _acc(acc)
+    lib().oc_grip_fr_att(_v2(v), _v2(u), edge_length, fr_scale, a)
+    return a
+
+
+def coincident_dir(self_id, other_id):
+    ux, uy = C.c_double(), C.c_double()
+    lib().oc_coincident_dir(self_id, other_id, C.byref(ux), C.byref(uy))
+    return ux.value, uy.value
 
 
 def _ptr(a):
@@ -220,6 +304,23 @@ def force_compute(g: Graph, pos, p: ForceParams, pool: Pool | None = None):
                            _ptr(inter))
     return {"disp": disp, "structForce": sf, "attMag": am, "repMag": rm,
             "interactions": inter}
+
+
+def force_term_sums(g: Graph, pos, p: ForceParams, pool: Pool | None = None,
+                    interactions=False):
+    """(disp (n, 2), termSum (n,)[, interactions (n,)]): the evaluation of
+    force_compute and, per vertex, the sum of the magnitudes of every term of
+    its force."""
+    n = g.n
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 2)
+    mass = vertex_mass(p.model, g.degree)
+    deg = np.ascontiguousarray(g.degree, dtype=np.int64)
+    disp = np.zeros((n, 2))
+    ts = np.zeros(n)
+    inter = np.zeros(n, dtype=np.int64) if interactions else None
+    lib().oc_force_term_sums(C.byref(g.c), pos.reshape(-1), mass, deg, C.byref(p),
+                             pool.h if pool else None, _ptr(disp), _ptr(inter), ts)
+    return (disp, ts, inter) if interactions else (disp, ts)
 
 
 class ForceRun:
