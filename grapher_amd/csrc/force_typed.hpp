@@ -59,6 +59,14 @@ __device__ __forceinline__ float2 coincident_dir_f(int64_t selfId, int64_t other
 // (include/core/gvizVec.h:552-575).
 template <class Real> struct Phys;
 
+// FP32 force math over FP64 positions: the FP32 gap d - (r_v + r_u) carries
+// an absolute error of about 4e-7 d (the FP32 distance and radius sum each
+// round a few times), and the magnitude m / gap amplifies it by d / gap.
+// Gaps closer to zero than kNearGap * d are recomputed from the FP64
+// difference and FP64 radii, which bounds a term's relative error by about
+// 4e-7 / kNearGap = 4e-6.
+constexpr float kNearGap = 0.1f;
+
 template <> struct Phys<double> {
   // FP64: the reference's operation order; sqrt and division are correctly
   // rounded, and this TU is compiled without FMA contraction.
@@ -158,7 +166,7 @@ template <> struct Phys<float> {
                                              const double *__restrict__ r64,
                                              int64_t n) {
     float gap = dist - radiusSum;
-    if (NEAR && radiusSum > 0.0f && fabsf(gap) < 1e-3f * dist) {
+    if (NEAR && radiusSum > 0.0f && fabsf(gap) < kNearGap * dist) {
       double rs = r64[selfId] + (otherId < n ? r64[otherId] : 0.0);
       gap = (float)(sqrt(dxd * dxd + dyd * dyd) - rs);
     }
@@ -425,7 +433,7 @@ template <class Real, class PosT> struct alignas(16) TravBody {
 // A pair with d2 == 0 makes the term non-finite; rep_pair_guarded gives it
 // the reference's treatment (coincident direction, magnitude num * oc).
 // Without overlap prevention every radius is 0, the gap is the distance and
-// the term is num * d / |d|^2. With it, a gap below 1e-3 of the distance is
+// the term is num * d / |d|^2. With it, a gap below kNearGap of the distance is
 // recomputed from the FP64 difference and radii when positions are FP64
 // (a zero radius sum never meets that test).
 template <bool OVERLAP, bool NEAR>
@@ -441,7 +449,7 @@ __device__ __forceinline__ void rep_pair(float dx, float dy, float d2, float num
     const float inv = __builtin_amdgcn_rsqf(d2);
     const float dist = d2 * inv;
     float gap = dist - radiusSum;
-    if (NEAR && fabsf(gap) < 1e-3f * dist) {
+    if (NEAR && fabsf(gap) < kNearGap * dist) {
       const double rs = r64[selfId] + (otherId < n ? r64[otherId] : 0.0);
       gap = (float)(sqrt(dxd * dxd + dyd * dyd) - rs);
     }
@@ -498,8 +506,8 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     const NodeRec<Real, PosT> *__restrict__ nodes,
     const typename V2<PosT>::type *__restrict__ spos,
     const typename V2<Real>::type *__restrict__ smr,
-    const uint32_t *__restrict__ sidx, int64_t n, Real theta, Real kSq, Real oc,
-    typename V2<Real>::type *__restrict__ rep, int32_t *__restrict__ inter,
+    const uint32_t *__restrict__ sidx, int64_t n, Real theta, double theta64, Real kSq,
+    Real oc, typename V2<Real>::type *__restrict__ rep, int32_t *__restrict__ inter,
     const double *__restrict__ r64, Scalars *__restrict__ sc,
     unsigned long long *__restrict__ tstats, SegView sv) {
   constexpr bool NEAR = sizeof(PosT) == sizeof(double);
@@ -806,7 +814,22 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
             // is opened, as the reference opens it at distance 0
             const Real d2 = dx * dx + dy * dy;
             const Real dist = d2 * __builtin_amdgcn_rsqf(d2);
-            if (Phys<Real>::accept(b.r, dist, theta)) {
+            bool accept = Phys<Real>::accept(b.r, dist, theta);
+            if constexpr (NEAR) {
+              // Within the FP32 test's rounding of the threshold (2h and the
+              // distance each carry a few ulps) the reference's FP64 test
+              // decides, so the interaction set stays the reference's.
+              // 2h = RN32(rootHalf) 2^(1-q) exactly, which gives back the
+              // FP64 2h = rootHalf 2^(1-q) through an exact power-of-two ratio.
+              const float rhs = theta * dist;
+              if (fabsf(b.r - rhs) <= 4e-6f * rhs) {
+                const double rh = sc->rootHalf;
+                const double twoh64 =
+                    b.r >= 1.2e-38f ? rh * ((double)b.r / (double)(float)rh) : (double)b.r;
+                accept = (twoh64 / sqrt(ex * ex + ey * ey)) < theta64;
+              }
+            }
+            if (accept) {
               rep_pair<OVERLAP, NEAR>(dx, dy, d2, b.m, mr.y, oc, ex, ey, r64, selfId, n, n,
                                       ax, ay);
               cnt++;
@@ -1217,7 +1240,8 @@ int launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int3
 #define GVH_TRAVERSE(M, OV)                                                              \
   hipLaunchKernelGGL((k_traverse<Real, PosT, M, OV>), dim3(tb), dim3(kTravThreads), 0, st, \
                      (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,        \
-                     (const R2 *)c.smr.p, c.vals.p, c.n, theta, kSq, oc, (R2 *)c.rep.p,   \
+                     (const R2 *)c.smr.p, c.vals.p, c.n, theta, (double)c.prm.theta, kSq,  \
+                     oc, (R2 *)c.rep.p,                                                   \
                      inter, c.radius64.p, sc, tstats, sv)
   if (model == 1) {
     if (ov)

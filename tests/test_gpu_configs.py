@@ -120,8 +120,11 @@ def test_ba1m_linlog_fp64_and_mixed_match_oracle(gpu, ba1m, pool):
         assert np.array_equal(got["interactions"], inter)
         assert norm <= 1e-7 and per <= 1e-7
         mixed.set_positions(pos)
-        norm, per = _force_errors(mixed.evaluate()["disp"], ref, ts)
-        log(f"BA1M {label} mixed: normwise {norm:.3g}, per vertex {per:.3g}")
+        got = mixed.evaluate(interactions=True)
+        norm, per = _force_errors(got["disp"], ref, ts)
+        log(f"BA1M {label} mixed: normwise {norm:.3g}, per vertex {per:.3g}, "
+            f"{int((got['interactions'] != inter).sum())} interaction counts differ")
+        assert np.array_equal(got["interactions"], inter)
         assert norm <= 1e-5 and per <= 1e-5
         ref_f, ts_f, inter_f = O.force_term_sums(g, pos, p_free, pool, interactions=True)
         f64_free.set_positions(pos)
