@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace gvh;
@@ -209,6 +210,40 @@ __global__ void k_begin(TutteScalars *sc, double epsilon) {
   sc->recordBase = sc->sweeps;
   sc->epsilon = epsilon;
   sc->stopped = 0;
+}
+
+// ---------------------------------------------------------------- no-alloc
+// Scratch of gvizhip_tutte_sweeps_noalloc: the step scalars in the first
+// 64 bytes, then one partial maximum per block of the sweep's grid. Each GPU
+// holds its own copy; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocBlocks = 2048;
+constexpr int kNoAllocHeader = 64 / sizeof(double);
+static_assert(sizeof(TutteScalars) <= 64, "step scalars exceed the scratch header");
+static __device__ __align__(256) double scratch[kNoAllocHeader + kNoAllocBlocks];
+
+__global__ void k_noalloc_begin(TutteScalars *sc, double epsilon) {
+  sc->sweeps = 0;
+  sc->recordBase = 0;
+  sc->epsilon = epsilon;
+  sc->stopped = 0;
+  sc->pad = 0;
+}
+
+// After the sweeps: an odd count left the latest positions in b; copy them
+// to a so the result is always in a. Reports the count.
+__global__ __launch_bounds__(kB) void k_noalloc_end(const TutteScalars *__restrict__ sc,
+                                                    double2 *__restrict__ a,
+                                                    const double2 *__restrict__ b,
+                                                    int64_t n, int64_t *sweepsRun) {
+  const int64_t done = sc->sweeps;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && sweepsRun)
+    *sweepsRun = done;
+  if ((done & 1) == 0)
+    return;
+  for (int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * kB)
+    a[u] = b[u];
 }
 
 } // namespace
@@ -694,6 +729,68 @@ int gvizhip_tutte_profile(gvizhip_tutte *t, int64_t sweeps, double alpha,
     *sweepMs = sum;
   if (totalMs)
     *totalMs = total;
+  return 0;
+}
+
+// Device address of this file's scratch array on the current device, looked
+// up once per device (the array's name is a host-side stand-in, not an
+// address a kernel may use).
+static int noalloc_scratch(double **out) {
+  static std::mutex mu;
+  static double *addr[64] = {};
+  int dev = -1;
+  GVH_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("gvizhip_tutte_sweeps_noalloc: device %d out of range", dev);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    GVH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    addr[dev] = (double *)p;
+  }
+  *out = addr[dev];
+  return 0;
+}
+
+int gvizhip_tutte_sweeps_noalloc(int64_t n, const int32_t *offsets, const int32_t *adj,
+                                 const uint8_t *isBoundary, double *posA, double *posB,
+                                 int64_t iters, double alpha, double epsilon,
+                                 double *maxDeltas, int64_t *sweepsRun, void *stream) {
+  if (n < 0 || iters < 0 || !(alpha > 0.0) || alpha > 1.0 ||
+      (n > 0 && (!offsets || !adj || !isBoundary || !posA || !posB)) ||
+      (iters > 0 && !maxDeltas)) {
+    set_error("gvizhip_tutte_sweeps_noalloc: invalid arguments");
+    return -1;
+  }
+  if (n > (int64_t)INT32_MAX - 1) {
+    set_error("gvizhip_tutte_sweeps_noalloc: %lld vertices exceed the 32-bit CSR "
+              "indices of the sweep (at most %d)", (long long)n, INT32_MAX - 1);
+    return -1;
+  }
+  double *base = nullptr;
+  if (noalloc_scratch(&base))
+    return -1;
+  TutteScalars *sc = (TutteScalars *)base;
+  double *partial = base + kNoAllocHeader;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned g =
+      (unsigned)std::min<int64_t>(grid_for(n > 0 ? n : 1, kB), kNoAllocBlocks);
+  hipLaunchKernelGGL(k_noalloc_begin, dim3(1), dim3(1), 0, st, sc, epsilon);
+  GVH_CHECK_LAUNCH();
+  for (int64_t k = 0; k < iters; k++) {
+    const double2 *in = (const double2 *)((k & 1) ? posB : posA);
+    double2 *out = (double2 *)((k & 1) ? posA : posB);
+    hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, st, offsets, adj,
+                       isBoundary, in, out, n, alpha, partial, sc);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, partial, (int)g, sc,
+                       maxDeltas, iters);
+    GVH_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_noalloc_end, dim3(g), dim3(kB), 0, st, sc, (double2 *)posA,
+                     (const double2 *)posB, n, sweepsRun);
+  GVH_CHECK_LAUNCH();
   return 0;
 }
 

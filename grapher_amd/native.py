@@ -120,6 +120,8 @@ def lib():
         "gvizhip_tutte_get_velocity": (C.c_int, [_vp, _vp]),
         "gvizhip_tutte_profile": (C.c_int, [_vp, _i64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "gvizhip_tutte_destroy": (None, [_vp]),
+        "gvizhip_tutte_sweeps_noalloc": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i64,
+                                                   C.c_double, C.c_double, _vp, _vp, _vp]),
         "gvizhip_grip_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
         "gvizhip_grip_mis_filtration": (_i64, [_vp, _i64, _vp, _vp]),
         "gvizhip_grip_set_filtration": (C.c_int, [_vp, _vp, _i64]),
@@ -184,6 +186,7 @@ DECLARED = [
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
     "gvizhip_tutte_sweeps", "gvizhip_tutte_profile", "gvizhip_tutte_destroy",
     "gvizhip_tutte_spring_steps", "gvizhip_tutte_set_velocity", "gvizhip_tutte_get_velocity",
+    "gvizhip_tutte_sweeps_noalloc",
     "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
     "gvizhip_grip_set_filtration", "gvizhip_grip_knn", "gvizhip_grip_knn_batch",
     "gvizhip_grip_upload", "gvizhip_grip_download", "gvizhip_grip_set_dimension",

@@ -177,6 +177,25 @@ int gvizhip_tutte_get_velocity(gvizhip_tutte *t, double *velocity);
 int gvizhip_tutte_profile(gvizhip_tutte *t, int64_t sweeps, double alpha,
                           double *sweepMs, double *totalMs);
 void gvizhip_tutte_destroy(gvizhip_tutte *t);
+/* No counterpart in the reference: an entry point that allocates nothing,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). The FP64 Jacobi
+ * sweeps of gvizhip_tutte_sweeps on caller-owned device memory: offsets[n+1]
+ * and adj (int32 CSR in insertion order), isBoundary[n] bytes, positions
+ * (x, y interleaved) ping-ponged between posA and posB, every call starting
+ * from posA; the result is always left in posA. Stops after the first sweep
+ * whose max displacement is < epsilon when epsilon >= 0. maxDeltas (device,
+ * iters doubles) receives each sweep's max displacement (entries past the
+ * last sweep run are untouched); sweepsRun (device int64, may be NULL) the
+ * count. Everything is enqueued on `stream` (a hipStream_t, NULL: the null
+ * stream) and nothing waits for it. Per-block maxima and the step scalars
+ * live in a 16 KiB static device array shared by every call on a GPU: calls
+ * that overlap on one GPU race. n is limited to 2^31 - 2 (32-bit indices).
+ * Results equal gvizhip_tutte_sweeps's bit for bit. 0 or -1. */
+int gvizhip_tutte_sweeps_noalloc(int64_t n, const int32_t *offsets, const int32_t *adj,
+                                 const uint8_t *isBoundary, double *posA, double *posB,
+                                 int64_t iters, double alpha, double epsilon,
+                                 double *maxDeltas, int64_t *sweepsRun, void *stream);
 
 /* ------------------------------------------------------------------- GRIP */
 typedef struct gvizhip_grip gvizhip_grip;
