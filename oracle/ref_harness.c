@@ -515,6 +515,46 @@ static int mode_api_grip(void) {
   return 0;
 }
 
+/* The reference's Embed schedule (gvizGRIPEmbedder.c:1262-1285: rounds of a
+ * layer until maxDisplacement < 10 * 1e-3 or 30 rounds, then NextStage) from
+ * Begin's placement with every coordinate scaled by 1 + perturb * u: the
+ * spread of the final layouts under rounding-size start differences.
+ * Writes the final positions. */
+static int mode_api_grip_spread(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t kcap = rd_i(), pk = rd_i(), rk = rd_i(), policy = rd_i(), dim = rd_i();
+  double perturb = rd_d();
+  int64_t perturbSeed = rd_i();
+  gvizGRIPState st;
+  memset(&st, 0, sizeof(st));
+  gvizGRIPEmbedderConfigureKnnCapacity(&st, (size_t)kcap);
+  if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, (size_t)dim) != 0)
+    return 1;
+  serial_grip(&st);
+  gvizGRIPEmbedderConfigureK(&st, (size_t)pk, (size_t)rk, (gvizGRIPKPolicy)policy);
+  gvizGRIPEmbedderBegin(&st);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
+  perturb_positions(e->embedding.vertexPositions, dim * n, perturb, perturbSeed);
+  const double epsilon = 10.0 * 1e-3;
+  for (;;) {
+    for (int round = 0; round < 30; round++) {
+      gvizGRIPEmbedderRefineRound(&st);
+      if (st.lastRoundStats.maxDisplacement < epsilon)
+        break;
+    }
+    if (st.currLayer == 0)
+      break;
+    gvizGRIPEmbedderNextStage(&st);
+  }
+  const double *p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < dim * n; i++)
+    wr_d(p[i]);
+  gvizGRIPEmbedderRelease(&st);
+  return 0;
+}
+
 static double now_s(void) {
   struct timespec t;
   clock_gettime(CLOCK_MONOTONIC, &t);
@@ -745,6 +785,8 @@ int main(int argc, char **argv) {
     rc = mode_api_force();
   else if (!strcmp(argv[1], "api_grip"))
     rc = mode_api_grip();
+  else if (!strcmp(argv[1], "api_grip_spread"))
+    rc = mode_api_grip_spread();
   else if (!strcmp(argv[1], "time_grip"))
     rc = mode_time_grip();
   else if (!strcmp(argv[1], "time_tutte"))

@@ -233,14 +233,32 @@ def api_cases():
                    edge_length=10.0, theta=1.0, gravity=0.0,
                    overlap_constant=100.0, radius_base=0.0,
                    radius_per_degree=0.0, jitter=1.0)
+
+
+def api_grip_case(name, n, e, kcap, placement_k, refinement_k, policy):
+    """gvizGRIPEmbedderEmbed, plus SPREAD_RUNS runs of the reference's Embed
+    schedule from Begin's placement scaled by 1 +- 1e-12: the stress / edge
+    energy spread of their final layouts measures how far rounding-size
+    differences carry through the heat feedback (the yardstick for any other
+    reduction order, the engine's included)."""
+    r = refio.api_grip(n, e, kcap, placement_k, refinement_k, policy)
+    st, en = [], []
+    for k in range(SPREAD_RUNS):
+        q = refio.api_grip_spread(n, e, kcap, placement_k, refinement_k, policy,
+                                  SPREAD_NOISE, k + 1)
+        a, b = layout_quality(n, e, np.zeros(0, np.int64), q, 10.0)
+        st.append(a)
+        en.append(b)
+    save(name, n=n, edges=e, kcap=kcap, placement_k=placement_k,
+         refinement_k=refinement_k, policy=policy, **r, spread_stress=np.array(st),
+         spread_energy=np.array(en), spread_noise=SPREAD_NOISE)
+
+
+def api_grip_cases():
     n, e, _ = G.random_geometric(400, 8.0, 5)
-    r = refio.api_grip(n, e, 12, 12, 12, 0)
-    save("api_grip_rgg400_constant", n=n, edges=e, kcap=12, placement_k=12,
-         refinement_k=12, policy=0, **r)
+    api_grip_case("api_grip_rgg400_constant", n, e, 12, 12, 12, 0)
     n, e = G.grid(16, 16)
-    r = refio.api_grip(n, e, 16, 16, 8, 1)
-    save("api_grip_grid16_decay", n=n, edges=e, kcap=16, placement_k=16,
-         refinement_k=8, policy=1, **r)
+    api_grip_case("api_grip_grid16_decay", n, e, 16, 16, 8, 1)
 
 
 def edges_text_cases():
@@ -334,7 +352,7 @@ def tutte_begin_cases():
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
              knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
              grip_nd=grip_nd_cases, tutte_begin=tutte_begin_cases,
-             api=api_cases, edges=edges_cases, spring_tutte=spring_tutte_cases)
+             api=api_cases, api_grip=api_grip_cases, edges=edges_cases, spring_tutte=spring_tutte_cases)
 
 if __name__ == "__main__":
     if not refio.harness_available():

@@ -262,6 +262,20 @@ def api_grip(n, edges, kcap, placement_k, refinement_k, policy, exe=None, dim=2)
     return out
 
 
+def api_grip_spread(n, edges, kcap, placement_k, refinement_k, policy, perturb,
+                    perturb_seed, exe=None, dim=2):
+    """Final positions of the reference's Embed schedule run from Begin's
+    placement with every coordinate scaled by 1 + perturb * u (u uniform in
+    [-1, 1), seeded)."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(kcap, placement_k, refinement_k, policy, dim)
+    w.d(perturb)
+    w.i(perturb_seed)
+    r = _R(_run("api_grip_spread", w.bytes(), exe))
+    return r.da(dim * n).reshape(n, dim)
+
+
 def edges_file(path, directed=0, skip_header=0, exe=None, csr=False):
     """The .edges loader on `path`: (rc, offsets, adj) of the adjacency lists
     in order (offsets/adj None when rc != 0). With csr=True (this

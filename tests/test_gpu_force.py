@@ -87,7 +87,8 @@ def test_fp64_trajectory_matches_reference(gpu, name):
 @pytest.mark.parametrize("precision", ["mixed", "fp32"])
 @pytest.mark.parametrize("name", FORCE_FIXTURES)
 def test_fp32_forces_within_1e5(gpu, name, precision):
-    """FP32 force math within 1e-5 normwise. With prevent-overlap on, the
+    """FP32 force math within 1e-5 normwise and per vertex (against the sum
+    of the magnitudes of the vertex's force terms). With prevent-overlap on, the
     repulsion magnitude m/gap is singular where two circles touch; the mixed
     mode recomputes near-contact gaps from FP64 positions and radii and keeps
     1e-5, while FP32 positions cannot resolve such gaps (bound 1e-3 there)."""
@@ -96,12 +97,25 @@ def test_fp32_forces_within_1e5(gpu, name, precision):
     tol = 1e-5
     if precision == "fp32" and bool(f["param_prevent_overlap"]):
         tol = 1e-3
+    off, adj = G.csr_from_edges(int(f["n"]), f["edges"])
+    g = O.Graph(off, adj)
+    p = O.params(model=int(f["param_model"]), barnes_hut=bool(f["param_barnes_hut"]),
+                 prevent_overlap=bool(f["param_prevent_overlap"]),
+                 npc=int(f["param_npc"]), theta=float(f["param_theta"]),
+                 gravity=float(f["param_gravity"]),
+                 radius_base=float(f["param_radius_base"]),
+                 radius_per_degree=float(f["param_radius_per_degree"]))
     pos = f["pos0"]
-    for it in f["dump_iters"][:3]:
+    for it in f["dump_iters"]:
         eng.set_positions(pos)
         got = eng.evaluate()
         d = got["disp"] - f[f"it{it}_disp"]
         assert np.linalg.norm(d) <= tol * np.linalg.norm(f[f"it{it}_disp"])
+        # per vertex: |dF_i| <= tol * sum_j |f_ij| over the terms of i's force
+        ref, ts = O.force_term_sums(g, pos, p)
+        assert np.array_equal(ref, f[f"it{it}_disp"])
+        per = np.linalg.norm(d, axis=1) / ts
+        assert per.max() <= tol, (it, per.max(), int(per.argmax()))
         pos = f[f"it{it}_pos_after"]
 
 
@@ -185,7 +199,8 @@ def test_run_stops_at_epsilon_and_step_matches_run(gpu):
 def test_large_ba_linlog_mixed_matches_fp64_engine(gpu):
     """1M-vertex Barabasi-Albert LinLog + gravity + overlap, theta 0.7: the
     mixed-precision forces against the FP64 engine on the same positions
-    (normwise 1e-5), and every quadtree leaf holds its own vertex."""
+    (normwise 1e-5), and every quadtree leaf holds its own vertex (uniform
+    positions have no duplicates, so no leaf at the depth limit holds two)."""
     n, e = G.barabasi_albert(1_000_000, 3, 0)
     off, adj = G.csr_from_edges(n, e)
     pos = G.uniform_positions(n, 0)
@@ -199,6 +214,9 @@ def test_large_ba_linlog_mixed_matches_fp64_engine(gpu):
     b = lo.evaluate()["disp"]
     assert np.all(np.isfinite(b))
     assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a)
+    depth, kh, kl, _ = lo.quadtree_leaves()
+    leaves = np.unique(np.stack([depth.astype(np.uint64), kh, kl], 1), axis=0)
+    assert len(leaves) == n  # one vertex per leaf: no two share a leaf
 
 
 @pytest.mark.parametrize("model,overlap,precision", [

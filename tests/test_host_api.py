@@ -135,7 +135,7 @@ def test_force_api_fp64_matches_reference(gpu, name):
     chaotic FA2 dynamics then amplify the difference: the run is checked for
     its length and the final layout's stress and edge energy against the
     band the reference itself spans when its start is scaled by 1 +- 1e-12
-    (eight such reference runs are stored in the fixture), widened by 2 %
+    (eight such reference runs are stored in the fixture), widened by 1 %
     (3 % or 2 steps for the run length)."""
     f = golden(name)
     print(f"host api force {name}", flush=True)
@@ -166,8 +166,8 @@ def test_force_api_fp64_matches_reference(gpu, name):
     print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g} (perturbed reference runs "
           f"{ss.min():.6g}..{ss.max():.6g}); energy {e_gpu:.6g} vs {e_ref:.6g} "
           f"({es.min():.6g}..{es.max():.6g})")
-    assert ss.min() - 0.02 * abs(s_ref) <= s_gpu <= ss.max() + 0.02 * abs(s_ref)
-    assert es.min() - 0.02 * abs(e_ref) <= e_gpu <= es.max() + 0.02 * abs(e_ref)
+    assert ss.min() - 0.01 * abs(s_ref) <= s_gpu <= ss.max() + 0.01 * abs(s_ref)
+    assert es.min() - 0.01 * abs(e_ref) <= e_gpu <= es.max() + 0.01 * abs(e_ref)
 
 
 @pytest.mark.gpu
@@ -209,7 +209,9 @@ def test_grip_embed_matches_reference(gpu, name):
     feedback (x1.45 per round while a vertex keeps its direction) amplifies
     those last-bit differences by about 2.5x per round: the series are
     compared to 1e-6 over the first 60 rounds and the final (unconverged)
-    layouts by stress."""
+    layouts by stress and edge energy: within 1 % of the reference, or
+    inside the reference's own spread under 1e-12 start perturbations
+    widened by 1 %."""
     f = golden(name)
     print(f"host api grip {name}", flush=True)
     r = refio.api_grip(int(f["n"]), f["edges"], int(f["kcap"]), int(f["placement_k"]),
@@ -233,8 +235,17 @@ def test_grip_embed_matches_reference(gpu, name):
     g = O.Graph(*G.csr_from_edges(int(f["n"]), f["edges"]))
     s_ref, e_ref = _layout_quality(g, f["pos"])
     s_gpu, e_gpu = _layout_quality(g, r["pos"])
-    print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g}; energy {e_gpu:.6g} vs {e_ref:.6g}")
-    assert abs(s_gpu - s_ref) <= 0.05 * abs(s_ref)
+    print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g} (perturbed reference runs "
+          f"{f['spread_stress'].min():.6g}..{f['spread_stress'].max():.6g}); energy "
+          f"{e_gpu:.6g} vs {e_ref:.6g} ({f['spread_energy'].min():.6g}.."
+          f"{f['spread_energy'].max():.6g})")
+    # within 1 % of the reference's run, or inside the spread of the
+    # reference's own runs from Begin's placement scaled by 1 +- 1e-12,
+    # widened by 1 %
+    for got, ref, spread in ((s_gpu, s_ref, f["spread_stress"]),
+                             (e_gpu, e_ref, f["spread_energy"])):
+        lo, hi = min(spread.min(), ref), max(spread.max(), ref)
+        assert abs(got - ref) <= 0.01 * abs(ref) or lo * 0.99 <= got <= hi * 1.01
 
 
 @pytest.mark.gpu
@@ -303,7 +314,7 @@ def test_grip_embed_matches_reference_nd(gpu, name):
     """gvizGRIPEmbedderEmbed in three and four dimensions end to end, by the
     bars of the 2-D run: exact filtration, draw mask and schedule, the
     per-round series to 1e-6 over the first 60 rounds, and the final
-    layout's stress within 5 %."""
+    layout's stress within 1 %."""
     f = golden(name)
     n, dim = int(f["n"]), int(f["dim"])
     print(f"host api grip {name}", flush=True)
@@ -322,7 +333,7 @@ def test_grip_embed_matches_reference_nd(gpu, name):
     s_ref = _stress_nd(n, f["edges"], f["pos"])
     s_gpu = _stress_nd(n, f["edges"], r["pos"])
     print(f"  final stress {s_gpu:.6g} vs {s_ref:.6g}")
-    assert abs(s_gpu - s_ref) <= 0.05 * abs(s_ref)
+    assert abs(s_gpu - s_ref) <= 0.01 * abs(s_ref)
 
 
 def _same_cycle_shift(a, b):
