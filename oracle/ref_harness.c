@@ -666,6 +666,78 @@ static int mode_edges_file(void) {
   return 0;
 }
 
+/* Text embedding I/O (gvizEmbeddedGraphSaveEmbedding / LoadEmbedding) on the
+ * full subgraph of a graph in `dim` dimensions: positions in, Save's return
+ * and the bytes it wrote, then for each file to load: positions reset to the
+ * input, Load's return and the positions after it. Input: graph, dim,
+ * positions, name, save path, load-path count, load paths (each as
+ * length + bytes). */
+static char *rd_str(void) {
+  const int64_t len = rd_i();
+  char *s = calloc((size_t)len + 1, 1);
+  if (len > 0 && fread(s, 1, (size_t)len, fin) != (size_t)len) {
+    free(s);
+    return NULL;
+  }
+  return s;
+}
+
+static int mode_embedding_io(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  const int64_t dim = rd_i();
+  double *pos = malloc(sizeof(double) * (size_t)(n * dim + 1));
+  for (int64_t i = 0; i < n * dim; i++)
+    pos[i] = rd_d();
+  char *name = rd_str(), *save = rd_str();
+  if (!name || !save)
+    return 2;
+  gvizEmbeddedGraph e;
+  if (gvizEmbeddedGraphInit(&e, gvizSubgraphCreateFull(&g), (size_t)dim) != 0)
+    return 1;
+  for (int64_t u = 0; u < n; u++)
+    gvizEmbeddedGraphSetVPosition(&e, (size_t)u, pos + dim * u);
+  wr_i(gvizEmbeddedGraphSaveEmbedding(&e, name, save));
+  FILE *f = fopen(save, "rb");
+  int64_t len = 0;
+  char *buf = NULL;
+  if (f) {
+    fseek(f, 0, SEEK_END);
+    len = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    buf = malloc((size_t)len + 1);
+    if (fread(buf, 1, (size_t)len, f) != (size_t)len)
+      len = 0;
+    fclose(f);
+  }
+  wr_i(len);
+  if (len > 0)
+    fwrite(buf, 1, (size_t)len, fout);
+  free(buf);
+  const int64_t loads = rd_i();
+  for (int64_t k = 0; k < loads; k++) {
+    char *path = rd_str();
+    if (!path)
+      return 2;
+    for (int64_t u = 0; u < n; u++)
+      gvizEmbeddedGraphSetVPosition(&e, (size_t)u, pos + dim * u);
+    wr_i(gvizEmbeddedGraphLoadEmbedding(&e, path));
+    for (int64_t u = 0; u < n; u++) {
+      const double *p = gvizEmbeddedGraphGetVPosition(&e, (size_t)u);
+      for (int64_t j = 0; j < dim; j++)
+        wr_d(p[j]);
+    }
+    free(path);
+  }
+  gvizEmbeddedGraphRelease(&e);
+  gvizGraphRelease(&g);
+  free(name);
+  free(save);
+  free(pos);
+  return 0;
+}
+
 /* Spring-driven Tutte: FixConvexPolygon + SeedInterior (+ Configure), then
  * `steps` Step(dt) calls and one Run(runTo, dt). Writes the seeded
  * positions, each step's return, positions and velocities after the steps,
@@ -793,6 +865,8 @@ int main(int argc, char **argv) {
     rc = mode_time_tutte();
   else if (!strcmp(argv[1], "edges_file"))
     rc = mode_edges_file();
+  else if (!strcmp(argv[1], "embedding_io"))
+    rc = mode_embedding_io();
   else if (!strcmp(argv[1], "spring_tutte"))
     rc = mode_spring_tutte();
   else if (!strcmp(argv[1], "tutte_begin"))

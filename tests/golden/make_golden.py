@@ -261,6 +261,68 @@ def api_grip_cases():
     api_grip_case("api_grip_grid16_decay", n, e, 16, 16, 8, 1)
 
 
+def embedding_io_texts(saved):
+    """Files for gvizEmbeddedGraphLoadEmbedding: the reference's own saved
+    text, then edits that exercise the parser (scientific notation, tabs and
+    blank lines, a long name line, a wrong vertex count, a wrong dimension, a
+    truncated list)."""
+    lines = saved.split(b"\n")
+    body = lines[2:]
+    sci = b"\n".join(lines[:2] + [b" ".join(f"{float(t):.6e}".encode() for t in ln.split())
+                                   for ln in body if ln.strip()]) + b"\n"
+    tabs = lines[0] + b"\n\n" + lines[1] + b"\n" + b"\t\n".join(
+        ln.replace(b" ", b"\t") for ln in body)
+    long_name = b"n" * 300 + b"\n" + b"\n".join(lines[1:])
+    n, dim = (int(t) for t in lines[1].split())
+    wrong_n = lines[0] + b"\n" + f"{n + 1} {dim}".encode() + b"\n" + b"\n".join(body)
+    wrong_dim = lines[0] + b"\n" + f"{n} {dim + 1}".encode() + b"\n" + b"\n".join(body)
+    truncated = b"\n".join(lines[:2 + n // 2]) + b"\n1.5 "
+    return [("saved", saved), ("scientific", sci), ("tabs", tabs), ("long_name", long_name),
+            ("wrong_count", wrong_n), ("wrong_dim", wrong_dim), ("truncated", truncated)]
+
+
+def embedding_io_cases():
+    """gvizEmbeddedGraphSaveEmbedding / LoadEmbedding of the reference
+    (src/embedders/gvizEmbeddedGraph.c:428-485) on positions that stress the
+    "%f" text: rounding at the sixth decimal, tiny and large magnitudes,
+    negative zero."""
+    import tempfile
+    rng = np.random.default_rng(17)
+    for dim in (2, 3):
+        n = 40
+        e = np.stack([np.arange(n - 1), np.arange(1, n)], 1)
+        pos = rng.uniform(-1e3, 1e3, (n, dim))
+        pos[0] = 0.0000005
+        pos[1] = -0.0000015
+        pos[2] = 1e-12
+        pos[3] = -0.0
+        pos[4] = 123456789.123456789
+        pos[5] = 2.5e-7
+        with tempfile.TemporaryDirectory() as td:
+            save_path = os.path.join(td, "saved.txt")
+            rc, saved, _ = refio.embedding_io(n, e, dim, pos, "layout", save_path, [])
+            texts = embedding_io_texts(saved)
+            paths = []
+            for name, text in texts:
+                p = os.path.join(td, name + ".txt")
+                with open(p, "wb") as f:
+                    f.write(text)
+                paths.append(p)
+            paths.append(os.path.join(td, "missing.txt"))
+            rc2, saved2, loads = refio.embedding_io(n, e, dim, pos, "layout", save_path,
+                                                    paths)
+        assert rc == rc2 == 0 and saved == saved2
+        arrays = dict(n=n, edges=e, dim=dim, pos=pos, save_rc=rc,
+                      saved=np.frombuffer(saved, np.uint8),
+                      names=np.array([t[0] for t in texts] + ["missing"]))
+        for k, (name, text) in enumerate(texts):
+            arrays[f"t{k}"] = np.frombuffer(text, np.uint8)
+        for k, (lrc, lpos) in enumerate(loads):
+            arrays[f"l{k}_rc"] = lrc
+            arrays[f"l{k}_pos"] = lpos
+        save(f"embedding_io_d{dim}", **arrays)
+
+
 def edges_text_cases():
     """(name, text, directed, skip_header) for the .edges loader: the
     reference's own test files (tests/data/*.edges) and files that exercise
@@ -352,7 +414,8 @@ def tutte_begin_cases():
 CASES = dict(quadtree=quadtree_cases, force=force_cases, tutte=tutte_cases,
              knn=knn_cases, grip=grip_cases, grip_chain=grip_chain_cases,
              grip_nd=grip_nd_cases, tutte_begin=tutte_begin_cases,
-             api=api_cases, api_grip=api_grip_cases, edges=edges_cases, spring_tutte=spring_tutte_cases)
+             api=api_cases, api_grip=api_grip_cases, edges=edges_cases,
+             embedding_io=embedding_io_cases, spring_tutte=spring_tutte_cases)
 
 if __name__ == "__main__":
     if not refio.harness_available():

@@ -276,6 +276,36 @@ def api_grip_spread(n, edges, kcap, placement_k, refinement_k, policy, perturb,
     return r.da(dim * n).reshape(n, dim)
 
 
+def embedding_io(n, edges, dim, positions, name, save_path, load_paths, exe=None):
+    """gvizEmbeddedGraphSaveEmbedding of `positions` to save_path (its return
+    and the bytes written), then for each of load_paths, from the same
+    positions, gvizEmbeddedGraphLoadEmbedding's return and the positions
+    after it."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(dim)
+    w.da(np.ascontiguousarray(positions, dtype=np.float64).reshape(-1))
+    for s in (name, save_path):
+        raw = os.fsencode(s)
+        w.i(len(raw))
+        w.parts.append(raw)
+    w.i(len(load_paths))
+    for s in load_paths:
+        raw = os.fsencode(s)
+        w.i(len(raw))
+        w.parts.append(raw)
+    r = _R(_run("embedding_io", w.bytes(), exe))
+    rc = r.i()
+    length = r.i()
+    text = r.buf[r.off:r.off + length]
+    r.off += length
+    loads = []
+    for _ in load_paths:
+        lrc = r.i()
+        loads.append((lrc, r.da(n * dim).reshape(n, dim)))
+    return rc, text, loads
+
+
 def edges_file(path, directed=0, skip_header=0, exe=None, csr=False):
     """The .edges loader on `path`: (rc, offsets, adj) of the adjacency lists
     in order (offsets/adj None when rc != 0). With csr=True (this
