@@ -19,12 +19,13 @@ int gvizEmbeddedGraphInit(gvizEmbeddedGraph *e, gvizSubgraph subgraph,
   e->embedding.dim = dimension;
   size_t n = gvizGraphSize(subgraph.g);
   e->embedding.vertexPositions = calloc((n ? n : 1) * dimension, sizeof(double));
-  e->visibleVertices = calloc(n ? n : 1, 1);
-  if (!e->embedding.vertexPositions || !e->visibleVertices)
+  e->drawMask.visibleVertices = gvizVertexSubsetCreateEmpty(subgraph.g);
+  if (!e->embedding.vertexPositions || !e->drawMask.visibleVertices)
     return -1;
   for (size_t u = 0; u < n; u++)
-    e->visibleVertices[u] = gvizSubgraphHasVertex(&subgraph, u) ? 1 : 0;
-  e->edgePolicy = GVIZ_DRAW_EDGES_ALL;
+    if (gvizSubgraphHasVertex(&subgraph, u))
+      gvizVertexSubsetShowVertex(e->drawMask.visibleVertices, u);
+  e->drawMask.edgePolicy = GVIZ_DRAW_EDGES_ALL;
   return 0;
 }
 
@@ -32,17 +33,17 @@ void gvizEmbeddedGraphRelease(gvizEmbeddedGraph *e) {
   if (!e)
     return;
   free(e->embedding.vertexPositions);
-  free(e->actions);
-  for (size_t i = 0; i < e->statCount; i++)
-    free(e->stats[i].samples);
-  free(e->stats);
-  free(e->visibleVertices);
+  free(e->actions.actions);
+  for (size_t i = 0; i < e->stats.count; i++)
+    free(e->stats.series[i].samples);
+  free(e->stats.series);
+  gvizVertexSubsetRelease(e->drawMask.visibleVertices);
   gvizSubgraphRelease(&e->subgraph);
+  gvizSubgraphRelease(&e->highlight);
   e->embedding.vertexPositions = NULL;
-  e->actions = NULL;
-  e->stats = NULL;
-  e->visibleVertices = NULL;
-  e->actionCount = e->statCount = 0;
+  memset(&e->actions, 0, sizeof(e->actions));
+  memset(&e->stats, 0, sizeof(e->stats));
+  memset(&e->drawMask, 0, sizeof(e->drawMask));
 }
 
 size_t gvizEmbeddedGraphDim(const gvizEmbeddedGraph *e) { return e->embedding.dim; }
@@ -87,9 +88,9 @@ void gvizEmbeddedGraphRandomizePositions(gvizEmbeddedGraph *e, double boxExtent,
 }
 
 static gvizAction *find_action(const gvizEmbeddedGraph *e, const char *name) {
-  for (size_t i = 0; i < e->actionCount; i++)
-    if (strcmp(e->actions[i].name, name) == 0)
-      return &e->actions[i];
+  for (size_t i = 0; i < e->actions.count; i++)
+    if (strcmp(e->actions.actions[i].name, name) == 0)
+      return &e->actions.actions[i];
   return NULL;
 }
 
@@ -103,15 +104,15 @@ int gvizEmbeddedGraphAddAction(gvizEmbeddedGraph *e, const char *name,
     existing->userData = userData;
     return 0;
   }
-  if (e->actionCount == e->actionCapacity) {
-    size_t cap = e->actionCapacity ? e->actionCapacity * 2 : 4;
-    gvizAction *grown = realloc(e->actions, cap * sizeof(gvizAction));
+  if (e->actions.count == e->actions.capacity) {
+    size_t cap = e->actions.capacity ? e->actions.capacity * 2 : 4;
+    gvizAction *grown = realloc(e->actions.actions, cap * sizeof(gvizAction));
     if (!grown)
       return -1;
-    e->actions = grown;
-    e->actionCapacity = cap;
+    e->actions.actions = grown;
+    e->actions.capacity = cap;
   }
-  e->actions[e->actionCount++] = (gvizAction){name, handler, userData};
+  e->actions.actions[e->actions.count++] = (gvizAction){name, handler, userData};
   return 0;
 }
 
@@ -121,7 +122,7 @@ int gvizEmbeddedGraphRemoveAction(gvizEmbeddedGraph *e, const char *name) {
   gvizAction *a = find_action(e, name);
   if (!a)
     return -1;
-  *a = e->actions[--e->actionCount];
+  *a = e->actions.actions[--e->actions.count];
   return 0;
 }
 
@@ -133,12 +134,12 @@ const gvizAction *gvizEmbeddedGraphFindAction(const gvizEmbeddedGraph *e,
 }
 
 size_t gvizEmbeddedGraphActionCount(const gvizEmbeddedGraph *e) {
-  return e->actionCount;
+  return e->actions.count;
 }
 
 const gvizAction *gvizEmbeddedGraphActionAt(const gvizEmbeddedGraph *e,
                                             size_t idx) {
-  return e && idx < e->actionCount ? &e->actions[idx] : NULL;
+  return e && idx < e->actions.count ? &e->actions.actions[idx] : NULL;
 }
 
 int gvizEmbeddedGraphInvokeAction(gvizEmbeddedGraph *e, const char *name,
@@ -152,9 +153,9 @@ int gvizEmbeddedGraphInvokeAction(gvizEmbeddedGraph *e, const char *name,
 }
 
 static gvizStatSeries *find_series(const gvizEmbeddedGraph *e, const char *name) {
-  for (size_t i = 0; i < e->statCount; i++)
-    if (strcmp(e->stats[i].name, name) == 0)
-      return &e->stats[i];
+  for (size_t i = 0; i < e->stats.count; i++)
+    if (strcmp(e->stats.series[i].name, name) == 0)
+      return &e->stats.series[i];
   return NULL;
 }
 
@@ -168,15 +169,15 @@ gvizStatSeries *gvizEmbeddedGraphAddStatSeries(gvizEmbeddedGraph *e,
     s->kind = kind;
     return s;
   }
-  if (e->statCount == e->statCapacity) {
-    size_t cap = e->statCapacity ? e->statCapacity * 2 : 4;
-    gvizStatSeries *grown = realloc(e->stats, cap * sizeof(gvizStatSeries));
+  if (e->stats.count == e->stats.capacity) {
+    size_t cap = e->stats.capacity ? e->stats.capacity * 2 : 4;
+    gvizStatSeries *grown = realloc(e->stats.series, cap * sizeof(gvizStatSeries));
     if (!grown)
       return NULL;
-    e->stats = grown;
-    e->statCapacity = cap;
+    e->stats.series = grown;
+    e->stats.capacity = cap;
   }
-  s = &e->stats[e->statCount++];
+  s = &e->stats.series[e->stats.count++];
   memset(s, 0, sizeof(*s));
   s->name = name;
   s->kind = kind;
@@ -214,12 +215,12 @@ void gvizEmbeddedGraphStatClear(gvizEmbeddedGraph *e, const char *name) {
 }
 
 size_t gvizEmbeddedGraphStatSeriesCount(const gvizEmbeddedGraph *e) {
-  return e->statCount;
+  return e->stats.count;
 }
 
 const gvizStatSeries *gvizEmbeddedGraphStatSeriesAt(const gvizEmbeddedGraph *e,
                                                     size_t idx) {
-  return e && idx < e->statCount ? &e->stats[idx] : NULL;
+  return e && idx < e->stats.count ? &e->stats.series[idx] : NULL;
 }
 
 const gvizStatSeries *gvizEmbeddedGraphFindStatSeries(const gvizEmbeddedGraph *e,
@@ -230,40 +231,44 @@ const gvizStatSeries *gvizEmbeddedGraphFindStatSeries(const gvizEmbeddedGraph *e
 /* Draw mask (src/embedders/gvizEmbeddedGraph.c:74-122): vertex edits do not
  * bump the revision; policy changes, resets and explicit notifications do. */
 void gvizEmbeddedGraphDrawMaskShowVertex(gvizEmbeddedGraph *e, size_t u) {
-  e->visibleVertices[u] = 1;
+  gvizVertexSubsetShowVertex(e->drawMask.visibleVertices, u);
 }
 void gvizEmbeddedGraphDrawMaskHideVertex(gvizEmbeddedGraph *e, size_t u) {
-  e->visibleVertices[u] = 0;
+  gvizVertexSubsetHideVertex(e->drawMask.visibleVertices, u);
 }
 void gvizEmbeddedGraphDrawMaskClearVertices(gvizEmbeddedGraph *e) {
-  memset(e->visibleVertices, 0, gvizGraphSize(e->subgraph.g));
+  gvizVertexSubsetClearAll(e->drawMask.visibleVertices, gvizGraphSize(e->subgraph.g));
 }
 void gvizEmbeddedGraphDrawMaskNotifyChanged(gvizEmbeddedGraph *e) {
-  e->drawMaskRevision++;
+  e->drawMask.revision++;
 }
 void gvizEmbeddedGraphResetDrawMask(gvizEmbeddedGraph *e) {
-  e->edgePolicy = GVIZ_DRAW_EDGES_ALL;
+  e->drawMask.edgePolicy = GVIZ_DRAW_EDGES_ALL;
   size_t n = gvizGraphSize(e->subgraph.g);
-  for (size_t u = 0; u < n; u++)
-    e->visibleVertices[u] = gvizSubgraphHasVertex(&e->subgraph, u) ? 1 : 0;
-  e->drawMaskRevision++;
+  for (size_t u = 0; u < n; u++) {
+    if (gvizSubgraphHasVertex(&e->subgraph, u))
+      gvizVertexSubsetShowVertex(e->drawMask.visibleVertices, u);
+    else
+      gvizVertexSubsetHideVertex(e->drawMask.visibleVertices, u);
+  }
+  e->drawMask.revision++;
 }
 uint64_t gvizEmbeddedGraphDrawMaskRevision(const gvizEmbeddedGraph *e) {
-  return e->drawMaskRevision;
+  return e->drawMask.revision;
 }
 void gvizEmbeddedGraphSetDrawMaskEdgePolicy(gvizEmbeddedGraph *e,
                                             gvizDrawEdgePolicy policy) {
-  e->edgePolicy = policy;
-  e->drawMaskRevision++;
+  e->drawMask.edgePolicy = policy;
+  e->drawMask.revision++;
 }
 bool gvizEmbeddedGraphIsVertexVisible(const gvizEmbeddedGraph *e, size_t u) {
   if (!gvizSubgraphHasVertex(&e->subgraph, u))
     return false;
-  return e->visibleVertices[u] != 0;
+  return gvizVertexSubsetTest(e->drawMask.visibleVertices, u);
 }
 bool gvizEmbeddedGraphIsEdgeVisible(const gvizEmbeddedGraph *e, size_t u,
                                     size_t v) {
-  switch (e->edgePolicy) {
+  switch (e->drawMask.edgePolicy) {
   case GVIZ_DRAW_EDGES_NONE:
     return false;
   case GVIZ_DRAW_EDGES_IF_BOTH_VISIBLE:
@@ -274,12 +279,7 @@ bool gvizEmbeddedGraphIsEdgeVisible(const gvizEmbeddedGraph *e, size_t u,
     if (!gvizSubgraphHasVertex(&e->subgraph, u) ||
         !gvizSubgraphHasVertex(&e->subgraph, v))
       return false;
-    size_t c;
-    const size_t *nb = gvizGraphNeighbors(e->subgraph.g, u, &c);
-    for (size_t i = 0; i < c; i++)
-      if (nb[i] == v)
-        return true;
-    return false;
+    return gvizSubgraphHasEdge(&e->subgraph, u, v);
   }
   }
 }
@@ -325,4 +325,8 @@ int gvizEmbeddedGraphLoadEmbedding(gvizEmbeddedGraph *e, const char *filename) {
   }
   fclose(f);
   return 0;
+}
+
+int gvizEmbeddedGraphIsPlanarEmbedded(const gvizEmbeddedGraph *e) {
+  return e && e->planarEmbedded;
 }

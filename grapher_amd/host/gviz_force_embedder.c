@@ -12,6 +12,7 @@
 #include "gviz/gvizForceEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_side.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -136,13 +137,31 @@ static double default_box_extent(size_t vertexCount, double edgeLength) {
   return 0.5 * sqrt((double)vertexCount) * edgeLength;
 }
 
-static void device_config(const gvizForceEmbedderState *s,
+/* The engine's extras beside the reference's state layout: the device
+ * handle, the configuration last handed to it (a setter call costs nothing
+ * until the next step; a change re-records the step's graph), the
+ * precision, the residency flag and the compact CSR the device was built
+ * from. */
+typedef struct force_side {
+  gvizhip_force *f; /* NULL before Begin */
+  gvizhip_force_config applied;
+  gvizForcePrecision precision;
+  int residentPositions;
+  int64_t *csrOffsets, *csrAdj;
+} force_side;
+
+static force_side *side_of(const gvizForceEmbedderState *s) {
+  return (force_side *)gviz_side_get(s);
+}
+
+static void device_config(const gvizForceEmbedderState *s, const force_side *x,
                           gvizhip_force_config *c) {
   memset(c, 0, sizeof(*c));
-  c->model = (int32_t)s->modelKind;
+  /* the built-in models are recognised by identity (their vtable pointers) */
+  c->model = s->model == gvizForceModelGet(GVIZ_FORCE_MODEL_LINLOG) ? 1 : 0;
   c->barnesHut = s->barnesHutEnabled ? 1 : 0;
   c->preventOverlap = s->preventOverlap ? 1 : 0;
-  c->precision = (int32_t)s->precision;
+  c->precision = (int32_t)x->precision;
   c->nodesPerCell = (int64_t)s->nodesPerCell;
   c->edgeLength = s->edgeLength;
   c->theta = s->theta;
@@ -153,23 +172,14 @@ static void device_config(const gvizForceEmbedderState *s,
   c->jitterTolerance = s->jitterTolerance;
 }
 
-/* The configuration last handed to the device, kept beside the handle so a
- * setter call costs nothing until the next step (a change re-records the
- * step's graph). */
-typedef struct device_box {
-  gvizhip_force *f;
-  gvizhip_force_config applied;
-} device_box;
-
-static int sync_config(gvizForceEmbedderState *s) {
-  device_box *box = (device_box *)s->device;
+static int sync_config(gvizForceEmbedderState *s, force_side *x) {
   gvizhip_force_config c;
-  device_config(s, &c);
-  if (memcmp(&c, &box->applied, sizeof(c)) == 0)
+  device_config(s, x, &c);
+  if (memcmp(&c, &x->applied, sizeof(c)) == 0)
     return 0;
-  if (gvizhip_force_configure(box->f, &c) != 0)
+  if (gvizhip_force_configure(x->f, &c) != 0)
     return -1;
-  box->applied = c;
+  x->applied = c;
   return 0;
 }
 
@@ -227,10 +237,17 @@ int gvizForceEmbedderInit(gvizForceEmbedderState *s, gvizSubgraph subgraph,
   if (res < 0)
     return res;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
-  s->modelKind = model == GVIZ_FORCE_MODEL_LINLOG ? GVIZ_FORCE_MODEL_LINLOG
-                                                  : GVIZ_FORCE_MODEL_FRUCHTERMAN_REINGOLD;
-  s->model = gvizForceModelGet(s->modelKind);
-  if (gvizSubgraphToCompactCSR(&e->subgraph, &s->csrOffsets, &s->csrAdj,
+  s->model = gvizForceModelGet(model == GVIZ_FORCE_MODEL_LINLOG
+                                   ? GVIZ_FORCE_MODEL_LINLOG
+                                   : GVIZ_FORCE_MODEL_FRUCHTERMAN_REINGOLD);
+  force_side *x = calloc(1, sizeof(force_side)), *prev = NULL;
+  if (!x || gviz_side_put(s, x, (void **)&prev) != 0) {
+    free(x);
+    return -1;
+  }
+  free(prev); /* a state initialised again without Release */
+  x->precision = GVIZ_FORCE_PRECISION_MIXED;
+  if (gvizSubgraphToCompactCSR(&e->subgraph, &x->csrOffsets, &x->csrAdj,
                                &s->vertices, &s->vertexCount) != 0)
     return -1;
   size_t n = s->vertexCount;
@@ -241,7 +258,7 @@ int gvizForceEmbedderInit(gvizForceEmbedderState *s, gvizSubgraph subgraph,
     return -1;
   double sumDegPlusOne = 0.0;
   for (size_t i = 0; i < n; i++) {
-    s->degree[i] = (size_t)(s->csrOffsets[i + 1] - s->csrOffsets[i]);
+    s->degree[i] = (size_t)(x->csrOffsets[i + 1] - x->csrOffsets[i]);
     s->mass[i] = s->model->vertexMass(s->degree[i]);
     sumDegPlusOne += (double)(s->degree[i] + 1);
   }
@@ -258,8 +275,6 @@ int gvizForceEmbedderInit(gvizForceEmbedderState *s, gvizSubgraph subgraph,
   s->overlapConstant = GVIZ_FORCE_EMBEDDER_OVERLAP_CONSTANT_DEFAULT;
   s->preventOverlap = 0;
   s->barnesHutEnabled = 1;
-  s->precision = GVIZ_FORCE_PRECISION_MIXED;
-  s->residentPositions = 0;
 
   if (gvizEmbeddedGraphAddAction(e, "forceEmbedder.step", action_step, NULL) < 0)
     return -1;
@@ -279,24 +294,23 @@ int gvizForceEmbedderInit(gvizForceEmbedderState *s, gvizSubgraph subgraph,
 void gvizForceEmbedderRelease(gvizForceEmbedderState *s) {
   if (!s)
     return;
-  if (s->device) {
-    device_box *box = (device_box *)s->device;
-    gvizhip_force_destroy(box->f);
-    free(box);
-    s->device = NULL;
+  force_side *x = (force_side *)gviz_side_take(s);
+  if (x) {
+    if (x->f)
+      gvizhip_force_destroy(x->f);
+    free(x->csrOffsets);
+    free(x->csrAdj);
+    free(x);
   }
   gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
   free(s->vertices);
   free(s->degree);
   free(s->mass);
   free(s->positionsScratch);
-  free(s->csrOffsets);
-  free(s->csrAdj);
   s->vertices = NULL;
   s->degree = NULL;
   s->mass = NULL;
   s->positionsScratch = NULL;
-  s->csrOffsets = s->csrAdj = NULL;
 }
 
 /* Setters: src/embedders/gvizForceEmbedder.c:357-410 */
@@ -356,21 +370,31 @@ void gvizForceEmbedderSetPreventOverlapEnabled(gvizForceEmbedderState *s,
 
 void gvizForceEmbedderSetPrecision(gvizForceEmbedderState *s,
                                    gvizForcePrecision precision) {
-  s->precision = precision;
+  force_side *x = side_of(s);
+  if (x)
+    x->precision = precision;
+}
+
+gvizForcePrecision gvizForceEmbedderPrecision(const gvizForceEmbedderState *s) {
+  force_side *x = side_of(s);
+  return x ? x->precision : GVIZ_FORCE_PRECISION_MIXED;
 }
 
 void gvizForceEmbedderSetResidentPositions(gvizForceEmbedderState *s,
                                            int enabled) {
-  if (s->residentPositions && !enabled && s->begun)
+  force_side *x = side_of(s);
+  if (!x)
+    return;
+  if (x->residentPositions && !enabled && s->begun)
     gvizForceEmbedderSyncPositions(s);
-  s->residentPositions = enabled;
+  x->residentPositions = enabled;
 }
 
 int gvizForceEmbedderSyncPositions(gvizForceEmbedderState *s) {
-  if (!s->device)
+  force_side *x = side_of(s);
+  if (!x || !x->f)
     return s->vertexCount == 0 ? 0 : -1;
-  device_box *box = (device_box *)s->device;
-  if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
+  if (gvizhip_force_download_positions(x->f, s->positionsScratch) != 0)
     return -1;
   scatter_positions(s);
   return 0;
@@ -382,22 +406,22 @@ static const char kStateFileMagic[8] = {'G', 'V', 'Z', 'F', 'E', 'S', '0', '1'};
 
 int gvizForceEmbedderSaveState(gvizForceEmbedderState *s, const char *filename) {
   host_error = "";
-  if (!s->begun || (!s->device && s->vertexCount > 0) || !filename) {
+  force_side *x = side_of(s);
+  if (!s->begun || !x || (!x->f && s->vertexCount > 0) || !filename) {
     host_error = "gvizForceEmbedderSaveState: not begun or no file name";
     return -1;
   }
-  device_box *box = (device_box *)s->device;
   int64_t bytes = 0;
   void *blob = NULL;
-  if (box) {
-    if (!s->residentPositions) {
+  if (x->f) {
+    if (!x->residentPositions) {
       gather_positions(s);
-      if (gvizhip_force_upload_positions(box->f, s->positionsScratch) != 0)
+      if (gvizhip_force_upload_positions(x->f, s->positionsScratch) != 0)
         return -1;
     }
-    bytes = gvizhip_force_state_bytes(box->f);
+    bytes = gvizhip_force_state_bytes(x->f);
     blob = bytes > 0 ? malloc((size_t)bytes) : NULL;
-    if (!blob || gvizhip_force_save_state(box->f, blob, bytes) != 0) {
+    if (!blob || gvizhip_force_save_state(x->f, blob, bytes) != 0) {
       free(blob);
       host_error = "gvizForceEmbedderSaveState: device state not read";
       return -1;
@@ -424,7 +448,8 @@ int gvizForceEmbedderSaveState(gvizForceEmbedderState *s, const char *filename) 
 
 int gvizForceEmbedderLoadState(gvizForceEmbedderState *s, const char *filename) {
   host_error = "";
-  if (!s->begun || (!s->device && s->vertexCount > 0) || !filename) {
+  force_side *x = side_of(s);
+  if (!s->begun || !x || (!x->f && s->vertexCount > 0) || !filename) {
     host_error = "gvizForceEmbedderLoadState: Begin the embedder on the same graph first";
     return -1;
   }
@@ -445,20 +470,19 @@ int gvizForceEmbedderLoadState(gvizForceEmbedderState *s, const char *filename) 
     ok = blob && fread(blob, 1, (size_t)bytes, f) == (size_t)bytes;
   }
   fclose(f);
-  device_box *box = (device_box *)s->device;
-  if (ok && box && sync_config(s) != 0)
+  if (ok && x->f && sync_config(s, x) != 0)
     ok = 0;
-  if (ok && box) {
-    ok = gvizhip_force_load_state(box->f, blob, bytes) == 0;
+  if (ok && x->f) {
+    ok = gvizhip_force_load_state(x->f, blob, bytes) == 0;
     double gs = 0.0, eff = 0.0;
     int64_t dit = 0;
     if (ok)
-      ok = gvizhip_force_get_speed(box->f, &gs, &eff, &dit) == 0;
+      ok = gvizhip_force_get_speed(x->f, &gs, &eff, &dit) == 0;
     if (ok) {
       s->globalSpeed = gs;
       s->speedEfficiency = eff;
     }
-    if (ok && !s->residentPositions)
+    if (ok && !x->residentPositions)
       ok = gvizForceEmbedderSyncPositions(s) == 0;
   } else if (ok && bytes != 0) {
     ok = 0;
@@ -473,21 +497,15 @@ int gvizForceEmbedderLoadState(gvizForceEmbedderState *s, const char *filename) 
   return 0;
 }
 
-static int ensure_device(gvizForceEmbedderState *s) {
-  if (s->device)
-    return sync_config(s);
-  device_box *box = calloc(1, sizeof(device_box));
-  if (!box) {
-    host_error = "gvizForceEmbedder: out of host memory";
+static int ensure_device(gvizForceEmbedderState *s, force_side *x) {
+  if (x->f)
+    return sync_config(s, x);
+  device_config(s, x, &x->applied);
+  if (gvizhip_force_create(&x->f, (int64_t)s->vertexCount, x->csrOffsets,
+                           x->csrAdj, &x->applied) != 0) {
+    x->f = NULL;
     return -1;
   }
-  device_config(s, &box->applied);
-  if (gvizhip_force_create(&box->f, (int64_t)s->vertexCount, s->csrOffsets,
-                           s->csrAdj, &box->applied) != 0) {
-    free(box);
-    return -1;
-  }
-  s->device = box;
   return 0;
 }
 
@@ -502,10 +520,15 @@ int gvizForceEmbedderBegin(gvizForceEmbedderState *s, unsigned int seed) {
   s->begun = 1;
   if (s->vertexCount == 0)
     return 0;
-  gather_positions(s);
-  if (ensure_device(s) != 0)
+  force_side *x = side_of(s);
+  if (!x) {
+    host_error = "gvizForceEmbedderBegin: the state was not initialised";
     return -1;
-  return gvizhip_force_begin(((device_box *)s->device)->f, s->positionsScratch);
+  }
+  gather_positions(s);
+  if (ensure_device(s, x) != 0)
+    return -1;
+  return gvizhip_force_begin(x->f, s->positionsScratch);
 }
 
 /* Appends the stats of a step on an empty subgraph (every mean is 0). */
@@ -523,23 +546,23 @@ double gvizForceEmbedderStep(gvizForceEmbedderState *s) {
   host_error = "";
   if (s->vertexCount == 0)
     return empty_step(s);
-  if (!s->device) {
+  force_side *x = side_of(s);
+  if (!x || !x->f) {
     host_error = "gvizForceEmbedderStep: Begin has not created the GPU engine";
     return -1.0;
   }
-  device_box *box = (device_box *)s->device;
-  if (sync_config(s) != 0)
+  if (sync_config(s, x) != 0)
     return -1.0;
-  if (!s->residentPositions) {
+  if (!x->residentPositions) {
     gather_positions(s);
-    if (gvizhip_force_upload_positions(box->f, s->positionsScratch) != 0)
+    if (gvizhip_force_upload_positions(x->f, s->positionsScratch) != 0)
       return -1.0;
   }
   gvizhip_force_stats r;
-  if (gvizhip_force_step(box->f, &r) != 0)
+  if (gvizhip_force_step(x->f, &r) != 0)
     return -1.0;
-  if (!s->residentPositions) {
-    if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
+  if (!x->residentPositions) {
+    if (gvizhip_force_download_positions(x->f, s->positionsScratch) != 0)
       return -1.0;
     scatter_positions(s);
   }
@@ -565,14 +588,14 @@ int gvizForceEmbedderRun(gvizForceEmbedderState *s, size_t maxIters,
     }
     return (int)rounds;
   }
-  if (!s->device)
+  force_side *x = side_of(s);
+  if (!x || !x->f)
     return -1;
-  device_box *box = (device_box *)s->device;
-  if (sync_config(s) != 0)
+  if (sync_config(s, x) != 0)
     return -1;
-  if (!s->residentPositions) {
+  if (!x->residentPositions) {
     gather_positions(s);
-    if (gvizhip_force_upload_positions(box->f, s->positionsScratch) != 0)
+    if (gvizhip_force_upload_positions(x->f, s->positionsScratch) != 0)
       return -1;
   }
   enum { kChunk = 4096 };
@@ -585,7 +608,7 @@ int gvizForceEmbedderRun(gvizForceEmbedderState *s, size_t maxIters,
   int failed = 0;
   while (rounds < maxIters) {
     size_t want = maxIters - rounds < kChunk ? maxIters - rounds : kChunk;
-    int64_t ran = gvizhip_force_run(box->f, (int64_t)want, epsilon, rec);
+    int64_t ran = gvizhip_force_run(x->f, (int64_t)want, epsilon, rec);
     if (ran < 0) {
       failed = 1;
       break;
@@ -597,8 +620,8 @@ int gvizForceEmbedderRun(gvizForceEmbedderState *s, size_t maxIters,
       break;
   }
   free(rec);
-  if (!s->residentPositions) {
-    if (gvizhip_force_download_positions(box->f, s->positionsScratch) != 0)
+  if (!x->residentPositions) {
+    if (gvizhip_force_download_positions(x->f, s->positionsScratch) != 0)
       return -1;
     scatter_positions(s);
   }

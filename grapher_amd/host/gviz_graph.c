@@ -1,160 +1,438 @@
 /*
- * gviz_graph.c -- adjacency lists, vertex subsets and subgraph views of the
- * host layer, and their flattening to CSR for the device engine. Neighbour
- * order is insertion order (reference: src/ds/gvizGraph.c:163-172,
- * src/ds/gvizSubgraph.c:466-519).
+ * gviz_graph.c -- arrays, adjacency lists, bit-array vertex/edge subsets and
+ * subgraph views of the host layer, and their flattening to CSR for the
+ * device engines. Neighbour order is insertion order (reference:
+ * src/ds/gvizGraph.c:131-172, src/ds/gvizSubgraph.c:105-540).
  */
 #include "gviz/gvizGraph.h"
 
 #include <stdlib.h>
 #include <string.h>
 
-static int adj_push(gvizAdjList *l, size_t v) {
-  if (l->count == l->capacity) {
-    size_t cap = l->capacity ? l->capacity * 2 : 4;
-    size_t *grown = realloc(l->items, cap * sizeof(size_t));
-    if (!grown)
-      return -1;
-    l->items = grown;
-    l->capacity = cap;
+/* ---------------------------------------------------------------- arrays */
+
+int gvizArrayInitAtCapacity(gvizArray *a, size_t elementSize, size_t capacity) {
+  if (!a || elementSize == 0)
+    return -1;
+  a->elementSize = elementSize;
+  a->count = 0;
+  a->capacity = capacity;
+  a->arr = capacity ? malloc(elementSize * capacity) : NULL;
+  if (capacity && !a->arr) {
+    a->capacity = 0;
+    return -1;
   }
-  l->items[l->count++] = v;
   return 0;
 }
 
-int gvizGraphInit(gvizGraph *g, int directed) {
+int gvizArrayInit(gvizArray *a, size_t elementSize) {
+  return gvizArrayInitAtCapacity(a, elementSize, 4);
+}
+
+static int array_reserve(gvizArray *a, size_t need) {
+  if (need <= a->capacity)
+    return 0;
+  size_t cap = a->capacity ? a->capacity : 4;
+  while (cap < need)
+    cap *= 2;
+  void *grown = realloc(a->arr, a->elementSize * cap);
+  if (!grown)
+    return -1;
+  a->arr = grown;
+  a->capacity = cap;
+  return 0;
+}
+
+int gvizArrayPush(gvizArray *a, void *item) {
+  if (!a || !item || array_reserve(a, a->count + 1) != 0)
+    return -1;
+  memcpy((char *)a->arr + a->count * a->elementSize, item, a->elementSize);
+  a->count++;
+  return 0;
+}
+
+int gvizArrayPop(gvizArray *a, void *res) {
+  if (!a || a->count == 0)
+    return -1;
+  a->count--;
+  if (res)
+    memcpy(res, (char *)a->arr + a->count * a->elementSize, a->elementSize);
+  return 0;
+}
+
+int gvizArrayIsEmpty(const gvizArray *a) { return !a || a->count == 0; }
+
+int gvizArrayCopy(gvizArray *dest, const gvizArray *src) {
+  if (!dest || !src || dest->elementSize != src->elementSize ||
+      array_reserve(dest, src->count) != 0)
+    return -1;
+  if (src->count)
+    memcpy(dest->arr, src->arr, src->count * src->elementSize);
+  dest->count = src->count;
+  return 0;
+}
+
+void gvizArrayRelease(gvizArray *a) {
+  if (!a)
+    return;
+  free(a->arr);
+  a->arr = NULL;
+  a->count = a->capacity = 0;
+}
+
+void *gvizArrayAtIndex(const gvizArray *a, size_t i) {
+  if (!a || i >= a->count)
+    return NULL;
+  return (char *)a->arr + i * a->elementSize;
+}
+
+void *gvizArrayTail(const gvizArray *a) {
+  return a && a->count ? gvizArrayAtIndex(a, a->count - 1) : NULL;
+}
+
+/* ------------------------------------------------------------ bit arrays */
+
+static GVIZ_BIT_ARRAY bits_alloc(size_t nbits) {
+  size_t units = GVIZ_ARRAY_UNITS(nbits);
+  return calloc(units ? units : 1, 1);
+}
+
+static void bits_fill(GVIZ_BIT_ARRAY a, size_t nbits) {
+  if (!a || nbits == 0)
+    return;
+  memset(a, 0xFF, nbits / 8);
+  for (size_t k = nbits / 8 * 8; k < nbits; k++)
+    gvizSetBit(a, k);
+}
+
+static size_t bits_count(const GVIZ_BIT_UNIT *a, size_t from, size_t to) {
+  size_t c = 0;
+  for (size_t k = from; k < to; k++)
+    c += gvizTestBit(a, k) != 0;
+  return c;
+}
+
+/* ---------------------------------------------------------------- graphs */
+
+int gvizVertexInitAtCapacity(gvizVertex *v, void *data, size_t initialCapacity) {
+  if (!v)
+    return -1;
+  v->data = data;
+  return gvizArrayInitAtCapacity(&v->neighbors, sizeof(size_t), initialCapacity);
+}
+
+int gvizVertexInit(gvizVertex *v, void *data) {
+  return gvizVertexInitAtCapacity(v, data, 4);
+}
+
+void gvizVertexRelease(gvizVertex *v) {
+  if (v)
+    gvizArrayRelease(&v->neighbors);
+}
+
+int gvizGraphInitAtCapacity(gvizGraph *g, int directed, size_t initialCapacity) {
   if (!g)
     return -1;
   memset(g, 0, sizeof(*g));
   g->directed = directed;
-  return 0;
+  return gvizArrayInitAtCapacity(&g->vertices, sizeof(gvizVertex),
+                                 initialCapacity ? initialCapacity : 16);
+}
+
+int gvizGraphInit(gvizGraph *g, int directed) {
+  return gvizGraphInitAtCapacity(g, directed, 16);
+}
+
+static gvizVertex *vertex_at(const gvizGraph *g, size_t u) {
+  return (gvizVertex *)gvizArrayAtIndex(&g->vertices, u);
 }
 
 void gvizGraphRelease(gvizGraph *g) {
   if (!g)
     return;
-  for (size_t i = 0; i < g->nvertices; i++)
-    free(g->adj[i].items);
-  free(g->adj);
+  for (size_t i = 0; i < g->vertices.count; i++)
+    gvizVertexRelease(vertex_at(g, i));
+  gvizArrayRelease(&g->vertices);
+  if (g->layout) {
+    free(g->layout->vertexOffsets);
+    free(g->layout);
+  }
+  free(g->map);
   memset(g, 0, sizeof(*g));
 }
 
-size_t gvizGraphSize(const gvizGraph *g) { return g ? g->nvertices : 0; }
+size_t gvizGraphSize(const gvizGraph *g) { return g ? g->vertices.count : 0; }
 int gvizGraphIsDirected(const gvizGraph *g) { return g->directed; }
 
-int gvizGraphAddVertex(gvizGraph *g, void *data, void *in, void *out) {
-  (void)data;
-  (void)in;
-  (void)out;
+int gvizGraphAddVertex(gvizGraph *g, void *data, gvizArray *in, gvizArray *out) {
   if (!g)
     return -1;
-  if (g->nvertices == g->capacity) {
-    size_t cap = g->capacity ? g->capacity * 2 : 16;
-    gvizAdjList *grown = realloc(g->adj, cap * sizeof(gvizAdjList));
-    if (!grown)
-      return -1;
-    g->adj = grown;
-    g->capacity = cap;
+  gvizVertex v;
+  if (gvizVertexInitAtCapacity(&v, data, out && out->count ? out->count : 4) != 0)
+    return -1;
+  if (out && gvizArrayCopy(&v.neighbors, out) != 0) {
+    gvizVertexRelease(&v);
+    return -1;
   }
-  memset(&g->adj[g->nvertices], 0, sizeof(gvizAdjList));
-  g->nvertices++;
+  if (gvizArrayPush(&g->vertices, &v) != 0) {
+    gvizVertexRelease(&v);
+    return -1;
+  }
+  if (in) {
+    size_t idx = g->vertices.count - 1;
+    for (size_t i = 0; i < in->count; i++) {
+      size_t u = *(size_t *)gvizArrayAtIndex(in, i);
+      if (u >= g->vertices.count || gvizArrayPush(&vertex_at(g, u)->neighbors, &idx) != 0)
+        return -1;
+    }
+  }
   return 0;
 }
 
 int gvizGraphAddEdge(gvizGraph *g, size_t from, size_t to) {
-  if (!g || from >= g->nvertices || to >= g->nvertices)
+  if (!g || from >= gvizGraphSize(g) || to >= gvizGraphSize(g))
     return -1;
-  if (adj_push(&g->adj[from], to) != 0)
+  if (gvizArrayPush(&vertex_at(g, from)->neighbors, &to) != 0)
     return -1;
   if (!g->directed)
-    return adj_push(&g->adj[to], from);
+    return gvizArrayPush(&vertex_at(g, to)->neighbors, &from);
+  return 0;
+}
+
+int gvizGraphEdgeExists(gvizGraph *g, size_t from, size_t to) {
+  size_t c = 0;
+  const size_t *nb = gvizGraphNeighbors(g, from, &c);
+  for (size_t k = 0; k < c; k++)
+    if (nb[k] == to)
+      return 1;
   return 0;
 }
 
 size_t gvizGraphEdgeCount(const gvizGraph *g) {
   size_t total = 0;
-  for (size_t i = 0; i < g->nvertices; i++)
-    total += g->adj[i].count;
+  for (size_t i = 0; i < gvizGraphSize(g); i++)
+    total += vertex_at(g, i)->neighbors.count;
   return g->directed ? total : total / 2;
 }
 
-const size_t *gvizGraphNeighbors(const gvizGraph *g, size_t u, size_t *count) {
-  if (!g || u >= g->nvertices) {
-    if (count)
-      *count = 0;
-    return NULL;
-  }
-  if (count)
-    *count = g->adj[u].count;
-  return g->adj[u].items;
-}
-
-gvizVertexSubset gvizVertexSubsetCreateEmpty(const gvizGraph *g) {
+/* src/ds/gvizGraph.c:96-129: prefix sums of the adjacency lengths */
+void gvizGraphBuildLayout(gvizGraph *g) {
+  if (!g)
+    return;
   size_t n = gvizGraphSize(g);
-  return calloc(n ? n : 1, 1);
+  if (!g->layout) {
+    g->layout = calloc(1, sizeof(gvizGraphLayout));
+    if (!g->layout)
+      return;
+  }
+  size_t *off = realloc(g->layout->vertexOffsets, (n + 1) * sizeof(size_t));
+  if (!off) {
+    free(g->layout->vertexOffsets);
+    free(g->layout);
+    g->layout = NULL;
+    return;
+  }
+  off[0] = 0;
+  for (size_t u = 0; u < n; u++)
+    off[u + 1] = off[u] + vertex_at(g, u)->neighbors.count;
+  g->layout->vertexOffsets = off;
+  g->layout->nvertices = n;
+  g->layout->edgeCount = off[n];
 }
 
-void gvizVertexSubsetShowVertex(gvizVertexSubset vs, size_t u) { vs[u] = 1; }
-void gvizVertexSubsetHideVertex(gvizVertexSubset vs, size_t u) { vs[u] = 0; }
+gvizArray *gvizGraphGetVertexNeighbors(const gvizGraph *g, size_t idx) {
+  gvizVertex *v = g ? vertex_at(g, idx) : NULL;
+  return v ? &v->neighbors : NULL;
+}
+
+void *gvizGraphGetVertexData(gvizGraph *g, size_t idx) {
+  gvizVertex *v = g ? vertex_at(g, idx) : NULL;
+  return v ? v->data : NULL;
+}
+
+void gvizGraphSetVertexData(gvizGraph *g, size_t idx, void *data) {
+  gvizVertex *v = g ? vertex_at(g, idx) : NULL;
+  if (v)
+    v->data = data;
+}
+
+const size_t *gvizGraphNeighbors(const gvizGraph *g, size_t u, size_t *count) {
+  gvizArray *nb = g ? gvizGraphGetVertexNeighbors(g, u) : NULL;
+  if (count)
+    *count = nb ? nb->count : 0;
+  return nb ? (const size_t *)nb->arr : NULL;
+}
+
+/* ------------------------------------------------------- vertex subsets */
+
+gvizVertexSubset gvizVertexSubsetCreateEmpty(const struct gvizGraph *g) {
+  return bits_alloc(gvizGraphSize(g));
+}
+void gvizVertexSubsetShowVertex(const gvizVertexSubset vs, size_t u) { gvizSetBit(vs, u); }
+void gvizVertexSubsetHideVertex(const gvizVertexSubset vs, size_t u) { gvizClearBit(vs, u); }
+void gvizVertexSubsetClearAll(const gvizVertexSubset vs, size_t nvertices) {
+  if (vs)
+    memset(vs, 0, GVIZ_ARRAY_UNITS(nvertices));
+}
+void gvizVertexSubsetRelease(const gvizVertexSubset vs) { free(vs); }
 bool gvizVertexSubsetTest(const gvizVertexSubset vs, size_t u) {
-  return vs[u] != 0;
+  return gvizTestBit(vs, u) != 0;
 }
 size_t gvizVertexSubsetCount(const gvizVertexSubset vs, size_t nvertices) {
-  size_t c = 0;
-  for (size_t i = 0; i < nvertices; i++)
-    c += vs[i] != 0;
-  return c;
+  return vs ? bits_count(vs, 0, nvertices) : 0;
 }
-void gvizVertexSubsetRelease(gvizVertexSubset vs) { free(vs); }
+void gvizVertexSubsetToBytes(const gvizVertexSubset vs, size_t n, uint8_t *out) {
+  for (size_t u = 0; u < n; u++)
+    out[u] = vs && gvizTestBit(vs, u) ? 1 : 0;
+}
 
-gvizSubgraph gvizSubgraphCreateFull(const gvizGraph *g) {
-  gvizSubgraph sg = {g, NULL};
+/* --------------------------------------------------------- edge subsets */
+
+gvizEdgeSubset gvizEdgeSubsetCreateEmpty(const struct gvizGraph *g) {
+  gvizEdgeSubset es = {NULL, NULL};
+  if (!g || !g->layout)
+    return es;
+  es.bitset = bits_alloc(g->layout->edgeCount);
+  es.layout = es.bitset ? g->layout : NULL;
+  return es;
+}
+void gvizEdgeSubsetShowEdge(const gvizEdgeSubset es, size_t u, size_t idx) {
+  gvizSetBit(es.bitset, es.layout->vertexOffsets[u] + idx);
+}
+void gvizEdgeSubsetHideEdge(const gvizEdgeSubset es, size_t u, size_t idx) {
+  gvizClearBit(es.bitset, es.layout->vertexOffsets[u] + idx);
+}
+bool gvizEdgeSubsetTestEdge(const gvizEdgeSubset es, size_t u, size_t idx) {
+  return gvizTestBit(es.bitset, es.layout->vertexOffsets[u] + idx) != 0;
+}
+void gvizEdgeSubsetRelease(const gvizEdgeSubset es) { free(es.bitset); }
+
+/* ------------------------------------------------------------ subgraphs */
+
+gvizSubgraph gvizSubgraphCreateEmpty(const struct gvizGraph *g) {
+  gvizSubgraph sg = {NULL, NULL, {NULL, NULL}, 0};
+  if (!g)
+    return sg;
+  sg.vs = gvizVertexSubsetCreateEmpty(g);
+  if (!sg.vs)
+    return sg;
+  sg.es = gvizEdgeSubsetCreateEmpty(g);
+  if (!sg.es.bitset) {
+    gvizVertexSubsetRelease(sg.vs);
+    sg.vs = NULL;
+    return sg;
+  }
+  sg.g = g;
+  sg.vertexCapacity = gvizGraphSize(g);
   return sg;
 }
 
-gvizSubgraph gvizSubgraphCreateVertexInduced(const gvizGraph *g,
-                                             gvizVertexSubset v) {
-  gvizSubgraph sg = {g, v};
+gvizSubgraph gvizSubgraphCreateFull(const struct gvizGraph *g) {
+  gvizSubgraph sg = {NULL, NULL, {NULL, NULL}, 0};
+  if (!g || !g->layout)
+    return sg;
+  sg = gvizSubgraphCreateEmpty(g);
+  if (!sg.vs)
+    return sg;
+  bits_fill(sg.vs, gvizGraphSize(g));
+  bits_fill(sg.es.bitset, g->layout->edgeCount);
+  return sg;
+}
+
+gvizSubgraph gvizSubgraphCreateVertexInduced(const struct gvizGraph *g,
+                                             gvizVertexSubset vs) {
+  gvizSubgraph sg = {g, vs, {NULL, NULL}, gvizGraphSize(g)};
   return sg;
 }
 
 void gvizSubgraphRelease(gvizSubgraph *sg) {
   if (!sg)
     return;
-  free(sg->vs);
-  sg->vs = NULL;
+  gvizVertexSubsetRelease(sg->vs);
+  gvizEdgeSubsetRelease(sg->es);
+  memset(sg, 0, sizeof(*sg));
+}
+
+bool gvizSubgraphIsFull(const gvizSubgraph *sg) {
+  return sg && sg->g && sg->vs && sg->es.bitset;
 }
 
 bool gvizSubgraphHasVertex(const gvizSubgraph *sg, size_t u) {
-  if (u >= gvizGraphSize(sg->g))
+  if (!sg || !sg->g || !sg->vs || u >= sg->vertexCapacity)
     return false;
-  return sg->vs ? sg->vs[u] != 0 : true;
+  return gvizVertexSubsetTest(sg->vs, u);
 }
 
 size_t gvizSubgraphVertexCount(const gvizSubgraph *sg) {
-  size_t n = gvizGraphSize(sg->g);
-  return sg->vs ? gvizVertexSubsetCount(sg->vs, n) : n;
+  if (!sg || !sg->g || !sg->vs)
+    return 0;
+  return gvizVertexSubsetCount(sg->vs, sg->vertexCapacity);
 }
 
-/* src/ds/gvizSubgraph.c:407-425: neighbours inside the subgraph */
+/* Whether adjacency entry idx of u is an edge of the subgraph: its edge bit
+ * in a full subgraph, its far end in the subset of an induced one. */
+static bool edge_kept(const gvizSubgraph *sg, size_t u, size_t idx, size_t v) {
+  if (gvizSubgraphIsFull(sg))
+    return gvizEdgeSubsetTestEdge(sg->es, u, idx);
+  return gvizSubgraphHasVertex(sg, v);
+}
+
+bool gvizSubgraphHasEdge(const gvizSubgraph *sg, size_t u, size_t v) {
+  if (!gvizSubgraphHasVertex(sg, u) || !gvizSubgraphHasVertex(sg, v))
+    return false;
+  size_t c = 0;
+  const size_t *nb = gvizGraphNeighbors(sg->g, u, &c);
+  for (size_t k = 0; k < c; k++)
+    if (nb[k] == v && edge_kept(sg, u, k, v))
+      return true;
+  return false;
+}
+
+/* src/ds/gvizSubgraph.c:407-425: adjacency entries inside the subgraph */
 size_t gvizSubgraphDegree(const gvizSubgraph *sg, size_t u) {
   if (!gvizSubgraphHasVertex(sg, u))
     return 0;
-  const gvizAdjList *l = &sg->g->adj[u];
-  if (!sg->vs)
-    return l->count;
-  size_t d = 0;
-  for (size_t k = 0; k < l->count; k++)
-    d += sg->vs[l->items[k]] != 0;
+  size_t c = 0, d = 0;
+  const size_t *nb = gvizGraphNeighbors(sg->g, u, &c);
+  for (size_t k = 0; k < c; k++)
+    d += edge_kept(sg, u, k, nb[k]) ? 1 : 0;
   return d;
 }
 
 size_t gvizSubgraphEdgeCount(const gvizSubgraph *sg) {
+  if (!sg || !sg->g)
+    return 0;
   size_t n = gvizGraphSize(sg->g), total = 0;
   for (size_t u = 0; u < n; u++)
     total += gvizSubgraphDegree(sg, u);
   return sg->g->directed ? total : total / 2;
 }
+
+void gvizSubgraphShowVertex(const gvizSubgraph *sg, size_t u) {
+  if (sg && sg->vs && u < sg->vertexCapacity)
+    gvizVertexSubsetShowVertex(sg->vs, u);
+}
+
+/* Hiding a vertex of a full subgraph also drops its incident edges. */
+void gvizSubgraphHideVertex(const gvizSubgraph *sg, size_t u) {
+  if (!sg || !sg->vs || u >= sg->vertexCapacity)
+    return;
+  gvizVertexSubsetHideVertex(sg->vs, u);
+  if (!gvizSubgraphIsFull(sg))
+    return;
+  size_t c = 0;
+  const size_t *nb = gvizGraphNeighbors(sg->g, u, &c);
+  for (size_t k = 0; k < c; k++) {
+    gvizEdgeSubsetHideEdge(sg->es, u, k);
+    size_t cv = 0;
+    const size_t *back = gvizGraphNeighbors(sg->g, nb[k], &cv);
+    for (size_t j = 0; j < cv; j++)
+      if (back[j] == u)
+        gvizEdgeSubsetHideEdge(sg->es, nb[k], j);
+  }
+}
+
+/* ----------------------------------------------------------------- CSR */
 
 int gvizSubgraphToCSR(const gvizSubgraph *sg, int64_t **offsets,
                       int64_t **adj) {
@@ -177,10 +455,11 @@ int gvizSubgraphToCSR(const gvizSubgraph *sg, int64_t **offsets,
   for (size_t u = 0; u < n; u++) {
     if (!gvizSubgraphHasVertex(sg, u))
       continue;
-    const gvizAdjList *l = &sg->g->adj[u];
-    for (size_t j = 0; j < l->count; j++)
-      if (!sg->vs || sg->vs[l->items[j]])
-        a[k++] = (int64_t)l->items[j];
+    size_t c = 0;
+    const size_t *nb = gvizGraphNeighbors(sg->g, u, &c);
+    for (size_t j = 0; j < c; j++)
+      if (edge_kept(sg, u, j, nb[j]))
+        a[k++] = (int64_t)nb[j];
   }
   *offsets = off;
   *adj = a;
@@ -227,10 +506,11 @@ int gvizSubgraphToCompactCSR(const gvizSubgraph *sg, int64_t **offsets,
   }
   size_t k = 0;
   for (size_t i = 0; i < m; i++) {
-    const gvizAdjList *l = &sg->g->adj[vert[i]];
-    for (size_t j = 0; j < l->count; j++)
-      if (index[l->items[j]] >= 0)
-        a[k++] = index[l->items[j]];
+    size_t cn = 0;
+    const size_t *nb = gvizGraphNeighbors(sg->g, vert[i], &cn);
+    for (size_t j = 0; j < cn; j++)
+      if (edge_kept(sg, vert[i], j, nb[j]))
+        a[k++] = index[nb[j]];
   }
   free(index);
   *offsets = off;

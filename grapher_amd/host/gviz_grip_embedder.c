@@ -8,14 +8,15 @@
  * refinement round run on the device (include/gviz_hip.h, GRIP section),
  * which keeps positions, per-vertex state and kNN lists in HBM. Positions
  * are uploaded before and read back after each device phase that uses
- * them, unless residentPositions is set (then gvizGRIPEmbedderSyncPositions
- * fetches them). Dimensions 2, 3 and 4 (the ones whose rotation removal
+ * them, unless resident positions are on (gvizGRIPEmbedderSetResidentPositions;
+ * then gvizGRIPEmbedderSyncPositions fetches them). Dimensions 2, 3 and 4 (the ones whose rotation removal
  * the reference defines, gvizGRIPEmbedder.c:1192-1261); Init returns -2
  * for another.
  */
 #include "gviz/gvizGRIPEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_side.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -34,7 +35,31 @@
 #define MIS_MAX_LAYERS 128
 
 static size_t border_at(const gvizGRIPState *s, size_t i) {
-  return s->misBorder[i];
+  return *(const size_t *)gvizArrayAtIndex(&s->misBorder, i);
+}
+
+/* The engine's extras beside the reference's state layout: the device
+ * state, the residency flags, and a byte-per-vertex copy of a vertex subset
+ * (the form the device takes). */
+typedef struct grip_side {
+  gvizhip_grip *device;
+  int residentPositions; /* 1: rounds keep positions on the device */
+  int devicePositionsFresh;
+  uint8_t *bytes;        /* n entries */
+} grip_side;
+
+static grip_side *side_of(const gvizGRIPState *s) {
+  return (grip_side *)gviz_side_get(s);
+}
+
+static const uint8_t *subset_bytes(gvizGRIPState *s, const gvizVertexSubset vs) {
+  grip_side *x = side_of(s);
+  gvizVertexSubsetToBytes(vs, gvizGraphSize(((gvizEmbeddedGraph *)s)->subgraph.g), x->bytes);
+  return x->bytes;
+}
+
+static const uint8_t *visible_bytes(gvizGRIPState *s) {
+  return subset_bytes(s, ((gvizEmbeddedGraph *)s)->drawMask.visibleVertices);
 }
 
 static size_t clamp_k(size_t k, size_t minK, size_t maxK) {
@@ -184,12 +209,24 @@ int gvizGRIPEmbedderInit(gvizGRIPState *s, gvizSubgraph subgraph,
   if (res < 0)
     return res;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
-  s->misFiltration = calloc(activeCount, sizeof(size_t));
-  s->misBorder = calloc(MIS_MAX_LAYERS, sizeof(size_t));
-  if (!s->misFiltration || !s->misBorder)
+  grip_side *x = calloc(1, sizeof(grip_side)), *prev = NULL;
+  if (!x || gviz_side_put(s, x, (void **)&prev) != 0) {
+    free(x);
     return -1;
-  s->misBorder[0] = activeCount;
-  s->misBorderCount = 1;
+  }
+  if (prev) { /* a state initialised again without Release */
+    if (prev->device)
+      gvizhip_grip_destroy(prev->device);
+    free(prev->bytes);
+    free(prev);
+  }
+  size_t nAll = gvizGraphSize(subgraph.g);
+  x->bytes = malloc(nAll ? nAll : 1);
+  s->misFiltration = calloc(activeCount ? activeCount : 1, sizeof(size_t));
+  if (!x->bytes || !s->misFiltration ||
+      gvizArrayInitAtCapacity(&s->misBorder, sizeof(size_t), MIS_MAX_LAYERS) != 0 ||
+      gvizArrayPush(&s->misBorder, &activeCount) != 0)
+    return -1;
   gvizEmbeddedGraphDrawMaskClearVertices(e);
 
   if (gvizEmbeddedGraphAddAction(e, "grip.refineRound", action_refine_round, NULL) < 0)
@@ -209,13 +246,16 @@ int gvizGRIPEmbedderInit(gvizGRIPState *s, gvizSubgraph subgraph,
 void gvizGRIPEmbedderRelease(gvizGRIPState *s) {
   if (!s)
     return;
-  if (s->device)
-    gvizhip_grip_destroy(s->device);
-  s->device = NULL;
+  grip_side *x = (grip_side *)gviz_side_take(s);
+  if (x) {
+    if (x->device)
+      gvizhip_grip_destroy(x->device);
+    free(x->bytes);
+    free(x);
+  }
   free(s->misFiltration);
-  free(s->misBorder);
+  gvizArrayRelease(&s->misBorder);
   s->misFiltration = NULL;
-  s->misBorder = NULL;
   gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
 }
 
@@ -258,24 +298,38 @@ static double *host_positions(gvizGRIPState *s) {
 }
 
 static int push_positions(gvizGRIPState *s) {
-  if (s->residentPositions && s->devicePositionsFresh)
+  grip_side *x = side_of(s);
+  if (x->residentPositions && x->devicePositionsFresh)
     return 0;
-  if (gvizhip_grip_upload(s->device, host_positions(s), NULL, NULL, NULL) != 0)
+  if (gvizhip_grip_upload(x->device, host_positions(s), NULL, NULL, NULL) != 0)
     return -1;
-  s->devicePositionsFresh = 1;
+  x->devicePositionsFresh = 1;
   return 0;
 }
 
 static int pull_positions(gvizGRIPState *s) {
-  if (s->residentPositions)
+  grip_side *x = side_of(s);
+  if (x->residentPositions)
     return 0;
-  return gvizhip_grip_download(s->device, host_positions(s), NULL, NULL, NULL);
+  return gvizhip_grip_download(x->device, host_positions(s), NULL, NULL, NULL);
 }
 
 int gvizGRIPEmbedderSyncPositions(gvizGRIPState *s) {
-  if (!s || !s->device)
+  grip_side *x = s ? side_of(s) : NULL;
+  if (!x || !x->device)
     return -1;
-  return gvizhip_grip_download(s->device, host_positions(s), NULL, NULL, NULL);
+  return gvizhip_grip_download(x->device, host_positions(s), NULL, NULL, NULL);
+}
+
+void gvizGRIPEmbedderSetResidentPositions(gvizGRIPState *s, int enabled) {
+  grip_side *x = s ? side_of(s) : NULL;
+  if (!x)
+    return;
+  if (x->residentPositions && !enabled && x->device && s->layerCount > 0)
+    gvizGRIPEmbedderSyncPositions(s);
+  if (enabled && !x->residentPositions)
+    x->devicePositionsFresh = 0;
+  x->residentPositions = enabled;
 }
 
 /* Binary checkpoint: magic, n, kNN capacity, layer count, current layer and
@@ -286,19 +340,20 @@ static void sync_draw_mask(gvizGRIPState *s);
 static const char kGripStateMagic[8] = {'G', 'V', 'Z', 'G', 'R', 'S', '0', '2'};
 
 int gvizGRIPEmbedderSaveState(gvizGRIPState *s, const char *filename) {
-  if (!s || !s->device || !filename || s->layerCount == 0)
+  grip_side *x = s ? side_of(s) : NULL;
+  if (!x || !x->device || !filename || s->layerCount == 0)
     return -1;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   const int64_t n = (int64_t)gvizGraphSize(e->subgraph.g);
   const int64_t kcap = (int64_t)s->knnCapacity, active = (int64_t)border_at(s, 0);
   const int64_t dim = (int64_t)e->embedding.dim;
-  if (s->residentPositions && gvizGRIPEmbedderSyncPositions(s) != 0)
+  if (x->residentPositions && gvizGRIPEmbedderSyncPositions(s) != 0)
     return -1;
   gvizhip_grip_vertex_nd *dec =
       malloc(sizeof(gvizhip_grip_vertex_nd) * (size_t)(n ? n : 1));
-  int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
-  int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
-  int ok = dec && kv && kd && gvizhip_grip_download_nd(s->device, NULL, dec, kv, kd) == 0;
+  int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap > 0 ? n * kcap : 1));
+  int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap > 0 ? n * kcap : 1));
+  int ok = dec && kv && kd && gvizhip_grip_download_nd(x->device, NULL, dec, kv, kd) == 0;
   FILE *f = ok ? fopen(filename, "wb") : NULL;
   ok = ok && f;
   if (ok) {
@@ -312,7 +367,7 @@ int gvizGRIPEmbedderSaveState(gvizGRIPState *s, const char *filename) {
       const int64_t v = (int64_t)s->misFiltration[i];
       ok = fwrite(&v, 8, 1, f) == 1;
     }
-    ok = ok && fwrite(e->visibleVertices, 1, (size_t)n, f) == (size_t)n &&
+    ok = ok && fwrite(visible_bytes(s), 1, (size_t)n, f) == (size_t)n &&
          fwrite(host_positions(s), sizeof(double), (size_t)(dim * n), f) ==
              (size_t)(dim * n) &&
          fwrite(dec, sizeof(gvizhip_grip_vertex_nd), (size_t)n, f) == (size_t)n &&
@@ -328,7 +383,8 @@ int gvizGRIPEmbedderSaveState(gvizGRIPState *s, const char *filename) {
 }
 
 int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
-  if (!s || !s->device || !filename || s->layerCount == 0)
+  grip_side *x = s ? side_of(s) : NULL;
+  if (!x || !x->device || !filename || s->layerCount == 0)
     return -1;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   const int64_t n = (int64_t)gvizGraphSize(e->subgraph.g);
@@ -351,11 +407,11 @@ int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
   }
   const int64_t kcap = ok ? head[1] : 0;
   uint8_t *vis = malloc((size_t)(n ? n : 1));
-  double *pos = malloc(sizeof(double) * (size_t)(dim * n ? dim * n : 1));
+  double *pos = malloc(sizeof(double) * (size_t)(dim * n > 0 ? dim * n : 1));
   gvizhip_grip_vertex_nd *dec =
       malloc(sizeof(gvizhip_grip_vertex_nd) * (size_t)(n ? n : 1));
-  int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
-  int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap ? n * kcap : 1));
+  int64_t *kv = malloc(sizeof(int64_t) * (size_t)(n * kcap > 0 ? n * kcap : 1));
+  int64_t *kd = malloc(sizeof(int64_t) * (size_t)(n * kcap > 0 ? n * kcap : 1));
   ok = ok && vis && pos && dec && kv && kd &&
        fread(vis, 1, (size_t)n, f) == (size_t)n &&
        fread(pos, sizeof(double), (size_t)(dim * n), f) == (size_t)(dim * n) &&
@@ -365,17 +421,22 @@ int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
   fclose(f);
   if (ok) {
     memcpy(host_positions(s), pos, sizeof(double) * (size_t)(dim * n));
-    ok = gvizhip_grip_upload_nd(s->device, pos, dec, kv, kd) == 0;
+    ok = gvizhip_grip_upload_nd(x->device, pos, dec, kv, kd) == 0;
   }
   if (ok) {
-    memcpy(e->visibleVertices, vis, (size_t)n);
+    for (int64_t u = 0; u < n; u++) {
+      if (vis[u])
+        gvizVertexSubsetShowVertex(e->drawMask.visibleVertices, (size_t)u);
+      else
+        gvizVertexSubsetHideVertex(e->drawMask.visibleVertices, (size_t)u);
+    }
     gvizEmbeddedGraphDrawMaskNotifyChanged(e);
     s->currLayer = (size_t)head[3];
     s->currRound = (size_t)head[4];
     s->lastRoundStats.maxDisplacement = st[0];
     s->lastRoundStats.meanDisplacement = st[1];
     s->lastRoundStats.meanForce = st[2];
-    s->devicePositionsFresh = 1;
+    x->devicePositionsFresh = 1;
     sync_draw_mask(s);
   }
   free(vis);
@@ -391,13 +452,14 @@ int gvizGRIPEmbedderLoadState(gvizGRIPState *s, const char *filename) {
 static int refresh_layer(gvizGRIPState *s) {
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   size_t end = border_at(s, s->currLayer);
-  if (gvizhip_grip_clear_decorators(s->device, (int64_t)end) != 0)
+  grip_side *x = side_of(s);
+  if (gvizhip_grip_clear_decorators(x->device, (int64_t)end) != 0)
     return -1;
   size_t n = gvizGraphSize(e->subgraph.g);
-  size_t visibleCount = gvizVertexSubsetCount(e->visibleVertices, n);
+  size_t visibleCount = gvizVertexSubsetCount(e->drawMask.visibleVertices, n);
   int batch = end > 0 && visibleCount > 0 && visibleCount < end &&
               prefer_batch(s, visibleCount, end);
-  return gvizhip_grip_update_knn(s->device, (int64_t)end, e->visibleVertices,
+  return gvizhip_grip_update_knn(x->device, (int64_t)end, visible_bytes(s),
                                  (int64_t)refinement_k(s), batch);
 }
 
@@ -409,23 +471,29 @@ static void sync_draw_mask(gvizGRIPState *s) {
 }
 
 static int ensure_device(gvizGRIPState *s) {
-  if (s->device)
+  grip_side *x = side_of(s);
+  if (!x)
+    return -1;
+  if (x->device)
     return 0;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   int64_t *off = NULL, *adj = NULL;
   if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
     return -1;
-  int rc = gvizhip_grip_create(&s->device, (int64_t)gvizGraphSize(e->subgraph.g),
-                               off, adj, e->subgraph.vs, (int64_t)s->knnCapacity);
+  size_t n = gvizGraphSize(e->subgraph.g);
+  for (size_t u = 0; u < n; u++)
+    x->bytes[u] = gvizSubgraphHasVertex(&e->subgraph, u) ? 1 : 0;
+  int rc = gvizhip_grip_create(&x->device, (int64_t)n, off, adj, x->bytes,
+                               (int64_t)s->knnCapacity);
   free(off);
   free(adj);
   if (rc != 0) {
-    s->device = NULL;
+    x->device = NULL;
     return -1;
   }
-  if (gvizhip_grip_set_dimension(s->device, (int64_t)e->embedding.dim) != 0) {
-    gvizhip_grip_destroy(s->device);
-    s->device = NULL;
+  if (gvizhip_grip_set_dimension(x->device, (int64_t)e->embedding.dim) != 0) {
+    gvizhip_grip_destroy(x->device);
+    x->device = NULL;
     return -1;
   }
   return 0;
@@ -444,7 +512,8 @@ void gvizGRIPEmbedderBegin(gvizGRIPState *s) {
   int64_t borders[MIS_MAX_LAYERS];
   if (!filt)
     return;
-  int64_t layers = gvizhip_grip_mis_filtration(s->device, (int64_t)e->embedding.dim,
+  grip_side *x = side_of(s);
+  int64_t layers = gvizhip_grip_mis_filtration(x->device, (int64_t)e->embedding.dim,
                                                filt, borders);
   if (layers <= 0 || layers > MIS_MAX_LAYERS) {
     free(filt);
@@ -454,9 +523,11 @@ void gvizGRIPEmbedderBegin(gvizGRIPState *s) {
   for (size_t i = 0; i < active; i++)
     s->misFiltration[i] = (size_t)filt[i];
   free(filt);
-  for (int64_t i = 0; i < layers; i++)
-    s->misBorder[i] = (size_t)borders[i];
-  s->misBorderCount = (size_t)layers;
+  s->misBorder.count = 0;
+  for (int64_t i = 0; i < layers; i++) {
+    size_t b = (size_t)borders[i];
+    gvizArrayPush(&s->misBorder, &b);
+  }
   s->layerCount = (size_t)layers;
   s->currLayer = s->layerCount - 1;
   s->currRound = 0;
@@ -468,7 +539,7 @@ void gvizGRIPEmbedderBegin(gvizGRIPState *s) {
     gvizEmbeddedGraphDrawMaskShowVertex(e, s->misFiltration[j]);
     gvizEmbeddedGraphSetVPosition(e, s->misFiltration[j], simplex + j * dim);
   }
-  s->devicePositionsFresh = 0;
+  x->devicePositionsFresh = 0;
   if (push_positions(s) != 0 || refresh_layer(s) != 0) {
     s->layerCount = 0;
     return;
@@ -489,14 +560,14 @@ void gvizGRIPEmbedderNextStage(gvizGRIPState *s) {
   size_t end = border_at(s, s->currLayer);
   size_t placeCount = end - begin;
   size_t n = gvizGraphSize(e->subgraph.g);
-  size_t visibleCount = gvizVertexSubsetCount(e->visibleVertices, n);
+  size_t visibleCount = gvizVertexSubsetCount(e->drawMask.visibleVertices, n);
   int batch = placeCount > 0 && visibleCount > 0 && visibleCount < placeCount &&
               prefer_batch(s, visibleCount, placeCount);
   if (push_positions(s) != 0)
     return;
   if (placeCount > 0 &&
-      gvizhip_grip_place(s->device, (int64_t)begin, (int64_t)end,
-                         e->visibleVertices, (int64_t)placement_k(s), batch) != 0)
+      gvizhip_grip_place(side_of(s)->device, (int64_t)begin, (int64_t)end,
+                         visible_bytes(s), (int64_t)placement_k(s), batch) != 0)
     return;
   for (size_t i = begin; i < end; i++)
     gvizEmbeddedGraphDrawMaskShowVertex(e, s->misFiltration[i]);
@@ -516,7 +587,8 @@ void gvizGRIPEmbedderRefineRound(gvizGRIPState *s) {
   if (push_positions(s) != 0)
     return;
   gvizhip_grip_round_stats st;
-  if (gvizhip_grip_refine_round(s->device, (int64_t)active, (int64_t)layer, &st) != 0)
+  if (gvizhip_grip_refine_round(side_of(s)->device, (int64_t)active, (int64_t)layer,
+                                &st) != 0)
     return;
   if (pull_positions(s) != 0)
     return;

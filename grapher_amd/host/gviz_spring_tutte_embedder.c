@@ -11,11 +11,67 @@
 #include "gviz/gvizSpringTutteEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_side.h"
 #include "gviz_planar.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+
+/* The engine's extras beside the reference's state layout: the device
+ * engine, its precision, whether its positions match the host's, and a
+ * byte-per-vertex copy of the boundary bits (the form the device takes). */
+typedef struct spring_side {
+  gvizhip_tutte *device;
+  int precision; /* GVIZHIP_PRECISION_FP64 (default) or _FP32 */
+  int fresh;
+  uint8_t *bytes;
+} spring_side;
+
+static spring_side *side_of(const gvizSpringTutteState *s) {
+  return (spring_side *)gviz_side_get(s);
+}
+
+static void mark_stale(gvizSpringTutteState *s) {
+  spring_side *x = side_of(s);
+  if (x)
+    x->fresh = 0;
+}
+
+static const uint8_t *boundary_bytes(gvizSpringTutteState *s) {
+  spring_side *x = side_of(s);
+  gvizVertexSubsetToBytes(s->isBoundary,
+                          gvizGraphSize(((gvizEmbeddedGraph *)s)->subgraph.g), x->bytes);
+  return x->bytes;
+}
+
+static int attach_side(gvizSpringTutteState *s, size_t n) {
+  spring_side *x = calloc(1, sizeof(spring_side)), *prev = NULL;
+  if (!x || gviz_side_put(s, x, (void **)&prev) != 0) {
+    free(x);
+    return -1;
+  }
+  if (prev) { /* a state initialised again without Release */
+    if (prev->device)
+      gvizhip_tutte_destroy(prev->device);
+    free(prev->bytes);
+    free(prev);
+  }
+  x->precision = GVIZHIP_PRECISION_FP64;
+  x->bytes = malloc(n ? n : 1);
+  return x->bytes ? 0 : -1;
+}
+
+static void detach_side(gvizSpringTutteState *s) {
+  spring_side *x = (spring_side *)gviz_side_take(s);
+  if (!x)
+    return;
+  if (x->device)
+    gvizhip_tutte_destroy(x->device);
+  free(x->bytes);
+  free(x);
+}
 
 static size_t num_vertices(const gvizSpringTutteState *s) {
   return gvizGraphSize(((const gvizEmbeddedGraph *)s)->subgraph.g);
@@ -50,14 +106,13 @@ int gvizSpringTutteEmbedderInit(gvizSpringTutteState *s, gvizSubgraph subgraph,
   if (res < 0)
     return res;
   const size_t n = gvizGraphSize(subgraph.g);
-  s->isBoundary = calloc(n ? n : 1, 1);
-  s->velocity = calloc(n * dimension ? n * dimension : 1, sizeof(double));
-  if (!s->isBoundary || !s->velocity)
+  s->isBoundary = gvizVertexSubsetCreateEmpty(subgraph.g);
+  s->velocity = calloc(n * dimension > 0 ? n * dimension : 1, sizeof(double));
+  if (!s->isBoundary || !s->velocity || attach_side(s, n) != 0)
     return -1;
   s->epsilon = epsilon > 0.0 ? epsilon : GVIZ_SPRING_TUTTE_DEFAULT_EPSILON;
   s->stiffness = GVIZ_SPRING_TUTTE_DEFAULT_STIFFNESS;
   s->damping = GVIZ_SPRING_TUTTE_DEFAULT_DAMPING;
-  s->precision = GVIZHIP_PRECISION_FP64;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   if (gvizEmbeddedGraphAddAction(e, "springTutte.step", action_step, NULL) < 0)
     return -1;
@@ -72,15 +127,15 @@ int gvizSpringTutteEmbedderInit(gvizSpringTutteState *s, gvizSubgraph subgraph,
 void gvizSpringTutteEmbedderRelease(gvizSpringTutteState *s) {
   if (!s)
     return;
-  if (s->device)
-    gvizhip_tutte_destroy(s->device);
-  s->device = NULL;
+  detach_side(s);
   free(s->boundary);
-  free(s->isBoundary);
+  gvizVertexSubsetRelease(s->isBoundary);
   free(s->velocity);
+  free(s->scratchPos);
   s->boundary = NULL;
   s->isBoundary = NULL;
   s->velocity = NULL;
+  s->scratchPos = NULL;
   gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
 }
 
@@ -139,7 +194,7 @@ int gvizSpringTutteEmbedderSetBoundary(gvizSpringTutteState *s, const size_t *bo
   for (size_t i = 0; i < count; i++)
     if (boundary[i] >= n)
       return -1;
-  memset(s->isBoundary, 0, n);
+  gvizVertexSubsetClearAll(s->isBoundary, n);
   free(s->boundary);
   s->boundary = malloc(sizeof(size_t) * count);
   if (!s->boundary) {
@@ -150,11 +205,11 @@ int gvizSpringTutteEmbedderSetBoundary(gvizSpringTutteState *s, const size_t *bo
   s->boundaryCount = count;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   for (size_t i = 0; i < count; i++) {
-    s->isBoundary[boundary[i]] = 1;
+    gvizSetBit(s->isBoundary, boundary[i]);
     gvizEmbeddedGraphSetVPosition(e, boundary[i], positions + i * d);
     memset(s->velocity + boundary[i] * d, 0, sizeof(double) * d);
   }
-  s->deviceFresh = 0;
+  mark_stale(s);
   return 0;
 }
 
@@ -175,14 +230,14 @@ void gvizSpringTutteEmbedderSeedInterior(gvizSpringTutteState *s) {
     for (size_t k = 0; k < d; k++)
       centroid[k] /= (double)s->boundaryCount;
   for (size_t u = 0; u < n; u++)
-    if (!s->isBoundary[u])
+    if (!gvizTestBit(s->isBoundary, u))
       gvizEmbeddedGraphSetVPosition(e, u, centroid);
   free(centroid);
   memset(s->velocity, 0, sizeof(double) * n * d);
   s->iteration = 0;
   s->lastMaxDelta = 0.0;
   s->converged = 0;
-  s->deviceFresh = 0;
+  mark_stale(s);
 }
 
 /* src/embedders/gvizSpringTutteEmbedder.c:252-270 */
@@ -209,33 +264,36 @@ static int ensure_device(gvizSpringTutteState *s) {
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   if (dim_of(s) != 2)
     return -1;
-  if (!s->device) {
+  spring_side *x = side_of(s);
+  if (!x)
+    return -1;
+  if (!x->device) {
     int64_t *off = NULL, *adj = NULL;
     if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
       return -1;
-    int rc = gvizhip_tutte_create(&s->device, (int64_t)num_vertices(s), off, adj,
-                                  s->precision);
+    int rc = gvizhip_tutte_create(&x->device, (int64_t)num_vertices(s), off, adj,
+                                  x->precision);
     free(off);
     free(adj);
     if (rc != 0) {
-      s->device = NULL;
+      x->device = NULL;
       return -1;
     }
-    s->deviceFresh = 0;
+    x->fresh = 0;
   }
-  if (!s->deviceFresh) {
-    if (gvizhip_tutte_upload(s->device, e->embedding.vertexPositions, s->isBoundary) != 0 ||
-        gvizhip_tutte_set_velocity(s->device, s->velocity) != 0)
+  if (!x->fresh) {
+    if (gvizhip_tutte_upload(x->device, e->embedding.vertexPositions, boundary_bytes(s)) != 0 ||
+        gvizhip_tutte_set_velocity(x->device, s->velocity) != 0)
       return -1;
-    s->deviceFresh = 1;
+    x->fresh = 1;
   }
   return 0;
 }
 
 static int read_back(gvizSpringTutteState *s) {
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
-  if (gvizhip_tutte_download(s->device, e->embedding.vertexPositions) != 0 ||
-      gvizhip_tutte_get_velocity(s->device, s->velocity) != 0)
+  if (gvizhip_tutte_download(side_of(s)->device, e->embedding.vertexPositions) != 0 ||
+      gvizhip_tutte_get_velocity(side_of(s)->device, s->velocity) != 0)
     return -1;
   return 0;
 }
@@ -254,11 +312,11 @@ static void record_step(gvizSpringTutteState *s, double maxDelta) {
 double gvizSpringTutteEmbedderStep(gvizSpringTutteState *s, double dt) {
   if (dt <= 0.0)
     return 0.0;
-  s->deviceFresh = 0;
+  mark_stale(s);
   if (ensure_device(s) != 0)
     return -1.0;
   double maxDelta = 0.0;
-  if (gvizhip_tutte_spring_steps(s->device, 1, dt, s->stiffness, s->damping, -1.0,
+  if (gvizhip_tutte_spring_steps(side_of(s)->device, 1, dt, s->stiffness, s->damping, -1.0,
                                  &maxDelta) != 1)
     return -1.0;
   if (read_back(s) != 0)
@@ -280,7 +338,7 @@ int gvizSpringTutteEmbedderRun(gvizSpringTutteState *s, size_t maxIters, double 
      * the call returns without stepping */
     return (int)s->iteration;
   }
-  s->deviceFresh = 0;
+  mark_stale(s);
   if (ensure_device(s) != 0)
     return -1;
   enum { kChunk = 65536 };
@@ -291,7 +349,7 @@ int gvizSpringTutteEmbedderRun(gvizSpringTutteState *s, size_t maxIters, double 
   while (!s->converged && s->iteration < maxIters) {
     const size_t left = maxIters - s->iteration;
     const int64_t want = (int64_t)(left < kChunk ? left : kChunk);
-    const int64_t ran = gvizhip_tutte_spring_steps(s->device, want, dt, s->stiffness,
+    const int64_t ran = gvizhip_tutte_spring_steps(side_of(s)->device, want, dt, s->stiffness,
                                                    s->damping, s->epsilon, deltas);
     if (ran <= 0) {
       failed = 1;

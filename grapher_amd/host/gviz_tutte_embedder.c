@@ -14,11 +14,67 @@
 #include "gviz/gvizTutteEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_side.h"
 #include "gviz_planar.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+
+/* The engine's extras beside the reference's state layout: the device
+ * engine, its precision, whether its positions match the host's, and a
+ * byte-per-vertex copy of the boundary bits (the form the device takes). */
+typedef struct tutte_side {
+  gvizhip_tutte *device;
+  int precision; /* GVIZHIP_PRECISION_FP64 (default) or _FP32 */
+  int fresh;
+  uint8_t *bytes;
+} tutte_side;
+
+static tutte_side *side_of(const gvizTutteState *s) {
+  return (tutte_side *)gviz_side_get(s);
+}
+
+static void mark_stale(gvizTutteState *s) {
+  tutte_side *x = side_of(s);
+  if (x)
+    x->fresh = 0;
+}
+
+static const uint8_t *boundary_bytes(gvizTutteState *s) {
+  tutte_side *x = side_of(s);
+  gvizVertexSubsetToBytes(s->isBoundary,
+                          gvizGraphSize(((gvizEmbeddedGraph *)s)->subgraph.g), x->bytes);
+  return x->bytes;
+}
+
+static int attach_side(gvizTutteState *s, size_t n) {
+  tutte_side *x = calloc(1, sizeof(tutte_side)), *prev = NULL;
+  if (!x || gviz_side_put(s, x, (void **)&prev) != 0) {
+    free(x);
+    return -1;
+  }
+  if (prev) { /* a state initialised again without Release */
+    if (prev->device)
+      gvizhip_tutte_destroy(prev->device);
+    free(prev->bytes);
+    free(prev);
+  }
+  x->precision = GVIZHIP_PRECISION_FP64;
+  x->bytes = malloc(n ? n : 1);
+  return x->bytes ? 0 : -1;
+}
+
+static void detach_side(gvizTutteState *s) {
+  tutte_side *x = (tutte_side *)gviz_side_take(s);
+  if (!x)
+    return;
+  if (x->device)
+    gvizhip_tutte_destroy(x->device);
+  free(x->bytes);
+  free(x);
+}
 
 static size_t num_vertices(const gvizTutteState *s) {
   return gvizGraphSize(((const gvizEmbeddedGraph *)s)->subgraph.g);
@@ -53,12 +109,11 @@ int gvizTutteEmbedderInit(gvizTutteState *s, gvizSubgraph subgraph,
   if (res < 0)
     return res;
   size_t n = gvizGraphSize(subgraph.g);
-  s->isBoundary = calloc(n ? n : 1, 1);
-  if (!s->isBoundary)
+  s->isBoundary = gvizVertexSubsetCreateEmpty(subgraph.g);
+  if (!s->isBoundary || attach_side(s, n) != 0)
     return -1;
   s->epsilon = epsilon > 0.0 ? epsilon : GVIZ_TUTTE_DEFAULT_EPSILON;
   s->relaxationRate = 5.0;
-  s->precision = GVIZHIP_PRECISION_FP64;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   if (gvizEmbeddedGraphAddAction(e, "tutte.step", action_step, NULL) < 0)
     return -1;
@@ -73,13 +128,13 @@ int gvizTutteEmbedderInit(gvizTutteState *s, gvizSubgraph subgraph,
 void gvizTutteEmbedderRelease(gvizTutteState *s) {
   if (!s)
     return;
-  if (s->device)
-    gvizhip_tutte_destroy(s->device);
-  s->device = NULL;
+  detach_side(s);
   free(s->boundary);
-  free(s->isBoundary);
+  gvizVertexSubsetRelease(s->isBoundary);
+  free(s->scratch);
   s->boundary = NULL;
   s->isBoundary = NULL;
+  s->scratch = NULL;
   gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
 }
 
@@ -128,7 +183,7 @@ int gvizTutteEmbedderSetBoundary(gvizTutteState *s, const size_t *boundary,
   for (size_t i = 0; i < count; i++)
     if (boundary[i] >= n)
       return -1;
-  memset(s->isBoundary, 0, n);
+  gvizVertexSubsetClearAll(s->isBoundary, n);
   free(s->boundary);
   s->boundary = malloc(sizeof(size_t) * count);
   if (!s->boundary) {
@@ -139,10 +194,10 @@ int gvizTutteEmbedderSetBoundary(gvizTutteState *s, const size_t *boundary,
   s->boundaryCount = count;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   for (size_t i = 0; i < count; i++) {
-    s->isBoundary[boundary[i]] = 1;
+    gvizSetBit(s->isBoundary, boundary[i]);
     gvizEmbeddedGraphSetVPosition(e, boundary[i], positions + i * d);
   }
-  s->devicePositionsFresh = 0;
+  mark_stale(s);
   return 0;
 }
 
@@ -163,13 +218,13 @@ void gvizTutteEmbedderSeedInterior(gvizTutteState *s) {
     for (size_t k = 0; k < d; k++)
       centroid[k] /= (double)s->boundaryCount;
   for (size_t u = 0; u < n; u++)
-    if (!s->isBoundary[u])
+    if (!gvizTestBit(s->isBoundary, u))
       gvizEmbeddedGraphSetVPosition(e, u, centroid);
   free(centroid);
   s->iteration = 0;
   s->lastMaxDelta = 0.0;
   s->converged = 0;
-  s->devicePositionsFresh = 0;
+  mark_stale(s);
 }
 
 /* src/embedders/gvizTutteEmbedder.c:232-249 */
@@ -195,25 +250,28 @@ static int ensure_device(gvizTutteState *s) {
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
   if (dim_of(s) != 2)
     return -1;
-  if (!s->device) {
+  tutte_side *x = side_of(s);
+  if (!x)
+    return -1;
+  if (!x->device) {
     int64_t *off = NULL, *adj = NULL;
     if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
       return -1;
-    int rc = gvizhip_tutte_create(&s->device, (int64_t)num_vertices(s), off,
-                                  adj, s->precision);
+    int rc = gvizhip_tutte_create(&x->device, (int64_t)num_vertices(s), off,
+                                  adj, x->precision);
     free(off);
     free(adj);
     if (rc != 0) {
-      s->device = NULL;
+      x->device = NULL;
       return -1;
     }
-    s->devicePositionsFresh = 0;
+    x->fresh = 0;
   }
-  if (!s->devicePositionsFresh) {
-    if (gvizhip_tutte_upload(s->device, e->embedding.vertexPositions,
-                             s->isBoundary) != 0)
+  if (!x->fresh) {
+    if (gvizhip_tutte_upload(x->device, e->embedding.vertexPositions,
+                             boundary_bytes(s)) != 0)
       return -1;
-    s->devicePositionsFresh = 1;
+    x->fresh = 1;
   }
   return 0;
 }
@@ -237,13 +295,13 @@ double gvizTutteEmbedderStep(gvizTutteState *s, double dt) {
     alpha = 1.0;
   if (s->useGaussSeidel)
     return -1.0;
-  s->devicePositionsFresh = 0;
+  mark_stale(s);
   if (ensure_device(s) != 0)
     return -1.0;
   double maxDelta = 0.0;
-  if (gvizhip_tutte_sweeps(s->device, 1, alpha, -1.0, &maxDelta) != 1)
+  if (gvizhip_tutte_sweeps(side_of(s)->device, 1, alpha, -1.0, &maxDelta) != 1)
     return -1.0;
-  if (gvizhip_tutte_download(s->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
+  if (gvizhip_tutte_download(side_of(s)->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
     return -1.0;
   record_sweep(s, maxDelta);
   return maxDelta;
@@ -264,7 +322,7 @@ int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters) {
     alpha = 1.0;
   if (s->useGaussSeidel)
     return -1;
-  s->devicePositionsFresh = 0;
+  mark_stale(s);
   if (ensure_device(s) != 0)
     return -1;
   enum { kChunk = 65536 };
@@ -275,7 +333,7 @@ int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters) {
   while (!s->converged && s->iteration < maxIters) {
     size_t left = maxIters - s->iteration;
     int64_t want = (int64_t)(left < kChunk ? left : kChunk);
-    int64_t ran = gvizhip_tutte_sweeps(s->device, want, alpha, s->epsilon, deltas);
+    int64_t ran = gvizhip_tutte_sweeps(side_of(s)->device, want, alpha, s->epsilon, deltas);
     if (ran <= 0) {
       failed = 1;
       break;
@@ -284,7 +342,7 @@ int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters) {
       record_sweep(s, deltas[i]);
   }
   free(deltas);
-  if (gvizhip_tutte_download(s->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
+  if (gvizhip_tutte_download(side_of(s)->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
     failed = 1;
   return failed ? -1 : (int)s->iteration;
 }

@@ -38,6 +38,12 @@ typedef struct gvizAction {
   void *userData;
 } gvizAction;
 
+typedef struct gvizActionRegistry {
+  gvizAction *actions;
+  size_t count;
+  size_t capacity;
+} gvizActionRegistry;
+
 typedef enum gvizStatChartKind {
   GVIZ_STAT_CHART_LINE = 0,
   GVIZ_STAT_CHART_LINE_LOG = 1,
@@ -52,22 +58,32 @@ typedef struct gvizStatSeries {
   uint64_t revision;
 } gvizStatSeries;
 
+typedef struct gvizStatRegistry {
+  gvizStatSeries *series;
+  size_t count;
+  size_t capacity;
+} gvizStatRegistry;
+
 typedef enum gvizDrawEdgePolicy {
   GVIZ_DRAW_EDGES_ALL = 0,
   GVIZ_DRAW_EDGES_NONE = 1,
   GVIZ_DRAW_EDGES_IF_BOTH_VISIBLE = 2,
 } gvizDrawEdgePolicy;
 
-typedef struct gvizEmbeddedGraph {
-  gvizSubgraph subgraph;
-  gvizEmbedding embedding;
-  gvizAction *actions;
-  size_t actionCount, actionCapacity;
-  gvizStatSeries *stats;
-  size_t statCount, statCapacity;
-  gvizVertexSubset visibleVertices; /* draw mask */
+typedef struct gvizDrawMask {
+  gvizVertexSubset visibleVertices; /* owned; all subgraph vertices after Init */
   gvizDrawEdgePolicy edgePolicy;
-  uint64_t drawMaskRevision;
+  uint64_t revision; /* bumped on reset, policy change and notification */
+} gvizDrawMask;
+
+typedef struct gvizEmbeddedGraph {
+  gvizSubgraph subgraph;      /* owned */
+  gvizEmbedding embedding;
+  gvizActionRegistry actions;
+  gvizDrawMask drawMask;
+  gvizStatRegistry stats;
+  int planarEmbedded;         /* 0 here: Begin keeps the insertion order */
+  gvizSubgraph highlight;     /* a view the front end highlights; empty here */
 } gvizEmbeddedGraph;
 
 int gvizEmbeddedGraphInit(gvizEmbeddedGraph *e, gvizSubgraph subgraph,
@@ -133,6 +149,7 @@ int gvizEmbeddedGraphSaveEmbedding(gvizEmbeddedGraph *e, const char *name,
                                    const char *filename);
 int gvizEmbeddedGraphLoadEmbedding(gvizEmbeddedGraph *e,
                                    const char *filename);
+int gvizEmbeddedGraphIsPlanarEmbedded(const gvizEmbeddedGraph *e);
 
 #ifdef __cplusplus
 }

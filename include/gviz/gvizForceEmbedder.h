@@ -9,6 +9,7 @@
 #define GVIZ_HOST_FORCE_EMBEDDER_H
 
 #include "gvizEmbeddedGraph.h"
+#include "gvizQuadtree.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -51,14 +52,20 @@ typedef enum gvizForcePrecision {
   GVIZ_FORCE_PRECISION_MIXED = 2,
 } gvizForcePrecision;
 
-struct gvizhip_force;
-
+/* The reference's layout (include/embedders/gvizForceEmbedder.h:42-138),
+ * field for field, so callers compiled against the reference's headers read
+ * the same offsets. The gfx950 engine's handle, precision and residency
+ * flag live beside the state (keyed by its address), set through
+ * gvizForceEmbedderSetPrecision / SetResidentPositions. The per-vertex
+ * force buffers of the reference (disp, attForceMag, repForceMag,
+ * structForce, oldStructForce, appliedDisp, swinging, traction) stay on the
+ * device and are NULL here (gvizhip_force_evaluate reads them); the host
+ * quadtree stays empty and `pool` NULL. */
 typedef struct gvizForceEmbedderState {
   gvizEmbeddedGraph graph; /* MUST be first */
   size_t *vertices;        /* compact index i -> parent id (ascending) */
   size_t vertexCount;
   const gvizForceModel *model;
-  gvizForceModelKind modelKind;
   double *mass;            /* vertexCount */
   size_t *degree;          /* vertexCount, subgraph degree */
   double radiusBase;
@@ -66,6 +73,15 @@ typedef struct gvizForceEmbedderState {
   double overlapConstant;
   int preventOverlap;
   double meanDegreePlusOne;
+  double *disp;
+  double *positionsScratch; /* compact positions, 2 * vertexCount */
+  double *attForceMag;
+  double *repForceMag;
+  double *structForce;
+  double *oldStructForce;
+  double *appliedDisp;
+  double *swinging;
+  double *traction;
   double jitterTolerance;
   double globalSpeed;      /* mirrored from the device after each Step/Run */
   double speedEfficiency;
@@ -78,12 +94,9 @@ typedef struct gvizForceEmbedderState {
   double theta;
   size_t nodesPerCell;
   int barnesHutEnabled;
-  /* gfx950 engine */
-  gvizForcePrecision precision; /* default MIXED */
-  int residentPositions;        /* 1: positions stay on the device */
-  int64_t *csrOffsets, *csrAdj; /* compact CSR handed to the device */
-  double *positionsScratch;     /* compact positions, 2 * vertexCount */
-  void *device;                 /* opaque engine handle (NULL before Begin) */
+  gvizQuadtree quadtree;
+  int quadtreeReady;
+  gvizThreadPool *pool;
 } gvizForceEmbedderState;
 
 /* Returns 0, -1 (allocation) or -2 (dimension != 2). No GPU is touched
@@ -124,6 +137,8 @@ int gvizForceEmbedderRun(gvizForceEmbedderState *state, size_t maxIters,
 /* Engine settings with no reference counterpart. */
 void gvizForceEmbedderSetPrecision(gvizForceEmbedderState *state,
                                    gvizForcePrecision precision);
+/* The precision Step and Run use (GVIZ_FORCE_PRECISION_MIXED after Init). */
+gvizForcePrecision gvizForceEmbedderPrecision(const gvizForceEmbedderState *state);
 /* Resident mode: Step neither reads nor writes the host positions; call
  * gvizForceEmbedderSyncPositions to fetch them. */
 void gvizForceEmbedderSetResidentPositions(gvizForceEmbedderState *state,

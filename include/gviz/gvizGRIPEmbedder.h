@@ -8,6 +8,8 @@
 #define GVIZ_HOST_GRIP_EMBEDDER_H
 
 #include "gvizEmbeddedGraph.h"
+#include "gvizKNearest.h"
+#include "gvizQuadtree.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -27,16 +29,37 @@ typedef struct gvizGRIPRoundStats {
   double meanForce;
 } gvizGRIPRoundStats;
 
-struct gvizhip_grip;
+typedef struct gvizGRIPDecorators {
+  gvizArray knn;
+  double *disp;
+  double *oldDisp;
+  double oldCos;
+  double heat;
+  double lastForceMag;
+} gvizGRIPDecorators;
 
+/* The reference's layout (include/embedders/gvizGRIPEmbedder.h:50-78),
+ * field for field. The per-vertex decorators, the computed-this-round bits,
+ * the BFS scratch and the kNN scratch live on the device (dec,
+ * dispCalculated, radiusBfsStamp, radiusBfsQueue, knnScratch and pool stay
+ * NULL / empty here; gvizGRIPEmbedderSaveState writes them out); the
+ * device handle and residency flag live beside the state, keyed by its
+ * address. */
 typedef struct gvizGRIPState {
   gvizEmbeddedGraph graph; /* MUST be first */
   size_t layerCount;
   size_t currLayer;
   size_t currRound;
   size_t *misFiltration;   /* activeCount entries */
-  size_t *misBorder;       /* layerCount entries */
-  size_t misBorderCount;
+  gvizArray misBorder;     /* size_t, layerCount entries */
+  gvizGRIPDecorators *dec;
+  GVIZ_BIT_ARRAY dispCalculated;
+  size_t *radiusBfsStamp;
+  size_t radiusBfsEpoch;
+  gvizDeque radiusBfsQueue;
+  gvizKNearestScratch *knnScratch;
+  size_t knnScratchCount;
+  gvizThreadPool *pool;
   size_t placementKMax;
   size_t refinementKMax;
   size_t knnCapacity;
@@ -44,9 +67,6 @@ typedef struct gvizGRIPState {
   gvizGRIPRoundStats lastRoundStats;
   uint8_t statsRegisterMode; /* survives Init (pre-Init configuration) */
   bool statsEnabled;
-  int residentPositions;     /* 1: RefineRound keeps positions on the device */
-  int devicePositionsFresh;
-  struct gvizhip_grip *device;
 } gvizGRIPState;
 
 int gvizGRIPEmbedderInit(gvizGRIPState *state, gvizSubgraph subgraph,
@@ -64,6 +84,9 @@ void gvizGRIPEmbedderNextStage(gvizGRIPState *state);
 void gvizGRIPEmbedderRefineRound(gvizGRIPState *state);
 /* Positions computed on the device are copied to the host positions. */
 int gvizGRIPEmbedderSyncPositions(gvizGRIPState *state);
+/* Resident mode (no reference counterpart): rounds and stages keep the
+ * positions on the device; gvizGRIPEmbedderSyncPositions fetches them. */
+void gvizGRIPEmbedderSetResidentPositions(gvizGRIPState *state, int enabled);
 /* Exact binary checkpoint (no reference counterpart): layer, round, last
  * round stats, draw mask, positions, every vertex's displacement / heat /
  * cosine / kNN state and the kNN lists. LoadState needs Begin on the same

@@ -26,13 +26,17 @@ extern "C" {
 #define GVIZ_SPRING_TUTTE_DEFAULT_STIFFNESS 30.0
 #define GVIZ_SPRING_TUTTE_DEFAULT_DAMPING 6.0
 
-struct gvizhip_tutte;
 
+/* The reference's layout (include/embedders/gvizSpringTutteEmbedder.h:15-30).
+ * The position double buffer lives on the device (`scratchPos` stays
+ * NULL); the device engine and its precision live beside the state, keyed
+ * by its address. */
 typedef struct gvizSpringTutteState {
   gvizEmbeddedGraph graph; /* MUST be first */
   size_t *boundary;
   size_t boundaryCount;
-  uint8_t *isBoundary;     /* one byte per parent vertex */
+  GVIZ_BIT_ARRAY isBoundary; /* one bit per parent vertex */
+  double *scratchPos;
   double *velocity;        /* N * dim; zero for boundary vertices */
   size_t iteration;
   double lastMaxDelta;
@@ -41,9 +45,6 @@ typedef struct gvizSpringTutteState {
   double stiffness;
   double damping;
   int begun;
-  int precision;           /* GVIZHIP_PRECISION_FP64 (default) or _FP32 */
-  int deviceFresh;         /* device positions and velocity match the host */
-  struct gvizhip_tutte *device;
 } gvizSpringTutteState;
 
 /* Registers action "springTutte.step" (and "springTutte.fixOuterFace") and

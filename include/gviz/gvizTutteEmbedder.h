@@ -17,13 +17,17 @@ extern "C" {
 
 #define GVIZ_TUTTE_DEFAULT_EPSILON 1e-5
 
-struct gvizhip_tutte;
 
+/* The reference's layout (include/embedders/gvizTutteEmbedder.h:22-35). The
+ * sweep's double buffer lives on the device (`scratch` stays NULL); the
+ * device engine and its precision live beside the state, keyed by its
+ * address. */
 typedef struct gvizTutteState {
   gvizEmbeddedGraph graph; /* MUST be first */
   size_t *boundary;
   size_t boundaryCount;
-  uint8_t *isBoundary;     /* one byte per parent vertex */
+  GVIZ_BIT_ARRAY isBoundary; /* one bit per parent vertex */
+  double *scratch;
   size_t iteration;
   double lastMaxDelta;
   int converged;
@@ -31,9 +35,6 @@ typedef struct gvizTutteState {
   double epsilon;
   double relaxationRate;
   int begun;
-  int precision;           /* GVIZHIP_PRECISION_FP64 (default) or _FP32 */
-  int devicePositionsFresh;/* device copy matches the host positions */
-  struct gvizhip_tutte *device;
 } gvizTutteState;
 
 int gvizTutteEmbedderInit(gvizTutteState *s, gvizSubgraph subgraph,

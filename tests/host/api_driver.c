@@ -54,6 +54,7 @@ static void read_graph(gvizGraph *g, int64_t *nOut) {
     int64_t u = rd_i(), v = rd_i();
     gvizGraphAddEdge(g, (size_t)u, (size_t)v);
   }
+  gvizGraphBuildLayout(g);
   *nOut = n;
 }
 
@@ -179,7 +180,7 @@ static int mode_api_grip(void) {
   wr_i(rc);
   wr_i((int64_t)st.layerCount);
   for (size_t i = 0; i < st.layerCount; i++)
-    wr_i((int64_t)st.misBorder[i]);
+    wr_i((int64_t) * (size_t *)gvizArrayAtIndex(&st.misBorder, i));
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
   const double *p = gvizEmbeddedGraphPositions(e);
   for (int64_t i = 0; i < dim * n; i++)
@@ -245,6 +246,7 @@ static void make_path(gvizGraph *g, size_t n) {
     gvizGraphAddVertex(g, NULL, NULL, NULL);
   for (size_t i = 0; i + 1 < n; i++)
     gvizGraphAddEdge(g, i, i + 1);
+  gvizGraphBuildLayout(g);
 }
 
 static void test_graph_insertionOrderAndCSR(void) {
@@ -287,6 +289,31 @@ static void test_graph_insertionOrderAndCSR(void) {
   free(off);
   free(adj);
   gvizSubgraphRelease(&sg);
+  /* a full subgraph needs the layout, as in the reference */
+  gvizSubgraph none = gvizSubgraphCreateFull(&g);
+  CHECK(none.g == NULL && none.vs == NULL);
+  gvizGraphBuildLayout(&g);
+  CHECK(g.layout && g.layout->nvertices == 5 && g.layout->edgeCount == 8);
+  CHECK(g.layout->vertexOffsets[1] == 3 && g.layout->vertexOffsets[5] == 8);
+  gvizSubgraph full = gvizSubgraphCreateFull(&g);
+  CHECK(gvizSubgraphIsFull(&full) && gvizSubgraphVertexCount(&full) == 5);
+  CHECK(gvizSubgraphEdgeCount(&full) == 4 && gvizSubgraphDegree(&full, 0) == 3);
+  CHECK(gvizSubgraphHasEdge(&full, 3, 4) && !gvizSubgraphHasEdge(&full, 1, 2));
+  gvizSubgraphHideVertex(&full, 3);
+  CHECK(gvizSubgraphVertexCount(&full) == 4 && gvizSubgraphEdgeCount(&full) == 2);
+  CHECK(gvizSubgraphDegree(&full, 0) == 2 && gvizSubgraphDegree(&full, 4) == 0);
+  /* bits are least significant first, one byte per eight vertices */
+  CHECK(full.vs[0] == (unsigned char)(0x1F & ~(1u << 3)));
+  gvizSubgraphRelease(&full);
+  gvizArray in;
+  gvizArrayInit(&in, sizeof(size_t));
+  size_t u0 = 0, u4 = 4;
+  gvizArrayPush(&in, &u0);
+  gvizArrayPush(&in, &u4);
+  CHECK(gvizGraphAddVertex(&g, NULL, &in, NULL) == 0 && gvizGraphSize(&g) == 6);
+  CHECK(gvizGraphEdgeExists(&g, 0, 5) && gvizGraphEdgeExists(&g, 4, 5) &&
+        !gvizGraphEdgeExists(&g, 5, 0));
+  gvizArrayRelease(&in);
   gvizGraphRelease(&g);
 }
 
@@ -414,7 +441,8 @@ static void test_forceEmbedder_initConfigure(void) {
   CHECK(s.barnesHutEnabled == 1 && s.preventOverlap == 0 && s.gravityK == 0.0);
   CHECK(s.mass[0] == 2.0 && s.mass[5] == 3.0 && s.degree[5] == 2);
   CHECK(s.meanDegreePlusOne == (2.0 * 2 + 3.0 * 8) / 10.0);
-  CHECK(s.precision == GVIZ_FORCE_PRECISION_MIXED && s.device == NULL);
+  CHECK(gvizForceEmbedderPrecision(&s) == GVIZ_FORCE_PRECISION_MIXED);
+  CHECK(s.disp == NULL && s.pool == NULL && s.quadtree.root == NULL);
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
   CHECK(gvizEmbeddedGraphFindAction(e, "forceEmbedder.step") != NULL);
   CHECK(gvizEmbeddedGraphFindAction(e, "forceEmbedder.toggleOverlapPrevention") != NULL);
@@ -480,6 +508,7 @@ static void test_tutte_boundaryAndSeed(void) {
   gvizGraph g;
   make_path(&g, 6);
   gvizGraphAddEdge(&g, 5, 0);
+  gvizGraphBuildLayout(&g); /* edges changed: renumber them */
   gvizTutteState s;
   CHECK(gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, 0.0) == 0);
   CHECK(s.epsilon == GVIZ_TUTTE_DEFAULT_EPSILON && s.relaxationRate == 5.0);
@@ -493,7 +522,7 @@ static void test_tutte_boundaryAndSeed(void) {
   CHECK(gvizTutteEmbedderSetBoundary(&s, two, 2, pos) == -1);
   CHECK(gvizTutteEmbedderSetBoundary(&s, bad, 3, pos) == -1);
   gvizTutteFixConvexPolygon(&s, tri, 4, 10.0);
-  CHECK(s.boundaryCount == 4 && s.isBoundary[2] && !s.isBoundary[1]);
+  CHECK(s.boundaryCount == 4 && gvizTestBit(s.isBoundary, 2) && !gvizTestBit(s.isBoundary, 1));
   const double *p2 = gvizEmbeddedGraphGetVPosition(e, 2);
   CHECK(fabs(p2[0]) < 1e-12 && fabs(p2[1] - 10.0) < 1e-12);
   gvizTutteEmbedderSeedInterior(&s);
@@ -512,6 +541,7 @@ static void test_tutte_boundaryAndSeed(void) {
   for (size_t i = 0; i < 5; i++)
     for (size_t j = i + 1; j < 5; j++)
       gvizGraphAddEdge(&k5, i, j);
+  gvizGraphBuildLayout(&k5);
   CHECK(gvizTutteEmbedderInit(&s, gvizSubgraphCreateFull(&k5), 2, 0.0) == 0);
   CHECK(gvizTutteEmbedderBegin(&s) == -2 && !s.begun);
   gvizTutteEmbedderRelease(&s);

@@ -37,6 +37,7 @@ def _declared():
         text = open(os.path.join(hdir, h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"typedef[^;]*;", "", text, flags=re.S)
+        text = re.sub(r"^#define(?:[^\n]*\\\n)*[^\n]*$", "", text, flags=re.M)
         names.update(re.findall(r"\b(gviz[A-Z][A-Za-z0-9]*)\s*\(", text))
     return sorted(names)
 
