@@ -118,6 +118,67 @@ __global__ __launch_bounds__(kB) void k_sweep(
   }
 }
 
+// Gauss-Seidel (src/embedders/gvizTutteEmbedder.c:258-262, 305-320 with
+// useGaussSeidel): the reference relaxes the interior vertices one at a time
+// in ascending id, each reading the current positions, so a vertex sees the
+// new positions of lower-id neighbours and the old ones of higher-id
+// neighbours. Level l holds the vertices whose lower-id interior neighbours
+// all sit in levels < l (level = 1 + the highest such neighbour's level), so
+// two vertices of one level are never adjacent: a sweep copies the positions
+// into the other buffer and relaxes it in place one level per launch, which
+// reads exactly what the sequential sweep reads.
+template <class Real>
+__global__ __launch_bounds__(kB) void k_gs_copy(const typename V2<Real>::type *__restrict__ in,
+                                                typename V2<Real>::type *__restrict__ out,
+                                                int64_t n, const TutteScalars *__restrict__ sc) {
+  if (sc->stopped)
+    return;
+  for (int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * kB)
+    out[u] = in[u];
+}
+
+template <class Real>
+__global__ __launch_bounds__(kB) void k_gs_level(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const int32_t *__restrict__ order, int32_t count, typename V2<Real>::type *pos,
+    Real alpha, double *__restrict__ partial, const TutteScalars *__restrict__ sc) {
+  __shared__ double sm[16];
+  if (sc->stopped)
+    return;
+  double vmax = 0.0;
+  const int32_t i = (int32_t)(blockIdx.x * kB + threadIdx.x);
+  if (i < count) {
+    const int32_t u = order[i];
+    const typename V2<Real>::type old = pos[u];
+    const int32_t b = off[u], e = off[u + 1];
+    Real bx = 0, by = 0;
+    row_sum<Real>(adj, pos, b, e, bx, by);
+    const Real cnt = (Real)(e - b);
+    bx /= cnt;
+    by /= cnt;
+    Real nx = old.x + alpha * (bx - old.x);
+    Real ny = old.y + alpha * (by - old.y);
+    Real d0 = nx - old.x, d1 = ny - old.y;
+    Real delta = (Real)0;
+    delta += d0 * d0;
+    delta += d1 * d1;
+    vmax = (double)sqrt(delta);
+    pos[u] = {nx, ny};
+  }
+  vmax = wave_max(vmax);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0)
+    sm[wave] = vmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kB / kWave; w++)
+      m = sm[w] > m ? sm[w] : m;
+    partial[blockIdx.x] = m;
+  }
+}
+
 // One spring step (src/embedders/gvizSpringTutteEmbedder.c:313-330): for an
 // interior vertex with neighbours, a = k (bary - p) - c v, v += a dt,
 // p' = p + v dt, per coordinate in that order; pinned and isolated vertices
@@ -266,6 +327,16 @@ struct gvizhip_tutte {
   double graphKey[4] = {-1.0, 0.0, 0.0, 0.0}; // step kind and its parameters
   TutteScalars *pinned = nullptr;
   std::vector<int32_t> perm; // stored position -> caller's vertex id
+  // host copies of the stored CSR and boundary flags, for the Gauss-Seidel
+  // schedule (rebuilt after an upload changes the boundary)
+  std::vector<int32_t> hostOff, hostAdj;
+  std::vector<uint8_t> hostBoundary;
+  bool gsDirty = true;
+  DevBuf<int32_t> gsOrder;       // stored ids, level by level
+  std::vector<int32_t> gsStart;  // level -> first entry of gsOrder (levels + 1)
+  std::vector<int64_t> gsSlot;   // level -> its first partial-max slot
+  DevBuf<double> gsPartial;      // one slot per (level, block)
+  int64_t gsSlots = 0;
 };
 
 namespace {
@@ -281,9 +352,33 @@ struct StepKind {
   double a, b, c;
 };
 
+template <class Real> void enqueue_gs(gvizhip_tutte *t, int from, Real alpha) {
+  using P = typename V2<Real>::type;
+  hipLaunchKernelGGL(k_gs_copy<Real>, dim3((unsigned)t->blocks), dim3(kB), 0, t->stream,
+                     (const P *)t->pos[from].p, (P *)t->pos[1 - from].p, t->n, t->sc.p);
+  const int levels = (int)t->gsStart.size() - 1;
+  for (int l = 0; l < levels; l++) {
+    const int32_t cnt = t->gsStart[l + 1] - t->gsStart[l];
+    hipLaunchKernelGGL(k_gs_level<Real>, dim3((unsigned)grid_for(cnt, kB)), dim3(kB), 0,
+                       t->stream, t->off.p, t->adj.p, t->gsOrder.p + t->gsStart[l], cnt,
+                       (P *)t->pos[1 - from].p, alpha, t->gsPartial.p + t->gsSlot[l],
+                       t->sc.p);
+  }
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, t->stream, t->gsPartial.p,
+                     (int)t->gsSlots, t->sc.p, t->records.p, t->recordCap);
+}
+
 int enqueue_step(gvizhip_tutte *t, int from, const StepKind &k) {
   const unsigned g = (unsigned)t->blocks;
   const bool f64 = t->precision == GVIZHIP_PRECISION_FP64;
+  if (k.kind == 2) {
+    if (f64)
+      enqueue_gs<double>(t, from, k.a);
+    else
+      enqueue_gs<float>(t, from, (float)k.a);
+    GVH_CHECK_LAUNCH();
+    return 0;
+  }
   if (k.kind == 0 && f64)
     hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
                        t->adj.p, t->boundary.p, (const double2 *)t->pos[from].p,
@@ -349,6 +444,59 @@ int read_sc(gvizhip_tutte *t, TutteScalars *out) {
                            hipMemcpyDeviceToHost, t->stream));
   GVH_CHECK(hipStreamSynchronize(t->stream));
   *out = *t->pinned;
+  return 0;
+}
+
+// The Gauss-Seidel levels over the stored vertices: in ascending caller id
+// (the reference's order), an interior vertex with neighbours gets one more
+// than the highest level of its interior lower-id neighbours.
+int build_gs_schedule(gvizhip_tutte *t) {
+  if (!t->gsDirty)
+    return 0;
+  const int64_t n = t->n;
+  std::vector<int32_t> inv(n), level(n, -1);
+  for (int64_t i = 0; i < n; i++)
+    inv[t->perm[i]] = (int32_t)i;
+  int32_t levels = 0;
+  auto active = [&](int32_t i) {
+    return !t->hostBoundary[i] && t->hostOff[i + 1] > t->hostOff[i];
+  };
+  for (int64_t ou = 0; ou < n; ou++) {
+    const int32_t i = inv[ou];
+    if (!active(i))
+      continue;
+    int32_t l = 0;
+    for (int32_t e = t->hostOff[i]; e < t->hostOff[i + 1]; e++) {
+      const int32_t j = t->hostAdj[e];
+      if (active(j) && t->perm[j] < ou && level[j] + 1 > l)
+        l = level[j] + 1;
+    }
+    level[i] = l;
+    levels = l + 1 > levels ? l + 1 : levels;
+  }
+  t->gsStart.assign(levels + 1, 0);
+  for (int64_t i = 0; i < n; i++)
+    if (level[i] >= 0)
+      t->gsStart[level[i] + 1]++;
+  for (int32_t l = 0; l < levels; l++)
+    t->gsStart[l + 1] += t->gsStart[l];
+  std::vector<int32_t> order(t->gsStart[levels] > 0 ? t->gsStart[levels] : 1);
+  std::vector<int32_t> at(t->gsStart.begin(), t->gsStart.end() - 1);
+  for (int64_t ou = 0; ou < n; ou++) {
+    const int32_t i = inv[ou];
+    if (level[i] >= 0)
+      order[at[level[i]]++] = i;
+  }
+  t->gsSlot.assign(levels + 1, 0);
+  for (int32_t l = 0; l < levels; l++)
+    t->gsSlot[l + 1] = t->gsSlot[l] + grid_for(t->gsStart[l + 1] - t->gsStart[l], kB);
+  t->gsSlots = t->gsSlot[levels];
+  if (t->gsOrder.alloc(order.size()) || t->gsPartial.alloc(t->gsSlots > 0 ? t->gsSlots : 1))
+    return -1;
+  GVH_CHECK(hipMemcpy(t->gsOrder.p, order.data(), sizeof(int32_t) * order.size(),
+                      hipMemcpyHostToDevice));
+  drop_graph(t); // a captured Gauss-Seidel step holds the old schedule
+  t->gsDirty = false;
   return 0;
 }
 
@@ -481,6 +629,9 @@ int gvizhip_tutte_create(gvizhip_tutte **out, int64_t n, const int64_t *offsets,
   }
   gvizhip_tutte *t = new gvizhip_tutte();
   t->perm = std::move(perm);
+  t->hostOff = o32;
+  t->hostAdj = a32;
+  t->hostBoundary.assign(n > 0 ? n : 1, 0);
   t->n = n;
   t->m2 = m2;
   t->precision = precision == GVIZHIP_PRECISION_FP64 ? GVIZHIP_PRECISION_FP64
@@ -552,6 +703,8 @@ int gvizhip_tutte_upload(gvizhip_tutte *t, const double *positions,
     for (int64_t i = 0; i < n; i++)
       b[i] = isBoundary[perm[i]];
     GVH_CHECK(hipMemcpy(t->boundary.p, b.data(), n, hipMemcpyHostToDevice));
+    t->hostBoundary.assign(b.begin(), b.end());
+    t->gsDirty = true;
   }
   TutteScalars s{};
   s.epsilon = -1.0;
@@ -599,6 +752,19 @@ int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
     return -1;
   }
   return run_steps(t, iters, epsilon, maxDeltas, StepKind{0, alpha, 0.0, 0.0});
+}
+
+int64_t gvizhip_tutte_gs_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
+                                double epsilon, double *maxDeltas) {
+  if (!t || iters < 0 || !(alpha > 0.0) || alpha > 1.0) {
+    set_error("gvizhip_tutte_gs_sweeps: invalid arguments");
+    return -1;
+  }
+  if (t->n == 0 || iters == 0)
+    return 0;
+  if (build_gs_schedule(t))
+    return -1;
+  return run_steps(t, iters, epsilon, maxDeltas, StepKind{2, alpha, 0.0, 0.0});
 }
 
 int64_t gvizhip_tutte_spring_steps(gvizhip_tutte *t, int64_t iters, double dt,

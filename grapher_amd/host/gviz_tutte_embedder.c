@@ -4,12 +4,11 @@
  *
  * Boundary placement, interior seeding and the convex polygon are host
  * bookkeeping done as the reference does them; every relaxation sweep runs
- * on the device (gvizhip_tutte_sweeps), which keeps the live positions and
- * the Jacobi snapshot in HBM. Positions cross the bus when the host changed
+ * on the device (gvizhip_tutte_sweeps for Jacobi, gvizhip_tutte_gs_sweeps
+ * when useGaussSeidel is set), which keeps the live positions and the
+ * Jacobi snapshot in HBM. Positions cross the bus when the host changed
  * them (boundary, seeding) and after each Step / Run. Begin's planarity
- * test and face search run on the host (gviz_planar.c). Gauss-Seidel sweeps
- * and FixOuterFace (a face picked in a viewer) are not part of this engine
- * and return -1.
+ * test and face search run on the host (gviz_planar.c).
  */
 #include "gviz/gvizTutteEmbedder.h"
 
@@ -284,7 +283,8 @@ static void record_sweep(gvizTutteState *s, double maxDelta) {
   gvizEmbeddedGraphStatAppend((gvizEmbeddedGraph *)s, "tutte.maxDelta", maxDelta);
 }
 
-/* src/embedders/gvizTutteEmbedder.c:305-335 (Jacobi). Host positions are
+/* src/embedders/gvizTutteEmbedder.c:305-335 (Jacobi, or Gauss-Seidel with
+ * useGaussSeidel set). Host positions are
  * re-uploaded before the sweep (a caller may have moved vertices) and read
  * back after it. */
 double gvizTutteEmbedderStep(gvizTutteState *s, double dt) {
@@ -293,13 +293,13 @@ double gvizTutteEmbedderStep(gvizTutteState *s, double dt) {
     return 0.0;
   if (alpha > 1.0)
     alpha = 1.0;
-  if (s->useGaussSeidel)
-    return -1.0;
   mark_stale(s);
   if (ensure_device(s) != 0)
     return -1.0;
   double maxDelta = 0.0;
-  if (gvizhip_tutte_sweeps(side_of(s)->device, 1, alpha, -1.0, &maxDelta) != 1)
+  gvizhip_tutte *d = side_of(s)->device;
+  if ((s->useGaussSeidel ? gvizhip_tutte_gs_sweeps(d, 1, alpha, -1.0, &maxDelta)
+                         : gvizhip_tutte_sweeps(d, 1, alpha, -1.0, &maxDelta)) != 1)
     return -1.0;
   if (gvizhip_tutte_download(side_of(s)->device, ((gvizEmbeddedGraph *)s)->embedding.vertexPositions) != 0)
     return -1.0;
@@ -320,8 +320,6 @@ int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters) {
     return (int)s->iteration; /* the reference's loop never advances here */
   if (alpha > 1.0)
     alpha = 1.0;
-  if (s->useGaussSeidel)
-    return -1;
   mark_stale(s);
   if (ensure_device(s) != 0)
     return -1;
@@ -333,7 +331,10 @@ int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters) {
   while (!s->converged && s->iteration < maxIters) {
     size_t left = maxIters - s->iteration;
     int64_t want = (int64_t)(left < kChunk ? left : kChunk);
-    int64_t ran = gvizhip_tutte_sweeps(side_of(s)->device, want, alpha, s->epsilon, deltas);
+    gvizhip_tutte *d = side_of(s)->device;
+    int64_t ran = s->useGaussSeidel
+                      ? gvizhip_tutte_gs_sweeps(d, want, alpha, s->epsilon, deltas)
+                      : gvizhip_tutte_sweeps(d, want, alpha, s->epsilon, deltas);
     if (ran <= 0) {
       failed = 1;
       break;

@@ -114,6 +114,7 @@ def lib():
         "gvizhip_tutte_upload": (C.c_int, [_vp, _vp, _vp]),
         "gvizhip_tutte_download": (C.c_int, [_vp, _vp]),
         "gvizhip_tutte_sweeps": (_i64, [_vp, _i64, C.c_double, C.c_double, _vp]),
+        "gvizhip_tutte_gs_sweeps": (_i64, [_vp, _i64, C.c_double, C.c_double, _vp]),
         "gvizhip_tutte_spring_steps": (_i64, [_vp, _i64, C.c_double, C.c_double, C.c_double,
                                               C.c_double, _vp]),
         "gvizhip_tutte_set_velocity": (C.c_int, [_vp, _vp]),
@@ -185,6 +186,7 @@ DECLARED = [
     "gvizhip_force_profile", "gvizhip_force_destroy",
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
     "gvizhip_tutte_sweeps", "gvizhip_tutte_profile", "gvizhip_tutte_destroy",
+    "gvizhip_tutte_gs_sweeps",
     "gvizhip_tutte_spring_steps", "gvizhip_tutte_set_velocity", "gvizhip_tutte_get_velocity",
     "gvizhip_tutte_sweeps_noalloc",
     "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
@@ -400,6 +402,14 @@ class TutteEngine:
         r = lib().gvizhip_tutte_sweeps(self.h, iters, alpha, epsilon, _p(out))
         if r < 0:
             _err("gvizhip_tutte_sweeps")
+        return (out[:r] if record else None), int(r)
+
+    def gs_sweeps(self, iters, alpha=1.0, epsilon=-1.0, record=True):
+        """Gauss-Seidel sweeps (gvizhip_tutte_gs_sweeps): (max deltas, sweeps)."""
+        out = np.zeros(max(iters, 1)) if record else None
+        r = lib().gvizhip_tutte_gs_sweeps(self.h, iters, alpha, epsilon, _p(out))
+        if r < 0:
+            _err("gvizhip_tutte_gs_sweeps")
         return (out[:r] if record else None), int(r)
 
     def spring_steps(self, iters, dt, stiffness=30.0, damping=6.0, epsilon=-1.0,

@@ -1,10 +1,9 @@
 /*
  * gvizTutteEmbedder.h -- the reference's Tutte barycentric embedder API
- * (include/embedders/gvizTutteEmbedder.h) with the Jacobi sweep on the GPU.
- * Begin tests planarity on the host (left-right test) and fixes the largest
- * face on a convex polygon: 0, -2 for a graph that is not planar. Gauss-
- * Seidel sweeps and FixOuterFace (a face picked in a viewer) are outside
- * this engine: they return -1 (Step: a negative value).
+ * (include/embedders/gvizTutteEmbedder.h) with the Jacobi and Gauss-Seidel
+ * sweeps on the GPU. Begin tests planarity on the host (left-right test)
+ * and fixes the largest face on a convex polygon: 0, -2 for a graph that is
+ * not planar. FixOuterFace (a face picked in a viewer) returns -1.
  */
 #ifndef GVIZ_HOST_TUTTE_EMBEDDER_H
 #define GVIZ_HOST_TUTTE_EMBEDDER_H

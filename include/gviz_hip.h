@@ -157,6 +157,15 @@ int gvizhip_tutte_download(gvizhip_tutte *t, double *positions);
  * max displacement. Returns the number of sweeps run. */
 int64_t gvizhip_tutte_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
                              double epsilon, double *maxDeltas);
+/* Gauss-Seidel sweeps (gvizTutteEmbedderStep with useGaussSeidel,
+ * src/embedders/gvizTutteEmbedder.c:258-262, 305-320): interior vertices
+ * relaxed in ascending id, each from the current positions; the same
+ * arguments, stop rule and records as gvizhip_tutte_sweeps. The schedule
+ * (vertices grouped so that no two of a group are adjacent and every
+ * lower-id neighbour sits in an earlier group) is built on the first call
+ * after an upload changed the boundary. */
+int64_t gvizhip_tutte_gs_sweeps(gvizhip_tutte *t, int64_t iters, double alpha,
+                                double epsilon, double *maxDeltas);
 /* Spring variant (replaces the loop of gvizSpringTutteEmbedderStep / Run,
  * src/embedders/gvizSpringTutteEmbedder.c:313-358): `iters` steps of
  * a = stiffness (barycentre - p) - damping v, v += a dt, p += v dt for every

@@ -85,3 +85,73 @@ def test_converges_to_laplacian_solution_and_stops(gpu):
     got = t.positions()
     assert np.abs(got[inner] - x).max() <= 1e-5 * 200.0
     assert np.array_equal(got[hull], p0[hull])
+
+
+def test_gs_sweeps_bit_exact_against_reference(gpu):
+    """Gauss-Seidel sweeps (useGaussSeidel) against the reference's run."""
+    f = golden("tutte_grid6_gs")
+    t = _setup(gpu, f, "fp64")
+    alpha = min(5.0 * float(f["dt"]), 1.0)
+    deltas, r = t.gs_sweeps(int(f["iters"]), alpha)
+    assert r == int(f["iters"])
+    assert np.array_equal(deltas, f["deltas"])
+    assert np.array_equal(t.positions(), f["p1"])
+
+
+def _gs_case(kind):
+    if kind == "delaunay":
+        n, e, _, hull = G.delaunay(20000, 5)
+        boundary = hull
+    else:  # row-major grid: dependency chains run the length of the grid
+        n, e = G.grid(70, 90)
+        boundary = np.array([j for j in range(90)] + [i * 90 + 89 for i in range(1, 70)] +
+                            [69 * 90 + j for j in range(88, -1, -1)] +
+                            [i * 90 for i in range(68, 0, -1)], dtype=np.int64)
+    off, adj = G.csr_from_edges(n, e)
+    isb = np.zeros(n, np.uint8)
+    isb[boundary] = 1
+    ring = O.tutte_polygon(len(boundary), 200.0)
+    pos = np.zeros((n, 2))
+    pos[:] = ring.mean(0)
+    pos[boundary] = ring
+    pos[isb == 0] += np.random.default_rng(1).normal(0, 5.0, (int((isb == 0).sum()), 2))
+    return off, adj, isb, pos
+
+
+@pytest.mark.parametrize("alpha", [1.0, 0.35])
+@pytest.mark.parametrize("kind", ["delaunay", "grid"])
+def test_gs_sweeps_bit_exact_against_oracle(gpu, kind, alpha):
+    off, adj, isb, pos = _gs_case(kind)
+    g = O.Graph(off, adj)
+    t = gpu.TutteEngine(off, adj, "fp64")
+    t.upload(pos, isb)
+    ref = pos.copy()
+    scratch = np.zeros_like(ref)
+    want = np.array([O.tutte_step(g, isb, ref, scratch, rate=alpha, dt=1.0,
+                                  gauss_seidel=True) for _ in range(25)])
+    got, r = t.gs_sweeps(25, alpha)
+    assert r == 25
+    assert np.array_equal(got, want)
+    assert np.array_equal(t.positions(), ref)
+    # a Jacobi sweep after the Gauss-Seidel ones continues from their result
+    t.sweeps(1, alpha)
+    O.tutte_step(g, isb, ref, scratch, rate=alpha, dt=1.0)
+    assert np.array_equal(t.positions(), ref)
+
+
+def test_gs_sweeps_stop_at_epsilon_like_the_reference_run(gpu):
+    off, adj, isb, pos = _gs_case("delaunay")
+    g = O.Graph(off, adj)
+    ref = pos.copy()
+    scratch = np.zeros_like(ref)
+    eps = 0.5
+    count = 0
+    while count < 400:
+        count += 1
+        if O.tutte_step(g, isb, ref, scratch, gauss_seidel=True) < eps:
+            break
+    t = gpu.TutteEngine(off, adj, "fp64")
+    t.upload(pos, isb)
+    _, r = t.gs_sweeps(400, 1.0, eps)
+    assert r == count < 400
+    assert np.array_equal(t.positions(), ref)

@@ -251,12 +251,13 @@ def test_grip_embed_matches_reference(gpu, name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["tutte_grid6_jacobi", "tutte_delaunay300_jacobi",
-                                  "tutte_delaunay300_dt60", "tutte_k4"])
+                                  "tutte_delaunay300_dt60", "tutte_k4", "tutte_grid6_gs"])
 def test_tutte_api_bit_exact(gpu, name):
     f = golden(name)
     print(f"host api tutte {name}", flush=True)
     p0, deltas, p1 = refio.tutte(int(f["n"]), f["edges"], f["boundary"], int(f["iters"]),
-                                 float(f["radius"]), float(f["dt"]), False, exe=DRIVER)
+                                 float(f["radius"]), float(f["dt"]), name.endswith("_gs"),
+                                 exe=DRIVER)
     assert np.array_equal(p0, f["p0"])
     assert np.array_equal(deltas, f["deltas"])
     assert np.array_equal(p1, f["p1"])
