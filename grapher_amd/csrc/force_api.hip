@@ -33,6 +33,9 @@ struct gvizhip_force {
   DevBuf<unsigned char> sfBackup;
   std::vector<int64_t> degreeHost;
   Scalars *pinned = nullptr; // host mirror for asynchronous reads
+  // the key/sort buffers hold the tree of the last step's starting positions
+  // (cleared when positions are replaced or another tree is built over them)
+  bool treeFresh = false;
 };
 
 namespace {
@@ -323,6 +326,7 @@ int gvizhip_force_begin(gvizhip_force *f, const double *positions) {
     return -1;
   }
   ForceCommon &c = f->c;
+  f->treeFresh = false;
   if (c.n > 0 && positions)
     GVH_CHECK(hipMemcpyAsync(c.pos64.p, positions, sizeof(double) * 2 * c.n,
                              hipMemcpyHostToDevice, c.stream));
@@ -345,6 +349,7 @@ int gvizhip_force_upload_positions(gvizhip_force *f, const double *positions) {
     set_error("gvizhip_force_upload_positions: invalid arguments");
     return -1;
   }
+  f->treeFresh = false;
   if (f->c.n > 0)
     GVH_CHECK(hipMemcpyAsync(f->c.pos64.p, positions, sizeof(double) * 2 * f->c.n,
                              hipMemcpyHostToDevice, f->c.stream));
@@ -405,6 +410,8 @@ int64_t gvizhip_force_run(gvizhip_force *f, int64_t maxIters, double epsilon,
     return -1;
   }
   const int64_t rounds = after.iteration - before.iteration;
+  if (rounds > 0)
+    f->treeFresh = c.prm.barnesHut != 0;
   if (stats && rounds > 0) {
     std::vector<StepRecord> rec(rounds);
     GVH_CHECK(hipMemcpy(rec.data(), c.records.p, sizeof(StepRecord) * rounds,
@@ -452,6 +459,7 @@ int gvizhip_force_evaluate(gvizhip_force *f, double *disp, double *structForce,
   c.wantInteractions = 0;
   if (rc)
     return -1;
+  f->treeFresh = c.prm.barnesHut != 0;
   GVH_CHECK(hipStreamSynchronize(c.stream));
   if (download_real(f, c.disp, 2 * n, disp) ||
       download_real(f, c.sf, 2 * n, structForce) ||
@@ -565,6 +573,7 @@ int gvizhip_force_load_state(gvizhip_force *f, const void *buf, int64_t bytes) {
   }
   memcpy(&h, buf, sizeof h);
   ForceCommon &c = f->c;
+  f->treeFresh = false;
   if (memcmp(h.magic, kStateMagic, 8) != 0 || h.n != c.n ||
       (h.realSize != 4 && h.realSize != 8) ||
       bytes < (int64_t)sizeof h + 2 * h.n * (8 + (int64_t)h.realSize)) {
@@ -636,6 +645,7 @@ int read_tree(gvizhip_force *f, TreeHost &t) {
     return -1;
   if (dispatch_step(f, kStepTreeOnly))
     return -1;
+  f->treeFresh = false; // now the tree of the current positions
   GVH_CHECK(hipStreamSynchronize(c.stream));
   if (read_scalars(f, &s))
     return -1;
@@ -862,6 +872,7 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
       rc = -1;
     c.timeTraversal = 0;
     c.wantInteractions = 0;
+    f->treeFresh = c.prm.barnesHut != 0;
     if (rc == 0 && hipStreamSynchronize(c.stream) != hipSuccess)
       rc = -1;
     float ms = 0.0f;
@@ -916,4 +927,148 @@ extern "C" void gvizhip_force_destroy(gvizhip_force *f) {
   if (f->pinned)
     (void)hipHostFree(f->pinned);
   delete f;
+}
+
+// ------------------------------------------------ quadtree keys for the host
+namespace {
+
+// Copies the tree buffers of c (root cell, key order, sorted keys) to the
+// host; the keyLo buffer is read in key order from `keyLoSorted`, or
+// permuted into it from the per-point buffer when that is null.
+int download_keys(ForceCommon &c, Scalars *pinned, const uint64_t *keyLoSorted,
+                  double *root, uint32_t *order, uint64_t *keyHi, uint64_t *keyLo) {
+  const int64_t n = c.n;
+  GVH_CHECK(hipMemcpyAsync(pinned, c.sc.p, sizeof(Scalars), hipMemcpyDeviceToHost,
+                           c.stream));
+  GVH_CHECK(hipMemcpyAsync(order, c.vals.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                           c.stream));
+  GVH_CHECK(hipMemcpyAsync(keyHi, c.keyHi.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost,
+                           c.stream));
+  if (keyLoSorted) {
+    GVH_CHECK(hipMemcpyAsync(keyLo, keyLoSorted, sizeof(uint64_t) * n,
+                             hipMemcpyDeviceToHost, c.stream));
+    GVH_CHECK(hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<uint64_t> lo(n);
+    GVH_CHECK(hipMemcpyAsync(lo.data(), c.keyLo.p, sizeof(uint64_t) * n,
+                             hipMemcpyDeviceToHost, c.stream));
+    GVH_CHECK(hipStreamSynchronize(c.stream));
+    for (int64_t s = 0; s < n; s++)
+      keyLo[s] = lo[order[s]];
+  }
+  root[0] = pinned->rootCx;
+  root[1] = pinned->rootCy;
+  root[2] = pinned->rootHalf;
+  root[3] = (double)pinned->maxDepth;
+  return 0;
+}
+
+} // namespace
+
+extern "C" int gvizhip_force_quadtree_keys(gvizhip_force *f, double *root,
+                                           uint32_t *order, uint64_t *keyHi,
+                                           uint64_t *keyLo, double *points) {
+  if (!f || !root || (f->c.n > 0 && (!order || !keyHi || !keyLo))) {
+    set_error("gvizhip_force_quadtree_keys: invalid arguments");
+    return -1;
+  }
+  ForceCommon &c = f->c;
+  const int64_t n = c.n;
+  if (n == 0)
+    return 0;
+  if (!f->treeFresh) {
+    Scalars s;
+    if (read_scalars(f, &s))
+      return -1;
+    s.stopped = 0;
+    if (write_scalars(f, s) || dispatch_step(f, kStepTreeOnly))
+      return -1;
+    f->treeFresh = true;
+  }
+  if (download_keys(c, f->pinned, c.keyLoS.p, root, order, keyHi, keyLo))
+    return -1;
+  if (points) {
+    std::vector<double> sp(2 * n);
+    GVH_CHECK(hipMemcpy(sp.data(), c.spos64.p, sizeof(double) * 2 * n,
+                        hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < n; s++) {
+      points[2 * (int64_t)order[s]] = sp[2 * s];
+      points[2 * (int64_t)order[s] + 1] = sp[2 * s + 1];
+    }
+  }
+  return 0;
+}
+
+struct gvizhip_qtree {
+  ForceCommon c;
+  Scalars *pinned = nullptr;
+};
+
+extern "C" int gvizhip_qtree_create(gvizhip_qtree **out) {
+  if (!out) {
+    set_error("gvizhip_qtree_create: null output");
+    return -1;
+  }
+  *out = nullptr;
+  gvizhip_qtree *q = new gvizhip_qtree();
+  ForceCommon &c = q->c;
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&q->pinned, sizeof(Scalars), hipHostMallocDefault) !=
+          hipSuccess) {
+    set_error("gvizhip_qtree_create: stream or pinned allocation failed");
+    gvizhip_qtree_destroy(q);
+    return -1;
+  }
+  if (c.sc.alloc(1) || c.partials.alloc(4 * 1024)) {
+    gvizhip_qtree_destroy(q);
+    return -1;
+  }
+  // k_keys reads the stop flag; nothing here ever sets it
+  if (hipMemset(c.sc.p, 0, sizeof(Scalars)) != hipSuccess) {
+    set_error("gvizhip_qtree_create: scalar reset failed");
+    gvizhip_qtree_destroy(q);
+    return -1;
+  }
+  *out = q;
+  return 0;
+}
+
+extern "C" int gvizhip_qtree_build(gvizhip_qtree *q, int64_t count, const double *points,
+                                   double *root, uint32_t *order, uint64_t *keyHi,
+                                   uint64_t *keyLo) {
+  if (!q || count < 0 || !root ||
+      (count > 0 && (!points || !order || !keyHi || !keyLo))) {
+    set_error("gvizhip_qtree_build: invalid arguments");
+    return -1;
+  }
+  if (count > (int64_t)UINT32_MAX) {
+    set_error("gvizhip_qtree_build: %lld points exceed the 32-bit point ids",
+              (long long)count);
+    return -1;
+  }
+  if (count == 0)
+    return 0;
+  ForceCommon &c = q->c;
+  c.n = count;
+  if (c.pos64.alloc(2 * count) || c.keyHi.alloc(count) || c.keyLo.alloc(count) ||
+      c.vals.alloc(count) || c.sorter.reserve(count))
+    return -1;
+  GVH_CHECK(hipMemcpyAsync(c.pos64.p, points, sizeof(double) * 2 * count,
+                           hipMemcpyHostToDevice, c.stream));
+  if (enqueue_keys_and_sort(c))
+    return -1;
+  return download_keys(c, q->pinned, nullptr, root, order, keyHi, keyLo);
+}
+
+extern "C" void gvizhip_qtree_destroy(gvizhip_qtree *q) {
+  if (!q)
+    return;
+  if (q->c.stream) {
+    (void)hipStreamSynchronize(q->c.stream);
+    (void)hipStreamDestroy(q->c.stream);
+    q->c.stream = nullptr;
+  }
+  if (q->pinned)
+    (void)hipHostFree(q->pinned);
+  delete q;
 }

@@ -181,6 +181,8 @@ enum StepMode : int { kStepFull = 0, kStepForcesOnly = 1, kStepTreeOnly = 2 };
 // neutral scalars (not stopped, depth limit 64). Instantiated next to
 // typed_step, so each precision is compiled exactly as its single-graph step.
 template <class Real, class PosT> int typed_batch_step(ForceCommon &c, BatchState &b);
+// Bounding box, root cell, keys and key sort of c.pos64 (c.n points).
+int enqueue_keys_and_sort(ForceCommon &c);
 int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b);
 int enqueue_batch_fa2(ForceCommon &c, BatchState &b);
 int enqueue_batch_stats(ForceCommon &c, BatchState &b);

@@ -12,6 +12,7 @@
 #include "gviz/gvizForceEmbedder.h"
 
 #include "gviz_hip.h"
+#include "gviz_quadtree_view.h"
 #include "gviz_side.h"
 
 #include <math.h>
@@ -231,6 +232,7 @@ static void action_toggle_overlap(gvizEmbeddedGraph *embedding, void *userData,
 int gvizForceEmbedderInit(gvizForceEmbedderState *s, gvizSubgraph subgraph,
                           size_t dimension, gvizForceModelKind model) {
   memset(s, 0, sizeof(*s));
+  gvizQuadtreeRelease(&s->quadtree); /* drops a host-view hook left at this address */
   if (dimension != 2)
     return -2;
   int res = gvizEmbeddedGraphInit((gvizEmbeddedGraph *)s, subgraph, dimension);
@@ -302,6 +304,8 @@ void gvizForceEmbedderRelease(gvizForceEmbedderState *s) {
     free(x->csrAdj);
     free(x);
   }
+  gvizQuadtreeRelease(&s->quadtree);
+  s->quadtreeReady = 0;
   gvizEmbeddedGraphRelease((gvizEmbeddedGraph *)s);
   free(s->vertices);
   free(s->degree);
@@ -400,6 +404,8 @@ int gvizForceEmbedderSyncPositions(gvizForceEmbedderState *s) {
   return 0;
 }
 
+static void quadtree_changed(gvizForceEmbedderState *s);
+
 /* Binary checkpoint file: magic, the host's step count and last maxDisp,
  * then the device state (gvizhip_force_save_state). */
 static const char kStateFileMagic[8] = {'G', 'V', 'Z', 'F', 'E', 'S', '0', '1'};
@@ -494,7 +500,47 @@ int gvizForceEmbedderLoadState(gvizForceEmbedderState *s, const char *filename) 
   }
   s->iteration = (size_t)it;
   s->lastMaxDisplacement = last;
+  quadtree_changed(s);
   return 0;
+}
+
+/* The state's quadtree on demand (gvizQuadtreeRoot): the device tree of the
+ * last step's starting positions, which the reference's Step rebuilds from
+ * positionsScratch and mass (src/embedders/gvizForceEmbedder.c:430-452);
+ * positionsScratch receives those positions. The arena keeps the
+ * nodesPerCell of its first build, as the reference's Rebuild does. */
+static int refresh_quadtree(void *owner) {
+  gvizForceEmbedderState *s = (gvizForceEmbedderState *)owner;
+  force_side *x = side_of(s);
+  size_t n = s->vertexCount;
+  if (!x || !x->f || n == 0)
+    return -1;
+  gvizQuadtree *t = &s->quadtree;
+  if (!s->quadtreeReady) {
+    if (gviz_quadtree_prepare(t, s->nodesPerCell) != 0)
+      return -1;
+    s->quadtreeReady = 1;
+  }
+  uint32_t *order = malloc(sizeof(uint32_t) * n);
+  uint64_t *hi = malloc(sizeof(uint64_t) * n), *lo = malloc(sizeof(uint64_t) * n);
+  double root[4];
+  int rc = -1;
+  if (order && hi && lo &&
+      gvizhip_force_quadtree_keys(x->f, root, order, hi, lo, s->positionsScratch) == 0) {
+    t->points = s->positionsScratch;
+    t->masses = s->mass;
+    t->pointCount = n;
+    rc = gviz_quadtree_from_keys(t, root, order, hi, lo);
+  }
+  free(order);
+  free(hi);
+  free(lo);
+  return rc;
+}
+
+static void quadtree_changed(gvizForceEmbedderState *s) {
+  if (s->barnesHutEnabled && s->vertexCount > 0)
+    gviz_quadtree_set_refresh(&s->quadtree, refresh_quadtree, s);
 }
 
 static int ensure_device(gvizForceEmbedderState *s, force_side *x) {
@@ -528,7 +574,10 @@ int gvizForceEmbedderBegin(gvizForceEmbedderState *s, unsigned int seed) {
   gather_positions(s);
   if (ensure_device(s, x) != 0)
     return -1;
-  return gvizhip_force_begin(x->f, s->positionsScratch);
+  if (gvizhip_force_begin(x->f, s->positionsScratch) != 0)
+    return -1;
+  quadtree_changed(s);
+  return 0;
 }
 
 /* Appends the stats of a step on an empty subgraph (every mean is 0). */
@@ -561,6 +610,7 @@ double gvizForceEmbedderStep(gvizForceEmbedderState *s) {
   gvizhip_force_stats r;
   if (gvizhip_force_step(x->f, &r) != 0)
     return -1.0;
+  quadtree_changed(s);
   if (!x->residentPositions) {
     if (gvizhip_force_download_positions(x->f, s->positionsScratch) != 0)
       return -1.0;
@@ -620,6 +670,8 @@ int gvizForceEmbedderRun(gvizForceEmbedderState *s, size_t maxIters,
       break;
   }
   free(rec);
+  if (rounds > 0)
+    quadtree_changed(s);
   if (!x->residentPositions) {
     if (gvizhip_force_download_positions(x->f, s->positionsScratch) != 0)
       return -1;

@@ -108,7 +108,11 @@ def lib():
         "gvizhip_force_load_state": (C.c_int, [_vp, _vp, _i64]),
         "gvizhip_force_quadtree_leaves": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "gvizhip_force_quadtree_cells": (_i64, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_force_quadtree_keys": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "gvizhip_force_profile": (C.c_int, [_vp, _i64, C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(C.c_double)]),
+        "gvizhip_qtree_create": (C.c_int, [C.POINTER(_vp)]),
+        "gvizhip_qtree_build": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+        "gvizhip_qtree_destroy": (None, [_vp]),
         "gvizhip_force_destroy": (None, [_vp]),
         "gvizhip_tutte_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, C.c_int32]),
         "gvizhip_tutte_upload": (C.c_int, [_vp, _vp, _vp]),
@@ -183,6 +187,8 @@ DECLARED = [
     "gvizhip_force_get_speed", "gvizhip_force_set_speed",
     "gvizhip_force_state_bytes", "gvizhip_force_save_state", "gvizhip_force_load_state",
     "gvizhip_force_quadtree_leaves", "gvizhip_force_quadtree_cells",
+    "gvizhip_force_quadtree_keys",
+    "gvizhip_qtree_create", "gvizhip_qtree_build", "gvizhip_qtree_destroy",
     "gvizhip_force_profile", "gvizhip_force_destroy",
     "gvizhip_tutte_create", "gvizhip_tutte_upload", "gvizhip_tutte_download",
     "gvizhip_tutte_sweeps", "gvizhip_tutte_profile", "gvizhip_tutte_destroy",

@@ -24,7 +24,6 @@ extern "C" {
 #define GVIZ_FORCE_EMBEDDER_SPEED_EFFICIENCY_MIN 0.05
 #define GVIZ_FORCE_EMBEDDER_SPEED_EFFICIENCY_MAX 1.0
 #define GVIZ_FORCE_EMBEDDER_SPEED_MAX_RISE 0.5
-#define GVIZ_QUADTREE_NODES_PER_CELL_DEFAULT 1
 
 typedef enum gvizForceModelKind {
   GVIZ_FORCE_MODEL_FRUCHTERMAN_REINGOLD = 0,

@@ -129,6 +129,16 @@ int64_t gvizhip_force_quadtree_cells(gvizhip_force *f, int64_t cap,
                                      uint64_t *keyLo, double *mass,
                                      double *comX, double *comY,
                                      int64_t *count);
+/* The tree behind the host view of gvizForceEmbedderState.quadtree
+ * (src/embedders/gvizForceEmbedder.c:430-452 rebuild it from the positions
+ * at the start of each Step): the tree of the last step's starting
+ * positions, or of the current positions when none was built since they
+ * last changed. root[4] = centre x, centre y, half size, depth limit;
+ * order[n] = compact vertex ids in key order; keyHi/keyLo[n] = their
+ * quadrant-path keys (levels 1-32, 33-64) in that order; points[2n] (may be
+ * NULL) = the positions the tree was built from, in compact order. */
+int gvizhip_force_quadtree_keys(gvizhip_force *f, double *root, uint32_t *order,
+                                uint64_t *keyHi, uint64_t *keyLo, double *points);
 /* Measurement hook (no reference counterpart): runs `steps` steps outside
  * the captured graph with HIP events on the engine's stream around the
  * repulsion kernel (Barnes-Hut traversal, or all-pairs when Barnes-Hut is
@@ -138,6 +148,23 @@ int64_t gvizhip_force_quadtree_cells(gvizhip_force *f, int64_t cap,
 int gvizhip_force_profile(gvizhip_force *f, int64_t steps, double *traversalMs,
                           int64_t *interactions, double *stepMs);
 void gvizhip_force_destroy(gvizhip_force *f);
+
+/* ------------------------------------------------------------- Quadtree */
+/* The device half of gvizQuadtreeInit / gvizQuadtreeRebuild
+ * (src/ds/gvizQuadtree.c:159-238): the root cell (bounding-box centre, half
+ * the larger span or 1.0, the depth at which the half size first falls to
+ * <= 1e-9) and every point's quadrant path by the reference's FP64 centre
+ * recurrence, sorted by (path, id). The host turns the sorted paths into
+ * the reference's node records. One context serves any number of builds
+ * (its buffers grow to the largest). */
+typedef struct gvizhip_qtree gvizhip_qtree;
+int gvizhip_qtree_create(gvizhip_qtree **out);
+/* count points (x, y interleaved, FP64; count < 2^32). Outputs as
+ * gvizhip_force_quadtree_keys. 0 or -1. */
+int gvizhip_qtree_build(gvizhip_qtree *q, int64_t count, const double *points,
+                        double *root, uint32_t *order, uint64_t *keyHi,
+                        uint64_t *keyLo);
+void gvizhip_qtree_destroy(gvizhip_qtree *q);
 
 /* ------------------------------------------------------------------ Tutte */
 typedef struct gvizhip_tutte gvizhip_tutte;

@@ -85,18 +85,39 @@ def _run(mode, payload: bytes, exe=None) -> bytes:
             return f.read()
 
 
-def quadtree(points, masses, npc):
+def quadtree(points, masses, npc, exe=None):
     points = np.asarray(points, dtype=np.float64).reshape(-1, 2)
     n = points.shape[0]
     w = _W()
     w.i(n, npc)
     w.da(points.reshape(-1))
     w.da(masses)
-    r = _R(_run("quadtree", w.bytes()))
+    return _read_tree(_R(_run("quadtree", w.bytes(), exe)), n)
+
+
+def force_quadtree(n, edges, pos, *, model, barnes_hut, npc, steps, theta, exe):
+    """The embedder state's quadtree (host API driver only): positions before
+    the last step, whether a root exists, quadtreeReady, the tree's point and
+    mass buffers, and the tree."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(model, barnes_hut, npc, steps)
+    w.d(theta)
+    w.da(np.asarray(pos, dtype=np.float64).reshape(-1))
+    r = _R(_run("force_quadtree", w.bytes(), exe))
+    before = r.da(2 * n).reshape(n, 2)
+    has_root, ready = r.i(), r.i()
+    points = r.da(2 * n).reshape(n, 2)
+    masses = r.da(n)
+    tree = _read_tree(r, n)
+    return before, bool(has_root), ready, points, masses, tree
+
+
+def _read_tree(r, n):
     nodes = []
     # node records are written in DFS order before the per-point table; read
     # them until the table fits exactly at the end of the buffer.
-    total = len(r.buf) // 8
+    total = (len(r.buf) - r.off) // 8
     node_words = total - 3 * n - 1
     count = node_words // 9
     for _ in range(count):

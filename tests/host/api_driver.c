@@ -17,6 +17,7 @@
 #include "gviz/gvizTutteEmbedder.h"
 #include "gviz/gvizSpringTutteEmbedder.h"
 #include "gviz/gvizGraphLoader.h"
+#include "gviz/gvizQuadtree.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -915,6 +916,251 @@ static int mode_spring_tutte(void) {
   return 0;
 }
 
+/* ---------------------------------------------------------------- quadtree */
+/* Node records in the reference-side harness's framing: parent before
+ * children in quadrant order; depth, path key, leaf flag, point count,
+ * mass, centre of mass, half size; then per point its leaf's depth and key,
+ * then the node count. */
+static void qt_dump_node(const gvizQuadtreeNode *nd, int depth, uint64_t kh, uint64_t kl,
+                         int64_t *leafDepth, uint64_t *leafHi, uint64_t *leafLo,
+                         int64_t *nodeCount) {
+  if (!nd)
+    return;
+  (*nodeCount)++;
+  int leaf = gvizQuadtreeNodeIsLeaf(nd);
+  wr_i(depth);
+  wr_i((int64_t)kh);
+  wr_i((int64_t)kl);
+  wr_i(leaf);
+  wr_i(leaf ? (int64_t)gvizQuadtreeNodePointCount(nd) : 0);
+  wr_d(nd->mass);
+  wr_d(nd->comX);
+  wr_d(nd->comY);
+  wr_d(nd->halfSize);
+  if (leaf) {
+    for (size_t i = 0; i < gvizQuadtreeNodePointCount(nd); i++) {
+      size_t p = gvizQuadtreeNodePointAt(nd, i);
+      leafDepth[p] = depth;
+      leafHi[p] = kh;
+      leafLo[p] = kl;
+    }
+    return;
+  }
+  for (int q = 0; q < 4; q++) {
+    uint64_t h = kh, l = kl;
+    int level = depth + 1;
+    if (level <= 32)
+      h |= (uint64_t)q << (2 * (32 - level));
+    else if (level <= 64)
+      l |= (uint64_t)q << (2 * (64 - level));
+    qt_dump_node(gvizQuadtreeNodeChild(nd, (gvizQuadrant)q), depth + 1, h, l, leafDepth,
+                 leafHi, leafLo, nodeCount);
+  }
+}
+
+static void qt_dump(const gvizQuadtree *t, int64_t n) {
+  int64_t *ld = calloc(n ? n : 1, 8);
+  uint64_t *lh = calloc(n ? n : 1, 8), *ll = calloc(n ? n : 1, 8);
+  int64_t nodeCount = 0;
+  qt_dump_node(gvizQuadtreeRoot(t), 0, 0, 0, ld, lh, ll, &nodeCount);
+  for (int64_t i = 0; i < n; i++) {
+    wr_i(ld[i]);
+    wr_i((int64_t)lh[i]);
+    wr_i((int64_t)ll[i]);
+  }
+  wr_i(nodeCount);
+  free(ld);
+  free(lh);
+  free(ll);
+}
+
+static int mode_quadtree(void) {
+  int64_t n = rd_i(), npc = rd_i();
+  double *pts = malloc(sizeof(double) * 2 * (n ? n : 1));
+  double *ms = malloc(sizeof(double) * (n ? n : 1));
+  for (int64_t i = 0; i < 2 * n; i++)
+    pts[i] = rd_d();
+  for (int64_t i = 0; i < n; i++)
+    ms[i] = rd_d();
+  gvizQuadtree t;
+  if (gvizQuadtreeInit(&t, pts, ms, (size_t)n, (size_t)npc) != 0)
+    return fail_device("gvizQuadtreeInit");
+  qt_dump(&t, n);
+  gvizQuadtreeRelease(&t);
+  free(pts);
+  free(ms);
+  return 0;
+}
+
+/* The force state's quadtree after Begin and `steps` steps with the
+ * positions overwritten after Begin: writes the positions before the last
+ * step (or after Begin), the tree's point buffer, then the tree. */
+static int mode_force_quadtree(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t model = rd_i(), bh = rd_i(), npc = rd_i(), steps = rd_i();
+  double theta = rd_d();
+  double *pos = malloc(sizeof(double) * 2 * n);
+  for (int64_t i = 0; i < 2 * n; i++)
+    pos[i] = rd_d();
+  gvizForceEmbedderState s;
+  if (gvizForceEmbedderInit(&s, gvizSubgraphCreateFull(&g), 2, (gvizForceModelKind)model) != 0)
+    return 1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&s;
+  gvizForceEmbedderSetPrecision(&s, env_precision());
+  gvizForceEmbedderConfigureBarnesHut(&s, theta, (size_t)npc);
+  gvizForceEmbedderSetBarnesHutEnabled(&s, (int)bh);
+  if (gvizForceEmbedderBegin(&s, 7) != 0)
+    return fail_device("gvizForceEmbedderBegin");
+  double *p = gvizEmbeddedGraphPositions(e);
+  if (steps == 0) {
+    /* the tree of Begin's own positions */
+  } else {
+    memcpy(p, pos, sizeof(double) * 2 * n);
+  }
+  for (int64_t k = 0; k + 1 < steps; k++)
+    if (gvizForceEmbedderStep(&s) < 0.0)
+      return fail_device("gvizForceEmbedderStep");
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  if (steps > 0 && gvizForceEmbedderStep(&s) < 0.0)
+    return fail_device("gvizForceEmbedderStep");
+  const gvizQuadtreeNode *root = gvizQuadtreeRoot(&s.quadtree);
+  wr_i(root != NULL);
+  wr_i(s.quadtreeReady);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(root ? s.quadtree.points[i] : 0.0);
+  for (int64_t i = 0; i < n; i++)
+    wr_d(root ? s.quadtree.masses[i] : 0.0);
+  qt_dump(&s.quadtree, n);
+  gvizForceEmbedderRelease(&s);
+  gvizGraphRelease(&g);
+  free(pos);
+  return 0;
+}
+
+/* The reference's quadtree unit cases (tests/ds/gvizQuadtreeTests.c:13-343)
+ * restated against the host view; prints each failed check. */
+#define QT_CHECK(c)                                                           \
+  do {                                                                        \
+    if (!(c)) {                                                               \
+      fprintf(stderr, "quadtree check failed at line %d: %s\n", __LINE__, #c); \
+      bad++;                                                                  \
+    }                                                                         \
+  } while (0)
+
+static int near(double a, double b) { return fabs(a - b) <= 1e-12 * fmax(1.0, fabs(b)); }
+
+static int mode_quadtree_cases(void) {
+  int bad = 0;
+  gvizQuadtree t;
+  double cx, cy;
+  QT_CHECK(GVIZ_QUADTREE_NODES_PER_CELL_DEFAULT == 1);
+  /* empty */
+  QT_CHECK(gvizQuadtreeInit(&t, NULL, NULL, 0, 1) == 0);
+  QT_CHECK(gvizQuadtreeRoot(&t) == NULL);
+  gvizQuadtreeRelease(&t);
+  /* one point */
+  {
+    double pts[] = {3.0, 4.0}, ms[] = {1.0};
+    QT_CHECK(gvizQuadtreeInit(&t, pts, ms, 1, 1) == 0);
+    const gvizQuadtreeNode *r = gvizQuadtreeRoot(&t);
+    QT_CHECK(r && gvizQuadtreeNodeIsLeaf(r) && gvizQuadtreeNodePointCount(r) == 1 &&
+             gvizQuadtreeNodePointAt(r, 0) == 0);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(cx == 3.0 && cy == 4.0);
+    gvizQuadtreeRelease(&t);
+  }
+  /* four corners split the root into four single-point leaves */
+  double corners[] = {-5.0, 5.0, 5.0, 5.0, -5.0, -5.0, 5.0, -5.0}, ones[] = {1, 1, 1, 1, 1};
+  {
+    QT_CHECK(gvizQuadtreeInit(&t, corners, ones, 4, 1) == 0);
+    const gvizQuadtreeNode *r = gvizQuadtreeRoot(&t);
+    QT_CHECK(r && !gvizQuadtreeNodeIsLeaf(r) && r->mass == 4.0);
+    for (int q = 0; q < 4; q++) {
+      const gvizQuadtreeNode *c = gvizQuadtreeNodeChild(r, (gvizQuadrant)q);
+      QT_CHECK(c && gvizQuadtreeNodeIsLeaf(c) && gvizQuadtreeNodePointCount(c) == 1 &&
+               gvizQuadtreeNodePointAt(c, 0) == (size_t)q);
+    }
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(fabs(cx) <= 1e-9 && fabs(cy) <= 1e-9);
+    QT_CHECK(t.nodesUsed == 5);
+    /* rebuilds reuse the arena */
+    size_t blocks = t.nodeBlocks.count;
+    QT_CHECK(blocks > 0);
+    for (int k = 0; k < 5; k++) {
+      QT_CHECK(gvizQuadtreeRebuild(&t, corners, ones, 4) == 0);
+      QT_CHECK(t.nodeBlocks.count == blocks && t.pointBlocks.count == blocks);
+    }
+    double moved[] = {100.0, 100.0, 101.0, 100.0, 100.0, 101.0, 101.0, 101.0};
+    QT_CHECK(gvizQuadtreeRebuild(&t, moved, ones, 4) == 0);
+    r = gvizQuadtreeRoot(&t);
+    QT_CHECK(r && r->mass == 4.0);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(near(cx, 100.5) && near(cy, 100.5));
+    gvizQuadtreeRelease(&t);
+  }
+  /* a leaf holds up to nodesPerCell points */
+  {
+    double pts[] = {1.0, 1.0, 2.0, 2.0, 3.0, 3.0};
+    QT_CHECK(gvizQuadtreeInit(&t, pts, ones, 3, 3) == 0);
+    const gvizQuadtreeNode *r = gvizQuadtreeRoot(&t);
+    QT_CHECK(r && gvizQuadtreeNodeIsLeaf(r) && gvizQuadtreeNodePointCount(r) == 3 &&
+             r->mass == 3.0);
+    gvizQuadtreeRelease(&t);
+  }
+  /* coincident points end in an overflow leaf; rebuilds do not pile up
+   * overflow buffers */
+  {
+    double dup[] = {2.0, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0, 2.0};
+    QT_CHECK(gvizQuadtreeInit(&t, dup, ones, 3, 1) == 0);
+    const gvizQuadtreeNode *r = gvizQuadtreeRoot(&t);
+    QT_CHECK(r && r->mass == 3.0);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(near(cx, 2.0) && near(cy, 2.0));
+    gvizQuadtreeRelease(&t);
+    QT_CHECK(gvizQuadtreeInit(&t, dup, ones, 4, 1) == 0);
+    size_t over = t.overflowBuffers.count;
+    QT_CHECK(over > 0);
+    QT_CHECK(gvizQuadtreeRebuild(&t, dup, ones, 4) == 0);
+    QT_CHECK(t.overflowBuffers.count == over);
+    gvizQuadtreeRelease(&t);
+  }
+  /* centre of mass is the mean; weights skew it */
+  {
+    double pts[] = {0.0, 0.0, 10.0, 0.0, 0.0, 10.0, 10.0, 10.0, 4.0, 4.0};
+    QT_CHECK(gvizQuadtreeInit(&t, pts, ones, 5, 1) == 0);
+    const gvizQuadtreeNode *r = gvizQuadtreeRoot(&t);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(near(cx, 24.0 / 5.0) && near(cy, 24.0 / 5.0) && r->mass == 5.0);
+    gvizQuadtreeRelease(&t);
+    double two[] = {0.0, 0.0, 10.0, 0.0}, w[] = {1.0, 3.0};
+    QT_CHECK(gvizQuadtreeInit(&t, two, w, 2, 1) == 0);
+    r = gvizQuadtreeRoot(&t);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(r->mass == 4.0 && near(cx, 7.5) && cy == 0.0);
+    gvizQuadtreeRelease(&t);
+    double same[] = {2.0, 2.0, 2.0, 2.0}, w2[] = {5.0, 10.0};
+    QT_CHECK(gvizQuadtreeInit(&t, same, w2, 2, 1) == 0);
+    r = gvizQuadtreeRoot(&t);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(r->mass == 15.0 && near(cx, 2.0) && near(cy, 2.0));
+    gvizQuadtreeRelease(&t);
+  }
+  /* zero total mass keeps the centre of mass where it was (:118-123) */
+  {
+    double pts[] = {10.0, 10.0, 0.0, 0.0}, w[] = {1.0, -1.0};
+    QT_CHECK(gvizQuadtreeInit(&t, pts, w, 2, 1) == 0);
+    const gvizQuadtreeNode *r = gvizQuadtreeRoot(&t);
+    gvizQuadtreeNodeCenterOfMass(r, &cx, &cy);
+    QT_CHECK(r->mass == 0.0 && cx == 10.0 && cy == 10.0);
+    gvizQuadtreeRelease(&t);
+  }
+  wr_i(bad);
+  return bad ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
   if (argc == 2 && !strcmp(argv[1], "unit"))
     return mode_unit();
@@ -945,6 +1191,12 @@ int main(int argc, char **argv) {
     rc = mode_grip_checkpoint();
   else if (!strcmp(argv[1], "force_checkpoint"))
     rc = mode_force_checkpoint();
+  else if (!strcmp(argv[1], "quadtree"))
+    rc = mode_quadtree();
+  else if (!strcmp(argv[1], "force_quadtree"))
+    rc = mode_force_quadtree();
+  else if (!strcmp(argv[1], "quadtree_cases"))
+    rc = mode_quadtree_cases();
   fclose(fin);
   fclose(fout);
   return rc;
