@@ -28,10 +28,6 @@
 #define GRIP_K_MAX_DEFAULT 128             /* :34 */
 #define GRIP_EMBED_MAX_ROUNDS_PER_LAYER 30 /* :1263 */
 #define GRIP_EMBED_CONVERGENCE_FACTOR 1e-3 /* :1264 */
-/* include/algorithms/search/gvizKNearest.h:9-13 */
-#define KNN_BATCH_VISIBLE_MAX 256
-#define KNN_BATCH_MIN_AVG_DEGREE 32.0
-#define KNN_BATCH_TARGET_RATIO 8
 #define MIS_MAX_LAYERS 128
 
 static size_t border_at(const gvizGRIPState *s, size_t i) {
@@ -123,17 +119,8 @@ static size_t refinement_k(const gvizGRIPState *s) {
  * seed beats one BFS per target. */
 static int prefer_batch(const gvizGRIPState *s, size_t visibleCount,
                         size_t targetCount) {
-  if (visibleCount == 0 || visibleCount >= targetCount ||
-      visibleCount > KNN_BATCH_VISIBLE_MAX)
-    return 0;
-  const gvizSubgraph *sg = &((const gvizEmbeddedGraph *)s)->subgraph;
-  size_t vc = gvizSubgraphVertexCount(sg);
-  if (vc == 0)
-    return 0;
-  double avgDeg = (2.0 * (double)gvizSubgraphEdgeCount(sg)) / (double)vc;
-  if (avgDeg > KNN_BATCH_MIN_AVG_DEGREE)
-    return 1;
-  return targetCount >= KNN_BATCH_TARGET_RATIO * visibleCount;
+  return gvizSearchKNearestPreferBatch(&((const gvizEmbeddedGraph *)s)->subgraph,
+                                       visibleCount, targetCount);
 }
 
 void gvizGRIPEmbedderConfigureStats(gvizGRIPState *s, bool enable) {

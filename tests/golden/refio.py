@@ -178,13 +178,13 @@ def tutte(n, edges, boundary, iters, radius, dt, gauss_seidel=False, exe=None):
     return p0, deltas, p1
 
 
-def knn(n, edges, visible, k, sources):
+def knn(n, edges, visible, k, sources, exe=None):
     w = _W()
     w.graph(n, edges)
     w.i(k, len(sources))
     w.ia(np.asarray(visible, dtype=np.int64))
     w.ia(np.asarray(sources, dtype=np.int64))
-    r = _R(_run("knn", w.bytes()))
+    r = _R(_run("knn", w.bytes(), exe))
     per = []
     for _ in sources:
         c = r.i()

@@ -18,6 +18,7 @@
 #include "gviz/gvizSpringTutteEmbedder.h"
 #include "gviz/gvizGraphLoader.h"
 #include "gviz/gvizQuadtree.h"
+#include "gviz/gvizKNearest.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -1161,6 +1162,178 @@ static int mode_quadtree_cases(void) {
   return bad ? 1 : 0;
 }
 
+/* --------------------------------------------------------------------- kNN */
+/* Same framing as the reference-side harness: per source the count and k
+ * (vertex, dist) pairs of gvizSearchKNearest over the visible filter, then
+ * the batched search with every non-visible vertex as a target. */
+static int mode_knn(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t k = rd_i(), nsrc = rd_i();
+  gvizSubgraph sg = gvizSubgraphCreateFull(&g);
+  gvizVertexSubset vis = gvizVertexSubsetCreateEmpty(&g);
+  for (int64_t i = 0; i < n; i++)
+    if (rd_i())
+      gvizVertexSubsetShowVertex(vis, (size_t)i);
+  gvizFoundVertex *out = malloc(sizeof(gvizFoundVertex) * (k + 1));
+  for (int64_t q = 0; q < nsrc; q++) {
+    int64_t src = rd_i();
+    int c = gvizSearchKNearest(&sg, out, (size_t)k, (size_t)src, vis);
+    wr_i(c);
+    for (int64_t j = 0; j < k; j++) {
+      wr_i(j < c ? (int64_t)out[j].v : -1);
+      wr_i(j < c ? (int64_t)out[j].dist : -1);
+    }
+  }
+  int64_t nt = 0;
+  gvizKNNBatchTarget *targets = malloc(sizeof(*targets) * n);
+  gvizFoundVertex *found = malloc(sizeof(gvizFoundVertex) * n * k);
+  for (int64_t i = 0; i < n; i++)
+    if (!gvizVertexSubsetTest(vis, (size_t)i)) {
+      targets[nt] = (gvizKNNBatchTarget){(size_t)i, found + nt * k, 0};
+      nt++;
+    }
+  gvizKNearestScratch scratch;
+  if (gvizKNearestScratchInit(&scratch, (size_t)n) != 0)
+    return 1;
+  int rc = gvizSearchKNearestFromVisibleBatch(&sg, vis, (size_t)k, targets, (size_t)nt,
+                                              &scratch);
+  if (rc < 0)
+    return fail_device("gvizSearchKNearestFromVisibleBatch");
+  wr_i(rc);
+  wr_i(nt);
+  for (int64_t t = 0; t < nt; t++) {
+    wr_i((int64_t)targets[t].vertex);
+    wr_i((int64_t)targets[t].count);
+    for (int64_t j = 0; j < k; j++) {
+      int ok = j < (int64_t)targets[t].count;
+      wr_i(ok ? (int64_t)targets[t].out[j].v : -1);
+      wr_i(ok ? (int64_t)targets[t].out[j].dist : -1);
+    }
+  }
+  gvizKNearestScratchRelease(&scratch);
+  free(out);
+  free(targets);
+  free(found);
+  gvizSubgraphRelease(&sg);
+  gvizVertexSubsetRelease(vis);
+  gvizGraphRelease(&g);
+  return 0;
+}
+
+/* The reference's kNN unit cases (tests/algorithms/search/gvizSearchTests.c:
+ * 94-131, 184-207, 287-360) restated. */
+static int mode_knn_cases(void) {
+  int bad = 0;
+  gvizFoundVertex f[8];
+  /* triangle: both other vertices at distance 1, in adjacency order */
+  {
+    gvizGraph g;
+    gvizGraphInit(&g, 0);
+    for (int i = 0; i < 3; i++)
+      gvizGraphAddVertex(&g, NULL, NULL, NULL);
+    gvizGraphAddEdge(&g, 0, 1);
+    gvizGraphAddEdge(&g, 1, 2);
+    gvizGraphAddEdge(&g, 0, 2);
+    gvizGraphBuildLayout(&g);
+    gvizSubgraph sg = gvizSubgraphCreateFull(&g);
+    int c = gvizSearchKNearest(&sg, f, 2, 0, NULL);
+    QT_CHECK(c == 2 && f[0].v == 1 && f[0].dist == 1 && f[1].v == 2 && f[1].dist == 1);
+    /* a filter admits only its vertices */
+    gvizVertexSubset filt = gvizVertexSubsetCreateEmpty(&g);
+    gvizVertexSubsetShowVertex(filt, 2);
+    c = gvizSearchKNearest(&sg, f, 2, 0, filt);
+    QT_CHECK(c == 1 && f[0].v == 2 && f[0].dist == 1);
+    /* k beyond the graph, k = 0, a source outside the graph */
+    c = gvizSearchKNearest(&sg, f, 5, 0, NULL);
+    QT_CHECK(c == 2);
+    QT_CHECK(gvizSearchKNearest(&sg, NULL, 0, 0, NULL) == 0);
+    QT_CHECK(gvizSearchKNearest(&sg, f, 2, 7, NULL) == -1);
+    QT_CHECK(gvizSearchKNearest(NULL, f, 2, 0, NULL) == -1);
+    gvizVertexSubsetRelease(filt);
+    gvizSubgraphRelease(&sg);
+    gvizGraphRelease(&g);
+  }
+  /* path 0-1-2-3: the scratch form equals the stateless one, reused */
+  {
+    gvizGraph g;
+    make_path(&g, 4);
+    gvizSubgraph sg = gvizSubgraphCreateFull(&g);
+    gvizKNearestScratch sc;
+    QT_CHECK(gvizKNearestScratchInit(&sc, 4) == 0);
+    gvizFoundVertex a[4], b[4];
+    for (size_t src = 0; src < 4; src++)
+      for (int rep = 0; rep < 2; rep++) {
+        int ca = gvizSearchKNearest(&sg, a, 3, src, NULL);
+        int cb = gvizSearchKNearestScratch(&sg, b, 3, src, NULL, &sc);
+        QT_CHECK(ca == cb && ca == 3);
+        for (int i = 0; i < ca && i < 4; i++)
+          QT_CHECK(a[i].v == b[i].v && a[i].dist == b[i].dist);
+      }
+    gvizFoundVertex c3[3];
+    QT_CHECK(gvizSearchKNearest(&sg, c3, 3, 0, NULL) == 3 && c3[2].v == 3 &&
+             c3[2].dist == 3);
+    /* a scratch too small for the graph */
+    gvizKNearestScratch small;
+    QT_CHECK(gvizKNearestScratchInit(&small, 2) == 0);
+    QT_CHECK(gvizSearchKNearestScratch(&sg, a, 2, 0, NULL, &small) == -1);
+    gvizKNearestScratchRelease(&small);
+    QT_CHECK(gvizKNearestScratchInit(&small, 0) == -1);
+    gvizKNearestScratchRelease(&sc);
+    gvizSubgraphRelease(&sg);
+    gvizGraphRelease(&g);
+  }
+  /* the batched search gives every target the per-vertex distances */
+  {
+    enum { N = 64, K = 4 };
+    gvizGraph g;
+    gvizGraphInit(&g, 0);
+    for (int i = 0; i < N; i++)
+      gvizGraphAddVertex(&g, NULL, NULL, NULL);
+    for (int i = 0; i + 1 < N; i++)
+      gvizGraphAddEdge(&g, i, i + 1);
+    for (int i = 0; i + 8 < N; i += 8)
+      gvizGraphAddEdge(&g, i, i + 8);
+    gvizGraphBuildLayout(&g);
+    gvizSubgraph sg = gvizSubgraphCreateFull(&g);
+    gvizVertexSubset vis = gvizVertexSubsetCreateEmpty(&g);
+    for (int i = 0; i < N; i += 16)
+      gvizVertexSubsetShowVertex(vis, i);
+    gvizKNNBatchTarget tg[N];
+    gvizFoundVertex store[N * K];
+    size_t nt = 0;
+    for (int i = 0; i < N; i++)
+      if (!gvizVertexSubsetTest(vis, i)) {
+        tg[nt] = (gvizKNNBatchTarget){(size_t)i, store + nt * K, 0};
+        nt++;
+      }
+    gvizKNearestScratch sc;
+    QT_CHECK(gvizKNearestScratchInit(&sc, N) == 0);
+    QT_CHECK(gvizSearchKNearestPreferBatch(&sg, gvizVertexSubsetCount(vis, N), nt) == 1);
+    QT_CHECK(gvizSearchKNearestFromVisibleBatch(&sg, vis, K, tg, nt, &sc) == 0);
+    for (size_t t = 0; t < nt; t++) {
+      gvizFoundVertex direct[K];
+      int dc = gvizSearchKNearest(&sg, direct, K, tg[t].vertex, vis);
+      QT_CHECK(dc == (int)tg[t].count);
+      for (int i = 0; i < dc; i++)
+        QT_CHECK(direct[i].dist == tg[t].out[i].dist);
+    }
+    /* the batch path does not apply: no seeds, or not fewer seeds than targets */
+    gvizVertexSubset none = gvizVertexSubsetCreateEmpty(&g);
+    QT_CHECK(gvizSearchKNearestFromVisibleBatch(&sg, none, K, tg, nt, &sc) == 1);
+    QT_CHECK(gvizSearchKNearestFromVisibleBatch(&sg, vis, K, tg, 2, &sc) == 1);
+    QT_CHECK(gvizSearchKNearestPreferBatch(&sg, 0, nt) == 0);
+    gvizVertexSubsetRelease(none);
+    gvizKNearestScratchRelease(&sc);
+    gvizVertexSubsetRelease(vis);
+    gvizSubgraphRelease(&sg);
+    gvizGraphRelease(&g);
+  }
+  wr_i(bad);
+  return bad ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
   if (argc == 2 && !strcmp(argv[1], "unit"))
     return mode_unit();
@@ -1197,6 +1370,10 @@ int main(int argc, char **argv) {
     rc = mode_force_quadtree();
   else if (!strcmp(argv[1], "quadtree_cases"))
     rc = mode_quadtree_cases();
+  else if (!strcmp(argv[1], "knn"))
+    rc = mode_knn();
+  else if (!strcmp(argv[1], "knn_cases"))
+    rc = mode_knn_cases();
   fclose(fin);
   fclose(fout);
   return rc;
