@@ -1634,6 +1634,148 @@ int set_flag(gvizhip_grip *g, int idx, int32_t v) {
   return 0;
 }
 
+// Layer 1 (makeFirstMISPartition, gvizGRIPEmbedder.c:450-483): the
+// lexicographically-first MIS of the subgraph in ascending id (`mis`), and
+// the other subgraph vertices in the order the reference marks them (each
+// MIS vertex marks its unmarked neighbours in adjacency order), which it
+// writes backwards from the end of misFiltration (`tail`).
+int first_layer(gvizhip_grip *g, std::vector<int64_t> &mis, std::vector<int64_t> &tail) {
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  GVH_CHECK(hipMemsetAsync(g->state.p, 0, n, st));
+  int32_t undecided = 1;
+  int rounds = 0;
+  while (undecided) {
+    if (set_flag(g, 0, 0))
+      return -1;
+    for (int r = 0; r < 4; r++)
+      hipLaunchKernelGGL(k_lfmis_graph, dim3(grid_for(n, kB)), dim3(kB), 0, st,
+                         g->off.p, g->adj.p, g->active.p, n, g->state.p, g->flags.p);
+    GVH_CHECK_LAUNCH();
+    if (read_flag(g, 0, &undecided))
+      return -1;
+    if (++rounds > 4 * (n + 4)) {
+      set_error("gvizhip_grip_mis_filtration: layer-1 rounds did not settle");
+      return -1;
+    }
+  }
+  std::vector<uint8_t> state(n);
+  GVH_CHECK(hipMemcpy(state.data(), g->state.p, n, hipMemcpyDeviceToHost));
+  mis.clear();
+  for (int64_t v = 0; v < n; v++)
+    if (g->hactive[v] && state[v] == kIn)
+      mis.push_back(v);
+  const int64_t tailCount = g->activeCount - (int64_t)mis.size();
+  tail.assign(tailCount > 0 ? tailCount : 0, 0);
+  if (tailCount > 0) {
+    if (g->keys.alloc(tailCount) || g->vals.alloc(tailCount))
+      return -1;
+    if (set_flag(g, 0, 0))
+      return -1;
+    hipLaunchKernelGGL(k_tail_keys, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->off.p,
+                       g->adj.p, g->active.p, g->state.p, n, g->keys.p, g->vals.p,
+                       g->flags.p);
+    GVH_CHECK_LAUNCH();
+    if (g->sorter.sort(g->keys.p, g->vals.p, tailCount, 0, 64, st))
+      return -1;
+    std::vector<uint32_t> order(tailCount);
+    GVH_CHECK(hipMemcpyAsync(order.data(), g->vals.p, sizeof(uint32_t) * tailCount,
+                             hipMemcpyDeviceToHost, st));
+    GVH_CHECK(hipStreamSynchronize(st));
+    for (int64_t r = 0; r < tailCount; r++)
+      tail[r] = order[r];
+  }
+  return 0;
+}
+
+// Device buffers of the layer selection, kept across the layers of one
+// filtration (the conflict-list stride only grows).
+struct LayerScratch {
+  DevBuf<int32_t> candDev, lists, listsD, listCount;
+  DevBuf<uint8_t> candFlag, cstate;
+  int64_t stride = 32;
+};
+
+// Layer i >= 2 (iterMISFiltration, gvizGRIPEmbedder.c:672-739): of the
+// candidates `cur` (ascending), the lexicographically-first set in which no
+// two lie within BFS distance 2^(i-1) through the whole subgraph; picked[j]
+// = 1 for the chosen.
+int next_layer(gvizhip_grip *g, int64_t i, const std::vector<int64_t> &cur,
+               std::vector<uint8_t> &picked, LayerScratch &ls) {
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  const int64_t m = (int64_t)cur.size();
+  if (ls.candFlag.alloc(n))
+    return -1;
+  const int64_t radius = (int64_t)1 << (i - 1 < 62 ? i - 1 : 62);
+  std::vector<int32_t> c32(m > 0 ? m : 1);
+  std::vector<uint8_t> flag(n, 0);
+  for (int64_t j = 0; j < m; j++) {
+    c32[j] = (int32_t)cur[j];
+    flag[cur[j]] = 1;
+  }
+  if (ls.candDev.alloc(m > 0 ? m : 1) || ls.cstate.alloc(m > 0 ? m : 1) ||
+      ls.listCount.alloc(m > 0 ? m : 1))
+    return -1;
+  GVH_CHECK(hipMemcpyAsync(ls.candDev.p, c32.data(), sizeof(int32_t) * (m > 0 ? m : 1),
+                           hipMemcpyHostToDevice, st));
+  GVH_CHECK(hipMemcpyAsync(ls.candFlag.p, flag.data(), n, hipMemcpyHostToDevice, st));
+  // conflict lists: lower-id candidates within the radius (candidate index
+  // space: the candidates are ascending, so a lower id is a lower index)
+  int64_t &stride = ls.stride;
+  for (;;) {
+    if (ls.lists.alloc(m * stride) || ls.listsD.alloc(m * stride) || set_flag(g, 0, 0))
+      return -1;
+    const int maxDepth = radius > INT32_MAX ? INT32_MAX : (int)radius;
+    if (run_queries(g, ls.candDev.p, nullptr, m, (int)stride, stride, ls.candFlag.p, 0,
+                    maxDepth, true, false, ls.lists.p, ls.listsD.p, ls.listCount.p))
+      return -1;
+    int32_t ov = 0;
+    if (read_flag(g, 0, &ov))
+      return -1;
+    if (!ov)
+      break;
+    stride *= 2;
+  }
+  // map conflict vertex ids to candidate indices on the host
+  std::vector<int32_t> lst(m * stride), cnt(m > 0 ? m : 1);
+  GVH_CHECK(hipMemcpy(lst.data(), ls.lists.p, sizeof(int32_t) * m * stride,
+                      hipMemcpyDeviceToHost));
+  GVH_CHECK(hipMemcpy(cnt.data(), ls.listCount.p, sizeof(int32_t) * (m > 0 ? m : 1),
+                      hipMemcpyDeviceToHost));
+  for (int64_t j = 0; j < m; j++)
+    for (int32_t t = 0; t < cnt[j]; t++) {
+      int32_t v = lst[j * stride + t];
+      lst[j * stride + t] =
+          (int32_t)(std::lower_bound(cur.begin(), cur.end(), (int64_t)v) - cur.begin());
+    }
+  GVH_CHECK(hipMemcpy(ls.lists.p, lst.data(), sizeof(int32_t) * m * stride,
+                      hipMemcpyHostToDevice));
+  GVH_CHECK(hipMemsetAsync(ls.cstate.p, 0, m > 0 ? m : 1, st));
+  int32_t undecided = m > 0;
+  int64_t guard = 0;
+  while (undecided) {
+    if (set_flag(g, 0, 0))
+      return -1;
+    for (int r = 0; r < 4; r++)
+      hipLaunchKernelGGL(k_lfmis_lists, dim3(grid_for(m, kB)), dim3(kB), 0, st,
+                         ls.lists.p, ls.listCount.p, m, stride, ls.cstate.p, g->flags.p);
+    GVH_CHECK_LAUNCH();
+    if (read_flag(g, 0, &undecided))
+      return -1;
+    if (++guard > 4 * (m + 4)) {
+      set_error("gvizhip_grip_mis_filtration: layer rounds did not settle");
+      return -1;
+    }
+  }
+  std::vector<uint8_t> cs(m > 0 ? m : 1);
+  GVH_CHECK(hipMemcpy(cs.data(), ls.cstate.p, m > 0 ? m : 1, hipMemcpyDeviceToHost));
+  picked.assign(m, 0);
+  for (int64_t j = 0; j < m; j++)
+    picked[j] = cs[j] == kIn;
+  return 0;
+}
+
 // gripPickFarCandidate / migrateOneToFinalLayer (gvizGRIPEmbedder.c:507-597):
 // a multi-source BFS of depth <= 64 from the final set, run on the host (the
 // final layer holds at most dim + 1 vertices).
@@ -1783,133 +1925,27 @@ int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
     set_error("gvizhip_grip_mis_filtration: invalid arguments");
     return -1;
   }
-  const int64_t n = g->n;
-  hipStream_t st = g->stream;
   std::vector<int64_t> bord;
   std::vector<int64_t> filt(g->activeCount);
   bord.push_back(g->activeCount);
   // ---- layer 1: lexicographically-first MIS of the subgraph
-  GVH_CHECK(hipMemsetAsync(g->state.p, 0, n, st));
-  int32_t undecided = 1;
-  int rounds = 0;
-  while (undecided) {
-    if (set_flag(g, 0, 0))
-      return -1;
-    for (int r = 0; r < 4; r++)
-      hipLaunchKernelGGL(k_lfmis_graph, dim3(grid_for(n, kB)), dim3(kB), 0, st,
-                         g->off.p, g->adj.p, g->active.p, n, g->state.p, g->flags.p);
-    GVH_CHECK_LAUNCH();
-    if (read_flag(g, 0, &undecided))
-      return -1;
-    if (++rounds > 4 * (n + 4)) {
-      set_error("gvizhip_grip_mis_filtration: layer-1 rounds did not settle");
-      return -1;
-    }
-  }
-  std::vector<uint8_t> state(n);
-  GVH_CHECK(hipMemcpy(state.data(), g->state.p, n, hipMemcpyDeviceToHost));
-  std::vector<int64_t> cur;
-  for (int64_t v = 0; v < n; v++)
-    if (g->hactive[v] && state[v] == kIn)
-      cur.push_back(v);
-  const int64_t tailCount = g->activeCount - (int64_t)cur.size();
-  if (tailCount > 0) {
-    if (g->keys.alloc(tailCount) || g->vals.alloc(tailCount))
-      return -1;
-    if (set_flag(g, 0, 0))
-      return -1;
-    hipLaunchKernelGGL(k_tail_keys, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->off.p,
-                       g->adj.p, g->active.p, g->state.p, n, g->keys.p, g->vals.p,
-                       g->flags.p);
-    GVH_CHECK_LAUNCH();
-    if (g->sorter.sort(g->keys.p, g->vals.p, tailCount, 0, 64, st))
-      return -1;
-    std::vector<uint32_t> order(tailCount);
-    GVH_CHECK(hipMemcpyAsync(order.data(), g->vals.p, sizeof(uint32_t) * tailCount,
-                             hipMemcpyDeviceToHost, st));
-    GVH_CHECK(hipStreamSynchronize(st));
-    for (int64_t r = 0; r < tailCount; r++)
-      filt[g->activeCount - 1 - r] = order[r];
-  }
+  std::vector<int64_t> cur, tail;
+  if (first_layer(g, cur, tail))
+    return -1;
+  for (size_t r = 0; r < tail.size(); r++)
+    filt[g->activeCount - 1 - (int64_t)r] = tail[r];
   bord.push_back((int64_t)cur.size());
   // ---- layers i >= 2: LF-MIS of the distance-2^(i-1) conflict graph
-  DevBuf<int32_t> candDev, lists, listsD, listCount;
-  DevBuf<uint8_t> candFlag, cstate;
-  if (candFlag.alloc(n))
-    return -1;
+  LayerScratch ls;
   int64_t i = 2;
-  int64_t stride = 32;
   for (;;) {
-    const int64_t m = (int64_t)cur.size();
-    const int64_t radius = (int64_t)1 << (i - 1 < 62 ? i - 1 : 62);
-    std::vector<int32_t> c32(m > 0 ? m : 1);
-    std::vector<uint8_t> flag(n, 0);
-    for (int64_t j = 0; j < m; j++) {
-      c32[j] = (int32_t)cur[j];
-      flag[cur[j]] = 1;
-    }
-    if (candDev.alloc(m > 0 ? m : 1) || cstate.alloc(m > 0 ? m : 1) ||
-        listCount.alloc(m > 0 ? m : 1))
+    std::vector<uint8_t> picked;
+    if (next_layer(g, i, cur, picked, ls))
       return -1;
-    GVH_CHECK(hipMemcpyAsync(candDev.p, c32.data(), sizeof(int32_t) * (m > 0 ? m : 1),
-                             hipMemcpyHostToDevice, st));
-    GVH_CHECK(hipMemcpyAsync(candFlag.p, flag.data(), n, hipMemcpyHostToDevice, st));
-    // conflict lists: lower-id candidates within the radius (candidate index
-    // space: the candidates are ascending, so a lower id is a lower index)
-    std::vector<int32_t> idxOf;
-    for (;;) {
-      if (lists.alloc(m * stride) || listsD.alloc(m * stride) || set_flag(g, 0, 0))
-        return -1;
-      const int maxDepth = radius > INT32_MAX ? INT32_MAX : (int)radius;
-      if (run_queries(g, candDev.p, nullptr, m, (int)stride, stride, candFlag.p,
-                      0, maxDepth, true, false, lists.p, listsD.p,
-                      listCount.p))
-        return -1;
-      int32_t ov = 0;
-      if (read_flag(g, 0, &ov))
-        return -1;
-      if (!ov)
-        break;
-      stride *= 2;
-    }
-    // map conflict vertex ids to candidate indices on the host
-    std::vector<int32_t> lst(m * stride), cnt(m > 0 ? m : 1);
-    GVH_CHECK(hipMemcpy(lst.data(), lists.p, sizeof(int32_t) * m * stride,
-                        hipMemcpyDeviceToHost));
-    GVH_CHECK(hipMemcpy(cnt.data(), listCount.p, sizeof(int32_t) * (m > 0 ? m : 1),
-                        hipMemcpyDeviceToHost));
-    for (int64_t j = 0; j < m; j++)
-      for (int32_t t = 0; t < cnt[j]; t++) {
-        int32_t v = lst[j * stride + t];
-        lst[j * stride + t] = (int32_t)(std::lower_bound(cur.begin(), cur.end(),
-                                                         (int64_t)v) -
-                                        cur.begin());
-      }
-    GVH_CHECK(hipMemcpy(lists.p, lst.data(), sizeof(int32_t) * m * stride,
-                        hipMemcpyHostToDevice));
-    GVH_CHECK(hipMemsetAsync(cstate.p, 0, m > 0 ? m : 1, st));
-    undecided = m > 0;
-    int64_t guard = 0;
-    while (undecided) {
-      if (set_flag(g, 0, 0))
-        return -1;
-      for (int r = 0; r < 4; r++)
-        hipLaunchKernelGGL(k_lfmis_lists, dim3(grid_for(m, kB)), dim3(kB), 0, st,
-                           lists.p, listCount.p, m, stride, cstate.p, g->flags.p);
-      GVH_CHECK_LAUNCH();
-      if (read_flag(g, 0, &undecided))
-        return -1;
-      if (++guard > 4 * (m + 4)) {
-        set_error("gvizhip_grip_mis_filtration: layer rounds did not settle");
-        return -1;
-      }
-    }
-    std::vector<uint8_t> cs(m > 0 ? m : 1);
-    GVH_CHECK(hipMemcpy(cs.data(), cstate.p, m > 0 ? m : 1, hipMemcpyDeviceToHost));
     std::vector<int64_t> next;
     int64_t ptr = bord[i - 1] - 1;
-    for (int64_t j = 0; j < m; j++) {
-      if (cs[j] == kIn)
+    for (size_t j = 0; j < cur.size(); j++) {
+      if (picked[j])
         next.push_back(cur[j]);
       else
         filt[ptr--] = cur[j];
@@ -1942,6 +1978,76 @@ int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
   if (gvizhip_grip_set_filtration(g, filtration, g->activeCount))
     return -1;
   return layers;
+}
+
+int64_t gvizhip_grip_mis_first(gvizhip_grip *g, uint8_t *mis, int64_t *tail) {
+  if (!g || !mis || !tail) {
+    set_error("gvizhip_grip_mis_first: invalid arguments");
+    return -1;
+  }
+  std::vector<int64_t> cur, t;
+  if (first_layer(g, cur, t))
+    return -1;
+  memset(mis, 0, (size_t)g->n);
+  for (int64_t v : cur)
+    mis[v] = 1;
+  for (size_t r = 0; r < t.size(); r++)
+    tail[r] = t[r];
+  return (int64_t)cur.size();
+}
+
+int64_t gvizhip_grip_mis_layer(gvizhip_grip *g, int64_t layer, const int64_t *cand,
+                               int64_t count, uint8_t *picked) {
+  if (!g || layer < 2 || count < 0 || (count > 0 && (!cand || !picked))) {
+    set_error("gvizhip_grip_mis_layer: invalid arguments");
+    return -1;
+  }
+  std::vector<int64_t> cur(cand, cand + count);
+  for (int64_t j = 0; j < count; j++)
+    if (cur[j] < 0 || cur[j] >= g->n || (j > 0 && cur[j] <= cur[j - 1])) {
+      set_error("gvizhip_grip_mis_layer: candidates must be ascending vertex ids");
+      return -1;
+    }
+  LayerScratch ls;
+  std::vector<uint8_t> pk;
+  if (next_layer(g, layer, cur, pk, ls))
+    return -1;
+  int64_t c = 0;
+  for (int64_t j = 0; j < count; j++) {
+    picked[j] = pk[j];
+    c += pk[j];
+  }
+  return c;
+}
+
+int gvizhip_grip_migrate(gvizhip_grip *g, int64_t *filtration, int64_t filtrationCount,
+                         int64_t *borders, int64_t layerCount, int64_t count) {
+  if (!g || !filtration || !borders || count < 1 || count > layerCount ||
+      filtrationCount != g->activeCount) {
+    set_error("gvizhip_grip_migrate: invalid arguments");
+    return -1;
+  }
+  std::vector<int64_t> filt(filtration, filtration + filtrationCount);
+  std::vector<int64_t> bord(borders, borders + layerCount);
+  for (int64_t k = 0; k < layerCount; k++)
+    if (bord[k] < 0 || bord[k] > filtrationCount) {
+      set_error("gvizhip_grip_migrate: border out of range");
+      return -1;
+    }
+  for (int64_t v : filt)
+    if (v < 0 || v >= g->n) {
+      set_error("gvizhip_grip_migrate: filtration entry out of range");
+      return -1;
+    }
+  const int64_t before = bord[count - 1];
+  migrate_one(g, filt, bord, count);
+  if (bord[count - 1] == before)
+    return -1; // every pool is exhausted
+  for (int64_t k = 0; k < filtrationCount; k++)
+    filtration[k] = filt[k];
+  for (int64_t k = 0; k < layerCount; k++)
+    borders[k] = bord[k];
+  return 0;
 }
 
 int gvizhip_grip_knn(gvizhip_grip *g, const uint8_t *visible, const int64_t *sources,

@@ -14,6 +14,7 @@
  * for another.
  */
 #include "gviz/gvizGRIPEmbedder.h"
+#include "gviz/gvizGRIPInternal.h"
 
 #include "gviz_hip.h"
 #include "gviz_side.h"
@@ -486,36 +487,139 @@ static int ensure_device(gvizGRIPState *s) {
   return 0;
 }
 
+/* ------------------------------------------------- white-box stages */
+
+static size_t active_count(const gvizGRIPState *s) { return border_at(s, 0); }
+
+/* src/embedders/gvizGRIPEmbedder.c:450-483 */
+void makeFirstMISPartition(gvizGRIPState *state, gvizVertexSubset out) {
+  if (ensure_device(state) != 0)
+    return;
+  grip_side *x = side_of(state);
+  const gvizSubgraph *sg = &((gvizEmbeddedGraph *)state)->subgraph;
+  size_t n = gvizGraphSize(sg->g), active = active_count(state);
+  uint8_t *mis = malloc(n ? n : 1);
+  int64_t *tail = malloc(sizeof(int64_t) * (active ? active : 1));
+  int64_t cnt = mis && tail ? gvizhip_grip_mis_first(x->device, mis, tail) : -1;
+  if (cnt >= 0) {
+    for (size_t v = 0; v < n; v++)
+      if (mis[v])
+        gvizVertexSubsetShowVertex(out, v);
+    for (size_t r = 0; r + (size_t)cnt < active; r++)
+      state->misFiltration[active - 1 - r] = (size_t)tail[r];
+    size_t border = gvizVertexSubsetCount(out, n);
+    gvizArrayPush(&state->misBorder, &border);
+  }
+  free(mis);
+  free(tail);
+}
+
+/* :672-739 */
+int iterMISFiltration(gvizGRIPState *state, size_t i, gvizVertexSubset vertices) {
+  if (i < 2 || i > state->misBorder.count || ensure_device(state) != 0)
+    return 0;
+  grip_side *x = side_of(state);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)state;
+  size_t n = gvizGraphSize(e->subgraph.g);
+  int64_t *cand = malloc(sizeof(int64_t) * (n ? n : 1));
+  uint8_t *picked = malloc(n ? n : 1);
+  int cont = 0;
+  if (!cand || !picked)
+    goto done;
+  size_t m = 0;
+  for (size_t v = 0; v < n; v++)
+    if (gvizVertexSubsetTest(vertices, v))
+      cand[m++] = (int64_t)v;
+  int64_t chosen = gvizhip_grip_mis_layer(x->device, (int64_t)i, cand, (int64_t)m, picked);
+  if (chosen < 0)
+    goto done;
+  /* the rest, in ascending order, backwards from the previous layer's end */
+  size_t ptr = border_at(state, i - 1);
+  for (size_t j = 0; j < m; j++)
+    if (!picked[j])
+      state->misFiltration[--ptr] = (size_t)cand[j];
+  size_t border = (size_t)chosen;
+  if (gvizArrayPush(&state->misBorder, &border) != 0)
+    goto done;
+  gvizVertexSubsetClearAll(vertices, n);
+  for (size_t j = 0; j < m; j++)
+    if (picked[j])
+      gvizVertexSubsetShowVertex(vertices, (size_t)cand[j]);
+  cont = border > e->embedding.dim + 1;
+done:
+  free(cand);
+  free(picked);
+  return cont;
+}
+
+/* :576-597 */
+int migrateOneToFinalLayer(gvizGRIPState *state, GVIZ_BIT_ARRAY finalLayer, size_t count) {
+  (void)finalLayer;
+  if (count < 1 || count > state->misBorder.count || ensure_device(state) != 0)
+    return -1;
+  grip_side *x = side_of(state);
+  size_t active = active_count(state), layers = state->misBorder.count;
+  int64_t *filt = malloc(sizeof(int64_t) * (active ? active : 1));
+  int64_t *bord = malloc(sizeof(int64_t) * layers);
+  int rc = -1;
+  if (filt && bord) {
+    for (size_t k = 0; k < active; k++)
+      filt[k] = (int64_t)state->misFiltration[k];
+    for (size_t k = 0; k < layers; k++)
+      bord[k] = (int64_t)border_at(state, k);
+    rc = gvizhip_grip_migrate(x->device, filt, (int64_t)active, bord, (int64_t)layers,
+                              (int64_t)count);
+    if (rc == 0) {
+      for (size_t k = 0; k < active; k++)
+        state->misFiltration[k] = (size_t)filt[k];
+      for (size_t k = 0; k < layers; k++)
+        *(size_t *)gvizArrayAtIndex(&state->misBorder, k) = (size_t)bord[k];
+    }
+  }
+  free(filt);
+  free(bord);
+  return rc;
+}
+
+/* :741-777 */
+size_t createMISFiltration(gvizGRIPState *state) {
+  if (ensure_device(state) != 0)
+    return 0;
+  grip_side *x = side_of(state);
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)state;
+  size_t active = active_count(state);
+  int64_t *filt = malloc(sizeof(int64_t) * (active ? active : 1));
+  int64_t borders[MIS_MAX_LAYERS];
+  if (!filt)
+    return 0;
+  int64_t layers = gvizhip_grip_mis_filtration(x->device, (int64_t)e->embedding.dim, filt,
+                                               borders);
+  if (layers <= 0 || layers > MIS_MAX_LAYERS) {
+    free(filt);
+    return 0;
+  }
+  for (size_t i = 0; i < active; i++)
+    state->misFiltration[i] = (size_t)filt[i];
+  free(filt);
+  state->misBorder.count = 0;
+  for (int64_t i = 0; i < layers; i++) {
+    size_t b = (size_t)borders[i];
+    gvizArrayPush(&state->misBorder, &b);
+  }
+  return (size_t)layers;
+}
+
 /* src/embedders/gvizGRIPEmbedder.c:1008-1030. Without a usable GPU, or for
  * a filtration the device cannot finish (a subgraph that is not connected),
  * layerCount stays 0 (gvizhip_last_error() says why). */
 void gvizGRIPEmbedderBegin(gvizGRIPState *s) {
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
-  size_t active = border_at(s, 0);
   s->layerCount = 0;
-  if (ensure_device(s) != 0)
-    return;
-  int64_t *filt = malloc(sizeof(int64_t) * active);
-  int64_t borders[MIS_MAX_LAYERS];
-  if (!filt)
+  size_t layers = createMISFiltration(s);
+  if (layers == 0)
     return;
   grip_side *x = side_of(s);
-  int64_t layers = gvizhip_grip_mis_filtration(x->device, (int64_t)e->embedding.dim,
-                                               filt, borders);
-  if (layers <= 0 || layers > MIS_MAX_LAYERS) {
-    free(filt);
-    s->layerCount = 0;
-    return;
-  }
-  for (size_t i = 0; i < active; i++)
-    s->misFiltration[i] = (size_t)filt[i];
-  free(filt);
-  s->misBorder.count = 0;
-  for (int64_t i = 0; i < layers; i++) {
-    size_t b = (size_t)borders[i];
-    gvizArrayPush(&s->misBorder, &b);
-  }
-  s->layerCount = (size_t)layers;
+  s->layerCount = layers;
   s->currLayer = s->layerCount - 1;
   s->currRound = 0;
 

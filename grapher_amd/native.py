@@ -130,6 +130,9 @@ def lib():
         "gvizhip_grip_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, _i64]),
         "gvizhip_grip_mis_filtration": (_i64, [_vp, _i64, _vp, _vp]),
         "gvizhip_grip_set_filtration": (C.c_int, [_vp, _vp, _i64]),
+        "gvizhip_grip_mis_first": (_i64, [_vp, _vp, _vp]),
+        "gvizhip_grip_mis_layer": (_i64, [_vp, _i64, _vp, _i64, _vp]),
+        "gvizhip_grip_migrate": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _i64]),
         "gvizhip_grip_knn": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
         "gvizhip_grip_knn_batch": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp]),
         "gvizhip_grip_upload": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -197,6 +200,7 @@ DECLARED = [
     "gvizhip_tutte_sweeps_noalloc",
     "gvizhip_grip_create", "gvizhip_grip_mis_filtration",
     "gvizhip_grip_set_filtration", "gvizhip_grip_knn", "gvizhip_grip_knn_batch",
+    "gvizhip_grip_mis_first", "gvizhip_grip_mis_layer", "gvizhip_grip_migrate",
     "gvizhip_grip_upload", "gvizhip_grip_download", "gvizhip_grip_set_dimension",
     "gvizhip_grip_upload_nd", "gvizhip_grip_download_nd",
     "gvizhip_grip_clear_decorators", "gvizhip_grip_update_knn",

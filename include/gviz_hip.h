@@ -249,6 +249,26 @@ int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
  * filtration on the device; returns the layer count. */
 int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
                                     int64_t *filtration, int64_t *borders);
+/* The stages of that filtration, for the reference's white-box entry points
+ * (src/embedders/gvizGRIPInternal.h:22-40).
+ * makeFirstMISPartition (:450-483): mis[n] = 1 on the lexicographically-
+ * first MIS of the subgraph; tail[activeCount - misCount] = the other
+ * subgraph vertices in the order the reference writes them backwards from
+ * the end of misFiltration (tail[r] is misFiltration[activeCount-1-r]).
+ * Returns misCount, or -1. */
+int64_t gvizhip_grip_mis_first(gvizhip_grip *g, uint8_t *mis, int64_t *tail);
+/* iterMISFiltration's selection (:672-739) for layer >= 2: of the
+ * candidates cand[count] (ascending ids), picked[j] = 1 on the
+ * lexicographically-first set whose members lie more than 2^(layer-1) BFS
+ * steps apart through the whole subgraph. Returns the picked count. */
+int64_t gvizhip_grip_mis_layer(gvizhip_grip *g, int64_t layer, const int64_t *cand,
+                               int64_t count, uint8_t *picked);
+/* migrateOneToFinalLayer (:576-597) on a filtration[activeCount] and its
+ * borders[layerCount] (borders[0] = activeCount): moves the candidate
+ * farthest from the final set (layer count-1) into it. 0, or -1 when every
+ * pool is exhausted (arrays unchanged). */
+int gvizhip_grip_migrate(gvizhip_grip *g, int64_t *filtration, int64_t filtrationCount,
+                         int64_t *borders, int64_t layerCount, int64_t count);
 /* Installs a filtration computed elsewhere (count entries). */
 int gvizhip_grip_set_filtration(gvizhip_grip *g, const int64_t *filtration,
                                 int64_t count);

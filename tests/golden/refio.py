@@ -430,3 +430,21 @@ def tutte_begin(n, edges, spring, iters, epsilon, exe=None):
         out["run"] = r.i()
         out["pos"] = r.da(2 * n).reshape(n, 2)
     return out
+
+
+def grip_stages(n, edges, dim, exe):
+    """The filtration built through the white-box stage functions and by
+    createMISFiltration (host API driver only): two (layers, borders,
+    filtration) triples."""
+    w = _W()
+    w.graph(n, edges)
+    w.i(dim)
+    r = _R(_run("grip_stages", w.bytes(), exe))
+    out = []
+    for _ in range(2):
+        layers = r.i()
+        nb = r.i()
+        borders = r.ia(nb)
+        filt = r.ia(int(borders[0]))
+        out.append((layers, borders, filt))
+    return out

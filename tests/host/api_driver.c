@@ -14,6 +14,7 @@
  */
 #include "gviz/gvizForceEmbedder.h"
 #include "gviz/gvizGRIPEmbedder.h"
+#include "gviz/gvizGRIPInternal.h"
 #include "gviz/gvizTutteEmbedder.h"
 #include "gviz/gvizSpringTutteEmbedder.h"
 #include "gviz/gvizGraphLoader.h"
@@ -1334,6 +1335,63 @@ static int mode_knn_cases(void) {
   return bad ? 1 : 0;
 }
 
+/* ------------------------------------------------------- GRIP stages */
+static void write_filtration(const gvizGRIPState *st, size_t layers) {
+  size_t active = *(const size_t *)gvizArrayAtIndex(&st->misBorder, 0);
+  wr_i((int64_t)layers);
+  wr_i((int64_t)st->misBorder.count);
+  for (size_t k = 0; k < st->misBorder.count; k++)
+    wr_i((int64_t)*(const size_t *)gvizArrayAtIndex(&st->misBorder, k));
+  for (size_t k = 0; k < active; k++)
+    wr_i((int64_t)st->misFiltration[k]);
+}
+
+/* The filtration twice: built stage by stage through the white-box
+ * functions as the reference's createMISFiltration composes them
+ * (src/embedders/gvizGRIPEmbedder.c:741-777), then by createMISFiltration
+ * on a fresh state. Each: layer count, borders, filtration. */
+static int mode_grip_stages(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t dim = rd_i();
+  for (int pass = 0; pass < 2; pass++) {
+    gvizGRIPState st;
+    memset(&st, 0, sizeof st);
+    if (gvizGRIPEmbedderInit(&st, gvizSubgraphCreateFull(&g), 0, (size_t)dim) != 0)
+      return 1;
+    size_t layers;
+    if (pass == 0) {
+      gvizVertexSubset cur = gvizVertexSubsetCreateEmpty(&g);
+      makeFirstMISPartition(&st, cur);
+      if (st.misBorder.count != 2)
+        return fail_device("makeFirstMISPartition");
+      size_t i = 2;
+      while (iterMISFiltration(&st, i, cur))
+        i++;
+      if (st.misBorder.count != i + 1)
+        return fail_device("iterMISFiltration");
+      size_t k = 0;
+      for (size_t v = 0; v < (size_t)n; v++)
+        if (gvizVertexSubsetTest(cur, v))
+          st.misFiltration[k++] = v;
+      while (*(size_t *)gvizArrayAtIndex(&st.misBorder, i) < (size_t)dim + 1)
+        if (migrateOneToFinalLayer(&st, cur, i + 1) < 0)
+          break;
+      layers = i + 1;
+      gvizVertexSubsetRelease(cur);
+    } else {
+      layers = createMISFiltration(&st);
+      if (layers == 0)
+        return fail_device("createMISFiltration");
+    }
+    write_filtration(&st, layers);
+    gvizGRIPEmbedderRelease(&st);
+  }
+  gvizGraphRelease(&g);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc == 2 && !strcmp(argv[1], "unit"))
     return mode_unit();
@@ -1374,6 +1432,8 @@ int main(int argc, char **argv) {
     rc = mode_knn();
   else if (!strcmp(argv[1], "knn_cases"))
     rc = mode_knn_cases();
+  else if (!strcmp(argv[1], "grip_stages"))
+    rc = mode_grip_stages();
   fclose(fin);
   fclose(fout);
   return rc;

@@ -38,7 +38,10 @@ def _declared():
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"typedef[^;]*;", "", text, flags=re.S)
         text = re.sub(r"^#define(?:[^\n]*\\\n)*[^\n]*$", "", text, flags=re.M)
-        names.update(re.findall(r"\b(gviz[A-Z][A-Za-z0-9]*)\s*\(", text))
+        found = set(re.findall(r"\b(gviz[A-Z][A-Za-z0-9]*)\s*\(", text))
+        # header-only helpers (static inline) are not library symbols
+        found -= set(re.findall(r"static\s+inline\s+[\w\s\*]*?\b(gviz[A-Z]\w*)\s*\(", text))
+        names.update(found)
     return sorted(names)
 
 
@@ -385,3 +388,20 @@ def test_tutte_begin_matches_reference_up_to_polygon_symmetry(gpu, name):
     print(f"  boundary {len(b)} (shift {sym[0]}, reversed {sym[1]}), run {r['run']} vs "
           f"{ref_run}, max deviation {err:.3g}", flush=True)
     assert err <= 1e-6 * 200.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["grip_rgg300", "grip_grid10", "grip_ba200", "grip_grid16_k4",
+                                  "grip_rgg300_d3", "grip_rgg300_d4"])
+def test_grip_white_box_stages_build_reference_filtration(gpu, name):
+    """makeFirstMISPartition, iterMISFiltration and migrateOneToFinalLayer
+    composed as the reference's createMISFiltration composes them, and
+    createMISFiltration itself, both give the reference's filtration and
+    borders (src/embedders/gvizGRIPInternal.h:22-40)."""
+    f = golden(name)
+    dim = int(f["dim"]) if "dim" in f.files else 2
+    print(f"grip stages {name}", flush=True)
+    for layers, borders, filt in refio.grip_stages(int(f["n"]), f["edges"], dim, exe=DRIVER):
+        assert layers == len(f["borders"])
+        assert np.array_equal(borders, f["borders"])
+        assert np.array_equal(filt, f["filtration"])
