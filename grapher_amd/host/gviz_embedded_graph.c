@@ -330,3 +330,86 @@ int gvizEmbeddedGraphLoadEmbedding(gvizEmbeddedGraph *e, const char *filename) {
 int gvizEmbeddedGraphIsPlanarEmbedded(const gvizEmbeddedGraph *e) {
   return e && e->planarEmbedded;
 }
+
+/* ------------------------------------------------ structure and growth */
+
+const gvizSubgraph *gvizEmbeddedGraphStructure(const gvizEmbeddedGraph *e) {
+  return &e->subgraph;
+}
+
+const gvizDrawMask *gvizEmbeddedGraphGetDrawMask(const gvizEmbeddedGraph *e) {
+  return &e->drawMask;
+}
+
+/* src/embedders/gvizEmbeddedGraph.c:154-192: a bare vertex, present in the
+ * subgraph and visible; positions and mask grow with the subgraph's vertex
+ * capacity (new positions zero). */
+int gvizEmbeddedGraphAddVertex(gvizEmbeddedGraph *e, void *data) {
+  if (!e)
+    return -1;
+  gvizGraph *g = (gvizGraph *)e->subgraph.g;
+  size_t oldCap = e->subgraph.vertexCapacity;
+  if (gvizGraphAddVertex(g, data, NULL, NULL) != 0)
+    return -1;
+  size_t idx = gvizGraphSize(g) - 1;
+  if (gvizSubgraphRebuild(&e->subgraph) != 0)
+    return -1;
+  size_t newCap = e->subgraph.vertexCapacity, dim = e->embedding.dim;
+  if (newCap > oldCap) {
+    double *pos = realloc(e->embedding.vertexPositions, sizeof(double) * newCap * dim);
+    if (!pos)
+      return -1;
+    memset(pos + oldCap * dim, 0, sizeof(double) * (newCap - oldCap) * dim);
+    e->embedding.vertexPositions = pos;
+    GVIZ_BIT_ARRAY mask = calloc(GVIZ_ARRAY_UNITS(newCap) ? GVIZ_ARRAY_UNITS(newCap) : 1, 1);
+    if (!mask)
+      return -1;
+    memcpy(mask, e->drawMask.visibleVertices, GVIZ_ARRAY_UNITS(oldCap));
+    gvizVertexSubsetRelease(e->drawMask.visibleVertices);
+    e->drawMask.visibleVertices = mask;
+  }
+  gvizSubgraphShowVertex(&e->subgraph, idx);
+  gvizEmbeddedGraphDrawMaskShowVertex(e, idx);
+  gvizEmbeddedGraphDrawMaskNotifyChanged(e);
+  return 0;
+}
+
+/* :194-211: a full subgraph rebuilds its edge subset and shows the edge */
+int gvizEmbeddedGraphAddEdge(gvizEmbeddedGraph *e, size_t from, size_t to) {
+  if (!e)
+    return -1;
+  if (gvizGraphAddEdge((gvizGraph *)e->subgraph.g, from, to) != 0)
+    return -1;
+  if (gvizSubgraphIsFull(&e->subgraph)) {
+    if (gvizSubgraphRebuild(&e->subgraph) != 0)
+      return -1;
+    gvizSubgraphShowEdge(&e->subgraph, from, to);
+    gvizEmbeddedGraphDrawMaskNotifyChanged(e);
+  }
+  return 0;
+}
+
+/* --------------------------------------------------------------- highlight */
+/* :491-516: the highlight owns its subsets */
+void gvizEmbeddedGraphClearHighlight(gvizEmbeddedGraph *e) {
+  if (!e)
+    return;
+  if (e->highlight.vs || e->highlight.es.bitset)
+    gvizSubgraphRelease(&e->highlight);
+  memset(&e->highlight, 0, sizeof(e->highlight));
+}
+
+void gvizEmbeddedGraphSetHighlight(gvizEmbeddedGraph *e, gvizSubgraph highlight) {
+  if (!e)
+    return;
+  gvizEmbeddedGraphClearHighlight(e);
+  e->highlight = highlight;
+}
+
+int gvizEmbeddedGraphHasHighlight(const gvizEmbeddedGraph *e) {
+  return e && e->highlight.vs != NULL;
+}
+
+const gvizSubgraph *gvizEmbeddedGraphGetHighlight(const gvizEmbeddedGraph *e) {
+  return gvizEmbeddedGraphHasHighlight(e) ? &e->highlight : NULL;
+}

@@ -519,3 +519,254 @@ int gvizSubgraphToCompactCSR(const gvizSubgraph *sg, int64_t **offsets,
   *count = m;
   return 0;
 }
+
+/* ------------------------------------------------------------ iterators */
+/* src/ds/gvizBitArray.c:44-110: set bits in ascending index order */
+
+gvizBitArrayIterator gvizBitArrayIteratorCreate(GVIZ_BIT_ARRAY arr, size_t size) {
+  gvizBitArrayIterator it = {arr, arr ? size : 0, SIZE_MAX};
+  return it;
+}
+
+gvizBitArrayIterator gvizBitArrayIteratorCreateRange(GVIZ_BIT_ARRAY arr, size_t start,
+                                                     size_t end) {
+  gvizBitArrayIterator it = {arr, arr ? end : 0, start > 0 ? start - 1 : SIZE_MAX};
+  return it;
+}
+
+bool gvizBitArrayIterate(gvizBitArrayIterator *it, size_t *out_idx) {
+  if (!it || !it->arr)
+    return false;
+  for (size_t k = it->idx + 1; k < it->size; k++) {
+    /* skip whole zero bytes */
+    if ((k & 7) == 0 && k + 8 <= it->size && it->arr[k >> 3] == 0) {
+      k += 7;
+      continue;
+    }
+    if (gvizTestBit(it->arr, k)) {
+      it->idx = k;
+      *out_idx = k;
+      return true;
+    }
+  }
+  it->idx = it->size ? it->size - 1 : SIZE_MAX;
+  return false;
+}
+
+gvizBitArrayIterator gvizVertexSubsetIteratorCreate(const gvizVertexSubset vs,
+                                                    size_t nvertices) {
+  return gvizBitArrayIteratorCreate(vs, nvertices);
+}
+
+/* src/ds/gvizSubgraph.c:451-519 */
+gvizSubgraphVertexIterator gvizSubgraphVertexIteratorCreate(const gvizSubgraph *sg) {
+  gvizSubgraphVertexIterator it = {sg, {NULL, 0, SIZE_MAX}};
+  if (sg && sg->g && sg->vs)
+    it.it = gvizVertexSubsetIteratorCreate(sg->vs, sg->vertexCapacity);
+  return it;
+}
+
+bool gvizSubgraphVertexIterate(gvizSubgraphVertexIterator *it, size_t *out_u) {
+  if (!it || !it->sg || !it->sg->g || !it->sg->vs)
+    return false;
+  return gvizBitArrayIterate(&it->it, out_u);
+}
+
+gvizSubgraphNeighborIterator gvizSubgraphNeighborIteratorCreate(const gvizSubgraph *sg,
+                                                                size_t u) {
+  gvizSubgraphNeighborIterator it = {sg, u, 0, {NULL, 0, SIZE_MAX}, 0, NULL,
+                                     GVIZ_SUBGRAPH_NEIGHBOR_ITER_NONE};
+  if (!gvizSubgraphHasVertex(sg, u))
+    return it;
+  it.nb = gvizGraphGetVertexNeighbors(sg->g, u);
+  if (!it.nb)
+    return it;
+  if (gvizSubgraphIsFull(sg)) {
+    it.base = sg->es.layout->vertexOffsets[u];
+    it.it = gvizBitArrayIteratorCreateRange(sg->es.bitset, it.base,
+                                            sg->es.layout->vertexOffsets[u + 1]);
+    it.adj_idx = SIZE_MAX;
+    it.mode = GVIZ_SUBGRAPH_NEIGHBOR_ITER_FULL;
+  } else {
+    it.mode = GVIZ_SUBGRAPH_NEIGHBOR_ITER_INDUCED;
+  }
+  return it;
+}
+
+bool gvizSubgraphNeighborIterate(gvizSubgraphNeighborIterator *it, size_t *out_v) {
+  if (!it)
+    return false;
+  if (it->mode == GVIZ_SUBGRAPH_NEIGHBOR_ITER_FULL) {
+    size_t pos;
+    if (!gvizBitArrayIterate(&it->it, &pos))
+      return false;
+    size_t idx = pos - it->base;
+    if (idx >= it->nb->count)
+      return false;
+    *out_v = *(size_t *)gvizArrayAtIndex(it->nb, idx);
+    return true;
+  }
+  if (it->mode == GVIZ_SUBGRAPH_NEIGHBOR_ITER_INDUCED) {
+    while (it->adj_idx < it->nb->count) {
+      size_t v = *(size_t *)gvizArrayAtIndex(it->nb, it->adj_idx++);
+      if (gvizVertexSubsetTest(it->sg->vs, v)) {
+        *out_v = v;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+/* ------------------------------------------- edge subsets and rebuilds */
+
+size_t gvizEdgeSubsetCount(const gvizEdgeSubset es) {
+  if (!es.bitset || !es.layout)
+    return 0;
+  return bits_count(es.bitset, 0, es.layout->vertexOffsets[es.layout->nvertices]);
+}
+
+size_t gvizEdgeSubsetVertexEdgeCount(const gvizEdgeSubset es, size_t u) {
+  if (!es.bitset || !es.layout || u >= es.layout->nvertices)
+    return 0;
+  return bits_count(es.bitset, es.layout->vertexOffsets[u], es.layout->vertexOffsets[u + 1]);
+}
+
+void gvizEdgeSubsetClearVertex(const gvizEdgeSubset es, size_t u) {
+  if (!es.bitset || !es.layout || u >= es.layout->nvertices)
+    return;
+  for (size_t k = es.layout->vertexOffsets[u]; k < es.layout->vertexOffsets[u + 1]; k++)
+    gvizClearBit(es.bitset, k);
+}
+
+int gvizArrayFindOne(const gvizArray *a, void *item) {
+  if (!a || !item)
+    return -1;
+  for (size_t i = 0; i < a->count; i++)
+    if (!memcmp((char *)a->arr + i * a->elementSize, item, a->elementSize))
+      return (int)i;
+  return -1;
+}
+
+static int find_entry(const gvizGraph *g, size_t u, size_t v, size_t *idx) {
+  gvizArray *nb = gvizGraphGetVertexNeighbors(g, u);
+  int i = nb ? gvizArrayFindOne(nb, &v) : -1;
+  if (i < 0)
+    return -1;
+  *idx = (size_t)i;
+  return 0;
+}
+
+/* src/ds/gvizSubgraph.c:273-289: the first (u, v) entry's bit */
+void gvizSubgraphShowEdge(const gvizSubgraph *sg, size_t u, size_t v) {
+  size_t idx;
+  if (gvizSubgraphIsFull(sg) && find_entry(sg->g, u, v, &idx) == 0)
+    gvizEdgeSubsetShowEdge(sg->es, u, idx);
+}
+
+void gvizSubgraphHideEdge(const gvizSubgraph *sg, size_t u, size_t v) {
+  size_t idx;
+  if (gvizSubgraphIsFull(sg) && find_entry(sg->g, u, v, &idx) == 0)
+    gvizEdgeSubsetHideEdge(sg->es, u, idx);
+}
+
+/* :521-528 */
+void gvizSubgraphMakeFull(gvizSubgraph *sg) {
+  if (!sg || !sg->g || !sg->g->layout || !sg->vs || !sg->es.bitset)
+    return;
+  bits_fill(sg->vs, sg->g->layout->nvertices);
+  bits_fill(sg->es.bitset, sg->g->layout->edgeCount);
+}
+
+/* :236-257: an edge subset holding every parent edge between subset vertices */
+void gvizSubgraphMakeEdgeSubset(gvizSubgraph *sg) {
+  if (!sg || !sg->g || !sg->g->layout || !sg->vs || sg->es.bitset)
+    return;
+  sg->es = gvizEdgeSubsetCreateEmpty(sg->g);
+  if (!sg->es.bitset)
+    return;
+  size_t n = sg->g->layout->nvertices;
+  for (size_t u = 0; u < n; u++) {
+    if (!gvizVertexSubsetTest(sg->vs, u))
+      continue;
+    size_t c = 0;
+    const size_t *nb = gvizGraphNeighbors(sg->g, u, &c);
+    for (size_t k = 0; k < c; k++)
+      if (gvizVertexSubsetTest(sg->vs, nb[k]))
+        gvizEdgeSubsetShowEdge(sg->es, u, k);
+  }
+}
+
+/* :291-378: after the parent graph changed. A vertex-induced view grows its
+ * vertex capacity (doubling) to the graph's size; a full view's edge bits
+ * keep their (vertex, adjacency index) under the rebuilt layout, new
+ * entries start hidden. */
+int gvizSubgraphRebuild(gvizSubgraph *sg) {
+  if (!sg || !sg->g || !sg->vs)
+    return -1;
+  size_t need = gvizGraphSize(sg->g);
+  if (need > sg->vertexCapacity) {
+    size_t cap = sg->vertexCapacity ? sg->vertexCapacity * 2 : 64;
+    while (cap < need)
+      cap *= 2;
+    GVIZ_BIT_ARRAY vs = bits_alloc(cap);
+    if (!vs)
+      return -1;
+    memcpy(vs, sg->vs, GVIZ_ARRAY_UNITS(sg->vertexCapacity));
+    free(sg->vs);
+    sg->vs = vs;
+    sg->vertexCapacity = cap;
+  }
+  if (!sg->es.bitset)
+    return 0;
+  const gvizGraphLayout *old = sg->es.layout;
+  size_t oldN = old ? old->nvertices : 0;
+  size_t *oldOff = NULL;
+  if (old) {
+    oldOff = malloc(sizeof(size_t) * (oldN + 1));
+    if (!oldOff)
+      return -1;
+    memcpy(oldOff, old->vertexOffsets, sizeof(size_t) * (oldN + 1));
+  }
+  gvizGraphBuildLayout((gvizGraph *)sg->g);
+  const gvizGraphLayout *L = sg->g->layout;
+  GVIZ_BIT_ARRAY bits = L ? bits_alloc(L->edgeCount) : NULL;
+  if (!bits) {
+    free(oldOff);
+    return -1;
+  }
+  for (size_t u = 0; u < oldN && u < L->nvertices; u++) {
+    size_t len = oldOff[u + 1] - oldOff[u], now = L->vertexOffsets[u + 1] - L->vertexOffsets[u];
+    for (size_t k = 0; k < len && k < now; k++)
+      if (gvizTestBit(sg->es.bitset, oldOff[u] + k))
+        gvizSetBit(bits, L->vertexOffsets[u] + k);
+  }
+  free(oldOff);
+  free(sg->es.bitset);
+  sg->es.bitset = bits;
+  sg->es.layout = L;
+  return 0;
+}
+
+/* src/ds/gvizGraph.c:174-184: removes the first (from, to) entry, and for
+ * an undirected graph the first (to, from). 0, or -1 when absent. */
+int gvizGraphRemoveEdge(gvizGraph *g, size_t from, size_t to) {
+  if (!g || from >= gvizGraphSize(g) || to >= gvizGraphSize(g))
+    return -1;
+  size_t idx;
+  if (find_entry(g, from, to, &idx) != 0)
+    return -1;
+  gvizArray *a = gvizGraphGetVertexNeighbors(g, from);
+  memmove((size_t *)a->arr + idx, (size_t *)a->arr + idx + 1,
+          sizeof(size_t) * (a->count - idx - 1));
+  a->count--;
+  if (g->directed)
+    return 0;
+  if (find_entry(g, to, from, &idx) != 0)
+    return -1;
+  a = gvizGraphGetVertexNeighbors(g, to);
+  memmove((size_t *)a->arr + idx, (size_t *)a->arr + idx + 1,
+          sizeof(size_t) * (a->count - idx - 1));
+  a->count--;
+  return 0;
+}

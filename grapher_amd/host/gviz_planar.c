@@ -24,6 +24,8 @@
  */
 #include "gviz_planar.h"
 
+#include "gviz/gvizEmbeddedGraph.h"
+
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -412,6 +414,12 @@ static void embedding(LR *g, int64_t root) {
 
 int gviz_planar_largest_face(int64_t n, const int64_t *off, const int64_t *adj,
                              size_t *out, size_t max, size_t *count) {
+  return gviz_planar_embed(n, off, adj, out, max, count, NULL, NULL);
+}
+
+int gviz_planar_embed(int64_t n, const int64_t *off, const int64_t *adj, size_t *out,
+                      size_t max, size_t *count, int64_t *rotation,
+                      int64_t *rotationCount) {
   if (n < 0 || !off || (n > 0 && !adj) || !out || !count)
     return -1;
   *count = 0;
@@ -541,6 +549,25 @@ int gviz_planar_largest_face(int64_t n, const int64_t *off, const int64_t *adj,
     } while (d != bestStart);
     *count = k;
   }
+  /* each vertex's darts in cw order (the face walk takes the dart before
+   * the reverse one), from its earliest dart in adjacency order */
+  if (rotation && rotationCount)
+    for (int64_t v = 0; v < n; v++) {
+      rotationCount[v] = 0;
+      const int64_t f = g.first[v];
+      if (f < 0)
+        continue;
+      int64_t start = f;
+      for (int64_t d = g.cw[f]; d != f; d = g.cw[d])
+        if (d < start)
+          start = d;
+      int64_t k = off[v], d = start;
+      do {
+        rotation[k++] = dst(&g, d);
+        d = g.cw[d];
+      } while (d != start);
+      rotationCount[v] = k - off[v];
+    }
   rc = 0;
 done:
   free(g.twin);
@@ -569,4 +596,145 @@ done:
   free(g.tmp);
   free(seen);
   return rc;
+}
+
+/* ------------------------------------------------- embedded-graph side */
+
+/* gvizPlanarApplyRotationToEmbedding (src/embedders/gvizPlanarEmbedder.c:
+ * 74-104, 160-181, 582-590) with gvizPlanarLargestFaceBoundary. */
+int gviz_planar_embed_largest_face(gvizEmbeddedGraph *e, size_t *out, size_t max,
+                                   size_t *count) {
+  gvizSubgraph *sg = &e->subgraph;
+  gvizGraph *G = (gvizGraph *)sg->g;
+  const size_t n = gvizGraphSize(G);
+  int64_t *off = NULL, *adj = NULL;
+  if (gvizSubgraphToCSR(sg, &off, &adj) != 0)
+    return -1;
+  int64_t *rot = malloc(sizeof(int64_t) * (off[n] > 0 ? off[n] : 1));
+  int64_t *rc = malloc(sizeof(int64_t) * (n ? n : 1));
+  size_t *list = NULL;
+  int res = -1;
+  if (!rot || !rc)
+    goto done;
+  res = gviz_planar_embed((int64_t)n, off, adj, out, max, count, rot, rc);
+  if (res != 0)
+    goto done;
+  res = -1;
+  /* each subgraph vertex's list: its rotation, then the entries that are
+   * not subgraph edges in their order (a neighbour already listed is not
+   * repeated; repeated subgraph edges and loops, which the embedding
+   * ignores, are dropped as the reference drops them) */
+  {
+    size_t cap = 0;
+    for (size_t u = 0; u < n; u++) {
+      size_t c = 0;
+      gvizGraphNeighbors(G, u, &c);
+      if (c > cap)
+        cap = c;
+    }
+    list = malloc(sizeof(size_t) * (cap ? cap : 1));
+    if (!list)
+      goto done;
+    for (size_t u = 0; u < n; u++) {
+      if (!gvizSubgraphHasVertex(sg, u))
+        continue;
+      gvizArray *nb = gvizGraphGetVertexNeighbors(G, u);
+      size_t k = 0;
+      for (int64_t j = 0; j < rc[u]; j++)
+        list[k++] = (size_t)rot[off[u] + j];
+      for (size_t j = 0; j < nb->count; j++) {
+        size_t v = ((size_t *)nb->arr)[j];
+        if (gvizSubgraphHasEdge(sg, u, v))
+          continue;
+        size_t t = 0;
+        while (t < k && list[t] != v)
+          t++;
+        if (t == k)
+          list[k++] = v;
+      }
+      memcpy(nb->arr, list, sizeof(size_t) * k);
+      nb->count = k;
+    }
+  }
+  gvizGraphBuildLayout(G);
+  if (!G->layout)
+    goto done;
+  if (gvizSubgraphIsFull(sg)) {
+    const gvizGraphLayout *L = G->layout;
+    GVIZ_BIT_ARRAY bits = calloc(GVIZ_ARRAY_UNITS(L->edgeCount) ? GVIZ_ARRAY_UNITS(L->edgeCount) : 1, 1);
+    if (!bits)
+      goto done;
+    for (size_t u = 0; u < n; u++)
+      if (gvizSubgraphHasVertex(sg, u))
+        for (int64_t j = 0; j < rc[u]; j++)
+          gvizSetBit(bits, L->vertexOffsets[u] + (size_t)j);
+    free(sg->es.bitset);
+    sg->es.bitset = bits;
+    sg->es.layout = L;
+  }
+  e->planarEmbedded = 1;
+  res = 0;
+done:
+  free(off);
+  free(adj);
+  free(rot);
+  free(rc);
+  free(list);
+  return res;
+}
+
+/* the adjacency entry of v before u (cyclically): gvizPlanarPrevNeighborCCW
+ * (src/embedders/gvizPlanarEmbedder.c:205-211) */
+static int prev_neighbor(const gvizGraph *G, size_t v, size_t u, size_t *w) {
+  size_t c = 0;
+  const size_t *nb = gvizGraphNeighbors(G, v, &c);
+  for (size_t i = 0; i < c; i++)
+    if (nb[i] == u) {
+      *w = nb[i == 0 ? c - 1 : i - 1];
+      return 0;
+    }
+  return -1;
+}
+
+/* boundaryCycleFromSubgraph (src/embedders/gvizTutteEmbedder.c:105-143):
+ * the first edge (u, v) of the highlight (u ascending, v in adjacency
+ * order), then the face walk (u, v) -> (v, entry of v before u) through the
+ * graph's rotation until it returns to (u, v). */
+int gviz_planar_face_from_highlight(const gvizEmbeddedGraph *e, size_t *out, size_t max,
+                                    size_t *count) {
+  *count = 0;
+  const gvizSubgraph *h = gvizEmbeddedGraphGetHighlight(e);
+  if (!h)
+    return -1;
+  size_t su = 0, sv = 0;
+  int found = 0;
+  gvizSubgraphVertexIterator vit = gvizSubgraphVertexIteratorCreate(h);
+  size_t u;
+  while (!found && gvizSubgraphVertexIterate(&vit, &u)) {
+    gvizSubgraphNeighborIterator nit = gvizSubgraphNeighborIteratorCreate(h, u);
+    size_t v;
+    while (gvizSubgraphNeighborIterate(&nit, &v))
+      if (gvizSubgraphHasEdge(h, u, v)) {
+        su = u;
+        sv = v;
+        found = 1;
+        break;
+      }
+  }
+  if (!found)
+    return -1;
+  size_t cu = su, cv = sv;
+  for (;;) {
+    if (*count >= max)
+      return -1;
+    out[(*count)++] = cv;
+    size_t w;
+    if (prev_neighbor(h->g, cv, cu, &w) != 0)
+      return -1;
+    cu = cv;
+    cv = w;
+    if (cu == su && cv == sv)
+      break;
+  }
+  return *count >= 3 ? 0 : -1;
 }

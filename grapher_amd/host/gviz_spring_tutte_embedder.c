@@ -148,29 +148,29 @@ void gvizSpringTutteEmbedderConfigure(gvizSpringTutteState *s, double stiffness,
     s->damping = damping;
 }
 
-/* src/embedders/gvizSpringTutteEmbedder.c:164-191: planarity test and rotation system, then the largest face
- * (gviz_planar.c) fixed on a convex polygon of radius 200 and the interior
- * seeded at its centroid. -2 for a graph that is not planar, -1 for another
- * dimension than 2 or a failure. The reference's planarity suite installs
- * its rotation into the parent's adjacency order; this engine keeps the
- * insertion order (only the order of each barycentre's sum differs). */
+/* src/embedders/gvizSpringTutteEmbedder.c:164-191: planarity test and rotation system
+ * (gviz_planar.c), the rotation written into the graph's adjacency order
+ * as the reference's planarity step writes its own (planarEmbedded set),
+ * then the largest face fixed on a convex polygon of radius 200 and the
+ * interior seeded at its centroid. -2 for a graph that is not planar, -1 for
+ * another dimension than 2 or a failure. The device engine is rebuilt from
+ * the new adjacency order at the next step. */
 int gvizSpringTutteEmbedderBegin(gvizSpringTutteState *s) {
   if (!s || dim_of(s) != 2)
     return -1;
   gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
-  int64_t *off = NULL, *adj = NULL;
-  if (gvizSubgraphToCSR(&e->subgraph, &off, &adj) != 0)
-    return -1;
   const size_t N = num_vertices(s);
   size_t *boundary = malloc(sizeof(size_t) * (N ? N : 1));
   size_t count = 0;
-  int rc = boundary ? gviz_planar_largest_face((int64_t)N, off, adj, boundary, N, &count)
-                    : -1;
-  free(off);
-  free(adj);
+  int rc = boundary ? gviz_planar_embed_largest_face(e, boundary, N, &count) : -1;
   if (rc != 0) {
     free(boundary);
     return rc == -2 ? -2 : -1;
+  }
+  spring_side *x = side_of(s);
+  if (x && x->device) {
+    gvizhip_tutte_destroy(x->device);
+    x->device = NULL;
   }
   gvizSpringTutteFixConvexPolygon(s, boundary, count, 200.0);
   free(boundary);
@@ -179,9 +179,28 @@ int gvizSpringTutteEmbedderBegin(gvizSpringTutteState *s) {
   return 0;
 }
 
+/* src/embedders/gvizSpringTutteEmbedder.c:371-397: the face walked from the highlight's first
+ * edge through the rotation Begin installed, fixed on a convex polygon of
+ * radius 200 (interior positions kept); -1 without a planar embedding or a
+ * highlight. */
 int gvizSpringTutteEmbedderFixOuterFace(gvizSpringTutteState *s) {
-  (void)s;
-  return -1;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)s;
+  if (!s || !gvizEmbeddedGraphIsPlanarEmbedded(e) || !gvizEmbeddedGraphHasHighlight(e))
+    return -1;
+  const size_t N = num_vertices(s);
+  size_t *boundary = malloc(sizeof(size_t) * (N ? N : 1));
+  size_t count = 0;
+  if (!boundary || gviz_planar_face_from_highlight(e, boundary, N, &count) != 0) {
+    free(boundary);
+    return -1;
+  }
+  gvizSpringTutteFixConvexPolygon(s, boundary, count, 200.0);
+  free(boundary);
+  s->iteration = 0;
+  s->lastMaxDelta = 0.0;
+  s->converged = 0;
+  s->begun = 1;
+  return 0;
 }
 
 /* src/embedders/gvizSpringTutteEmbedder.c:193-223: pins the vertices at the

@@ -82,12 +82,25 @@ typedef struct gvizEmbeddedGraph {
   gvizActionRegistry actions;
   gvizDrawMask drawMask;
   gvizStatRegistry stats;
-  int planarEmbedded;         /* 0 here: Begin keeps the insertion order */
-  gvizSubgraph highlight;     /* a view the front end highlights; empty here */
+  int planarEmbedded;         /* 1 once a Tutte Begin installed its rotation */
+  gvizSubgraph highlight;     /* a view the front end highlights (owned) */
 } gvizEmbeddedGraph;
 
 int gvizEmbeddedGraphInit(gvizEmbeddedGraph *e, gvizSubgraph subgraph,
                           size_t dimension);
+/* The embedded subgraph / the draw mask (read-only). */
+const gvizSubgraph *gvizEmbeddedGraphStructure(const gvizEmbeddedGraph *e);
+const gvizDrawMask *gvizEmbeddedGraphGetDrawMask(const gvizEmbeddedGraph *e);
+/* Grows the underlying graph: a bare vertex (shown in the subgraph and the
+ * mask; positions zero) or an edge (shown when the subgraph is full).
+ * Embedders keep the graph they saw at Init / Begin. 0 / -1. */
+int gvizEmbeddedGraphAddVertex(gvizEmbeddedGraph *e, void *data);
+int gvizEmbeddedGraphAddEdge(gvizEmbeddedGraph *e, size_t from, size_t to);
+/* The highlighted view (a face picked in a viewer; Set takes ownership). */
+void gvizEmbeddedGraphSetHighlight(gvizEmbeddedGraph *e, gvizSubgraph highlight);
+void gvizEmbeddedGraphClearHighlight(gvizEmbeddedGraph *e);
+int gvizEmbeddedGraphHasHighlight(const gvizEmbeddedGraph *e);
+const gvizSubgraph *gvizEmbeddedGraphGetHighlight(const gvizEmbeddedGraph *e);
 void gvizEmbeddedGraphRelease(gvizEmbeddedGraph *e);
 size_t gvizEmbeddedGraphDim(const gvizEmbeddedGraph *e);
 size_t gvizEmbeddedGraphPositionCount(const gvizEmbeddedGraph *e);

@@ -64,6 +64,23 @@ typedef unsigned char GVIZ_BIT_UNIT;
 #define gvizTestBit(A, k) (A[(k) / GVIZ_BITS_PER_UNIT] & (1 << ((k) % GVIZ_BITS_PER_UNIT)))
 #define GVIZ_ARRAY_UNITS(N) ((N + GVIZ_BITS_PER_UNIT - 1) / GVIZ_BITS_PER_UNIT)
 
+/* Set bits in ascending index order (reference include/ds/gvizBitArray.h:
+ * 111-115): idx is the last index returned, SIZE_MAX before the first. */
+typedef struct {
+  GVIZ_BIT_ARRAY arr;
+  size_t size;
+  size_t idx;
+} gvizBitArrayIterator;
+
+gvizBitArrayIterator gvizBitArrayIteratorCreate(GVIZ_BIT_ARRAY arr, size_t size);
+/* Bits [start, end). */
+gvizBitArrayIterator gvizBitArrayIteratorCreateRange(GVIZ_BIT_ARRAY arr, size_t start,
+                                                     size_t end);
+bool gvizBitArrayIterate(gvizBitArrayIterator *it, size_t *out_idx);
+
+/* Index of the first element equal (bytewise) to *item, or -1. */
+int gvizArrayFindOne(const gvizArray *a, void *item);
+
 /* ---------------------------------------------------------------- graphs */
 typedef struct Vertex {
   void *data;          /* caller's data, not owned */
@@ -107,6 +124,9 @@ void gvizGraphBuildLayout(gvizGraph *g);
 gvizArray *gvizGraphGetVertexNeighbors(const gvizGraph *g, size_t idx);
 void *gvizGraphGetVertexData(gvizGraph *g, size_t idx);
 void gvizGraphSetVertexData(gvizGraph *g, size_t idx, void *data);
+/* Removes the first (from, to) entry and, undirected, the first (to, from).
+ * 0 / -1. */
+int gvizGraphRemoveEdge(gvizGraph *g, size_t from, size_t to);
 /* Neighbours of u in insertion order (no reference counterpart). */
 const size_t *gvizGraphNeighbors(const gvizGraph *g, size_t u, size_t *count);
 
@@ -155,6 +175,55 @@ size_t gvizSubgraphDegree(const gvizSubgraph *sg, size_t u);
 size_t gvizSubgraphEdgeCount(const gvizSubgraph *sg);
 void gvizSubgraphShowVertex(const gvizSubgraph *sg, size_t u);
 void gvizSubgraphHideVertex(const gvizSubgraph *sg, size_t u);
+
+gvizBitArrayIterator gvizVertexSubsetIteratorCreate(const gvizVertexSubset vs,
+                                                    size_t nvertices);
+size_t gvizEdgeSubsetCount(const gvizEdgeSubset es);
+size_t gvizEdgeSubsetVertexEdgeCount(const gvizEdgeSubset es, size_t u);
+void gvizEdgeSubsetClearVertex(const gvizEdgeSubset es, size_t u);
+/* Show / hide the first (u, v) adjacency entry of a full subgraph. */
+void gvizSubgraphShowEdge(const gvizSubgraph *sg, size_t u, size_t v);
+void gvizSubgraphHideEdge(const gvizSubgraph *sg, size_t u, size_t v);
+/* Every vertex and edge of a full subgraph shown. */
+void gvizSubgraphMakeFull(gvizSubgraph *sg);
+/* Gives a vertex-induced subgraph an edge subset of its induced edges
+ * (needs the layout). */
+void gvizSubgraphMakeEdgeSubset(gvizSubgraph *sg);
+/* After the parent graph changed: grows the vertex capacity (doubling);
+ * a full subgraph's edge bits move to the rebuilt layout by (vertex,
+ * adjacency index), new entries hidden. 0 / -1. */
+int gvizSubgraphRebuild(gvizSubgraph *sg);
+
+/* Iterators (reference include/ds/gvizSubgraph.h:40-62, 202-211): the
+ * subgraph's vertices in ascending id; a vertex's subgraph neighbours in
+ * adjacency order (edge bits of a full subgraph, the vertex subset of an
+ * induced one). */
+typedef struct {
+  const gvizSubgraph *sg;
+  gvizBitArrayIterator it;
+} gvizSubgraphVertexIterator;
+
+typedef enum {
+  GVIZ_SUBGRAPH_NEIGHBOR_ITER_NONE = 0,
+  GVIZ_SUBGRAPH_NEIGHBOR_ITER_FULL,
+  GVIZ_SUBGRAPH_NEIGHBOR_ITER_INDUCED,
+} gvizSubgraphNeighborIterMode;
+
+typedef struct {
+  const gvizSubgraph *sg;
+  size_t u;
+  size_t base;
+  gvizBitArrayIterator it;
+  size_t adj_idx;
+  const gvizArray *nb;
+  gvizSubgraphNeighborIterMode mode;
+} gvizSubgraphNeighborIterator;
+
+gvizSubgraphVertexIterator gvizSubgraphVertexIteratorCreate(const gvizSubgraph *sg);
+bool gvizSubgraphVertexIterate(gvizSubgraphVertexIterator *it, size_t *out_u);
+gvizSubgraphNeighborIterator gvizSubgraphNeighborIteratorCreate(const gvizSubgraph *sg,
+                                                                size_t u);
+bool gvizSubgraphNeighborIterate(gvizSubgraphNeighborIterator *it, size_t *out_v);
 
 /* CSR of a subgraph over parent ids (edges of inactive endpoints dropped),
  * insertion order kept: offsets[n+1], adj[offsets[n]] (malloc'ed; free()).

@@ -5,10 +5,11 @@
  * a = stiffness (bary - P) - damping v, v += a dt, P += v dt, from a Jacobi
  * snapshot of the positions. Same fixed point as the Tutte sweep.
  *
- * Begin tests planarity on the host (left-right test) and fixes the
- * largest face on a convex polygon (0, or -2 for a graph that is not
- * planar); FixOuterFace (a face picked in a viewer) is outside this engine
- * and returns -1. The Jacobi snapshot lives on the
+ * Begin tests planarity on the host (left-right test), writes the rotation
+ * system into the graph's adjacency order and fixes the largest face on a
+ * convex polygon (0, or -2 for a graph that is not planar); FixOuterFace
+ * pins the face walked from the highlight's first edge instead. The Jacobi
+ * snapshot lives on the
  * device, so the state has no scratchPos; velocity is mirrored on the host
  * after every Step / Run. The device step is two-dimensional: Step and Run
  * fail (negative / -1) for another dimension.
@@ -67,6 +68,8 @@ double gvizSpringTutteEmbedderStep(gvizSpringTutteState *s, double dt);
 /* Steps until converged or iteration == maxIters; returns the iteration
  * count, or -1 without a boundary (or on device failure). */
 int gvizSpringTutteEmbedderRun(gvizSpringTutteState *s, size_t maxIters, double dt);
+/* The highlighted face as the boundary (after Begin; a highlight set with
+ * gvizEmbeddedGraphSetHighlight). 0, or -1 without either. */
 int gvizSpringTutteEmbedderFixOuterFace(gvizSpringTutteState *s);
 
 #ifdef __cplusplus

@@ -1,9 +1,11 @@
 /*
  * gvizTutteEmbedder.h -- the reference's Tutte barycentric embedder API
  * (include/embedders/gvizTutteEmbedder.h) with the Jacobi and Gauss-Seidel
- * sweeps on the GPU. Begin tests planarity on the host (left-right test)
- * and fixes the largest face on a convex polygon: 0, -2 for a graph that is
- * not planar. FixOuterFace (a face picked in a viewer) returns -1.
+ * sweeps on the GPU. Begin tests planarity on the host (left-right test),
+ * writes the rotation system into the graph's adjacency order (as the
+ * reference's planarity step does; planarEmbedded is set) and fixes the
+ * largest face on a convex polygon: 0, -2 for a graph that is not planar.
+ * FixOuterFace pins the face walked from the highlight's first edge.
  */
 #ifndef GVIZ_HOST_TUTTE_EMBEDDER_H
 #define GVIZ_HOST_TUTTE_EMBEDDER_H
@@ -47,6 +49,8 @@ void gvizTutteFixConvexPolygon(gvizTutteState *s, const size_t *boundary,
                                size_t count, double radius);
 double gvizTutteEmbedderStep(gvizTutteState *s, double dt);
 int gvizTutteEmbedderRun(gvizTutteState *s, size_t maxIters);
+/* The highlighted face as the boundary (after Begin; a highlight set with
+ * gvizEmbeddedGraphSetHighlight). 0, or -1 without either. */
 int gvizTutteEmbedderFixOuterFace(gvizTutteState *s);
 
 #ifdef __cplusplus

@@ -31,6 +31,9 @@ STRUCTS = [
     ("gvizGraph", ["vertices", "map", "directed", "layout"]),
     ("gvizEdgeSubset", ["bitset", "layout"]),
     ("gvizSubgraph", ["g", "vs", "es", "vertexCapacity"]),
+    ("gvizBitArrayIterator", ["arr", "size", "idx"]),
+    ("gvizSubgraphVertexIterator", ["sg", "it"]),
+    ("gvizSubgraphNeighborIterator", ["sg", "u", "base", "it", "adj_idx", "nb", "mode"]),
     ("gvizEmbedding", ["dim", "vertexPositions"]),
     ("gvizActionPayload", ["worldX", "worldY", "deltaTime", "iarg", "darg"]),
     ("gvizAction", ["name", "handler", "userData"]),
@@ -73,7 +76,8 @@ STRUCTS = [
 ]
 # enums and scalar typedefs whose size is part of the layout
 SCALARS = ["gvizForceModelKind", "gvizGRIPKPolicy", "gvizStatChartKind",
-           "gvizDrawEdgePolicy", "GVIZ_BIT_UNIT", "gvizVertexSubset"]
+           "gvizDrawEdgePolicy", "GVIZ_BIT_UNIT", "gvizVertexSubset",
+           "gvizSubgraphNeighborIterMode", "gvizQuadrant"]
 
 REF_INCLUDES = ["ds/gvizArray.h", "ds/gvizDeque.h", "ds/gvizBitArray.h", "ds/gvizGraph.h",
                 "ds/gvizSubgraph.h", "ds/gvizQuadtree.h", "embedders/gvizEmbeddedGraph.h",

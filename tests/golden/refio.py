@@ -448,3 +448,27 @@ def grip_stages(n, edges, dim, exe):
         filt = r.ia(int(borders[0]))
         out.append((layers, borders, filt))
     return out
+
+
+def tutte_outer_face(n, edges, *, spring, iters, hl_vertices, hl_edges, exe):
+    """Tutte Begin, the rotated adjacency lists, FixOuterFace on a highlight,
+    Run (host API driver only)."""
+    w = _W()
+    w.graph(n, edges)
+    hl_edges = np.asarray(hl_edges, dtype=np.int64).reshape(-1, 2)
+    w.i(1 if spring else 0, iters, len(hl_vertices), len(hl_edges))
+    w.ia(np.asarray(hl_vertices, dtype=np.int64))
+    w.ia(hl_edges.reshape(-1))
+    r = _R(_run("tutte_outer_face", w.bytes(), exe))
+    begin_rc, planar = r.i(), r.i()
+    lists = []
+    for _ in range(n):
+        c = r.i()
+        lists.append(r.ia(c))
+    fix_rc = r.i()
+    bc = r.i()
+    boundary = r.ia(bc)
+    iters_run = r.i()
+    pos = r.da(2 * n).reshape(n, 2)
+    return {"begin_rc": begin_rc, "planar": planar, "lists": lists, "fix_rc": fix_rc,
+            "boundary": boundary, "iterations": iters_run, "positions": pos}

@@ -880,6 +880,75 @@ static int mode_tutte_begin(void) {
   return 0;
 }
 
+/* Begin, then the adjacency lists Begin left (the rotation), then a
+ * highlight (vertices and edges read from the input) fixed as the outer
+ * face, then Run: rc of Begin and FixOuterFace, planarEmbedded, per vertex
+ * its list, the boundary, the iterations and positions. */
+static void write_lists(const gvizGraph *g, int64_t n) {
+  for (int64_t u = 0; u < n; u++) {
+    size_t c = 0;
+    const size_t *nb = gvizGraphNeighbors(g, (size_t)u, &c);
+    wr_i((int64_t)c);
+    for (size_t k = 0; k < c; k++)
+      wr_i((int64_t)nb[k]);
+  }
+}
+
+static int mode_tutte_outer_face(void) {
+  gvizGraph g;
+  int64_t n;
+  read_graph(&g, &n);
+  int64_t spring = rd_i(), iters = rd_i(), hv = rd_i(), he = rd_i();
+  int64_t *verts = malloc(sizeof(int64_t) * (hv ? hv : 1));
+  int64_t *edges = malloc(sizeof(int64_t) * 2 * (he ? he : 1));
+  for (int64_t i = 0; i < hv; i++)
+    verts[i] = rd_i();
+  for (int64_t i = 0; i < 2 * he; i++)
+    edges[i] = rd_i();
+  union {
+    gvizTutteState t;
+    gvizSpringTutteState sp;
+  } st;
+  gvizEmbeddedGraph *e = (gvizEmbeddedGraph *)&st;
+  int rc = spring ? gvizSpringTutteEmbedderInit(&st.sp, gvizSubgraphCreateFull(&g), 2, 1e-5)
+                  : gvizTutteEmbedderInit(&st.t, gvizSubgraphCreateFull(&g), 2, 1e-5);
+  if (rc != 0)
+    return 1;
+  rc = spring ? gvizSpringTutteEmbedderBegin(&st.sp) : gvizTutteEmbedderBegin(&st.t);
+  wr_i(rc);
+  wr_i(e->planarEmbedded);
+  write_lists(&g, n);
+  /* the highlight: a subgraph with an edge subset, as a viewer's face pick */
+  gvizSubgraph h = gvizSubgraphCreateEmpty(&g);
+  for (int64_t i = 0; i < hv; i++)
+    gvizSubgraphShowVertex(&h, (size_t)verts[i]);
+  for (int64_t i = 0; i < he; i++)
+    gvizSubgraphShowEdge(&h, (size_t)edges[2 * i], (size_t)edges[2 * i + 1]);
+  gvizEmbeddedGraphSetHighlight(e, h);
+  rc = spring ? gvizSpringTutteEmbedderFixOuterFace(&st.sp)
+              : gvizTutteEmbedderFixOuterFace(&st.t);
+  wr_i(rc);
+  size_t bc = spring ? st.sp.boundaryCount : st.t.boundaryCount;
+  const size_t *b = spring ? st.sp.boundary : st.t.boundary;
+  wr_i((int64_t)bc);
+  for (size_t i = 0; i < bc; i++)
+    wr_i((int64_t)b[i]);
+  int it = spring ? gvizSpringTutteEmbedderRun(&st.sp, (size_t)iters, 1.0 / 60.0)
+                  : gvizTutteEmbedderRun(&st.t, (size_t)iters);
+  wr_i(it);
+  const double *p = gvizEmbeddedGraphPositions(e);
+  for (int64_t i = 0; i < 2 * n; i++)
+    wr_d(p[i]);
+  if (spring)
+    gvizSpringTutteEmbedderRelease(&st.sp);
+  else
+    gvizTutteEmbedderRelease(&st.t);
+  gvizGraphRelease(&g);
+  free(verts);
+  free(edges);
+  return 0;
+}
+
 static int mode_spring_tutte(void) {
   gvizGraph g;
   int64_t n;
@@ -1434,6 +1503,8 @@ int main(int argc, char **argv) {
     rc = mode_knn_cases();
   else if (!strcmp(argv[1], "grip_stages"))
     rc = mode_grip_stages();
+  else if (!strcmp(argv[1], "tutte_outer_face"))
+    rc = mode_tutte_outer_face();
   fclose(fin);
   fclose(fout);
   return rc;

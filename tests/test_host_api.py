@@ -24,6 +24,7 @@ import pytest
 
 from conftest import ROOT, golden
 import refio
+from grapher_amd import graphs as G
 from grapher_amd import native
 
 DRIVER = os.path.join(ROOT, "tests", "host", "api_driver")
@@ -405,3 +406,82 @@ def test_grip_white_box_stages_build_reference_filtration(gpu, name):
         assert layers == len(f["borders"])
         assert np.array_equal(borders, f["borders"])
         assert np.array_equal(filt, f["filtration"])
+
+
+def _face_walk(lists, u, v):
+    """The reference's face walk (src/embedders/gvizPlanarEmbedder.c:220-254):
+    (u, v) -> (v, the entry of v's list before u), until (u, v) returns."""
+    out, su, sv = [], u, v
+    while True:
+        out.append(int(v))
+        nb = [int(x) for x in lists[v]]
+        w = nb[nb.index(u) - 1]
+        u, v = v, w
+        if (u, v) == (su, sv):
+            return out
+
+
+def _outer_face_case(kind):
+    if kind == "grid":
+        n, e = G.grid(6, 6)
+        c = [2 * 6 + 2, 2 * 6 + 3, 3 * 6 + 3, 3 * 6 + 2]  # an interior cell
+        return n, e, c, [(c[0], c[1]), (c[1], c[2]), (c[2], c[3]), (c[3], c[0])]
+    n, e, pts, hull = G.delaunay(300, 9)
+    from scipy.spatial import Delaunay
+    tri = Delaunay(pts)
+    onhull = set(int(h) for h in hull)
+    t = next(s for s in tri.simplices if not (set(int(x) for x in s) & onhull))
+    t = [int(x) for x in t]
+    return n, e, t, [(t[0], t[1]), (t[1], t[2]), (t[2], t[0])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spring", [False, True])
+@pytest.mark.parametrize("kind", ["grid", "delaunay"])
+def test_tutte_fix_outer_face(gpu, kind, spring):
+    """Begin writes a planar rotation system into the adjacency lists
+    (Euler's formula over its faces, the same edges as before) and sets
+    planarEmbedded; FixOuterFace pins the face walked from the highlight's
+    first edge (src/embedders/gvizTutteEmbedder.c:105-143, 347-373) on the
+    radius-200 polygon, and Run relaxes the rest inside it."""
+    n, e, hv, he = _outer_face_case(kind)
+    print(f"tutte outer face {kind} spring={spring}", flush=True)
+    r = refio.tutte_outer_face(n, e, spring=spring, iters=20000, hl_vertices=hv, hl_edges=he,
+                               exe=DRIVER)
+    assert r["begin_rc"] == 0 and r["planar"] == 1
+    lists = r["lists"]
+    got = sorted((u, int(v)) for u in range(n) for v in lists[u])
+    want = sorted([(int(a), int(b)) for a, b in e] + [(int(b), int(a)) for a, b in e])
+    assert got == want
+    seen, faces = set(), 0
+    for u in range(n):
+        for v in lists[u]:
+            if (u, int(v)) in seen:
+                continue
+            faces += 1
+            a, b = u, int(v)
+            while (a, b) not in seen:
+                seen.add((a, b))
+                nb = [int(x) for x in lists[b]]
+                a, b = b, nb[nb.index(a) - 1]
+    assert n - len(e) + faces == 2
+    # the highlight shows the entry (a -> b) of each listed edge (a, b); its
+    # first edge is the smallest vertex's first shown entry
+    u0 = min(hv)
+    v0 = next(int(v) for v in lists[u0] if (u0, int(v)) in set(map(tuple, he)))
+    walk = _face_walk(lists, u0, v0)
+    assert r["fix_rc"] == 0
+    assert list(r["boundary"]) == walk
+    assert u0 in walk and v0 in walk and 3 <= len(walk) <= max(len(hv), 8)
+    pos = r["positions"]
+    k = np.arange(len(walk))
+    ring = 200.0 * np.stack([np.cos(2 * np.pi * k / len(walk)),
+                             np.sin(2 * np.pi * k / len(walk))], axis=1)
+    assert np.array_equal(pos[walk], ring)
+    inner = np.setdiff1d(np.arange(n), walk)
+    assert np.all(np.hypot(pos[inner, 0], pos[inner, 1]) < 200.0)
+    if not spring:
+        assert r["iterations"] < 20000
+        off, adj = G.csr_from_edges(n, e)
+        bary = np.array([pos[adj[off[u]:off[u + 1]]].mean(axis=0) for u in inner])
+        assert np.abs(bary - pos[inner]).max() < 1e-3
