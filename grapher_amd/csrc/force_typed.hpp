@@ -753,21 +753,20 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
       if (op)
         kids = (r.child[0] != kNone) + (r.child[1] != kNone) +
                (r.child[2] != kNone) + (r.child[3] != kNone);
-      int incl = kids;
-#pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        int v = __shfl_up(incl, off, kWave);
-        if (lane >= off)
-          incl += v;
-      }
-      const int total = rl(incl, kWave - 1);
+      // each lane's first slot: the children of the lanes below it, counted
+      // from ballots of the three bits of the child count (no cross-lane
+      // shuffles)
+      const uint64_t k1 = __ballot(kids & 1), k2 = __ballot(kids & 2), k4 = __ballot(kids & 4);
+      const int below = (int)__popcll(k1 & ltMask) + 2 * (int)__popcll(k2 & ltMask) +
+                        4 * (int)__popcll(k4 & ltMask);
+      const int total = (int)__popcll(k1) + 2 * (int)__popcll(k2) + 4 * (int)__popcll(k4);
       if (sp + total > kTravStack) {
         sc->traversalOverflow = 1;
         break;
       }
       if (op) {
         // children go on in reverse so NW is popped first
-        int at = sp + incl - 1;
+        int at = sp + below + kids - 1;
 #pragma unroll
         for (int c = 0; c < 4; c++)
           if (r.child[c] != kNone) {
@@ -797,10 +796,9 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
         b.id = node;
         dBody[wave][nd + (int)__popcll(am & ltMask)] = b;
       }
-      for (int k = 0; am; k++) {
-        const int l = __builtin_ctzll(am);
-        am &= am - 1;
-        const TravBody<Real, PosT> b = dBody[wave][nd + k];
+      // two staged cells per pass: their LDS reads and per-lane tests
+      // overlap; put-backs keep the order of the cells
+      auto ambTest = [&](const TravBody<Real, PosT> &b) -> bool {
         if (tstats) {
           st[3]++;
           st[4] += __popcll(rl(b.mask, 0));
@@ -849,21 +847,34 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
             }
           }
         }
-        const uint64_t om = __ballot(open);
+        return open;
+      };
+      auto putBack = [&](uint64_t om, int32_t id) -> bool {
         if (om) {
           // the cell goes back on the stack once, tagged, and is opened for
           // the lanes of om (in parallel with other entries) when taken
           if (sp + 1 > kTravStack) {
             sc->traversalOverflow = 1;
             sp = 0;
-            break;
+            return false;
           }
           if (lane == 0) {
-            sNode[wave][sp] = (int32_t)(-1 - n - (int64_t)b.id);
+            sNode[wave][sp] = (int32_t)(-1 - n - (int64_t)id);
             sMask[wave][sp] = om;
           }
           sp++;
         }
+        return true;
+      };
+      for (int k = 0; k < na; k += 2) {
+        const bool two = k + 1 < na;
+        const TravBody<Real, PosT> b0 = dBody[wave][nd + k];
+        const TravBody<Real, PosT> b1 = dBody[wave][nd + k + (two ? 1 : 0)];
+        const bool o0 = ambTest(b0);
+        const bool o1 = two ? ambTest(b1) : false;
+        const uint64_t om0 = __ballot(o0), om1 = __ballot(o1);
+        if (!putBack(om0, b0.id) || !putBack(om1, b1.id))
+          break;
       }
     }
   }
