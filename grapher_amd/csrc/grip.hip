@@ -180,8 +180,9 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
 }
 
 // ---- block-cooperative search for calls with few queries: one block of
-// kBB threads takes kBB (queued vertex, adjacency entry) pairs per step of
-// the same FIFO stream wave_bfs walks 64 at a time. A vertex is fresh at its
+// kBB threads takes kBB * kPairs (queued vertex, adjacency entry) pairs per
+// step (kPairs consecutive pairs per thread, so each thread has that many
+// loads in flight) of the same FIFO stream wave_bfs walks 64 at a time. A vertex is fresh at its
 // first pair of the step (the smallest pair index, found through an LDS hash
 // table) when no earlier step visited it, so the fresh vertices, their queue
 // order and the hits are exactly wave_bfs's; only the step size differs.
@@ -190,7 +191,8 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
 #endif
 constexpr int kBB = GVH_GRIP_BB;
 constexpr int kBW = kBB / kWave;
-constexpr int kHash = 2 * kBB;
+constexpr int kPairs = 4;                // adjacency pairs per thread per pass
+constexpr int kHash = 2 * kBB * kPairs;  // LDS hash over one pass's pairs
 
 // Inclusive block scan of one int per thread; `total` gets the block sum.
 __device__ __forceinline__ int block_scan(int v, int *sWave, int &total) {
@@ -258,63 +260,95 @@ __device__ int block_bfs(const int32_t *__restrict__ off, const int32_t *__restr
       hVal[i] = INT32_MAX;
     }
     __syncthreads();
-    const int pairs = total < kBB ? total : kBB;
-    const bool valid = t < pairs;
-    int32_t w = -1, ownDepth = 0;
-    int owner = 0;
-    if (valid) {
-      // owner = first queued vertex whose inclusive sum exceeds t
+    // this thread's pairs are p0 .. p0 + kPairs - 1, in queue order
+    const int pairs = total < kBB * kPairs ? total : kBB * kPairs;
+    const int p0 = t * kPairs;
+    int32_t w[kPairs], ownDepth[kPairs];
+    if (p0 < pairs) {
+      // owner = first queued vertex whose inclusive sum exceeds p0
       int lo = 0, hi = avail - 1;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (sIncl[mid] > t)
+        if (sIncl[mid] > p0)
           hi = mid;
         else
           lo = mid + 1;
       }
-      owner = lo;
-      const int excl = owner ? sIncl[owner - 1] : 0;
-      w = adj[sStart[owner] + (t - excl)];
-      ownDepth = sDepth[owner];
-      // smallest pair index per vertex (open addressing)
-      uint32_t h = ((uint32_t)w * 2654435761u) & (kHash - 1);
-      while (true) {
-        const int32_t prev = atomicCAS(&hKey[h], -1, w);
-        if (prev == -1 || prev == w) {
-          atomicMin(&hVal[h], t);
-          break;
+      int owner = lo;
+#pragma unroll
+      for (int i = 0; i < kPairs; i++) {
+        const int p = p0 + i;
+        w[i] = -1;
+        ownDepth[i] = 0;
+        if (p < pairs) {
+          while (sIncl[owner] <= p)
+            owner++;
+          const int excl = owner ? sIncl[owner - 1] : 0;
+          w[i] = adj[sStart[owner] + (p - excl)];
+          ownDepth[i] = sDepth[owner];
+          if (p == pairs - 1)
+            sLast = owner;
         }
-        h = (h + 1) & (kHash - 1);
+      }
+#pragma unroll
+      for (int i = 0; i < kPairs; i++) {
+        if (w[i] < 0)
+          continue;
+        // smallest pair index per vertex (open addressing)
+        uint32_t h = ((uint32_t)w[i] * 2654435761u) & (kHash - 1);
+        while (true) {
+          const int32_t prev = atomicCAS(&hKey[h], -1, w[i]);
+          if (prev == -1 || prev == w[i]) {
+            atomicMin(&hVal[h], p0 + i);
+            break;
+          }
+          h = (h + 1) & (kHash - 1);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kPairs; i++) {
+        w[i] = -1;
+        ownDepth[i] = 0;
       }
     }
-    if (t == pairs - 1)
-      sLast = owner;
     __syncthreads();
-    bool fresh = false;
-    if (valid) {
-      uint32_t h = ((uint32_t)w * 2654435761u) & (kHash - 1);
-      while (hKey[h] != w)
-        h = (h + 1) & (kHash - 1);
-      if (hVal[h] == t) {
-        const uint32_t bit = 1u << (w & 31);
-        fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
+    bool fresh[kPairs], pass[kPairs];
+    int nFresh = 0, nPass = 0;
+#pragma unroll
+    for (int i = 0; i < kPairs; i++) {
+      fresh[i] = false;
+      if (w[i] >= 0) {
+        uint32_t h = ((uint32_t)w[i] * 2654435761u) & (kHash - 1);
+        while (hKey[h] != w[i])
+          h = (h + 1) & (kHash - 1);
+        if (hVal[h] == p0 + i) {
+          const uint32_t bit = 1u << (w[i] & 31);
+          fresh[i] = !(atomicOr(&bits[w[i] >> 5], bit) & bit);
+        }
       }
+      pass[i] = fresh[i] && (!filter || filter[w[i]]) && (!lowerOnly || w[i] < source);
+      nFresh += fresh[i] ? 1 : 0;
+      nPass += pass[i] ? 1 : 0;
     }
-    const bool pass = fresh && (!filter || filter[w]) && (!lowerOnly || w < source);
     int counts;
-    const int packed = block_scan((fresh ? 1 : 0) | (pass ? 1 << 16 : 0), sWave, counts);
-    const int exFresh = (packed & 0xFFFF) - (fresh ? 1 : 0);
-    const int exPass = (packed >> 16) - (pass ? 1 : 0);
-    if (pass) {
-      const int slot = found + exPass;
-      if (slot < k) {
-        outV[slot] = w;
-        outD[slot] = ownDepth + 1;
+    const int packed = block_scan(nFresh | (nPass << 16), sWave, counts);
+    int atFresh = (packed & 0xFFFF) - nFresh;
+    int atPass = (packed >> 16) - nPass;
+#pragma unroll
+    for (int i = 0; i < kPairs; i++) {
+      if (pass[i]) {
+        const int slot = found + atPass++;
+        if (slot < k) {
+          outV[slot] = w[i];
+          outD[slot] = ownDepth[i] + 1;
+        }
       }
-    }
-    if (fresh) {
-      st_l2(queue + tail + exFresh, w);
-      st_l2(qdist + tail + exFresh, ownDepth + 1);
+      if (fresh[i]) {
+        st_l2(queue + tail + atFresh, w[i]);
+        st_l2(qdist + tail + atFresh, ownDepth[i] + 1);
+        atFresh++;
+      }
     }
     found += counts >> 16;
     tail += counts & 0xFFFF;
