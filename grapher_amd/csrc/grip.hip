@@ -69,12 +69,17 @@ struct BfsSlots {
 // duplicates inside a step resolved in favour of the earlier lane the set of
 // fresh vertices and their order are exactly those of the one-vertex-at-a-time
 // search.
+// A step's first pair per vertex: lane indices through a per-wave LDS hash
+// (2 * 64 slots, open addressing, smallest lane kept).
+constexpr int kWaveHash = 2 * kWave;
+
 __device__ int wave_bfs(const int32_t *__restrict__ off,
                         const int32_t *__restrict__ adj,
                         const uint8_t *__restrict__ filter, int32_t source, int k,
                         int maxDepth, bool lowerOnly, bool stopAtK,
                         int64_t filterTotal, uint32_t *bits, int32_t *queue,
-                        int32_t *qdist, int32_t *outV, int32_t *outD) {
+                        int32_t *qdist, int32_t *outV, int32_t *outD,
+                        int32_t *hKey, int32_t *hVal) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t lt = lane ? (~0ull >> (kWave - lane)) : 0ull;
   if (lane == 0) {
@@ -121,12 +126,25 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
     const bool valid = lane < pairs;
     const int32_t w = valid ? adj[ownStart + (lane - ownExcl)] : -1;
     // a vertex listed twice among this step's pairs: only its first counts
-    bool dup = false;
-    for (int o = 1; o < kWave; o++) {
-      const int32_t other = __shfl_up(w, o, kWave);
-      if (lane >= o && other == w)
-        dup = true;
+    hKey[lane] = -1;
+    hKey[lane + kWave] = -1;
+    hVal[lane] = kWave;
+    hVal[lane + kWave] = kWave;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t h = ((uint32_t)w * 2654435761u) & (kWaveHash - 1);
+    if (valid) {
+      while (true) {
+        const int32_t prev = atomicCAS(&hKey[h], -1, w);
+        if (prev == -1 || prev == w) {
+          atomicMin(&hVal[h], lane);
+          break;
+        }
+        h = (h + 1) & (kWaveHash - 1);
+      }
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const bool dup = valid && hVal[h] != lane;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     bool fresh = false;
     if (valid && !dup) {
       const uint32_t bit = 1u << (w & 31);
@@ -420,17 +438,20 @@ __global__ __launch_bounds__(kB) void k_bfs_queries(
     int maxDepth, int lowerOnly, int stopAtK, int64_t filterTotal, BfsSlots sl,
     int64_t slots, int32_t *__restrict__ outV, int32_t *__restrict__ outD,
     int32_t *__restrict__ outCount, int32_t *__restrict__ overflow) {
+  __shared__ int32_t sHashK[kWpb][kWaveHash], sHashV[kWpb][kWaveHash];
   const int64_t slot = (int64_t)blockIdx.x * kWpb + threadIdx.x / kWave;
   if (slot >= slots)
     return;
   uint32_t *bits = sl.bits + slot * sl.words;
   int32_t *q = sl.queue + slot * sl.cap;
   int32_t *qd = sl.qdist + slot * sl.cap;
+  const int wv = threadIdx.x / kWave;
   for (int64_t qi = slot; qi < count; qi += slots) {
     const int32_t src = sources[qi];
     const int64_t o = (outIndex ? (int64_t)outIndex[qi] : qi) * stride;
     int found = wave_bfs(off, adj, filter, src, k, maxDepth, lowerOnly != 0,
-                         stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o);
+                         stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o,
+                         sHashK[wv], sHashV[wv]);
     if ((threadIdx.x & (kWave - 1)) == 0) {
       outCount[outIndex ? outIndex[qi] : qi] = found < k ? found : k;
       if (!stopAtK && found > k)
