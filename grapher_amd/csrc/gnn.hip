@@ -3,13 +3,15 @@
 // inference on gfx950.
 //
 // Model (no reference counterpart; it sits in front of
-// gvizForceEmbedderBegin): input features x_v (16 wide: two random
-// coordinates, a degree term, a constant, zero padding), three layers
+// gvizForceEmbedderBegin): input features x_v (16 wide: random start
+// coordinates and their Chebyshev low-pass filters over the graph, a degree
+// term, a constant, zero padding), three layers
 //     h' = relu([h | mean_{u in N(v)} h_u] W + b)      (hidden width 64)
 // and a linear head y = h W4 + b4 (2 outputs, padded to 16). Weights and
 // activations are BF16 and every product accumulates in FP32 on the matrix
-// cores (v_mfma_f32_16x16x16_bf16); neighbour means, losses and the Adam
-// update are FP32. Training fits a force-embedder layout T up to rotation,
+// cores (v_mfma_f32_16x16x32_bf16 where the depth is a multiple of 32,
+// v_mfma_f32_16x16x16_bf16 otherwise); neighbour means, losses and the
+// Adam update are FP32. Training fits a force-embedder layout T up to rotation,
 // reflection and scale: with P and T centred and R, s the orthogonal
 // Procrustes fit of P to T,
 //     L = |s P R - T|^2 / |T|^2 = 1 - (sigma1 + sigma2)^2 / (|P|^2 |T|^2),
@@ -65,24 +67,60 @@ __global__ void k_weights_bf16(const float *__restrict__ p, uint16_t *__restrict
   wt[(int64_t)c * K + r] = b;
 }
 
-// M[v] = mean over neighbours of X (width F in {16, 64}); 64 / F vertices per
-// wavefront, one feature per lane.
+// Eight BF16 values of a 16-byte load, as floats (element 2m in the low half
+// of word m).
+__device__ __forceinline__ void bf8_to_f(const uint4 r, float *f) {
+  const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    f[2 * m] = __uint_as_float(w[m] << 16);
+    f[2 * m + 1] = __uint_as_float(w[m] & 0xFFFF0000u);
+  }
+}
+
+// M[v] = mean over neighbours of X (width F in {16, 64}): F / 8 lanes per
+// vertex, 16 bytes (8 features) per lane, neighbour rows fetched four at a
+// time and added in adjacency order.
 template <int F>
 __global__ __launch_bounds__(kB) void k_aggregate(const int32_t *__restrict__ off,
                                                   const int32_t *__restrict__ adj,
                                                   const float *__restrict__ invDeg,
                                                   const uint16_t *__restrict__ X,
                                                   uint16_t *__restrict__ M, int64_t n) {
-  constexpr int per = kWave / F;
+  constexpr int L = F / 8;
   const int64_t gt = (int64_t)blockIdx.x * kB + threadIdx.x;
-  const int64_t v = (gt / kWave) * per + (threadIdx.x & (kWave - 1)) / F;
-  const int f = threadIdx.x & (F - 1);
+  const int64_t v = gt / L;
+  const int c = (int)(gt % L) * 8;
   if (v >= n)
     return;
-  float acc = 0.0f;
-  for (int32_t e = off[v]; e < off[v + 1]; e++)
-    acc += bf2f(X[(int64_t)adj[e] * F + f]);
-  M[v * F + f] = f2bf(acc * invDeg[v]);
+  float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, f[8];
+  const int32_t e1 = off[v + 1];
+  int32_t e = off[v];
+  for (; e + 4 <= e1; e += 4) {
+    uint4 r[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      r[k] = *(const uint4 *)(X + (int64_t)adj[e + k] * F + c);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      bf8_to_f(r[k], f);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        acc[j] += f[j];
+    }
+  }
+  for (; e < e1; e++) {
+    bf8_to_f(*(const uint4 *)(X + (int64_t)adj[e] * F + c), f);
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      acc[j] += f[j];
+  }
+  const float s = invDeg[v];
+  uint32_t w[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+    w[m] = (uint32_t)f2bf(acc[2 * m] * s) | ((uint32_t)f2bf(acc[2 * m + 1] * s) << 16);
+  *(uint4 *)(M + v * F + c) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // C = [A1 | A2] B with B given as Bt (Nc x K, BF16), one wavefront per 16
@@ -119,6 +157,61 @@ __global__ __launch_bounds__(kB) void k_gemm(const uint16_t *__restrict__ A1,
       const uint2 rb = *(const uint2 *)(Bt + (int64_t)(t * 16 + li) * K + kc);
       const v4s b = __builtin_bit_cast(v4s, rb);
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int col = t * 16 + li;
+    const float bcol = bias ? bias[col] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int64_t r = r0 + 4 * lg + i;
+      if (r >= n)
+        continue;
+      float z = acc[t][i] + bcol;
+      if (relu)
+        z = z > 0.0f ? z : 0.0f;
+      if (outB)
+        outB[r * Nc + col] = f2bf(z);
+      if (outF)
+        outF[r * Nc + col] = z;
+    }
+  }
+}
+
+// k_gemm on the 32-deep BF16 product (v_mfma_f32_16x16x32_bf16) for K a
+// multiple of 32: each lane loads 8 consecutive k of its row of A and its
+// column of B (16 bytes each), so a 128-deep row takes four loads.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+template <int NT>
+__global__ __launch_bounds__(kB) void k_gemm32(const uint16_t *__restrict__ A1,
+                                               const uint16_t *__restrict__ A2, int Ka,
+                                               int64_t n, const uint16_t *__restrict__ Bt,
+                                               int K, const float *__restrict__ bias,
+                                               int relu, uint16_t *__restrict__ outB,
+                                               float *__restrict__ outF) {
+  constexpr int Nc = 16 * NT;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t r0 = ((int64_t)blockIdx.x * (kB / kWave) + threadIdx.x / kWave) * 16;
+  if (r0 >= n)
+    return;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t row = r0 + li;
+  v4f acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+    acc[t] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+  for (int kb = 0; kb < K; kb += 32) {
+    const int kc = kb + 8 * lg;
+    uint4 raw = make_uint4(0, 0, 0, 0);
+    if (row < n)
+      raw = *(const uint4 *)(kc < Ka ? A1 + row * Ka + kc : A2 + row * Ka + (kc - Ka));
+    const v8bf a = __builtin_bit_cast(v8bf, raw);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const uint4 rb = *(const uint4 *)(Bt + (int64_t)(t * 16 + li) * K + kc);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(v8bf, rb),
+                                                       acc[t], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -326,29 +419,79 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g,
 }
 
 
-// One lazy random-walk step of a 2-column signal: y = (x + mean of the
-// neighbours' x) / 2 (isolated vertices keep x).
-__global__ __launch_bounds__(kB) void k_diffuse(const int32_t *__restrict__ off,
-                                                const int32_t *__restrict__ adj,
-                                                const float *__restrict__ invDeg,
-                                                const float2 *__restrict__ x, int64_t n,
-                                                float2 *__restrict__ y) {
-  const int64_t v = (int64_t)blockIdx.x * kB + threadIdx.x;
+// Chebyshev low-pass filter of the lazy random walk P = (I + D^-1 A) / 2
+// (isolated vertices: P x = x). mu(P) = (2 P - c I) / c maps the eigenvalues
+// in [0, c] onto [-1, 1]; y_0 = x, y_1 = mu(P) y_0,
+// y_{k+1} = 2 mu(P) y_k - y_{k-1}, so y_d = T_d(mu(P)) x: every
+// eigen-component of P below c is damped by at least 1 / T_d(mu(1))
+// relative to the component at eigenvalue 1 (d = 16, c = 0.9: 5.5e-5), in
+// d neighbour sweeps where the plain walk P^t needs t ~ d^2.
+constexpr float kChebC = 0.9f;
+constexpr int kChebDeg = 16;
+constexpr int kChebMarks[4] = {2, 4, 8, 16};
+
+// out = mu(P) y (first) or 2 mu(P) y - yPrev. kChebLanes lanes per vertex:
+// lane j adds neighbours j, j + 8, ... in adjacency order, then the lanes'
+// sums are combined pairwise (j + 4, j + 2, j + 1).
+constexpr int kChebLanes = 8;
+__global__ __launch_bounds__(kB) void k_cheb(const int32_t *__restrict__ off,
+                                             const int32_t *__restrict__ adj,
+                                             const float *__restrict__ invDeg,
+                                             const float2 *__restrict__ y,
+                                             const float2 *__restrict__ yPrev, int first,
+                                             int64_t n, float2 *__restrict__ out) {
+  const int64_t gt = (int64_t)blockIdx.x * kB + threadIdx.x;
+  const int64_t v = gt / kChebLanes;
+  const int j = (int)(gt % kChebLanes);
+  // whole vertex groups share a wavefront, so a group exits together
   if (v >= n)
     return;
   const int32_t e0 = off[v], e1 = off[v + 1];
-  const float2 xv = x[v];
-  if (e0 == e1) {
-    y[v] = xv;
-    return;
-  }
   float sx = 0.0f, sy = 0.0f;
-  for (int32_t e = e0; e < e1; e++) {
-    const float2 u = x[adj[e]];
+  for (int32_t e = e0 + j; e < e1; e += kChebLanes) {
+    const float2 u = y[adj[e]];
     sx += u.x;
     sy += u.y;
   }
-  y[v] = make_float2(0.5f * (xv.x + sx * invDeg[v]), 0.5f * (xv.y + sy * invDeg[v]));
+#pragma unroll
+  for (int o = kChebLanes / 2; o > 0; o >>= 1) {
+    sx += __shfl_xor(sx, o, kChebLanes);
+    sy += __shfl_xor(sy, o, kChebLanes);
+  }
+  if (j != 0)
+    return;
+  const float2 yv = y[v];
+  float px = yv.x, py = yv.y;
+  if (e1 > e0) {
+    px = 0.5f * (yv.x + sx * invDeg[v]);
+    py = 0.5f * (yv.y + sy * invDeg[v]);
+  }
+  // mu(P) y = (2 / c) P y - y
+  float mx = (2.0f / kChebC) * px - yv.x, my = (2.0f / kChebC) * py - yv.y;
+  if (!first) {
+    const float2 q = yPrev[v];
+    mx = 2.0f * mx - q.x;
+    my = 2.0f * my - q.y;
+  }
+  out[v] = make_float2(mx, my);
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014) of seed + (v + 1) * golden gamma:
+// bits 40-63 and 16-39 as two uniform coordinates in [-1, 1).
+__device__ __host__ inline uint64_t splitmix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_uniform_start(uint64_t seed, int64_t n, float2 *__restrict__ out) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  const uint64_t z = splitmix64(seed + (uint64_t)(v + 1) * 0x9E3779B97F4A7C15ull);
+  const float sc = 1.0f / 16777216.0f;
+  out[v] = make_float2((float)(z >> 40) * sc * 2.0f - 1.0f,
+                       (float)((z >> 16) & 0xFFFFFFull) * sc * 2.0f - 1.0f);
 }
 
 // Per-block sums of x, y, x^2, y^2 (FP64).
@@ -380,14 +523,61 @@ __global__ __launch_bounds__(kB) void k_moments2(const float2 *__restrict__ x, i
   }
 }
 
-// Feature columns c, c+1 = (x - mean) * invStd, per coordinate, as BF16.
-__global__ void k_put_feature(const float2 *__restrict__ x, int64_t n, int c, float mx,
-                              float my, float sx, float sy, uint16_t *__restrict__ X0) {
+// Totals of k_moments2's per-block partials: stats = {sum x, sum y,
+// sum x^2, sum y^2} (one block; partials summed in block order).
+__global__ void k_stats_finish(const double *__restrict__ part, int blocks,
+                               double *__restrict__ stats) {
+  __shared__ double sm[kB / kWave][4];
+  double m[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < blocks; b += kB)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      m[j] += part[4 * b + j];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double w = wave_sum(m[j]);
+    if (lane == 0)
+      sm[wave][j] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < kB / kWave; w++)
+      t += sm[w][threadIdx.x];
+    stats[threadIdx.x] = t;
+  }
+}
+
+// Feature columns c, c+1 = (x - mean) * invStd per coordinate, as BF16, from
+// the totals of k_stats_finish.
+__global__ void k_put_feature(const float2 *__restrict__ x, int64_t n, int c,
+                              const double *__restrict__ stats, uint16_t *__restrict__ X0) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n)
     return;
-  X0[v * kF + c] = f2bf((x[v].x - mx) * sx);
-  X0[v * kF + c + 1] = f2bf((x[v].y - my) * sy);
+  const double nn = (double)n;
+  const double mx = stats[0] / nn, my = stats[1] / nn;
+  const double vx = stats[2] / nn - mx * mx, vy = stats[3] / nn - my * my;
+  const float sx = vx > 1e-30 ? (float)(1.0 / sqrt(vx)) : 0.0f;
+  const float sy = vy > 1e-30 ? (float)(1.0 / sqrt(vy)) : 0.0f;
+  X0[v * kF + c] = f2bf((x[v].x - (float)mx) * sx);
+  X0[v * kF + c + 1] = f2bf((x[v].y - (float)my) * sy);
+}
+
+// Proposal placement: out = (y - mean) * radius / RMS radius of y, in FP64,
+// from the totals of k_stats_finish (a zero spread leaves y centred).
+__global__ void k_place(const float2 *__restrict__ y, int64_t n,
+                        const double *__restrict__ stats, double radius,
+                        double2 *__restrict__ out) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n)
+    return;
+  const double nn = (double)n;
+  const double mx = stats[0] / nn, my = stats[1] / nn;
+  const double var = stats[2] / nn - mx * mx + stats[3] / nn - my * my;
+  const double s = var > 1e-300 ? radius / sqrt(var) : 1.0;
+  out[v] = make_double2(((double)y[v].x - mx) * s, ((double)y[v].y - my) * s);
 }
 
 // Degree term, constant and zero padding (columns 10..15).
@@ -679,7 +869,10 @@ struct gvizhip_gnn {
   DevBuf<uint16_t> wt[4], wr[4];
   DevBuf<float> partial;
   DevBuf<double> moments;
-  DevBuf<float> diff[2];
+  DevBuf<float> diff[3];
+  DevBuf<double> stats;
+  DevBuf<double> placed;
+  double *pinned = nullptr; // host staging of proposals (2 cap doubles)
   // pivot MDS
   DevBuf<int32_t> piv, changed;
   DevBuf<uint16_t> pdist;
@@ -714,8 +907,12 @@ void launch_gemm(gvizhip_gnn *g, const uint16_t *A1, const uint16_t *A2, int Ka,
                  float *outF) {
   const int64_t waves = (g->n + 15) / 16;
   const unsigned blocks = (unsigned)((waves + kB / kWave - 1) / (kB / kWave));
-  hipLaunchKernelGGL(k_gemm<NT>, dim3(blocks), dim3(kB), 0, g->stream, A1, A2, Ka, g->n,
-                     Bt, K, bias, relu, outB, outF);
+  if (K % 32 == 0)
+    hipLaunchKernelGGL(k_gemm32<NT>, dim3(blocks), dim3(kB), 0, g->stream, A1, A2, Ka,
+                       g->n, Bt, K, bias, relu, outB, outF);
+  else
+    hipLaunchKernelGGL(k_gemm<NT>, dim3(blocks), dim3(kB), 0, g->stream, A1, A2, Ka, g->n,
+                       Bt, K, bias, relu, outB, outF);
 }
 
 int forward(gvizhip_gnn *g) {
@@ -727,12 +924,12 @@ int forward(gvizhip_gnn *g) {
   for (int l = 0; l < 4; l++)
     b[l] = g->params.p + kOffW[l] + kK[l] * kN[l];
   // layer 1
-  hipLaunchKernelGGL(k_aggregate<kF>, dim3(grid_for(n * kF, kB)), dim3(kB), 0, st,
+  hipLaunchKernelGGL(k_aggregate<kF>, dim3(grid_for(n * (kF / 8), kB)), dim3(kB), 0, st,
                      g->off.p, g->adj.p, g->invDeg.p, g->X0.p, g->M0.p, n);
   launch_gemm<4>(g, g->X0.p, g->M0.p, kF, g->wt[0].p, 2 * kF, b[0], 1, g->H[0].p, nullptr);
   // layers 2 and 3
   for (int l = 1; l < 3; l++) {
-    hipLaunchKernelGGL(k_aggregate<kH>, dim3(grid_for(n * kH, kB)), dim3(kB), 0, st,
+    hipLaunchKernelGGL(k_aggregate<kH>, dim3(grid_for(n * (kH / 8), kB)), dim3(kB), 0, st,
                        g->off.p, g->adj.p, g->invDeg.p, g->H[l - 1].p, g->M[l - 1].p, n);
     launch_gemm<4>(g, g->H[l - 1].p, g->M[l - 1].p, kH, g->wt[l].p, 2 * kH, b[l], 1,
                    g->H[l].p, nullptr);
@@ -842,6 +1039,51 @@ int backward(gvizhip_gnn *g, const LossFit &fit) {
   return 0;
 }
 
+// Totals {sum x, sum y, sum x^2, sum y^2} of a 2-column signal into
+// stats[0..3] on the device (no host synchronisation).
+int moments_of(gvizhip_gnn *g, const float2 *x, double *stats) {
+  const int blocks = (int)std::min<int64_t>(grid_for(g->n, kB), 256);
+  hipLaunchKernelGGL(k_moments2, dim3(blocks), dim3(kB), 0, g->stream, x, g->n,
+                     g->moments.p);
+  hipLaunchKernelGGL(k_stats_finish, dim3(1), dim3(kB), 0, g->stream,
+                     (const double *)g->moments.p, blocks, stats);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
+// Input features from the start in diff[0]: columns 0-1 the start, 2-3 /
+// 4-5 / 6-7 / 8-9 its Chebyshev filters of degree 2 / 4 / 8 / 16 (each pair
+// centred and scaled to unit variance per coordinate), column 10
+// log(1 + degree) / 4, column 11 one, the rest zero. Enqueued only.
+int build_features(gvizhip_gnn *g) {
+  const int64_t n = g->n;
+  g->hasSkip = false;
+  const unsigned grid = grid_for(n, kB);
+  float2 *buf[3] = {(float2 *)g->diff[0].p, (float2 *)g->diff[1].p,
+                    (float2 *)g->diff[2].p};
+  // y_k lives in buf[k % 3]; the start stays in buf[0] until y_3 replaces it
+  int mark = 0;
+  for (int k = 0; k <= kChebDeg; k++) {
+    if (k > 0)
+      hipLaunchKernelGGL(k_cheb, dim3(grid_for(n * kChebLanes, kB)), dim3(kB), 0, g->stream,
+                         g->off.p, g->adj.p,
+                         g->invDeg.p, (const float2 *)buf[(k - 1) % 3],
+                         (const float2 *)buf[(k + 1) % 3], k == 1 ? 1 : 0, n,
+                         buf[k % 3]);
+    if (k == 0 || (mark < 4 && k == kChebMarks[mark])) {
+      const int f = k == 0 ? 0 : ++mark;
+      double *st = g->stats.p + 4 * f;
+      if (moments_of(g, buf[k % 3], st))
+        return -1;
+      hipLaunchKernelGGL(k_put_feature, dim3(grid), dim3(kB), 0, g->stream,
+                         (const float2 *)buf[k % 3], n, 2 * f, (const double *)st, g->X0.p);
+    }
+  }
+  hipLaunchKernelGGL(k_put_tail, dim3(grid), dim3(kB), 0, g->stream, g->off.p, n, g->X0.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
 int check_graph(gvizhip_gnn *g) {
   if (!g || g->n <= 0) {
     set_error("gvizhip_gnn: no graph set");
@@ -876,6 +1118,7 @@ int gvizhip_gnn_create(gvizhip_gnn **out, int64_t maxVertices, int64_t maxEntrie
              g->M0.alloc(n * kF) || g->dY.alloc(n * kO) || g->dZ.alloc(n * kH) ||
              g->Y.alloc(n * kO) || g->dA.alloc(n * 2 * kH) || g->target.alloc(2 * n) ||
              g->feat32.alloc(1) || g->diff[0].alloc(2 * n) || g->diff[1].alloc(2 * n) ||
+             g->diff[2].alloc(2 * n) || g->stats.alloc(4 * 8) || g->placed.alloc(2 * n) ||
              g->params.alloc(kParams) || g->grads.alloc(kParams) ||
              g->adamM.alloc(kParams) || g->adamV.alloc(kParams) ||
              g->partial.alloc(rb * 2 * kH * kH) || g->moments.alloc(10 * 256);
@@ -1005,45 +1248,59 @@ int gvizhip_gnn_set_start(gvizhip_gnn *g, const double *start) {
     return -1;
   }
   const int64_t n = g->n;
-  g->hasSkip = false;
   std::vector<float> x(2 * n);
   for (int64_t i = 0; i < 2 * n; i++)
     x[i] = (float)start[i];
   GVH_CHECK(hipMemcpyAsync(g->diff[0].p, x.data(), sizeof(float) * 2 * n,
                            hipMemcpyHostToDevice, g->stream));
-  const int marks[5] = {0, 4, 16, 64, 256};
-  int cur = 0, done = 0;
-  const int blocks = (int)std::min<int64_t>(grid_for(n, kB), 256);
-  std::vector<double> part(4 * blocks);
-  for (int f = 0; f < 5; f++) {
-    for (; done < marks[f]; done++) {
-      hipLaunchKernelGGL(k_diffuse, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream,
-                         g->off.p, g->adj.p, g->invDeg.p, (const float2 *)g->diff[cur].p,
-                         n, (float2 *)g->diff[1 - cur].p);
-      cur = 1 - cur;
-    }
-    hipLaunchKernelGGL(k_moments2, dim3(blocks), dim3(kB), 0, g->stream,
-                       (const float2 *)g->diff[cur].p, n, g->moments.p);
-    GVH_CHECK_LAUNCH();
-    GVH_CHECK(hipMemcpyAsync(part.data(), g->moments.p, sizeof(double) * 4 * blocks,
-                             hipMemcpyDeviceToHost, g->stream));
-    GVH_CHECK(hipStreamSynchronize(g->stream));
-    double m[4] = {0, 0, 0, 0};
-    for (int b = 0; b < blocks; b++)
-      for (int j = 0; j < 4; j++)
-        m[j] += part[4 * b + j];
-    const double mx = m[0] / n, my = m[1] / n;
-    const double vx = m[2] / n - mx * mx, vy = m[3] / n - my * my;
-    const float sx = vx > 1e-30 ? (float)(1.0 / std::sqrt(vx)) : 0.0f;
-    const float sy = vy > 1e-30 ? (float)(1.0 / std::sqrt(vy)) : 0.0f;
-    hipLaunchKernelGGL(k_put_feature, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream,
-                       (const float2 *)g->diff[cur].p, n, 2 * f, (float)mx, (float)my, sx,
-                       sy, g->X0.p);
-  }
-  hipLaunchKernelGGL(k_put_tail, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream, g->off.p,
-                     n, g->X0.p);
-  GVH_CHECK_LAUNCH();
+  if (build_features(g))
+    return -1;
   GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int gvizhip_gnn_set_start_seeded(gvizhip_gnn *g, uint64_t seed) {
+  if (check_graph(g))
+    return -1;
+  hipLaunchKernelGGL(k_uniform_start, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
+                     seed, g->n, (float2 *)g->diff[0].p);
+  GVH_CHECK_LAUNCH();
+  if (build_features(g))
+    return -1;
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int gvizhip_gnn_propose(gvizhip_gnn *g, uint64_t seed, double radius, double *positions) {
+  if (check_graph(g) || !positions || !(radius > 0.0)) {
+    if (g && g->n > 0)
+      set_error("gvizhip_gnn_propose: invalid arguments");
+    return -1;
+  }
+  const int64_t n = g->n;
+  if (!g->pinned &&
+      hipHostMalloc((void **)&g->pinned, sizeof(double) * 2 * g->cap) != hipSuccess) {
+    g->pinned = nullptr;
+    set_error("gvizhip_gnn_propose: pinned staging allocation failed");
+    return -1;
+  }
+  hipLaunchKernelGGL(k_uniform_start, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream, seed,
+                     n, (float2 *)g->diff[0].p);
+  GVH_CHECK_LAUNCH();
+  if (build_features(g) || forward(g))
+    return -1;
+  hipLaunchKernelGGL(k_head_xy, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream, g->Y.p, n,
+                     (float2 *)g->diff[0].p);
+  if (moments_of(g, (const float2 *)g->diff[0].p, g->stats.p + 20))
+    return -1;
+  hipLaunchKernelGGL(k_place, dim3(grid_for(n, kB)), dim3(kB), 0, g->stream,
+                     (const float2 *)g->diff[0].p, n, (const double *)(g->stats.p + 20),
+                     radius, (double2 *)g->placed.p);
+  GVH_CHECK_LAUNCH();
+  GVH_CHECK(hipMemcpyAsync(g->pinned, g->placed.p, sizeof(double) * 2 * n,
+                           hipMemcpyDeviceToHost, g->stream));
+  GVH_CHECK(hipStreamSynchronize(g->stream));
+  std::memcpy(positions, g->pinned, sizeof(double) * 2 * n);
   return 0;
 }
 
@@ -1280,6 +1537,8 @@ void gvizhip_gnn_destroy(gvizhip_gnn *g) {
     (void)hipStreamSynchronize(g->stream);
     (void)hipStreamDestroy(g->stream);
   }
+  if (g->pinned)
+    (void)hipHostFree(g->pinned);
   delete g;
 }
 

@@ -165,6 +165,8 @@ def lib():
         "gvizhip_gnn_set_graph_diffused": (C.c_int, [_vp, _i64, _vp, _vp, _vp]),
         "gvizhip_gnn_set_pivots": (C.c_int, [_vp, _vp, _i64, _vp]),
         "gvizhip_gnn_set_start": (C.c_int, [_vp, _vp]),
+        "gvizhip_gnn_set_start_seeded": (C.c_int, [_vp, C.c_uint64]),
+        "gvizhip_gnn_propose": (C.c_int, [_vp, C.c_uint64, C.c_double, _vp]),
         "gvizhip_gnn_forward": (C.c_int, [_vp, _vp]),
         "gvizhip_gnn_gradients": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), _vp]),
         "gvizhip_gnn_train_step": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
@@ -213,7 +215,8 @@ DECLARED = [
     "gvizhip_stress_create", "gvizhip_stress_evaluate", "gvizhip_stress_destroy",
     "gvizhip_gnn_create", "gvizhip_gnn_param_count", "gvizhip_gnn_set_params",
     "gvizhip_gnn_get_params", "gvizhip_gnn_set_graph", "gvizhip_gnn_set_graph_diffused",
-    "gvizhip_gnn_set_pivots", "gvizhip_gnn_set_start", "gvizhip_gnn_forward",
+    "gvizhip_gnn_set_pivots", "gvizhip_gnn_set_start", "gvizhip_gnn_set_start_seeded",
+    "gvizhip_gnn_propose", "gvizhip_gnn_forward",
     "gvizhip_gnn_gradients", "gvizhip_gnn_train_step", "gvizhip_gnn_destroy",
 ]
 
@@ -627,6 +630,21 @@ class GnnEngine:
         st = np.ascontiguousarray(start, dtype=np.float64).reshape(self.n, 2)
         if lib().gvizhip_gnn_set_start(self.h, _p(st)) != 0:
             _err("gvizhip_gnn_set_start")
+
+    def set_start_seeded(self, seed):
+        """New diffused features of the current graph from a start drawn on
+        the device from `seed`."""
+        if lib().gvizhip_gnn_set_start_seeded(self.h, int(seed) & (2**64 - 1)) != 0:
+            _err("gvizhip_gnn_set_start_seeded")
+
+    def propose(self, seed, radius):
+        """Network output from a device-drawn start, centred and scaled to RMS
+        radius `radius` on the device (n x 2)."""
+        out = np.empty((self.n, 2))
+        if lib().gvizhip_gnn_propose(self.h, int(seed) & (2**64 - 1), float(radius),
+                                     _p(out)) != 0:
+            _err("gvizhip_gnn_propose")
+        return out
 
     def set_pivots(self, pivots):
         """Pivot-MDS features and skip term from BFS distances to `pivots`

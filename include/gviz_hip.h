@@ -420,10 +420,13 @@ int gvizhip_gnn_get_params(gvizhip_gnn *g, float *params);
 int gvizhip_gnn_set_graph(gvizhip_gnn *g, int64_t n, const int64_t *offsets,
                           const int64_t *adj, const float *features);
 /* The same graph upload with the features built on the device from a start
- * layout (n x 2): columns 0-1 the start, 2-3 / 4-5 / 6-7 / 8-9 the start
- * after 4 / 16 / 64 / 256 lazy random-walk steps x <- (x + mean of the
- * neighbours' x) / 2, each pair centred and scaled to unit variance per
- * coordinate; column 10 log(1 + degree) / 4, column 11 one, the rest 0. */
+ * layout (n x 2): columns 0-1 the start, 2-3 / 4-5 / 6-7 / 8-9 its Chebyshev
+ * low-pass filters T_d(mu(P)) x of degree 2 / 4 / 8 / 16, where P is the
+ * lazy random walk x <- (x + mean of the neighbours' x) / 2 and
+ * mu(P) = (2 P - 0.9 I) / 0.9 (recurrence y_1 = mu y_0,
+ * y_{k+1} = 2 mu y_k - y_{k-1}), each pair centred and scaled to unit
+ * variance per coordinate; column 10 log(1 + degree) / 4, column 11 one, the
+ * rest 0. */
 int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n,
                                    const int64_t *offsets, const int64_t *adj,
                                    const double *start);
@@ -431,6 +434,16 @@ int gvizhip_gnn_set_graph_diffused(gvizhip_gnn *g, int64_t n,
  * gvizhip_gnn_set_graph_diffused does, without uploading the graph again;
  * removes the pivot-MDS skip term. */
 int gvizhip_gnn_set_start(gvizhip_gnn *g, const double *start);
+/* The same from a start drawn on the device: vertex v's coordinates are
+ * bits 40-63 and 16-39 of SplitMix64(seed + (v + 1) * 0x9E3779B97F4A7C15),
+ * each scaled to [-1, 1) as 2 b / 2^24 - 1. */
+int gvizhip_gnn_set_start_seeded(gvizhip_gnn *g, uint64_t seed);
+/* A proposed start for the graph set last, with no host work between the
+ * steps: seeded start and features as gvizhip_gnn_set_start_seeded, the
+ * forward pass, the two outputs centred and scaled to RMS radius `radius`
+ * in FP64, one copy to positions (n x 2). */
+int gvizhip_gnn_propose(gvizhip_gnn *g, uint64_t seed, double radius,
+                        double *positions);
 /* Pivot MDS (Brandes & Pich) of the graph set last: BFS hop distances to
  * `count` (<= 64) pivot vertices, D^2 double-centred into C, axes C v / mu^(1/4)
  * for the top two eigenpairs (v, mu) of C^T C, centred and scaled to RMS

@@ -69,22 +69,58 @@ def mean_agg_T(off, adj, D, inv):
     return S
 
 
+CHEB_C = 0.9
+CHEB_MARKS = (2, 4, 8, 16)
+
+
+def lazy_walk(off, adj, x):
+    """P x for the lazy random walk P = (I + D^-1 A) / 2; isolated vertices
+    keep x."""
+    deg = np.diff(off)
+    m, _ = mean_agg(off, adj, x)
+    return np.where(deg[:, None] > 0, 0.5 * (x + m), x)
+
+
+def cheb_filters(off, adj, x, degrees=CHEB_MARKS, c=CHEB_C):
+    """T_d(mu(P)) x for each d in `degrees`, mu(P) = (2 P - c I) / c, by the
+    three-term recurrence y_1 = mu y_0, y_{k+1} = 2 mu y_k - y_{k-1}."""
+    mu = lambda y: (2.0 / c) * lazy_walk(off, adj, y) - y
+    y_prev, y = None, np.asarray(x, dtype=np.float64)
+    out = {0: y}
+    for k in range(1, max(degrees) + 1):
+        y_next = mu(y) if k == 1 else 2.0 * mu(y) - y_prev
+        y_prev, y = y, y_next
+        out[k] = y
+    return [out[d] for d in degrees]
+
+
+def uniform_start(n, seed):
+    """The device-drawn start (gvizhip_gnn_set_start_seeded): SplitMix64
+    (Steele, Lea & Flood 2014) of seed + (v + 1) * 0x9E3779B97F4A7C15, bits
+    40-63 and 16-39 as 2 b / 2^24 - 1."""
+    M = (1 << 64) - 1
+    out = np.zeros((n, 2))
+    for v in range(n):
+        z = (seed + (v + 1) * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out[v, 0] = (z >> 40) / 16777216.0 * 2.0 - 1.0
+        out[v, 1] = ((z >> 16) & 0xFFFFFF) / 16777216.0 * 2.0 - 1.0
+    return out
+
+
 def diffused_features(off, adj, start):
     """The device-built inputs (gvizhip_gnn_set_graph_diffused): the start and
-    its images after 4, 16, 64, 256 lazy random-walk steps
-    x <- (x + mean of neighbours' x) / 2, each pair centred and scaled to unit
-    variance per coordinate; log(1 + degree) / 4; a constant one."""
+    its Chebyshev low-pass filters of degree 2, 4, 8, 16 (cheb_filters), each
+    pair centred and scaled to unit variance per coordinate;
+    log(1 + degree) / 4; a constant one."""
     n = len(off) - 1
     deg = np.diff(off)
-    x = np.asarray(start, dtype=np.float64).reshape(n, 2).copy()
+    x = np.asarray(start, dtype=np.float64).reshape(n, 2)
     X = np.zeros((n, F))
-    done = 0
-    for f, mark in enumerate((0, 4, 16, 64, 256)):
-        while done < mark:
-            m, _ = mean_agg(off, adj, x)
-            x = np.where(deg[:, None] > 0, 0.5 * (x + m), x)
-            done += 1
-        c = x - x.mean(0)
+    for f, y in enumerate([x] + cheb_filters(off, adj, x)):
+        c = y - y.mean(0)
         sd = np.sqrt((c * c).mean(0))
         X[:, 2 * f:2 * f + 2] = c / np.where(sd > 1e-15, sd, np.inf)
     X[:, 10] = np.log1p(deg) / 4.0

@@ -138,3 +138,63 @@ def test_skip_term_enters_output_and_gradients():
         fd = (N.loss_and_grads(tp, off, adj, X, T, skip=skip)[0] -
               N.loss_and_grads(tm, off, adj, X, T, skip=skip)[0]) / (2 * h)
         assert abs(fd - grad[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, grad[i])
+
+
+def _dense_lazy_walk(off, adj):
+    n = len(off) - 1
+    P = np.zeros((n, n))
+    for v in range(n):
+        d = off[v + 1] - off[v]
+        P[v, v] = 0.5 if d else 1.0
+        for u in adj[off[v]:off[v + 1]]:
+            P[v, u] += 0.5 / d
+    return P
+
+
+def test_cheb_filter_of_a_cycle_eigenvector_is_the_chebyshev_value():
+    """On the cycle C_n, cos(2 pi j v / n) is an eigenvector of the lazy walk
+    with eigenvalue (1 + cos(2 pi j / n)) / 2; the filter multiplies it by
+    T_d(mu), mu = (2 lambda - c) / c, T_d the Chebyshev polynomial."""
+    n = 60
+    e = np.array([(i, (i + 1) % n) for i in range(n)])
+    off, adj = G.csr_from_edges(n, e)
+    v = np.arange(n)
+    for j in (1, 3, 11):
+        x = np.stack([np.cos(2 * np.pi * j * v / n), np.sin(2 * np.pi * j * v / n)], 1)
+        lam = 0.5 * (1 + np.cos(2 * np.pi * j / n))
+        mu = (2 * lam - N.CHEB_C) / N.CHEB_C
+        got = N.cheb_filters(off, adj, x)
+        for d, y in zip(N.CHEB_MARKS, got):
+            t = np.polynomial.chebyshev.chebval(mu, [0] * d + [1])
+            assert np.allclose(y, t * x, rtol=1e-9, atol=1e-9 * max(1.0, abs(t)))
+
+
+def test_cheb_filter_equals_the_matrix_polynomial():
+    """On an irregular graph with an isolated vertex: T_d(mu(P)) from the
+    eigendecomposition-free matrix polynomial (numpy's Chebyshev series of
+    the dense P) equals the recurrence."""
+    n, e = G.erdos_renyi(30, 3.0, 7)
+    off, adj = G.csr_from_edges(n + 1, e)  # vertex n is isolated
+    P = _dense_lazy_walk(off, adj)
+    M = (2.0 * P - N.CHEB_C * np.eye(n + 1)) / N.CHEB_C
+    x = np.random.default_rng(2).normal(size=(n + 1, 2))
+    for d, y in zip(N.CHEB_MARKS, N.cheb_filters(off, adj, x)):
+        T0, T1 = np.eye(n + 1), M
+        for _ in range(d - 1):
+            T0, T1 = T1, 2 * M @ T1 - T0
+        assert np.allclose(y, T1 @ x, rtol=1e-10, atol=1e-10 * np.abs(T1 @ x).max())
+
+
+def test_uniform_start_is_splitmix64():
+    """SplitMix64's published first outputs from state 0 (Vigna's reference
+    C, which adds the gamma before mixing): 0xE220A8397B1DCDAF,
+    0x6E789E6AA1B965F4, 0x06C45D188009454F; coordinates are their bit fields
+    40-63 and 16-39, and every value lies in [-1, 1)."""
+    first = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    S = N.uniform_start(3, 0)
+    for v, z in enumerate(first):
+        assert S[v, 0] == (z >> 40) / 2 ** 24 * 2 - 1
+        assert S[v, 1] == ((z >> 16) & 0xFFFFFF) / 2 ** 24 * 2 - 1
+    U = N.uniform_start(4000, 12345)
+    assert U.min() >= -1.0 and U.max() < 1.0
+    assert abs(U.mean()) < 0.03 and abs(U.var() - 1 / 3) < 0.02

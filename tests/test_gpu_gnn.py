@@ -63,6 +63,28 @@ def test_device_features_match_oracle(gpu):
     assert _norm_err(y_gpu, y_ref) <= 2e-2, _norm_err(y_gpu, y_ref)
 
 
+def test_seeded_start_and_proposal_match_oracle(gpu):
+    """Features from the device-drawn start (SplitMix64, restated in the
+    oracle) and the proposal: the network's output centred and scaled to
+    the requested RMS radius on the device."""
+    n, off, adj, _ = _case(5, 700, 4)
+    th = _theta(8)
+    eng = gpu.GnnEngine(n, len(adj))
+    eng.set_params(th)
+    eng.set_graph_diffused(off, adj, np.zeros((n, 2)))
+    eng.set_start_seeded(77)
+    y_gpu = eng.forward()
+    X = N.diffused_features(off, adj, N.uniform_start(n, 77))
+    y_ref = N.forward(th.astype(np.float64), off, adj, X, rb=True)[0][:, :2]
+    assert _norm_err(y_gpu, y_ref) <= 2e-2, _norm_err(y_gpu, y_ref)
+    p = eng.propose(77, 123.0)
+    c = y_gpu - y_gpu.mean(0)
+    want = c * (123.0 / np.sqrt((c * c).sum(1).mean()))
+    assert _norm_err(p, want) <= 1e-6, _norm_err(p, want)
+    assert abs(np.sqrt((p * p).sum(1).mean()) - 123.0) <= 1e-9 * 123.0
+    assert np.abs(p.mean(0)).max() <= 1e-9 * 123.0
+
+
 def test_gradients_match_oracle(gpu):
     n, off, adj, x = _case(2, 900, 9)
     th = _theta(3)
