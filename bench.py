@@ -521,7 +521,7 @@ def bench_batch(args):
                gravity=0.0)
     b = native.BatchEngine(graphs, **cfg)
     b.begin(start)
-    order, spl = b.autotune(4)
+    order, spl, tw = b.autotune(4)
     b.begin(start)
     max_steps, eps = 800, 1.0
     t1 = time.perf_counter()
@@ -540,7 +540,8 @@ def bench_batch(args):
                      "FP32; each graph until maxDisp < 1.0 (a tenth of an edge) or 800 steps",
                      {"graphs": len(graphs), "vertices": int(sizes.sum()),
                       "edges": int(sum(len(a) for _, a in graphs) // 2),
-                      "schedule": {"wave_order": order, "steps_per_launch": spl}})
+                      "schedule": {"wave_order": order, "steps_per_launch": spl,
+                                   "traversal_waves_per_workgroup": tw}})
     line["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2),
                           "p99": round(float(np.percentile(lat, 99)), 2),
                           "max": round(float(lat.max()), 2)}

@@ -329,13 +329,16 @@ int gvizhip_batch_begin(gvizhip_batch *bt, const double *positions) {
   return 0;
 }
 
-int gvizhip_batch_schedule(gvizhip_batch *bt, int32_t waveOrder, int32_t stepsPerLaunch) {
+int gvizhip_batch_schedule(gvizhip_batch *bt, int32_t waveOrder, int32_t stepsPerLaunch,
+                           int32_t traversalWaves) {
   if (!bt || waveOrder < 0 || waveOrder > 1 || stepsPerLaunch < 1 ||
-      stepsPerLaunch > 64) {
+      stepsPerLaunch > 64 ||
+      (traversalWaves != 1 && traversalWaves != 2 && traversalWaves != 4)) {
     set_error("gvizhip_batch_schedule: invalid arguments");
     return -1;
   }
   bt->stepsPerLaunch = stepsPerLaunch;
+  bt->c.travWaves = traversalWaves;
   if (waveOrder != bt->waveOrder) {
     bt->waveOrder = waveOrder;
     if (upload_waves(bt))
@@ -426,7 +429,7 @@ int64_t gvizhip_batch_run(gvizhip_batch *bt, int64_t maxIters, double epsilon,
 }
 
 int gvizhip_batch_autotune(gvizhip_batch *bt, int64_t trialSteps, int32_t *waveOrder,
-                           int32_t *stepsPerLaunch) {
+                           int32_t *stepsPerLaunch, int32_t *traversalWaves) {
   if (!bt || trialSteps < 1) {
     set_error("gvizhip_batch_autotune: invalid arguments");
     return -1;
@@ -434,11 +437,13 @@ int gvizhip_batch_autotune(gvizhip_batch *bt, int64_t trialSteps, int32_t *waveO
   if (save_state(bt))
     return -1;
   double best = INFINITY;
-  int32_t bo = bt->waveOrder, bsp = bt->stepsPerLaunch;
+  int32_t bo = bt->waveOrder, bsp = bt->stepsPerLaunch, btw = bt->c.travWaves;
   const int32_t spl[3] = {1, 4, 16};
+  const int32_t tws[3] = {1, 2, 4};
   for (int32_t order = 0; order <= 1; order++)
-    for (int32_t k = 0; k < 3; k++) {
-      if (gvizhip_batch_schedule(bt, order, spl[k]) || restore_state(bt))
+    for (int32_t k = 0; k < 3; k++)
+      for (int32_t tw : tws) {
+      if (gvizhip_batch_schedule(bt, order, spl[k], tw) || restore_state(bt))
         return -1;
       // one launch to build and warm, then the timed trial
       if (gvizhip_batch_run(bt, spl[k], -1.0, nullptr, nullptr, nullptr) < 0)
@@ -454,14 +459,17 @@ int gvizhip_batch_autotune(gvizhip_batch *bt, int64_t trialSteps, int32_t *waveO
         best = per;
         bo = order;
         bsp = spl[k];
+        btw = tw;
       }
     }
-  if (gvizhip_batch_schedule(bt, bo, bsp) || restore_state(bt))
+  if (gvizhip_batch_schedule(bt, bo, bsp, btw) || restore_state(bt))
     return -1;
   if (waveOrder)
     *waveOrder = bo;
   if (stepsPerLaunch)
     *stepsPerLaunch = bsp;
+  if (traversalWaves)
+    *traversalWaves = btw;
   return 0;
 }
 

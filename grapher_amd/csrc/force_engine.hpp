@@ -132,6 +132,7 @@ struct ForceCommon {
   // measurement hook: events around the traversal kernel (never captured)
   hipEvent_t travBegin = nullptr, travEnd = nullptr;
   int timeTraversal = 0;
+  int travWaves = 1; // traversal wavefronts per workgroup (1, 2 or 4)
   DevBuf<unsigned long long> travStats; // null unless GVIZHIP_TRAVERSAL_STATS
   int64_t stepsSinceRecordBase = 0;
   int reductionBlocks = 0;

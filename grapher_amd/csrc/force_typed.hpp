@@ -401,11 +401,6 @@ constexpr int64_t kTravMaxVertices = int64_t(1) << 29;
 #ifndef GVH_TRAV_MINBLOCKS
 #define GVH_TRAV_MINBLOCKS 1
 #endif
-#ifndef GVH_TRAV_THREADS
-#define GVH_TRAV_THREADS 64
-#endif
-constexpr int kTravThreads = GVH_TRAV_THREADS;  // one wavefront per block
-constexpr int kTravWaves = kTravThreads / kWave;
 constexpr int kTravStack = GVH_TRAV_STACK;      // pending (node, mask) per wave
 constexpr int kTravDefer = GVH_TRAV_DEFER;      // deferred bodies per wave
 // Stack entries held back from the wide phase. Once the stack holds more than
@@ -501,8 +496,10 @@ __device__ __forceinline__ double uniform(double v) {
 // Stack entries: a cell c >= 0; a point p as -1 - p; a cell to be opened
 // for the entry's lanes (put back by the undecided test) as -1 - n - c,
 // which keeps every entry an int32 for n < 2^29 vertices.
-template <class Real, class PosT, int MODEL, bool OVERLAP>
-__global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
+// WPB wavefronts per workgroup: the waves never synchronise with each
+// other, so WPB only changes how the hardware packs them onto the CUs.
+template <class Real, class PosT, int MODEL, bool OVERLAP, int WPB>
+__global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
     const NodeRec<Real, PosT> *__restrict__ nodes,
     const typename V2<PosT>::type *__restrict__ spos,
     const typename V2<Real>::type *__restrict__ smr,
@@ -514,9 +511,9 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   // FP32 force math: pair terms from rep_pair, flushes replayed with guards
   // when their sum is not finite; FP64 keeps the reference's formulas
   constexpr bool FAST = sizeof(Real) == sizeof(float);
-  __shared__ int32_t sNode[kTravWaves][kTravStack];
-  __shared__ uint64_t sMask[kTravWaves][kTravStack];
-  __shared__ TravBody<Real, PosT> dBody[kTravWaves][kTravDefer];
+  __shared__ int32_t sNode[WPB][kTravStack];
+  __shared__ uint64_t sMask[WPB][kTravStack];
+  __shared__ TravBody<Real, PosT> dBody[WPB][kTravDefer];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   int64_t s;
@@ -524,7 +521,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
   int32_t root;
   if (sv.waveGraph) {
     // batch: this wave serves 64 consecutive positions of one graph
-    const int64_t w = (int64_t)blockIdx.x * kTravWaves + wave;
+    const int64_t w = (int64_t)blockIdx.x * WPB + wave;
     if (w >= sv.waves)
       return;
     const int32_t g = sv.waveGraph[w];
@@ -536,7 +533,7 @@ __global__ __launch_bounds__(kTravThreads, GVH_TRAV_MINBLOCKS) void k_traverse(
     if (root == -1)
       root = -(int32_t)(sv.gStart[g] + 1);
   } else {
-    s = ((int64_t)blockIdx.x * kTravWaves + wave) * kWave + lane;
+    s = ((int64_t)blockIdx.x * WPB + wave) * kWave + lane;
     valid = s < n;
     root = n >= 2 ? 0 : -1;
   }
@@ -1236,7 +1233,7 @@ int enqueue_stats(ForceCommon &c, int blocks);
 // k_traverse for the configured model; the overlap-free FP32 form when no
 // vertex has a radius (FP64 always takes the reference's formulas).
 template <class Real, class PosT>
-int launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int32_t *inter,
+int launch_traverse(ForceCommon &c, int model, int64_t waves, hipStream_t st, int32_t *inter,
                     Real theta, Real kSq, Real oc, Scalars *sc,
                     unsigned long long *tstats, SegView sv) {
   using P2 = typename V2<PosT>::type;
@@ -1248,12 +1245,23 @@ int launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int3
   }
   constexpr bool F64 = sizeof(Real) == sizeof(double);
   const bool ov = F64 || c.prm.preventOverlap != 0;
-#define GVH_TRAVERSE(M, OV)                                                              \
-  hipLaunchKernelGGL((k_traverse<Real, PosT, M, OV>), dim3(tb), dim3(kTravThreads), 0, st, \
+  const int wpb = c.travWaves == 2 || c.travWaves == 4 ? c.travWaves : 1;
+  const unsigned tb = (unsigned)((waves + wpb - 1) / wpb);
+#define GVH_TRAVERSE_W(M, OV, W)                                                           \
+  hipLaunchKernelGGL((k_traverse<Real, PosT, M, OV, W>), dim3(tb), dim3(W * kWave), 0, st,  \
                      (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,        \
                      (const R2 *)c.smr.p, c.vals.p, c.n, theta, (double)c.prm.theta, kSq,  \
                      oc, (R2 *)c.rep.p,                                                   \
                      inter, c.radius64.p, sc, tstats, sv)
+#define GVH_TRAVERSE(M, OV)                                                              \
+  do {                                                                                   \
+    if (wpb == 4)                                                                        \
+      GVH_TRAVERSE_W(M, OV, 4);                                                          \
+    else if (wpb == 2)                                                                   \
+      GVH_TRAVERSE_W(M, OV, 2);                                                          \
+    else                                                                                 \
+      GVH_TRAVERSE_W(M, OV, 1);                                                          \
+  } while (0)
   if (model == 1) {
     if (ov)
       GVH_TRAVERSE(1, true);
@@ -1266,6 +1274,7 @@ int launch_traverse(ForceCommon &c, int model, unsigned tb, hipStream_t st, int3
       GVH_TRAVERSE(0, F64);
   }
 #undef GVH_TRAVERSE
+#undef GVH_TRAVERSE_W
   return 0;
 }
 
@@ -1346,10 +1355,10 @@ int typed_step(ForceCommon &c, int mode) {
     // attraction (memory-bound) overlaps the traversal (ALU-bound)
     if (fork_attraction<Real, PosT>(c, posC, model))
       return -1;
-    const unsigned tb = (unsigned)((n + kTravThreads - 1) / kTravThreads);
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
-    if (launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, c.sc.p,
+    if (launch_traverse<Real, PosT>(c, model, (n + kWave - 1) / kWave, st, inter, theta,
+                                    kSq, oc, c.sc.p,
                                     c.travStats.p, SegView{}))
       return -1;
     GVH_CHECK_LAUNCH();
@@ -1555,11 +1564,10 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
   // attraction (memory-bound) overlaps the traversal (ALU-bound)
   if (fork_attraction<Real, PosT>(c, posC, model))
     return -1;
-  const unsigned tb = (unsigned)((bs.waves + kTravWaves - 1) / kTravWaves);
   int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travBegin, st));
-  if (launch_traverse<Real, PosT>(c, model, tb, st, inter, theta, kSq, oc, bs.scG.p,
+  if (launch_traverse<Real, PosT>(c, model, bs.waves, st, inter, theta, kSq, oc, bs.scG.p,
                                   (unsigned long long *)nullptr, sv))
     return -1;
   GVH_CHECK_LAUNCH();

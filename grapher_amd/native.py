@@ -148,8 +148,9 @@ def lib():
         "gvizhip_grip_destroy": (None, [_vp]),
         "gvizhip_batch_create": (C.c_int, [C.POINTER(_vp), _i64, _vp, _vp, _vp, C.POINTER(ForceConfig)]),
         "gvizhip_batch_begin": (C.c_int, [_vp, _vp]),
-        "gvizhip_batch_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32]),
-        "gvizhip_batch_autotune": (C.c_int, [_vp, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "gvizhip_batch_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
+        "gvizhip_batch_autotune": (C.c_int, [_vp, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32)]),
         "gvizhip_batch_run": (_i64, [_vp, _i64, C.c_double, _vp, _vp, _vp]),
         "gvizhip_batch_profile": (C.c_int, [_vp, _i64, C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(C.c_double)]),
         "gvizhip_batch_download_positions": (C.c_int, [_vp, _vp]),
@@ -756,15 +757,19 @@ class BatchEngine:
         if lib().gvizhip_batch_begin(self.h, _p(p)) != 0:
             _err("gvizhip_batch_begin")
 
-    def schedule(self, wave_order=0, steps_per_launch=4):
-        if lib().gvizhip_batch_schedule(self.h, wave_order, steps_per_launch) != 0:
+    def schedule(self, wave_order=0, steps_per_launch=4, traversal_waves=1):
+        if lib().gvizhip_batch_schedule(self.h, wave_order, steps_per_launch,
+                                        traversal_waves) != 0:
             _err("gvizhip_batch_schedule")
 
     def autotune(self, trial_steps=8):
-        o, s = C.c_int32(), C.c_int32()
-        if lib().gvizhip_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(s)) != 0:
+        """(wave order, steps per launch, traversal waves per workgroup) of the
+        fastest schedule, which the engine keeps."""
+        o, s, t = C.c_int32(), C.c_int32(), C.c_int32()
+        if lib().gvizhip_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(s),
+                                        C.byref(t)) != 0:
             _err("gvizhip_batch_autotune")
-        return o.value, s.value
+        return o.value, s.value, t.value
 
     def run(self, iters, epsilon=-1.0):
         """(steps launched, per-graph steps, per-graph latency ms, last maxDisp)."""

@@ -429,6 +429,43 @@ def evaluate_many(ws, specs, n=200_000, workers=12, **kw):
         return [evaluate(ws, n, b, s, graph=g, **kw) for (b, s), g in zip(specs, built)]
 
 
+def headroom(ws, n, blocks, seed, chunk=2, plain_iters=500, precision="mixed"):
+    """Steps to the plain method's final stress (within 1%) from four starts
+    on one connected SBM graph: the random start, the network's, a
+    converged FP64 layout from another random start (what a perfect network
+    would propose) and that layout's block centroids with each block's
+    vertices scattered uniformly within half the block's RMS radius around
+    them (the global arrangement alone). None: not reached in plain_iters."""
+    n0, e0, labels = sbm(n, blocks, seed)
+    act, inside = G.largest_component(n0, e0)
+    ids = np.flatnonzero(act)
+    index = -np.ones(n0, np.int64)
+    index[ids] = np.arange(ids.size)
+    m, lab = ids.size, labels[ids]
+    off, adj = G.csr_from_edges(m, index[inside])
+    meter = native.StressMeter(off, adj, np.random.default_rng(seed).choice(m, 64, replace=False))
+    it_p, _, traj = time_to_stress(off, adj, G.uniform_positions(m, seed + 1), meter, None,
+                                   plain_iters, chunk, precision)
+    target = 1.01 * traj[-1][1]
+    out = {"blocks": blocks, "final": round(traj[-1][1], 5),
+           "random": next(a for a, b in traj if b <= target)}
+    ws.prepare(off, adj)
+    conv = force_layout(off, adj, seed=seed + 5, iters=400)
+    cent = np.zeros((blocks, 2))
+    rad = np.zeros(blocks)
+    for k in range(blocks):
+        pk = conv[lab == k]
+        if len(pk):
+            cent[k] = pk.mean(0)
+            rad[k] = np.sqrt(((pk - cent[k]) ** 2).sum(1).mean())
+    glob = cent[lab] + 0.5 * rad[lab][:, None] * G.uniform_positions(m, seed + 9)
+    for name, st in (("network", ws.propose(seed=seed + 1)), ("converged", conv),
+                     ("centroids", glob)):
+        it, _, tr = time_to_stress(off, adj, st, meter, target, plain_iters, chunk, precision)
+        out[name] = int(it) if tr[-1][1] <= target else None
+    return out
+
+
 def summarize(rows):
     """Totals and medians of evaluate() rows: summed plain seconds over
     summed warm seconds (graphs the warm start brought to the target), the
@@ -483,6 +520,16 @@ def main():
     ts.add_argument("--out", default=WEIGHTS)
     ts.add_argument("--checkpoint", default=None,
                     help="weights + progress after every chunk; resumed from when present")
+    hr = sub.add_parser("headroom", help="steps to the target from four kinds of start")
+    hr.add_argument("--n", type=int, default=200000)
+    hr.add_argument("--graphs", type=int, default=10)
+    hr.add_argument("--seed", type=int, default=99)
+    hr.add_argument("--seed-base", type=int, default=50000)
+    tp = sub.add_parser("time-propose", help="time proposals on one SBM graph")
+    tp.add_argument("--n", type=int, default=200000)
+    tp.add_argument("--blocks", type=int, default=12)
+    tp.add_argument("--seed", type=int, default=7)
+    tp.add_argument("--reps", type=int, default=5)
     ev = sub.add_parser("eval")
     ev.add_argument("--n", type=int, default=200000)
     ev.add_argument("--graphs", type=int, default=5)
@@ -499,6 +546,25 @@ def main():
         for r in rows:
             print(json.dumps(r), flush=True)
         print("# summary " + json.dumps(summarize(rows)), flush=True)
+        return
+    if args.cmd == "headroom":
+        import json
+        ws = WarmStart.load()
+        rng = np.random.default_rng(args.seed)
+        for i in range(args.graphs):
+            print(json.dumps(headroom(ws, args.n, int(rng.integers(2, 33)),
+                                      args.seed_base + i)), flush=True)
+        return
+    if args.cmd == "time-propose":
+        ws = WarmStart.load()
+        off, adj = _sbm_csr((args.n, args.blocks, args.seed))
+        ws.prepare(off, adj)
+        ws.propose(seed=0)  # pinned staging and first launches
+        for rep in range(args.reps):
+            t0 = time.perf_counter()
+            p = ws.propose(seed=rep + 1)
+            print(f"n={len(off) - 1}: proposal {1e3 * (time.perf_counter() - t0):.3f} ms, "
+                  f"RMS radius {rms_radius(p):.2f}", flush=True)
         return
     if args.cmd == "train-stream":
         t0 = time.time()

@@ -362,13 +362,16 @@ int gvizhip_batch_create(gvizhip_batch **out, int64_t graphs,
 /* Positions (2n) for every graph; speed 1, efficiency 1, forces zero. */
 int gvizhip_batch_begin(gvizhip_batch *b, const double *positions);
 /* Schedule knobs that never change results: traversal wave order (0: graph
- * order, 1: largest graphs first) and steps per captured launch (1..64). */
+ * order, 1: largest graphs first), steps per captured launch (1..64) and
+ * traversal wavefronts per workgroup (1, 2 or 4; the waves never
+ * synchronise, so this only changes how they are packed onto the CUs). */
 int gvizhip_batch_schedule(gvizhip_batch *b, int32_t waveOrder,
-                           int32_t stepsPerLaunch);
-/* Times every schedule on `trialSteps` steps from the current state, puts
- * the state back and keeps the fastest schedule. */
+                           int32_t stepsPerLaunch, int32_t traversalWaves);
+/* Times every schedule (2 x 3 x 3) on `trialSteps` steps from the current
+ * state, puts the state back and keeps the fastest; outputs may be NULL. */
 int gvizhip_batch_autotune(gvizhip_batch *b, int64_t trialSteps,
-                           int32_t *waveOrder, int32_t *stepsPerLaunch);
+                           int32_t *waveOrder, int32_t *stepsPerLaunch,
+                           int32_t *traversalWaves);
 /* Steps every graph until its maxDisp < epsilon (epsilon >= 0) or maxIters
  * steps. Per graph (any may be NULL): steps taken, milliseconds from the
  * call to the check that saw it stop (the call's total when it ran out of

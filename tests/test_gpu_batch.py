@@ -53,15 +53,15 @@ def test_batch_schedules_do_not_change_results(gpu):
     graphs = _mixed_graphs(2)
     start = np.concatenate([G.uniform_positions(len(o) - 1, i) for i, (o, _) in enumerate(graphs)])
     ref = None
-    for order, spl in ((0, 1), (1, 4), (1, 16)):
+    for order, spl, tw in ((0, 1, 1), (1, 4, 1), (1, 16, 2), (0, 4, 4)):
         b = gpu.BatchEngine(graphs, precision="fp32", **PARAMS)
         b.begin(start)
-        b.schedule(order, spl)
+        b.schedule(order, spl, tw)
         b.run(20)
         p = b.positions()
         if ref is None:
             ref = p
-        assert np.array_equal(p, ref), (order, spl)
+        assert np.array_equal(p, ref), (order, spl, tw)
     b = gpu.BatchEngine(graphs, precision="fp32", **PARAMS)
     b.begin(start)
     choice = b.autotune(4)
