@@ -413,13 +413,19 @@ def bench_grip(args):
     f_ms, r_ms = eng.profile_rounds(na, 0, 5)
     launch_s = f_ms / 1e3 / 5
     deg_sum = int(adj.shape[0])
-    # what k_refine reads and writes once per active vertex at layer 0: its
-    # filtration entry (4), four dim-vectors of FP64 (position read,
-    # displacement read and written, previous displacement written), CSR
-    # offsets (8), kNN count (4), heat and cosine read and written (32), last
-    # force magnitude (8), the computed flag read and written (2); 4 B per
-    # adjacency entry and 4 B per kNN id (layer 0 reads no kNN distances)
-    bytes_per = na * (58 + 32 * dim) + 4 * deg_sum + 4 * k * na
+    # the fixed convention for a GRIP round's algorithmic bytes: per active
+    # vertex k kNN slots of a 4-byte id and a 2-byte distance, 4 bytes per
+    # adjacency entry, an FP32 position read (4 dim) and written (4 dim), the
+    # displacement and previous displacement (8 dim), heat and previous
+    # cosine (8); plus the CSR offsets
+    bytes_per = na * (6 * k + 16 * dim + 8) + 4 * deg_sum + 4 * (n + 1)
+    # what this engine's k_refine itself moves once per active vertex at
+    # layer 0 (FP64 state): its filtration entry (4), four dim-vectors of
+    # FP64 (position read, displacement read and written, previous
+    # displacement written), CSR offsets (8), kNN count (4), heat and cosine
+    # read and written (32), last force magnitude (8), the computed flag
+    # read and written (2); 4 B per adjacency entry and per kNN id
+    bytes_engine = na * (58 + 32 * dim) + 4 * deg_sum + 4 * k * na
     achieved = bytes_per / launch_s / 1e9
     metric = "GRIP refinement rounds/s at the finest layer, RGG N=4,000,000, k=10"
     if dim != 2:
@@ -439,12 +445,16 @@ def bench_grip(args):
                         "traffic": None,
                         "kernel": "k_refine (layer 0)" if dim == 2 else
                                   f"k_refine_nd<{dim}> (layer 0)",
-                        "note": "algorithmic bytes per launch = per active vertex its "
-                                "position, CSR row, kNN list and per-vertex state read and "
-                                "written once; neighbour position gathers not counted",
+                        "note": "algorithmic bytes per launch = N_active (6k + 16 dim + 8) "
+                                "+ 4 per adjacency entry + 4 (N + 1): kNN slots of a 4-byte "
+                                "id and 2-byte distance, FP32 position read and written, "
+                                "displacement and previous displacement, heat and previous "
+                                "cosine, CSR; neighbour position gathers not counted",
                         "launch_ms": round(launch_s * 1e3, 4),
                         "round_ms": round(r_ms / 5, 4),
-                        "bytes_per_launch": int(bytes_per)}
+                        "bytes_per_launch": int(bytes_per),
+                        "engine_bytes_per_launch": int(bytes_engine),
+                        "engine_frac": round(bytes_engine / launch_s / 1e9 / 8000.0, 5)}
     cpu = None
     if dim == 2 and not args.no_cpu_baseline and os.path.exists(HARNESS):
         threads = min(16, len(os.sched_getaffinity(0)))
