@@ -588,7 +588,7 @@ __global__ __launch_bounds__(kB) void k_refine(
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
        i += (int64_t)gridDim.x * kB) {
-    const int32_t v = filt[i];
+    const int32_t v = filt ? filt[i] : (int32_t)i;
     const double2 d0 = disp[v];
     oldDisp[v] = d0;
     const double2 pv = pos[v];
@@ -730,7 +730,7 @@ __global__ __launch_bounds__(kB) void k_rotation_sums(
   const double mx = r->meanX, my = r->meanY, cx = r->comX, cy = r->comY;
   for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
        i += (int64_t)gridDim.x * kB) {
-    const int32_t v = filt[i];
+    const int32_t v = filt ? filt[i] : (int32_t)i;
     double2 d = disp[v];
     d.x += -1.0 * mx;
     d.y += -1.0 * my;
@@ -780,7 +780,7 @@ __global__ __launch_bounds__(kB) void k_round_apply(
   const double w = r->omega, cx = r->comX, cy = r->comY;
   for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
        i += (int64_t)gridDim.x * kB) {
-    const int32_t v = filt[i];
+    const int32_t v = filt ? filt[i] : (int32_t)i;
     double2 d = disp[v];
     double2 p = pos[v];
     if (rot) {
@@ -821,6 +821,182 @@ __global__ void k_round_stats(const double *__restrict__ partial, int blocks,
     r->sumDisp = v[1];
     r->sumHeat = v[2];
     r->sumForce = v[3];
+  }
+}
+
+// ------------------------------------------------ compact 2-D rounds
+// A layer's active vertices renumbered 0..active-1 in the Morton order of
+// their positions (built at the layer's first round). Vertex ids follow the
+// order the graph came in, which for most inputs is unrelated to where the
+// vertices lie, so the neighbour gathers of one wavefront touch a cache line
+// each; in layout order they share lines. The round kernels run unchanged on
+// the compact copy (filt == nullptr: vertex i is compact index i).
+
+// Block partials of the bounding box of the active vertices' positions:
+// (-min x, -min y, max x, max y), all reduced as maxima.
+__global__ __launch_bounds__(kB) void k_active_bbox(const int32_t *__restrict__ filt,
+                                                    int64_t active,
+                                                    const double2 *__restrict__ pos,
+                                                    double *__restrict__ partial) {
+  __shared__ double sm[4 * 16];
+  double acc[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x; i < active;
+       i += (int64_t)gridDim.x * kB) {
+    const double2 p = pos[filt[i]];
+    acc[0] = fmax(acc[0], -p.x);
+    acc[1] = fmax(acc[1], -p.y);
+    acc[2] = fmax(acc[2], p.x);
+    acc[3] = fmax(acc[3], p.y);
+  }
+  const bool isMax[4] = {true, true, true, true};
+  block_reduce<4>(acc, isMax, sm);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++)
+      partial[4 * blockIdx.x + k] = acc[k];
+}
+
+__global__ void k_bbox_finish(const double *__restrict__ partial, int blocks,
+                              double *__restrict__ box) {
+  __shared__ double sm[4 * 16];
+  double v[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+    for (int k = 0; k < 4; k++)
+      v[k] = fmax(v[k], partial[4 * b + k]);
+  const bool isMax[4] = {true, true, true, true};
+  block_reduce<4>(v, isMax, sm);
+  if (threadIdx.x == 0) {
+    box[0] = -v[0];
+    box[1] = -v[1];
+    box[2] = v[2];
+    box[3] = v[3];
+  }
+}
+
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+  x &= 0xFFFFu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+// 32-bit Morton key of each active vertex's position in the box (16 bits per
+// axis); value = its index in filt.
+__global__ __launch_bounds__(kB) void k_layout_keys(const int32_t *__restrict__ filt,
+                                                    int64_t active,
+                                                    const double2 *__restrict__ pos,
+                                                    const double *__restrict__ box,
+                                                    uint64_t *__restrict__ keys,
+                                                    uint32_t *__restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= active)
+    return;
+  const double2 p = pos[filt[i]];
+  const double wx = box[2] - box[0], wy = box[3] - box[1];
+  const double fx = wx > 0.0 ? (p.x - box[0]) / wx : 0.0;
+  const double fy = wy > 0.0 ? (p.y - box[1]) / wy : 0.0;
+  const uint32_t qx = (uint32_t)fmin(fmax(fx * 65535.0, 0.0), 65535.0);
+  const uint32_t qy = (uint32_t)fmin(fmax(fy * 65535.0, 0.0), 65535.0);
+  keys[i] = (uint64_t)(spread16(qx) | (spread16(qy) << 1));
+  vals[i] = (uint32_t)i;
+}
+
+// Compact copies of the per-vertex round state, in sorted key order.
+__global__ __launch_bounds__(kB) void k_compact_gather(
+    const uint32_t *__restrict__ sortedVals, const int32_t *__restrict__ filt,
+    int64_t active, int32_t *__restrict__ cOrder, int32_t *__restrict__ cMap,
+    const double2 *__restrict__ pos, const double2 *__restrict__ disp,
+    const double2 *__restrict__ oldDisp, const double *__restrict__ heat,
+    const double *__restrict__ oldCos, const double *__restrict__ lastF,
+    const uint8_t *__restrict__ dispCalc, double2 *__restrict__ cPos,
+    double2 *__restrict__ cDisp, double2 *__restrict__ cOldDisp,
+    double *__restrict__ cHeat, double *__restrict__ cOldCos, double *__restrict__ cLastF,
+    uint8_t *__restrict__ cDispCalc) {
+  const int64_t c = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (c >= active)
+    return;
+  const int32_t v = filt[sortedVals[c]];
+  cOrder[c] = v;
+  cMap[v] = (int32_t)c;
+  cPos[c] = pos[v];
+  cDisp[c] = disp[v];
+  cOldDisp[c] = oldDisp[v];
+  cHeat[c] = heat[v];
+  cOldCos[c] = oldCos[v];
+  cLastF[c] = lastF[v];
+  cDispCalc[c] = dispCalc[v];
+}
+
+// The compact state written back to the id-ordered arrays.
+__global__ __launch_bounds__(kB) void k_compact_scatter(
+    const int32_t *__restrict__ cOrder, int64_t active, const double2 *__restrict__ cPos,
+    const double2 *__restrict__ cDisp, const double2 *__restrict__ cOldDisp,
+    const double *__restrict__ cHeat, const double *__restrict__ cOldCos,
+    const double *__restrict__ cLastF, const uint8_t *__restrict__ cDispCalc,
+    double2 *__restrict__ pos, double2 *__restrict__ disp, double2 *__restrict__ oldDisp,
+    double *__restrict__ heat, double *__restrict__ oldCos, double *__restrict__ lastF,
+    uint8_t *__restrict__ dispCalc) {
+  const int64_t c = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (c >= active)
+    return;
+  const int32_t v = cOrder[c];
+  pos[v] = cPos[c];
+  disp[v] = cDisp[c];
+  oldDisp[v] = cOldDisp[c];
+  heat[v] = cHeat[c];
+  oldCos[v] = cOldCos[c];
+  lastF[v] = cLastF[c];
+  dispCalc[v] = cDispCalc[c];
+}
+
+// kNN lists in compact indices. A target outside the layer (never the case
+// for lists built over the layer's visible vertices) sets *bad and is
+// replaced by the vertex itself so every later access stays in bounds.
+__global__ __launch_bounds__(kB) void k_compact_knn(
+    const int32_t *__restrict__ cOrder, int64_t active, const int32_t *__restrict__ cMap,
+    int64_t n, const int32_t *__restrict__ knnV, const int32_t *__restrict__ knnD,
+    const int32_t *__restrict__ knnCount, int64_t kcap, int32_t *__restrict__ cKnnV,
+    int32_t *__restrict__ cKnnD, int32_t *__restrict__ cKnnCount,
+    int32_t *__restrict__ bad) {
+  const int64_t c = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (c >= active)
+    return;
+  const int32_t v = cOrder[c];
+  int32_t cnt = knnCount[v];
+  cnt = cnt < 0 ? 0 : (cnt > kcap ? (int32_t)kcap : cnt);
+  cKnnCount[c] = cnt;
+  for (int32_t j = 0; j < cnt; j++) {
+    const int32_t u = knnV[(int64_t)v * kcap + j];
+    int32_t m = (u >= 0 && u < n) ? cMap[u] : -1;
+    if (m < 0 || m >= active) {
+      *bad = 1;
+      m = (int32_t)c;
+    }
+    cKnnV[c * kcap + j] = m;
+    cKnnD[c * kcap + j] = knnD[(int64_t)v * kcap + j];
+  }
+}
+
+// Layer-0 adjacency rows in compact order (row starts cOff from the host),
+// neighbours in compact indices; a neighbour outside the layer sets *bad.
+__global__ __launch_bounds__(kB) void k_compact_adj(
+    const int32_t *__restrict__ cOrder, int64_t active, const int32_t *__restrict__ cMap,
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+    const int32_t *__restrict__ cOff, int32_t *__restrict__ cAdj,
+    int32_t *__restrict__ bad) {
+  const int64_t c = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (c >= active)
+    return;
+  const int32_t v = cOrder[c];
+  const int32_t b = off[v], e = off[v + 1], o = cOff[c];
+  for (int32_t k = b; k < e; k++) {
+    int32_t m = cMap[adj[k]];
+    if (m < 0 || m >= active) {
+      *bad = 1;
+      m = (int32_t)c;
+    }
+    cAdj[o + (k - b)] = m;
   }
 }
 
@@ -1533,9 +1709,23 @@ struct gvizhip_grip {
   int64_t slots = 0;
   int64_t filtCount = 0;
   int64_t blockQueries = -1; // see kBlockQueries
+  // compact 2-D rounds (k_compact_*): the active vertices of the current
+  // layer in layout order, their round state, kNN lists and (layer 0)
+  // adjacency in compact indices
+  DevBuf<int32_t> cOrder, cMap, cOff, cAdj, cKnnV, cKnnD, cKnnCount;
+  DevBuf<double2> cPos, cDisp, cOldDisp;
+  DevBuf<double> cHeat, cOldCos, cLastF, cBox;
+  DevBuf<uint8_t> cDispCalc;
+  int64_t cActive = -1;  // layer size the compact copy holds (-1: none)
+  bool cAdjReady = false; // cOff / cAdj built for cActive
+  bool cAhead = false;    // the compact state is newer than the id-ordered one
+  int64_t compactMin = -1; // see kCompactMin
 };
 
 namespace {
+
+// the compact round state (defined with the rounds below)
+int compact_sync(gvizhip_grip *g, bool keep);
 
 // calls with fewer searches than this give each search a block (the
 // environment variable GVIZHIP_GRIP_BLOCK_QUERIES overrides it per engine;
@@ -1962,6 +2152,8 @@ int gvizhip_grip_set_filtration(gvizhip_grip *g, const int64_t *filtration,
     set_error("gvizhip_grip_set_filtration: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   std::vector<int32_t> f32(count > 0 ? count : 1);
   for (int64_t i = 0; i < count; i++)
     f32[i] = (int32_t)filtration[i];
@@ -1980,6 +2172,8 @@ int64_t gvizhip_grip_mis_filtration(gvizhip_grip *g, int64_t dim,
     set_error("gvizhip_grip_mis_filtration: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   std::vector<int64_t> bord;
   std::vector<int64_t> filt(g->activeCount);
   bord.push_back(g->activeCount);
@@ -2040,6 +2234,8 @@ int64_t gvizhip_grip_mis_first(gvizhip_grip *g, uint8_t *mis, int64_t *tail) {
     set_error("gvizhip_grip_mis_first: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   std::vector<int64_t> cur, t;
   if (first_layer(g, cur, t))
     return -1;
@@ -2057,6 +2253,8 @@ int64_t gvizhip_grip_mis_layer(gvizhip_grip *g, int64_t layer, const int64_t *ca
     set_error("gvizhip_grip_mis_layer: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   std::vector<int64_t> cur(cand, cand + count);
   for (int64_t j = 0; j < count; j++)
     if (cur[j] < 0 || cur[j] >= g->n || (j > 0 && cur[j] <= cur[j - 1])) {
@@ -2082,6 +2280,8 @@ int gvizhip_grip_migrate(gvizhip_grip *g, int64_t *filtration, int64_t filtratio
     set_error("gvizhip_grip_migrate: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   std::vector<int64_t> filt(filtration, filtration + filtrationCount);
   std::vector<int64_t> bord(borders, borders + layerCount);
   for (int64_t k = 0; k < layerCount; k++)
@@ -2112,6 +2312,8 @@ int gvizhip_grip_knn(gvizhip_grip *g, const uint8_t *visible, const int64_t *sou
     set_error("gvizhip_grip_knn: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, true))
+    return -1;
   if (count == 0)
     return 0;
   for (int64_t q = 0; q < count; q++)
@@ -2168,6 +2370,8 @@ int gvizhip_grip_knn_batch(gvizhip_grip *g, const uint8_t *visible,
     set_error("gvizhip_grip_knn_batch: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, true))
+    return -1;
   if (count == 0)
     return 0;
   std::vector<int32_t> t32(count);
@@ -2204,6 +2408,8 @@ int gvizhip_grip_set_dimension(gvizhip_grip *g, int64_t dim) {
     set_error("gvizhip_grip_set_dimension: dimension must be 2, 3 or 4");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   g->dim = dim;
   if (dim == 2)
     return 0;
@@ -2228,6 +2434,8 @@ int gvizhip_grip_upload_nd(gvizhip_grip *g, const double *positions,
     set_error("gvizhip_grip_upload_nd: null context");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   // The copies below run on the null stream, which does not order against
   // the engine's non-blocking stream: let queued kernels finish first.
   GVH_CHECK(hipStreamSynchronize(g->stream));
@@ -2278,6 +2486,8 @@ int gvizhip_grip_download_nd(gvizhip_grip *g, double *positions,
     set_error("gvizhip_grip_download_nd: null context");
     return -1;
   }
+  if (compact_sync(g, true))
+    return -1;
   const int64_t n = g->n, D = g->dim;
   if (D == 2) {
     std::vector<gvizhip_grip_vertex> d2(dec ? n : 0);
@@ -2337,6 +2547,8 @@ int gvizhip_grip_upload(gvizhip_grip *g, const double *positions,
     set_error("gvizhip_grip_upload: null context");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   // Null-stream copies: wait for kernels queued on the engine's stream.
   GVH_CHECK(hipStreamSynchronize(g->stream));
   const int64_t n = g->n;
@@ -2397,6 +2609,8 @@ int gvizhip_grip_download(gvizhip_grip *g, double *positions, gvizhip_grip_verte
     set_error("gvizhip_grip_download: null context");
     return -1;
   }
+  if (compact_sync(g, true))
+    return -1;
   const int64_t n = g->n;
   GVH_CHECK(hipStreamSynchronize(g->stream));
   if (g->dim != 2 && (positions || dec)) {
@@ -2456,6 +2670,8 @@ int gvizhip_grip_clear_decorators(gvizhip_grip *g, int64_t active) {
     set_error("gvizhip_grip_clear_decorators: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   if (g->dim == 3)
     hipLaunchKernelGGL(k_clear_nd<3>, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
                        g->filt.p, active, g->n, g->dispN.p, g->oldDispN.p, g->heat.p,
@@ -2479,6 +2695,8 @@ int gvizhip_grip_update_knn(gvizhip_grip *g, int64_t active, const uint8_t *visi
     set_error("gvizhip_grip_update_knn: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   if (active == 0 || k == 0)
     return 0;
   if (upload_visible(g, visible))
@@ -2501,6 +2719,8 @@ int gvizhip_grip_place(gvizhip_grip *g, int64_t begin, int64_t end,
     set_error("gvizhip_grip_place: invalid arguments");
     return -1;
   }
+  if (compact_sync(g, false))
+    return -1;
   const int64_t count = end - begin;
   if (count == 0)
     return 0;
@@ -2616,6 +2836,124 @@ int enqueue_round_nd(gvizhip_grip *g, int64_t active, int64_t layer, hipEvent_t 
   return 0;
 }
 
+constexpr int kCompactFlag = 4; // flags slot of the compact builds
+// layers with at least this many active vertices run their rounds compact
+// (smaller ones fit the caches in any order); the environment variable
+// GVIZHIP_GRIP_COMPACT_MIN overrides it per engine (0: every layer)
+constexpr int64_t kCompactMin = 32768;
+
+int64_t compact_min(gvizhip_grip *g) {
+  if (g->compactMin < 0) {
+    const char *e = getenv("GVIZHIP_GRIP_COMPACT_MIN");
+    g->compactMin = e ? std::max<int64_t>(0, atoll(e)) : kCompactMin;
+  }
+  return g->compactMin;
+}
+
+// Writes the compact state back to the id-ordered arrays when it is newer;
+// `keep` leaves the compact copy in use (a read of the state), otherwise the
+// next round rebuilds it (the state, the kNN lists or the layers change).
+int compact_sync(gvizhip_grip *g, bool keep) {
+  if (g->cAhead && g->cActive > 0) {
+    hipLaunchKernelGGL(k_compact_scatter, dim3(grid_for(g->cActive, kB)), dim3(kB), 0,
+                       g->stream, g->cOrder.p, g->cActive, g->cPos.p, g->cDisp.p,
+                       g->cOldDisp.p, g->cHeat.p, g->cOldCos.p, g->cLastF.p,
+                       g->cDispCalc.p, g->pos.p, g->disp.p, g->oldDisp.p, g->heat.p,
+                       g->oldCos.p, g->lastF.p, g->dispCalc.p);
+    GVH_CHECK_LAUNCH();
+  }
+  g->cAhead = false;
+  if (!keep) {
+    g->cActive = -1;
+    g->cAdjReady = false;
+  }
+  return 0;
+}
+
+// Builds the compact copy of layer `active` (layer 0 also needs the
+// adjacency). Returns 1 when the layer cannot run compact (a kNN target or
+// neighbour outside it), 0 when built, -1 on error.
+int ensure_compact(gvizhip_grip *g, int64_t active, int64_t layer) {
+  hipStream_t st = g->stream;
+  const int64_t n = g->n;
+  if (g->cActive != active) {
+    if (compact_sync(g, false))
+      return -1;
+    if (g->cOrder.alloc(active) || g->cMap.alloc(n) || g->cPos.alloc(active) ||
+        g->cDisp.alloc(active) || g->cOldDisp.alloc(active) || g->cHeat.alloc(active) ||
+        g->cOldCos.alloc(active) || g->cLastF.alloc(active) ||
+        g->cDispCalc.alloc(active) || g->cKnnV.alloc(active * g->kcap) ||
+        g->cKnnD.alloc(active * g->kcap) || g->cKnnCount.alloc(active) ||
+        g->cBox.alloc(4) || g->keys.alloc(active) || g->vals.alloc(active))
+      return -1;
+    const unsigned bb = std::min<unsigned>(grid_for(active, kB), 1024);
+    hipLaunchKernelGGL(k_active_bbox, dim3(bb), dim3(kB), 0, st, g->filt.p, active,
+                       g->pos.p, g->partial.p);
+    hipLaunchKernelGGL(k_bbox_finish, dim3(1), dim3(256), 0, st, g->partial.p, (int)bb,
+                       g->cBox.p);
+    hipLaunchKernelGGL(k_layout_keys, dim3(grid_for(active, kB)), dim3(kB), 0, st,
+                       g->filt.p, active, g->pos.p, g->cBox.p, g->keys.p, g->vals.p);
+    GVH_CHECK_LAUNCH();
+    if (g->sorter.sort(g->keys.p, g->vals.p, active, 0, 32, st))
+      return -1;
+    GVH_CHECK(hipMemsetAsync(g->cMap.p, 0xFF, sizeof(int32_t) * n, st));
+    hipLaunchKernelGGL(k_compact_gather, dim3(grid_for(active, kB)), dim3(kB), 0, st,
+                       g->vals.p, g->filt.p, active, g->cOrder.p, g->cMap.p, g->pos.p,
+                       g->disp.p, g->oldDisp.p, g->heat.p, g->oldCos.p, g->lastF.p,
+                       g->dispCalc.p, g->cPos.p, g->cDisp.p, g->cOldDisp.p, g->cHeat.p,
+                       g->cOldCos.p, g->cLastF.p, g->cDispCalc.p);
+    if (set_flag(g, kCompactFlag, 0))
+      return -1;
+    hipLaunchKernelGGL(k_compact_knn, dim3(grid_for(active, kB)), dim3(kB), 0, st,
+                       g->cOrder.p, active, g->cMap.p, n, g->knnV.p, g->knnD.p,
+                       g->knnCount.p, g->kcap, g->cKnnV.p, g->cKnnD.p, g->cKnnCount.p,
+                       g->flags.p + kCompactFlag);
+    GVH_CHECK_LAUNCH();
+    g->cActive = active;
+    g->cAdjReady = false;
+    g->cAhead = false;
+    int32_t bad = 0;
+    if (read_flag(g, kCompactFlag, &bad))
+      return -1;
+    if (bad) {
+      g->cActive = -1;
+      return 1;
+    }
+  }
+  if (layer == 0 && !g->cAdjReady) {
+    // row starts on the host (it keeps the offsets), rows on the device
+    std::vector<int32_t> order(active), cOff(active + 1);
+    GVH_CHECK(hipMemcpyAsync(order.data(), g->cOrder.p, sizeof(int32_t) * active,
+                             hipMemcpyDeviceToHost, st));
+    GVH_CHECK(hipStreamSynchronize(st));
+    int64_t acc = 0;
+    for (int64_t c = 0; c < active; c++) {
+      cOff[c] = (int32_t)acc;
+      acc += g->hoff[order[c] + 1] - g->hoff[order[c]];
+    }
+    cOff[active] = (int32_t)acc;
+    if (g->cOff.alloc(active + 1) || g->cAdj.alloc(acc > 0 ? acc : 1))
+      return -1;
+    GVH_CHECK(hipMemcpyAsync(g->cOff.p, cOff.data(), sizeof(int32_t) * (active + 1),
+                             hipMemcpyHostToDevice, st));
+    if (set_flag(g, kCompactFlag, 0))
+      return -1;
+    hipLaunchKernelGGL(k_compact_adj, dim3(grid_for(active, kB)), dim3(kB), 0, st,
+                       g->cOrder.p, active, g->cMap.p, g->off.p, g->adj.p, g->cOff.p,
+                       g->cAdj.p, g->flags.p + kCompactFlag);
+    GVH_CHECK_LAUNCH();
+    int32_t bad = 0;
+    if (read_flag(g, kCompactFlag, &bad))
+      return -1;
+    if (bad) {
+      g->cActive = -1;
+      return 1;
+    }
+    g->cAdjReady = true;
+  }
+  return 0;
+}
+
 int enqueue_round(gvizhip_grip *g, int64_t active, int64_t layer, unsigned *blocksOut,
                   hipEvent_t f0, hipEvent_t f1) {
   hipStream_t st = g->stream;
@@ -2628,25 +2966,47 @@ int enqueue_round(gvizhip_grip *g, int64_t active, int64_t layer, unsigned *bloc
                        : enqueue_round_nd<4>(g, active, layer, f0, f1);
   }
   const unsigned blocks = std::min<unsigned>(grid_for(active, kB), 1024);
+  // the compact copy of the layer when it applies, else the id-ordered state
+  int cm = 1;
+  if (active >= compact_min(g))
+    cm = ensure_compact(g, active, layer);
+  else if (compact_sync(g, false))
+    return -1;
+  if (cm < 0)
+    return -1;
+  const bool compact = cm == 0;
+  const int32_t *ord = compact ? nullptr : g->order.p;
+  const int32_t *off = compact ? g->cOff.p : g->off.p;
+  const int32_t *adj = compact ? g->cAdj.p : g->adj.p;
+  double2 *pos = compact ? g->cPos.p : g->pos.p;
+  const int32_t *kv = compact ? g->cKnnV.p : g->knnV.p;
+  const int32_t *kd = compact ? g->cKnnD.p : g->knnD.p;
+  const int32_t *kc = compact ? g->cKnnCount.p : g->knnCount.p;
+  double2 *disp = compact ? g->cDisp.p : g->disp.p;
+  double2 *oldDisp = compact ? g->cOldDisp.p : g->oldDisp.p;
+  double *heat = compact ? g->cHeat.p : g->heat.p;
+  double *oldCos = compact ? g->cOldCos.p : g->oldCos.p;
+  double *lastF = compact ? g->cLastF.p : g->lastF.p;
+  uint8_t *dispCalc = compact ? g->cDispCalc.p : g->dispCalc.p;
   if (f0)
     GVH_CHECK(hipEventRecord(f0, st));
   // one thread per vertex for the gathers: every CU's wave slots filled
   const unsigned rblocks = grid_for(active, kB);
-  hipLaunchKernelGGL(k_refine, dim3(rblocks), dim3(kB), 0, st, g->order.p, active,
-                     layer, g->off.p, g->adj.p, g->pos.p, g->knnV.p, g->knnD.p,
-                     g->knnCount.p, g->kcap, g->disp.p, g->oldDisp.p, g->heat.p,
-                     g->oldCos.p, g->lastF.p, g->dispCalc.p, g->partial.p);
+  hipLaunchKernelGGL(k_refine, dim3(rblocks), dim3(kB), 0, st, ord, active, layer, off,
+                     adj, pos, kv, kd, kc, g->kcap, disp, oldDisp, heat, oldCos, lastF,
+                     dispCalc, g->partial.p);
   if (f1)
     GVH_CHECK(hipEventRecord(f1, st));
   hipLaunchKernelGGL(k_round_means, dim3(1), dim3(256), 0, st, g->partial.p,
                      (int)rblocks, active, g->round.p);
-  hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, g->order.p,
-                     active, g->pos.p, g->disp.p, g->round.p, g->partial.p);
+  hipLaunchKernelGGL(k_rotation_sums, dim3(blocks), dim3(kB), 0, st, ord, active, pos,
+                     disp, g->round.p, g->partial.p);
   hipLaunchKernelGGL(k_round_omega, dim3(1), dim3(256), 0, st, g->partial.p,
                      (int)blocks, active, g->round.p);
-  hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, g->order.p,
-                     active, g->pos.p, g->disp.p, g->heat.p, g->lastF.p,
-                     g->round.p, g->partial.p);
+  hipLaunchKernelGGL(k_round_apply, dim3(blocks), dim3(kB), 0, st, ord, active, pos, disp,
+                     heat, lastF, g->round.p, g->partial.p);
+  if (compact)
+    g->cAhead = true;
   hipLaunchKernelGGL(k_round_stats, dim3(1), dim3(256), 0, st, g->partial.p,
                      (int)blocks, active, g->round.p);
   GVH_CHECK_LAUNCH();

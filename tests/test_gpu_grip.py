@@ -372,3 +372,97 @@ def test_refine_round_nd_matches_oracle_on_larger_graph(gpu, dim, layer):
     assert d <= 1e-11 * np.abs(ref_dec["disp"][ids]).max()
     np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
                                [ref.maxDisp, ref.meanDisp, ref.meanForce], rtol=1e-11)
+
+
+def _random_round_state(n, kcap, active, filt, rng, targets_in_layer):
+    pos = rng.uniform(-100.0, 100.0, (n, 2))
+    pool = filt[:active] if targets_in_layer else np.arange(n)
+    kv = rng.choice(pool, (n, kcap)).astype(np.int64)
+    kd = rng.integers(1, 6, (n, kcap)).astype(np.int64)
+    from grapher_amd import native
+    dec = np.zeros(n, dtype=native.GRIP_VERTEX_DTYPE)
+    dec["dispX"], dec["dispY"] = rng.normal(0.0, 2.0, n), rng.normal(0.0, 2.0, n)
+    dec["oldDispX"], dec["oldDispY"] = rng.normal(0.0, 2.0, n), rng.normal(0.0, 2.0, n)
+    dec["heat"] = rng.uniform(0.1, 5.0, n)
+    dec["oldCos"] = rng.uniform(-1.0, 1.0, n)
+    dec["knnCount"] = kcap
+    dec["dispCalculated"] = rng.integers(0, 2, n)
+    return pos, dec, kv, kd
+
+
+@pytest.mark.parametrize("targets_in_layer", [True, False])
+@pytest.mark.parametrize("layer", [0, 1])
+def test_refine_round_2d_compact_matches_oracle_on_larger_graph(gpu, layer,
+                                                                targets_in_layer):
+    """Layers of >= 32,768 active vertices run their rounds on a compact copy
+    in the Morton order of the positions (k_compact_*). One round on a
+    40,000-vertex graph from a random state against the oracle's serial
+    round; kNN targets inside the layer (the compact path) or anywhere in
+    the graph (a target outside the layer: the id-ordered path)."""
+    n, kcap = 40000, 8
+    _, e, _ = G.random_geometric(n, 8.0, 17)
+    off, adj = G.csr_from_edges(n, e)
+    rng = np.random.default_rng(7 + layer + 2 * targets_in_layer)
+    filt = rng.permutation(n).astype(np.int64)
+    active = n - 4000 * layer
+    pos, dec, kv, kd = _random_round_state(n, kcap, active, filt, rng, targets_in_layer)
+    eng = gpu.GripEngine(off, adj, kcap=kcap)
+    eng.set_filtration(filt)
+    eng.upload(pos, dec, kv, kd)
+    st = eng.refine_round(active, layer)
+    got_pos, got_dec, _, _ = eng.download()
+    odec = np.zeros(n, dtype=O.GRIP_VERTEX_DTYPE)
+    for k in ("dispX", "dispY", "oldDispX", "oldDispY", "heat", "oldCos",
+              "lastForceMag", "knnCount"):
+        odec[k] = dec[k]
+    odec["dispCalculated"] = dec["dispCalculated"].astype(np.uint8)
+    ref_pos = np.ascontiguousarray(pos.copy())
+    ref = O.grip_refine_round(O.Graph(off, adj), filt, active, layer,
+                              np.stack([kv, kd], axis=2), kcap, odec, ref_pos)
+    ids = filt[:active]
+    assert np.abs(got_pos - ref_pos).max() <= 1e-12 * np.abs(ref_pos).max()
+    for k in ("heat", "oldCos", "lastForceMag"):
+        np.testing.assert_allclose(got_dec[k][ids], odec[k][ids], rtol=1e-12, err_msg=k)
+    for k in ("dispX", "dispY", "oldDispX", "oldDispY"):
+        d = np.abs(got_dec[k][ids] - odec[k][ids]).max()
+        assert d <= 1e-11 * np.abs(odec[k][ids]).max(), k
+    assert np.array_equal(got_dec["dispCalculated"][ids], odec["dispCalculated"][ids])
+    np.testing.assert_allclose([st.maxDisp, st.meanDisp, st.meanForce],
+                               [ref.maxDisp, ref.meanDisp, ref.meanForce], rtol=1e-11)
+
+
+def test_compact_rounds_unchanged_by_reads_in_between(gpu):
+    """Downloads between compact rounds write the compact state back without
+    dropping it: four rounds with a download after each equal four rounds
+    without, bit for bit; an upload in between starts a fresh compact copy
+    from the uploaded state."""
+    n, kcap = 40000, 8
+    _, e, _ = G.random_geometric(n, 8.0, 19)
+    off, adj = G.csr_from_edges(n, e)
+    rng = np.random.default_rng(3)
+    filt = rng.permutation(n).astype(np.int64)
+    pos, dec, kv, kd = _random_round_state(n, kcap, n, filt, rng, True)
+    runs = []
+    for reads in (False, True):
+        eng = gpu.GripEngine(off, adj, kcap=kcap)
+        eng.set_filtration(filt)
+        eng.upload(pos, dec, kv, kd)
+        for _ in range(4):
+            eng.refine_round(n, 0)
+            if reads:
+                eng.download()
+        runs.append(eng.download())
+        if reads:
+            p2, d2, kv2, kd2 = runs[-1]
+            eng.upload(p2, d2, kv2, kd2)
+            eng.refine_round(n, 0)
+            after_upload = eng.download()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    eng = gpu.GripEngine(off, adj, kcap=kcap)
+    eng.set_filtration(filt)
+    eng.upload(*runs[0])
+    eng.refine_round(n, 0)
+    fresh = eng.download()
+    assert np.array_equal(after_upload[0], fresh[0])
+    assert np.array_equal(after_upload[1], fresh[1])
