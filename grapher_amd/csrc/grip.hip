@@ -460,6 +460,101 @@ __global__ __launch_bounds__(kB) void k_bfs_queries(
   }
 }
 
+// ---- multi-source ball search for MIS layers with few candidates (the
+// coarse layers, whose balls of radius 2^(i-1) cover much of the graph):
+// up to 64 candidates at once, one bit each per vertex, expanded level by
+// level with every CU taking part, where one search per block leaves most
+// of the chip idle. `seen` holds the sources whose ball holds the vertex,
+// `front` the bits that reached it at the current level.
+
+// Level d: every frontier vertex offers its new bits to its neighbours; the
+// bits a neighbour did not hold yet form its frontier at level d + 1. The
+// frontier entry is cleared once read. act[d] = 1 when any bit moved.
+__global__ __launch_bounds__(kB) void k_ms_push(const int32_t *__restrict__ off,
+                                                const int32_t *__restrict__ adj, int64_t n,
+                                                unsigned long long *__restrict__ seen,
+                                                unsigned long long *__restrict__ front,
+                                                unsigned long long *__restrict__ next,
+                                                int32_t *__restrict__ act) {
+  const int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (u >= n)
+    return;
+  const unsigned long long f = front[u];
+  if (!f)
+    return;
+  front[u] = 0ull;
+  bool moved = false;
+  for (int32_t e = off[u]; e < off[u + 1]; e++) {
+    const int32_t w = adj[e];
+    const unsigned long long nb = f & ~seen[w];
+    if (!nb)
+      continue;
+    const unsigned long long fresh = nb & ~atomicOr(&seen[w], nb);
+    if (fresh) {
+      atomicOr(&next[w], fresh);
+      moved = true;
+    }
+  }
+  if (moved)
+    *act = 1;
+}
+
+// Conflict list of source j of the batch (candidate index s0 + j; the
+// candidates are ascending, so its lower-id candidates are cand[0 .. s0+j)):
+// those whose ball bit j is set, in ascending id, one block per source.
+// count = min(found, stride); found > stride sets *overflow.
+__global__ __launch_bounds__(kB) void k_ms_lists(const int32_t *__restrict__ cand,
+                                                 int64_t s0, int cnt,
+                                                 const unsigned long long *__restrict__ seen,
+                                                 int64_t stride, int32_t *__restrict__ outV,
+                                                 int32_t *__restrict__ outD,
+                                                 int32_t *__restrict__ outCount,
+                                                 int32_t *__restrict__ overflow) {
+  __shared__ int sWave[kWpb];
+  __shared__ int sBase;
+  const int j = blockIdx.x;
+  if (j >= cnt)
+    return;
+  const int64_t qi = s0 + j;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (threadIdx.x == 0)
+    sBase = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < qi; c0 += kB) {
+    const int64_t c = c0 + threadIdx.x;
+    bool hit = false;
+    int32_t v = 0;
+    if (c < qi) {
+      v = cand[c];
+      hit = (seen[v] >> j) & 1ull;
+    }
+    const uint64_t b = __ballot(hit);
+    if (lane == 0)
+      sWave[wave] = __popcll(b);
+    __syncthreads();
+    int before = sBase;
+    for (int w = 0; w < wave; w++)
+      before += sWave[w];
+    if (hit) {
+      const int at = before + (int)__popcll(b & (lane ? (~0ull >> (kWave - lane)) : 0ull));
+      if (at < stride) {
+        outV[qi * stride + at] = v;
+        outD[qi * stride + at] = 0;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 0; w < kWpb; w++)
+        sBase += sWave[w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    outCount[qi] = sBase < stride ? sBase : (int32_t)stride;
+    if (sBase > stride)
+      atomicOr(overflow, 1);
+  }
+}
+
 // ------------------------------------------------------- MIS decisions
 // Lexicographically-first MIS over conflict lists of lower-id neighbours:
 // IN when every lower conflict is OUT, OUT as soon as one is IN.
@@ -1939,7 +2034,64 @@ struct LayerScratch {
   DevBuf<int32_t> candDev, lists, listsD, listCount;
   DevBuf<uint8_t> candFlag, cstate;
   int64_t stride = 32;
+  // multi-source ball search (layers with at most kMultiSources candidates)
+  DevBuf<unsigned long long> msSeen, msFront, msNext;
+  DevBuf<int32_t> msAct;
 };
+
+// Layers with at most this many candidates take the multi-source search
+// (one pass over the chip per level for all of them); larger ones give each
+// candidate its own search.
+constexpr int64_t kMultiSources = 64;
+
+__global__ void k_ms_seed(const int32_t *__restrict__ cand, int64_t m,
+                          unsigned long long *__restrict__ seen,
+                          unsigned long long *__restrict__ front) {
+  const int j = threadIdx.x;
+  if (j < m) {
+    const int32_t v = cand[j];
+    seen[v] = 1ull << j;
+    front[v] = 1ull << j;
+  }
+}
+
+// The balls of radius `radius` of the m (<= 64) candidates: ls.msSeen holds,
+// per vertex, the candidates whose ball contains it. Levels run in groups
+// of 16; a group whose last level moved no bit ends the search.
+int multi_ball(gvizhip_grip *g, LayerScratch &ls, int64_t m, int64_t radius) {
+  const int64_t n = g->n;
+  hipStream_t st = g->stream;
+  constexpr int kGroup = 16;
+  if (ls.msSeen.alloc(n) || ls.msFront.alloc(n) || ls.msNext.alloc(n) ||
+      ls.msAct.alloc(kGroup))
+    return -1;
+  const size_t bytes = sizeof(unsigned long long) * n;
+  GVH_CHECK(hipMemsetAsync(ls.msSeen.p, 0, bytes, st));
+  GVH_CHECK(hipMemsetAsync(ls.msFront.p, 0, bytes, st));
+  GVH_CHECK(hipMemsetAsync(ls.msNext.p, 0, bytes, st));
+  hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(kWave), 0, st, ls.candDev.p, m, ls.msSeen.p,
+                     ls.msFront.p);
+  GVH_CHECK_LAUNCH();
+  unsigned long long *front = ls.msFront.p, *next = ls.msNext.p;
+  const int64_t levels = std::min<int64_t>(radius, n);
+  for (int64_t d0 = 1; d0 <= levels; d0 += kGroup) {
+    GVH_CHECK(hipMemsetAsync(ls.msAct.p, 0, sizeof(int32_t) * kGroup, st));
+    const int64_t d1 = std::min<int64_t>(d0 + kGroup - 1, levels);
+    for (int64_t d = d0; d <= d1; d++) {
+      hipLaunchKernelGGL(k_ms_push, dim3(grid_for(n, kB)), dim3(kB), 0, st, g->off.p,
+                         g->adj.p, n, ls.msSeen.p, front, next, ls.msAct.p + (d - d0));
+      std::swap(front, next);
+    }
+    GVH_CHECK_LAUNCH();
+    int32_t moved = 0;
+    GVH_CHECK(hipMemcpyAsync(&moved, ls.msAct.p + (d1 - d0), sizeof(int32_t),
+                             hipMemcpyDeviceToHost, st));
+    GVH_CHECK(hipStreamSynchronize(st));
+    if (!moved)
+      break;
+  }
+  return 0;
+}
 
 // Layer i >= 2 (iterMISFiltration, gvizGRIPEmbedder.c:672-739): of the
 // candidates `cur` (ascending), the lexicographically-first set in which no
@@ -1968,13 +2120,23 @@ int next_layer(gvizhip_grip *g, int64_t i, const std::vector<int64_t> &cur,
   // conflict lists: lower-id candidates within the radius (candidate index
   // space: the candidates are ascending, so a lower id is a lower index)
   int64_t &stride = ls.stride;
+  const bool multi = m > 0 && m <= kMultiSources;
+  if (multi && multi_ball(g, ls, m, radius))
+    return -1;
   for (;;) {
     if (ls.lists.alloc(m * stride) || ls.listsD.alloc(m * stride) || set_flag(g, 0, 0))
       return -1;
     const int maxDepth = radius > INT32_MAX ? INT32_MAX : (int)radius;
-    if (run_queries(g, ls.candDev.p, nullptr, m, (int)stride, stride, ls.candFlag.p, 0,
-                    maxDepth, true, false, ls.lists.p, ls.listsD.p, ls.listCount.p))
+    if (multi) {
+      hipLaunchKernelGGL(k_ms_lists, dim3((unsigned)m), dim3(kB), 0, st, ls.candDev.p,
+                         (int64_t)0, (int)m, ls.msSeen.p, stride, ls.lists.p, ls.listsD.p,
+                         ls.listCount.p, g->flags.p);
+      GVH_CHECK_LAUNCH();
+    } else if (run_queries(g, ls.candDev.p, nullptr, m, (int)stride, stride,
+                           ls.candFlag.p, 0, maxDepth, true, false, ls.lists.p,
+                           ls.listsD.p, ls.listCount.p)) {
       return -1;
+    }
     int32_t ov = 0;
     if (read_flag(g, 0, &ov))
       return -1;
