@@ -847,12 +847,12 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
     GVH_CHECK(hipEventCreate(&c.travEnd));
   }
   if (getenv("GVIZHIP_TRAVERSAL_STATS") && !c.travStats.p) {
-    if (c.travStats.alloc(6))
+    if (c.travStats.alloc(8))
       return -1;
     f->graphValid = false;
   }
   if (c.travStats.p)
-    GVH_CHECK(hipMemset(c.travStats.p, 0, 6 * sizeof(unsigned long long)));
+    GVH_CHECK(hipMemset(c.travStats.p, 0, 8 * sizeof(unsigned long long)));
   hipEvent_t s0, s1;
   GVH_CHECK(hipEventCreate(&s0));
   GVH_CHECK(hipEventCreate(&s1));
@@ -899,12 +899,13 @@ extern "C" int gvizhip_force_profile(gvizhip_force *f, int64_t steps,
     return -1;
   }
   if (c.travStats.p) {
-    unsigned long long t[6];
+    unsigned long long t[8];
     if (hipMemcpy(t, c.travStats.p, sizeof(t), hipMemcpyDeviceToHost) == hipSuccess)
       fprintf(stderr,
               "[gvizhip] traversal stats: steps %llu, deferred %llu (mask lanes %llu),"
-              " undecided %llu (mask lanes %llu), max stack %llu\n",
-              t[0], t[1], t[2], t[3], t[4], t[5]);
+              " undecided %llu (mask lanes %llu), max stack %llu; far form %llu"
+              " (%llu with full masks)\n",
+              t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7]);
   }
   if (traversalMs)
     *traversalMs = trav;

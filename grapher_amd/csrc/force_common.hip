@@ -85,6 +85,8 @@ __global__ void k_root(const double *__restrict__ partial, int blocks,
     }
     sc->maxDepth = d;
     sc->rootRef = n >= 2 ? 0 : -1;
+    // every record lies at most d levels deep (a point one more)
+    sc->treeDepth = d + 2;
   }
 }
 
@@ -482,6 +484,8 @@ __global__ __launch_bounds__(kBlock) void k_bbox_root_g(const double2 *__restric
     }
     sc->maxDepth = d;
     sc->rootRef = e - b >= 2 ? 0 : -1;
+    // every record of the graph lies at most d levels deep (a point one more)
+    sc->treeDepth = (d < kBatchLevels ? d : kBatchLevels) + 2;
     if (d > kBatchLevels)
       sc->traversalOverflow = 2;
   }

@@ -57,7 +57,9 @@ struct Scalars {
   int32_t stopped;
   int32_t traversalOverflow; // 1 stack overflow, 2 tree too deep
   double meanDegreePlusOne;
-  double pad[4];
+  int32_t treeDepth; // digit levels + 2: a bound on the traversal's deepest walk
+  int32_t pad32;
+  double pad[3];
 };
 
 constexpr int32_t kNone = INT32_MIN;

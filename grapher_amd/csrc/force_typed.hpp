@@ -404,11 +404,13 @@ constexpr int64_t kTravMaxVertices = int64_t(1) << 29;
 constexpr int kTravStack = GVH_TRAV_STACK;      // pending (node, mask) per wave
 constexpr int kTravDefer = GVH_TRAV_DEFER;      // deferred bodies per wave
 // Stack entries held back from the wide phase. Once the stack holds more than
-// kTravStack - kTravReserve - 3 entries the wavefront takes one entry per
-// step, which is a depth-first walk: each popped cell adds at most 3 net
-// entries per level it descends, and a cell lies at most 64 levels deep, so
-// the walk needs at most 3 * 64 + 3 slots above that point and the stack can
-// never overflow.
+// kTravStack - reserve - 3 entries the wavefront takes one entry per step,
+// which is a depth-first walk: each popped cell adds at most 3 net entries
+// per level it descends, so a tree D levels deep needs at most 3 D + 3 slots
+// above that point and the stack can never overflow. The traversal takes
+// reserve = 3 D + 4 with D the levels the root cell's size allows before the
+// depth limit, plus 2 (Scalars::treeDepth), at most kTravReserve: a cell lies
+// at most 64 levels deep.
 constexpr int kTravReserve = 3 * 64 + 4;
 static_assert(kTravStack - kTravReserve >= kWave, "traversal stack too small");
 
@@ -562,13 +564,37 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
   // margin is far above the rounding of either form
   const double thLo = (double)theta * (1.0 - margin);
   const double thHi = (double)theta * (1.0 + margin);
-  const double thLo2 = thLo * thLo, thHi2 = thHi * thHi;
+  // (in scalar registers: theta reaches the kernel in the force math's type)
+  const double thLo2 = uniform(thLo * thLo), thHi2 = uniform(thHi * thHi);
+  // Largest target radius of the wavefront. A body farther from the targets'
+  // box than (rHi + its radius) / 0.9, with a margin far above the FP32
+  // rounding of the gap, meets every target with a positive gap outside the
+  // near band (kNearGap): its terms take num * inv / gap with no tests.
+  const double rHi = uniform(wave_max(valid ? (double)mr.y : -INFINITY));
+  // interactions per target are counted only when asked for
+  const bool counting = inter != nullptr;
+  // stack entries held back from the wide phase for this step's tree
+  const int32_t depth = __builtin_amdgcn_readfirstlane(sc->treeDepth);
+  const int reserve = 3 * depth + 4 < kTravReserve ? 3 * depth + 4 : kTravReserve;
   Real ax = 0, ay = 0;
   int32_t cnt = 0;
   int nd = 0;
   // optional counters (GVIZHIP_TRAVERSAL_STATS): steps, deferred bodies,
-  // their mask lanes, undecided cells, their mask lanes, max stack depth
-  unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+  // their mask lanes, undecided cells, their mask lanes, max stack depth,
+  // bodies taken in the plain far form, those of them with every lane in
+  // their mask
+  unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // the far test above for a body at (x, y) with radius rb
+  auto farBody = [&](double x, double y, double rb) -> bool {
+    if constexpr (!OVERLAP) {
+      return true;
+    } else {
+      const double gx = fmax(fmax(bx0 - x, x - bx1), 0.0);
+      const double gy = fmax(fmax(by0 - y, y - by1), 0.0);
+      const double sr = rHi + rb;
+      return 0.8099 * (gx * gx + gy * gy) > sr * sr;
+    }
+  };
 
   // The source's numerator as listed in a body (rep_pair); FP64 lists the mass.
   auto numOf = [&](Real m) -> Real {
@@ -597,13 +623,38 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
                                               oc, selfId, (int64_t)b.id, ax, ay, ex, ey,
                                               r64, n);
       }
-      cnt++;
     }
   };
-  // Evaluates the deferred bodies in the order they were deferred, two per
-  // iteration so the second body's LDS loads overlap the first's math. In
-  // FP32 a lane whose sums are no longer finite (a pair with d2 == 0)
-  // replays the list from the sums before it with guarded terms.
+  // A far body (FP32 force math): the gap is positive and outside the near
+  // band for every target, so rep_pair's term is num * inv / gap (num / d^2
+  // without overlap prevention). ALL: every lane holds a target of its mask.
+  auto farTerm = [&](const TravBody<Real, PosT> &b, bool all) {
+    if constexpr (FAST) {
+      if (all || ((b.mask >> lane) & 1ull)) {
+        const PosT ex = b.x - me.x, ey = b.y - me.y;
+        const float dx = (float)ex, dy = (float)ey;
+        const float d2 = dx * dx + dy * dy;
+        float s;
+        if constexpr (OVERLAP) {
+          const float inv = __builtin_amdgcn_rsqf(d2);
+          const float gap = d2 * inv - (mr.y + b.r);
+          s = b.m * __builtin_amdgcn_rcpf(gap) * inv;
+        } else {
+          s = b.m * __builtin_amdgcn_rcpf(d2);
+        }
+        ax -= s * dx;
+        ay -= s * dy;
+      }
+    }
+  };
+
+  // Evaluates the deferred bodies 64 at a time, two per iteration so the
+  // second body's LDS loads overlap the first's math. In FP32 each lane first
+  // classifies one body of the 64 and the list is reordered into general
+  // bodies, far bodies with partial masks and far bodies with full masks, each
+  // run evaluated in its own loop in list order; a lane whose sums are no
+  // longer finite (a pair with d2 == 0) replays the list from the sums before
+  // it with guarded terms.
   auto flush = [&]() {
     if (tstats) {
       st[1] += nd;
@@ -611,25 +662,71 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
         st[2] += __popcll(rl(dBody[wave][t].mask, 0));
     }
     const Real ax0 = ax, ay0 = ay;
-    const int32_t c0 = cnt;
-    int t = 0;
-    for (; t + 2 <= nd; t += 2) {
-      const TravBody<Real, PosT> b0 = dBody[wave][t];
-      const TravBody<Real, PosT> b1 = dBody[wave][t + 1];
-      body(b0, false);
-      body(b1, false);
+    for (int base = 0; base < nd; base += kWave) {
+      const int end = nd < base + kWave ? nd : base + kWave;
+      int g1 = end, f1 = end;
+      if constexpr (FAST) {
+        // lane k classifies body base + k and moves it, in place, to its
+        // class's run: general bodies first, then far ones with partial
+        // masks, then far ones every lane takes
+        const bool on = base + lane < end;
+        const TravBody<Real, PosT> c = dBody[wave][on ? base + lane : base];
+        bool f = false, u = false;
+        if (on) {
+          f = farBody((double)c.x, (double)c.y, (double)c.r);
+          u = f && c.mask == ~0ull;
+        }
+        const uint64_t gm = __ballot(on && !f), pm = __ballot(f && !u), um = __ballot(u);
+        g1 = base + (int)__popcll(gm);
+        f1 = g1 + (int)__popcll(pm);
+        if (tstats) {
+          st[6] += __popcll(pm) + __popcll(um);
+          st[7] += __popcll(um);
+        }
+        if (on)
+          dBody[wave][!f ? base + (int)__popcll(gm & ltMask)
+                         : !u ? g1 + (int)__popcll(pm & ltMask)
+                              : f1 + (int)__popcll(um & ltMask)] = c;
+      }
+      int t = base;
+      for (; t + 2 <= g1; t += 2) {
+        const TravBody<Real, PosT> b0 = dBody[wave][t];
+        const TravBody<Real, PosT> b1 = dBody[wave][t + 1];
+        body(b0, false);
+        body(b1, false);
+      }
+      if (t < g1)
+        body(dBody[wave][t], false);
+      if constexpr (FAST) {
+        for (t = g1; t + 2 <= f1; t += 2) {
+          const TravBody<Real, PosT> b0 = dBody[wave][t];
+          const TravBody<Real, PosT> b1 = dBody[wave][t + 1];
+          farTerm(b0, false);
+          farTerm(b1, false);
+        }
+        if (t < f1)
+          farTerm(dBody[wave][t], false);
+        for (t = f1; t + 2 <= end; t += 2) {
+          const TravBody<Real, PosT> b0 = dBody[wave][t];
+          const TravBody<Real, PosT> b1 = dBody[wave][t + 1];
+          farTerm(b0, true);
+          farTerm(b1, true);
+        }
+        if (t < end)
+          farTerm(dBody[wave][t], true);
+      }
     }
-    if (t < nd)
-      body(dBody[wave][t], false);
     if constexpr (FAST) {
       if (!__builtin_isfinite(ax + ay)) {
         ax = ax0;
         ay = ay0;
-        cnt = c0;
         for (int u = 0; u < nd; u++)
           body(dBody[wave][u], true);
       }
     }
+    if (counting)
+      for (int u = 0; u < nd; u++)
+        cnt += (int32_t)((dBody[wave][u].mask >> lane) & 1ull);
     nd = 0;
   };
   // Writes point p's body at list slot `at`.
@@ -659,7 +756,7 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
     }
     // entries the stack can take children for: each open entry adds <= 3
     int take = sp < kWave ? sp : kWave;
-    const int room = (kTravStack - kTravReserve - sp) / 3;
+    const int room = (kTravStack - reserve - sp) / 3;
     if (take > room)
       take = room > 0 ? room : 1;
     const int j = sp - 1 - lane; // lane 0 takes the top entry
@@ -668,12 +765,9 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
     const uint64_t msk = has ? sMask[wave][j] : 0ull;
     sp -= take;
     int kind = kDefer;
+    // read only by lanes that loaded a record (kinds kRange, kOpen, kAmbig
+    // and deferred cells)
     NodeRec<Real, PosT> r;
-    r.child[0] = r.child[1] = r.child[2] = r.child[3] = kNone;
-    r.count = 0;
-    r.first = 0;
-    r.comx = r.comy = (PosT)0;
-    r.mass = r.twoh = (Real)0;
     if (has && node >= 0) {
       r = nodes[node];
       if (r.count > 0 && r.twoh == (Real)0) {
@@ -827,7 +921,6 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
             if (accept) {
               rep_pair<OVERLAP, NEAR>(dx, dy, d2, b.m, mr.y, oc, ex, ey, r64, selfId, n, n,
                                       ax, ay);
-              cnt++;
             } else {
               open = true;
             }
@@ -838,7 +931,6 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
               Phys<Real>::template rep<MODEL, NEAR>(dx, dy, dist, inv, mr.y + (Real)0, mr.x,
                                                     b.m, kSq, oc, selfId, n, ax, ay, ex, ey,
                                                     r64, n);
-              cnt++;
             } else {
               open = true;
             }
@@ -870,6 +962,10 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
         const bool o0 = ambTest(b0);
         const bool o1 = two ? ambTest(b1) : false;
         const uint64_t om0 = __ballot(o0), om1 = __ballot(o1);
+        // the lanes of a cell's mask that did not open it took it
+        if (counting)
+          cnt += (int32_t)(((b0.mask & ~om0) >> lane) & 1ull) +
+                 (two ? (int32_t)(((b1.mask & ~om1) >> lane) & 1ull) : 0);
         if (!putBack(om0, b0.id) || !putBack(om1, b1.id))
           break;
       }
@@ -880,6 +976,8 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
     for (int k = 0; k < 5; k++)
       atomicAdd(&tstats[k], st[k]);
     atomicMax(&tstats[5], st[5]);
+    for (int k = 6; k < 8; k++)
+      atomicAdd(&tstats[k], st[k]);
   }
   if (valid) {
     if (FAST && MODEL == 1) {
@@ -1355,15 +1453,23 @@ int typed_step(ForceCommon &c, int mode) {
     // attraction (memory-bound) overlaps the traversal (ALU-bound)
     if (fork_attraction<Real, PosT>(c, posC, model))
       return -1;
+    // a timed launch runs without counting; the counts (and the optional
+    // statistics) come from a second launch over the same tree and
+    // positions, which repeats the same interactions and sums
+    const bool split = c.timeTraversal && inter;
+    const int64_t waves = (n + kWave - 1) / kWave;
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travBegin, st));
-    if (launch_traverse<Real, PosT>(c, model, (n + kWave - 1) / kWave, st, inter, theta,
-                                    kSq, oc, c.sc.p,
-                                    c.travStats.p, SegView{}))
+    if (launch_traverse<Real, PosT>(c, model, waves, st, split ? nullptr : inter, theta, kSq,
+                                    oc, c.sc.p, split ? nullptr : c.travStats.p,
+                                    SegView{}))
       return -1;
     GVH_CHECK_LAUNCH();
     if (c.timeTraversal)
       GVH_CHECK(hipEventRecord(c.travEnd, st));
+    if (split && launch_traverse<Real, PosT>(c, model, waves, st, inter, theta, kSq, oc,
+                                             c.sc.p, c.travStats.p, SegView{}))
+      return -1;
   } else {
     if (fork_attraction<Real, PosT>(c, posC, model))
       return -1;
@@ -1565,14 +1671,19 @@ int typed_batch_step(ForceCommon &c, BatchState &bs) {
   if (fork_attraction<Real, PosT>(c, posC, model))
     return -1;
   int32_t *inter = c.wantInteractions ? c.interactions.p : nullptr;
+  // timed without counting; counted by a second launch over the same tree
+  const bool split = c.timeTraversal && inter;
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travBegin, st));
-  if (launch_traverse<Real, PosT>(c, model, bs.waves, st, inter, theta, kSq, oc, bs.scG.p,
-                                  (unsigned long long *)nullptr, sv))
+  if (launch_traverse<Real, PosT>(c, model, bs.waves, st, split ? nullptr : inter, theta,
+                                  kSq, oc, bs.scG.p, (unsigned long long *)nullptr, sv))
     return -1;
   GVH_CHECK_LAUNCH();
   if (c.timeTraversal)
     GVH_CHECK(hipEventRecord(c.travEnd, st));
+  if (split && launch_traverse<Real, PosT>(c, model, bs.waves, st, inter, theta, kSq, oc,
+                                           bs.scG.p, (unsigned long long *)nullptr, sv))
+    return -1;
   const unsigned vb = (unsigned)bs.vblocks;
   GVH_CHECK(hipStreamWaitEvent(st, c.evJoin, 0));
   if (model == 1) {
