@@ -670,9 +670,10 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
         // class's run: general bodies first, then far ones with partial
         // masks, then far ones every lane takes
         const bool on = base + lane < end;
-        const TravBody<Real, PosT> c = dBody[wave][on ? base + lane : base];
+        const int src = on ? base + lane : base;
         bool f = false, u = false;
         if (on) {
+          const TravBody<Real, PosT> &c = dBody[wave][src];
           f = farBody((double)c.x, (double)c.y, (double)c.r);
           u = f && c.mask == ~0ull;
         }
@@ -683,6 +684,10 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
           st[6] += __popcll(pm) + __popcll(um);
           st[7] += __popcll(um);
         }
+        // the body is read again rather than held across the test, which
+        // keeps it out of registers the classification needs
+        __asm__ volatile("" ::: "memory");
+        const TravBody<Real, PosT> c = dBody[wave][src];
         if (on)
           dBody[wave][!f ? base + (int)__popcll(gm & ltMask)
                          : !u ? g1 + (int)__popcll(pm & ltMask)
