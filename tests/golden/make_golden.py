@@ -387,8 +387,10 @@ def spring_tutte_cases():
 
 def tutte_begin_cases():
     """Begin through the planarity suite on 3-connected planar graphs (a
-    Delaunay triangulation, a wheel-like grid) and a non-planar graph, then
-    Run to convergence, for the Tutte and the spring-driven Tutte embedder."""
+    Delaunay triangulation, a wheel-like grid), a 2-connected graph that is
+    not 3-connected (a plain grid), a graph with a cut vertex (two grids
+    sharing a corner) and a non-planar graph, then Run to convergence, for
+    the Tutte and the spring-driven Tutte embedder."""
     from scipy.spatial import Delaunay
     rng = np.random.default_rng(11)
     pts = rng.random((400, 2))
@@ -403,7 +405,14 @@ def tutte_begin_cases():
     d = [(10 * r + c, 10 * (r + 1) + c + 1) for r in range(9) for c in range(9)]
     e2 = np.concatenate([np.asarray(e2, np.int64), np.array(d, np.int64)])
     k33 = np.array([(i, 3 + j) for i in range(3) for j in range(3)], np.int64)
-    for name, n, ee in (("delaunay400", 400, e), ("trigrid10", n2, e2), ("k33", 6, k33)):
+    n3, e3 = G.grid(8, 8)
+    e3 = np.asarray(e3, np.int64)
+    n6, e6 = G.grid(6, 6)
+    e6 = np.asarray(e6, np.int64)
+    # the second grid's corner 0 is the first grid's corner n6 - 1
+    e7 = np.concatenate([e6, e6 + (n6 - 1)])
+    for name, n, ee in (("delaunay400", 400, e), ("trigrid10", n2, e2), ("k33", 6, k33),
+                        ("grid8", n3, e3), ("twogrids6", 2 * n6 - 1, e7)):
         for spring in (0, 1):
             r = refio.tutte_begin(n, ee, spring, 20000, 1e-9)
             save(f"tutte_begin_{name}_{'spring' if spring else 'jacobi'}", n=n, edges=ee,

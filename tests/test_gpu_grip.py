@@ -337,6 +337,21 @@ def test_refine_round_nd_matches_oracle_on_larger_graph(gpu, dim, layer):
     computed, full kNN lists with graph distances 1..5), device against the
     oracle's serial round: layer 0 runs the FR-like edge and kNN forces,
     layer 1 the Kamada-Kawai kNN springs."""
+    _round_nd_against_oracle(gpu, dim, layer, flat=())
+
+
+@pytest.mark.parametrize("layer", [0, 1])
+def test_refine_round_4d_with_a_flat_middle_plane_matches_oracle(gpu, layer):
+    """The 4-D round with coordinates 1 and 2 the same for every vertex: the
+    (1, 2) plane, fourth of the six in the reference's order, has no inertia
+    and is skipped (its rotate flag 0), while the planes before and after it
+    still rotate; every later pass recomputes the earlier removals per
+    vertex, so the skip must reach the last pass exactly as the oracle's
+    stored displacements do."""
+    _round_nd_against_oracle(gpu, 4, layer, flat=((1, 3.0), (2, -7.0)))
+
+
+def _round_nd_against_oracle(gpu, dim, layer, flat):
     n, kcap = 20000, 8
     _, e, _ = G.random_geometric(n, 8.0, 13)
     off, adj = G.csr_from_edges(n, e)
@@ -344,6 +359,8 @@ def test_refine_round_nd_matches_oracle_on_larger_graph(gpu, dim, layer):
     filt = rng.permutation(n).astype(np.int64)
     active = n - 1000 * layer  # layer 1: a prefix of the filtration
     pos = rng.uniform(-100.0, 100.0, (n, dim))
+    for c, value in flat:
+        pos[:, c] = value
     kv = rng.integers(0, n, (n, kcap)).astype(np.int64)
     kd = rng.integers(1, 6, (n, kcap)).astype(np.int64)
     dec = np.zeros(n, dtype=O.GRIP_VERTEX_ND_DTYPE)

@@ -358,16 +358,19 @@ def _same_cycle_shift(a, b):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["delaunay400_jacobi", "delaunay400_spring",
                                   "trigrid10_jacobi", "trigrid10_spring",
+                                  "grid8_jacobi", "grid8_spring",
                                   "k33_jacobi", "k33_spring"])
 def test_tutte_begin_matches_reference_up_to_polygon_symmetry(gpu, name):
     """Tutte / spring-Tutte Begin (planarity test, largest face, convex
     polygon, seeded interior) and Run to convergence. The reference embeds
     with the Boyer-Myrvold suite, this engine with the left-right test; on
-    these 3-connected graphs the largest face is the same cycle, which may
+    the 3-connected graphs the largest face is the same cycle, which may
     start at another vertex or run the other way round, so the converged
     layout equals the reference's under that symmetry of the polygon (an
-    orthogonal map fitted on the boundary). A non-planar graph returns -2
-    from both."""
+    orthogonal map fitted on the boundary). The plain 8 x 8 grid is
+    2-connected but not 3-connected (a corner's two neighbours separate it);
+    both embeddings still make its border the largest face. A non-planar
+    graph returns -2 from both."""
     f = golden("tutte_begin_" + name)
     n = int(f["n"])
     print(f"tutte begin {name}", flush=True)
@@ -389,6 +392,50 @@ def test_tutte_begin_matches_reference_up_to_polygon_symmetry(gpu, name):
     print(f"  boundary {len(b)} (shift {sym[0]}, reversed {sym[1]}), run {r['run']} vs "
           f"{ref_run}, max deviation {err:.3g}", flush=True)
     assert err <= 1e-6 * 200.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["twogrids6_jacobi", "twogrids6_spring"])
+def test_tutte_begin_with_a_cut_vertex_documented_difference(gpu, name):
+    """Two 6 x 6 grids sharing one corner (a cut vertex). A graph with a cut
+    vertex has several planar embeddings that differ in which face of one
+    block holds the other. The reference's Boyer-Myrvold embedding keeps
+    both grids in one outer face, whose walk of 40 darts visits the cut
+    vertex twice; the left-right embedding built here places the second grid
+    inside the square face at the first grid's corner, and its largest face
+    has 24 darts (the square's two edges at the corner, the second grid's
+    border, the cut vertex twice). That difference is asserted as it
+    stands, together with what both layouts are: the boundary is a closed
+    walk of the graph, Run converges, every boundary vertex sits on the
+    radius-200 polygon and every other vertex at the mean of its
+    neighbours, the Tutte fixed point."""
+    f = golden("tutte_begin_" + name)
+    n = int(f["n"])
+    edges = np.asarray(f["edges"])
+    print(f"tutte begin {name}", flush=True)
+    r = refio.tutte_begin(n, edges, int(f["spring"]), int(f["iters"]), float(f["epsilon"]),
+                          exe=DRIVER)
+    assert int(f["begin_rc"]) == 0 and r["begin_rc"] == 0
+    ref_b, b = [int(x) for x in f["boundary"]], [int(x) for x in r["boundary"]]
+    cut = 35
+    assert len(ref_b) == 40 and ref_b.count(cut) == 2
+    assert len(b) == 24 and b.count(cut) == 2
+    es = {(int(u), int(v)) for u, v in edges} | {(int(v), int(u)) for u, v in edges}
+    assert all((b[i], b[(i + 1) % len(b)]) in es for i in range(len(b)))
+    assert r["run"] < int(f["iters"])
+    pos = r["pos"]
+    on = np.zeros(n, bool)
+    on[b] = True
+    assert np.allclose(np.linalg.norm(pos[on], axis=1), 200.0, rtol=1e-12)
+    nb = [[] for _ in range(n)]
+    for u, v in edges:
+        nb[int(u)].append(int(v))
+        nb[int(v)].append(int(u))
+    inner = [v for v in range(n) if not on[v]]
+    dev = max(np.abs(pos[v] - pos[nb[v]].mean(axis=0)).max() for v in inner)
+    print(f"  boundary {len(b)} darts (reference {len(ref_b)}), run {r['run']} vs "
+          f"{int(f['run'])}, fixed-point deviation {dev:.3g}", flush=True)
+    assert dev <= 1e-6 * 200.0
 
 
 @pytest.mark.gpu
