@@ -6,7 +6,9 @@
 // relaxation alpha and writes the next positions into the other half of a
 // ping-pong pair; pinned boundary vertices and isolated vertices are copied.
 // Each sweep's max displacement is reduced per block and finished by a
-// one-block kernel that records it and raises the stop flag. Compiled with
+// one-block kernel that records it and raises the stop flag; in a run of
+// Jacobi sweeps the first workgroup of the next sweep finishes it instead
+// (k_sweep<..., true>), which saves a launch per sweep. Compiled with
 // -ffp-contract=off: in FP64 every sum and product rounds as the reference's.
 //
 // The spring variant (src/embedders/gvizSpringTutteEmbedder.c:289-358) runs
@@ -33,11 +35,11 @@ namespace {
 constexpr int kB = 256;
 
 struct TutteScalars {
-  int64_t sweeps;    // sweeps applied since upload
+  int64_t sweeps;    // sweeps applied (and finished) since upload
   int64_t recordBase;
   double epsilon;
   int32_t stopped;
-  int32_t pad;
+  int32_t pending;   // a fused sweep ran whose maximum is not finished yet
 };
 
 template <class T> struct V2;
@@ -71,16 +73,80 @@ __device__ __forceinline__ void row_sum(const int32_t *__restrict__ adj,
   }
 }
 
-template <class Real>
+// Max over `blocks` partial maxima by one workgroup (every thread calls it;
+// the result is valid in thread 0).
+__device__ double max_of_partials(const double *__restrict__ partial, int blocks,
+                                  double *sm) {
+  double m = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
+    m = partial[b] > m ? partial[b] : m;
+  m = wave_max(m);
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  if (lane == 0)
+    sm[wave] = m;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x / kWave); w++)
+      r = sm[w] > r ? sm[w] : r;
+  __syncthreads();
+  return r;
+}
+
+// Thread 0 of the finishing workgroup: the "tutte.maxDelta" record of sweep
+// sc->sweeps, the count, and the stop flag of gvizTutteEmbedderRun
+// (converged once maxDelta < epsilon).
+__device__ void finish_sweep(TutteScalars *sc, double r, double *rec, int64_t recCap) {
+  const int64_t slot = sc->sweeps - sc->recordBase;
+  if (slot >= 0 && slot < recCap)
+    rec[slot] = r;
+  sc->sweeps++;
+  if (sc->epsilon >= 0.0 && r < sc->epsilon)
+    sc->stopped = 1;
+}
+
+// FUSED: a sweep of a run, whose partial maxima go to the half `slot` of
+// `partial` (2 x gridDim.x); its first workgroup first finishes the sweep
+// before it (in the half sc->sweeps & 1) if one is pending, then marks this
+// one pending unless that finish stopped the run. A workgroup that starts
+// after the stop flag was raised returns, so a sweep running when the run
+// stops may be applied in part; it only writes the buffer the sweep before
+// did not, and sc->sweeps keeps pointing at that one.
+template <class Real, bool FUSED>
 __global__ __launch_bounds__(kB) void k_sweep(
     const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
     const uint8_t *__restrict__ isBoundary,
     const typename V2<Real>::type *__restrict__ in,
     typename V2<Real>::type *__restrict__ out, int64_t n, Real alpha,
-    double *__restrict__ partial, const TutteScalars *__restrict__ sc) {
+    double *__restrict__ partial, TutteScalars *__restrict__ sc, int slot,
+    double *__restrict__ rec, int64_t recCap) {
   __shared__ double sm[16];
-  if (sc->stopped)
+  if constexpr (FUSED) {
+    if (blockIdx.x == 0) {
+      __shared__ int32_t sStop;
+      if (sc->pending && !sc->stopped) {
+        const double r =
+            max_of_partials(partial + (sc->sweeps & 1) * gridDim.x, (int)gridDim.x, sm);
+        if (threadIdx.x == 0) {
+          finish_sweep(sc, r, rec, recCap);
+          sc->pending = 0;
+        }
+      }
+      if (threadIdx.x == 0) {
+        sStop = sc->stopped;
+        if (!sStop)
+          sc->pending = 1;
+      }
+      __syncthreads();
+      if (sStop)
+        return;
+    } else if (sc->stopped) {
+      return;
+    }
+    partial += slot * gridDim.x;
+  } else if (sc->stopped) {
     return;
+  }
   double vmax = 0.0;
   for (int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x; u < n;
        u += (int64_t)gridDim.x * kB) {
@@ -246,24 +312,22 @@ __global__ void k_finish(const double *__restrict__ partial, int blocks,
   __shared__ double sm[16];
   if (sc->stopped)
     return;
-  double m = 0.0;
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x)
-    m = partial[b] > m ? partial[b] : m;
-  m = wave_max(m);
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  if (lane == 0)
-    sm[wave] = m;
-  __syncthreads();
+  const double r = max_of_partials(partial, blocks, sm);
+  if (threadIdx.x == 0)
+    finish_sweep(sc, r, rec, recCap);
+}
+
+// Finishes the last fused sweep of a run, if it is still pending.
+__global__ void k_finish_pending(const double *__restrict__ partial, int blocks,
+                                 TutteScalars *__restrict__ sc, double *__restrict__ rec,
+                                 int64_t recCap) {
+  __shared__ double sm[16];
+  if (!sc->pending || sc->stopped)
+    return;
+  const double r = max_of_partials(partial + (sc->sweeps & 1) * blocks, blocks, sm);
   if (threadIdx.x == 0) {
-    double r = 0.0;
-    for (int w = 0; w < (int)(blockDim.x / kWave); w++)
-      r = sm[w] > r ? sm[w] : r;
-    int64_t slot = sc->sweeps - sc->recordBase;
-    if (slot >= 0 && slot < recCap)
-      rec[slot] = r;
-    sc->sweeps++;
-    if (sc->epsilon >= 0.0 && r < sc->epsilon)
-      sc->stopped = 1;
+    finish_sweep(sc, r, rec, recCap);
+    sc->pending = 0;
   }
 }
 
@@ -271,6 +335,7 @@ __global__ void k_begin(TutteScalars *sc, double epsilon) {
   sc->recordBase = sc->sweeps;
   sc->epsilon = epsilon;
   sc->stopped = 0;
+  sc->pending = 0;
 }
 
 // ---------------------------------------------------------------- no-alloc
@@ -288,7 +353,7 @@ __global__ void k_noalloc_begin(TutteScalars *sc, double epsilon) {
   sc->recordBase = 0;
   sc->epsilon = epsilon;
   sc->stopped = 0;
-  sc->pad = 0;
+  sc->pending = 0;
 }
 
 // After the sweeps: an odd count left the latest positions in b; copy them
@@ -379,15 +444,31 @@ int enqueue_step(gvizhip_tutte *t, int from, const StepKind &k) {
     GVH_CHECK_LAUNCH();
     return 0;
   }
+  if (k.kind == 3) {
+    // a fused Jacobi sweep: finished by the next one or k_finish_pending
+    if (f64)
+      hipLaunchKernelGGL((k_sweep<double, true>), dim3(g), dim3(kB), 0, t->stream, t->off.p,
+                         t->adj.p, t->boundary.p, (const double2 *)t->pos[from].p,
+                         (double2 *)t->pos[1 - from].p, t->n, k.a, t->partial.p, t->sc.p,
+                         from, t->records.p, t->recordCap);
+    else
+      hipLaunchKernelGGL((k_sweep<float, true>), dim3(g), dim3(kB), 0, t->stream, t->off.p,
+                         t->adj.p, t->boundary.p, (const float2 *)t->pos[from].p,
+                         (float2 *)t->pos[1 - from].p, t->n, (float)k.a, t->partial.p,
+                         t->sc.p, from, t->records.p, t->recordCap);
+    GVH_CHECK_LAUNCH();
+    return 0;
+  }
   if (k.kind == 0 && f64)
-    hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+    hipLaunchKernelGGL((k_sweep<double, false>), dim3(g), dim3(kB), 0, t->stream, t->off.p,
                        t->adj.p, t->boundary.p, (const double2 *)t->pos[from].p,
-                       (double2 *)t->pos[1 - from].p, t->n, k.a, t->partial.p, t->sc.p);
+                       (double2 *)t->pos[1 - from].p, t->n, k.a, t->partial.p, t->sc.p, 0,
+                       nullptr, (int64_t)0);
   else if (k.kind == 0)
-    hipLaunchKernelGGL(k_sweep<float>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+    hipLaunchKernelGGL((k_sweep<float, false>), dim3(g), dim3(kB), 0, t->stream, t->off.p,
                        t->adj.p, t->boundary.p, (const float2 *)t->pos[from].p,
                        (float2 *)t->pos[1 - from].p, t->n, (float)k.a, t->partial.p,
-                       t->sc.p);
+                       t->sc.p, 0, nullptr, (int64_t)0);
   else if (f64)
     hipLaunchKernelGGL(k_spring<double>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
                        t->adj.p, t->boundary.p, (const double2 *)t->pos[from].p,
@@ -520,14 +601,17 @@ int64_t run_steps(gvizhip_tutte *t, int64_t iters, double epsilon, double *maxDe
   GVH_CHECK_LAUNCH();
   int64_t done = 0;
   const int cur = (int)(before.sweeps & 1);
+  // Jacobi sweeps run fused (each finished by the next); the other kinds
+  // keep their own finish kernel.
+  const StepKind kr = k.kind == 0 ? StepKind{3, k.a, 0.0, 0.0} : k;
   // An odd starting parity takes one plain step so the two-step graph
   // always starts from buffer 0.
   if (cur == 1) {
-    if (enqueue_step(t, 1, k))
+    if (enqueue_step(t, 1, kr))
       return -1;
     done++;
   }
-  if (iters - done >= 2 && build_graph(t, k))
+  if (iters - done >= 2 && build_graph(t, kr))
     return -1;
   int64_t sinceCheck = 0;
   while (iters - done >= 2) {
@@ -542,6 +626,11 @@ int64_t run_steps(gvizhip_tutte *t, int64_t iters, double epsilon, double *maxDe
       if (s.stopped)
         break;
     }
+  }
+  if (kr.kind == 3) {
+    hipLaunchKernelGGL(k_finish_pending, dim3(1), dim3(256), 0, t->stream, t->partial.p,
+                       t->blocks, t->sc.p, t->records.p, t->recordCap);
+    GVH_CHECK_LAUNCH();
   }
   if (done < iters) {
     TutteScalars s;
@@ -644,7 +733,7 @@ int gvizhip_tutte_create(gvizhip_tutte **out, int64_t n, const int64_t *offsets,
       t->off.alloc(n1 + 1) || t->adj.alloc(m2 > 0 ? m2 : 1) ||
       t->boundary.alloc(n1) || t->pos[0].alloc(n1 * sizeof(double2)) ||
       t->pos[1].alloc(n1 * sizeof(double2)) || t->vel.alloc(n1 * sizeof(double2)) ||
-      t->partial.alloc(t->blocks) ||
+      t->partial.alloc(2 * (size_t)t->blocks) ||
       t->records.alloc(4096) || t->sc.alloc(1)) {
     if (!gvizhip_last_error()[0])
       set_error("gvizhip_tutte_create: allocation failed");
@@ -861,24 +950,26 @@ int gvizhip_tutte_profile(gvizhip_tutte *t, int64_t sweeps, double alpha,
   int cur = (int)(s.sweeps & 1);
   const unsigned g = (unsigned)t->blocks;
   GVH_CHECK(hipEventRecord(ev[2 * sweeps], t->stream));
+  // fused sweeps, as a run applies them: each finishes the one before
   for (int64_t k = 0; k < sweeps; k++) {
     GVH_CHECK(hipEventRecord(ev[2 * k], t->stream));
     if (t->precision == GVIZHIP_PRECISION_FP64)
-      hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+      hipLaunchKernelGGL((k_sweep<double, true>), dim3(g), dim3(kB), 0, t->stream, t->off.p,
                          t->adj.p, t->boundary.p, (const double2 *)t->pos[cur].p,
                          (double2 *)t->pos[1 - cur].p, t->n, alpha, t->partial.p,
-                         t->sc.p);
+                         t->sc.p, cur, t->records.p, t->recordCap);
     else
-      hipLaunchKernelGGL(k_sweep<float>, dim3(g), dim3(kB), 0, t->stream, t->off.p,
+      hipLaunchKernelGGL((k_sweep<float, true>), dim3(g), dim3(kB), 0, t->stream, t->off.p,
                          t->adj.p, t->boundary.p, (const float2 *)t->pos[cur].p,
                          (float2 *)t->pos[1 - cur].p, t->n, (float)alpha,
-                         t->partial.p, t->sc.p);
+                         t->partial.p, t->sc.p, cur, t->records.p, t->recordCap);
     GVH_CHECK(hipEventRecord(ev[2 * k + 1], t->stream));
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, t->stream, t->partial.p,
-                       (int)g, t->sc.p, t->records.p, t->recordCap);
     GVH_CHECK_LAUNCH();
     cur = 1 - cur;
   }
+  hipLaunchKernelGGL(k_finish_pending, dim3(1), dim3(256), 0, t->stream, t->partial.p,
+                     (int)g, t->sc.p, t->records.p, t->recordCap);
+  GVH_CHECK_LAUNCH();
   GVH_CHECK(hipEventRecord(ev[2 * sweeps + 1], t->stream));
   GVH_CHECK(hipStreamSynchronize(t->stream));
   double sum = 0.0;
@@ -948,8 +1039,9 @@ int gvizhip_tutte_sweeps_noalloc(int64_t n, const int32_t *offsets, const int32_
   for (int64_t k = 0; k < iters; k++) {
     const double2 *in = (const double2 *)((k & 1) ? posB : posA);
     double2 *out = (double2 *)((k & 1) ? posA : posB);
-    hipLaunchKernelGGL(k_sweep<double>, dim3(g), dim3(kB), 0, st, offsets, adj,
-                       isBoundary, in, out, n, alpha, partial, sc);
+    hipLaunchKernelGGL((k_sweep<double, false>), dim3(g), dim3(kB), 0, st, offsets, adj,
+                       isBoundary, in, out, n, alpha, partial, sc, 0, nullptr,
+                       (int64_t)0);
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, st, partial, (int)g, sc,
                        maxDeltas, iters);
     GVH_CHECK_LAUNCH();
