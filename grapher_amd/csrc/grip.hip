@@ -45,6 +45,16 @@ __device__ __forceinline__ void st_l2(int32_t *p, int32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A search slot's visited bits are read and written by one wavefront (or
+// one workgroup) within a kernel, and kernel boundaries order them between
+// kernels, so their atomics need only workgroup scope.
+__device__ __forceinline__ uint32_t bits_or(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void bits_clear(uint32_t *p) {
+  __hip_atomic_store(p, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 struct BfsSlots {
   uint32_t *bits;  // S * words
   int32_t *queue;  // S * cap
@@ -71,7 +81,16 @@ struct BfsSlots {
 // search.
 // A step's first pair per vertex: lane indices through a per-wave LDS hash
 // (2 * 64 slots, open addressing, smallest lane kept).
+// The graph may be a renumbered copy (breadth-first order, which keeps a
+// ball's vertices, rows and visited bits in few cache lines): `source`,
+// `filter` and the visited bits use its ids, `orig` maps them back to the
+// caller's for the output and the lower-id test (null: the same ids). The
+// rows keep their order, so the discovery order is the original graph's.
 constexpr int kWaveHash = 2 * kWave;
+
+__device__ __forceinline__ int32_t orig_id(const int32_t *__restrict__ orig, int32_t v) {
+  return orig ? orig[v] : v;
+}
 
 __device__ int wave_bfs(const int32_t *__restrict__ off,
                         const int32_t *__restrict__ adj,
@@ -79,11 +98,12 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
                         int maxDepth, bool lowerOnly, bool stopAtK,
                         int64_t filterTotal, uint32_t *bits, int32_t *queue,
                         int32_t *qdist, int32_t *outV, int32_t *outD,
-                        int32_t *hKey, int32_t *hVal) {
+                        int32_t *hKey, int32_t *hVal, const int32_t *__restrict__ orig) {
   const int lane = threadIdx.x & (kWave - 1);
+  const int32_t srcOrig = lowerOnly ? orig_id(orig, source) : 0;
   const uint64_t lt = lane ? (~0ull >> (kWave - lane)) : 0ull;
   if (lane == 0) {
-    atomicOr(&bits[source >> 5], 1u << (source & 31));
+    bits_or(&bits[source >> 5], 1u << (source & 31));
     st_l2(queue, source);
     st_l2(qdist, 0);
   }
@@ -148,15 +168,16 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
     bool fresh = false;
     if (valid && !dup) {
       const uint32_t bit = 1u << (w & 31);
-      fresh = !(atomicOr(&bits[w >> 5], bit) & bit);
+      fresh = !(bits_or(&bits[w >> 5], bit) & bit);
     }
     const uint64_t fm = __ballot(fresh);
-    const bool pass = fresh && (!filter || filter[w]) && (!lowerOnly || w < source);
+    const bool pass = fresh && (!filter || filter[w]) &&
+                      (!lowerOnly || orig_id(orig, w) < srcOrig);
     const uint64_t pm = __ballot(pass);
     if (pass) {
       const int slot = found + (int)__popcll(pm & lt);
       if (slot < k) {
-        outV[slot] = w;
+        outV[slot] = orig_id(orig, w);
         outD[slot] = ownDepth + 1;
       }
     }
@@ -192,7 +213,7 @@ __device__ int wave_bfs(const int32_t *__restrict__ off,
   // clear this query's bits: every visited vertex sits in the queue
   for (int i = lane; i < tail; i += kWave) {
     const int32_t w = ld_l2(queue + i);
-    atomicAnd(&bits[w >> 5], 0u);
+    bits_clear(&bits[w >> 5]);
   }
   return found;
 }
@@ -241,14 +262,15 @@ __device__ int block_bfs(const int32_t *__restrict__ off, const int32_t *__restr
                          const uint8_t *__restrict__ filter, int32_t source, int k,
                          int maxDepth, bool lowerOnly, bool stopAtK, int64_t filterTotal,
                          uint32_t *bits, int32_t *queue, int32_t *qdist, int32_t *outV,
-                         int32_t *outD) {
+                         int32_t *outD, const int32_t *__restrict__ orig) {
+  const int32_t srcOrig = lowerOnly ? orig_id(orig, source) : 0;
   __shared__ int32_t sIncl[kBB], sStart[kBB], sDepth[kBB];
   __shared__ int32_t hKey[kHash], hVal[kHash];
   __shared__ int sWave[kBW];
   __shared__ int sLast;
   const int t = threadIdx.x;
   if (t == 0) {
-    atomicOr(&bits[source >> 5], 1u << (source & 31));
+    bits_or(&bits[source >> 5], 1u << (source & 31));
     st_l2(queue, source);
     st_l2(qdist, 0);
   }
@@ -342,10 +364,11 @@ __device__ int block_bfs(const int32_t *__restrict__ off, const int32_t *__restr
           h = (h + 1) & (kHash - 1);
         if (hVal[h] == p0 + i) {
           const uint32_t bit = 1u << (w[i] & 31);
-          fresh[i] = !(atomicOr(&bits[w[i] >> 5], bit) & bit);
+          fresh[i] = !(bits_or(&bits[w[i] >> 5], bit) & bit);
         }
       }
-      pass[i] = fresh[i] && (!filter || filter[w[i]]) && (!lowerOnly || w[i] < source);
+      pass[i] = fresh[i] && (!filter || filter[w[i]]) &&
+                (!lowerOnly || orig_id(orig, w[i]) < srcOrig);
       nFresh += fresh[i] ? 1 : 0;
       nPass += pass[i] ? 1 : 0;
     }
@@ -358,7 +381,7 @@ __device__ int block_bfs(const int32_t *__restrict__ off, const int32_t *__restr
       if (pass[i]) {
         const int slot = found + atPass++;
         if (slot < k) {
-          outV[slot] = w[i];
+          outV[slot] = orig_id(orig, w[i]);
           outD[slot] = ownDepth[i] + 1;
         }
       }
@@ -396,7 +419,7 @@ __device__ int block_bfs(const int32_t *__restrict__ off, const int32_t *__restr
   // clear this query's bits: every visited vertex sits in the queue
   for (int i = t; i < tail; i += kBB) {
     const int32_t w = ld_l2(queue + i);
-    atomicAnd(&bits[w >> 5], 0u);
+    bits_clear(&bits[w >> 5]);
   }
   __syncthreads();
   return found;
@@ -410,7 +433,8 @@ __global__ __launch_bounds__(kBB) void k_bfs_queries_block(
     const int32_t *__restrict__ outIndex, int64_t count, int k, int64_t stride,
     int maxDepth, int lowerOnly, int stopAtK, int64_t filterTotal, BfsSlots sl,
     int64_t slots, int32_t *__restrict__ outV, int32_t *__restrict__ outD,
-    int32_t *__restrict__ outCount, int32_t *__restrict__ overflow) {
+    int32_t *__restrict__ outCount, int32_t *__restrict__ overflow,
+    const int32_t *__restrict__ newId, const int32_t *__restrict__ orig) {
   const int64_t slot = blockIdx.x;
   if (slot >= slots)
     return;
@@ -418,10 +442,11 @@ __global__ __launch_bounds__(kBB) void k_bfs_queries_block(
   int32_t *q = sl.queue + slot * sl.cap;
   int32_t *qd = sl.qdist + slot * sl.cap;
   for (int64_t qi = slot; qi < count; qi += slots) {
-    const int32_t src = sources[qi];
+    const int32_t src = newId ? newId[sources[qi]] : sources[qi];
     const int64_t o = (outIndex ? (int64_t)outIndex[qi] : qi) * stride;
     const int found = block_bfs(off, adj, filter, src, k, maxDepth, lowerOnly != 0,
-                                stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o);
+                                stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o,
+                                orig);
     if (threadIdx.x == 0) {
       outCount[outIndex ? outIndex[qi] : qi] = found < k ? found : k;
       if (!stopAtK && found > k)
@@ -437,7 +462,8 @@ __global__ __launch_bounds__(kB) void k_bfs_queries(
     const int32_t *__restrict__ outIndex, int64_t count, int k, int64_t stride,
     int maxDepth, int lowerOnly, int stopAtK, int64_t filterTotal, BfsSlots sl,
     int64_t slots, int32_t *__restrict__ outV, int32_t *__restrict__ outD,
-    int32_t *__restrict__ outCount, int32_t *__restrict__ overflow) {
+    int32_t *__restrict__ outCount, int32_t *__restrict__ overflow,
+    const int32_t *__restrict__ newId, const int32_t *__restrict__ orig) {
   __shared__ int32_t sHashK[kWpb][kWaveHash], sHashV[kWpb][kWaveHash];
   const int64_t slot = (int64_t)blockIdx.x * kWpb + threadIdx.x / kWave;
   if (slot >= slots)
@@ -447,11 +473,11 @@ __global__ __launch_bounds__(kB) void k_bfs_queries(
   int32_t *qd = sl.qdist + slot * sl.cap;
   const int wv = threadIdx.x / kWave;
   for (int64_t qi = slot; qi < count; qi += slots) {
-    const int32_t src = sources[qi];
+    const int32_t src = newId ? newId[sources[qi]] : sources[qi];
     const int64_t o = (outIndex ? (int64_t)outIndex[qi] : qi) * stride;
     int found = wave_bfs(off, adj, filter, src, k, maxDepth, lowerOnly != 0,
                          stopAtK != 0, filterTotal, bits, q, qd, outV + o, outD + o,
-                         sHashK[wv], sHashV[wv]);
+                         sHashK[wv], sHashV[wv], orig);
     if ((threadIdx.x & (kWave - 1)) == 0) {
       outCount[outIndex ? outIndex[qi] : qi] = found < k ? found : k;
       if (!stopAtK && found > k)
@@ -1779,6 +1805,11 @@ struct gvizhip_grip {
   std::vector<int64_t> hoff, hadj;
   std::vector<uint8_t> hactive;
   DevBuf<int32_t> off, adj, filt;
+  // the graph in breadth-first order for the BFS searches: rows in that
+  // order with their entries in the original order, ids mapped both ways,
+  // and a search's filter flags in that order
+  DevBuf<int32_t> bfsOff, bfsAdj, bfsNew, bfsOrig;
+  DevBuf<uint8_t> bfsFilter;
   DevBuf<uint8_t> active, visible, state;
   DevBuf<double2> pos, disp, oldDisp;
   DevBuf<double> heat, oldCos, lastF, partial;
@@ -1881,8 +1912,19 @@ int upload_visible(gvizhip_grip *g, const uint8_t *visible) {
   return 0;
 }
 
+// Filter flags in the searches' breadth-first order.
+__global__ __launch_bounds__(kB) void k_flags_in_order(const uint8_t *__restrict__ in,
+                                                      const int32_t *__restrict__ orig,
+                                                      int64_t n, uint8_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n)
+    out[i] = in[orig[i]];
+}
+
 // BFS queries on the device: sources (device int32), results into outV/outD
-// rows of `stride` selected by outIndex (device, may be null).
+// rows of `stride` selected by outIndex (device, may be null). The searches
+// run on the breadth-first copy of the graph; sources, filter and results
+// are mapped at its boundary.
 
 int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
                 int64_t count, int k, int64_t stride, const uint8_t *filterDev,
@@ -1892,23 +1934,30 @@ int run_queries(gvizhip_grip *g, const int32_t *srcDev, const int32_t *outIndex,
     return 0;
   if (ensure_slots(g))
     return -1;
+  const uint8_t *filter = nullptr;
+  if (filterDev) {
+    hipLaunchKernelGGL(k_flags_in_order, dim3(grid_for(g->n, kB)), dim3(kB), 0, g->stream,
+                       filterDev, g->bfsOrig.p, g->n, g->bfsFilter.p);
+    filter = g->bfsFilter.p;
+  }
   if (count < g->blockQueries) {
     // few searches (the coarse layers): a block per search
     const int64_t nb = std::min<int64_t>(g->slots, count);
     hipLaunchKernelGGL(k_bfs_queries_block, dim3((unsigned)nb), dim3(kBB), 0, g->stream,
-                       g->off.p, g->adj.p, filterDev, srcDev, outIndex, count, k, stride,
+                       g->bfsOff.p, g->bfsAdj.p, filter, srcDev, outIndex, count, k, stride,
                        maxDepth, lowerOnly ? 1 : 0, stopAtK ? 1 : 0, filterTotal,
-                       slot_view(g), nb, outV, outD, outCount, g->flags.p);
+                       slot_view(g), nb, outV, outD, outCount, g->flags.p, g->bfsNew.p,
+                       g->bfsOrig.p);
     GVH_CHECK_LAUNCH();
     return 0;
   }
   const int64_t waves = std::min<int64_t>(g->slots, count);
   const unsigned blocks = (unsigned)((waves + kWpb - 1) / kWpb);
   hipLaunchKernelGGL(k_bfs_queries, dim3(blocks), dim3(kB), 0, g->stream,
-                     g->off.p, g->adj.p, filterDev, srcDev, outIndex, count, k,
+                     g->bfsOff.p, g->bfsAdj.p, filter, srcDev, outIndex, count, k,
                      stride, maxDepth, lowerOnly ? 1 : 0, stopAtK ? 1 : 0,
                      filterTotal, slot_view(g), (int64_t)blocks * kWpb, outV, outD,
-                     outCount, g->flags.p);
+                     outCount, g->flags.p, g->bfsNew.p, g->bfsOrig.p);
   GVH_CHECK_LAUNCH();
   return 0;
 }
@@ -2272,6 +2321,37 @@ int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
   for (int64_t v = 0; v < n; v++)
     g->activeCount += g->hactive[v] != 0;
   g->edgeCount = m2 / 2;
+  // breadth-first order (from vertex 0, then from each vertex not reached
+  // yet) for the BFS searches: a ball's vertices get nearby ids there
+  std::vector<int32_t> bfsOrig(n), bfsNew(n, -1);
+  {
+    int64_t tail = 0;
+    for (int64_t s0 = 0; s0 < n; s0++) {
+      if (bfsNew[s0] >= 0)
+        continue;
+      bfsNew[s0] = (int32_t)tail;
+      bfsOrig[tail++] = (int32_t)s0;
+      for (int64_t h = tail - 1; h < tail; h++) {
+        const int32_t u = bfsOrig[h];
+        for (int32_t e = o32[u]; e < o32[u + 1]; e++) {
+          const int32_t w = a32[e];
+          if (bfsNew[w] < 0) {
+            bfsNew[w] = (int32_t)tail;
+            bfsOrig[tail++] = w;
+          }
+        }
+      }
+    }
+  }
+  std::vector<int32_t> bo(n + 1), ba(m2 > 0 ? m2 : 1);
+  bo[0] = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t u = bfsOrig[i];
+    int32_t at = bo[i];
+    for (int32_t e = o32[u]; e < o32[u + 1]; e++)
+      ba[at++] = bfsNew[a32[e]];
+    bo[i + 1] = at;
+  }
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
       g->off.alloc(n + 1) || g->adj.alloc(m2 > 0 ? m2 : 1) || g->filt.alloc(n) ||
       g->active.alloc(n) || g->visible.alloc(n) || g->state.alloc(n) ||
@@ -2280,12 +2360,21 @@ int gvizhip_grip_create(gvizhip_grip **out, int64_t n, const int64_t *offsets,
       g->dispCalc.alloc(n) || g->knnV.alloc(n * kcap) || g->knnD.alloc(n * kcap) ||
       g->knnCount.alloc(n) || g->partial.alloc(4 * std::max<int64_t>(1024, (int64_t)grid_for(n > 0 ? n : 1, kB))) || g->flags.alloc(8) ||
       g->round.alloc(1) || g->qSrc.alloc(n) || g->qOut.alloc(n) ||
-      g->hitCount.alloc(n)) {
+      g->hitCount.alloc(n) || g->bfsOff.alloc(n + 1) || g->bfsAdj.alloc(m2 > 0 ? m2 : 1) ||
+      g->bfsNew.alloc(n) || g->bfsOrig.alloc(n) || g->bfsFilter.alloc(n)) {
     gvizhip_grip_destroy(g);
     return -1;
   }
   hipStream_t st = g->stream;
   if (hipMemcpyAsync(g->off.p, o32.data(), sizeof(int32_t) * (n + 1),
+                     hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(g->bfsOff.p, bo.data(), sizeof(int32_t) * (n + 1),
+                     hipMemcpyHostToDevice, st) != hipSuccess ||
+      (m2 > 0 && hipMemcpyAsync(g->bfsAdj.p, ba.data(), sizeof(int32_t) * m2,
+                                hipMemcpyHostToDevice, st) != hipSuccess) ||
+      hipMemcpyAsync(g->bfsNew.p, bfsNew.data(), sizeof(int32_t) * n,
+                     hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(g->bfsOrig.p, bfsOrig.data(), sizeof(int32_t) * n,
                      hipMemcpyHostToDevice, st) != hipSuccess ||
       (m2 > 0 && hipMemcpyAsync(g->adj.p, a32.data(), sizeof(int32_t) * m2,
                                 hipMemcpyHostToDevice, st) != hipSuccess) ||
