@@ -155,3 +155,48 @@ def test_gs_sweeps_stop_at_epsilon_like_the_reference_run(gpu):
     _, r = t.gs_sweeps(400, 1.0, eps)
     assert r == count < 400
     assert np.array_equal(t.positions(), ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_runs_of_sweeps_equal_single_sweeps(gpu, precision):
+    """A run of Jacobi sweeps finishes each sweep inside the next one (one
+    launch per sweep, the last finished after the run); single sweeps keep
+    their own finish. On a 5,000-point Delaunay graph: runs from both buffer
+    parities, of even and odd lengths, and a run that stops on epsilon give
+    bit for bit the positions, the maxDelta records and the sweep counts of
+    the same sweeps taken one at a time."""
+    n, e, _, hull = G.delaunay(5000, 4)
+    off, adj = G.csr_from_edges(n, e)
+    isb = np.zeros(n, np.uint8)
+    isb[hull] = 1
+    ang = 2.0 * np.pi * np.arange(len(hull)) / len(hull)
+    p0 = np.zeros((n, 2))
+    p0[hull] = 200.0 * np.stack([np.cos(ang), np.sin(ang)], 1)
+    p0[~isb.astype(bool)] = p0[hull].mean(0)
+    a = gpu.TutteEngine(off, adj, precision)
+    b = gpu.TutteEngine(off, adj, precision)
+    a.upload(p0, isb)
+    b.upload(p0, isb)
+    for length in (1, 6, 7, 40, 33):
+        da, ra = a.sweeps(length)
+        db = []
+        for _ in range(length):
+            d, r = b.sweeps(1)
+            assert r == 1
+            db.extend(d)
+        assert ra == length
+        assert np.array_equal(da, np.asarray(db)), length
+        assert np.array_equal(a.positions(), b.positions()), length
+    # a run stopping on epsilon: the same sweep count, records and layout
+    eps = float(np.asarray(db)[-1]) * 0.5
+    da, ra = a.sweeps(10000, 1.0, eps)
+    count, db = 0, []
+    while count < 10000:
+        d, r = b.sweeps(1, 1.0, eps)
+        count += r
+        db.extend(d)
+        if d[-1] < eps:
+            break
+    assert 0 < ra < 10000 and ra == count
+    assert np.array_equal(da, np.asarray(db))
+    assert np.array_equal(a.positions(), b.positions())
