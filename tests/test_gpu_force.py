@@ -344,3 +344,28 @@ def test_tiny_and_degenerate_graphs_match_oracle(gpu, model, barnes_hut):
             assert np.all(np.isfinite(got)), (n, overlap)
             assert np.linalg.norm(got - ref) <= 1e-11 * max(np.linalg.norm(ref), 1e-300), \
                 (n, overlap, got, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+@pytest.mark.parametrize("overlap", [True, False])
+def test_counting_interactions_leaves_forces_unchanged(gpu, precision, overlap):
+    """The traversal counts interactions per target only when asked (a
+    separate pass over each flush's masks and the undecided cells' ballots),
+    and a timed launch runs without counting: the forces of an evaluation
+    with counts equal bit for bit those of one without, and a second
+    evaluation repeats forces and counts."""
+    n, e = G.barabasi_albert(20000, 3, 5)
+    off, adj = G.csr_from_edges(n, e)
+    pos = G.uniform_positions(n, 6) * 50.0
+    eng = gpu.ForceEngine(off, adj, precision=precision, model=1, theta=0.7,
+                          gravity=0.05, prevent_overlap=overlap, radius_base=0.1,
+                          radius_per_degree=1.0)
+    eng.set_positions(pos)
+    plain = eng.evaluate()
+    counted = eng.evaluate(interactions=True)
+    again = eng.evaluate(interactions=True)
+    for k in ("disp", "structForce", "repMag"):
+        assert np.array_equal(plain[k], counted[k]), k
+        assert np.array_equal(counted[k], again[k]), k
+    assert np.array_equal(counted["interactions"], again["interactions"])
+    assert counted["interactions"].min() > 0
