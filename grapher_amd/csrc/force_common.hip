@@ -132,78 +132,139 @@ constexpr int kSortFrom = GVH_SORT_FROM_BIT; // keyHi bits below this are not ra
 
 // The radix sort orders by the top 32 bits of keyHi only (one graph: the
 // first 16 levels; a batch: the graph id and 10 levels), stable, so equal
-// top halves stay in id order. A run of equal top halves -- rare, and long
-// only in tight clusters -- is put in full (keyHi, keyLo, id) order here by
-// the thread at its head: insertion sort up to 32 entries, heap sort
-// (O(L log L); every key is distinct once the id is included) beyond.
-__global__ __launch_bounds__(kBlock) void k_fix_runs32(
+// top halves stay in id order. Runs of equal top halves are then put in full
+// (keyHi, keyLo, id) order in place; every key is distinct once the id is
+// included. They are common (in a layout whose root box is set by a few far
+// vertices, most points share their first 16 levels with a few others), so a
+// run of up to kRankRun entries is ordered by rank: a thread per entry counts
+// the run's entries below it and writes itself to start + rank. A workgroup
+// stages its kBlock positions with kRunHalo entries either side in LDS and
+// orders the short runs whose first entry is among its positions, including
+// the tail of one that runs into the next workgroup's (threads t < kRankRun
+// take position end + t as a second entry). A longer run -- a tight cluster
+// -- is heap-sorted in place by the thread at its head (O(L log L)).
+//
+// Every run has one owner, and only the owner writes its positions. Another
+// workgroup that stages them reads only their top halves, which are equal
+// within a run whatever order its owner has written so far, so no result
+// depends on the order in which workgroups run.
+constexpr int kRankRun = 64;
+constexpr int kRunHalo = kRankRun + 1;
+constexpr int kRunWindow = kBlock + 2 * kRunHalo;
+
+__global__ __launch_bounds__(kBlock) void k_order_runs32(
     uint64_t *__restrict__ keyHi, const uint64_t *__restrict__ keyLo,
     uint32_t *__restrict__ vals, int64_t n, const Scalars *__restrict__ sc) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || sc->stopped)
+  __shared__ uint64_t wk[kRunWindow];
+  __shared__ uint32_t wv[kRunWindow];
+  if (sc->stopped)
     return;
-  const uint64_t top = keyHi[s] >> kSortFrom;
-  if (s > 0 && (keyHi[s - 1] >> kSortFrom) == top)
-    return;
-  int64_t e = s + 1;
-  while (e < n && (keyHi[e] >> kSortFrom) == top)
-    e++;
-  const int64_t len = e - s;
-  if (len < 2)
-    return;
-  uint64_t *kh = keyHi + s;
-  uint32_t *vv = vals + s;
-  auto less = [&](uint64_t ha, uint32_t va, uint64_t hb, uint32_t vb) {
-    if (ha != hb)
-      return ha < hb;
-    const uint64_t la = keyLo[va], lb = keyLo[vb];
-    if (la != lb)
-      return la < lb;
-    return va < vb;
-  };
-  if (len <= 32) {
-    for (int64_t a = 1; a < len; a++) {
-      const uint64_t h = kh[a];
-      const uint32_t v = vv[a];
-      int64_t b = a - 1;
-      while (b >= 0 && less(h, v, kh[b], vv[b])) {
-        kh[b + 1] = kh[b];
-        vv[b + 1] = vv[b];
-        b--;
+  // window entry i is sorted position base + i
+  const int64_t start = (int64_t)blockIdx.x * kBlock;
+  const int64_t end = start + kBlock < n ? start + kBlock : n;
+  const int64_t base = start - kRunHalo;
+  for (int i = threadIdx.x; i < kRunWindow; i += kBlock) {
+    const int64_t g = base + i;
+    if (g >= 0 && g < n) {
+      wk[i] = keyHi[g];
+      wv[i] = vals[g];
+    }
+  }
+  __syncthreads();
+  // Orders the entry at position s (start <= s < end + kRankRun). Reads stay
+  // in the window: b - 1 >= s - kRunHalo, and e <= b + kRankRun with
+  // b < end.
+  auto order = [&](int64_t s, bool second) {
+    const uint64_t h = wk[s - base];
+    const uint32_t v = wv[s - base];
+    const uint64_t top = h >> kSortFrom;
+    int64_t b = s;
+    while (b > 0 && s - b <= kRankRun && (wk[b - 1 - base] >> kSortFrom) == top)
+      b--;
+    if (second && b >= end)
+      return; // the next workgroup's run
+    int64_t e = s + 1;
+    while (e < n && e - b <= kRankRun && (wk[e - base] >> kSortFrom) == top)
+      e++;
+    if (e - b <= kRankRun) {
+      if (b < start)
+        return; // the previous workgroup's run
+      int64_t rank = 0;
+      bool haveLo = false;
+      uint64_t lo = 0;
+      for (int64_t j = b; j < e; j++) {
+        if (j == s)
+          continue;
+        const uint64_t hj = wk[j - base];
+        const uint32_t vj = wv[j - base];
+        bool below;
+        if (hj != h) {
+          below = hj < h;
+        } else {
+          if (!haveLo) {
+            lo = keyLo[v];
+            haveLo = true;
+          }
+          const uint64_t lj = keyLo[vj];
+          below = lj != lo ? lj < lo : vj < v;
+        }
+        rank += below;
       }
-      kh[b + 1] = h;
-      vv[b + 1] = v;
+      keyHi[b + rank] = h;
+      vals[b + rank] = v;
+      return;
     }
-    return;
-  }
-  // heap sort: max-heap on the same order, then repeated extraction
-  auto sift = [&](int64_t root, int64_t end) {
-    while (2 * root + 1 < end) {
-      int64_t c = 2 * root + 1;
-      if (c + 1 < end && less(kh[c], vv[c], kh[c + 1], vv[c + 1]))
-        c++;
-      if (!less(kh[root], vv[root], kh[c], vv[c]))
-        return;
-      const uint64_t th = kh[root];
-      kh[root] = kh[c];
-      kh[c] = th;
-      const uint32_t tv = vv[root];
-      vv[root] = vv[c];
-      vv[c] = tv;
-      root = c;
+    if (second || (s > 0 && (wk[s - 1 - base] >> kSortFrom) == top))
+      return; // a long run is ordered by the thread at its head
+    e = s + 1;
+    while (e < n && (keyHi[e] >> kSortFrom) == top)
+      e++;
+    const int64_t L = e - s;
+    uint64_t *kh = keyHi + s;
+    uint32_t *vv = vals + s;
+    auto less = [&](uint64_t ha, uint32_t va, uint64_t hb, uint32_t vb) {
+      if (ha != hb)
+        return ha < hb;
+      const uint64_t la = keyLo[va], lb = keyLo[vb];
+      if (la != lb)
+        return la < lb;
+      return va < vb;
+    };
+    // heap sort: max-heap on the same order, then repeated extraction
+    auto sift = [&](int64_t root, int64_t stop) {
+      while (2 * root + 1 < stop) {
+        int64_t c = 2 * root + 1;
+        if (c + 1 < stop && less(kh[c], vv[c], kh[c + 1], vv[c + 1]))
+          c++;
+        if (!less(kh[root], vv[root], kh[c], vv[c]))
+          return;
+        const uint64_t th = kh[root];
+        kh[root] = kh[c];
+        kh[c] = th;
+        const uint32_t tv = vv[root];
+        vv[root] = vv[c];
+        vv[c] = tv;
+        root = c;
+      }
+    };
+    for (int64_t r = L / 2 - 1; r >= 0; r--)
+      sift(r, L);
+    for (int64_t last = L - 1; last > 0; last--) {
+      const uint64_t th = kh[0];
+      kh[0] = kh[last];
+      kh[last] = th;
+      const uint32_t tv = vv[0];
+      vv[0] = vv[last];
+      vv[last] = tv;
+      sift(0, last);
     }
   };
-  for (int64_t r = len / 2 - 1; r >= 0; r--)
-    sift(r, len);
-  for (int64_t end = len - 1; end > 0; end--) {
-    const uint64_t th = kh[0];
-    kh[0] = kh[end];
-    kh[end] = th;
-    const uint32_t tv = vv[0];
-    vv[0] = vv[end];
-    vv[end] = tv;
-    sift(0, end);
-  }
+  const int64_t s = start + threadIdx.x;
+  if (s < end)
+    order(s, false);
+  const int64_t s2 = end + threadIdx.x;
+  if (threadIdx.x < kRankRun && end == start + kBlock && s2 < n)
+    order(s2, true);
 }
 
 // Longest common prefix in bits of the 128-bit keys at sorted positions i, j;
@@ -617,6 +678,15 @@ void ForceCommon::close_streams() {
   evFork = evJoin = nullptr;
 }
 
+// Full key order within runs of equal top halves (after the radix sort).
+static int enqueue_order_runs(ForceCommon &c, hipStream_t st) {
+  const int64_t n = c.n;
+  hipLaunchKernelGGL(k_order_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
+  GVH_CHECK_LAUNCH();
+  return 0;
+}
+
 // Bounding box, keys, sort and radix tree (everything before the typed
 // gather). Typed gather + prefix sums + quad records follow in the typed TU.
 int enqueue_keys_and_sort(ForceCommon &c) {
@@ -633,10 +703,7 @@ int enqueue_keys_and_sort(ForceCommon &c) {
   GVH_CHECK_LAUNCH();
   if (c.sorter.sort(c.keyHi.p, c.vals.p, n, kSortFrom, 64, st))
     return -1;
-  hipLaunchKernelGGL(k_fix_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                     c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
-  GVH_CHECK_LAUNCH();
-  return 0;
+  return enqueue_order_runs(c, st);
 }
 
 int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b) {
@@ -652,10 +719,7 @@ int enqueue_batch_keys_and_sort(ForceCommon &c, BatchState &b) {
   // are put in full order as for one graph
   if (c.sorter.sort(c.keyHi.p, c.vals.p, n, kSortFrom, 64, st))
     return -1;
-  hipLaunchKernelGGL(k_fix_runs32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                     c.keyHi.p, c.keyLo.p, c.vals.p, n, c.sc.p);
-  GVH_CHECK_LAUNCH();
-  return 0;
+  return enqueue_order_runs(c, st);
 }
 
 int enqueue_batch_fa2(ForceCommon &c, BatchState &b) {
