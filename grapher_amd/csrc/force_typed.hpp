@@ -240,7 +240,8 @@ template <> struct Phys<float> {
 
 // ------------------------------------------------------------ kernels
 
-// Sorted copies: FP64 positions (prefix sums), typed positions and
+// Sorted copies: FP64 positions (prefix sums; the traversal's positions too
+// when they are FP64), FP32 positions where the traversal takes those,
 // (mass, radius) for the traversal, and the low key word for the tree.
 template <class Real, class PosT>
 __global__ __launch_bounds__(kTB) void k_gather(
@@ -256,7 +257,8 @@ __global__ __launch_bounds__(kTB) void k_gather(
   uint32_t i = vals[s];
   double2 p = pos64[i];
   spos64[s] = p;
-  spos[s] = {(PosT)p.x, (PosT)p.y};
+  if constexpr (sizeof(PosT) != sizeof(double))
+    spos[s] = {(PosT)p.x, (PosT)p.y}; // FP64 positions are read from spos64
   const double2 mr = mr64[i];
   smr[s] = {(Real)mr.x, (Real)mr.y};
   smass64[s] = mr.x;
@@ -1350,9 +1352,12 @@ int launch_traverse(ForceCommon &c, int model, int64_t waves, hipStream_t st, in
   const bool ov = F64 || c.prm.preventOverlap != 0;
   const int wpb = c.travWaves == 2 || c.travWaves == 4 ? c.travWaves : 1;
   const unsigned tb = (unsigned)((waves + wpb - 1) / wpb);
+  // FP64 positions in key order: k_gather writes them once, into spos64
+  const P2 *sposT = sizeof(PosT) == sizeof(double) ? (const P2 *)c.spos64.p
+                                                   : (const P2 *)c.spos.p;
 #define GVH_TRAVERSE_W(M, OV, W)                                                           \
   hipLaunchKernelGGL((k_traverse<Real, PosT, M, OV, W>), dim3(tb), dim3(W * kWave), 0, st,  \
-                     (const NodeRec<Real, PosT> *)c.nodes.p, (const P2 *)c.spos.p,        \
+                     (const NodeRec<Real, PosT> *)c.nodes.p, sposT,                       \
                      (const R2 *)c.smr.p, c.vals.p, c.n, theta, (double)c.prm.theta, kSq,  \
                      oc, (R2 *)c.rep.p,                                                   \
                      inter, c.radius64.p, sc, tstats, sv)
