@@ -315,6 +315,42 @@ def test_clustered_points_sort_and_tree_match_oracle(gpu, npc):
             np.testing.assert_array_equal(got["interactions"], ref["interactions"])
 
 
+@pytest.mark.parametrize("npc", [1, 4])
+def test_runs_across_workgroups_sort_and_tree_match_oracle(gpu, npc):
+    """Runs of points that share their first 16 quadrant levels are put in
+    full key order by rank, each run by the workgroup of 256 sorted positions
+    that holds its first entry, runs longer than 64 by one thread. 400
+    clusters of 2-70 points (each inside a 1e-6 square, a few exact
+    duplicates among them) among 3,000 uniform points put run boundaries
+    across most workgroup boundaries and take both paths: every point's leaf
+    and the FP64 interaction counts equal the oracle's."""
+    rng = np.random.default_rng(47)
+    parts = [rng.uniform(-1, 1, (3000, 2))]
+    sizes = rng.integers(2, 71, 400)
+    for k, size in enumerate(sizes):
+        c = rng.uniform(-0.9, 0.9, 2)
+        if k % 10 == 0:
+            parts.append(np.tile(c, (int(size), 1)))
+        else:
+            parts.append(c + rng.uniform(0, 1e-6, (int(size), 2)))
+    pts = np.concatenate(parts)
+    pts = pts[rng.permutation(len(pts))]
+    n = len(pts)
+    assert (sizes > 64).sum() >= 10 and n > 40 * 256
+    o = O.quadtree(pts, np.ones(n), npc)
+    eng = gpu.ForceEngine(np.zeros(n + 1, np.int64), np.zeros(0, np.int64),
+                          precision="fp64", nodes_per_cell=npc)
+    eng.set_positions(pts)
+    depth, kh, kl, _ = eng.quadtree_leaves()
+    np.testing.assert_array_equal(depth, o["leaf_depth"])
+    np.testing.assert_array_equal(kh, o["leaf_keyHi"])
+    np.testing.assert_array_equal(kl, o["leaf_keyLo"])
+    got = eng.evaluate(interactions=True)
+    g = O.Graph(np.zeros(n + 1, np.int64), np.zeros(0, np.int64))
+    ref = O.force_compute(g, pts, O.params(npc=npc))
+    np.testing.assert_array_equal(got["interactions"], ref["interactions"])
+
+
 @pytest.mark.parametrize("barnes_hut", [True, False])
 @pytest.mark.parametrize("model", [0, 1])
 def test_tiny_and_degenerate_graphs_match_oracle(gpu, model, barnes_hut):
