@@ -537,7 +537,13 @@ __global__ __launch_bounds__(WPB * kWave, GVH_TRAV_MINBLOCKS) void k_traverse(
     if (root == -1)
       root = -(int32_t)(sv.gStart[g] + 1);
   } else {
-    s = ((int64_t)blockIdx.x * WPB + wave) * kWave + lane;
+    // workgroups go to the 8 XCDs in turn (b mod 8): XCD x takes the x-th
+    // contiguous eighth of the target groups, so each L2 serves one region
+    // of the tree
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    const int64_t q = G >> 3, r = G & 7, x = b & 7;
+    const int64_t wb = x * q + (x < r ? x : r) + (b >> 3);
+    s = (wb * WPB + wave) * kWave + lane;
     valid = s < n;
     root = n >= 2 ? 0 : -1;
   }
