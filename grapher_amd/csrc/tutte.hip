@@ -112,6 +112,20 @@ __device__ void finish_sweep(TutteScalars *sc, double r, double *rec, int64_t re
 // after the stop flag was raised returns, so a sweep running when the run
 // stops may be applied in part; it only writes the buffer the sweep before
 // did not, and sc->sweeps keeps pointing at that one.
+// The kB-row tiles [t0, t1) a workgroup sweeps: a contiguous range per
+// workgroup, and the workgroups of one XCD (they are dealt out in turn,
+// blockIdx mod 8) take a contiguous eighth of the ranges, so rows in
+// breadth-first order read their neighbours' positions mostly through their
+// own XCD's L2.
+__device__ __forceinline__ void tile_range(int64_t n, int64_t &t0, int64_t &t1) {
+  const int64_t G = gridDim.x, bi = blockIdx.x;
+  const int64_t q = G >> 3, r = G & 7, x = bi & 7;
+  const int64_t w = x * q + (x < r ? x : r) + (bi >> 3);
+  const int64_t tiles = (n + kB - 1) / kB;
+  t0 = w * tiles / G;
+  t1 = (w + 1) * tiles / G;
+}
+
 template <class Real, bool FUSED>
 __global__ __launch_bounds__(kB) void k_sweep(
     const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
@@ -148,8 +162,12 @@ __global__ __launch_bounds__(kB) void k_sweep(
     return;
   }
   double vmax = 0.0;
-  for (int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x; u < n;
-       u += (int64_t)gridDim.x * kB) {
+  int64_t t0, t1;
+  tile_range(n, t0, t1);
+  for (int64_t t = t0; t < t1; t++) {
+    const int64_t u = t * kB + threadIdx.x;
+    if (u >= n)
+      continue;
     typename V2<Real>::type old = in[u];
     const int32_t b = off[u], e = off[u + 1];
     if (isBoundary[u] || e == b) {
@@ -261,8 +279,12 @@ __global__ __launch_bounds__(kB) void k_spring(
   if (sc->stopped)
     return;
   double vmax = 0.0;
-  for (int64_t u = (int64_t)blockIdx.x * kB + threadIdx.x; u < n;
-       u += (int64_t)gridDim.x * kB) {
+  int64_t t0, t1;
+  tile_range(n, t0, t1);
+  for (int64_t t = t0; t < t1; t++) {
+    const int64_t u = t * kB + threadIdx.x;
+    if (u >= n)
+      continue;
     typename V2<Real>::type old = in[u];
     const int32_t b = off[u], e = off[u + 1];
     if (isBoundary[u] || e == b) {
